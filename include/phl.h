@@ -1,0 +1,171 @@
+/*
+ * phl.h -- C-ABI of phenlei_amd, the MI355X (gfx950) implementation of the
+ * PHengLEI unstructured compressible residual + LU-SGS path.
+ *
+ * The entry points mirror the granularity of the reference's GPU call
+ * surface (the CallGPU* hooks that NSSolverUnstruct early-returns into under
+ * USE_CUDA, and the lifecycle calls of GPU/Manage).  Each declaration cites
+ * the reference interface it replaces.  Plain pointers and sizes only.
+ *
+ * Arrays exchanged with the host are structure-of-arrays over the
+ * reference's cell numbering: five-component fields are [5][nTotal] with
+ * nTotal = nCell + nBFace, the ghost cell of boundary face f being nCell+f
+ * (API/Geometry/src/Geo_UnstructGrid.cpp:2007).
+ *
+ * Errors: every call returns a phl_status; nothing calls exit().  The
+ * reference instead prints and exits (GPU/Manage/src/cudaErrorHandle.cu:4-26).
+ * phl_last_error() returns a thread-local message for the last failure.
+ */
+#ifndef PHL_H
+#define PHL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum phl_status {
+    PHL_OK = 0,
+    PHL_ERR_ARG = 1,         /* bad argument or inconsistent sizes */
+    PHL_ERR_HIP = 2,         /* HIP runtime error */
+    PHL_ERR_NUMERIC = 3,     /* NaN/Inf residual (reference: Residual.cpp:579-589 exits) */
+    PHL_ERR_UNSUPPORTED = 4, /* option outside the implemented path */
+    PHL_ERR_ALLOC = 5        /* host or device allocation failed */
+} phl_status;
+
+/* Solver parameters: the cfd_para.hypara keys on this path
+ * (PHengLEI/examples/bin/cfd_para.hypara:327-612,1143-1145). */
+typedef struct phl_params {
+    double mach;            /* refMachNumber */
+    double alpha_deg;       /* attackd */
+    double beta_deg;        /* angleSlide */
+    double gamma;           /* refGama (1.4) */
+    double reynolds;        /* refReNumber per unit grid length */
+    double t_ref;           /* refDimensionalTemperature [K] (288.15) */
+    double prandtl;         /* prl (0.72) */
+    double prandtl_turb;    /* prt (0.9) */
+    double t_wall;          /* wallTemperature [K]; <= 0 adiabatic (-1) */
+    double venkat_k;        /* venkatCoeff (5.0) */
+    double entropy_scale;   /* roeEntropyScale (1.0) */
+    double cfl_start;       /* CFLStart (0.01) */
+    double cfl_end;         /* CFLEnd (10.0) */
+    double ktmax;           /* ktmax (1e10) */
+    double lusgs_tol;       /* LUSGSTolerance (0.01) */
+    double force_ref_length;/* forceReferenceLength (1.0) */
+    double force_ref_area;  /* forceReferenceArea (1.0) */
+    double force_ref_x, force_ref_y, force_ref_z; /* TorqueRefX/Y/Z (0) */
+    int viscous;            /* viscousType: 0 inviscid, 1 laminar */
+    int gradient;           /* gradientName: 0 "ggcell" */
+    int limiter;            /* uns_limiter_name: 0 "vencat" (ivencat=7), 1 "nolim", 2 "1st" */
+    int entropy_fix;        /* roeEntropyFixMethod (3) */
+    int cfl_vary_step;      /* CFLVaryStep (500) */
+    int n_lusgs_sweeps;     /* nLUSGSSweeps (1) */
+    int precision;          /* 0 FP64; 1 mixed (FP32 faces, FP64 accumulation/LU-SGS) */
+    int reserved;
+} phl_params;
+
+/* Field identifiers for phl_get_field (same numbering as the oracle). */
+enum {
+    PHL_FIELD_Q = 0, PHL_FIELD_RES = 1, PHL_FIELD_DQDX = 2, PHL_FIELD_DQDY = 3,
+    PHL_FIELD_DQDZ = 4, PHL_FIELD_LIMIT = 5, PHL_FIELD_DQ = 6, PHL_FIELD_SPEC_INV = 7,
+    PHL_FIELD_SPEC_VIS = 8, PHL_FIELD_DT = 9, PHL_FIELD_DIAG = 10, PHL_FIELD_T = 11,
+    PHL_FIELD_VISL = 12, PHL_FIELD_DTDX = 13, PHL_FIELD_DTDY = 14, PHL_FIELD_DTDZ = 15,
+    PHL_FIELD_DELTA_FLUX = 16
+};
+
+/* Boundary-condition codes (API/Geometry/include/Geo_SimpleBC.h:32-39). */
+enum { PHL_BC_EXTRAPOLATION = 1, PHL_BC_SOLID_SURFACE = 2, PHL_BC_SYMMETRY = 3,
+       PHL_BC_FARFIELD = 4, PHL_BC_INFLOW = 5, PHL_BC_OUTFLOW = 6 };
+
+typedef struct phl_mesh phl_mesh; /* host-side unstructured grid + metrics */
+typedef struct phl_ctx phl_ctx;   /* device-resident solver state for one zone */
+
+const char *phl_last_error(void);
+void phl_params_default(phl_params *p);
+
+/* ---------------- meshes (host) ----------------------------------------
+ * Replaces the grid read + UnstructGrid::ComputeMetrics step
+ * (Main/src/Simulation.cpp:205-247; Geo_UnstructGrid.cpp:5695,5911,6545). */
+
+/* Build from explicit connectivity.  Faces 0..nBFace-1 are boundary faces
+ * (right[f] must be nCell+f), face normals point from left to right. */
+phl_status phl_mesh_create(int nNode, const double *x, const double *y, const double *z,
+                           int nFace, const int *faceNodePtr, const int *faceNodes,
+                           const int *left, const int *right, int nBFace, const int *bcType,
+                           int nCell, const int *cellNodePtr, const int *cellNodes,
+                           phl_mesh **out);
+/* Channel x in [-1.5,1.5], wall-fitted between y_w = h*exp(-25x^2) and 0.8,
+ * z in [0,0.2]; nx*ny*nz hexahedra.  Inlet/outlet FARFIELD, lower wall
+ * SOLID_SURFACE, top and sides SYMMETRY. */
+phl_status phl_mesh_bump_hex(int nx, int ny, int nz, double bump, phl_mesh **out);
+phl_status phl_mesh_sizes(const phl_mesh *m, int *nNode, int *nCell, int *nFace, int *nBFace,
+                          int *nFaceNodes, int *nCellNodes);
+phl_status phl_mesh_connectivity(const phl_mesh *m, double *x, double *y, double *z,
+                                 int *faceNodePtr, int *faceNodes, int *left, int *right,
+                                 int *bcType, int *cellNodePtr, int *cellNodes);
+/* Face arrays are [nFace]; cell centroid and volume arrays are [nTotal]
+ * (ghost centroid mirrored across the face); skew is [nCell] in degrees. */
+phl_status phl_mesh_metrics(const phl_mesh *m, double *xfc, double *yfc, double *zfc,
+                            double *xfn, double *yfn, double *zfn, double *area,
+                            double *xcc, double *ycc, double *zcc, double *vol, double *skew);
+void phl_mesh_destroy(phl_mesh *m);
+
+/* ---------------- device and context lifecycle ------------------------- */
+
+/* GPUDeviceMultiGPU / GPUInfoQuery (GPU/Manage/src/GPUDeviceControl.cu:41-83). */
+phl_status phl_device_info(int device, char *name, int name_len, int *n_cu, double *hbm_bytes);
+/* GPUGeomInfoAllocCopy + GPUFlowVarMemAlloc + GPUFaceColorMain
+ * (GPU/Kernels/src/BasicDeviceVariables.cu:459,2679; GPUFaceColor.cu:24):
+ * uploads geometry, allocates flow fields, builds the LU-SGS level schedule. */
+phl_status phl_create(const phl_mesh *mesh, const phl_params *p, int device, phl_ctx **out);
+/* GPUMemoryFree (GPU/Kernels/include/BasicDeviceVariables.h:689). */
+void phl_destroy(phl_ctx *ctx);
+
+/* ---------------- state transfer -------------------------------------- */
+phl_status phl_set_state(phl_ctx *ctx, const double *q);          /* [5][nTotal] primitive */
+/* CallGPUFlowVariablesCopyBack (GPU/Kernels/include/TemporaryOperations.h:33). */
+phl_status phl_get_state(phl_ctx *ctx, double *q);
+/* GPUResCopyBack (GPU/Kernels/include/TemporaryOperations.h:80). */
+phl_status phl_get_residual(phl_ctx *ctx, double *res);
+/* Any field of the path, laid out like the oracle's orc_get. */
+phl_status phl_get_field(phl_ctx *ctx, int field, double *out);
+/* Controller::InitFlow freestream fill (CFD/Solver/src/Controller.cpp:81-140). */
+phl_status phl_init_freestream(phl_ctx *ctx);
+
+/* ---------------- the path, one call per reference hook --------------- */
+/* CallGPUBoundary (NSSolverUnstruct.cpp:13406). */
+phl_status phl_boundary(phl_ctx *ctx);
+/* CallGPUComputeGamaAndTemperature (:4668) and
+ * CallGPUComputeViscousCoefficientWithoutChemical (:6724). */
+phl_status phl_gamma_t(phl_ctx *ctx);
+/* NSSolverUnstruct::ComputeGradient -> GetGradientField (:18171,18244). */
+phl_status phl_gradient(phl_ctx *ctx);
+/* NSSolverUnstruct::ComputeLimiter (:18189). */
+phl_status phl_limiter(phl_ctx *ctx);
+/* LoadResiduals + NSSolver::UpdateResiduals (NSSolverUnstruct.cpp:4118;
+ * NSSolver.cpp:71): boundary, gradients, limiter, inviscid (CallGPUGetQlQr ..
+ * CallGPULoadFlux) and viscous (CallGPUComputeVisflux) face loops. */
+phl_status phl_update_residuals(phl_ctx *ctx);
+/* CallGPUSpectrumRadiusInviscid / Viscous (:5751,5854). */
+phl_status phl_spectrum_radius(phl_ctx *ctx);
+/* CallGPULocalTimeStep + CallGPUComputeMinTimeStep + CallGPUReduceMaxTimeStep
+ * (:6001,5705,5719); step is the outer iteration number (CFL ramp). */
+phl_status phl_time_step(phl_ctx *ctx, int step);
+/* Controller::LUSGS (CFD/Solver/src/Controller.cpp:1878): Diagonal, sweeps,
+ * CallGPUUpdateFlowField.  n_fixed receives the SolutionFix count. */
+phl_status phl_lusgs(phl_ctx *ctx, int *n_fixed);
+/* ComputeResidualonGrid sums (API/Physics/src/Residual.cpp:534): sum5[m] =
+ * sum over real cells of res_m^2. */
+phl_status phl_residual_norm(phl_ctx *ctx, double *sum5);
+
+/* ---------------- host driver (Controller::Relaxation loop) ----------- */
+/* n_steps outer iterations numbered first_step..; rms_hist[i] (may be NULL)
+ * receives the density-residual RMS of each step (Residual.cpp:221-224). */
+phl_status phl_iterate(phl_ctx *ctx, int first_step, int n_steps, double *rms_hist);
+/* Iterate until R_k/R_1 <= drop or max_steps; returns the step count. */
+phl_status phl_solve(phl_ctx *ctx, int max_steps, double drop, int *steps_done, double *rms_hist);
+phl_status phl_synchronize(phl_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

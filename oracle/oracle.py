@@ -1,0 +1,182 @@
+"""ctypes wrapper of the CPU oracle (liborc.so).
+
+TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the CPU-baseline
+leg of bench.py to check the HIP implementation.  The product never imports it.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liborc.so")
+
+FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
+             diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16)
+FIELD_ROWS = dict(q=5, res=5, dqdx=5, dqdy=5, dqdz=5, limit=1, dq=5, spec_inv=1, spec_vis=1, dt=1,
+                  diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5)
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_P = ctypes.c_void_p
+_lib = None
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} not built (run phenlei_amd/build.py)")
+        lib = ctypes.CDLL(LIB_PATH)
+        lib.orc_face_metrics.argtypes = [ctypes.c_int, _I, _I, _D, _D, _D] + [_D] * 7
+        lib.orc_cell_metrics.argtypes = [ctypes.c_int] * 3 + [_I, _I, _I, _I, _I, _I, _D, _D, _D] + [_D] * 7 + [_D] * 4
+        lib.orc_skewness.argtypes = [ctypes.c_int] * 3 + [_I, _I] + [_D] * 10 + [_D]
+        lib.orc_create.restype = _P
+        lib.orc_create.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I] + [_D] * 12
+        lib.orc_destroy.argtypes = [_P]
+        lib.orc_get.argtypes = [_P, ctypes.c_int, _D]
+        lib.orc_set.argtypes = [_P, ctypes.c_int, _D]
+        lib.orc_freestream.argtypes = [_P, _D]
+        for name in ("orc_init_freestream", "orc_boundary", "orc_gamma_t", "orc_viscosity", "orc_gradient",
+                     "orc_limiter", "orc_zero_residual", "orc_inviscid_flux", "orc_viscous_flux",
+                     "orc_update_residuals", "orc_spectrum"):
+            getattr(lib, name).argtypes = [_P]
+        lib.orc_time_step.argtypes = [_P, ctypes.c_int]
+        lib.orc_lusgs.argtypes = [_P, _D]
+        lib.orc_lusgs.restype = ctypes.c_int
+        lib.orc_residual_norm.argtypes = [_P, _D]
+        lib.orc_iterate.argtypes = [_P, ctypes.c_int, ctypes.c_int, _D]
+        lib.orc_iterate.restype = ctypes.c_int
+        lib.orc_roe_flux.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]
+        lib.orc_harten.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.orc_harten.restype = ctypes.c_double
+        lib.orc_venfun.argtypes = [ctypes.c_double] * 3
+        lib.orc_venfun.restype = ctypes.c_double
+        lib.orc_mxdq.argtypes = [_D] + [ctypes.c_double] * 5 + [_D, ctypes.c_double, ctypes.c_int, _D]
+        lib.orc_entropy_coef.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.orc_entropy_coef.restype = ctypes.c_double
+        lib.orc_ptotal.argtypes = [ctypes.c_double, ctypes.c_double]
+        lib.orc_ptotal.restype = ctypes.c_double
+        _lib = lib
+    return _lib
+
+
+def face_metrics(conn):
+    lib = load()
+    nf = len(conn["left"])
+    out = {k: np.empty(nf) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")}
+    lib.orc_face_metrics(nf, _ip(conn["face_node_ptr"]), _ip(conn["face_nodes"]), _dp(conn["x"]),
+                         _dp(conn["y"]), _dp(conn["z"]), *[_dp(out[k]) for k in
+                                                          ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")])
+    return out
+
+
+def cell_metrics(conn, fm):
+    lib = load()
+    nf = len(conn["left"])
+    nb = conn["n_bface"]
+    nc = len(conn["cell_node_ptr"]) - 1
+    nt = nc + nb
+    out = {k: np.empty(nt) for k in ("xcc", "ycc", "zcc", "vol")}
+    lib.orc_cell_metrics(nc, nb, nf, _ip(conn["left"]), _ip(conn["right"]), _ip(conn["face_node_ptr"]),
+                         _ip(conn["face_nodes"]), _ip(conn["cell_node_ptr"]), _ip(conn["cell_nodes"]),
+                         _dp(conn["x"]), _dp(conn["y"]), _dp(conn["z"]),
+                         *[_dp(fm[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")],
+                         *[_dp(out[k]) for k in ("xcc", "ycc", "zcc", "vol")])
+    skew = np.empty(nc)
+    lib.orc_skewness(nc, nb, nf, _ip(conn["left"]), _ip(conn["right"]),
+                     *[_dp(fm[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")],
+                     *[_dp(out[k]) for k in ("xcc", "ycc", "zcc")], _dp(skew))
+    out["skew"] = skew
+    return out
+
+
+def roe_flux(qL, qR, n, gamma=1.4, fix=3, eps=1e-4):
+    lib = load()
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    out = np.empty(5)
+    lib.orc_roe_flux(_dp(qL), _dp(qR), float(n[0]), float(n[1]), float(n[2]), gamma, fix, eps, eps, _dp(out))
+    return out
+
+
+def mxdq(prim, n, area, gamma, dq, radius, ipn):
+    lib = load()
+    prim = np.ascontiguousarray(prim, dtype=np.float64)
+    dq = np.ascontiguousarray(dq, dtype=np.float64)
+    out = np.empty(5)
+    lib.orc_mxdq(_dp(prim), float(n[0]), float(n[1]), float(n[2]), area, gamma, _dp(dq), radius, ipn, _dp(out))
+    return out
+
+
+class OracleSolver:
+    """Sequential FP64 restatement of NSSolverUnstruct on one zone."""
+
+    def __init__(self, conn, metrics, params):
+        lib = load()
+        self.n_cell = len(conn["cell_node_ptr"]) - 1
+        self.n_bface = conn["n_bface"]
+        self.n_total = self.n_cell + self.n_bface
+        self._keep = [np.ascontiguousarray(conn[k], dtype=np.int32) for k in ("left", "right")]
+        bc = np.ascontiguousarray(conn["bc_type"], dtype=np.int32)
+        if bc.size == 0:
+            bc = np.zeros(1, np.int32)
+        self._keep.append(bc)
+        m = [np.ascontiguousarray(metrics[k], dtype=np.float64) for k in
+             ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area", "xcc", "ycc", "zcc", "vol", "skew")]
+        self._keep += m
+        self.params = params
+        self._h = lib.orc_create(ctypes.byref(params), self.n_cell, self.n_bface, len(conn["left"]),
+                                 _ip(self._keep[0]), _ip(self._keep[1]), _ip(self._keep[2]), *[_dp(a) for a in m])
+
+    def get(self, name):
+        rows = FIELD_ROWS[name]
+        out = np.empty((rows, self.n_total))
+        load().orc_get(self._h, FIELD[name], _dp(out))
+        return out if rows > 1 else out[0]
+
+    def set(self, name, value):
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
+        load().orc_set(self._h, FIELD[name], _dp(v))
+
+    def freestream(self):
+        out = np.empty(5)
+        load().orc_freestream(self._h, _dp(out))
+        return out
+
+    def __getattr__(self, name):
+        fn = getattr(load(), "orc_" + name, None)
+        if fn is None:
+            raise AttributeError(name)
+        return lambda *a: fn(self._h, *a)
+
+    def lusgs(self):
+        sweeps = np.empty(8)
+        return load().orc_lusgs(self._h, _dp(sweeps))
+
+    def residual_norm(self):
+        out = np.empty(5)
+        load().orc_residual_norm(self._h, _dp(out))
+        return out
+
+    def iterate(self, first_step, n_steps):
+        hist = np.empty(max(n_steps, 1))
+        nfix = load().orc_iterate(self._h, first_step, n_steps, _dp(hist))
+        return hist[:n_steps], nfix
+
+    def close(self):
+        if getattr(self, "_h", None):
+            load().orc_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,1427 @@
+/*
+ * phl_oracle.c -- CPU oracle for the unstructured residual + LU-SGS path.
+ *
+ * TEST INFRASTRUCTURE ONLY: a sequential FP64 restatement of the reference
+ * solver (PHengLEI NSSolverUnstruct, one zone, perfect gas) used to check the
+ * HIP implementation.  It is loaded by tests/, by the smoke check and by the
+ * CPU-baseline leg of bench.py, never by the product library.
+ *
+ * Build with -O2 -ffp-contract=off so that every multiply and add rounds as
+ * written, like the reference's own x86-64 build.
+ *
+ * Index conventions follow the reference: faces 0..nBFace-1 are boundary
+ * faces, the ghost cell of boundary face f is nCell+f
+ * (API/Geometry/src/Geo_UnstructGrid.cpp:2007), cell arrays are SoA over
+ * nTotal = nCell+nBFace, primitive variables are (rho,u,v,w,p)
+ * (API/Common/include/Constants.h:252-282).
+ */
+#include "phl_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#define IR 0
+#define IU 1
+#define IV 2
+#define IW 3
+#define IP 4
+
+static const double SMALL = 1.0e-40; /* API/Common/include/Precision.h:25 */
+static const double TINY  = 1.0e-40; /* API/Common/include/Precision.h:26 */
+static const double LARGE = 1.0e40;  /* API/Common/include/Precision.h:24 */
+static const double PI_   = 3.141592653589793; /* Constants.h:110 */
+
+static inline double MIN(double a, double b) { return (a < b) ? a : b; } /* Math_BasisFunction.h:31 */
+static inline double MAX(double a, double b) { return (a > b) ? a : b; } /* Math_BasisFunction.h:44 */
+static inline double ABS(double a) { return (a < 0) ? -a : a; }          /* Math_BasisFunction.h:58 */
+
+struct orc_solver {
+    orc_params p;
+    int nCell, nBFace, nFace, nTotal;
+    int *left, *right, *bcType;
+    double *xfc, *yfc, *zfc, *xfn, *yfn, *zfn, *area;
+    double *xcc, *ycc, *zcc, *vol, *skew;
+    double avgVol;
+    int *c2fPtr, *c2f;
+    /* flow fields */
+    double *q[5], *res[5], *dqdx[5], *dqdy[5], *dqdz[5], *dq[5], *dflux[5];
+    double *lim, *specInv, *specVis, *dt, *diag, *gama, *t, *visl, *vist;
+    double *dtdx, *dtdy, *dtdz;
+    /* derived constants */
+    double primInf[5];
+    double R;           /* coefficientOfStateEquation, PerfectGas.cpp:1401 */
+    double pTotal;      /* PerfectGas.cpp:1616-1641 */
+    double entropyCoef; /* Param_NSSolver.cpp:225-266 */
+    double tsuth;       /* 110.4 / T_ref, PerfectGas.cpp:1480-1494 */
+};
+
+/* ------------------------------------------------------------------ */
+/* Reference constants                                                 */
+/* ------------------------------------------------------------------ */
+
+/* Param_NSSolver::SetEntropyFixCoefficients, CFD/NSSolver/src/Param_NSSolver.cpp:225-266 */
+double orc_entropy_coef(double mach, double scale)
+{
+    double e;
+    if (mach <= 0.6) {
+        e = 0.0001;
+    } else if (mach <= 2.0) {
+        double x0 = 0.6, x1 = 2.0, y0 = 0.0001, y1 = 0.01;
+        e = y0 * (mach - x1) / (x0 - x1) + y1 * (mach - x0) / (x1 - x0);
+    } else if (mach <= 5.0) {
+        double x0 = 2.0, x1 = 5.0, y0 = 0.01, y1 = 0.5;
+        e = y0 * (mach - x1) / (x0 - x1) + y1 * (mach - x0) / (x1 - x0);
+    } else {
+        e = 1.0;
+    }
+    return e * scale;
+}
+
+/* PerfectGas::ComputePTotal, API/Physics/src/PerfectGas.cpp:1616-1641 */
+double orc_ptotal(double mach, double gama)
+{
+    double R = 1.0 / (gama * mach * mach);
+    double pref = R * 1.0 * 1.0 / 1.0;
+    double m12 = mach * mach;
+    double gamp1 = gama + 1.0;
+    double gamm1 = gama - 1.0;
+    double c1 = 2.0 / gamm1 + m12;
+    double c2 = 2.0 * gama / gamm1 * m12 - 1.0;
+    double cc = gama / (gama - 1.0);
+    double t0t = 1.0 + 0.5 * gamm1 * c1 / c2;
+    if (mach > 1.0) pref *= (2.0 * gama / gamp1 * m12 - gamm1 / gamp1);
+    return pref * pow(t0t, cc);
+}
+
+/* ------------------------------------------------------------------ */
+/* Geometry                                                            */
+/* ------------------------------------------------------------------ */
+
+/* UnstructGrid::ComputeFaceCenterAreaNormal3DNew2,
+ * API/Geometry/src/Geo_UnstructGrid.cpp:5911-6157 (static grid). */
+void orc_face_metrics(int nFace, const int *fp, const int *fn,
+                      const double *x, const double *y, const double *z,
+                      double *xfc, double *yfc, double *zfc,
+                      double *xfn, double *yfn, double *zfn, double *area)
+{
+    for (int f = 0; f < nFace; ++f) {
+        int nn = fp[f + 1] - fp[f];
+        xfc[f] = 0.0; yfc[f] = 0.0; zfc[f] = 0.0;
+        for (int j = 0; j < nn; ++j) {
+            int n1 = fn[fp[f] + j];
+            xfc[f] += x[n1]; yfc[f] += y[n1]; zfc[f] += z[n1];
+        }
+        double coef = 1.0 / nn;
+        xfc[f] *= coef; yfc[f] *= coef; zfc[f] *= coef;
+    }
+    for (int cyc = 0; cyc <= 4; ++cyc) {
+        int last = (cyc == 4);
+        for (int f = 0; f < nFace; ++f) {
+            int nn = fp[f + 1] - fp[f];
+            double xt = 0.0, yt = 0.0, zt = 0.0, at = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+            for (int j = 0; j < nn; ++j) {
+                int p1 = fn[fp[f] + j];
+                int p2 = fn[fp[f] + (j + 1) % nn];
+                double dx1 = x[p1] - xfc[f], dy1 = y[p1] - yfc[f], dz1 = z[p1] - zfc[f];
+                double dx2 = x[p2] - xfc[f], dy2 = y[p2] - yfc[f], dz2 = z[p2] - zfc[f];
+                double anx = dy1 * dz2 - dy2 * dz1;
+                double any = dz1 * dx2 - dz2 * dx1;
+                double anz = dx1 * dy2 - dx2 * dy1;
+                double a = sqrt(anx * anx + any * any + anz * anz);
+                double x00 = x[p1] + x[p2] + xfc[f];
+                double y00 = y[p1] + y[p2] + yfc[f];
+                double z00 = z[p1] + z[p2] + zfc[f];
+                at += a;
+                xt += a * x00; yt += a * y00; zt += a * z00;
+                sx += anx; sy += any; sz += anz;
+            }
+            if (at > SMALL) {
+                double a = 1.0 / at / 3.0;
+                xfc[f] = xt * a; yfc[f] = yt * a; zfc[f] = zt * a;
+            }
+            if (last) {
+                double a = sqrt(sx * sx + sy * sy + sz * sz);
+                area[f] = a * 0.5;
+                double oa = 1.0 / (a + SMALL);
+                xfn[f] = sx * oa; yfn[f] = sy * oa; zfn[f] = sz * oa;
+            }
+        }
+    }
+}
+
+/* UnstructGrid::ComputeCellCenterVol3DNew2,
+ * API/Geometry/src/Geo_UnstructGrid.cpp:6545-6820.  Output arrays are sized
+ * nTotal; the ghost centroid is the mirror of the owner across the face. */
+void orc_cell_metrics(int nCell, int nBFace, int nFace, const int *left, const int *right,
+                      const int *fp, const int *fn, const int *cp, const int *cn,
+                      const double *x, const double *y, const double *z,
+                      const double *xfc, const double *yfc, const double *zfc,
+                      const double *xfn, const double *yfn, const double *zfn,
+                      const double *area,
+                      double *xcc, double *ycc, double *zcc, double *vol)
+{
+    for (int c = 0; c < nCell; ++c) {
+        int nn = cp[c + 1] - cp[c];
+        xcc[c] = 0.0; ycc[c] = 0.0; zcc[c] = 0.0;
+        for (int j = 0; j < nn; ++j) {
+            int n2 = cn[cp[c] + j];
+            xcc[c] += x[n2]; ycc[c] += y[n2]; zcc[c] += z[n2];
+        }
+        xcc[c] /= nn; ycc[c] /= nn; zcc[c] /= nn;
+    }
+    double *xt = (double *)calloc((size_t)nCell, sizeof(double));
+    double *yt = (double *)calloc((size_t)nCell, sizeof(double));
+    double *zt = (double *)calloc((size_t)nCell, sizeof(double));
+    for (int c = 0; c < nCell; ++c) vol[c] = 0.0;
+    for (int f = 0; f < nFace; ++f) {
+        int nn = fp[f + 1] - fp[f];
+        int le = left[f], re = right[f];
+        int interior = f >= nBFace;
+        for (int j = 0; j < nn; ++j) {
+            int p2 = fn[fp[f] + j];
+            int p3 = fn[fp[f] + (j + 1) % nn];
+            double x21 = x[p2] - xfc[f], y21 = y[p2] - yfc[f], z21 = z[p2] - zfc[f];
+            double x31 = x[p3] - xfc[f], y31 = y[p3] - yfc[f], z31 = z[p3] - zfc[f];
+            double nx = y21 * z31 - y31 * z21;
+            double ny = z21 * x31 - z31 * x21;
+            double nz = x21 * y31 - x31 * y21;
+            double x41 = xcc[le] - xfc[f], y41 = ycc[le] - yfc[f], z41 = zcc[le] - zfc[f];
+            double xc = xfc[f] + x[p2] + x[p3] + xcc[le];
+            double yc = yfc[f] + y[p2] + y[p3] + ycc[le];
+            double zc = zfc[f] + z[p2] + z[p3] + zcc[le];
+            double tmp = nx * x41 + ny * y41 + nz * z41;
+            tmp = -tmp;
+            xc *= tmp; yc *= tmp; zc *= tmp;
+            xt[le] += xc; yt[le] += yc; zt[le] += zc; vol[le] += tmp;
+            if (interior) {
+                x41 = xcc[re] - xfc[f]; y41 = ycc[re] - yfc[f]; z41 = zcc[re] - zfc[f];
+                xc = xfc[f] + x[p2] + x[p3] + xcc[re];
+                yc = yfc[f] + y[p2] + y[p3] + ycc[re];
+                zc = zfc[f] + z[p2] + z[p3] + zcc[re];
+                tmp = nx * x41 + ny * y41 + nz * z41;
+                xc *= tmp; yc *= tmp; zc *= tmp;
+                xt[re] += xc; yt[re] += yc; zt[re] += zc; vol[re] += tmp;
+            }
+        }
+    }
+    for (int c = 0; c < nCell; ++c) {
+        double tmp = 1.0 / (4.0 * vol[c] + SMALL);
+        xt[c] *= tmp; yt[c] *= tmp; zt[c] *= tmp;
+        vol[c] /= 6.0;
+    }
+    int *mark = (int *)malloc((size_t)nCell * sizeof(int));
+    for (int c = 0; c < nCell; ++c) mark[c] = 1;
+    for (int f = 0; f < nFace; ++f) {
+        int le = left[f];
+        double dx = xt[le] - xfc[f], dy = yt[le] - yfc[f], dz = zt[le] - zfc[f];
+        double tmp = dx * xfn[f] + dy * yfn[f] + dz * zfn[f];
+        if (tmp > 0.0) mark[le] = 0;
+        if (f >= nBFace) {
+            int re = right[f];
+            dx = xt[re] - xfc[f]; dy = yt[re] - yfc[f]; dz = zt[re] - zfc[f];
+            tmp = dx * xfn[f] + dy * yfn[f] + dz * zfn[f];
+            if (tmp < 0.0) mark[re] = 0;
+        }
+    }
+    for (int c = 0; c < nCell; ++c) {
+        if (mark[c]) { xcc[c] = xt[c]; ycc[c] = yt[c]; zcc[c] = zt[c]; }
+    }
+    for (int f = 0; f < nBFace; ++f) {
+        int le = left[f];
+        int re = nCell + f;
+        if (area[f] > SMALL) {
+            double tmp = 2.0 * ((xcc[le] - xfc[f]) * xfn[f] + (ycc[le] - yfc[f]) * yfn[f] +
+                                (zcc[le] - zfc[f]) * zfn[f]);
+            xcc[re] = xcc[le] - xfn[f] * tmp;
+            ycc[re] = ycc[le] - yfn[f] * tmp;
+            zcc[re] = zcc[le] - zfn[f] * tmp;
+        } else {
+            xcc[re] = -xcc[le] + 2.0 * xfc[f];
+            ycc[re] = -ycc[le] + 2.0 * yfc[f];
+            zcc[re] = -zcc[le] + 2.0 * zfc[f];
+        }
+        vol[re] = vol[le];
+    }
+    free(mark); free(xt); free(yt); free(zt);
+}
+
+/* UnstructGrid::SkewnessSummary, API/Geometry/src/Geo_UnstructGrid.cpp:2206-2300:
+ * per-cell minimum over its faces of the complement angle (degrees). */
+void orc_skewness(int nCell, int nBFace, int nFace, const int *left, const int *right,
+                  const double *xfc, const double *yfc, const double *zfc,
+                  const double *xfn, const double *yfn, const double *zfn, const double *area,
+                  const double *xcc, const double *ycc, const double *zcc, double *skew)
+{
+    const double straightAngle = 180.0;
+    for (int c = 0; c < nCell; ++c) skew[c] = straightAngle;
+    for (int f = nBFace; f < nFace; ++f) {
+        if (area[f] <= TINY) continue;
+        int le = left[f], re = right[f];
+        double x1 = xfc[f] - xcc[le], y1 = yfc[f] - ycc[le], z1 = zfc[f] - zcc[le];
+        double d1 = sqrt(x1 * x1 + y1 * y1 + z1 * z1);
+        double p1 = (xfn[f] * x1 + yfn[f] * y1 + zfn[f] * z1) / (d1 + TINY);
+        if (p1 > 1.0) p1 = 1.0;
+        if (p1 < -1.0) p1 = -1.0;
+        double x2 = xcc[re] - xfc[f], y2 = ycc[re] - yfc[f], z2 = zcc[re] - zfc[f];
+        double d2 = sqrt(x2 * x2 + y2 * y2 + z2 * z2);
+        double p2 = (xfn[f] * x2 + yfn[f] * y2 + zfn[f] * z2) / (d2 + TINY);
+        if (p2 > 1.0) p2 = 1.0;
+        if (p2 < -1.0) p2 = -1.0;
+        double angle = MIN(asin(p1), asin(p2)) * straightAngle / PI_;
+        skew[le] = MIN(skew[le], angle);
+        skew[re] = MIN(skew[re], angle);
+    }
+    for (int f = 0; f < nBFace; ++f) {
+        if (area[f] <= TINY) continue;
+        int le = left[f];
+        double x1 = xfc[f] - xcc[le], y1 = yfc[f] - ycc[le], z1 = zfc[f] - zcc[le];
+        double d1 = sqrt(x1 * x1 + y1 * y1 + z1 * z1);
+        double p1 = (xfn[f] * x1 + yfn[f] * y1 + zfn[f] * z1) / (d1 + TINY);
+        if (p1 > 1.0) p1 = 1.0;
+        if (p1 < -1.0) p1 = -1.0;
+        double angle = asin(p1) * straightAngle / PI_;
+        skew[le] = MIN(skew[le], angle);
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Solver state                                                        */
+/* ------------------------------------------------------------------ */
+
+static double *dup_d(const double *a, int n)
+{
+    double *b = (double *)malloc((size_t)n * sizeof(double));
+    memcpy(b, a, (size_t)n * sizeof(double));
+    return b;
+}
+static int *dup_i(const int *a, int n)
+{
+    int *b = (int *)malloc((size_t)n * sizeof(int));
+    memcpy(b, a, (size_t)n * sizeof(int));
+    return b;
+}
+static double *zeros(int n) { return (double *)calloc((size_t)n, sizeof(double)); }
+
+/* Freestream primitive state: PerfectGas::ComputeReferencePrimitive
+ * (API/Physics/src/PerfectGas.cpp:1276-1300) with the velocity direction of
+ * GlobalBoundaryCondition::SetSpeedDrictionForReferenceVar
+ * (API/Geometry/src/Geo_SimpleBC.cpp:1200-1215). */
+void orc_freestream(const orc_solver *s, double *prim)
+{
+    double attack = s->p.alpha_deg * PI_ / 180.0;
+    double sideslip = s->p.beta_deg * PI_ / 180.0;
+    prim[IR] = 1.0;
+    prim[IU] = 1.0 * (cos(attack) * cos(sideslip));
+    prim[IV] = 1.0 * (sin(attack) * cos(sideslip));
+    prim[IW] = 1.0 * sin(sideslip);
+    prim[IP] = s->R * 1.0 * 1.0 / 1.0;
+}
+
+orc_solver *orc_create(const orc_params *p, int nCell, int nBFace, int nFace,
+                       const int *left, const int *right, const int *bcType,
+                       const double *xfc, const double *yfc, const double *zfc,
+                       const double *xfn, const double *yfn, const double *zfn,
+                       const double *area, const double *xcc, const double *ycc,
+                       const double *zcc, const double *vol, const double *skew)
+{
+    orc_solver *s = (orc_solver *)calloc(1, sizeof(orc_solver));
+    s->p = *p;
+    s->nCell = nCell; s->nBFace = nBFace; s->nFace = nFace;
+    int nT = s->nTotal = nCell + nBFace;
+    s->left = dup_i(left, nFace); s->right = dup_i(right, nFace);
+    s->bcType = dup_i(bcType, nBFace > 0 ? nBFace : 1);
+    s->xfc = dup_d(xfc, nFace); s->yfc = dup_d(yfc, nFace); s->zfc = dup_d(zfc, nFace);
+    s->xfn = dup_d(xfn, nFace); s->yfn = dup_d(yfn, nFace); s->zfn = dup_d(zfn, nFace);
+    s->area = dup_d(area, nFace);
+    s->xcc = dup_d(xcc, nT); s->ycc = dup_d(ycc, nT); s->zcc = dup_d(zcc, nT);
+    s->vol = dup_d(vol, nT); s->skew = dup_d(skew, nCell);
+    /* Region.cpp:3203: mean volume over real cells. */
+    double tv = 0.0;
+    for (int c = 0; c < nCell; ++c) tv += vol[c];
+    s->avgVol = tv / nCell;
+    /* cell -> face in increasing face order (as built from the face loop). */
+    s->c2fPtr = (int *)calloc((size_t)nCell + 1, sizeof(int));
+    for (int f = 0; f < nFace; ++f) {
+        s->c2fPtr[left[f] + 1]++;
+        if (f >= nBFace) s->c2fPtr[right[f] + 1]++;
+    }
+    for (int c = 0; c < nCell; ++c) s->c2fPtr[c + 1] += s->c2fPtr[c];
+    s->c2f = (int *)malloc((size_t)s->c2fPtr[nCell] * sizeof(int));
+    int *fill = (int *)calloc((size_t)nCell, sizeof(int));
+    for (int f = 0; f < nFace; ++f) {
+        int le = left[f];
+        s->c2f[s->c2fPtr[le] + fill[le]++] = f;
+        if (f >= nBFace) { int re = right[f]; s->c2f[s->c2fPtr[re] + fill[re]++] = f; }
+    }
+    free(fill);
+    for (int m = 0; m < 5; ++m) {
+        s->q[m] = zeros(nT); s->res[m] = zeros(nT); s->dq[m] = zeros(nT);
+        s->dqdx[m] = zeros(nT); s->dqdy[m] = zeros(nT); s->dqdz[m] = zeros(nT);
+        s->dflux[m] = zeros(nT);
+    }
+    s->lim = zeros(nT); s->specInv = zeros(nT); s->specVis = zeros(nT);
+    s->dt = zeros(nT); s->diag = zeros(nT); s->gama = zeros(nT); s->t = zeros(nT);
+    s->visl = zeros(nT); s->vist = zeros(nT);
+    s->dtdx = zeros(nT); s->dtdy = zeros(nT); s->dtdz = zeros(nT);
+    s->R = 1.0 / (p->gamma * p->mach * p->mach);
+    s->pTotal = orc_ptotal(p->mach, p->gamma);
+    s->entropyCoef = orc_entropy_coef(p->mach, p->entropy_scale);
+    s->tsuth = 110.4 / p->t_ref;
+    orc_freestream(s, s->primInf);
+    for (int c = 0; c < nT; ++c) s->gama[c] = p->gamma;
+    return s;
+}
+
+void orc_destroy(orc_solver *s)
+{
+    if (!s) return;
+    free(s->left); free(s->right); free(s->bcType);
+    free(s->xfc); free(s->yfc); free(s->zfc); free(s->xfn); free(s->yfn); free(s->zfn);
+    free(s->area); free(s->xcc); free(s->ycc); free(s->zcc); free(s->vol); free(s->skew);
+    free(s->c2fPtr); free(s->c2f);
+    for (int m = 0; m < 5; ++m) {
+        free(s->q[m]); free(s->res[m]); free(s->dq[m]); free(s->dqdx[m]);
+        free(s->dqdy[m]); free(s->dqdz[m]); free(s->dflux[m]);
+    }
+    free(s->lim); free(s->specInv); free(s->specVis); free(s->dt); free(s->diag);
+    free(s->gama); free(s->t); free(s->visl); free(s->vist);
+    free(s->dtdx); free(s->dtdy); free(s->dtdz);
+    free(s);
+}
+
+static double **field5(orc_solver *s, int field)
+{
+    switch (field) {
+    case ORC_Q: return s->q;
+    case ORC_RES: return s->res;
+    case ORC_DQDX: return s->dqdx;
+    case ORC_DQDY: return s->dqdy;
+    case ORC_DQDZ: return s->dqdz;
+    case ORC_DQ: return s->dq;
+    case ORC_DELTA_FLUX: return s->dflux;
+    default: return NULL;
+    }
+}
+static double *field1(orc_solver *s, int field)
+{
+    switch (field) {
+    case ORC_LIMIT: return s->lim;
+    case ORC_SPEC_INV: return s->specInv;
+    case ORC_SPEC_VIS: return s->specVis;
+    case ORC_DT: return s->dt;
+    case ORC_DIAG: return s->diag;
+    case ORC_T: return s->t;
+    case ORC_VISL: return s->visl;
+    case ORC_DTDX: return s->dtdx;
+    case ORC_DTDY: return s->dtdy;
+    case ORC_DTDZ: return s->dtdz;
+    default: return NULL;
+    }
+}
+
+/* Fields are exchanged SoA over nTotal: five-component fields as [5][nTotal]. */
+int orc_get(orc_solver *s, int field, double *out)
+{
+    double **f5 = field5(s, field);
+    if (f5) {
+        for (int m = 0; m < 5; ++m) memcpy(out + (size_t)m * s->nTotal, f5[m], (size_t)s->nTotal * sizeof(double));
+        return 5;
+    }
+    double *f1 = field1(s, field);
+    if (f1) { memcpy(out, f1, (size_t)s->nTotal * sizeof(double)); return 1; }
+    return 0;
+}
+
+int orc_set(orc_solver *s, int field, const double *in)
+{
+    double **f5 = field5(s, field);
+    if (f5) {
+        for (int m = 0; m < 5; ++m) memcpy(f5[m], in + (size_t)m * s->nTotal, (size_t)s->nTotal * sizeof(double));
+        return 5;
+    }
+    double *f1 = field1(s, field);
+    if (f1) { memcpy(f1, in, (size_t)s->nTotal * sizeof(double)); return 1; }
+    return 0;
+}
+
+/* Controller::InitFlow: uniform freestream on every cell and ghost. */
+void orc_init_freestream(orc_solver *s)
+{
+    for (int m = 0; m < 5; ++m)
+        for (int c = 0; c < s->nTotal; ++c) s->q[m][c] = s->primInf[m];
+}
+
+/* ------------------------------------------------------------------ */
+/* Boundary conditions: NSSolverUnstruct::Boundary                     */
+/* CFD/NSSolver/src/NSSolverUnstruct.cpp:13404-13530                  */
+/* ------------------------------------------------------------------ */
+
+static int is_wall(int bc) { return bc == ORC_BC_SOLID_SURFACE; } /* Geo_SimpleBC.h:389 */
+
+/* SymmetryBCRegion, NSSolverUnstruct.cpp:19413-19449 (also the inviscid wall). */
+static void bc_symmetry(orc_solver *s, int f)
+{
+    int le = s->left[f], re = s->right[f];
+    for (int m = 0; m < 5; ++m) s->q[m][re] = s->q[m][le];
+    double uL = s->q[IU][le], vL = s->q[IV][le], wL = s->q[IW][le];
+    double vn = s->xfn[f] * uL + s->yfn[f] * vL + s->zfn[f] * wL - 0.0;
+    s->q[IU][re] = uL - 2.0 * s->xfn[f] * vn;
+    s->q[IV][re] = vL - 2.0 * s->yfn[f] * vn;
+    s->q[IW][re] = wL - 2.0 * s->zfn[f] * vn;
+}
+
+/* ViscousAdiabaticWallBCRegion, NSSolverUnstruct.cpp:20988-21104 (static wall). */
+static void bc_viscous_wall(orc_solver *s, int f)
+{
+    int le = s->left[f], re = s->right[f];
+    for (int m = 0; m < 5; ++m) s->q[m][re] = s->q[m][le];
+    s->q[IU][re] = -s->q[IU][le] + 2.0 * 0.0;
+    s->q[IV][re] = -s->q[IV][le] + 2.0 * 0.0;
+    s->q[IW][re] = -s->q[IW][le] + 2.0 * 0.0;
+}
+
+/* FarfieldBCRegion, NSSolverUnstruct.cpp:21195-21330 (no preconditioning). */
+static void bc_farfield(orc_solver *s, int f)
+{
+    int le = s->left[f], re = s->right[f];
+    const double *pf = s->primInf;
+    double gl = s->gama[le];
+    double refGama = s->p.gamma;
+    double rO = pf[IR], uO = pf[IU], vO = pf[IV], wO = pf[IW], pO = pf[IP];
+    double cO = sqrt(ABS(refGama * pO / rO));
+    for (int m = 0; m < 5; ++m) s->q[m][re] = s->q[m][le];
+    double rI = s->q[IR][le], uI = s->q[IU][le], vI = s->q[IV][le], wI = s->q[IW][le], pI = s->q[IP][le];
+    double nx = s->xfn[f], ny = s->yfn[f], nz = s->zfn[f];
+    double vnO = nx * uO + ny * vO + nz * wO - 0.0;
+    double vnI = nx * uI + ny * vI + nz * wI - 0.0;
+    double cI = sqrt(ABS(gl * pI / rI));
+    double gamm1 = gl - 1.0;
+    double velI = sqrt(uI * uI + vI * vI + wI * wI);
+    if (velI > cI) {
+        if (vnI >= 0.0) {
+            for (int m = 0; m < 5; ++m) s->q[m][re] = s->q[m][le];
+        } else {
+            for (int m = 0; m < 5; ++m) s->q[m][re] = pf[m];
+        }
+    } else {
+        double Rp = vnI + 2.0 * cI / gamm1;
+        double Rm = vnO - 2.0 * cO / gamm1;
+        double vnB = 0.5 * (Rp + Rm);
+        double cB = 0.25 * (Rp - Rm) * gamm1;
+        double S, tx, ty, tz;
+        if (vnB >= 0.0) {
+            S = pI / pow(rI, gl);
+            tx = uI - nx * vnI; ty = vI - ny * vnI; tz = wI - nz * vnI;
+        } else {
+            S = pO / pow(rO, gl);
+            tx = uO - nx * vnO; ty = vO - ny * vnO; tz = wO - nz * vnO;
+        }
+        double rB = pow((cB * cB / (S * gl)), 1.0 / gamm1);
+        s->q[IR][re] = rB;
+        s->q[IU][re] = tx + nx * vnB;
+        s->q[IV][re] = ty + ny * vnB;
+        s->q[IW][re] = tz + nz * vnB;
+        s->q[IP][re] = cB * cB * rB / gl;
+    }
+}
+
+/* OutflowBCRegion, NSSolverUnstruct.cpp:19326-19366. */
+static void bc_outflow(orc_solver *s, int f)
+{
+    int le = s->left[f], re = s->right[f];
+    for (int m = 0; m < 5; ++m) s->q[m][re] = s->q[m][le];
+}
+
+void orc_boundary(orc_solver *s)
+{
+    for (int f = 0; f < s->nBFace; ++f) {
+        int bc = s->bcType[f];
+        if (bc == ORC_BC_EXTRAPOLATION || bc == ORC_BC_OUTFLOW) bc_outflow(s, f);
+        else if (is_wall(bc)) {
+            if (!s->p.viscous) bc_symmetry(s, f);
+            else bc_viscous_wall(s, f); /* adiabatic wall; isothermal walls are not on the path */
+        } else if (bc == ORC_BC_SYMMETRY) bc_symmetry(s, f);
+        else if (bc == ORC_BC_FARFIELD) bc_farfield(s, f);
+    }
+}
+
+/* ComputeGamaAndTemperature, NSSolverUnstruct.cpp:4666-4732 (perfect gas). */
+void orc_gamma_t(orc_solver *s)
+{
+    for (int c = 0; c < s->nTotal; ++c) {
+        s->gama[c] = s->p.gamma;
+        s->t[c] = s->q[IP][c] / (s->R * s->q[IR][c]);
+    }
+}
+
+/* ComputeViscousCoefficientWithoutChemical, NSSolverUnstruct.cpp:6722-6746. */
+void orc_viscosity(orc_solver *s)
+{
+    double ts = s->tsuth;
+    for (int c = 0; c < s->nTotal; ++c) {
+        double tt = s->t[c];
+        s->visl[c] = tt * sqrt(tt) * (1.0 + ts) / (tt + ts);
+        s->vist[c] = 0.0;
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Gradients: UnstructGrid::CompGradientGGCell                         */
+/* API/Geometry/src/Geo_UnstructGrid.cpp:3220-3284 (ghost gradient 0)  */
+/* ------------------------------------------------------------------ */
+static void grad_ggcell(const orc_solver *s, const double *q, double *gx, double *gy, double *gz)
+{
+    int nT = s->nTotal;
+    for (int c = 0; c < nT; ++c) { gx[c] = 0.0; gy[c] = 0.0; gz[c] = 0.0; }
+    for (int f = 0; f < s->nBFace; ++f) {
+        int le = s->left[f], re = f + s->nCell;
+        double qfc = 0.5 * (q[le] + q[re]);
+        double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
+        gx[le] += qfc * nx; gy[le] += qfc * ny; gz[le] += qfc * nz;
+    }
+    for (int f = s->nBFace; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double qfc = 0.5 * (q[le] + q[re]);
+        double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
+        gx[le] += qfc * nx; gy[le] += qfc * ny; gz[le] += qfc * nz;
+        gx[re] -= qfc * nx; gy[re] -= qfc * ny; gz[re] -= qfc * nz;
+    }
+    for (int c = 0; c < s->nCell; ++c) {
+        double ovol = 1.0 / s->vol[c];
+        gx[c] *= ovol; gy[c] *= ovol; gz[c] *= ovol;
+    }
+}
+
+/* NSSolverUnstruct::GetGradientField, NSSolverUnstruct.cpp:18244-18338:
+ * gradients of the five primitives and of the temperature. */
+void orc_gradient(orc_solver *s)
+{
+    for (int m = 0; m < 5; ++m) grad_ggcell(s, s->q[m], s->dqdx[m], s->dqdy[m], s->dqdz[m]);
+    grad_ggcell(s, s->t, s->dtdx, s->dtdy, s->dtdz);
+}
+
+/* ------------------------------------------------------------------ */
+/* Venkatakrishnan limiter (ivencat = 7, scalar, limited by rho and p)  */
+/* Limiter::Calculation, API/Physics/src/Limiter.cpp:47-145            */
+/* ------------------------------------------------------------------ */
+
+/* VenFun, API/Math/include/Math_Limiter.h:174-180 */
+double orc_venfun(double x, double y, double eps)
+{
+    double x2 = x * x;
+    double xy = x * y;
+    double y2 = y * y;
+    return ((x2 + eps + 2.0 * xy) / (x2 + 2.0 * y2 + xy + eps));
+}
+
+/* Limiter::MinMaxDiff, Limiter.cpp:1202-1254 (no interface faces here). */
+static void min_max_diff(const orc_solver *s, double *dmin, double *dmax, const double *q)
+{
+    for (int c = 0; c < s->nCell; ++c) { dmin[c] = q[c]; dmax[c] = q[c]; }
+    for (int f = s->nBFace; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        dmin[le] = MIN(dmin[le], q[re]); dmax[le] = MAX(dmax[le], q[re]);
+        dmin[re] = MIN(dmin[re], q[le]); dmax[re] = MAX(dmax[re], q[le]);
+    }
+    for (int c = 0; c < s->nCell; ++c) { dmin[c] -= q[c]; dmax[c] -= q[c]; }
+}
+
+/* Limiter::VencatLimiter7, Limiter.cpp:1091-1200. */
+static void vencat7(orc_solver *s, double *limit, const double *q, const double *gx,
+                    const double *gy, const double *gz)
+{
+    int nC = s->nCell;
+    double *dmin = (double *)malloc((size_t)s->nTotal * sizeof(double));
+    double *dmax = (double *)malloc((size_t)s->nTotal * sizeof(double));
+    min_max_diff(s, dmin, dmax, q);
+    double n = 3.0;
+    double *eps = (double *)malloc((size_t)nC * sizeof(double));
+    double eps_tmp = s->p.venkat_k * s->p.venkat_k * s->p.venkat_k;
+    eps_tmp *= 1.0 / pow(pow(s->avgVol, 1.0 / 3), n);
+    for (int c = 0; c < nC; ++c) {
+        eps[c] = eps_tmp * q[c] * q[c];
+        eps[c] *= pow(pow(s->vol[c], 1.0 / 3), n);
+    }
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double dx = s->xfc[f] - s->xcc[le], dy = s->yfc[f] - s->ycc[le], dz = s->zfc[f] - s->zcc[le];
+        double dqf = gx[le] * dx + gy[le] * dy + gz[le] * dz;
+        if (dqf > SMALL) limit[le] = MIN(limit[le], orc_venfun(dmax[le], dqf, eps[le]));
+        else if (dqf < -SMALL) limit[le] = MIN(limit[le], orc_venfun(dmin[le], dqf, eps[le]));
+        if (f < s->nBFace) continue;
+        dx = s->xfc[f] - s->xcc[re]; dy = s->yfc[f] - s->ycc[re]; dz = s->zfc[f] - s->zcc[re];
+        dqf = gx[re] * dx + gy[re] * dy + gz[re] * dz;
+        if (dqf > SMALL) limit[re] = MIN(limit[re], orc_venfun(dmax[re], dqf, eps[re]));
+        else if (dqf < -SMALL) limit[re] = MIN(limit[re], orc_venfun(dmin[re], dqf, eps[re]));
+    }
+    free(dmin); free(dmax); free(eps);
+}
+
+/* SetLimitBoundary, Geo_UnstructGrid.cpp:10744-10777: zero on both sides of
+ * every boundary face that is not interface, symmetry, wall or outflow. */
+static void set_limit_boundary(orc_solver *s, double *limit)
+{
+    for (int f = 0; f < s->nBFace; ++f) {
+        int bc = s->bcType[f];
+        if (bc != ORC_BC_SYMMETRY && !is_wall(bc) && bc != ORC_BC_OUTFLOW) {
+            limit[s->left[f]] = 0.0;
+            limit[s->right[f]] = 0.0;
+        }
+    }
+}
+
+/* NSSolverUnstruct::ComputeLimiter, NSSolverUnstruct.cpp:18189-18242. */
+void orc_limiter(orc_solver *s)
+{
+    int nT = s->nTotal;
+    if (s->p.limiter == 1) { /* ILMT_NOLIM */
+        for (int c = 0; c < nT; ++c) s->lim[c] = 1.0;
+        return;
+    }
+    for (int c = 0; c < nT; ++c) s->lim[c] = 0.0;
+    if (s->p.limiter == 2) return; /* first order: the limiter array is unused */
+    for (int c = 0; c < nT; ++c) s->lim[c] = 1.0; /* InitLimitData */
+    vencat7(s, s->lim, s->q[IR], s->dqdx[IR], s->dqdy[IR], s->dqdz[IR]);
+    vencat7(s, s->lim, s->q[IP], s->dqdx[IP], s->dqdy[IP], s->dqdz[IP]);
+    set_limit_boundary(s, s->lim);
+}
+
+/* ------------------------------------------------------------------ */
+/* Roe flux, conservative form                                         */
+/* ------------------------------------------------------------------ */
+
+/* Harten, API/Physics/include/Flux_RoeEntropyCorrection.h:66-74 */
+double orc_harten(double eig, double limit)
+{
+    double e = ABS(eig);
+    if (e < limit) e = 0.5 * (eig * eig + limit * limit) / limit;
+    return e;
+}
+
+/* Flux_RoeEntropyCorrection, API/Physics/src/Flux_RoeEntropyCorrection.cpp:93-166
+ * (methods 1, 3, 5; pressure-sensor methods 2/6 need a sensor field). */
+static void entropy_fix(double *e1, double *e2, double *e3, double vn, double cm,
+                        int method, double acoustic, double convect)
+{
+    if (method == 1) {
+        double mx = MAX(*e2, *e3);
+        double t0 = mx * acoustic, t1 = mx * acoustic;
+        *e1 = MAX(t0, *e1); *e2 = MAX(t1, *e2); *e3 = MAX(t1, *e3);
+    } else if (method == 3) {
+        double me = ABS(vn) + cm;
+        double al = acoustic * me;
+        double cl = convect * me;
+        *e1 = orc_harten(*e1, cl);
+        *e2 = orc_harten(*e2, al);
+        *e3 = orc_harten(*e3, al);
+    } else if (method == 5) {
+        double mx = MAX(*e2, *e3);
+        double d = mx * acoustic;
+        *e1 += d; *e2 += d; *e3 += d;
+    }
+}
+
+/* Roe_Scheme_ConservativeForm, API/Physics/src/Flux_Inviscid.cpp:136-442
+ * (perfect gas, no preconditioning, static mesh), one face, flux per unit area. */
+void orc_roe_flux(const double *qLin, const double *qRin, double xsn, double ysn, double zsn,
+                  double gama_, int fix_method, double acoustic, double convect, double *flux)
+{
+    const double LITTLE = 1.0E-10;
+    double gmL = gama_, gmR = gama_;
+    double rL = qLin[IR], uL = qLin[IU], vL = qLin[IV], wL = qLin[IW], pL = qLin[IP];
+    double rR = qRin[IR], uR = qRin[IU], vR = qRin[IV], wR = qRin[IW], pR = qRin[IP];
+    rL = MAX(rL, LITTLE); rR = MAX(rR, LITTLE);
+    pL = MAX(pL, LITTLE); pR = MAX(pR, LITTLE);
+    double v2L = uL * uL + vL * vL + wL * wL;
+    double v2R = uR * uR + vR * vR + wR * wR;
+    double hintL = (gmL / (gmL - 1.0)) * (pL / rL);
+    double hintR = (gmR / (gmR - 1.0)) * (pR / rR);
+    double hL = hintL + 0.5 * v2L;
+    double hR = hintR + 0.5 * v2R;
+    double vb = 0.0;
+    double vnL = xsn * uL + ysn * vL + zsn * wL - vb;
+    double vnR = xsn * uR + ysn * vR + zsn * wR - vb;
+    double rvnL = rL * vnL, rvnR = rR * vnR;
+    flux[0] = 0.5 * (rvnL + rvnR);
+    flux[1] = 0.5 * (rvnL * uL + xsn * pL + rvnR * uR + xsn * pR);
+    flux[2] = 0.5 * (rvnL * vL + ysn * pL + rvnR * vR + ysn * pR);
+    flux[3] = 0.5 * (rvnL * wL + zsn * pL + rvnR * wR + zsn * pR);
+    flux[4] = 0.5 * (rvnL * hL + vb * pL + rvnR * hR + vb * pR);
+
+    double ratio = sqrt(rR / rL);
+    double coef = 1.0 / (1.0 + ratio);
+    double rRoe = sqrt(rL * rR);
+    double uRoe = (uL + uR * ratio) * coef;
+    double vRoe = (vL + vR * ratio) * coef;
+    double wRoe = (wL + wR * ratio) * coef;
+    double pRoe = (pL + pR * ratio) * coef;
+    double gama = (gmL + gmR * ratio) * coef;
+    double gamm1 = gama - 1.0;
+    double v2Roe = uRoe * uRoe + vRoe * vRoe + wRoe * wRoe;
+    double absVel = sqrt(v2Roe);
+    (void)absVel;
+    double hRoe = gama / gamm1 * pRoe / rRoe + 0.5 * v2Roe;
+    double vn = xsn * uRoe + ysn * vRoe + zsn * wRoe - vb;
+    double theta = vn + vb;
+    double c2 = gamm1 * (hRoe - 0.5 * v2Roe);
+    double cm = sqrt(ABS(c2));
+    double e1 = ABS(vn), e2 = ABS(vn + cm), e3 = ABS(vn - cm);
+    double EL = (1.0 / (gmL - 1.0)) * (pL / rL) + 0.5 * v2L;
+    double ER = (1.0 / (gmR - 1.0)) * (pR / rR) + 0.5 * v2R;
+    double dq0 = rR - rL;
+    double dq1 = rR * uR - rL * uL;
+    double dq2 = rR * vR - rL * vL;
+    double dq3 = rR * wR - rL * wL;
+    double dq4 = rR * ER - rL * EL;
+    entropy_fix(&e1, &e2, &e3, vn, cm, fix_method, acoustic, convect);
+    double xi1 = (2.0 * e1 - e2 - e3) / (2.0 * c2);
+    double xi2 = (e2 - e3) / (2.0 * cm);
+    double dc = theta * dq0 - xsn * dq1 - ysn * dq2 - zsn * dq3;
+    double c2dc = c2 * dc;
+    double dh = -gamm1 * (uRoe * dq1 + vRoe * dq2 + wRoe * dq3 - dq4) + 0.5 * gamm1 * v2Roe * dq0;
+    flux[0] -= 0.5 * (e1 * dq0 - dh * xi1 - dc * xi2);
+    flux[1] -= 0.5 * (e1 * dq1 + (xsn * c2dc - uRoe * dh) * xi1 + (xsn * dh - uRoe * dc) * xi2);
+    flux[2] -= 0.5 * (e1 * dq2 + (ysn * c2dc - vRoe * dh) * xi1 + (ysn * dh - vRoe * dc) * xi2);
+    flux[3] -= 0.5 * (e1 * dq3 + (zsn * c2dc - wRoe * dh) * xi1 + (zsn * dh - wRoe * dc) * xi2);
+    flux[4] -= 0.5 * (e1 * dq4 + (theta * c2dc - hRoe * dh) * xi1 + (theta * dh - hRoe * dc) * xi2);
+}
+
+/* ------------------------------------------------------------------ */
+/* Inviscid flux: NSSolverUnstruct::InviscidFlux (:6912-6971)          */
+/* GetQlQr (:5507), ReConstructFaceValue reconmeth=1 (:14124-14189),   */
+/* BoundaryQlQrFix (:13725-13940), InviscidFluxWrap area scaling        */
+/* (:14476-14487), LoadFlux (:6435-6494).                               */
+/* ------------------------------------------------------------------ */
+
+static int positive_check(const double *q) /* FaceProxy.h:209-213 */
+{
+    return !(q[IR] <= 0.0 || q[IP] <= 0.0);
+}
+
+static void face_values(const orc_solver *s, int f, double *qL, double *qR)
+{
+    int le = s->left[f], re = s->right[f];
+    for (int m = 0; m < 5; ++m) { qL[m] = s->q[m][le]; qR[m] = s->q[m][re]; }
+    if (s->p.limiter != 2) {
+        double qt[5];
+        double lmin = MIN(s->lim[le], s->lim[re]);
+        double dx = s->xfc[f] - s->xcc[le], dy = s->yfc[f] - s->ycc[le], dz = s->zfc[f] - s->zcc[le];
+        for (int m = 0; m < 5; ++m) qt[m] = qL[m];
+        for (int m = 0; m < 5; ++m)
+            qt[m] += lmin * (s->dqdx[m][le] * dx + s->dqdy[m][le] * dy + s->dqdz[m][le] * dz);
+        if (positive_check(qt)) for (int m = 0; m < 5; ++m) qL[m] = qt[m];
+        dx = s->xfc[f] - s->xcc[re]; dy = s->yfc[f] - s->ycc[re]; dz = s->zfc[f] - s->zcc[re];
+        for (int m = 0; m < 5; ++m) qt[m] = qR[m];
+        for (int m = 0; m < 5; ++m)
+            qt[m] += lmin * (s->dqdx[m][re] * dx + s->dqdy[m][re] * dy + s->dqdz[m][re] * dz);
+        if (positive_check(qt)) for (int m = 0; m < 5; ++m) qR[m] = qt[m];
+    }
+    if (f < s->nBFace) {
+        int bc = s->bcType[f];
+        if (bc != ORC_BC_SOLID_SURFACE && bc != ORC_BC_SYMMETRY) return;
+        for (int m = 0; m < 5; ++m) {
+            double tmp = 0.5 * (s->q[m][le] + s->q[m][re]);
+            qL[m] = tmp; qR[m] = tmp;
+        }
+        if (bc == ORC_BC_SYMMETRY) {
+            double nx = s->xfn[f], ny = s->yfn[f], nz = s->zfn[f];
+            double vn = 2.0 * (nx * qL[IU] + ny * qL[IV] + nz * qL[IW] - 0.0);
+            qR[IR] = qL[IR];
+            qR[IU] = qL[IU] - vn * nx;
+            qR[IV] = qL[IV] - vn * ny;
+            qR[IW] = qL[IW] - vn * nz;
+            qR[IP] = qL[IP];
+        }
+        if (bc == ORC_BC_SOLID_SURFACE && s->p.viscous) {
+            qL[IU] = 0.0; qL[IV] = 0.0; qL[IW] = 0.0;
+            qR[IU] = 0.0; qR[IV] = 0.0; qR[IW] = 0.0;
+        }
+    }
+}
+
+void orc_zero_residual(orc_solver *s)
+{
+    /* ZeroResiduals (:5642) then LoadResiduals res = -rhs with rhs = 0 (:4116). */
+    for (int m = 0; m < 5; ++m)
+        for (int c = 0; c < s->nTotal; ++c) s->res[m][c] = -0.0;
+}
+
+void orc_inviscid_flux(orc_solver *s)
+{
+    double qL[5], qR[5], fl[5];
+    for (int f = 0; f < s->nFace; ++f) {
+        face_values(s, f, qL, qR);
+        orc_roe_flux(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma,
+                     s->p.entropy_fix, s->entropyCoef, s->entropyCoef, fl);
+        double a = s->area[f];
+        for (int m = 0; m < 5; ++m) fl[m] *= a;
+        int le = s->left[f], re = s->right[f];
+        for (int m = 0; m < 5; ++m) s->res[m][le] -= fl[m];
+        if (f >= s->nBFace)
+            for (int m = 0; m < 5; ++m) s->res[m][re] += fl[m];
+    }
+}
+
+/* ------------------------------------------------------------------ */
+/* Viscous flux: NSSolverUnstruct::ViscousFlux (:7058-7114), FaceWeight */
+/* (Geo_UnstructGrid.cpp:2420-2484), GetVisFaceValue (:7126-7331),      */
+/* ComputeVisflux (:12977-13396).  Laminar, perfect gas, static mesh.   */
+/* ------------------------------------------------------------------ */
+void orc_viscous_flux(orc_solver *s)
+{
+    if (!s->p.viscous) return;
+    const double two3rd = 2.0 / 3.0;
+    const double skewAngle = 60.0;
+    double oRe = 1.0 / s->p.reynolds;
+    double oprl = 1.0 / s->p.prandtl, oprt = 1.0 / s->p.prandtl_turb;
+    double M2 = s->p.mach * s->p.mach;
+    double cp = 1.0 / ((s->p.gamma - 1.0) * M2);
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        int bnd = f < s->nBFace;
+        int wall = bnd && is_wall(s->bcType[f]);
+        /* FaceWeight */
+        double dxL0 = s->xfc[f] - s->xcc[le], dyL0 = s->yfc[f] - s->ycc[le], dzL0 = s->zfc[f] - s->zcc[le];
+        double dxR0 = s->xfc[f] - s->xcc[re], dyR0 = s->yfc[f] - s->ycc[re], dzR0 = s->zfc[f] - s->zcc[re];
+        double d1 = sqrt(dxL0 * dxL0 + dyL0 * dyL0 + dzL0 * dzL0);
+        double d2 = sqrt(dxR0 * dxR0 + dyR0 * dyR0 + dzR0 * dzR0);
+        double dd = 1.0 / (d1 + d2 + SMALL);
+        double wL = d2 * dd, wR = d1 * dd;
+        if (bnd) { wL = 1.0; wR = 0.0; }
+        /* GetVisFaceValue */
+        double mul = 0.5 * (s->visl[le] + s->visl[re]);
+        double mut = 0.5 * (s->vist[le] + s->vist[re]);
+        double primF[5];
+        for (int m = 0; m < 5; ++m) primF[m] = 0.5 * (s->q[m][le] + s->q[m][re]);
+        double tmF = 0.5 * (s->t[le] + s->t[re]);
+        double kcp = (mul * oprl + mut * oprt) * cp;
+        if (wall) { primF[IU] = 0.0; primF[IV] = 0.0; primF[IW] = 0.0; }
+        /* ComputeVisflux */
+        double nxs = s->xfn[f], nys = s->yfn[f], nzs = s->zfn[f];
+        double t1x, t1y, t1z;
+        if (ABS(nxs) > SMALL) { t1x = nys; t1y = -nxs; t1z = 0.0; }
+        else if (ABS(nys) > SMALL) { t1x = -nys; t1y = nxs; t1z = 0.0; }
+        else if (ABS(nzs) > SMALL) { t1x = 0.0; t1y = -nzs; t1z = nys; }
+        else continue; /* degenerate face: zero flux */
+        double on = 1.0 / sqrt(t1x * t1x + t1y * t1y + t1z * t1z);
+        t1x *= on; t1y *= on; t1z *= on;
+        double t2x = nys * t1z - nzs * t1y;
+        double t2y = nzs * t1x - nxs * t1z;
+        double t2z = nxs * t1y - nys * t1x;
+        double dxL = s->xcc[le] - s->xfc[f], dyL = s->ycc[le] - s->yfc[f], dzL = s->zcc[le] - s->zfc[f];
+        double dxR = s->xcc[re] - s->xfc[f], dyR = s->ycc[re] - s->yfc[f], dzR = s->zcc[re] - s->zfc[f];
+        double dL = nxs * dxL + nys * dyL + nzs * dzL;
+        double dR = nxs * dxR + nys * dyR + nzs * dzR;
+        double dt_ = -dL / (sqrt(dxL * dxL + dyL * dyL + dzL * dzL) + SMALL);
+        dt_ = MIN(dt_, 1.0); dt_ = MAX(dt_, -1.0);
+        double angle1 = asin(dt_) * 180.0 / PI_;
+        dt_ = dR / (sqrt(dxR * dxR + dyR * dyR + dzR * dzR) + SMALL);
+        dt_ = MIN(dt_, 1.0); dt_ = MAX(dt_, -1.0);
+        double angle2 = asin(dt_) * 180.0 / PI_;
+        double dxnL = nxs * dL - dxL, dynL = nys * dL - dyL, dznL = nzs * dL - dzL;
+        double dxnR = nxs * dR - dxR, dynR = nys * dR - dyR, dznR = nzs * dR - dzR;
+        double f1[5], f2[5], fM[5];
+        for (int m = 0; m < 5; ++m) { f1[m] = s->q[m][le]; f2[m] = s->q[m][re]; }
+        double tL = s->t[le], tR = s->t[re];
+        for (int m = 0; m < 5; ++m) fM[m] = 0.5 * (f1[m] + f2[m]);
+        double tM = 0.5 * (tL + tR);
+        if (wall) {
+            tM = tmF;
+            fM[IR] = fM[IP] / (s->R * tM);
+        }
+        if (angle1 > skewAngle && angle2 > skewAngle) {
+            for (int m = 0; m < 5; ++m) {
+                f1[m] += s->dqdx[m][le] * dxnL + s->dqdy[m][le] * dynL + s->dqdz[m][le] * dznL;
+                f2[m] += s->dqdx[m][re] * dxnR + s->dqdy[m][re] * dynR + s->dqdz[m][re] * dznR;
+            }
+            tL += s->dtdx[le] * dxnL + s->dtdy[le] * dynL + s->dtdz[le] * dznL;
+            tR += s->dtdx[re] * dxnR + s->dtdy[re] * dynR + s->dtdz[re] * dznR;
+            if (tL < SMALL) tL = s->t[le];
+            if (tR < SMALL) tR = s->t[re];
+            for (int m = 0; m < 5; ++m) fM[m] = primF[m];
+            tM = tmF;
+        }
+        double dfdn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double dtdn = 0.0;
+        if (angle1 > 0.0 && angle2 > 0.0 && ABS(dL) > TINY && ABS(dR) > TINY) {
+            double dfd1[5], dfd2[5];
+            for (int m = 0; m < 5; ++m) dfd1[m] = (f1[m] - fM[m]) / dL;
+            for (int m = 0; m < 5; ++m) dfd2[m] = (f2[m] - fM[m]) / dR;
+            double dtd1 = (tL - tM) / dL;
+            double dtd2 = (tR - tM) / dR;
+            double dtmp = dL * dL + dR * dR;
+            double wl = dL * dL / dtmp;
+            double wr = 1.0 - wl;
+            if (bnd) { wl = 1.0; wr = 0.0; }
+            for (int m = 0; m < 5; ++m) dfdn[m] = wl * dfd1[m] + wr * dfd2[m];
+            dtdn = dtd1 * wl + dtd2 * wr;
+        }
+        double dfdx[5], dfdy[5], dfdz[5], dft1[5], dft2[5];
+        for (int m = 0; m < 5; ++m) {
+            dfdx[m] = wL * s->dqdx[m][le] + wR * s->dqdx[m][re];
+            dfdy[m] = wL * s->dqdy[m][le] + wR * s->dqdy[m][re];
+            dfdz[m] = wL * s->dqdz[m][le] + wR * s->dqdz[m][re];
+        }
+        for (int m = IU; m <= IW; ++m) {
+            dft1[m] = t1x * dfdx[m] + t1y * dfdy[m] + t1z * dfdz[m];
+            dft2[m] = t2x * dfdx[m] + t2y * dfdy[m] + t2z * dfdz[m];
+        }
+        for (int m = IU; m <= IW; ++m) {
+            dfdx[m] = nxs * dfdn[m] + t1x * dft1[m] + t2x * dft2[m];
+            dfdy[m] = nys * dfdn[m] + t1y * dft1[m] + t2y * dft2[m];
+            dfdz[m] = nzs * dfdn[m] + t1z * dft1[m] + t2z * dft2[m];
+        }
+        double dudx = dfdx[IU], dudy = dfdy[IU], dudz = dfdz[IU];
+        double dvdx = dfdx[IV], dvdy = dfdy[IV], dvdz = dfdz[IV];
+        double dwdx = dfdx[IW], dwdy = dfdy[IW], dwdz = dfdz[IW];
+        double qNorm = 0.0;
+        qNorm += kcp * dtdn;
+        double divv2p3 = two3rd * (dudx + dvdy + dwdz);
+        double vis = mul + mut;
+        double um = fM[1], vm = fM[2], wm = fM[3];
+        double txx = vis * (2.0 * dudx - divv2p3);
+        double tyy = vis * (2.0 * dvdy - divv2p3);
+        double tzz = vis * (2.0 * dwdz - divv2p3);
+        double txy = vis * (dudy + dvdx);
+        double txz = vis * (dudz + dwdx);
+        double tyz = vis * (dvdz + dwdy);
+        double fv[5];
+        fv[0] = 0.0;
+        fv[1] = nxs * txx + nys * txy + nzs * txz;
+        fv[2] = nxs * txy + nys * tyy + nzs * tyz;
+        fv[3] = nxs * txz + nys * tyz + nzs * tzz;
+        fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + qNorm;
+        double fl[5];
+        for (int m = 0; m < 5; ++m) fl[m] = -oRe * s->area[f] * fv[m];
+        for (int m = 0; m < 5; ++m) s->res[m][le] -= fl[m];
+        if (!bnd)
+            for (int m = 0; m < 5; ++m) s->res[m][re] += fl[m];
+    }
+}
+
+/* Controller::UpdateResiduals (Controller.cpp:1002-1061) + LoadResiduals:
+ * boundary, independent fields, then the inviscid and viscous face loops. */
+void orc_update_residuals(orc_solver *s)
+{
+    orc_boundary(s);
+    orc_gamma_t(s);
+    if (s->p.viscous) orc_viscosity(s);
+    orc_gradient(s);
+    orc_limiter(s);
+    orc_zero_residual(s);
+    orc_inviscid_flux(s);
+    orc_viscous_flux(s);
+}
+
+/* ------------------------------------------------------------------ */
+/* Spectral radius and time step                                       */
+/* ------------------------------------------------------------------ */
+
+/* SpectrumRadiusInviscid (:5749-5850), SpectrumRadiusViscous method 2 (:5926-5963). */
+void orc_spectrum(orc_solver *s)
+{
+    int nC = s->nCell;
+    for (int c = 0; c < nC; ++c) s->specInv[c] = 0.0;
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double nx = s->xfn[f], ny = s->yfn[f], nz = s->zfn[f];
+        double rl = s->q[IR][le], ul = s->q[IU][le], vl = s->q[IV][le], wl = s->q[IW][le], pl = s->q[IP][le];
+        double vnl = nx * ul + ny * vl + nz * wl - 0.0;
+        double rr = s->q[IR][re], ur = s->q[IU][re], vr = s->q[IV][re], wr = s->q[IW][re], pr = s->q[IP][re];
+        double vnr = nx * ur + ny * vr + nz * wr - 0.0;
+        double g = 0.5 * (s->gama[le] + s->gama[re]);
+        double pm = 0.5 * (pl + pr);
+        double rm = 0.5 * (rl + rr);
+        double vn = 0.5 * (vnl + vnr);
+        double cm = sqrt(g * pm / rm);
+        double sr = 0.5 * s->area[f] * (ABS(vn) + cm);
+        s->specInv[le] += sr;
+        if (re < nC) s->specInv[re] += sr;
+    }
+    for (int c = 0; c < nC; ++c) s->specVis[c] = 0.0;
+    if (!s->p.viscous) return;
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double dx = s->xcc[re] - s->xcc[le], dy = s->ycc[re] - s->ycc[le], dz = s->zcc[re] - s->zcc[le];
+        double ds = ABS(s->xfn[f] * dx + s->yfn[f] * dy + s->zfn[f] * dz);
+        double vl = 0.5 * (s->visl[le] + s->visl[re]);
+        double vt = 0.5 * (s->vist[le] + s->vist[re]);
+        double rho = 0.5 * (s->q[IR][le] + s->q[IR][re]);
+        double vis = vl + vt;
+        double sr = 2.0 * vis / (rho * ds * s->p.reynolds + SMALL);
+        sr *= 0.5 * s->area[f];
+        s->specVis[le] += sr;
+        if (re < nC) s->specVis[re] += sr;
+    }
+}
+
+/* NSSolver::ComputeCFL (CFD/NSSolver/src/NSSolver.cpp:595-668), CFLMethod 0. */
+static double compute_cfl(const orc_solver *s, int step)
+{
+    if (step >= s->p.cfl_vary_step) return s->p.cfl_end;
+    return s->p.cfl_start + (s->p.cfl_end - s->p.cfl_start) * step / s->p.cfl_vary_step;
+}
+
+/* LocalTimeStep (:5999-6050), LimitCFL (:6052-6118), ComputeMinTimeStep
+ * (:5697-5714), ReduceMaxTimeStep (:5716-5743). dt is stored divided by volume. */
+void orc_time_step(orc_solver *s, int step)
+{
+    int nC = s->nCell;
+    double cfl = compute_cfl(s, step);
+    double cflStart = s->p.cfl_start;
+    double pLower = 0.001 * s->pTotal, pUpper = 0.02 * s->pTotal;
+    for (int c = 0; c < nC; ++c) {
+        double spec = s->specInv[c];
+        if (s->p.viscous) spec += s->specVis[c];
+        double cc = cfl;
+        double p = s->q[IP][c];
+        if (p < pLower) cc = cflStart;
+        else if (p < pUpper) cc = (p - pUpper) * (cfl - cflStart) / (pUpper - pLower) + cfl;
+        double sk = s->skew[c];
+        if (sk < 0.1) cc *= 0.1;
+        else if (sk < 10.0) cc *= (1.0 - exp(0.0 - sk));
+        s->dt[c] = cc * s->vol[c] / spec;
+    }
+    double mn = LARGE;
+    for (int c = 0; c < nC; ++c) mn = MIN(mn, s->dt[c]);
+    if (s->p.ktmax > 0)
+        for (int c = 0; c < nC; ++c) s->dt[c] = MIN(s->dt[c], s->p.ktmax * mn);
+    for (int c = 0; c < nC; ++c) s->dt[c] /= s->vol[c];
+}
+
+/* ------------------------------------------------------------------ */
+/* LU-SGS                                                              */
+/* ------------------------------------------------------------------ */
+
+/* MXDQ_STD, API/Physics/src/Jacobian.cpp:13-86 with
+ * PerfectGas::ComputeTotalEnthalpyAndDH (PerfectGas.cpp:368-400). */
+void orc_mxdq(const double *prim, double nxs, double nys, double nzs, double area,
+              double gama, const double *dq, double radius, int ipn, double *f)
+{
+    double vgn = 0.0;
+    double rm = prim[IR], um = prim[IU], vm = prim[IV], wm = prim[IW], pm = prim[IP];
+    double c2 = ABS(gama * pm / rm);
+    double cm = sqrt(c2);
+    double cits = nxs * um + nys * vm + nzs * wm - vgn;
+    double lmd1 = cits, lmd2 = cits + cm, lmd3 = cits - cm;
+    double ipnrad = ipn * radius / MAX(SMALL, area);
+    lmd1 = 0.5 * (lmd1 + ipnrad);
+    lmd2 = 0.5 * (lmd2 + ipnrad);
+    lmd3 = 0.5 * (lmd3 + ipnrad);
+    double x1 = (lmd1 + lmd1 - lmd2 - lmd3) / (c2 + c2);
+    double x2 = (lmd2 - lmd3) / (cm + cm);
+    double dc = (cits + vgn) * dq[0] - nxs * dq[1] - nys * dq[2] - nzs * dq[3];
+    double c2dc = c2 * dc;
+    double v2 = um * um + vm * vm + wm * wm;
+    double gama1 = gama - 1.0;
+    double cc1 = 0.5 * gama1 * v2;
+    double dh = -gama1 * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
+    double enthalpy = (gama / gama1) * (pm / rm);
+    dh += cc1 * dq[0];
+    double hm = enthalpy + 0.5 * v2;
+    double a = dh * x1 + dc * x2;
+    double b = c2dc * x1 + dh * x2;
+    f[0] = lmd1 * dq[0] - a;
+    f[1] = lmd1 * dq[1] - um * a + nxs * b;
+    f[2] = lmd1 * dq[2] - vm * a + nys * b;
+    f[3] = lmd1 * dq[3] - wm * a + nzs * b;
+    f[4] = lmd1 * dq[4] - hm * a + (cits + vgn) * b;
+    for (int m = 0; m < 5; ++m) f[m] *= area;
+}
+
+/* RoeAveragedVariables (Flux_Inviscid.cpp:15440-15462) + GetRad (:15097-15116):
+ * (|V.n| + c) * S at the Roe-averaged face state. */
+static double lusgs_rad(const double *ql, double gl, const double *qr, double gr,
+                        double nx, double ny, double nz, double area, double gama)
+{
+    double hl = gl / (gl - 1.0) * ql[IP] / ql[IR] + 0.5 * (ql[IU] * ql[IU] + ql[IV] * ql[IV] + ql[IW] * ql[IW]);
+    double hr = gr / (gr - 1.0) * qr[IP] / qr[IR] + 0.5 * (qr[IU] * qr[IU] + qr[IV] * qr[IV] + qr[IW] * qr[IW]);
+    double t0 = sqrt(qr[IR] / ql[IR]);
+    double t1 = 1.0 / (1.0 + t0);
+    double rm = sqrt(qr[IR] * ql[IR]);
+    double um = (ql[IU] + qr[IU] * t0) * t1;
+    double vm = (ql[IV] + qr[IV] * t0) * t1;
+    double wm = (ql[IW] + qr[IW] * t0) * t1;
+    double hm = (hl + hr * t0) * t1;
+    double pm = rm * (hm - 0.5 * (um * um + vm * vm + wm * wm)) * (gl - 1.0) / gl;
+    double vn = nx * um + ny * vm + nz * wm - 0.0;
+    double eig = ABS(vn) + sqrt(gama * pm / rm);
+    return eig * area;
+}
+
+/* One neighbour contribution of SolveLUSGSForward/Backward
+ * (NSSolverUnstruct.cpp:15227-15305, 15582-15660). */
+static void lusgs_face(const orc_solver *s, int f, int cell, double *df)
+{
+    int le = s->left[f], re = s->right[f];
+    double nx = s->xfn[f], ny = s->yfn[f], nz = s->zfn[f], ar = s->area[f];
+    if (re == cell) { int t = le; le = re; re = t; nx = -nx; ny = -ny; nz = -nz; }
+    double ql[5], qr[5], dqn[5];
+    for (int m = 0; m < 5; ++m) { ql[m] = s->q[m][le]; qr[m] = s->q[m][re]; dqn[m] = s->dq[m][re]; }
+    double gama = s->gama[re];
+    double rad = lusgs_rad(ql, s->gama[le], qr, s->gama[re], nx, ny, nz, ar, gama);
+    orc_mxdq(qr, nx, ny, nz, ar, gama, dqn, rad, -1, df);
+    if (s->p.viscous) {
+        double dist = ABS(s->xfn[f] * (s->xcc[re] - s->xcc[le]) + s->yfn[f] * (s->ycc[re] - s->ycc[le]) +
+                          s->zfn[f] * (s->zcc[re] - s->zcc[le]));
+        double vl = 0.5 * (s->visl[le] + s->visl[re]);
+        double vt = 0.5 * (s->vist[le] + s->vist[re]);
+        double rho = 0.5 * (s->q[IR][le] + s->q[IR][re]);
+        double vis = vl + vt;
+        double sr = 2.0 * vis / (rho * dist * s->p.reynolds + SMALL);
+        sr *= 0.5 * ar;
+        for (int m = 0; m < 5; ++m) df[m] -= sr * dqn[m];
+    }
+}
+
+/* SolveLUSGSForward, NSSolverUnstruct.cpp:15118-15444. */
+static double lusgs_forward(orc_solver *s)
+{
+    double sweep = 0.0;
+    for (int c = 0; c < s->nCell; ++c) {
+        double rhs0[5], dqOld[5], Ux[5], df[5];
+        for (int m = 0; m < 5; ++m) {
+            rhs0[m] = 0.0;
+            dqOld[m] = s->dq[m][c];
+            s->dq[m][c] = s->res[m][c];
+            Ux[m] = s->dflux[m][c];
+            s->dflux[m][c] = 0.0;
+        }
+        for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
+            int f = s->c2f[k];
+            int le = s->left[f], re = s->right[f];
+            if (le > c || re > c) continue;
+            lusgs_face(s, f, c, df);
+            for (int m = 0; m < 5; ++m) rhs0[m] += df[m];
+        }
+        double dqTry[5];
+        for (int m = 0; m < 5; ++m) {
+            dqTry[m] = (s->dq[m][c] - Ux[m] - rhs0[m]) / s->diag[c];
+            s->dflux[m][c] += rhs0[m];
+        }
+        for (int m = 0; m < 5; ++m) {
+            s->dq[m][c] = dqTry[m];
+            double d = s->dq[m][c] - dqOld[m];
+            sweep += d * d;
+        }
+    }
+    return sweep;
+}
+
+/* SolveLUSGSBackward, NSSolverUnstruct.cpp:15446-15764. */
+static double lusgs_backward(orc_solver *s)
+{
+    double sweep = 0.0;
+    for (int c = s->nCell - 1; c >= 0; --c) {
+        double rhs0[5], dqOld[5], Lx[5], df[5];
+        for (int m = 0; m < 5; ++m) {
+            rhs0[m] = 0.0;
+            dqOld[m] = s->dq[m][c];
+            s->dq[m][c] = s->res[m][c];
+            Lx[m] = s->dflux[m][c];
+            s->dflux[m][c] = 0.0;
+        }
+        for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
+            int f = s->c2f[k];
+            int le = s->left[f], re = s->right[f];
+            if (le < c || re < c) continue;
+            lusgs_face(s, f, c, df);
+            for (int m = 0; m < 5; ++m) rhs0[m] += df[m];
+        }
+        double dqTry[5];
+        for (int m = 0; m < 5; ++m) {
+            dqTry[m] = (s->dq[m][c] - Lx[m] - rhs0[m]) / s->diag[c];
+            s->dflux[m][c] += rhs0[m];
+        }
+        for (int m = 0; m < 5; ++m) {
+            s->dq[m][c] = dqTry[m];
+            double d = s->dq[m][c] - dqOld[m];
+            sweep += d * d;
+        }
+    }
+    return sweep;
+}
+
+/* PerfectGas::Primitive2Conservative (PerfectGas.cpp:284-317, non-porous). */
+static void prim2cons(const double *prim, double gama, double *Q)
+{
+    double um = prim[IU], vm = prim[IV], wm = prim[IW], rho = prim[IR], p = prim[IP];
+    double v2 = um * um + vm * vm + wm * wm;
+    double em = (p / rho) / (gama - 1.0) + 0.5 * v2;
+    Q[0] = prim[IR];
+    Q[1] = prim[IR] * prim[IU];
+    Q[2] = prim[IR] * prim[IV];
+    Q[3] = prim[IR] * prim[IW];
+    Q[4] = prim[IR] * em;
+}
+
+/* SetGhostDQLUSGS (:14767-14824) with SetSymmBCGhostDQLUSGS (:14826-14919),
+ * SetWallBCGhostDQLUSGS (:14921-15000), SetExtraBCGhostDQLUSGS (:15002-15090). */
+static void set_ghost_dq(orc_solver *s)
+{
+    for (int f = 0; f < s->nBFace; ++f) {
+        int bc = s->bcType[f];
+        int le = s->left[f], re = s->right[f];
+        if (bc == ORC_BC_SYMMETRY || (is_wall(bc) && !s->p.viscous)) {
+            double prim[5], Q[5];
+            for (int m = 0; m < 5; ++m) prim[m] = s->q[m][le];
+            prim2cons(prim, s->gama[le], Q);
+            for (int m = 0; m < 5; ++m) Q[m] += s->dq[m][le];
+            double rm = s->q[IR][re], um = s->q[IU][re], vm = s->q[IV][re], wm = s->q[IW][re], pm = s->q[IP][re];
+            for (int m = 0; m < 5; ++m) s->dq[m][re] = s->dq[m][le];
+            double nx = s->xfn[f], ny = s->yfn[f], nz = s->zfn[f];
+            s->dq[IR][re] = s->dq[IR][le];
+            double rvn = 2.0 * (nx * s->dq[1][le] + ny * s->dq[2][le] + nz * s->dq[3][le]);
+            s->dq[1][re] = s->dq[1][le] - nx * rvn;
+            s->dq[2][re] = s->dq[2][le] - ny * rvn;
+            s->dq[3][re] = s->dq[3][le] - nz * rvn;
+            rvn = 2.0 * (nx * Q[1] + ny * Q[2] + nz * Q[3]);
+            double ru = Q[1] - nx * rvn, rv = Q[2] - ny * rvn, rw = Q[3] - nz * rvn;
+            double pnew = Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) / Q[0];
+            double gam1 = s->gama[le] - 1.0;
+            s->dq[4][re] = pnew - pm / gam1 + 0.5 * ((ru * ru + rv * rv + rw * rw) / Q[0] - rm * (um * um + vm * vm + wm * wm));
+        } else if (is_wall(bc) && s->p.viscous) {
+            for (int m = 0; m < 5; ++m) s->dq[m][re] = s->dq[m][le];
+            double ub = 0.0, vb = 0.0, wb = 0.0;
+            s->dq[0][re] = s->dq[0][le];
+            s->dq[1][re] = -s->dq[1][le] + 2.0 * ub * s->dq[0][le];
+            s->dq[2][re] = -s->dq[2][le] + 2.0 * vb * s->dq[0][le];
+            s->dq[3][re] = -s->dq[3][le] + 2.0 * wb * s->dq[0][le];
+            s->dq[4][re] = s->dq[4][le] + ub * (s->dq[1][re] - s->dq[1][le]) + vb * (s->dq[2][re] - s->dq[2][le]) +
+                           wb * (s->dq[3][re] - s->dq[3][le]);
+        } else if (bc == ORC_BC_EXTRAPOLATION || bc == ORC_BC_OUTFLOW) {
+            for (int m = 0; m < 5; ++m) s->dq[m][re] = s->dq[m][le];
+        } else {
+            for (int m = 0; m < 5; ++m) s->dq[m][re] = 0.0;
+        }
+    }
+}
+
+/* Diagonal (:15867-15960), steady. */
+static void diagonal(orc_solver *s)
+{
+    for (int c = 0; c < s->nCell; ++c) {
+        s->diag[c] = 1.0 / s->dt[c];
+        s->diag[c] += s->specInv[c];
+        if (s->p.viscous) s->diag[c] += s->specVis[c];
+    }
+}
+
+/* SolutionFix (:14509-14585): average of admissible neighbours, read from
+ * the partially updated field exactly as the in-place reference loop does. */
+static void solution_fix(const orc_solver *s, double *prim, int c)
+{
+    double roo = s->primInf[IR], poo = s->primInf[IP];
+    double dl = 1.0e-6 * roo, pl = 1.0e-6 * poo;
+    for (int m = 0; m < 5; ++m) prim[m] = 0.0;
+    double vs = 0.0;
+    for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
+        int f = s->c2f[k];
+        int le = s->left[f], re = s->right[f];
+        int nb = le;
+        if (c == le) nb = re;
+        if (s->q[IR][nb] < dl || s->q[IP][nb] < pl) continue;
+        double vsur = 1.0;
+        vs += vsur;
+        for (int m = 0; m < 5; ++m) {
+            double v = s->q[m][nb];
+            if (m == IR || m == IP) v = ABS(v);
+            prim[m] += v * vsur;
+        }
+    }
+    if (vs > 0.0) {
+        double o = 1.0 / vs;
+        for (int m = 0; m < 5; ++m) prim[m] *= o;
+    } else {
+        for (int m = 0; m < 5; ++m) prim[m] = s->primInf[m];
+    }
+}
+
+/* UpdateFlowField (:14587-14765), perfect gas, in place. */
+static int update_flow(orc_solver *s)
+{
+    double dl = 1.0e-6 * s->primInf[IR], pl = 1.0e-6 * s->primInf[IP];
+    int nNeg = 0;
+    for (int c = 0; c < s->nCell; ++c) {
+        double prim[5], Q[5];
+        for (int m = 0; m < 5; ++m) prim[m] = s->q[m][c];
+        double g = s->gama[c];
+        prim2cons(prim, g, Q);
+        for (int m = 0; m < 5; ++m) Q[m] += s->dq[m][c];
+        /* Conservative2Primitive, PerfectGas.cpp:327-352 */
+        if (Q[0] <= 0.0) {
+            prim[IR] = Q[0];
+        } else {
+            double od = 1.0 / Q[0];
+            double um = Q[1] * od, vm = Q[2] * od, wm = Q[3] * od;
+            double ie = Q[4] - 0.5 * Q[0] * (um * um + vm * vm + wm * wm);
+            prim[IR] = Q[0]; prim[IU] = um; prim[IV] = vm; prim[IW] = wm;
+            prim[IP] = (g - 1.0) * ie;
+        }
+        if (prim[IR] < dl || prim[IP] < pl) {
+            nNeg++;
+            solution_fix(s, prim, c);
+        }
+        for (int m = 0; m < 5; ++m) s->q[m][c] = prim[m];
+    }
+    return nNeg;
+}
+
+/* Controller::LUSGS, CFD/Solver/src/Controller.cpp:1878-1961, with
+ * LUSGSInitialization (:1732-1782). */
+int orc_lusgs(orc_solver *s, double *sweep_normals)
+{
+    diagonal(s);
+    int nS = s->p.n_lusgs_sweeps;
+    for (int m = 0; m < 5; ++m)
+        for (int c = 0; c < s->nTotal; ++c) {
+            s->dflux[m][c] = 0.0;
+            s->dq[m][c] = (nS == 1) ? s->res[m][c] : 0.0;
+        }
+    double init = 0.0, eps = 1.1;
+    for (int it = 0; it < nS; ++it) {
+        double sn = lusgs_forward(s);
+        set_ghost_dq(s);
+        sn += lusgs_backward(s);
+        if (sweep_normals) sweep_normals[it] = sn;
+        if (it == 0) init = sn;
+        else eps = sn / init;
+        if (eps < s->p.lusgs_tol) break;
+    }
+    return update_flow(s);
+}
+
+/* ComputeResidualonGrid (API/Physics/src/Residual.cpp:534-617), unit grid
+ * scale: per-equation sums of res^2 over real cells. */
+void orc_residual_norm(const orc_solver *s, double *sum5)
+{
+    for (int m = 0; m < 5; ++m) {
+        double acc = 0.0;
+        for (int c = 0; c < s->nCell; ++c) {
+            double r = s->res[m][c] * 1.0;
+            acc += r * r;
+        }
+        sum5[m] = acc;
+    }
+}
+
+/* Controller::Relaxation, LU-SGS branch (Controller.cpp:549-608).  Returns
+ * the number of cells SolutionFix repaired; rms_hist[i] is the density
+ * residual RMS of step first_step+i (ResidualGlobal, Residual.cpp:221-224). */
+int orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist)
+{
+    int nNeg = 0;
+    for (int i = 0; i < n_steps; ++i) {
+        orc_update_residuals(s);
+        orc_spectrum(s);
+        orc_time_step(s, first_step + i);
+        if (rms_hist) {
+            double sums[5];
+            orc_residual_norm(s, sums);
+            rms_hist[i] = sqrt(sums[0] / s->nCell);
+        }
+        nNeg += orc_lusgs(s, NULL);
+    }
+    return nNeg;
+}

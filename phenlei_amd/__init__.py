@@ -1,0 +1,315 @@
+"""phenlei_amd: MI355X (gfx950) implementation of PHengLEI's unstructured
+compressible residual + LU-SGS path, behind the C-ABI declared in include/phl.h.
+
+This module is a thin ctypes binding over libphl.so.  It is the host-side
+mirror used by tests and bench.py; a C++ host solver calls the same C-ABI
+directly.  There is no CPU fallback: if libphl.so is
+missing or does not load, importing the solver raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libphl.so")
+
+# Status codes (include/phl.h)
+PHL_OK = 0
+PHL_ERR_ARG = 1
+PHL_ERR_HIP = 2
+PHL_ERR_NUMERIC = 3
+PHL_ERR_UNSUPPORTED = 4
+PHL_ERR_ALLOC = 5
+
+# Field identifiers (include/phl.h), shared with the oracle.
+FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
+             diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16)
+FIELD_ROWS = dict(q=5, res=5, dqdx=5, dqdy=5, dqdz=5, limit=1, dq=5, spec_inv=1, spec_vis=1, dt=1,
+                  diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5)
+
+BC = dict(extrapolation=1, solid_surface=2, symmetry=3, farfield=4, inflow=5, outflow=6)
+
+
+class Params(ctypes.Structure):
+    """phl_params (include/phl.h); identical layout to the oracle's orc_params."""
+    _fields_ = [
+        ("mach", ctypes.c_double), ("alpha_deg", ctypes.c_double), ("beta_deg", ctypes.c_double),
+        ("gamma", ctypes.c_double), ("reynolds", ctypes.c_double), ("t_ref", ctypes.c_double),
+        ("prandtl", ctypes.c_double), ("prandtl_turb", ctypes.c_double), ("t_wall", ctypes.c_double),
+        ("venkat_k", ctypes.c_double), ("entropy_scale", ctypes.c_double),
+        ("cfl_start", ctypes.c_double), ("cfl_end", ctypes.c_double), ("ktmax", ctypes.c_double),
+        ("lusgs_tol", ctypes.c_double), ("force_ref_length", ctypes.c_double),
+        ("force_ref_area", ctypes.c_double), ("force_ref_x", ctypes.c_double),
+        ("force_ref_y", ctypes.c_double), ("force_ref_z", ctypes.c_double),
+        ("viscous", ctypes.c_int), ("gradient", ctypes.c_int), ("limiter", ctypes.c_int),
+        ("entropy_fix", ctypes.c_int), ("cfl_vary_step", ctypes.c_int),
+        ("n_lusgs_sweeps", ctypes.c_int), ("precision", ctypes.c_int), ("reserved", ctypes.c_int),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+def default_params(**overrides):
+    """Reference defaults of cfd_para.hypara on this path, with overrides."""
+    p = Params()
+    p.mach, p.alpha_deg, p.beta_deg, p.gamma = 0.5, 0.0, 0.0, 1.4
+    p.reynolds, p.t_ref, p.prandtl, p.prandtl_turb, p.t_wall = 6.5e6, 288.15, 0.72, 0.9, -1.0
+    p.venkat_k, p.entropy_scale = 5.0, 1.0
+    p.cfl_start, p.cfl_end, p.ktmax, p.lusgs_tol = 0.01, 10.0, 1.0e10, 0.01
+    p.force_ref_length, p.force_ref_area = 1.0, 1.0
+    p.viscous, p.gradient, p.limiter, p.entropy_fix = 0, 0, 0, 3
+    p.cfl_vary_step, p.n_lusgs_sweeps, p.precision = 500, 1, 0
+    for k, v in overrides.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+class PhlError(RuntimeError):
+    def __init__(self, status, msg):
+        super().__init__(f"phl status {status}: {msg}")
+        self.status = status
+
+
+_lib = None
+
+_P = ctypes.c_void_p
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+
+
+def _declare(lib):
+    sig = {
+        "phl_last_error": (ctypes.c_char_p, []),
+        "phl_params_default": (None, [ctypes.POINTER(Params)]),
+        "phl_mesh_create": (ctypes.c_int, [ctypes.c_int, _D, _D, _D, ctypes.c_int, _I, _I, _I, _I,
+                                           ctypes.c_int, _I, ctypes.c_int, _I, _I, ctypes.POINTER(_P)]),
+        "phl_mesh_bump_hex": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                             ctypes.POINTER(_P)]),
+        "phl_mesh_sizes": (ctypes.c_int, [_P, _I, _I, _I, _I, _I, _I]),
+        "phl_mesh_connectivity": (ctypes.c_int, [_P, _D, _D, _D, _I, _I, _I, _I, _I, _I, _I]),
+        "phl_mesh_metrics": (ctypes.c_int, [_P] + [_D] * 12),
+        "phl_mesh_destroy": (None, [_P]),
+        "phl_device_info": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I,
+                                           ctypes.POINTER(ctypes.c_double)]),
+        "phl_create": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_P)]),
+        "phl_destroy": (None, [_P]),
+        "phl_set_state": (ctypes.c_int, [_P, _D]),
+        "phl_get_state": (ctypes.c_int, [_P, _D]),
+        "phl_get_residual": (ctypes.c_int, [_P, _D]),
+        "phl_get_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+        "phl_init_freestream": (ctypes.c_int, [_P]),
+        "phl_boundary": (ctypes.c_int, [_P]),
+        "phl_gamma_t": (ctypes.c_int, [_P]),
+        "phl_gradient": (ctypes.c_int, [_P]),
+        "phl_limiter": (ctypes.c_int, [_P]),
+        "phl_update_residuals": (ctypes.c_int, [_P]),
+        "phl_spectrum_radius": (ctypes.c_int, [_P]),
+        "phl_time_step": (ctypes.c_int, [_P, ctypes.c_int]),
+        "phl_lusgs": (ctypes.c_int, [_P, _I]),
+        "phl_residual_norm": (ctypes.c_int, [_P, _D]),
+        "phl_iterate": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
+        "phl_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
+        "phl_synchronize": (ctypes.c_int, [_P]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load_library(path=LIB_PATH):
+    """Load libphl.so (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} not found: run __graft_entry__.build() or phenlei_amd/build.py")
+        _lib = _declare(ctypes.CDLL(path))
+    return _lib
+
+
+def exported_symbols():
+    """Names of the C-ABI functions declared in include/phl.h."""
+    import re
+    with open(os.path.join(os.path.dirname(_HERE), "include", "phl.h")) as fh:
+        text = fh.read()
+    return sorted(set(re.findall(r"\b(phl_[a-z_0-9]+)\s*\(", text)))
+
+
+def _check(status):
+    if status != PHL_OK:
+        msg = load_library().phl_last_error().decode(errors="replace")
+        raise PhlError(status, msg)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_D)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_I)
+
+
+class Mesh:
+    """Host-side unstructured grid (phl_mesh)."""
+
+    def __init__(self, handle):
+        self._h = handle
+        lib = load_library()
+        v = [ctypes.c_int() for _ in range(6)]
+        _check(lib.phl_mesh_sizes(self._h, *[ctypes.byref(x) for x in v]))
+        (self.n_node, self.n_cell, self.n_face, self.n_bface, self.n_face_nodes,
+         self.n_cell_nodes) = [x.value for x in v]
+        self.n_total = self.n_cell + self.n_bface
+
+    @classmethod
+    def bump_hex(cls, nx, ny, nz, bump=0.05):
+        h = _P()
+        _check(load_library().phl_mesh_bump_hex(nx, ny, nz, bump, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def from_arrays(cls, x, y, z, face_node_ptr, face_nodes, left, right, n_bface, bc_type,
+                    cell_node_ptr, cell_nodes):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z)]
+        b = [np.ascontiguousarray(v, dtype=np.int32) for v in
+             (face_node_ptr, face_nodes, left, right, bc_type, cell_node_ptr, cell_nodes)]
+        h = _P()
+        _check(load_library().phl_mesh_create(
+            len(a[0]), _dptr(a[0]), _dptr(a[1]), _dptr(a[2]), len(b[2]), _iptr(b[0]), _iptr(b[1]),
+            _iptr(b[2]), _iptr(b[3]), n_bface, _iptr(b[4]), len(b[5]) - 1, _iptr(b[5]), _iptr(b[6]),
+            ctypes.byref(h)))
+        return cls(h)
+
+    def connectivity(self):
+        n = self
+        out = dict(
+            x=np.empty(n.n_node), y=np.empty(n.n_node), z=np.empty(n.n_node),
+            face_node_ptr=np.empty(n.n_face + 1, np.int32), face_nodes=np.empty(n.n_face_nodes, np.int32),
+            left=np.empty(n.n_face, np.int32), right=np.empty(n.n_face, np.int32),
+            bc_type=np.empty(max(n.n_bface, 1), np.int32)[:n.n_bface],
+            cell_node_ptr=np.empty(n.n_cell + 1, np.int32), cell_nodes=np.empty(n.n_cell_nodes, np.int32))
+        bc = np.empty(max(n.n_bface, 1), np.int32)
+        _check(load_library().phl_mesh_connectivity(
+            self._h, _dptr(out["x"]), _dptr(out["y"]), _dptr(out["z"]), _iptr(out["face_node_ptr"]),
+            _iptr(out["face_nodes"]), _iptr(out["left"]), _iptr(out["right"]), _iptr(bc),
+            _iptr(out["cell_node_ptr"]), _iptr(out["cell_nodes"])))
+        out["bc_type"] = bc[:n.n_bface].copy()
+        out["n_bface"] = n.n_bface
+        return out
+
+    def metrics(self):
+        nf, nt = self.n_face, self.n_total
+        names_f = ["xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area"]
+        names_c = ["xcc", "ycc", "zcc", "vol"]
+        out = {k: np.empty(nf) for k in names_f}
+        out.update({k: np.empty(nt) for k in names_c})
+        out["skew"] = np.empty(self.n_cell)
+        _check(load_library().phl_mesh_metrics(self._h, *[_dptr(out[k]) for k in names_f + names_c + ["skew"]]))
+        return out
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phl_mesh_destroy(self._h)
+            self._h = None
+
+
+def device_info(device=0):
+    name = ctypes.create_string_buffer(256)
+    ncu = ctypes.c_int()
+    hbm = ctypes.c_double()
+    _check(load_library().phl_device_info(device, name, 256, ctypes.byref(ncu), ctypes.byref(hbm)))
+    return dict(name=name.value.decode(), n_cu=ncu.value, hbm_bytes=hbm.value)
+
+
+class Solver:
+    """Device context (phl_ctx).  Method names follow the reference hooks."""
+
+    def __init__(self, mesh, params, device=0):
+        self.mesh = mesh
+        self.params = params
+        h = _P()
+        _check(load_library().phl_create(mesh._h, ctypes.byref(params), device, ctypes.byref(h)))
+        self._h = h
+
+    # state ---------------------------------------------------------------
+    def set_state(self, q):
+        q = np.ascontiguousarray(q, dtype=np.float64).reshape(5, self.mesh.n_total)
+        _check(_lib.phl_set_state(self._h, _dptr(q)))
+
+    def get_state(self):
+        out = np.empty((5, self.mesh.n_total))
+        _check(_lib.phl_get_state(self._h, _dptr(out)))
+        return out
+
+    def get_residual(self):
+        out = np.empty((5, self.mesh.n_total))
+        _check(_lib.phl_get_residual(self._h, _dptr(out)))
+        return out
+
+    def get_field(self, name):
+        rows = FIELD_ROWS[name]
+        out = np.empty((rows, self.mesh.n_total))
+        _check(_lib.phl_get_field(self._h, FIELD[name], _dptr(out)))
+        return out if rows > 1 else out[0]
+
+    def init_freestream(self):
+        _check(_lib.phl_init_freestream(self._h))
+
+    # the path ------------------------------------------------------------
+    def boundary(self):
+        _check(_lib.phl_boundary(self._h))
+
+    def gamma_t(self):
+        _check(_lib.phl_gamma_t(self._h))
+
+    def gradient(self):
+        _check(_lib.phl_gradient(self._h))
+
+    def limiter(self):
+        _check(_lib.phl_limiter(self._h))
+
+    def update_residuals(self):
+        _check(_lib.phl_update_residuals(self._h))
+
+    def spectrum_radius(self):
+        _check(_lib.phl_spectrum_radius(self._h))
+
+    def time_step(self, step):
+        _check(_lib.phl_time_step(self._h, step))
+
+    def lusgs(self):
+        n = ctypes.c_int()
+        _check(_lib.phl_lusgs(self._h, ctypes.byref(n)))
+        return n.value
+
+    def residual_norm(self):
+        out = np.empty(5)
+        _check(_lib.phl_residual_norm(self._h, _dptr(out)))
+        return out
+
+    def iterate(self, first_step, n_steps, history=True):
+        hist = np.empty(max(n_steps, 1))
+        _check(_lib.phl_iterate(self._h, first_step, n_steps, _dptr(hist) if history else None))
+        return hist[:n_steps] if history else None
+
+    def solve(self, max_steps, drop):
+        hist = np.empty(max_steps)
+        n = ctypes.c_int()
+        _check(_lib.phl_solve(self._h, max_steps, drop, ctypes.byref(n), _dptr(hist)))
+        return n.value, hist
+
+    def synchronize(self):
+        _check(_lib.phl_synchronize(self._h))
+
+    def close(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            _lib.phl_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        self.close()

@@ -1,0 +1,784 @@
+// phl_api.cpp -- C-ABI entry points, device context and the host driver
+// (the LU-SGS branch of Controller::Relaxation, CFD/Solver/src/Controller.cpp:549-608).
+#include "../../include/phl.h"
+#include "phl_internal.h"
+#include "phl_mesh.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+struct phl_mesh {
+    phl::Mesh m;
+};
+
+struct phl_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    phl_params p{};
+    phl::DevMesh m{};
+    phl::Consts k{};
+    phl::DevFlow f{};
+    phl::Levels fwd{}, bwd{};
+    std::vector<void *> allocs;
+    std::vector<int> hfwdPtr, hbwdPtr;
+    int normSlots = 0;
+    double *hostNorm = nullptr;  // pinned
+    int *hostCount = nullptr;    // pinned
+    double primInf[5];
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+phl_status fail(phl_status s, const std::string &msg)
+{
+    g_err = msg;
+    return s;
+}
+
+#define HIPTRY(expr)                                                                        \
+    do {                                                                                    \
+        hipError_t e_ = (expr);                                                             \
+        if (e_ != hipSuccess)                                                               \
+            return fail(PHL_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e_));  \
+    } while (0)
+
+template <typename T>
+phl_status dalloc(phl_ctx *c, T **p, size_t n)
+{
+    void *ptr = nullptr;
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc(&ptr, n * sizeof(T));
+    if (e != hipSuccess) return fail(PHL_ERR_ALLOC, std::string("hipMalloc: ") + hipGetErrorString(e));
+    c->allocs.push_back(ptr);
+    *p = (T *)ptr;
+    return PHL_OK;
+}
+
+template <typename T>
+phl_status upload(phl_ctx *c, const T **dst, const std::vector<T> &src)
+{
+    T *d = nullptr;
+    phl_status s = dalloc(c, &d, src.size());
+    if (s != PHL_OK) return s;
+    if (!src.empty()) HIPTRY(hipMemcpy(d, src.data(), src.size() * sizeof(T), hipMemcpyHostToDevice));
+    *dst = d;
+    return PHL_OK;
+}
+
+#define TRY(expr)                          \
+    do {                                   \
+        phl_status s_ = (expr);            \
+        if (s_ != PHL_OK) return s_;       \
+    } while (0)
+
+// Cells of a zone ordered into dependency levels of the sequential sweep:
+// forward, a cell depends on its face neighbours with a lower index;
+// backward, on those with a higher index.  Cells of one level are
+// independent, so a level can be processed in parallel and the result equals
+// the reference's cell-by-cell loop.
+void build_levels(const phl::Mesh &m, const std::vector<int> &c2fPtr, const std::vector<int> &c2n,
+                  bool forward, std::vector<int> &ptr, std::vector<int> &cells)
+{
+    const int nC = m.nCell;
+    std::vector<int> level(nC, 0);
+    int nL = 0;
+    if (forward) {
+        for (int c = 0; c < nC; ++c) {
+            int lv = 0;
+            for (int k = c2fPtr[c]; k < c2fPtr[c + 1]; ++k) {
+                const int nb = c2n[k];
+                if (nb < c) lv = std::max(lv, level[nb] + 1);
+            }
+            level[c] = lv;
+            nL = std::max(nL, lv + 1);
+        }
+    } else {
+        for (int c = nC - 1; c >= 0; --c) {
+            int lv = 0;
+            for (int k = c2fPtr[c]; k < c2fPtr[c + 1]; ++k) {
+                const int nb = c2n[k];
+                if (nb > c && nb < nC) lv = std::max(lv, level[nb] + 1);
+            }
+            level[c] = lv;
+            nL = std::max(nL, lv + 1);
+        }
+    }
+    ptr.assign(nL + 1, 0);
+    for (int c = 0; c < nC; ++c) ptr[level[c] + 1]++;
+    for (int l = 0; l < nL; ++l) ptr[l + 1] += ptr[l];
+    cells.resize(nC);
+    std::vector<int> fill(ptr.begin(), ptr.end() - 1);
+    for (int c = 0; c < nC; ++c) cells[fill[level[c]]++] = c;
+}
+
+double compute_cfl(const phl_params &p, int step)
+{
+    // NSSolver::ComputeCFL (CFD/NSSolver/src/NSSolver.cpp:595-668), CFLMethod 0.
+    if (step >= p.cfl_vary_step) return p.cfl_end;
+    return p.cfl_start + (p.cfl_end - p.cfl_start) * step / p.cfl_vary_step;
+}
+
+double ptotal(double mach, double gama)
+{
+    // PerfectGas::ComputePTotal (API/Physics/src/PerfectGas.cpp:1616-1641).
+    const double R = 1.0 / (gama * mach * mach);
+    double pref = R * 1.0 * 1.0 / 1.0;
+    const double m12 = mach * mach;
+    const double gamp1 = gama + 1.0, gamm1 = gama - 1.0;
+    const double c1 = 2.0 / gamm1 + m12;
+    const double c2 = 2.0 * gama / gamm1 * m12 - 1.0;
+    const double cc = gama / (gama - 1.0);
+    const double t0t = 1.0 + 0.5 * gamm1 * c1 / c2;
+    if (mach > 1.0) pref *= (2.0 * gama / gamp1 * m12 - gamm1 / gamp1);
+    return pref * std::pow(t0t, cc);
+}
+
+double entropy_coef(double mach, double scale)
+{
+    // Param_NSSolver::SetEntropyFixCoefficients (CFD/NSSolver/src/Param_NSSolver.cpp:225-266).
+    double e;
+    if (mach <= 0.6) e = 0.0001;
+    else if (mach <= 2.0) {
+        const double x0 = 0.6, x1 = 2.0, y0 = 0.0001, y1 = 0.01;
+        e = y0 * (mach - x1) / (x0 - x1) + y1 * (mach - x0) / (x1 - x0);
+    } else if (mach <= 5.0) {
+        const double x0 = 2.0, x1 = 5.0, y0 = 0.01, y1 = 0.5;
+        e = y0 * (mach - x1) / (x0 - x1) + y1 * (mach - x0) / (x1 - x0);
+    } else e = 1.0;
+    return e * scale;
+}
+
+phl_status check_params(const phl_params &p)
+{
+    if (!(p.mach > 0.0) || !(p.gamma > 1.0)) return fail(PHL_ERR_ARG, "mach and gamma must be positive (gamma > 1)");
+    if (p.viscous != 0 && p.viscous != 1) return fail(PHL_ERR_UNSUPPORTED, "viscous must be 0 or 1");
+    if (p.viscous && !(p.reynolds > 0.0)) return fail(PHL_ERR_ARG, "reynolds must be positive for viscous flow");
+    if (p.gradient != 0) return fail(PHL_ERR_UNSUPPORTED, "only the ggcell gradient is implemented");
+    if (p.limiter < 0 || p.limiter > 2) return fail(PHL_ERR_UNSUPPORTED, "limiter must be 0 (vencat), 1 (nolim) or 2 (1st)");
+    if (p.entropy_fix != 1 && p.entropy_fix != 3 && p.entropy_fix != 5)
+        return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1, 3 or 5");
+    if (p.n_lusgs_sweeps != 1) return fail(PHL_ERR_UNSUPPORTED, "nLUSGSSweeps must be 1");
+    if (p.precision != 0) return fail(PHL_ERR_UNSUPPORTED, "precision must be 0 (FP64)");
+    if (p.t_wall > 0.0) return fail(PHL_ERR_UNSUPPORTED, "isothermal walls are not implemented (t_wall <= 0)");
+    if (p.cfl_vary_step <= 0) return fail(PHL_ERR_ARG, "cfl_vary_step must be positive");
+    return PHL_OK;
+}
+
+phl_status check_launch(phl_ctx *c)
+{
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return fail(PHL_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+    (void)c;
+    return PHL_OK;
+}
+
+phl_status copy_field(phl_ctx *c, const double *dsrc, int rows, int rowLen, int outRowLen, double *out)
+{
+    // rows of rowLen device values into rows of outRowLen host values (zero padded)
+    for (int r = 0; r < rows; ++r) {
+        double *o = out + (size_t)r * outRowLen;
+        if (rowLen > 0)
+            HIPTRY(hipMemcpyAsync(o, dsrc + (size_t)r * rowLen, (size_t)rowLen * sizeof(double),
+                                  hipMemcpyDeviceToHost, c->stream));
+        for (int i = rowLen; i < outRowLen; ++i) o[i] = 0.0;
+    }
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
+}
+
+void swap_state(phl_ctx *c) { std::swap(c->f.q, c->f.qAlt); }
+
+}  // namespace
+
+extern "C" {
+
+const char *phl_last_error(void) { return g_err.c_str(); }
+
+void phl_params_default(phl_params *p)
+{
+    std::memset(p, 0, sizeof(*p));
+    p->mach = 0.5;
+    p->alpha_deg = 0.0;
+    p->beta_deg = 0.0;
+    p->gamma = 1.4;
+    p->reynolds = 6.5e6;
+    p->t_ref = 288.15;
+    p->prandtl = 0.72;
+    p->prandtl_turb = 0.9;
+    p->t_wall = -1.0;
+    p->venkat_k = 5.0;
+    p->entropy_scale = 1.0;
+    p->cfl_start = 0.01;
+    p->cfl_end = 10.0;
+    p->ktmax = 1.0e10;
+    p->lusgs_tol = 0.01;
+    p->force_ref_length = 1.0;
+    p->force_ref_area = 1.0;
+    p->viscous = 0;
+    p->gradient = 0;
+    p->limiter = 0;
+    p->entropy_fix = 3;
+    p->cfl_vary_step = 500;
+    p->n_lusgs_sweeps = 1;
+    p->precision = 0;
+}
+
+// ------------------------------------------------------------------ meshes
+phl_status phl_mesh_create(int nNode, const double *x, const double *y, const double *z, int nFace,
+                           const int *faceNodePtr, const int *faceNodes, const int *left, const int *right,
+                           int nBFace, const int *bcType, int nCell, const int *cellNodePtr,
+                           const int *cellNodes, phl_mesh **out)
+{
+    if (!out || nNode <= 0 || nFace <= 0 || nCell <= 0 || nBFace < 0 || nBFace > nFace)
+        return fail(PHL_ERR_ARG, "phl_mesh_create: bad sizes");
+    if (!x || !y || !z || !faceNodePtr || !faceNodes || !left || !right || !cellNodePtr || !cellNodes ||
+        (nBFace > 0 && !bcType))
+        return fail(PHL_ERR_ARG, "phl_mesh_create: null array");
+    phl_mesh *pm = new (std::nothrow) phl_mesh;
+    if (!pm) return fail(PHL_ERR_ALLOC, "phl_mesh_create: out of memory");
+    phl::Mesh &m = pm->m;
+    m.nNode = nNode; m.nCell = nCell; m.nFace = nFace; m.nBFace = nBFace;
+    m.x.assign(x, x + nNode); m.y.assign(y, y + nNode); m.z.assign(z, z + nNode);
+    m.faceNodePtr.assign(faceNodePtr, faceNodePtr + nFace + 1);
+    m.faceNodes.assign(faceNodes, faceNodes + faceNodePtr[nFace]);
+    m.cellNodePtr.assign(cellNodePtr, cellNodePtr + nCell + 1);
+    m.cellNodes.assign(cellNodes, cellNodes + cellNodePtr[nCell]);
+    m.left.assign(left, left + nFace);
+    m.right.assign(right, right + nFace);
+    m.bcType.assign(bcType, bcType + nBFace);
+    for (int f = 0; f < nFace; ++f) {
+        const bool ok = left[f] >= 0 && left[f] < nCell &&
+                        (f < nBFace ? right[f] == nCell + f : (right[f] >= 0 && right[f] < nCell && right[f] != left[f]));
+        if (!ok) {
+            delete pm;
+            return fail(PHL_ERR_ARG, "phl_mesh_create: face " + std::to_string(f) + " has bad left/right cells");
+        }
+    }
+    m.computeMetrics();
+    *out = pm;
+    return PHL_OK;
+}
+
+phl_status phl_mesh_bump_hex(int nx, int ny, int nz, double bump, phl_mesh **out)
+{
+    if (!out) return fail(PHL_ERR_ARG, "null out");
+    phl_mesh *pm = new (std::nothrow) phl_mesh;
+    if (!pm) return fail(PHL_ERR_ALLOC, "out of memory");
+    std::string err;
+    if (!phl::makeBumpHex(nx, ny, nz, bump, pm->m, err)) {
+        delete pm;
+        return fail(PHL_ERR_ARG, err);
+    }
+    *out = pm;
+    return PHL_OK;
+}
+
+phl_status phl_mesh_sizes(const phl_mesh *pm, int *nNode, int *nCell, int *nFace, int *nBFace, int *nFaceNodes,
+                          int *nCellNodes)
+{
+    if (!pm) return fail(PHL_ERR_ARG, "null mesh");
+    const phl::Mesh &m = pm->m;
+    if (nNode) *nNode = m.nNode;
+    if (nCell) *nCell = m.nCell;
+    if (nFace) *nFace = m.nFace;
+    if (nBFace) *nBFace = m.nBFace;
+    if (nFaceNodes) *nFaceNodes = (int)m.faceNodes.size();
+    if (nCellNodes) *nCellNodes = (int)m.cellNodes.size();
+    return PHL_OK;
+}
+
+phl_status phl_mesh_connectivity(const phl_mesh *pm, double *x, double *y, double *z, int *faceNodePtr,
+                                 int *faceNodes, int *left, int *right, int *bcType, int *cellNodePtr,
+                                 int *cellNodes)
+{
+    if (!pm) return fail(PHL_ERR_ARG, "null mesh");
+    const phl::Mesh &m = pm->m;
+    auto cp = [](auto *dst, const auto &v) {
+        if (dst) std::copy(v.begin(), v.end(), dst);
+    };
+    cp(x, m.x); cp(y, m.y); cp(z, m.z);
+    cp(faceNodePtr, m.faceNodePtr); cp(faceNodes, m.faceNodes);
+    cp(left, m.left); cp(right, m.right); cp(bcType, m.bcType);
+    cp(cellNodePtr, m.cellNodePtr); cp(cellNodes, m.cellNodes);
+    return PHL_OK;
+}
+
+phl_status phl_mesh_metrics(const phl_mesh *pm, double *xfc, double *yfc, double *zfc, double *xfn, double *yfn,
+                            double *zfn, double *area, double *xcc, double *ycc, double *zcc, double *vol,
+                            double *skew)
+{
+    if (!pm) return fail(PHL_ERR_ARG, "null mesh");
+    const phl::Mesh &m = pm->m;
+    auto cp = [](double *dst, const std::vector<double> &v) {
+        if (dst) std::copy(v.begin(), v.end(), dst);
+    };
+    cp(xfc, m.xfc); cp(yfc, m.yfc); cp(zfc, m.zfc);
+    cp(xfn, m.xfn); cp(yfn, m.yfn); cp(zfn, m.zfn); cp(area, m.area);
+    cp(xcc, m.xcc); cp(ycc, m.ycc); cp(zcc, m.zcc); cp(vol, m.vol); cp(skew, m.skew);
+    return PHL_OK;
+}
+
+void phl_mesh_destroy(phl_mesh *m) { delete m; }
+
+// ------------------------------------------------------------- lifecycle
+phl_status phl_device_info(int device, char *name, int name_len, int *n_cu, double *hbm_bytes)
+{
+    hipDeviceProp_t prop;
+    HIPTRY(hipGetDeviceProperties(&prop, device));
+    if (name && name_len > 0) {
+        std::snprintf(name, (size_t)name_len, "%s (%s)", prop.name, prop.gcnArchName);
+    }
+    if (n_cu) *n_cu = prop.multiProcessorCount;
+    if (hbm_bytes) *hbm_bytes = (double)prop.totalGlobalMem;
+    return PHL_OK;
+}
+
+void phl_destroy(phl_ctx *c)
+{
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    for (void *p : c->allocs) hipFree(p);
+    if (c->hostNorm) hipHostFree(c->hostNorm);
+    if (c->hostCount) hipHostFree(c->hostCount);
+    if (c->stream) hipStreamDestroy(c->stream);
+    delete c;
+}
+
+static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
+{
+    const phl_params &p = c->p;
+    const int nC = m.nCell, nB = m.nBFace, nF = m.nFace, nT = m.nTotal();
+    HIPTRY(hipSetDevice(c->device));
+    HIPTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    // cell -> face (increasing face order) and cell -> neighbour
+    std::vector<int> c2fPtr(nC + 1, 0);
+    for (int f = 0; f < nF; ++f) {
+        c2fPtr[m.left[f] + 1]++;
+        if (f >= nB) c2fPtr[m.right[f] + 1]++;
+    }
+    for (int i = 0; i < nC; ++i) c2fPtr[i + 1] += c2fPtr[i];
+    std::vector<int> c2f(c2fPtr[nC]), c2n(c2fPtr[nC]), fill(nC, 0);
+    std::vector<char> isLeft(c2fPtr[nC]);
+    for (int f = 0; f < nF; ++f) {
+        const int le = m.left[f];
+        int k = c2fPtr[le] + fill[le]++;
+        c2f[k] = f; c2n[k] = m.right[f]; isLeft[k] = 1;
+        if (f >= nB) {
+            const int re = m.right[f];
+            k = c2fPtr[re] + fill[re]++;
+            c2f[k] = f; c2n[k] = le; isLeft[k] = 0;
+        }
+    }
+    int W = 0;
+    for (int i = 0; i < nC; ++i) W = std::max(W, c2fPtr[i + 1] - c2fPtr[i]);
+    std::vector<int> ellEnc((size_t)W * nC, 0), ellNb((size_t)W * nC, 0);
+    for (int i = 0; i < nC; ++i)
+        for (int k = c2fPtr[i]; k < c2fPtr[i + 1]; ++k) {
+            const int s = k - c2fPtr[i];
+            ellEnc[(size_t)s * nC + i] = isLeft[k] ? (c2f[k] + 1) : -(c2f[k] + 1);
+            ellNb[(size_t)s * nC + i] = c2n[k];
+        }
+
+    std::vector<phl::FaceRec> faces(nF);
+    for (int f = 0; f < nF; ++f)
+        faces[f] = {m.xfn[f], m.yfn[f], m.zfn[f], m.area[f], m.xfc[f], m.yfc[f], m.zfc[f], 0.0};
+
+    // Venkatakrishnan scale and CFL skewness factor (geometry only).
+    std::vector<double> vf(nC), cflSkew(nC);
+    for (int i = 0; i < nC; ++i) {
+        vf[i] = std::pow(std::pow(m.vol[i], 1.0 / 3), 3.0);
+        const double sk = m.skew[i];
+        cflSkew[i] = (sk < 0.1) ? 0.1 : ((sk < 10.0) ? (1.0 - std::exp(0.0 - sk)) : 1.0);
+    }
+    std::vector<unsigned char> limZero(nC, 0);
+    std::vector<double> limInit(nT, 0.0);
+    for (int i = 0; i < nT; ++i) limInit[i] = (p.limiter == 2) ? 0.0 : 1.0;
+    for (int f = 0; f < nB; ++f) {
+        const int bc = m.bcType[f];
+        const bool keep = bc == PHL_BC_SYMMETRY || bc == PHL_BC_SOLID_SURFACE || bc == PHL_BC_OUTFLOW;
+        if (!keep && p.limiter == 0) {
+            limZero[m.left[f]] = 1;
+            limInit[nC + f] = 0.0;
+        }
+    }
+    // Viscous face flags from the skewness angles (ComputeVisflux :13101-13113,13174,13209).
+    std::vector<unsigned char> visFlag(nF, 0);
+    const double kPi = 3.141592653589793, kSmall = 1.0e-40;
+    for (int f = 0; f < nF; ++f) {
+        const int le = m.left[f], re = m.right[f];
+        const double dxL = m.xcc[le] - m.xfc[f], dyL = m.ycc[le] - m.yfc[f], dzL = m.zcc[le] - m.zfc[f];
+        const double dxR = m.xcc[re] - m.xfc[f], dyR = m.ycc[re] - m.yfc[f], dzR = m.zcc[re] - m.zfc[f];
+        const double dL = m.xfn[f] * dxL + m.yfn[f] * dyL + m.zfn[f] * dzL;
+        const double dR = m.xfn[f] * dxR + m.yfn[f] * dyR + m.zfn[f] * dzR;
+        double t = -dL / (std::sqrt(dxL * dxL + dyL * dyL + dzL * dzL) + kSmall);
+        t = std::min(t, 1.0); t = std::max(t, -1.0);
+        const double a1 = std::asin(t) * 180.0 / kPi;
+        t = dR / (std::sqrt(dxR * dxR + dyR * dyR + dzR * dzR) + kSmall);
+        t = std::min(t, 1.0); t = std::max(t, -1.0);
+        const double a2 = std::asin(t) * 180.0 / kPi;
+        unsigned char fl = 0;
+        if (a1 > 60.0 && a2 > 60.0) fl |= 1;
+        if (a1 > 0.0 && a2 > 0.0) fl |= 2;
+        visFlag[f] = fl;
+    }
+
+    phl::DevMesh &dm = c->m;
+    dm.nCell = nC; dm.nBFace = nB; dm.nFace = nF; dm.nTotal = nT; dm.ellW = W;
+    TRY(upload(c, &dm.face, faces));
+    TRY(upload(c, &dm.bcType, m.bcType));
+    TRY(upload(c, &dm.fle, m.left));
+    TRY(upload(c, &dm.c2fEnc, ellEnc));
+    TRY(upload(c, &dm.c2n, ellNb));
+    TRY(upload(c, &dm.xcc, m.xcc));
+    TRY(upload(c, &dm.ycc, m.ycc));
+    TRY(upload(c, &dm.zcc, m.zcc));
+    TRY(upload(c, &dm.vol, m.vol));
+    TRY(upload(c, &dm.venkatVf, vf));
+    TRY(upload(c, &dm.cflSkew, cflSkew));
+    TRY(upload(c, &dm.limZero, limZero));
+    dm.limGhost = nullptr;
+    TRY(upload(c, &dm.visFlag, visFlag));
+
+    // LU-SGS level schedules
+    std::vector<int> cellsF, cellsB;
+    build_levels(m, c2fPtr, c2n, true, c->hfwdPtr, cellsF);
+    build_levels(m, c2fPtr, c2n, false, c->hbwdPtr, cellsB);
+    const int *tmp;
+    TRY(upload(c, &tmp, c->hfwdPtr)); c->fwd.ptr = const_cast<int *>(tmp);
+    TRY(upload(c, &tmp, cellsF)); c->fwd.cells = const_cast<int *>(tmp);
+    TRY(upload(c, &tmp, c->hbwdPtr)); c->bwd.ptr = const_cast<int *>(tmp);
+    TRY(upload(c, &tmp, cellsB)); c->bwd.cells = const_cast<int *>(tmp);
+    c->fwd.nLevels = (int)c->hfwdPtr.size() - 1; c->fwd.hptr = c->hfwdPtr.data();
+    c->bwd.nLevels = (int)c->hbwdPtr.size() - 1; c->bwd.hptr = c->hbwdPtr.data();
+
+    // constants
+    phl::Consts &k = c->k;
+    k.gamma = p.gamma;
+    k.R = 1.0 / (p.gamma * p.mach * p.mach);
+    k.mach = p.mach;
+    k.reynolds = p.reynolds;
+    k.oRe = p.reynolds > 0.0 ? 1.0 / p.reynolds : 0.0;
+    {
+        const double kPiD = 3.141592653589793;
+        const double attack = p.alpha_deg * kPiD / 180.0, slide = p.beta_deg * kPiD / 180.0;
+        c->primInf[0] = 1.0;
+        c->primInf[1] = 1.0 * (std::cos(attack) * std::cos(slide));
+        c->primInf[2] = 1.0 * (std::sin(attack) * std::cos(slide));
+        c->primInf[3] = 1.0 * std::sin(slide);
+        c->primInf[4] = k.R * 1.0 * 1.0 / 1.0;
+        for (int i = 0; i < 5; ++i) k.primInf[i] = c->primInf[i];
+    }
+    k.entropyCoef = entropy_coef(p.mach, p.entropy_scale);
+    {
+        double e = p.venkat_k * p.venkat_k * p.venkat_k;
+        e *= 1.0 / std::pow(std::pow(m.avgVol, 1.0 / 3), 3.0);
+        k.epsTmp = e;
+    }
+    const double pT = ptotal(p.mach, p.gamma);
+    k.pLower = 0.001 * pT;
+    k.pUpper = 0.02 * pT;
+    k.cflStart = p.cfl_start;
+    k.ktmax = p.ktmax;
+    k.tsuth = 110.4 / p.t_ref;
+    k.cp = 1.0 / ((p.gamma - 1.0) * (p.mach * p.mach));
+    k.oprl = 1.0 / p.prandtl;
+    k.oprt = 1.0 / p.prandtl_turb;
+    k.densLimit = 1.0e-6 * k.primInf[0];
+    k.presLimit = 1.0e-6 * k.primInf[4];
+    k.viscous = p.viscous;
+    k.limiter = p.limiter;
+    k.entropyFix = p.entropy_fix;
+    k.nSweeps = p.n_lusgs_sweeps;
+
+    // flow fields
+    phl::DevFlow &fl = c->f;
+    TRY(dalloc(c, &fl.q, (size_t)5 * nT));
+    TRY(dalloc(c, &fl.qAlt, (size_t)5 * nT));
+    TRY(dalloc(c, &fl.res, (size_t)5 * nC));
+    TRY(dalloc(c, &fl.grad, (size_t)15 * nC));
+    TRY(dalloc(c, &fl.gradT, (size_t)3 * nC));
+    TRY(dalloc(c, &fl.lim, (size_t)nT));
+    TRY(dalloc(c, &fl.dq, (size_t)5 * nT));
+    TRY(dalloc(c, &fl.dflux, (size_t)5 * nC));
+    TRY(dalloc(c, &fl.specInv, (size_t)nC));
+    TRY(dalloc(c, &fl.specVis, (size_t)nC));
+    TRY(dalloc(c, &fl.dtLoc, (size_t)nC));
+    TRY(dalloc(c, &fl.dt, (size_t)nC));
+    TRY(dalloc(c, &fl.diag, (size_t)nC));
+    TRY(dalloc(c, &fl.normPart, (size_t)5 * phl::flux_blocks(dm)));
+    c->normSlots = 1024;
+    TRY(dalloc(c, &fl.normOut, (size_t)5 * c->normSlots));
+    TRY(dalloc(c, &fl.dtMinBits, 1));
+    TRY(dalloc(c, &fl.negFlag, (size_t)nC));
+    TRY(dalloc(c, &fl.negCount, 1));
+    TRY(dalloc(c, &fl.sweepPart, 1));
+    TRY(dalloc(c, &fl.t, (size_t)nT));
+    TRY(dalloc(c, &fl.visl, (size_t)nT));
+    HIPTRY(hipMemset(fl.q, 0, sizeof(double) * 5 * nT));
+    HIPTRY(hipMemset(fl.qAlt, 0, sizeof(double) * 5 * nT));
+    HIPTRY(hipMemset(fl.res, 0, sizeof(double) * 5 * nC));
+    HIPTRY(hipMemset(fl.grad, 0, sizeof(double) * 15 * nC));
+    HIPTRY(hipMemset(fl.gradT, 0, sizeof(double) * 3 * nC));
+    HIPTRY(hipMemset(fl.dq, 0, sizeof(double) * 5 * nT));
+    HIPTRY(hipMemset(fl.dflux, 0, sizeof(double) * 5 * nC));
+    HIPTRY(hipMemset(fl.specVis, 0, sizeof(double) * nC));
+    HIPTRY(hipMemset(fl.negFlag, 0, (size_t)nC));
+    HIPTRY(hipMemset(fl.negCount, 0, sizeof(int)));
+    HIPTRY(hipMemcpy(fl.lim, limInit.data(), sizeof(double) * nT, hipMemcpyHostToDevice));
+    HIPTRY(hipHostMalloc((void **)&c->hostNorm, sizeof(double) * 5 * c->normSlots, hipHostMallocDefault));
+    HIPTRY(hipHostMalloc((void **)&c->hostCount, sizeof(int), hipHostMallocDefault));
+    HIPTRY(hipDeviceSynchronize());
+    return PHL_OK;
+}
+
+phl_status phl_create(const phl_mesh *pm, const phl_params *p, int device, phl_ctx **out)
+{
+    if (!pm || !p || !out) return fail(PHL_ERR_ARG, "phl_create: null argument");
+    TRY(check_params(*p));
+    int nDev = 0;
+    HIPTRY(hipGetDeviceCount(&nDev));
+    if (device < 0 || device >= nDev) return fail(PHL_ERR_ARG, "phl_create: no such device");
+    phl_ctx *c = new (std::nothrow) phl_ctx;
+    if (!c) return fail(PHL_ERR_ALLOC, "phl_create: out of memory");
+    c->device = device;
+    c->p = *p;
+    phl_status s = create_impl(c, pm->m);
+    if (s != PHL_OK) {
+        std::string keep = g_err;
+        phl_destroy(c);
+        g_err = keep;
+        return s;
+    }
+    *out = c;
+    return PHL_OK;
+}
+
+// ---------------------------------------------------------------- state
+phl_status phl_set_state(phl_ctx *c, const double *q)
+{
+    if (!c || !q) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    HIPTRY(hipMemcpyAsync(c->f.q, q, sizeof(double) * 5 * c->m.nTotal, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
+}
+
+phl_status phl_get_state(phl_ctx *c, double *q)
+{
+    if (!c || !q) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    return copy_field(c, c->f.q, 5, c->m.nTotal, c->m.nTotal, q);
+}
+
+phl_status phl_get_residual(phl_ctx *c, double *res)
+{
+    if (!c || !res) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    return copy_field(c, c->f.res, 5, c->m.nCell, c->m.nTotal, res);
+}
+
+phl_status phl_get_field(phl_ctx *c, int field, double *out)
+{
+    if (!c || !out) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    const int nC = c->m.nCell, nT = c->m.nTotal;
+    phl::DevFlow &f = c->f;
+    switch (field) {
+    case PHL_FIELD_Q: return copy_field(c, f.q, 5, nT, nT, out);
+    case PHL_FIELD_RES: return copy_field(c, f.res, 5, nC, nT, out);
+    case PHL_FIELD_DQDX: return copy_field(c, f.grad, 5, nC, nT, out);
+    case PHL_FIELD_DQDY: return copy_field(c, f.grad + (size_t)5 * nC, 5, nC, nT, out);
+    case PHL_FIELD_DQDZ: return copy_field(c, f.grad + (size_t)10 * nC, 5, nC, nT, out);
+    case PHL_FIELD_LIMIT: return copy_field(c, f.lim, 1, nT, nT, out);
+    case PHL_FIELD_DQ: return copy_field(c, f.dq, 5, nT, nT, out);
+    case PHL_FIELD_SPEC_INV: return copy_field(c, f.specInv, 1, nC, nT, out);
+    case PHL_FIELD_SPEC_VIS: return copy_field(c, f.specVis, 1, nC, nT, out);
+    case PHL_FIELD_DT: return copy_field(c, f.dt, 1, nC, nT, out);
+    case PHL_FIELD_DIAG: return copy_field(c, f.diag, 1, nC, nT, out);
+    case PHL_FIELD_T:
+    case PHL_FIELD_VISL:
+        phl::launch_gamma_t(c->m, c->k, f, c->stream);
+        TRY(check_launch(c));
+        return copy_field(c, field == PHL_FIELD_T ? f.t : f.visl, 1, nT, nT, out);
+    case PHL_FIELD_DTDX: return copy_field(c, f.gradT, 1, nC, nT, out);
+    case PHL_FIELD_DTDY: return copy_field(c, f.gradT + nC, 1, nC, nT, out);
+    case PHL_FIELD_DTDZ: return copy_field(c, f.gradT + (size_t)2 * nC, 1, nC, nT, out);
+    case PHL_FIELD_DELTA_FLUX: return copy_field(c, f.dflux, 5, nC, nT, out);
+    default: return fail(PHL_ERR_ARG, "phl_get_field: unknown field");
+    }
+}
+
+phl_status phl_init_freestream(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    const int nT = c->m.nTotal;
+    std::vector<double> h((size_t)5 * nT);
+    for (int m = 0; m < 5; ++m) std::fill(h.begin() + (size_t)m * nT, h.begin() + (size_t)(m + 1) * nT, c->primInf[m]);
+    return phl_set_state(c, h.data());
+}
+
+// ----------------------------------------------------------------- path
+phl_status phl_boundary(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_boundary(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_gamma_t(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_gamma_t(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_gradient(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_limiter(phl_ctx *c) { return phl_gradient(c); }
+
+static void enqueue_residual(phl_ctx *c, int slot)
+{
+    phl::launch_boundary(c->m, c->k, c->f, c->stream);
+    phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
+    phl::launch_flux(c->m, c->k, c->f, c->stream);
+    phl::launch_norm_final(c->m, c->f, slot, c->stream);
+}
+
+phl_status phl_update_residuals(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    enqueue_residual(c, 0);
+    return check_launch(c);
+}
+
+phl_status phl_spectrum_radius(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_spectrum(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_time_step(phl_ctx *c, int step)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_time_step(c->m, c->k, c->f, compute_cfl(c->p, step), c->stream);
+    phl::launch_dt_diag(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+static void enqueue_lusgs(phl_ctx *c)
+{
+    phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, c->stream);
+    phl::launch_ghost_dq(c->m, c->k, c->f, c->stream);
+    phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, c->stream);
+    phl::launch_update(c->m, c->k, c->f, c->stream);
+    swap_state(c);
+}
+
+phl_status phl_lusgs(phl_ctx *c, int *n_fixed)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    HIPTRY(hipMemsetAsync(c->f.negCount, 0, sizeof(int), c->stream));
+    enqueue_lusgs(c);
+    TRY(check_launch(c));
+    if (n_fixed) {
+        HIPTRY(hipMemcpyAsync(c->hostCount, c->f.negCount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPTRY(hipStreamSynchronize(c->stream));
+        *n_fixed = *c->hostCount;
+    }
+    return PHL_OK;
+}
+
+phl_status phl_residual_norm(phl_ctx *c, double *sum5)
+{
+    if (!c || !sum5) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    for (int m = 0; m < 5; ++m) sum5[m] = c->hostNorm[m];
+    return PHL_OK;
+}
+
+// ----------------------------------------------------------- host driver
+phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist)
+{
+    if (!c || n_steps < 0) return fail(PHL_ERR_ARG, "bad argument");
+    HIPTRY(hipSetDevice(c->device));
+    int done = 0;
+    while (done < n_steps) {
+        const int chunk = std::min(n_steps - done, c->normSlots);
+        for (int i = 0; i < chunk; ++i) {
+            const int step = first_step + done + i;
+            enqueue_residual(c, i);
+            phl::launch_time_step(c->m, c->k, c->f, compute_cfl(c->p, step), c->stream);
+            phl::launch_dt_diag(c->m, c->k, c->f, c->stream);
+            enqueue_lusgs(c);
+        }
+        TRY(check_launch(c));
+        HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5 * chunk, hipMemcpyDeviceToHost,
+                              c->stream));
+        HIPTRY(hipStreamSynchronize(c->stream));
+        for (int i = 0; i < chunk; ++i) {
+            const double s0 = c->hostNorm[(size_t)i * 5];
+            if (!std::isfinite(s0))
+                return fail(PHL_ERR_NUMERIC, "residual is not finite at step " + std::to_string(first_step + done + i));
+            if (rms_hist) rms_hist[done + i] = std::sqrt(s0 / c->m.nCell);
+        }
+        done += chunk;
+    }
+    return PHL_OK;
+}
+
+phl_status phl_solve(phl_ctx *c, int max_steps, double drop, int *steps_done, double *rms_hist)
+{
+    if (!c || max_steps <= 0 || !(drop > 0.0)) return fail(PHL_ERR_ARG, "bad argument");
+    std::vector<double> hist(max_steps);
+    const int chunk = 25;
+    int done = 0;
+    double r1 = 0.0;
+    int hit = -1;
+    while (done < max_steps && hit < 0) {
+        const int n = std::min(chunk, max_steps - done);
+        TRY(phl_iterate(c, 1 + done, n, hist.data() + done));
+        if (done == 0) r1 = hist[0];
+        for (int i = done; i < done + n; ++i)
+            if (hist[i] <= drop * r1) { hit = i; break; }
+        done += n;
+    }
+    if (steps_done) *steps_done = (hit >= 0) ? hit + 1 : done;
+    if (rms_hist) std::copy(hist.begin(), hist.begin() + done, rms_hist);
+    return PHL_OK;
+}
+
+phl_status phl_synchronize(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
+}
+
+}  // extern "C"

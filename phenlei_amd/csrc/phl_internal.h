@@ -1,0 +1,97 @@
+// phl_internal.h -- device data layout shared by the kernels and the host
+// driver.  Not part of the C-ABI.
+//
+// Layout in HBM (FP64 plain form):
+//   cell fields     SoA [m][nCell] or [m][nTotal] (ghosts after real cells)
+//   face geometry   AoS FaceRec, 64 B per face (one gather touches one line)
+//   cell->face map  ELL [W][nCell] (W = max faces per cell), faces in
+//                   increasing face order so per-cell accumulation follows
+//                   the reference's face-loop order exactly
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace phl {
+
+struct FaceRec {
+    double nx, ny, nz, area, xc, yc, zc, pad;
+};
+
+// Read-only mesh data on the device.
+struct DevMesh {
+    int nCell, nBFace, nFace, nTotal, ellW;
+    const FaceRec *face;        // [nFace]
+    const int *bcType;          // [nBFace]
+    const int *fle;             // [nFace] left cell
+    const int *c2fEnc;          // [W][nCell]: f+1 if the cell is left, -(f+1) if right, 0 pad
+    const int *c2n;             // [W][nCell]: the other cell (ghost index nCell+f on boundaries)
+    const double *xcc, *ycc, *zcc;  // [nTotal]
+    const double *vol;          // [nTotal]
+    const double *venkatVf;     // [nCell] (vol^(1/3))^3, Limiter.cpp:1126-1128
+    const double *cflSkew;      // [nCell] skewness CFL factor, NSSolverUnstruct.cpp:6106-6116
+    const unsigned char *limZero;  // [nCell] 1 if next to a farfield-type face
+    const double *limGhost;     // [nBFace] limiter value of the ghost cell
+    const unsigned char *visFlag;  // [nFace] bit0: skewness correction, bit1: normal derivative
+};
+
+// Scalars of the path, fixed per context.
+struct Consts {
+    double gamma, R, mach, reynolds, oRe;
+    double primInf[5];
+    double entropyCoef;     // RoeEntropyFixCoef1 == Coef2
+    double epsTmp;          // K^3 / (avgVol^(1/3))^3
+    double pLower, pUpper;  // 0.001, 0.02 of pTotal
+    double cflStart, ktmax;
+    double tsuth;           // 110.4 / T_ref
+    double cp;              // 1/((gamma-1) M^2)
+    double oprl, oprt;
+    double densLimit, presLimit;  // 1e-6 freestream
+    int viscous, limiter, entropyFix, nSweeps;
+};
+
+// Mutable flow state on the device.
+struct DevFlow {
+    double *q;        // [5][nTotal] primitive variables (current)
+    double *qAlt;     // [5][nTotal] update target, swapped with q
+    double *res;      // [5][nCell]
+    double *grad;     // [15][nCell]: d/dx of m at [m], d/dy at [5+m], d/dz at [10+m]
+    double *gradT;    // [3][nCell] (viscous)
+    double *lim;      // [nTotal]
+    double *dq;       // [5][nTotal]
+    double *dflux;    // [5][nCell]
+    double *specInv, *specVis, *dtLoc, *dt, *diag;  // [nCell]
+    double *normPart;  // [nBlocksFlux][5]
+    double *normOut;   // [maxSteps][5]
+    unsigned long long *dtMinBits;  // global min of dtLoc (positive doubles order as integers)
+    unsigned char *negFlag;          // [nCell]
+    int *negCount;
+    double *sweepPart;  // LU-SGS sweep norm partials
+    double *t, *visl;   // [nTotal], only for the standalone gamma/T hook
+};
+
+// LU-SGS level schedule.
+struct Levels {
+    int nLevels;
+    int *ptr;    // device [nLevels+1]
+    int *cells;  // device [nCell]
+    int *hptr;   // host copy of ptr
+};
+
+// Launchers (phl_kernels.hip).  All are asynchronous on `s`.
+void launch_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+int flux_blocks(const DevMesh &m);
+void launch_norm_final(const DevMesh &m, DevFlow &f, int slot, hipStream_t s);
+void launch_time_step(const DevMesh &m, const Consts &k, DevFlow &f, double cfl, hipStream_t s);
+void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,
+                  bool firstSweep, hipStream_t s);
+void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+
+}  // namespace phl

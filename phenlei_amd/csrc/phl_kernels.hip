@@ -1,0 +1,1067 @@
+// phl_kernels.hip -- gfx950 kernels of the residual + LU-SGS path (FP64).
+//
+// Every face loop of the reference (which scatters face contributions into
+// both neighbour cells) is evaluated here as an owner-computes gather: one
+// thread per cell walks its faces in increasing face order, recomputes each
+// face quantity and adds it with the sign the reference's scatter would use.
+// Because a cell receives its contributions in the same order as in the
+// reference face loop, the per-cell sums round identically; there are no
+// atomics on the residual.  Compile with -ffp-contract=off.
+//
+// Reference functions restated (CFD/NSSolver/src/NSSolverUnstruct.cpp unless
+// noted): Boundary :13404, FarfieldBCRegion :21195, SymmetryBCRegion :19413,
+// ViscousAdiabaticWallBCRegion :20988, OutflowBCRegion :19326;
+// CompGradientGGCell Geo_UnstructGrid.cpp:3220; VencatLimiter7 Limiter.cpp:1091;
+// ReConstructFaceValue :13991; BoundaryQlQrFix :13725; Roe_Scheme_ConservativeForm
+// Flux_Inviscid.cpp:136; Flux_RoeEntropyCorrection Flux_RoeEntropyCorrection.cpp:93;
+// LoadFlux :6435; ComputeVisflux :12977; SpectrumRadiusInviscid :5749;
+// SpectrumRadiusViscous :5852; LocalTimeStep :5999; LimitCFL :6052;
+// ReduceMaxTimeStep :5717; Diagonal :15867; SolveLUSGSForward :15118;
+// SolveLUSGSBackward :15446; MXDQ_STD Jacobian.cpp:13; SetGhostDQLUSGS :14767;
+// UpdateFlowField :14587; SolutionFix :14509.
+#include "phl_internal.h"
+
+namespace phl {
+
+namespace {
+
+constexpr double kSMALL = 1.0e-40;
+constexpr double kTINY = 1.0e-40;
+constexpr int kBC_EXTRAP = 1, kBC_WALL = 2, kBC_SYMM = 3, kBC_FAR = 4, kBC_OUTFLOW = 6;
+constexpr int kBlock = 256;
+
+__device__ __forceinline__ double dMIN(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double dMAX(double a, double b) { return (a > b) ? a : b; }
+__device__ __forceinline__ double dABS(double a) { return (a < 0) ? -a : a; }
+
+// Blocks are dispatched round-robin over the 8 XCDs (block b lands on XCD
+// b % 8).  Give each XCD a contiguous range of cells so that neighbouring
+// cells share an L2.
+__device__ __forceinline__ int xcd_block(int b, int nblk)
+{
+    const int q = nblk >> 3, r = nblk & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
+__device__ __forceinline__ double harten(double eig, double lim)
+{
+    double e = dABS(eig);
+    if (e < lim) e = 0.5 * (eig * eig + lim * lim) / lim;
+    return e;
+}
+
+// Roe flux per unit area, conservative form, static mesh, perfect gas.
+__device__ void roe_flux(const double *qLin, const double *qRin, double xsn, double ysn, double zsn,
+                         double gama_, int fixMethod, double ent, double *flux)
+{
+    const double LITTLE = 1.0E-10;
+    const double gmL = gama_, gmR = gama_;
+    double rL = qLin[0], uL = qLin[1], vL = qLin[2], wL = qLin[3], pL = qLin[4];
+    double rR = qRin[0], uR = qRin[1], vR = qRin[2], wR = qRin[3], pR = qRin[4];
+    rL = dMAX(rL, LITTLE); rR = dMAX(rR, LITTLE);
+    pL = dMAX(pL, LITTLE); pR = dMAX(pR, LITTLE);
+    const double v2L = uL * uL + vL * vL + wL * wL;
+    const double v2R = uR * uR + vR * vR + wR * wR;
+    const double hintL = (gmL / (gmL - 1.0)) * (pL / rL);
+    const double hintR = (gmR / (gmR - 1.0)) * (pR / rR);
+    const double hL = hintL + 0.5 * v2L;
+    const double hR = hintR + 0.5 * v2R;
+    const double vb = 0.0;
+    const double vnL = xsn * uL + ysn * vL + zsn * wL - vb;
+    const double vnR = xsn * uR + ysn * vR + zsn * wR - vb;
+    const double rvnL = rL * vnL, rvnR = rR * vnR;
+    flux[0] = 0.5 * (rvnL + rvnR);
+    flux[1] = 0.5 * (rvnL * uL + xsn * pL + rvnR * uR + xsn * pR);
+    flux[2] = 0.5 * (rvnL * vL + ysn * pL + rvnR * vR + ysn * pR);
+    flux[3] = 0.5 * (rvnL * wL + zsn * pL + rvnR * wR + zsn * pR);
+    flux[4] = 0.5 * (rvnL * hL + vb * pL + rvnR * hR + vb * pR);
+
+    const double ratio = sqrt(rR / rL);
+    const double coef = 1.0 / (1.0 + ratio);
+    const double rRoe = sqrt(rL * rR);
+    const double uRoe = (uL + uR * ratio) * coef;
+    const double vRoe = (vL + vR * ratio) * coef;
+    const double wRoe = (wL + wR * ratio) * coef;
+    const double pRoe = (pL + pR * ratio) * coef;
+    const double gama = (gmL + gmR * ratio) * coef;
+    const double gamm1 = gama - 1.0;
+    const double v2Roe = uRoe * uRoe + vRoe * vRoe + wRoe * wRoe;
+    const double hRoe = gama / gamm1 * pRoe / rRoe + 0.5 * v2Roe;
+    const double vn = xsn * uRoe + ysn * vRoe + zsn * wRoe - vb;
+    const double theta = vn + vb;
+    const double c2 = gamm1 * (hRoe - 0.5 * v2Roe);
+    const double cm = sqrt(dABS(c2));
+    double e1 = dABS(vn), e2 = dABS(vn + cm), e3 = dABS(vn - cm);
+    const double EL = (1.0 / (gmL - 1.0)) * (pL / rL) + 0.5 * v2L;
+    const double ER = (1.0 / (gmR - 1.0)) * (pR / rR) + 0.5 * v2R;
+    const double dq0 = rR - rL;
+    const double dq1 = rR * uR - rL * uL;
+    const double dq2 = rR * vR - rL * vL;
+    const double dq3 = rR * wR - rL * wL;
+    const double dq4 = rR * ER - rL * EL;
+    if (fixMethod == 1) {
+        const double mx = dMAX(e2, e3);
+        const double t0 = mx * ent, t1 = mx * ent;
+        e1 = dMAX(t0, e1); e2 = dMAX(t1, e2); e3 = dMAX(t1, e3);
+    } else if (fixMethod == 3) {
+        const double me = dABS(vn) + cm;
+        const double al = ent * me;
+        const double cl = ent * me;
+        e1 = harten(e1, cl);
+        e2 = harten(e2, al);
+        e3 = harten(e3, al);
+    } else if (fixMethod == 5) {
+        const double mx = dMAX(e2, e3);
+        const double d = mx * ent;
+        e1 += d; e2 += d; e3 += d;
+    }
+    const double xi1 = (2.0 * e1 - e2 - e3) / (2.0 * c2);
+    const double xi2 = (e2 - e3) / (2.0 * cm);
+    const double dc = theta * dq0 - xsn * dq1 - ysn * dq2 - zsn * dq3;
+    const double c2dc = c2 * dc;
+    const double dh = -gamm1 * (uRoe * dq1 + vRoe * dq2 + wRoe * dq3 - dq4) + 0.5 * gamm1 * v2Roe * dq0;
+    flux[0] -= 0.5 * (e1 * dq0 - dh * xi1 - dc * xi2);
+    flux[1] -= 0.5 * (e1 * dq1 + (xsn * c2dc - uRoe * dh) * xi1 + (xsn * dh - uRoe * dc) * xi2);
+    flux[2] -= 0.5 * (e1 * dq2 + (ysn * c2dc - vRoe * dh) * xi1 + (ysn * dh - vRoe * dc) * xi2);
+    flux[3] -= 0.5 * (e1 * dq3 + (zsn * c2dc - wRoe * dh) * xi1 + (zsn * dh - wRoe * dc) * xi2);
+    flux[4] -= 0.5 * (e1 * dq4 + (theta * c2dc - hRoe * dh) * xi1 + (theta * dh - hRoe * dc) * xi2);
+}
+
+__device__ __forceinline__ double venfun(double x, double y, double eps)
+{
+    const double x2 = x * x;
+    const double xy = x * y;
+    const double y2 = y * y;
+    return ((x2 + eps + 2.0 * xy) / (x2 + 2.0 * y2 + xy + eps));
+}
+
+__device__ __forceinline__ void load5(const double *__restrict__ a, int n, int i, double *v)
+{
+#pragma unroll
+    for (int m = 0; m < 5; ++m) v[m] = a[(size_t)m * n + i];
+}
+
+// ------------------------------------------------------------------ BCs
+__global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K, double *__restrict__ q)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= M.nBFace) return;
+    const int nT = M.nTotal;
+    const int le = M.fle[f], re = M.nCell + f;
+    double ql[5], qr[5];
+    load5(q, nT, le, ql);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qr[m] = ql[m];
+    const FaceRec fr = M.face[f];
+    const int bc = M.bcType[f];
+    const bool symm = (bc == kBC_SYMM) || (bc == kBC_WALL && !K.viscous);
+    if (symm) {
+        const double vn = fr.nx * ql[1] + fr.ny * ql[2] + fr.nz * ql[3] - 0.0;
+        qr[1] = ql[1] - 2.0 * fr.nx * vn;
+        qr[2] = ql[2] - 2.0 * fr.ny * vn;
+        qr[3] = ql[3] - 2.0 * fr.nz * vn;
+    } else if (bc == kBC_WALL) {
+        qr[1] = -ql[1] + 2.0 * 0.0;
+        qr[2] = -ql[2] + 2.0 * 0.0;
+        qr[3] = -ql[3] + 2.0 * 0.0;
+    } else if (bc == kBC_FAR) {
+        const double *pf = K.primInf;
+        const double gl = K.gamma;
+        const double rO = pf[0], uO = pf[1], vO = pf[2], wO = pf[3], pO = pf[4];
+        const double cO = sqrt(dABS(K.gamma * pO / rO));
+        const double rI = ql[0], uI = ql[1], vI = ql[2], wI = ql[3], pI = ql[4];
+        const double nx = fr.nx, ny = fr.ny, nz = fr.nz;
+        const double vnO = nx * uO + ny * vO + nz * wO - 0.0;
+        const double vnI = nx * uI + ny * vI + nz * wI - 0.0;
+        const double cI = sqrt(dABS(gl * pI / rI));
+        const double gamm1 = gl - 1.0;
+        const double velI = sqrt(uI * uI + vI * vI + wI * wI);
+        if (velI > cI) {
+            if (!(vnI >= 0.0)) {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) qr[m] = pf[m];
+            }
+        } else {
+            const double Rp = vnI + 2.0 * cI / gamm1;
+            const double Rm = vnO - 2.0 * cO / gamm1;
+            const double vnB = 0.5 * (Rp + Rm);
+            const double cB = 0.25 * (Rp - Rm) * gamm1;
+            double S, tx, ty, tz;
+            if (vnB >= 0.0) {
+                S = pI / pow(rI, gl);
+                tx = uI - nx * vnI; ty = vI - ny * vnI; tz = wI - nz * vnI;
+            } else {
+                S = pO / pow(rO, gl);
+                tx = uO - nx * vnO; ty = vO - ny * vnO; tz = wO - nz * vnO;
+            }
+            const double rB = pow((cB * cB / (S * gl)), 1.0 / gamm1);
+            qr[0] = rB;
+            qr[1] = tx + nx * vnB;
+            qr[2] = ty + ny * vnB;
+            qr[3] = tz + nz * vnB;
+            qr[4] = cB * cB * rB / gl;
+        }
+    }
+    // extrapolation / outflow: copy (already in qr)
+#pragma unroll
+    for (int m = 0; m < 5; ++m) q[(size_t)m * nT + re] = qr[m];
+}
+
+// Standalone gamma/T/viscosity hook (the fused kernels recompute T and mu).
+__global__ void __launch_bounds__(kBlock) k_gamma_t(DevMesh M, Consts K, const double *__restrict__ q,
+                                                    double *__restrict__ t, double *__restrict__ visl)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= M.nTotal) return;
+    const int nT = M.nTotal;
+    const double tt = q[(size_t)4 * nT + i] / (K.R * q[i]);
+    t[i] = tt;
+    visl[i] = K.viscous ? tt * sqrt(tt) * (1.0 + K.tsuth) / (tt + K.tsuth) : 0.0;
+}
+
+// ------------------------------------------------ gradients + limiter
+__global__ void __launch_bounds__(kBlock) k_grad_lim(DevMesh M, Consts K, const double *__restrict__ q,
+                                                     double *__restrict__ grad, double *__restrict__ gradT,
+                                                     double *__restrict__ lim)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int nC = M.nCell, nT = M.nTotal;
+    if (c >= nC) return;
+    double qc[5];
+    load5(q, nT, c, qc);
+    double gx[5], gy[5], gz[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) { gx[m] = 0.0; gy[m] = 0.0; gz[m] = 0.0; }
+    double tgx = 0.0, tgy = 0.0, tgz = 0.0;
+    const double tc = qc[4] / (K.R * qc[0]);
+    double rmin = qc[0], rmax = qc[0], pmin = qc[4], pmax = qc[4];
+    for (int s = 0; s < M.ellW; ++s) {
+        const int enc = M.c2fEnc[(size_t)s * nC + c];
+        if (enc == 0) break;
+        const bool isL = enc > 0;
+        const int f = (isL ? enc : -enc) - 1;
+        const int nb = M.c2n[(size_t)s * nC + c];
+        const FaceRec fr = M.face[f];
+        const double nx = fr.nx * fr.area, ny = fr.ny * fr.area, nz = fr.nz * fr.area;
+        double qn[5];
+        load5(q, nT, nb, qn);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const double qfc = 0.5 * (qc[m] + qn[m]);
+            if (isL) { gx[m] += qfc * nx; gy[m] += qfc * ny; gz[m] += qfc * nz; }
+            else     { gx[m] -= qfc * nx; gy[m] -= qfc * ny; gz[m] -= qfc * nz; }
+        }
+        if (K.viscous) {
+            const double tn = qn[4] / (K.R * qn[0]);
+            const double tfc = 0.5 * (tc + tn);
+            if (isL) { tgx += tfc * nx; tgy += tfc * ny; tgz += tfc * nz; }
+            else     { tgx -= tfc * nx; tgy -= tfc * ny; tgz -= tfc * nz; }
+        }
+        if (nb < nC) {
+            rmin = dMIN(rmin, qn[0]); rmax = dMAX(rmax, qn[0]);
+            pmin = dMIN(pmin, qn[4]); pmax = dMAX(pmax, qn[4]);
+        }
+    }
+    const double ovol = 1.0 / M.vol[c];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        gx[m] *= ovol; gy[m] *= ovol; gz[m] *= ovol;
+        grad[(size_t)m * nC + c] = gx[m];
+        grad[(size_t)(5 + m) * nC + c] = gy[m];
+        grad[(size_t)(10 + m) * nC + c] = gz[m];
+    }
+    if (K.viscous) {
+        tgx *= ovol; tgy *= ovol; tgz *= ovol;
+        gradT[c] = tgx; gradT[(size_t)nC + c] = tgy; gradT[(size_t)2 * nC + c] = tgz;
+    }
+    double l;
+    if (K.limiter == 1) {
+        l = 1.0;
+    } else if (K.limiter == 2) {
+        l = 0.0;
+    } else {
+        const double dminR = rmin - qc[0], dmaxR = rmax - qc[0];
+        const double dminP = pmin - qc[4], dmaxP = pmax - qc[4];
+        const double vf = M.venkatVf[c];
+        double epsR = K.epsTmp * qc[0] * qc[0];
+        epsR *= vf;
+        double epsP = K.epsTmp * qc[4] * qc[4];
+        epsP *= vf;
+        l = 1.0;
+        const double xc = M.xcc[c], yc = M.ycc[c], zc = M.zcc[c];
+        for (int s = 0; s < M.ellW; ++s) {
+            const int enc = M.c2fEnc[(size_t)s * nC + c];
+            if (enc == 0) break;
+            const int f = (enc > 0 ? enc : -enc) - 1;
+            const FaceRec fr = M.face[f];
+            const double dx = fr.xc - xc, dy = fr.yc - yc, dz = fr.zc - zc;
+            const double dqR = gx[0] * dx + gy[0] * dy + gz[0] * dz;
+            if (dqR > kSMALL) l = dMIN(l, venfun(dmaxR, dqR, epsR));
+            else if (dqR < -kSMALL) l = dMIN(l, venfun(dminR, dqR, epsR));
+            const double dqP = gx[4] * dx + gy[4] * dy + gz[4] * dz;
+            if (dqP > kSMALL) l = dMIN(l, venfun(dmaxP, dqP, epsP));
+            else if (dqP < -kSMALL) l = dMIN(l, venfun(dminP, dqP, epsP));
+        }
+        if (M.limZero[c]) l = 0.0;
+    }
+    lim[c] = l;
+}
+
+// ------------------------------------------------- face fluxes (gather)
+struct CellSide {
+    double q[5], g[15], lim, x, y, z;
+};
+
+__device__ __forceinline__ void load_side(const DevMesh &M, const double *__restrict__ q,
+                                          const double *__restrict__ grad, const double *__restrict__ lim,
+                                          int i, bool needGrad, CellSide &s)
+{
+    const int nT = M.nTotal, nC = M.nCell;
+    load5(q, nT, i, s.q);
+    s.x = M.xcc[i]; s.y = M.ycc[i]; s.z = M.zcc[i];
+    s.lim = lim[i];
+    if (needGrad && i < nC) {
+#pragma unroll
+        for (int m = 0; m < 15; ++m) s.g[m] = grad[(size_t)m * nC + i];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 15; ++m) s.g[m] = 0.0;
+    }
+}
+
+__device__ __forceinline__ void reconstruct(const CellSide &s, double lmin, const FaceRec &fr, double *out)
+{
+    double qt[5];
+    const double dx = fr.xc - s.x, dy = fr.yc - s.y, dz = fr.zc - s.z;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qt[m] = s.q[m];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qt[m] += lmin * (s.g[m] * dx + s.g[5 + m] * dy + s.g[10 + m] * dz);
+    const bool ok = !(qt[0] <= 0.0 || qt[4] <= 0.0);
+#pragma unroll
+    for (int m = 0; m < 5; ++m) out[m] = ok ? qt[m] : s.q[m];
+}
+
+// Viscous face flux (flux = -(1/Re) * A * fvis), one face, both sides given.
+__device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const FaceRec &fr,
+                             const CellSide &L, const CellSide &R, const double *gTL, const double *gTR,
+                             bool bnd, bool wall, double *fl)
+{
+    const double two3rd = 2.0 / 3.0;
+    const double tLc = L.q[4] / (K.R * L.q[0]);
+    const double tRc = R.q[4] / (K.R * R.q[0]);
+    const double vlL = tLc * sqrt(tLc) * (1.0 + K.tsuth) / (tLc + K.tsuth);
+    const double vlR = tRc * sqrt(tRc) * (1.0 + K.tsuth) / (tRc + K.tsuth);
+    // FaceWeight (Geo_UnstructGrid.cpp:2420)
+    const double dxL0 = fr.xc - L.x, dyL0 = fr.yc - L.y, dzL0 = fr.zc - L.z;
+    const double dxR0 = fr.xc - R.x, dyR0 = fr.yc - R.y, dzR0 = fr.zc - R.z;
+    const double d1 = sqrt(dxL0 * dxL0 + dyL0 * dyL0 + dzL0 * dzL0);
+    const double d2 = sqrt(dxR0 * dxR0 + dyR0 * dyR0 + dzR0 * dzR0);
+    const double dd = 1.0 / (d1 + d2 + kSMALL);
+    double wL = d2 * dd, wR = d1 * dd;
+    if (bnd) { wL = 1.0; wR = 0.0; }
+    // GetVisFaceValue (:7126)
+    const double mul = 0.5 * (vlL + vlR);
+    const double mut = 0.5 * (0.0 + 0.0);
+    double primF[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) primF[m] = 0.5 * (L.q[m] + R.q[m]);
+    const double tmF = 0.5 * (tLc + tRc);
+    const double kcp = (mul * K.oprl + mut * K.oprt) * K.cp;
+    if (wall) { primF[1] = 0.0; primF[2] = 0.0; primF[3] = 0.0; }
+    // ComputeVisflux (:12977)
+    const double nxs = fr.nx, nys = fr.ny, nzs = fr.nz;
+    double t1x, t1y, t1z;
+    if (dABS(nxs) > kSMALL) { t1x = nys; t1y = -nxs; t1z = 0.0; }
+    else if (dABS(nys) > kSMALL) { t1x = -nys; t1y = nxs; t1z = 0.0; }
+    else if (dABS(nzs) > kSMALL) { t1x = 0.0; t1y = -nzs; t1z = nys; }
+    else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) fl[m] = 0.0;
+        return;
+    }
+    const double on = 1.0 / sqrt(t1x * t1x + t1y * t1y + t1z * t1z);
+    t1x *= on; t1y *= on; t1z *= on;
+    const double t2x = nys * t1z - nzs * t1y;
+    const double t2y = nzs * t1x - nxs * t1z;
+    const double t2z = nxs * t1y - nys * t1x;
+    const double dxL = L.x - fr.xc, dyL = L.y - fr.yc, dzL = L.z - fr.zc;
+    const double dxR = R.x - fr.xc, dyR = R.y - fr.yc, dzR = R.z - fr.zc;
+    const double dL = nxs * dxL + nys * dyL + nzs * dzL;
+    const double dR = nxs * dxR + nys * dyR + nzs * dzR;
+    const double dxnL = nxs * dL - dxL, dynL = nys * dL - dyL, dznL = nzs * dL - dzL;
+    const double dxnR = nxs * dR - dxR, dynR = nys * dR - dyR, dznR = nzs * dR - dzR;
+    const unsigned flag = M.visFlag[f];
+    double f1[5], f2[5], fM[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) { f1[m] = L.q[m]; f2[m] = R.q[m]; }
+    double tL = tLc, tR = tRc;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) fM[m] = 0.5 * (f1[m] + f2[m]);
+    double tM = 0.5 * (tL + tR);
+    if (wall) {
+        tM = tmF;
+        fM[0] = fM[4] / (K.R * tM);
+    }
+    if (flag & 1u) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            f1[m] += L.g[m] * dxnL + L.g[5 + m] * dynL + L.g[10 + m] * dznL;
+            f2[m] += R.g[m] * dxnR + R.g[5 + m] * dynR + R.g[10 + m] * dznR;
+        }
+        tL += gTL[0] * dxnL + gTL[1] * dynL + gTL[2] * dznL;
+        tR += gTR[0] * dxnR + gTR[1] * dynR + gTR[2] * dznR;
+        if (tL < kSMALL) tL = tLc;
+        if (tR < kSMALL) tR = tRc;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) fM[m] = primF[m];
+        tM = tmF;
+    }
+    double dfdn[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double dtdn = 0.0;
+    if ((flag & 2u) && dABS(dL) > kTINY && dABS(dR) > kTINY) {
+        double dfd1[5], dfd2[5];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dfd1[m] = (f1[m] - fM[m]) / dL;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dfd2[m] = (f2[m] - fM[m]) / dR;
+        const double dtd1 = (tL - tM) / dL;
+        const double dtd2 = (tR - tM) / dR;
+        const double dtmp = dL * dL + dR * dR;
+        double wl = dL * dL / dtmp;
+        double wr = 1.0 - wl;
+        if (bnd) { wl = 1.0; wr = 0.0; }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dfdn[m] = wl * dfd1[m] + wr * dfd2[m];
+        dtdn = dtd1 * wl + dtd2 * wr;
+    }
+    double dfdx[5], dfdy[5], dfdz[5], dft1[5], dft2[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        dfdx[m] = wL * L.g[m] + wR * R.g[m];
+        dfdy[m] = wL * L.g[5 + m] + wR * R.g[5 + m];
+        dfdz[m] = wL * L.g[10 + m] + wR * R.g[10 + m];
+    }
+#pragma unroll
+    for (int m = 1; m <= 3; ++m) {
+        dft1[m] = t1x * dfdx[m] + t1y * dfdy[m] + t1z * dfdz[m];
+        dft2[m] = t2x * dfdx[m] + t2y * dfdy[m] + t2z * dfdz[m];
+    }
+#pragma unroll
+    for (int m = 1; m <= 3; ++m) {
+        dfdx[m] = nxs * dfdn[m] + t1x * dft1[m] + t2x * dft2[m];
+        dfdy[m] = nys * dfdn[m] + t1y * dft1[m] + t2y * dft2[m];
+        dfdz[m] = nzs * dfdn[m] + t1z * dft1[m] + t2z * dft2[m];
+    }
+    const double dudx = dfdx[1], dudy = dfdy[1], dudz = dfdz[1];
+    const double dvdx = dfdx[2], dvdy = dfdy[2], dvdz = dfdz[2];
+    const double dwdx = dfdx[3], dwdy = dfdy[3], dwdz = dfdz[3];
+    double qNorm = 0.0;
+    qNorm += kcp * dtdn;
+    const double divv2p3 = two3rd * (dudx + dvdy + dwdz);
+    const double vis = mul + mut;
+    const double um = fM[1], vm = fM[2], wm = fM[3];
+    const double txx = vis * (2.0 * dudx - divv2p3);
+    const double tyy = vis * (2.0 * dvdy - divv2p3);
+    const double tzz = vis * (2.0 * dwdz - divv2p3);
+    const double txy = vis * (dudy + dvdx);
+    const double txz = vis * (dudz + dwdx);
+    const double tyz = vis * (dvdz + dwdy);
+    double fv[5];
+    fv[0] = 0.0;
+    fv[1] = nxs * txx + nys * txy + nzs * txz;
+    fv[2] = nxs * txy + nys * tyy + nzs * tyz;
+    fv[3] = nxs * txz + nys * tyz + nzs * tzz;
+    fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + qNorm;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) fl[m] = -K.oRe * fr.area * fv[m];
+}
+
+// One thread per cell: inviscid (and viscous) residual, spectral radii and
+// the per-block partial sums of res^2.  SPEC_ONLY skips the fluxes.
+template <bool SPEC_ONLY>
+__global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const double *__restrict__ q,
+                                                 const double *__restrict__ grad,
+                                                 const double *__restrict__ gradT,
+                                                 const double *__restrict__ lim, double *__restrict__ res,
+                                                 double *__restrict__ specInv, double *__restrict__ specVis,
+                                                 double *__restrict__ normPart)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int c = lb * blockDim.x + threadIdx.x;
+    const int nC = M.nCell, nT = M.nTotal;
+    double acc[5] = {-0.0, -0.0, -0.0, -0.0, -0.0};
+    if (c < nC) {
+        const bool needG = !SPEC_ONLY && (K.limiter != 2 || K.viscous);
+        CellSide own;
+        load_side(M, q, grad, lim, c, needG, own);
+        double sInv = 0.0;
+        for (int s = 0; s < M.ellW; ++s) {
+            const int enc = M.c2fEnc[(size_t)s * nC + c];
+            if (enc == 0) break;
+            const bool isL = enc > 0;
+            const int f = (isL ? enc : -enc) - 1;
+            const int nb = M.c2n[(size_t)s * nC + c];
+            const FaceRec fr = M.face[f];
+            CellSide oth;
+            load_side(M, q, grad, lim, nb, !SPEC_ONLY && K.limiter != 2, oth);
+            const CellSide &L = isL ? own : oth;
+            const CellSide &R = isL ? oth : own;
+            if (!SPEC_ONLY) {
+                double qL[5], qR[5];
+                if (K.limiter != 2) {
+                    const double lmin = dMIN(L.lim, R.lim);
+                    reconstruct(L, lmin, fr, qL);
+                    reconstruct(R, lmin, fr, qR);
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) { qL[m] = L.q[m]; qR[m] = R.q[m]; }
+                }
+                if (f < M.nBFace) {
+                    const int bc = M.bcType[f];
+                    if (bc == kBC_WALL || bc == kBC_SYMM) {
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) {
+                            const double tmp = 0.5 * (L.q[m] + R.q[m]);
+                            qL[m] = tmp; qR[m] = tmp;
+                        }
+                        if (bc == kBC_SYMM) {
+                            const double vn = 2.0 * (fr.nx * qL[1] + fr.ny * qL[2] + fr.nz * qL[3] - 0.0);
+                            qR[0] = qL[0];
+                            qR[1] = qL[1] - vn * fr.nx;
+                            qR[2] = qL[2] - vn * fr.ny;
+                            qR[3] = qL[3] - vn * fr.nz;
+                            qR[4] = qL[4];
+                        }
+                        if (bc == kBC_WALL && K.viscous) {
+                            qL[1] = 0.0; qL[2] = 0.0; qL[3] = 0.0;
+                            qR[1] = 0.0; qR[2] = 0.0; qR[3] = 0.0;
+                        }
+                    }
+                }
+                double fl[5];
+                roe_flux(qL, qR, fr.nx, fr.ny, fr.nz, K.gamma, K.entropyFix, K.entropyCoef, fl);
+#pragma unroll
+                for (int m = 0; m < 5; ++m) fl[m] *= fr.area;
+                if (isL) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) acc[m] += fl[m];
+                }
+            }
+            // inviscid spectral radius at the face-averaged state
+            const double vnl = fr.nx * L.q[1] + fr.ny * L.q[2] + fr.nz * L.q[3] - 0.0;
+            const double vnr = fr.nx * R.q[1] + fr.ny * R.q[2] + fr.nz * R.q[3] - 0.0;
+            const double g = 0.5 * (K.gamma + K.gamma);
+            const double pm = 0.5 * (L.q[4] + R.q[4]);
+            const double rm = 0.5 * (L.q[0] + R.q[0]);
+            const double vn = 0.5 * (vnl + vnr);
+            const double cm = sqrt(g * pm / rm);
+            sInv += 0.5 * fr.area * (dABS(vn) + cm);
+        }
+        specInv[c] = sInv;
+        if (K.viscous) {
+            double sVis = 0.0;
+            for (int s = 0; s < M.ellW; ++s) {
+                const int enc = M.c2fEnc[(size_t)s * nC + c];
+                if (enc == 0) break;
+                const bool isL = enc > 0;
+                const int f = (isL ? enc : -enc) - 1;
+                const int nb = M.c2n[(size_t)s * nC + c];
+                const FaceRec fr = M.face[f];
+                CellSide oth;
+                load_side(M, q, grad, lim, nb, !SPEC_ONLY, oth);
+                const CellSide &L = isL ? own : oth;
+                const CellSide &R = isL ? oth : own;
+                const bool bnd = f < M.nBFace;
+                if (!SPEC_ONLY) {
+                    const int li = isL ? c : nb, ri = isL ? nb : c;
+                    double gTL[3] = {0.0, 0.0, 0.0}, gTR[3] = {0.0, 0.0, 0.0};
+                    if (li < nC) { gTL[0] = gradT[li]; gTL[1] = gradT[(size_t)nC + li]; gTL[2] = gradT[(size_t)2 * nC + li]; }
+                    if (ri < nC) { gTR[0] = gradT[ri]; gTR[1] = gradT[(size_t)nC + ri]; gTR[2] = gradT[(size_t)2 * nC + ri]; }
+                    const bool wall = bnd && M.bcType[f] == kBC_WALL;
+                    double fl[5];
+                    viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, fl);
+                    if (isL) {
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
+                    } else {
+#pragma unroll
+                        for (int m = 0; m < 5; ++m) acc[m] += fl[m];
+                    }
+                }
+                const double tl = L.q[4] / (K.R * L.q[0]);
+                const double tr = R.q[4] / (K.R * R.q[0]);
+                const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
+                                         tr * sqrt(tr) * (1.0 + K.tsuth) / (tr + K.tsuth));
+                const double vt = 0.5 * (0.0 + 0.0);
+                const double dx = R.x - L.x, dy = R.y - L.y, dz = R.z - L.z;
+                const double ds = dABS(fr.nx * dx + fr.ny * dy + fr.nz * dz);
+                const double rho = 0.5 * (L.q[0] + R.q[0]);
+                const double vis = vl + vt;
+                double sr = 2.0 * vis / (rho * ds * K.reynolds + kSMALL);
+                sr *= 0.5 * fr.area;
+                sVis += sr;
+            }
+            specVis[c] = sVis;
+        }
+        if (!SPEC_ONLY) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) res[(size_t)m * nC + c] = acc[m];
+        }
+    }
+    if (SPEC_ONLY) return;
+    // deterministic block partial sums of res^2
+    __shared__ double red[5][kBlock];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = (c < nC) ? acc[m] * acc[m] : 0.0;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) normPart[(size_t)lb * 5 + m] = red[m][0];
+    }
+}
+
+__global__ void k_norm_final(const double *__restrict__ part, int nPart, double *__restrict__ out)
+{
+    __shared__ double red[5][kBlock];
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nPart; i += blockDim.x) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a[m] += part[(size_t)i * 5 + m];
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = a[m];
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) out[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// ---------------------------------------------------------- time step
+__global__ void k_reset(unsigned long long *dtMinBits, int *negCount)
+{
+    *dtMinBits = 0x7FF0000000000000ull;  // +inf
+    *negCount = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_time_step(DevMesh M, Consts K, const double *__restrict__ q,
+                                                      const double *__restrict__ specInv,
+                                                      const double *__restrict__ specVis, double cfl,
+                                                      double *__restrict__ dtLoc,
+                                                      unsigned long long *__restrict__ dtMinBits)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long bits = 0x7FF0000000000000ull;
+    if (c < M.nCell) {
+        double spec = specInv[c];
+        if (K.viscous) spec += specVis[c];
+        double cc = cfl;
+        const double p = q[(size_t)4 * M.nTotal + c];
+        if (p < K.pLower) cc = K.cflStart;
+        else if (p < K.pUpper) cc = (p - K.pUpper) * (cfl - K.cflStart) / (K.pUpper - K.pLower) + cfl;
+        cc *= M.cflSkew[c];
+        const double dt = cc * M.vol[c] / spec;
+        dtLoc[c] = dt;
+        bits = (unsigned long long)__double_as_longlong(dt);
+    }
+    // wave-level min, then one atomic per wave (min is order independent)
+    for (int off = 32; off > 0; off >>= 1) {
+        const unsigned long long o = __shfl_down(bits, off, 64);
+        bits = (o < bits) ? o : bits;
+    }
+    if ((threadIdx.x & 63) == 0) atomicMin(dtMinBits, bits);
+}
+
+__global__ void __launch_bounds__(kBlock) k_dt_diag(DevMesh M, Consts K, const double *__restrict__ dtLoc,
+                                                    const double *__restrict__ specInv,
+                                                    const double *__restrict__ specVis,
+                                                    const unsigned long long *__restrict__ dtMinBits,
+                                                    double *__restrict__ dt, double *__restrict__ diag)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= M.nCell) return;
+    const double mn = __longlong_as_double((long long)*dtMinBits);
+    double d = dtLoc[c];
+    if (K.ktmax > 0) d = dMIN(d, K.ktmax * mn);
+    d /= M.vol[c];
+    dt[c] = d;
+    double D = 1.0 / d;
+    D += specInv[c];
+    if (K.viscous) D += specVis[c];
+    diag[c] = D;
+}
+
+// --------------------------------------------------------------- LU-SGS
+__device__ void mxdq(const double *prim, double nxs, double nys, double nzs, double area, double gama,
+                     const double *dq, double radius, double *f)
+{
+    const double vgn = 0.0;
+    const double rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
+    const double c2 = dABS(gama * pm / rm);
+    const double cm = sqrt(c2);
+    const double cits = nxs * um + nys * vm + nzs * wm - vgn;
+    double lmd1 = cits, lmd2 = cits + cm, lmd3 = cits - cm;
+    const double ipnrad = -1.0 * radius / dMAX(kSMALL, area);
+    lmd1 = 0.5 * (lmd1 + ipnrad);
+    lmd2 = 0.5 * (lmd2 + ipnrad);
+    lmd3 = 0.5 * (lmd3 + ipnrad);
+    const double x1 = (lmd1 + lmd1 - lmd2 - lmd3) / (c2 + c2);
+    const double x2 = (lmd2 - lmd3) / (cm + cm);
+    const double dc = (cits + vgn) * dq[0] - nxs * dq[1] - nys * dq[2] - nzs * dq[3];
+    const double c2dc = c2 * dc;
+    const double v2 = um * um + vm * vm + wm * wm;
+    const double gama1 = gama - 1.0;
+    const double cc1 = 0.5 * gama1 * v2;
+    double dh = -gama1 * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
+    const double enthalpy = (gama / gama1) * (pm / rm);
+    dh += cc1 * dq[0];
+    const double hm = enthalpy + 0.5 * v2;
+    const double a = dh * x1 + dc * x2;
+    const double b = c2dc * x1 + dh * x2;
+    f[0] = lmd1 * dq[0] - a;
+    f[1] = lmd1 * dq[1] - um * a + nxs * b;
+    f[2] = lmd1 * dq[2] - vm * a + nys * b;
+    f[3] = lmd1 * dq[3] - wm * a + nzs * b;
+    f[4] = lmd1 * dq[4] - hm * a + (cits + vgn) * b;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) f[m] *= area;
+}
+
+__device__ __forceinline__ double lusgs_rad(const double *ql, const double *qr, double nx, double ny,
+                                            double nz, double area, double gama)
+{
+    const double gl = gama, gr = gama;
+    const double hl = gl / (gl - 1.0) * ql[4] / ql[0] + 0.5 * (ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3]);
+    const double hr = gr / (gr - 1.0) * qr[4] / qr[0] + 0.5 * (qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+    const double t0 = sqrt(qr[0] / ql[0]);
+    const double t1 = 1.0 / (1.0 + t0);
+    const double rm = sqrt(qr[0] * ql[0]);
+    const double um = (ql[1] + qr[1] * t0) * t1;
+    const double vm = (ql[2] + qr[2] * t0) * t1;
+    const double wm = (ql[3] + qr[3] * t0) * t1;
+    const double hm = (hl + hr * t0) * t1;
+    const double pm = rm * (hm - 0.5 * (um * um + vm * vm + wm * wm)) * (gl - 1.0) / gl;
+    const double vn = nx * um + ny * vm + nz * wm - 0.0;
+    const double eig = dABS(vn) + sqrt(gama * pm / rm);
+    return eig * area;
+}
+
+__device__ __forceinline__ void sweep_cell(const DevMesh &M, const Consts &K, int c, bool forward,
+                                           bool firstSweep, const double *__restrict__ q,
+                                           const double *__restrict__ res, const double *__restrict__ diag,
+                                           double *dq, double *dflux)
+{
+    const int nC = M.nCell, nT = M.nTotal;
+    double qc[5];
+    load5(q, nT, c, qc);
+    double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < M.ellW; ++s) {
+        const int enc = M.c2fEnc[(size_t)s * nC + c];
+        if (enc == 0) break;
+        const int nb = M.c2n[(size_t)s * nC + c];
+        if (forward ? (nb > c) : (nb < c)) continue;
+        const bool isL = enc > 0;
+        const int f = (isL ? enc : -enc) - 1;
+        const FaceRec fr = M.face[f];
+        double nx = fr.nx, ny = fr.ny, nz = fr.nz;
+        if (!isL) { nx = -nx; ny = -ny; nz = -nz; }
+        double qn[5], dqn[5], df[5];
+        load5(q, nT, nb, qn);
+        load5(dq, nT, nb, dqn);
+        const double rad = lusgs_rad(qc, qn, nx, ny, nz, fr.area, K.gamma);
+        mxdq(qn, nx, ny, nz, fr.area, K.gamma, dqn, rad, df);
+        if (K.viscous) {
+            const double dist = dABS(fr.nx * (M.xcc[nb] - M.xcc[c]) + fr.ny * (M.ycc[nb] - M.ycc[c]) +
+                                     fr.nz * (M.zcc[nb] - M.zcc[c]));
+            const double tl = qc[4] / (K.R * qc[0]);
+            const double tr = qn[4] / (K.R * qn[0]);
+            const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
+                                     tr * sqrt(tr) * (1.0 + K.tsuth) / (tr + K.tsuth));
+            const double vt = 0.5 * (0.0 + 0.0);
+            const double rho = 0.5 * (qc[0] + qn[0]);
+            const double vis = vl + vt;
+            double sr = 2.0 * vis / (rho * dist * K.reynolds + kSMALL);
+            sr *= 0.5 * fr.area;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) df[m] -= sr * dqn[m];
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) rhs0[m] += df[m];
+    }
+    const double D = diag[c];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double U = (forward && firstSweep) ? 0.0 : dflux[(size_t)m * nC + c];
+        const double r = res[(size_t)m * nC + c];
+        dq[(size_t)m * nT + c] = (r - U - rhs0[m]) / D;
+        dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sweep_level(DevMesh M, Consts K, const int *__restrict__ cells,
+                                                        int begin, int end, bool forward, bool firstSweep,
+                                                        const double *__restrict__ q,
+                                                        const double *__restrict__ res,
+                                                        const double *__restrict__ diag,
+                                                        double *dq, double *dflux)
+{
+    const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= end) return;
+    sweep_cell(M, K, cells[i], forward, firstSweep, q, res, diag, dq, dflux);
+}
+
+// One workgroup walks a run of small levels; the barrier between levels
+// makes the previous level's dq visible to the whole workgroup.
+__global__ void __launch_bounds__(1024) k_sweep_block(DevMesh M, Consts K, const int *__restrict__ ptr,
+                                                      const int *__restrict__ cells, int l0, int l1,
+                                                      bool forward, bool firstSweep,
+                                                      const double *__restrict__ q,
+                                                      const double *__restrict__ res,
+                                                      const double *__restrict__ diag, double *dq,
+                                                      double *__restrict__ dflux)
+{
+    for (int L = l0; L < l1; ++L) {
+        const int b = ptr[L], e = ptr[L + 1];
+        for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x)
+            sweep_cell(M, K, cells[i], forward, firstSweep, q, res, diag, dq, dflux);
+        __syncthreads();
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_ghost_dq(DevMesh M, Consts K, const double *__restrict__ q,
+                                                     double *__restrict__ dq)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= M.nBFace) return;
+    const int nT = M.nTotal;
+    const int le = M.fle[f], re = M.nCell + f;
+    const int bc = M.bcType[f];
+    double dl[5], dr[5];
+    load5(dq, nT, le, dl);
+    if (bc == kBC_SYMM || (bc == kBC_WALL && !K.viscous)) {
+        double prim[5], Q[5];
+        load5(q, nT, le, prim);
+        const double g = K.gamma;
+        const double v2 = prim[1] * prim[1] + prim[2] * prim[2] + prim[3] * prim[3];
+        const double em = (prim[4] / prim[0]) / (g - 1.0) + 0.5 * v2;
+        Q[0] = prim[0]; Q[1] = prim[0] * prim[1]; Q[2] = prim[0] * prim[2]; Q[3] = prim[0] * prim[3];
+        Q[4] = prim[0] * em;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) Q[m] += dl[m];
+        double qr[5];
+        load5(q, nT, re, qr);
+        const FaceRec fr = M.face[f];
+        const double nx = fr.nx, ny = fr.ny, nz = fr.nz;
+        dr[0] = dl[0];
+        double rvn = 2.0 * (nx * dl[1] + ny * dl[2] + nz * dl[3]);
+        dr[1] = dl[1] - nx * rvn;
+        dr[2] = dl[2] - ny * rvn;
+        dr[3] = dl[3] - nz * rvn;
+        rvn = 2.0 * (nx * Q[1] + ny * Q[2] + nz * Q[3]);
+        const double ru = Q[1] - nx * rvn, rv = Q[2] - ny * rvn, rw = Q[3] - nz * rvn;
+        const double pnew = Q[4] - 0.5 * (Q[1] * Q[1] + Q[2] * Q[2] + Q[3] * Q[3]) / Q[0];
+        const double gam1 = g - 1.0;
+        dr[4] = pnew - qr[4] / gam1 +
+                0.5 * ((ru * ru + rv * rv + rw * rw) / Q[0] - qr[0] * (qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]));
+    } else if (bc == kBC_WALL) {
+        dr[0] = dl[0];
+        dr[1] = -dl[1] + 2.0 * 0.0 * dl[0];
+        dr[2] = -dl[2] + 2.0 * 0.0 * dl[0];
+        dr[3] = -dl[3] + 2.0 * 0.0 * dl[0];
+        dr[4] = dl[4] + 0.0 * (dr[1] - dl[1]) + 0.0 * (dr[2] - dl[2]) + 0.0 * (dr[3] - dl[3]);
+    } else if (bc == kBC_EXTRAP || bc == kBC_OUTFLOW) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dr[m] = dl[m];
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dr[m] = 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) dq[(size_t)m * nT + re] = dr[m];
+}
+
+// ------------------------------------------------------------- update
+__global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K, const double *__restrict__ q,
+                                                   const double *__restrict__ dq, double *__restrict__ qNew,
+                                                   unsigned char *__restrict__ negFlag, int *__restrict__ negCount)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nT = M.nTotal;
+    if (i >= nT) return;
+    double prim[5];
+    load5(q, nT, i, prim);
+    if (i < M.nCell) {
+        const double g = K.gamma;
+        const double v2 = prim[1] * prim[1] + prim[2] * prim[2] + prim[3] * prim[3];
+        const double em = (prim[4] / prim[0]) / (g - 1.0) + 0.5 * v2;
+        double Q[5];
+        Q[0] = prim[0]; Q[1] = prim[0] * prim[1]; Q[2] = prim[0] * prim[2]; Q[3] = prim[0] * prim[3];
+        Q[4] = prim[0] * em;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) Q[m] += dq[(size_t)m * nT + i];
+        if (Q[0] <= 0.0) {
+            prim[0] = Q[0];
+        } else {
+            const double od = 1.0 / Q[0];
+            const double um = Q[1] * od, vm = Q[2] * od, wm = Q[3] * od;
+            const double ie = Q[4] - 0.5 * Q[0] * (um * um + vm * vm + wm * wm);
+            prim[0] = Q[0]; prim[1] = um; prim[2] = vm; prim[3] = wm;
+            prim[4] = (g - 1.0) * ie;
+        }
+        if (prim[0] < K.densLimit || prim[4] < K.presLimit) {
+            negFlag[i] = 1;
+            atomicAdd(negCount, 1);
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) qNew[(size_t)m * nT + i] = prim[m];
+}
+
+// Sequential SolutionFix over the flagged cells in increasing index: a
+// neighbour below the cell is read from the updated field, one above from
+// the old field, exactly as the reference's in-place loop sees them.
+__global__ void k_fix(DevMesh M, Consts K, const double *__restrict__ qOld, double *__restrict__ qNew,
+                      unsigned char *__restrict__ negFlag, const int *__restrict__ negCount)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (*negCount == 0) return;
+    const int nC = M.nCell, nT = M.nTotal;
+    for (int c = 0; c < nC; ++c) {
+        if (!negFlag[c]) continue;
+        negFlag[c] = 0;
+        double prim[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+        double vs = 0.0;
+        for (int s = 0; s < M.ellW; ++s) {
+            const int enc = M.c2fEnc[(size_t)s * nC + c];
+            if (enc == 0) break;
+            const int nb = M.c2n[(size_t)s * nC + c];
+            const double *src = (nb < c) ? qNew : qOld;
+            const double rn = src[nb], pn = src[(size_t)4 * nT + nb];
+            if (rn < K.densLimit || pn < K.presLimit) continue;
+            vs += 1.0;
+            for (int m = 0; m < 5; ++m) {
+                double v = src[(size_t)m * nT + nb];
+                if (m == 0 || m == 4) v = dABS(v);
+                prim[m] += v * 1.0;
+            }
+        }
+        if (vs > 0.0) {
+            const double o = 1.0 / vs;
+            for (int m = 0; m < 5; ++m) prim[m] *= o;
+        } else {
+            for (int m = 0; m < 5; ++m) prim[m] = K.primInf[m];
+        }
+        for (int m = 0; m < 5; ++m) qNew[(size_t)m * nT + c] = prim[m];
+    }
+}
+
+inline int nblk(int n, int b) { return (n + b - 1) / b; }
+
+}  // namespace
+
+// ------------------------------------------------------------ launchers
+void launch_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    if (m.nBFace == 0) return;
+    hipLaunchKernelGGL(k_boundary, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, k, f.q);
+}
+
+void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gamma_t, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.t, f.visl);
+}
+
+void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_grad_lim, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
+                       f.lim);
+}
+
+int flux_blocks(const DevMesh &m) { return nblk(m.nCell, kBlock); }
+
+void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_flux<false>, dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
+                       f.lim, f.res, f.specInv, f.specVis, f.normPart);
+}
+
+void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_flux<true>, dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
+                       f.lim, f.res, f.specInv, f.specVis, f.normPart);
+}
+
+void launch_norm_final(const DevMesh &m, DevFlow &f, int slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, s, f.normPart, flux_blocks(m),
+                       f.normOut + (size_t)slot * 5);
+}
+
+void launch_time_step(const DevMesh &m, const Consts &k, DevFlow &f, double cfl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, f.dtMinBits, f.negCount);
+    hipLaunchKernelGGL(k_time_step, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.specInv,
+                       f.specVis, cfl, f.dtLoc, f.dtMinBits);
+}
+
+void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dt_diag, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.dtLoc, f.specInv,
+                       f.specVis, f.dtMinBits, f.dt, f.diag);
+}
+
+// Levels of at most kSmallLevel cells are grouped and walked by a single
+// workgroup; larger levels get one launch each.
+static const int kSmallLevel = 4096;
+
+void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,
+                  bool firstSweep, hipStream_t s)
+{
+    int L = 0;
+    while (L < lv.nLevels) {
+        const int sz = lv.hptr[L + 1] - lv.hptr[L];
+        if (sz <= kSmallLevel) {
+            int L1 = L + 1;
+            while (L1 < lv.nLevels && lv.hptr[L1 + 1] - lv.hptr[L1] <= kSmallLevel) ++L1;
+            hipLaunchKernelGGL(k_sweep_block, dim3(1), dim3(1024), 0, s, m, k, lv.ptr, lv.cells, L, L1, forward,
+                               firstSweep, f.q, f.res, f.diag, f.dq, f.dflux);
+            L = L1;
+        } else {
+            hipLaunchKernelGGL(k_sweep_level, dim3(nblk(sz, kBlock)), dim3(kBlock), 0, s, m, k, lv.cells,
+                               lv.hptr[L], lv.hptr[L + 1], forward, firstSweep, f.q, f.res, f.diag, f.dq,
+                               f.dflux);
+            ++L;
+        }
+    }
+}
+
+void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    if (m.nBFace == 0) return;
+    hipLaunchKernelGGL(k_ghost_dq, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.dq);
+}
+
+void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.dq, f.qAlt,
+                       f.negFlag, f.negCount);
+    hipLaunchKernelGGL(k_fix, dim3(1), dim3(1), 0, s, m, k, f.q, f.qAlt, f.negFlag, f.negCount);
+}
+
+}  // namespace phl

@@ -1,0 +1,430 @@
+// phl_mesh.cpp -- synthetic grid generation, face derivation and metrics.
+//
+// Metrics restate the reference's UnstructGrid routines so that the device
+// solver sees the same geometry the reference would compute from the same
+// nodes:
+//   face centroid/normal/area  Geo_UnstructGrid.cpp:5911-6157
+//   cell centroid/volume       Geo_UnstructGrid.cpp:6545-6820
+//   cell skewness              Geo_UnstructGrid.cpp:2206-2300
+//   mean volume                API/Physics/src/Region.cpp:3203
+#include "phl_mesh.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <numeric>
+
+namespace phl {
+
+namespace {
+const double kSmall = 1.0e-40;
+const double kTiny = 1.0e-40;
+const double kPi = 3.141592653589793;
+
+inline double dmin(double a, double b) { return (a < b) ? a : b; }
+}  // namespace
+
+void Mesh::computeMetrics()
+{
+    const int nT = nTotal();
+    xfc.assign(nFace, 0.0); yfc.assign(nFace, 0.0); zfc.assign(nFace, 0.0);
+    xfn.assign(nFace, 0.0); yfn.assign(nFace, 0.0); zfn.assign(nFace, 0.0);
+    area.assign(nFace, 0.0);
+    xcc.assign(nT, 0.0); ycc.assign(nT, 0.0); zcc.assign(nT, 0.0); vol.assign(nT, 0.0);
+    skew.assign(nCell, 0.0);
+
+    // Face centroid: node average, then four area-weighted refinements,
+    // then normal and area from the triangle fan about the centroid.
+    for (int f = 0; f < nFace; ++f) {
+        const int b = faceNodePtr[f], nn = faceNodePtr[f + 1] - b;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int j = 0; j < nn; ++j) {
+            const int n1 = faceNodes[b + j];
+            sx += x[n1]; sy += y[n1]; sz += z[n1];
+        }
+        const double coef = 1.0 / nn;
+        xfc[f] = sx * coef; yfc[f] = sy * coef; zfc[f] = sz * coef;
+    }
+    for (int cyc = 0; cyc <= 4; ++cyc) {
+        const bool last = (cyc == 4);
+        for (int f = 0; f < nFace; ++f) {
+            const int b = faceNodePtr[f], nn = faceNodePtr[f + 1] - b;
+            double xt = 0.0, yt = 0.0, zt = 0.0, at = 0.0, sx = 0.0, sy = 0.0, sz = 0.0;
+            const double cx = xfc[f], cy = yfc[f], cz = zfc[f];
+            for (int j = 0; j < nn; ++j) {
+                const int p1 = faceNodes[b + j];
+                const int p2 = faceNodes[b + (j + 1) % nn];
+                const double dx1 = x[p1] - cx, dy1 = y[p1] - cy, dz1 = z[p1] - cz;
+                const double dx2 = x[p2] - cx, dy2 = y[p2] - cy, dz2 = z[p2] - cz;
+                const double anx = dy1 * dz2 - dy2 * dz1;
+                const double any = dz1 * dx2 - dz2 * dx1;
+                const double anz = dx1 * dy2 - dx2 * dy1;
+                const double a = std::sqrt(anx * anx + any * any + anz * anz);
+                const double x00 = x[p1] + x[p2] + cx;
+                const double y00 = y[p1] + y[p2] + cy;
+                const double z00 = z[p1] + z[p2] + cz;
+                at += a;
+                xt += a * x00; yt += a * y00; zt += a * z00;
+                sx += anx; sy += any; sz += anz;
+            }
+            if (at > kSmall) {
+                const double a = 1.0 / at / 3.0;
+                xfc[f] = xt * a; yfc[f] = yt * a; zfc[f] = zt * a;
+            }
+            if (last) {
+                const double a = std::sqrt(sx * sx + sy * sy + sz * sz);
+                area[f] = a * 0.5;
+                const double oa = 1.0 / (a + kSmall);
+                xfn[f] = sx * oa; yfn[f] = sy * oa; zfn[f] = sz * oa;
+            }
+        }
+    }
+
+    // Cell centroid: node average, then the pyramid-volume-weighted centroid
+    // unless it falls outside a face plane (the reference's 'mark' test).
+    for (int c = 0; c < nCell; ++c) {
+        const int b = cellNodePtr[c], nn = cellNodePtr[c + 1] - b;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int j = 0; j < nn; ++j) {
+            const int n2 = cellNodes[b + j];
+            sx += x[n2]; sy += y[n2]; sz += z[n2];
+        }
+        xcc[c] = sx / nn; ycc[c] = sy / nn; zcc[c] = sz / nn;
+    }
+    std::vector<double> xt(nCell, 0.0), yt(nCell, 0.0), zt(nCell, 0.0);
+    for (int f = 0; f < nFace; ++f) {
+        const int b = faceNodePtr[f], nn = faceNodePtr[f + 1] - b;
+        const int le = left[f], re = right[f];
+        const bool interior = f >= nBFace;
+        for (int j = 0; j < nn; ++j) {
+            const int p2 = faceNodes[b + j];
+            const int p3 = faceNodes[b + (j + 1) % nn];
+            const double x21 = x[p2] - xfc[f], y21 = y[p2] - yfc[f], z21 = z[p2] - zfc[f];
+            const double x31 = x[p3] - xfc[f], y31 = y[p3] - yfc[f], z31 = z[p3] - zfc[f];
+            const double nx = y21 * z31 - y31 * z21;
+            const double ny = z21 * x31 - z31 * x21;
+            const double nz = x21 * y31 - x31 * y21;
+            double x41 = xcc[le] - xfc[f], y41 = ycc[le] - yfc[f], z41 = zcc[le] - zfc[f];
+            double xc = xfc[f] + x[p2] + x[p3] + xcc[le];
+            double yc = yfc[f] + y[p2] + y[p3] + ycc[le];
+            double zc = zfc[f] + z[p2] + z[p3] + zcc[le];
+            double tmp = nx * x41 + ny * y41 + nz * z41;
+            tmp = -tmp;
+            xc *= tmp; yc *= tmp; zc *= tmp;
+            xt[le] += xc; yt[le] += yc; zt[le] += zc; vol[le] += tmp;
+            if (interior) {
+                x41 = xcc[re] - xfc[f]; y41 = ycc[re] - yfc[f]; z41 = zcc[re] - zfc[f];
+                xc = xfc[f] + x[p2] + x[p3] + xcc[re];
+                yc = yfc[f] + y[p2] + y[p3] + ycc[re];
+                zc = zfc[f] + z[p2] + z[p3] + zcc[re];
+                tmp = nx * x41 + ny * y41 + nz * z41;
+                xc *= tmp; yc *= tmp; zc *= tmp;
+                xt[re] += xc; yt[re] += yc; zt[re] += zc; vol[re] += tmp;
+            }
+        }
+    }
+    for (int c = 0; c < nCell; ++c) {
+        const double tmp = 1.0 / (4.0 * vol[c] + kSmall);
+        xt[c] *= tmp; yt[c] *= tmp; zt[c] *= tmp;
+        vol[c] /= 6.0;
+    }
+    std::vector<char> mark(nCell, 1);
+    for (int f = 0; f < nFace; ++f) {
+        const int le = left[f];
+        double dx = xt[le] - xfc[f], dy = yt[le] - yfc[f], dz = zt[le] - zfc[f];
+        double tmp = dx * xfn[f] + dy * yfn[f] + dz * zfn[f];
+        if (tmp > 0.0) mark[le] = 0;
+        if (f >= nBFace) {
+            const int re = right[f];
+            dx = xt[re] - xfc[f]; dy = yt[re] - yfc[f]; dz = zt[re] - zfc[f];
+            tmp = dx * xfn[f] + dy * yfn[f] + dz * zfn[f];
+            if (tmp < 0.0) mark[re] = 0;
+        }
+    }
+    for (int c = 0; c < nCell; ++c)
+        if (mark[c]) { xcc[c] = xt[c]; ycc[c] = yt[c]; zcc[c] = zt[c]; }
+    for (int f = 0; f < nBFace; ++f) {
+        const int le = left[f], re = nCell + f;
+        if (area[f] > kSmall) {
+            const double tmp = 2.0 * ((xcc[le] - xfc[f]) * xfn[f] + (ycc[le] - yfc[f]) * yfn[f] +
+                                      (zcc[le] - zfc[f]) * zfn[f]);
+            xcc[re] = xcc[le] - xfn[f] * tmp;
+            ycc[re] = ycc[le] - yfn[f] * tmp;
+            zcc[re] = zcc[le] - zfn[f] * tmp;
+        } else {
+            xcc[re] = -xcc[le] + 2.0 * xfc[f];
+            ycc[re] = -ycc[le] + 2.0 * yfc[f];
+            zcc[re] = -zcc[le] + 2.0 * zfc[f];
+        }
+        vol[re] = vol[le];
+    }
+
+    // Skewness: minimum complement angle between the face normal and the
+    // face-to-centroid lines, interior faces first, then boundary faces.
+    for (int c = 0; c < nCell; ++c) skew[c] = 180.0;
+    for (int f = nBFace; f < nFace; ++f) {
+        if (area[f] <= kTiny) continue;
+        const int le = left[f], re = right[f];
+        const double x1 = xfc[f] - xcc[le], y1 = yfc[f] - ycc[le], z1 = zfc[f] - zcc[le];
+        const double d1 = std::sqrt(x1 * x1 + y1 * y1 + z1 * z1);
+        double p1 = (xfn[f] * x1 + yfn[f] * y1 + zfn[f] * z1) / (d1 + kTiny);
+        if (p1 > 1.0) p1 = 1.0;
+        if (p1 < -1.0) p1 = -1.0;
+        const double x2 = xcc[re] - xfc[f], y2 = ycc[re] - yfc[f], z2 = zcc[re] - zfc[f];
+        const double d2 = std::sqrt(x2 * x2 + y2 * y2 + z2 * z2);
+        double p2 = (xfn[f] * x2 + yfn[f] * y2 + zfn[f] * z2) / (d2 + kTiny);
+        if (p2 > 1.0) p2 = 1.0;
+        if (p2 < -1.0) p2 = -1.0;
+        const double angle = dmin(std::asin(p1), std::asin(p2)) * 180.0 / kPi;
+        skew[le] = dmin(skew[le], angle);
+        skew[re] = dmin(skew[re], angle);
+    }
+    for (int f = 0; f < nBFace; ++f) {
+        if (area[f] <= kTiny) continue;
+        const int le = left[f];
+        const double x1 = xfc[f] - xcc[le], y1 = yfc[f] - ycc[le], z1 = zfc[f] - zcc[le];
+        const double d1 = std::sqrt(x1 * x1 + y1 * y1 + z1 * z1);
+        double p1 = (xfn[f] * x1 + yfn[f] * y1 + zfn[f] * z1) / (d1 + kTiny);
+        if (p1 > 1.0) p1 = 1.0;
+        if (p1 < -1.0) p1 = -1.0;
+        const double angle = std::asin(p1) * 180.0 / kPi;
+        skew[le] = dmin(skew[le], angle);
+    }
+
+    double tv = 0.0;
+    for (int c = 0; c < nCell; ++c) tv += vol[c];
+    avgVol = tv / nCell;
+}
+
+namespace {
+
+// Local faces of each element kind (VTK node order), each listed cyclically.
+const int kTetFaces[4][4] = {{0, 1, 2, -1}, {0, 1, 3, -1}, {1, 2, 3, -1}, {0, 2, 3, -1}};
+const int kPyrFaces[5][4] = {{0, 1, 2, 3}, {0, 1, 4, -1}, {1, 2, 4, -1}, {2, 3, 4, -1}, {3, 0, 4, -1}};
+const int kPriFaces[5][4] = {{0, 1, 2, -1}, {3, 4, 5, -1}, {0, 1, 4, 3}, {1, 2, 5, 4}, {2, 0, 3, 5}};
+const int kHexFaces[6][4] = {{0, 1, 2, 3}, {4, 5, 6, 7}, {0, 1, 5, 4},
+                             {1, 2, 6, 5}, {2, 3, 7, 6}, {3, 0, 4, 7}};
+
+int nLocalFaces(int kind) { return kind == TET ? 4 : (kind == HEX ? 6 : 5); }
+const int *localFace(int kind, int i)
+{
+    switch (kind) {
+    case TET: return kTetFaces[i];
+    case PYRAMID: return kPyrFaces[i];
+    case PRISM: return kPriFaces[i];
+    default: return kHexFaces[i];
+    }
+}
+
+struct FaceRef {
+    int key[4];  // sorted node ids, -1 padded (sorts first)
+    int cell;
+    int local;
+};
+
+bool keyLess(const FaceRef &a, const FaceRef &b)
+{
+    for (int i = 0; i < 4; ++i)
+        if (a.key[i] != b.key[i]) return a.key[i] < b.key[i];
+    return a.cell < b.cell;
+}
+bool keyEq(const FaceRef &a, const FaceRef &b)
+{
+    return a.key[0] == b.key[0] && a.key[1] == b.key[1] && a.key[2] == b.key[2] && a.key[3] == b.key[3];
+}
+
+}  // namespace
+
+bool buildMesh(const ElementMesh &em, BcClassifier classify, const void *user, Mesh &out,
+               std::string &err)
+{
+    const int nCell = (int)em.kind.size();
+    const int nNode = (int)em.x.size();
+    if (nCell <= 0 || (int)em.cellNodePtr.size() != nCell + 1) {
+        err = "buildMesh: inconsistent cell arrays";
+        return false;
+    }
+    std::vector<FaceRef> refs;
+    refs.reserve((size_t)nCell * 6);
+    for (int c = 0; c < nCell; ++c) {
+        const int *cn = &em.cellNodes[em.cellNodePtr[c]];
+        for (int lf = 0; lf < nLocalFaces(em.kind[c]); ++lf) {
+            const int *loc = localFace(em.kind[c], lf);
+            FaceRef r;
+            for (int j = 0; j < 4; ++j) r.key[j] = (loc[j] >= 0) ? cn[loc[j]] : -1;
+            std::sort(r.key, r.key + 4);
+            r.cell = c;
+            r.local = lf;
+            refs.push_back(r);
+        }
+    }
+    std::sort(refs.begin(), refs.end(), keyLess);
+
+    struct Face { int le, re, local, localRe, bc; };
+    std::vector<Face> bfaces, ifaces;
+    for (size_t i = 0; i < refs.size();) {
+        if (i + 1 < refs.size() && keyEq(refs[i], refs[i + 1])) {
+            if (i + 2 < refs.size() && keyEq(refs[i], refs[i + 2])) {
+                err = "buildMesh: face shared by more than two cells";
+                return false;
+            }
+            ifaces.push_back({refs[i].cell, refs[i + 1].cell, refs[i].local, refs[i + 1].local, 0});
+            i += 2;
+        } else {
+            bfaces.push_back({refs[i].cell, -1, refs[i].local, -1, 0});
+            i += 1;
+        }
+    }
+
+    // Face node lists taken from the owner's local face, then oriented so the
+    // area vector points away from the owner's node-average centroid.
+    auto faceNodesOf = [&](const Face &fc, std::vector<int> &nodes) {
+        const int *cn = &em.cellNodes[em.cellNodePtr[fc.le]];
+        const int *loc = localFace(em.kind[fc.le], fc.local);
+        nodes.clear();
+        for (int j = 0; j < 4; ++j)
+            if (loc[j] >= 0) nodes.push_back(cn[loc[j]]);
+    };
+    auto centroid = [&](int c, double &cx, double &cy, double &cz) {
+        const int b = em.cellNodePtr[c], nn = em.cellNodePtr[c + 1] - b;
+        cx = cy = cz = 0.0;
+        for (int j = 0; j < nn; ++j) {
+            const int n = em.cellNodes[b + j];
+            cx += em.x[n]; cy += em.y[n]; cz += em.z[n];
+        }
+        cx /= nn; cy /= nn; cz /= nn;
+    };
+    auto orient = [&](int owner, std::vector<int> &nodes, double &fx, double &fy, double &fz) {
+        const int nn = (int)nodes.size();
+        fx = fy = fz = 0.0;
+        for (int j = 0; j < nn; ++j) { fx += em.x[nodes[j]]; fy += em.y[nodes[j]]; fz += em.z[nodes[j]]; }
+        fx /= nn; fy /= nn; fz /= nn;
+        double sx = 0.0, sy = 0.0, sz = 0.0;
+        for (int j = 0; j < nn; ++j) {
+            const int p1 = nodes[j], p2 = nodes[(j + 1) % nn];
+            const double dx1 = em.x[p1] - fx, dy1 = em.y[p1] - fy, dz1 = em.z[p1] - fz;
+            const double dx2 = em.x[p2] - fx, dy2 = em.y[p2] - fy, dz2 = em.z[p2] - fz;
+            sx += dy1 * dz2 - dy2 * dz1;
+            sy += dz1 * dx2 - dz2 * dx1;
+            sz += dx1 * dy2 - dx2 * dy1;
+        }
+        double cx, cy, cz;
+        centroid(owner, cx, cy, cz);
+        if (sx * (fx - cx) + sy * (fy - cy) + sz * (fz - cz) < 0.0) std::reverse(nodes.begin(), nodes.end());
+    };
+
+    std::vector<int> nodes;
+    for (Face &bf : bfaces) {
+        faceNodesOf(bf, nodes);
+        double fx, fy, fz;
+        orient(bf.le, nodes, fx, fy, fz);
+        bf.bc = classify(fx, fy, fz, user);
+    }
+    std::stable_sort(bfaces.begin(), bfaces.end(), [](const Face &a, const Face &b) {
+        if (a.bc != b.bc) return a.bc < b.bc;
+        return a.le < b.le;
+    });
+    for (Face &f : ifaces)
+        if (f.le > f.re) {  // owner = lower-numbered cell
+            std::swap(f.le, f.re);
+            std::swap(f.local, f.localRe);
+        }
+    std::sort(ifaces.begin(), ifaces.end(), [](const Face &a, const Face &b) {
+        if (a.le != b.le) return a.le < b.le;
+        return a.re < b.re;
+    });
+
+    Mesh &m = out;
+    m = Mesh();
+    m.nNode = nNode;
+    m.nCell = nCell;
+    m.nBFace = (int)bfaces.size();
+    m.nFace = m.nBFace + (int)ifaces.size();
+    m.x = em.x; m.y = em.y; m.z = em.z;
+    m.cellNodePtr = em.cellNodePtr;
+    m.cellNodes = em.cellNodes;
+    m.faceNodePtr.assign(1, 0);
+    m.left.resize(m.nFace);
+    m.right.resize(m.nFace);
+    m.bcType.resize(m.nBFace);
+    for (int f = 0; f < m.nFace; ++f) {
+        const bool bnd = f < m.nBFace;
+        Face fc = bnd ? bfaces[f] : ifaces[f - m.nBFace];
+        faceNodesOf(fc, nodes);
+        double fx, fy, fz;
+        orient(fc.le, nodes, fx, fy, fz);
+        for (int n : nodes) m.faceNodes.push_back(n);
+        m.faceNodePtr.push_back((int)m.faceNodes.size());
+        m.left[f] = fc.le;
+        m.right[f] = bnd ? nCell + f : fc.re;
+        if (bnd) m.bcType[f] = fc.bc;
+    }
+    m.computeMetrics();
+    for (int c = 0; c < nCell; ++c) {
+        if (!(m.vol[c] > 0.0)) {
+            err = "buildMesh: non-positive cell volume";
+            return false;
+        }
+    }
+    return true;
+}
+
+namespace {
+struct BumpGeom {
+    double x0, x1, ytop, zmax;
+};
+int classifyBump(double xc, double yc, double zc, const void *user)
+{
+    const BumpGeom *g = (const BumpGeom *)user;
+    const double tol = 1e-9;
+    if (std::fabs(xc - g->x0) < tol || std::fabs(xc - g->x1) < tol) return 4;  // FARFIELD
+    if (std::fabs(zc) < tol || std::fabs(zc - g->zmax) < tol) return 3;        // SYMMETRY
+    if (std::fabs(yc - g->ytop) < tol) return 3;                               // SYMMETRY (top)
+    return 2;                                                                  // SOLID_SURFACE
+}
+}  // namespace
+
+bool makeBumpHex(int nx, int ny, int nz, double bump, Mesh &out, std::string &err)
+{
+    if (nx < 1 || ny < 1 || nz < 1) {
+        err = "makeBumpHex: bad dimensions";
+        return false;
+    }
+    const double x0 = -1.5, x1 = 1.5, ytop = 0.8, zmax = 0.2;
+    ElementMesh em;
+    const int NX = nx + 1, NY = ny + 1, NZ = nz + 1;
+    em.x.resize((size_t)NX * NY * NZ);
+    em.y.resize(em.x.size());
+    em.z.resize(em.x.size());
+    for (int k = 0; k < NZ; ++k)
+        for (int j = 0; j < NY; ++j)
+            for (int i = 0; i < NX; ++i) {
+                const size_t n = (size_t)i + (size_t)NX * ((size_t)j + (size_t)NY * k);
+                const double xx = x0 + (x1 - x0) * i / nx;
+                const double yw = bump * std::exp(-25.0 * xx * xx);
+                em.x[n] = xx;
+                em.y[n] = yw + (ytop - yw) * j / ny;
+                em.z[n] = zmax * k / nz;
+            }
+    const size_t nC = (size_t)nx * ny * nz;
+    em.kind.assign(nC, HEX);
+    em.cellNodePtr.resize(nC + 1);
+    em.cellNodes.resize(nC * 8);
+    auto nid = [&](int i, int j, int k) { return i + NX * (j + NY * k); };
+    for (int k = 0; k < nz; ++k)
+        for (int j = 0; j < ny; ++j)
+            for (int i = 0; i < nx; ++i) {
+                const size_t c = (size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k);
+                int *cn = &em.cellNodes[c * 8];
+                cn[0] = nid(i, j, k);         cn[1] = nid(i + 1, j, k);
+                cn[2] = nid(i + 1, j + 1, k); cn[3] = nid(i, j + 1, k);
+                cn[4] = nid(i, j, k + 1);     cn[5] = nid(i + 1, j, k + 1);
+                cn[6] = nid(i + 1, j + 1, k + 1); cn[7] = nid(i, j + 1, k + 1);
+                em.cellNodePtr[c] = (int)(c * 8);
+            }
+    em.cellNodePtr[nC] = (int)(nC * 8);
+    BumpGeom g{x0, x1, ytop, zmax};
+    return buildMesh(em, classifyBump, &g, out, err);
+}
+
+}  // namespace phl

@@ -1,0 +1,49 @@
+// phl_mesh.h -- host-side unstructured grid for the residual + LU-SGS path.
+//
+// Connectivity follows the reference's UnstructGrid conventions: boundary
+// faces first, ghost cell of boundary face f = nCell + f, face normal from
+// left to right cell (API/Geometry/src/Geo_UnstructGrid.cpp:2007,5911-6157).
+#pragma once
+
+#include <string>
+#include <vector>
+
+namespace phl {
+
+struct Mesh {
+    int nNode = 0, nCell = 0, nFace = 0, nBFace = 0;
+    std::vector<double> x, y, z;
+    std::vector<int> faceNodePtr, faceNodes;  // CSR, nFace+1
+    std::vector<int> cellNodePtr, cellNodes;  // CSR, nCell+1
+    std::vector<int> left, right, bcType;     // bcType sized nBFace
+
+    // Metrics (UnstructGrid::ComputeMetrics).
+    std::vector<double> xfc, yfc, zfc, xfn, yfn, zfn, area;  // [nFace]
+    std::vector<double> xcc, ycc, zcc, vol;                  // [nTotal]
+    std::vector<double> skew;                                // [nCell], degrees
+    double avgVol = 0.0;
+
+    int nTotal() const { return nCell + nBFace; }
+    void computeMetrics();
+};
+
+// One cell of a generic element mesh, before faces are derived.
+enum CellKind { TET = 0, PYRAMID = 1, PRISM = 2, HEX = 3 };
+
+// Derives faces from cells (shared faces matched by node set), orients each
+// face from the lower-numbered to the higher-numbered cell (boundary faces
+// outward), orders boundary faces first, then computes metrics.  classify(f)
+// returns the BC code of a boundary face given its centroid.
+struct ElementMesh {
+    std::vector<double> x, y, z;
+    std::vector<int> kind;       // CellKind per cell
+    std::vector<int> cellNodes;  // 4/5/6/8 nodes per cell, VTK local order
+    std::vector<int> cellNodePtr;
+};
+typedef int (*BcClassifier)(double xc, double yc, double zc, const void *user);
+bool buildMesh(const ElementMesh &em, BcClassifier classify, const void *user, Mesh &out,
+               std::string &err);
+
+bool makeBumpHex(int nx, int ny, int nz, double bump, Mesh &out, std::string &err);
+
+}  // namespace phl
