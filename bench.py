@@ -1,0 +1,151 @@
+"""Benchmark of the residual + LU-SGS path on one MI355X.
+
+python bench.py [--gpus 1] [--steps K] [--warmup W] [--config NAME]
+
+A step is one outer iteration of the LU-SGS branch of Controller::Relaxation
+(boundary conditions, gradients, limiter, fluxes, spectral radius, time step,
+forward/backward sweeps, update) over the whole mesh, with the state already
+resident in HBM.  value = cells * steps / wall seconds (cell-updates/s).
+
+Besides the contract fields the JSON line carries:
+  roofline      the face-flux kernel's algorithmic bytes per launch over its
+                mean duration (HIP events on the solver stream, a second pass
+                of the same steps), against the 8 TB/s HBM peak
+  cpu_baseline  the CPU oracle (sequential FP64 restatement, one core) on a
+                bounded sample of the same workload
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+import phenlei_amd as pa  # noqa: E402
+
+METRIC = "cell-updates/s and time-to-1e-8 residual drop: mixed+warm-start vs plain FP64"
+HBM_PEAK_GBS = 8000.0
+
+# Workloads.  'c1' is the reference's CPU-runnable case (64x32x8 hex bump
+# channel, inviscid, M0.5, CFL 10).
+CONFIGS = {
+    "c1": dict(workload="bump channel, 64x32x8 hex (16,384 cells), inviscid Euler, M0.5, AoA 0, "
+                        "Roe + GG-cell + Venkat(K=5), LU-SGS CFL 10, FP64",
+               mesh=("bump_hex", 64, 32, 8, 0.05),
+               params=dict(mach=0.5, cfl_start=10.0, cfl_end=10.0), cpu_steps=500),
+    "c2": dict(workload="bump channel, hybrid 1,007,616 cells (15 prism layers, first height 1e-4, over a "
+                        "128x16 wall; tet fill of a seeded jittered lattice), laminar NS, M0.3, Re 5e5/unit "
+                        "length, Sutherland, Roe + GG-cell + Venkat(K=5), LU-SGS CFL 10, FP64",
+               mesh=("bump_hybrid", 128, 16, 15, 1e-4, 77, 0.1, 1, 0.05),
+               params=dict(mach=0.3, viscous=1, reynolds=5e5, cfl_start=10.0, cfl_end=10.0), cpu_steps=6),
+    "hex1m": dict(workload="bump channel, 256x64x64 hex (1,048,576 cells), inviscid Euler, M0.5, "
+                           "Roe + GG-cell + Venkat(K=5), LU-SGS CFL 10, FP64",
+                  mesh=("bump_hex", 256, 64, 64, 0.05),
+                  params=dict(mach=0.5, cfl_start=10.0, cfl_end=10.0), cpu_steps=8),
+}
+DEFAULT_CONFIG = "c2"
+
+
+def build_mesh(spec):
+    kind = spec[0]
+    if kind == "bump_hex":
+        return pa.Mesh.bump_hex(*spec[1:])
+    if kind == "bump_hybrid":
+        return pa.Mesh.bump_hybrid(*spec[1:])
+    raise ValueError(kind)
+
+
+def flux_algorithmic_bytes(n_cell, n_face, viscous):
+    """Compulsory HBM bytes of one face-flux launch (FP64): read q 40N, grad
+    120N, limiter 8N, centroids 24N, face centres 24F, normal+area 32F,
+    left/right 8F; write residual 40N and spectral radius 8N."""
+    b = 240 * n_cell + 64 * n_face
+    if viscous:
+        b += 24 * n_cell + 8 * n_cell  # temperature gradient, viscous spectral radius
+    return b
+
+
+def cpu_baseline(mesh, params, steps):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the checker, timed here as the CPU baseline only
+    conn, met = mesh.connectivity(), mesh.metrics()
+    cpu = orc.OracleSolver(conn, met, params)
+    cpu.init_freestream()
+    t0 = time.perf_counter()
+    cpu.iterate(1, steps)
+    dt = time.perf_counter() - t0
+    cpu.close()
+    return dict(value=mesh.n_cell * steps / dt, unit="cell-updates/s", cores=1, kind="port",
+                sample=f"{steps} FP64 iterations of the same mesh from freestream "
+                       f"({mesh.n_cell} cells, {dt:.1f} s, sequential C oracle)")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--config", default=DEFAULT_CONFIG, choices=sorted(CONFIGS))
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
+    ap.add_argument("--copy-probe", action="store_true", help="also run the HBM copy probe")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
+        sys.exit(2)
+
+    cfg = CONFIGS[args.config]
+    print(f"[bench] config {args.config}: building mesh", file=sys.stderr, flush=True)
+    mesh = build_mesh(cfg["mesh"])
+    params = pa.default_params(**cfg["params"])
+    solver = pa.Solver(mesh, params, device=0)
+    solver.init_freestream()
+    print(f"[bench] {mesh.n_cell} cells, {mesh.n_face} faces; warmup {args.warmup}", file=sys.stderr, flush=True)
+    solver.iterate(1, args.warmup, history=False)
+    solver.synchronize()
+
+    t0 = time.perf_counter()
+    solver.iterate(1 + args.warmup, args.steps, history=False)
+    solver.synchronize()
+    wall = time.perf_counter() - t0
+    value = mesh.n_cell * args.steps / wall
+    print(f"[bench] timed {args.steps} steps in {wall:.3f} s", file=sys.stderr, flush=True)
+
+    # roofline of the face-flux kernel: same steps again with HIP events
+    solver.set_kernel_timing(True)
+    solver.iterate(1 + args.warmup + args.steps, args.steps, history=False)
+    times, n_timed = solver.kernel_timing()
+    solver.set_kernel_timing(False)
+    flux_ms = times["flux"] / max(n_timed, 1)
+    fb = flux_algorithmic_bytes(mesh.n_cell, mesh.n_face, params.viscous)
+    achieved = fb / (flux_ms * 1e-3) / 1e9
+    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                kernel="k_flux", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb)
+    per_class = {k: round(v / max(n_timed, 1), 5) for k, v in times.items()}
+    print(f"[bench] mean ms per step by kernel class: {per_class}", file=sys.stderr, flush=True)
+    if args.copy_probe:
+        roof["copy_probe_gbs"] = round(pa.hbm_copy_gbps(0, 2 * 1024**3, 10), 1)
+
+    line = {
+        "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": cfg["workload"], "cells": mesh.n_cell, "faces": mesh.n_face,
+                   "precision": "fp64", "parallelism": "single GPU"},
+        "roofline": roof,
+        "kernel_ms_per_step": per_class,
+    }
+    if not args.no_cpu:
+        print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
+        line["cpu_baseline"] = cpu_baseline(mesh, params, cfg["cpu_steps"])
+    solver.close()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

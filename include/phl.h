@@ -53,6 +53,7 @@ typedef struct phl_params {
     double force_ref_length;/* forceReferenceLength (1.0) */
     double force_ref_area;  /* forceReferenceArea (1.0) */
     double force_ref_x, force_ref_y, force_ref_z; /* TorqueRefX/Y/Z (0) */
+    double force_ref_span;  /* forceReferenceLengthSpanWise (1.0) */
     int viscous;            /* viscousType: 0 inviscid, 1 laminar */
     int gradient;           /* gradientName: 0 "ggcell" */
     int limiter;            /* uns_limiter_name: 0 "vencat" (ivencat=7), 1 "nolim", 2 "1st" */
@@ -97,6 +98,19 @@ phl_status phl_mesh_create(int nNode, const double *x, const double *y, const do
  * z in [0,0.2]; nx*ny*nz hexahedra.  Inlet/outlet FARFIELD, lower wall
  * SOLID_SURFACE, top and sides SYMMETRY. */
 phl_status phl_mesh_bump_hex(int nx, int ny, int nz, double bump, phl_mesh **out);
+/* The same channel as a hybrid grid: n_prism prism layers over the
+ * triangulated wall (first height h1, geometric growth up to the tet row
+ * height) and n_tet rows of a seeded, jittered point lattice split into six
+ * tetrahedra per lattice cell. */
+phl_status phl_mesh_bump_hybrid(int nx, int nz, int n_prism, double h1, int n_tet, double jitter,
+                                unsigned seed, double bump, phl_mesh **out);
+/* Renumber the cells of a mesh into a new mesh.  method 0: greedy colouring
+ * (cells listed colour by colour, each colour in the old order); method 1:
+ * explicit permutation perm[new] = old.  The sweep order of LU-SGS is the cell
+ * order (NSSolverUnstruct.cpp:15211,15541), so this selects the sweep order;
+ * the reference's own renumbering step is API/PreProcess/src/Pre_Reorder.cpp. */
+phl_status phl_mesh_renumber(const phl_mesh *in, int method, const int *perm, int *n_colours,
+                             phl_mesh **out);
 phl_status phl_mesh_sizes(const phl_mesh *m, int *nNode, int *nCell, int *nFace, int *nBFace,
                           int *nFaceNodes, int *nCellNodes);
 phl_status phl_mesh_connectivity(const phl_mesh *m, double *x, double *y, double *z,
@@ -164,6 +178,20 @@ phl_status phl_iterate(phl_ctx *ctx, int first_step, int n_steps, double *rms_hi
 /* Iterate until R_k/R_1 <= drop or max_steps; returns the step count. */
 phl_status phl_solve(phl_ctx *ctx, int max_steps, double drop, int *steps_done, double *rms_hist);
 phl_status phl_synchronize(phl_ctx *ctx);
+
+/* ---------------- measurement (GPU/Manage/src/TimeMeasure.cu:15-67) ----
+ * With timing on, phl_iterate records HIP events on the context's stream
+ * around each kernel class of the iteration.  Classes, in launch order:
+ * 0 boundary, 1 gradient+limiter, 2 face flux (+spectral radius, local time
+ * step), 3 residual norm + minimum time step, 4 dt/diagonal, 5 forward sweep,
+ * 6 ghost dq, 7 backward sweep, 8 update.  ms_total[k] is the summed time of
+ * class k over n_steps[0] timed steps. */
+#define PHL_N_KERNEL_CLASSES 9
+phl_status phl_set_kernel_timing(phl_ctx *ctx, int enable);
+phl_status phl_get_kernel_timing(phl_ctx *ctx, double *ms_total, int *n_steps);
+/* Achievable HBM bandwidth: a streaming copy of `bytes` bytes, best of
+ * `iters` launches, in GB/s (reported beside the vendor peak). */
+phl_status phl_hbm_copy(int device, double bytes, int iters, double *gbps);
 
 #ifdef __cplusplus
 }

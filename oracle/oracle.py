@@ -55,6 +55,8 @@ def load():
         lib.orc_residual_norm.argtypes = [_P, _D]
         lib.orc_iterate.argtypes = [_P, ctypes.c_int, ctypes.c_int, _D]
         lib.orc_iterate.restype = ctypes.c_int
+        lib.orc_forces.argtypes = [_P, _D]
+        lib.orc_force_coefficients.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
         lib.orc_roe_flux.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]
         lib.orc_harten.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -62,6 +64,8 @@ def load():
         lib.orc_venfun.argtypes = [ctypes.c_double] * 3
         lib.orc_venfun.restype = ctypes.c_double
         lib.orc_mxdq.argtypes = [_D] + [ctypes.c_double] * 5 + [_D, ctypes.c_double, ctypes.c_int, _D]
+        lib.orc_lusgs_rad.argtypes = [_D, _D] + [ctypes.c_double] * 5
+        lib.orc_lusgs_rad.restype = ctypes.c_double
         lib.orc_entropy_coef.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.orc_entropy_coef.restype = ctypes.c_double
         lib.orc_ptotal.argtypes = [ctypes.c_double, ctypes.c_double]
@@ -118,6 +122,12 @@ def mxdq(prim, n, area, gamma, dq, radius, ipn):
     return out
 
 
+def lusgs_rad(ql, qr, n, area, gamma=1.4):
+    ql = np.ascontiguousarray(ql, dtype=np.float64)
+    qr = np.ascontiguousarray(qr, dtype=np.float64)
+    return load().orc_lusgs_rad(_dp(ql), _dp(qr), float(n[0]), float(n[1]), float(n[2]), area, gamma)
+
+
 class OracleSolver:
     """Sequential FP64 restatement of NSSolverUnstruct on one zone."""
 
@@ -166,6 +176,12 @@ class OracleSolver:
     def residual_norm(self):
         out = np.empty(5)
         load().orc_residual_norm(self._h, _dp(out))
+        return out
+
+    def forces(self):
+        """AirForceCoefParts + CompForce at the current state (22 values)."""
+        out = np.empty(22)
+        load().orc_forces(self._h, _dp(out))
         return out
 
     def iterate(self, first_step, n_steps):

@@ -1150,6 +1150,12 @@ static double lusgs_rad(const double *ql, double gl, const double *qr, double gr
     return eig * area;
 }
 
+double orc_lusgs_rad(const double *ql, const double *qr, double nx, double ny, double nz,
+                     double area, double gamma)
+{
+    return lusgs_rad(ql, gamma, qr, gamma, nx, ny, nz, area, gamma);
+}
+
 /* One neighbour contribution of SolveLUSGSForward/Backward
  * (NSSolverUnstruct.cpp:15227-15305, 15582-15660). */
 static void lusgs_face(const orc_solver *s, int f, int cell, double *df)
@@ -1424,4 +1430,117 @@ int orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist)
         nNeg += orc_lusgs(s, NULL);
     }
     return nNeg;
+}
+
+/* ------------------------------------------------------------------ */
+/* Wall forces                                                         */
+/* ------------------------------------------------------------------ */
+
+/* CorrectGradient, API/Physics/src/GradientOperation.cpp:21-31. */
+static void correct_gradient(double fl, double fr, double *gx, double *gy, double *gz,
+                             double dx, double dy, double dz, double ods)
+{
+    double fold = *gx * dx + *gy * dy + *gz * dz;
+    double fnew = (fr - fl) * ods;
+    double df = fnew - fold;
+    *gx += df * dx;
+    *gy += df * dy;
+    *gz += df * dz;
+}
+
+/* NSSolverUnstruct::AirForceCoefParts over all solid-surface faces
+ * (CFD/NSSolver/src/NSSolverUnstruct.cpp:16026-16250, partID = ALL_PART)
+ * followed by Force::CompForce (API/Physics/src/Force.cpp:439-538).  The
+ * boundary states, temperature, viscosity and gradients are first brought up
+ * to date with the current q.  out[ORC_N_FORCE] is laid out as phl_forces. */
+void orc_forces(orc_solver *s, double *out)
+{
+    orc_boundary(s);
+    orc_gamma_t(s);
+    if (s->p.viscous) orc_viscosity(s);
+    orc_gradient(s);
+
+    const double poo = s->primInf[IP];
+    const double oRe = 1.0 / s->p.reynolds;
+    const double tx0 = s->p.force_ref_x, ty0 = s->p.force_ref_y, tz0 = s->p.force_ref_z;
+    double CA_f = 0.0, CA_p = 0.0, CN_f = 0.0, CN_p = 0.0, CZ_f = 0.0, CZ_p = 0.0;
+    double Cl_f = 0.0, Cl_p = 0.0, Cn_f = 0.0, Cn_p = 0.0, Cm_f = 0.0, Cm_p = 0.0;
+    double cpx = 0.0, cpy = 0.0, cpz = 0.0;
+
+    for (int f = 0; f < s->nBFace; ++f) {
+        if (s->bcType[f] != ORC_BC_SOLID_SURFACE) continue;
+        int le = s->left[f], re = s->right[f];
+        double nx = s->xfn[f] * s->area[f];
+        double ny = s->yfn[f] * s->area[f];
+        double nz = s->zfn[f] * s->area[f];
+        double cp = 2.0 * (s->q[IP][le] - poo);
+        double dpx = nx * cp, dpy = ny * cp, dpz = nz * cp;
+        CA_p += dpx; CN_p += dpy; CZ_p += dpz;
+        double xc = s->xfc[f], yc = s->yfc[f], zc = s->zfc[f];
+        if (s->p.viscous) {
+            double dudx = s->dqdx[IU][le], dudy = s->dqdy[IU][le], dudz = s->dqdz[IU][le];
+            double dvdx = s->dqdx[IV][le], dvdy = s->dqdy[IV][le], dvdz = s->dqdz[IV][le];
+            double dwdx = s->dqdx[IW][le], dwdy = s->dqdy[IW][le], dwdz = s->dqdz[IW][le];
+            double dx = s->xcc[re] - s->xcc[le];
+            double dy = s->ycc[re] - s->ycc[le];
+            double dz = s->zcc[re] - s->zcc[le];
+            double ods = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+            dx = dx * ods; dy = dy * ods; dz = dz * ods;
+            correct_gradient(s->q[IU][le], s->q[IU][re], &dudx, &dudy, &dudz, dx, dy, dz, ods);
+            correct_gradient(s->q[IV][le], s->q[IV][re], &dvdx, &dvdy, &dvdz, dx, dy, dz, ods);
+            correct_gradient(s->q[IW][le], s->q[IW][re], &dwdx, &dwdy, &dwdz, dx, dy, dz, ods);
+            double vis = s->visl[le];
+            const double c23 = 2.0 / 3.0;
+            double txx = vis * c23 * (2.0 * dudx - dvdy - dwdz);
+            double tyy = vis * c23 * (2.0 * dvdy - dwdz - dudx);
+            double tzz = vis * c23 * (2.0 * dwdz - dudx - dvdy);
+            double txy = vis * (dudy + dvdx);
+            double txz = vis * (dudz + dwdx);
+            double tyz = vis * (dvdz + dwdy);
+            double fvsx = -2.0 * (nx * txx + ny * txy + nz * txz) * oRe;
+            double fvsy = -2.0 * (nx * txy + ny * tyy + nz * tyz) * oRe;
+            double fvsz = -2.0 * (nx * txz + ny * tyz + nz * tzz) * oRe;
+            CA_f += fvsx; CN_f += fvsy; CZ_f += fvsz;
+            Cl_f += (yc - ty0) * fvsz - (zc - tz0) * fvsy;
+            Cn_f += (zc - tz0) * fvsx - (xc - tx0) * fvsz;
+            Cm_f += (xc - tx0) * fvsy - (yc - ty0) * fvsx;
+        }
+        Cl_p += (yc - ty0) * dpz - (zc - tz0) * dpy;
+        Cn_p += (zc - tz0) * dpx - (xc - tx0) * dpz;
+        Cm_p += (xc - tx0) * dpy - (yc - ty0) * dpx;
+        cpx += dpx; cpy += dpy; cpz += dpz;
+    }
+
+    double scf = 1.0 / s->p.force_ref_area;
+    double vcm = 1.0 / (s->p.force_ref_area * s->p.force_ref_length);
+    double sb = 1.0 / (s->p.force_ref_area * s->p.force_ref_span);
+    double raw[15] = {CA_f * scf, CA_p * scf, CN_f * scf, CN_p * scf, CZ_f * scf, CZ_p * scf,
+                      cpx * scf, cpy * scf, cpz * scf,
+                      Cl_f * sb, Cl_p * sb, Cn_f * sb, Cn_p * sb, Cm_f * vcm, Cm_p * vcm};
+    orc_force_coefficients(raw, s->p.alpha_deg, s->p.beta_deg, out);
+}
+
+/* Force::CompForce totals (API/Physics/src/Force.cpp:512-533) from the
+ * scaled part sums raw[0..14]; out[0..14] = raw, out[15..21] = CL, CD,
+ * side force, CD_pr, CD_sf, CD - CL^2/(pi AR) with AR = 9.5, xcp. */
+void orc_force_coefficients(const double *raw, double alpha_deg, double beta_deg, double *out)
+{
+    double attack = alpha_deg * PI_ / 180.0, sideslip = beta_deg * PI_ / 180.0;
+    double sina = sin(attack), cosa = cos(attack), sinb = sin(sideslip), cosb = cos(sideslip);
+    for (int i = 0; i < 15; ++i) out[i] = raw[i];
+    double cfx = raw[0] + raw[1], cfy = raw[2] + raw[3], cfz = raw[4] + raw[5];
+    double cmz = raw[13] + raw[14];
+    double cl = -sina * cfx + cosa * cfy;
+    double cd = cosa * cosb * cfx + sina * cosb * cfy + sinb * cfz;
+    double side = -cosa * sinb * cfx - sina * sinb * cfy + cosb * cfz;
+    double cdp = cosa * cosb * raw[6] + sina * cosb * raw[7] + sinb * raw[8];
+    const double AR = 9.5;
+    out[15] = cl;
+    out[16] = cd;
+    out[17] = side;
+    out[18] = cdp;
+    out[19] = cd - cdp;
+    out[20] = cd - (cl * cl) / (PI_ * AR);
+    /* SIGN(one, cfy), Math_BasisFunction.h:150-155 */
+    out[21] = (cfy < 0.0 ? -1.0 : 1.0) * cmz / (ABS(cfy) + SMALL);
 }

@@ -40,6 +40,7 @@ typedef struct orc_params {
     double force_ref_length;
     double force_ref_area;
     double force_ref_x, force_ref_y, force_ref_z;
+    double force_ref_span;
     int viscous;            /* 0 inviscid, 1 laminar */
     int gradient;           /* 0 ggcell, 1 lsq */
     int limiter;            /* 0 vencat (ivencat=7), 1 nolim, 2 first order */
@@ -108,6 +109,9 @@ void orc_time_step(orc_solver *s, int step);
 int  orc_lusgs(orc_solver *s, double *sweep_normals); /* returns #negative cells fixed */
 void orc_residual_norm(const orc_solver *s, double *sum5);
 int  orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist);
+#define ORC_N_FORCE 22
+void orc_forces(orc_solver *s, double *out);        /* AirForceCoefParts + CompForce */
+void orc_force_coefficients(const double *raw, double alpha_deg, double beta_deg, double *out);
 
 /* ---- point functions, exposed for closed-form tests ---- */
 void   orc_roe_flux(const double *qL, const double *qR, double nx, double ny, double nz,
@@ -116,6 +120,8 @@ double orc_harten(double eig, double limit);
 double orc_venfun(double x, double y, double eps);
 void   orc_mxdq(const double *prim, double nx, double ny, double nz, double area,
                 double gamma, const double *dq, double radius, int ipn, double *f);
+double orc_lusgs_rad(const double *ql, const double *qr, double nx, double ny, double nz,
+                     double area, double gamma);
 double orc_entropy_coef(double mach, double scale);
 double orc_ptotal(double mach, double gamma);
 

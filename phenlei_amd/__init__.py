@@ -42,6 +42,7 @@ class Params(ctypes.Structure):
         ("lusgs_tol", ctypes.c_double), ("force_ref_length", ctypes.c_double),
         ("force_ref_area", ctypes.c_double), ("force_ref_x", ctypes.c_double),
         ("force_ref_y", ctypes.c_double), ("force_ref_z", ctypes.c_double),
+        ("force_ref_span", ctypes.c_double),
         ("viscous", ctypes.c_int), ("gradient", ctypes.c_int), ("limiter", ctypes.c_int),
         ("entropy_fix", ctypes.c_int), ("cfl_vary_step", ctypes.c_int),
         ("n_lusgs_sweeps", ctypes.c_int), ("precision", ctypes.c_int), ("reserved", ctypes.c_int),
@@ -58,7 +59,7 @@ def default_params(**overrides):
     p.reynolds, p.t_ref, p.prandtl, p.prandtl_turb, p.t_wall = 6.5e6, 288.15, 0.72, 0.9, -1.0
     p.venkat_k, p.entropy_scale = 5.0, 1.0
     p.cfl_start, p.cfl_end, p.ktmax, p.lusgs_tol = 0.01, 10.0, 1.0e10, 0.01
-    p.force_ref_length, p.force_ref_area = 1.0, 1.0
+    p.force_ref_length, p.force_ref_area, p.force_ref_span = 1.0, 1.0, 1.0
     p.viscous, p.gradient, p.limiter, p.entropy_fix = 0, 0, 0, 3
     p.cfl_vary_step, p.n_lusgs_sweeps, p.precision = 500, 1, 0
     for k, v in overrides.items():
@@ -90,6 +91,10 @@ def _declare(lib):
         "phl_mesh_bump_hex": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
                                              ctypes.POINTER(_P)]),
         "phl_mesh_sizes": (ctypes.c_int, [_P, _I, _I, _I, _I, _I, _I]),
+        "phl_mesh_bump_hybrid": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                                ctypes.c_int, ctypes.c_double, ctypes.c_uint, ctypes.c_double,
+                                                ctypes.POINTER(_P)]),
+        "phl_mesh_renumber": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, ctypes.POINTER(_P)]),
         "phl_mesh_connectivity": (ctypes.c_int, [_P, _D, _D, _D, _I, _I, _I, _I, _I, _I, _I]),
         "phl_mesh_metrics": (ctypes.c_int, [_P] + [_D] * 12),
         "phl_mesh_destroy": (None, [_P]),
@@ -114,6 +119,10 @@ def _declare(lib):
         "phl_iterate": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
         "phl_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
         "phl_synchronize": (ctypes.c_int, [_P]),
+        "phl_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
+        "phl_get_kernel_timing": (ctypes.c_int, [_P, _D, _I]),
+        "phl_hbm_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_double)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -173,6 +182,13 @@ class Mesh:
         return cls(h)
 
     @classmethod
+    def bump_hybrid(cls, nx, nz, n_prism=15, h1=1e-4, n_tet=20, jitter=0.1, seed=1, bump=0.05):
+        h = _P()
+        _check(load_library().phl_mesh_bump_hybrid(nx, nz, n_prism, h1, n_tet, jitter, seed, bump,
+                                                   ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
     def from_arrays(cls, x, y, z, face_node_ptr, face_nodes, left, right, n_bface, bc_type,
                     cell_node_ptr, cell_nodes):
         a = [np.ascontiguousarray(v, dtype=np.float64) for v in (x, y, z)]
@@ -184,6 +200,19 @@ class Mesh:
             _iptr(b[2]), _iptr(b[3]), n_bface, _iptr(b[4]), len(b[5]) - 1, _iptr(b[5]), _iptr(b[6]),
             ctypes.byref(h)))
         return cls(h)
+
+    def renumbered(self, method="colour", perm=None):
+        """New mesh with cells renumbered ('colour' or an explicit permutation new->old)."""
+        h = _P()
+        nc = ctypes.c_int()
+        if method == "colour":
+            _check(load_library().phl_mesh_renumber(self._h, 0, None, ctypes.byref(nc), ctypes.byref(h)))
+        else:
+            p = np.ascontiguousarray(perm, dtype=np.int32)
+            _check(load_library().phl_mesh_renumber(self._h, 1, _iptr(p), ctypes.byref(nc), ctypes.byref(h)))
+        out = Mesh(h)
+        out.n_colours = nc.value
+        return out
 
     def connectivity(self):
         n = self
@@ -216,6 +245,17 @@ class Mesh:
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phl_mesh_destroy(self._h)
             self._h = None
+
+
+KERNEL_CLASSES = ["boundary", "grad_lim", "flux", "norm", "dt_diag", "sweep_fwd", "ghost_dq",
+                  "sweep_bwd", "update"]
+
+
+def hbm_copy_gbps(device=0, nbytes=2 * 1024**3, iters=10):
+    """Best-of-iters streaming-copy bandwidth (read + write bytes / time)."""
+    g = ctypes.c_double()
+    _check(load_library().phl_hbm_copy(device, float(nbytes), iters, ctypes.byref(g)))
+    return g.value
 
 
 def device_info(device=0):
@@ -305,6 +345,16 @@ class Solver:
 
     def synchronize(self):
         _check(_lib.phl_synchronize(self._h))
+
+    def set_kernel_timing(self, enable=True):
+        _check(_lib.phl_set_kernel_timing(self._h, 1 if enable else 0))
+
+    def kernel_timing(self):
+        """{class: total ms}, number of timed steps (HIP events on the solver stream)."""
+        ms = np.zeros(len(KERNEL_CLASSES))
+        n = ctypes.c_int()
+        _check(_lib.phl_get_kernel_timing(self._h, _dptr(ms), ctypes.byref(n)))
+        return dict(zip(KERNEL_CLASSES, ms.tolist())), n.value
 
     def close(self):
         if getattr(self, "_h", None) and _lib is not None:

@@ -30,7 +30,14 @@ struct phl_ctx {
     int normSlots = 0;
     double *hostNorm = nullptr;  // pinned
     int *hostCount = nullptr;    // pinned
+    double *hostCfl = nullptr;   // pinned, CFL of each step of a chunk
+    int *hostZero = nullptr;     // pinned constant 0
     double primInf[5];
+    // kernel timing (HIP events on `stream`)
+    bool timing = false;
+    std::vector<hipEvent_t> events;
+    double kernelMs[PHL_N_KERNEL_CLASSES] = {0};
+    int timedSteps = 0;
 };
 
 namespace {
@@ -222,6 +229,7 @@ void phl_params_default(phl_params *p)
     p->lusgs_tol = 0.01;
     p->force_ref_length = 1.0;
     p->force_ref_area = 1.0;
+    p->force_ref_span = 1.0;
     p->viscous = 0;
     p->gradient = 0;
     p->limiter = 0;
@@ -277,6 +285,46 @@ phl_status phl_mesh_bump_hex(int nx, int ny, int nz, double bump, phl_mesh **out
         delete pm;
         return fail(PHL_ERR_ARG, err);
     }
+    *out = pm;
+    return PHL_OK;
+}
+
+phl_status phl_mesh_bump_hybrid(int nx, int nz, int n_prism, double h1, int n_tet, double jitter, unsigned seed,
+                                double bump, phl_mesh **out)
+{
+    if (!out) return fail(PHL_ERR_ARG, "null out");
+    phl_mesh *pm = new (std::nothrow) phl_mesh;
+    if (!pm) return fail(PHL_ERR_ALLOC, "out of memory");
+    std::string err;
+    if (!phl::makeBumpHybrid(nx, nz, n_prism, h1, n_tet, jitter, seed, bump, pm->m, err)) {
+        delete pm;
+        return fail(PHL_ERR_ARG, err);
+    }
+    *out = pm;
+    return PHL_OK;
+}
+
+phl_status phl_mesh_renumber(const phl_mesh *in, int method, const int *perm, int *n_colours, phl_mesh **out)
+{
+    if (!in || !out) return fail(PHL_ERR_ARG, "null argument");
+    std::vector<int> pv;
+    int nCol = 0;
+    if (method == 0) {
+        pv = phl::colourOrder(in->m, nCol);
+    } else if (method == 1) {
+        if (!perm) return fail(PHL_ERR_ARG, "phl_mesh_renumber: null permutation");
+        pv.assign(perm, perm + in->m.nCell);
+    } else {
+        return fail(PHL_ERR_ARG, "phl_mesh_renumber: unknown method");
+    }
+    phl_mesh *pm = new (std::nothrow) phl_mesh;
+    if (!pm) return fail(PHL_ERR_ALLOC, "out of memory");
+    std::string err;
+    if (!phl::renumberCells(in->m, pv, pm->m, err)) {
+        delete pm;
+        return fail(PHL_ERR_ARG, err);
+    }
+    if (n_colours) *n_colours = nCol;
     *out = pm;
     return PHL_OK;
 }
@@ -349,6 +397,9 @@ void phl_destroy(phl_ctx *c)
     for (void *p : c->allocs) hipFree(p);
     if (c->hostNorm) hipHostFree(c->hostNorm);
     if (c->hostCount) hipHostFree(c->hostCount);
+    if (c->hostCfl) hipHostFree(c->hostCfl);
+    if (c->hostZero) hipHostFree(c->hostZero);
+    for (hipEvent_t e : c->events) hipEventDestroy(e);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -518,10 +569,14 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.normPart, (size_t)5 * phl::flux_blocks(dm)));
     c->normSlots = 1024;
     TRY(dalloc(c, &fl.normOut, (size_t)5 * c->normSlots));
-    TRY(dalloc(c, &fl.dtMinBits, 1));
+    TRY(dalloc(c, &fl.dtMinPart, (size_t)std::max(phl::flux_blocks(dm), (nC + 255) / 256)));
+    TRY(dalloc(c, &fl.dtMin, 1));
+    TRY(dalloc(c, &fl.stepCtr, 1));
+    TRY(dalloc(c, &fl.cflArr, (size_t)c->normSlots));
     TRY(dalloc(c, &fl.negFlag, (size_t)nC));
     TRY(dalloc(c, &fl.negCount, 1));
-    TRY(dalloc(c, &fl.sweepPart, 1));
+    TRY(dalloc(c, &fl.negTotal, 1));
+    TRY(dalloc(c, &fl.coef, (size_t)nF * 2 * 16));
     TRY(dalloc(c, &fl.t, (size_t)nT));
     TRY(dalloc(c, &fl.visl, (size_t)nT));
     HIPTRY(hipMemset(fl.q, 0, sizeof(double) * 5 * nT));
@@ -534,9 +589,14 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     HIPTRY(hipMemset(fl.specVis, 0, sizeof(double) * nC));
     HIPTRY(hipMemset(fl.negFlag, 0, (size_t)nC));
     HIPTRY(hipMemset(fl.negCount, 0, sizeof(int)));
+    HIPTRY(hipMemset(fl.negTotal, 0, sizeof(int)));
+    HIPTRY(hipMemset(fl.stepCtr, 0, sizeof(int)));
     HIPTRY(hipMemcpy(fl.lim, limInit.data(), sizeof(double) * nT, hipMemcpyHostToDevice));
     HIPTRY(hipHostMalloc((void **)&c->hostNorm, sizeof(double) * 5 * c->normSlots, hipHostMallocDefault));
     HIPTRY(hipHostMalloc((void **)&c->hostCount, sizeof(int), hipHostMallocDefault));
+    HIPTRY(hipHostMalloc((void **)&c->hostCfl, sizeof(double) * c->normSlots, hipHostMallocDefault));
+    HIPTRY(hipHostMalloc((void **)&c->hostZero, sizeof(int), hipHostMallocDefault));
+    *c->hostZero = 0;
     HIPTRY(hipDeviceSynchronize());
     return PHL_OK;
 }
@@ -655,19 +715,25 @@ phl_status phl_gradient(phl_ctx *c)
 
 phl_status phl_limiter(phl_ctx *c) { return phl_gradient(c); }
 
-static void enqueue_residual(phl_ctx *c, int slot)
+static void reset_step_counter(phl_ctx *c)
 {
+    hipMemcpyAsync(c->f.stepCtr, c->hostZero, sizeof(int), hipMemcpyHostToDevice, c->stream);
+}
+
+static void enqueue_residual(phl_ctx *c)
+{
+    reset_step_counter(c);
     phl::launch_boundary(c->m, c->k, c->f, c->stream);
     phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
-    phl::launch_flux(c->m, c->k, c->f, c->stream);
-    phl::launch_norm_final(c->m, c->f, slot, c->stream);
+    phl::launch_flux(c->m, c->k, c->f, c->stream, false);
+    phl::launch_norm_final(c->m, c->f, false, c->stream);
 }
 
 phl_status phl_update_residuals(phl_ctx *c)
 {
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
-    enqueue_residual(c, 0);
+    enqueue_residual(c);
     return check_launch(c);
 }
 
@@ -693,7 +759,7 @@ static void enqueue_lusgs(phl_ctx *c)
     phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, c->stream);
     phl::launch_ghost_dq(c->m, c->k, c->f, c->stream);
     phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, c->stream);
-    phl::launch_update(c->m, c->k, c->f, c->stream);
+    phl::launch_update(c->m, c->k, c->f, false, c->stream);
     swap_state(c);
 }
 
@@ -723,24 +789,67 @@ phl_status phl_residual_norm(phl_ctx *c, double *sum5)
 }
 
 // ----------------------------------------------------------- host driver
+static const int kTimedChunk = 64;
+
+// One outer iteration: LoadResiduals+UpdateResiduals, SpectrumRadius,
+// TimeStep, LUSGS (Controller::Relaxation, Controller.cpp:567-608).  With
+// `ev` non-null an event is recorded before the first and after every
+// kernel class.
+static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
+{
+    hipStream_t s = c->stream;
+    int e = 0;
+    auto mark = [&]() {
+        if (ev) hipEventRecord(ev[e++], s);
+    };
+    mark();
+    phl::launch_boundary(c->m, c->k, c->f, s); mark();
+    phl::launch_grad_lim(c->m, c->k, c->f, s); mark();
+    phl::launch_flux(c->m, c->k, c->f, s, true); mark();
+    phl::launch_norm_final(c->m, c->f, true, s); mark();
+    phl::launch_dt_diag(c->m, c->k, c->f, s); mark();
+    phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, s); mark();
+    phl::launch_ghost_dq(c->m, c->k, c->f, s); mark();
+    phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, s); mark();
+    phl::launch_update(c->m, c->k, c->f, true, s); mark();
+    swap_state(c);
+}
+
 phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist)
 {
     if (!c || n_steps < 0) return fail(PHL_ERR_ARG, "bad argument");
     HIPTRY(hipSetDevice(c->device));
+    const int K = PHL_N_KERNEL_CLASSES;
+    if (c->timing && c->events.empty()) {
+        c->events.resize((size_t)kTimedChunk * (K + 1));
+        for (hipEvent_t &e : c->events) HIPTRY(hipEventCreate(&e));
+    }
     int done = 0;
     while (done < n_steps) {
-        const int chunk = std::min(n_steps - done, c->normSlots);
-        for (int i = 0; i < chunk; ++i) {
-            const int step = first_step + done + i;
-            enqueue_residual(c, i);
-            phl::launch_time_step(c->m, c->k, c->f, compute_cfl(c->p, step), c->stream);
-            phl::launch_dt_diag(c->m, c->k, c->f, c->stream);
-            enqueue_lusgs(c);
-        }
+        const int cap = c->timing ? kTimedChunk : c->normSlots;
+        const int chunk = std::min(n_steps - done, cap);
+        // the chunk's CFL schedule and step counter live in device memory so
+        // that the kernel sequence of a step does not depend on the step
+        HIPTRY(hipStreamSynchronize(c->stream));  // hostCfl may still be in use
+        for (int i = 0; i < chunk; ++i) c->hostCfl[i] = compute_cfl(c->p, first_step + done + i);
+        HIPTRY(hipMemcpyAsync(c->f.cflArr, c->hostCfl, sizeof(double) * chunk, hipMemcpyHostToDevice, c->stream));
+        reset_step_counter(c);
+        for (int i = 0; i < chunk; ++i)
+            enqueue_step(c, c->timing ? &c->events[(size_t)i * (K + 1)] : nullptr);
         TRY(check_launch(c));
         HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5 * chunk, hipMemcpyDeviceToHost,
                               c->stream));
         HIPTRY(hipStreamSynchronize(c->stream));
+        if (c->timing) {
+            for (int i = 0; i < chunk; ++i)
+                for (int k = 0; k < K; ++k) {
+                    float ms = 0.f;
+                    HIPTRY(hipEventElapsedTime(&ms, c->events[(size_t)i * (K + 1) + k],
+                                               c->events[(size_t)i * (K + 1) + k + 1]));
+                    c->kernelMs[k] += ms;
+                }
+            c->timedSteps += chunk;
+        }
         for (int i = 0; i < chunk; ++i) {
             const double s0 = c->hostNorm[(size_t)i * 5];
             if (!std::isfinite(s0))
@@ -749,6 +858,55 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         }
         done += chunk;
     }
+    return PHL_OK;
+}
+
+phl_status phl_set_kernel_timing(phl_ctx *c, int enable)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    c->timing = enable != 0;
+    for (double &v : c->kernelMs) v = 0.0;
+    c->timedSteps = 0;
+    return PHL_OK;
+}
+
+phl_status phl_get_kernel_timing(phl_ctx *c, double *ms_total, int *n_steps)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    if (ms_total)
+        for (int k = 0; k < PHL_N_KERNEL_CLASSES; ++k) ms_total[k] = c->kernelMs[k];
+    if (n_steps) *n_steps = c->timedSteps;
+    return PHL_OK;
+}
+
+phl_status phl_hbm_copy(int device, double bytes, int iters, double *gbps)
+{
+    if (!(bytes > 0) || iters <= 0 || !gbps) return fail(PHL_ERR_ARG, "bad argument");
+    HIPTRY(hipSetDevice(device));
+    const size_t n = ((size_t)(bytes / 2 / sizeof(double)) / 2) * 2;  // doubles per buffer
+    double *a = nullptr, *b = nullptr;
+    HIPTRY(hipMalloc(&a, n * sizeof(double)));
+    hipError_t e = hipMalloc(&b, n * sizeof(double));
+    if (e != hipSuccess) { hipFree(a); return fail(PHL_ERR_ALLOC, "hipMalloc failed"); }
+    hipStream_t s;
+    hipEvent_t e0, e1;
+    HIPTRY(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    HIPTRY(hipEventCreate(&e0));
+    HIPTRY(hipEventCreate(&e1));
+    HIPTRY(hipMemsetAsync(a, 0, n * sizeof(double), s));
+    double best = 0.0;
+    for (int i = 0; i < iters + 1; ++i) {
+        hipEventRecord(e0, s);
+        phl::launch_copy(a, b, n, s);
+        hipEventRecord(e1, s);
+        HIPTRY(hipEventSynchronize(e1));
+        float ms = 0.f;
+        HIPTRY(hipEventElapsedTime(&ms, e0, e1));
+        if (i > 0) best = std::max(best, 2.0 * n * sizeof(double) / (ms * 1e-3) / 1e9);
+    }
+    hipEventDestroy(e0); hipEventDestroy(e1); hipStreamDestroy(s);
+    hipFree(a); hipFree(b);
+    *gbps = best;
     return PHL_OK;
 }
 

@@ -62,12 +62,16 @@ struct DevFlow {
     double *dq;       // [5][nTotal]
     double *dflux;    // [5][nCell]
     double *specInv, *specVis, *dtLoc, *dt, *diag;  // [nCell]
-    double *normPart;  // [nBlocksFlux][5]
-    double *normOut;   // [maxSteps][5]
-    unsigned long long *dtMinBits;  // global min of dtLoc (positive doubles order as integers)
-    unsigned char *negFlag;          // [nCell]
-    int *negCount;
-    double *sweepPart;  // LU-SGS sweep norm partials
+    double *normPart;   // [nBlocksFlux][5]
+    double *normOut;    // [maxSteps][5], row = *stepCtr
+    double *dtMinPart;  // [nBlocksFlux] block minima of the local time step
+    double *dtMin;      // global minimum (ComputeMinTimeStep)
+    int *stepCtr;       // step index within the current chunk (device side)
+    double *cflArr;     // [maxSteps] CFL of each step of the chunk
+    unsigned char *negFlag;  // [nCell]
+    int *negCount;      // SolutionFix count of the last update
+    int *negTotal;      // running SolutionFix count
+    double *coef;       // [nFace][2][16] LU-SGS face coefficients per side
     double *t, *visl;   // [nTotal], only for the standalone gamma/T hook
 };
 
@@ -83,15 +87,16 @@ struct Levels {
 void launch_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
-void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt);
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 int flux_blocks(const DevMesh &m);
-void launch_norm_final(const DevMesh &m, DevFlow &f, int slot, hipStream_t s);
+void launch_norm_final(const DevMesh &m, DevFlow &f, bool withDt, hipStream_t s);
 void launch_time_step(const DevMesh &m, const Consts &k, DevFlow &f, double cfl, hipStream_t s);
 void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,
                   bool firstSweep, hipStream_t s);
 void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
-void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, hipStream_t s);
+void launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
 
 }  // namespace phl

@@ -220,10 +220,136 @@ __global__ void __launch_bounds__(kBlock) k_gamma_t(DevMesh M, Consts K, const d
     visl[i] = K.viscous ? tt * sqrt(tt) * (1.0 + K.tsuth) / (tt + K.tsuth) : 0.0;
 }
 
+// ------------------------------------------------------ LU-SGS operator
+// The sweep term of a face, MXDQ_STD(q_nb, n, S, gamma, dq_nb, rad, -1) minus
+// the viscous radius times dq_nb, is linear in dq_nb.  Its coefficients depend
+// only on the two cell states, so they are evaluated once per face side in
+// the (fully parallel) gradient kernel and the level-by-level sweep applies
+// them to dq.  The split keeps every operation and its order, so results
+// are identical to evaluating MXDQ_STD inside the sweep.
+constexpr int kCoef = 16;  // doubles per face side
+__device__ void mxdq(const double *prim, double nxs, double nys, double nzs, double area, double gama,
+                     const double *dq, double radius, double *f)
+{
+    const double vgn = 0.0;
+    const double rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
+    const double c2 = dABS(gama * pm / rm);
+    const double cm = sqrt(c2);
+    const double cits = nxs * um + nys * vm + nzs * wm - vgn;
+    double lmd1 = cits, lmd2 = cits + cm, lmd3 = cits - cm;
+    const double ipnrad = -1.0 * radius / dMAX(kSMALL, area);
+    lmd1 = 0.5 * (lmd1 + ipnrad);
+    lmd2 = 0.5 * (lmd2 + ipnrad);
+    lmd3 = 0.5 * (lmd3 + ipnrad);
+    const double x1 = (lmd1 + lmd1 - lmd2 - lmd3) / (c2 + c2);
+    const double x2 = (lmd2 - lmd3) / (cm + cm);
+    const double dc = (cits + vgn) * dq[0] - nxs * dq[1] - nys * dq[2] - nzs * dq[3];
+    const double c2dc = c2 * dc;
+    const double v2 = um * um + vm * vm + wm * wm;
+    const double gama1 = gama - 1.0;
+    const double cc1 = 0.5 * gama1 * v2;
+    double dh = -gama1 * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
+    const double enthalpy = (gama / gama1) * (pm / rm);
+    dh += cc1 * dq[0];
+    const double hm = enthalpy + 0.5 * v2;
+    const double a = dh * x1 + dc * x2;
+    const double b = c2dc * x1 + dh * x2;
+    f[0] = lmd1 * dq[0] - a;
+    f[1] = lmd1 * dq[1] - um * a + nxs * b;
+    f[2] = lmd1 * dq[2] - vm * a + nys * b;
+    f[3] = lmd1 * dq[3] - wm * a + nzs * b;
+    f[4] = lmd1 * dq[4] - hm * a + (cits + vgn) * b;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) f[m] *= area;
+}
+
+__device__ __forceinline__ double lusgs_rad(const double *ql, const double *qr, double nx, double ny,
+                                            double nz, double area, double gama)
+{
+    const double gl = gama, gr = gama;
+    const double hl = gl / (gl - 1.0) * ql[4] / ql[0] + 0.5 * (ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3]);
+    const double hr = gr / (gr - 1.0) * qr[4] / qr[0] + 0.5 * (qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
+    const double t0 = sqrt(qr[0] / ql[0]);
+    const double t1 = 1.0 / (1.0 + t0);
+    const double rm = sqrt(qr[0] * ql[0]);
+    const double um = (ql[1] + qr[1] * t0) * t1;
+    const double vm = (ql[2] + qr[2] * t0) * t1;
+    const double wm = (ql[3] + qr[3] * t0) * t1;
+    const double hm = (hl + hr * t0) * t1;
+    const double pm = rm * (hm - 0.5 * (um * um + vm * vm + wm * wm)) * (gl - 1.0) / gl;
+    const double vn = nx * um + ny * vm + nz * wm - 0.0;
+    const double eig = dABS(vn) + sqrt(gama * pm / rm);
+    return eig * area;
+}
+
+
+__device__ __forceinline__ void sweep_coef(const Consts &K, const double *qc, const double *qn, const FaceRec &fr,
+                                           bool isL, double dist, double *co)
+{
+    double nx = fr.nx, ny = fr.ny, nz = fr.nz;
+    if (!isL) { nx = -nx; ny = -ny; nz = -nz; }
+    const double gama = K.gamma;
+    const double rad = lusgs_rad(qc, qn, nx, ny, nz, fr.area, gama);
+    const double vgn = 0.0;
+    const double rm = qn[0], um = qn[1], vm = qn[2], wm = qn[3], pm = qn[4];
+    const double c2 = dABS(gama * pm / rm);
+    const double cm = sqrt(c2);
+    const double cits = nx * um + ny * vm + nz * wm - vgn;
+    double lmd1 = cits, lmd2 = cits + cm, lmd3 = cits - cm;
+    const double ipnrad = -1.0 * rad / dMAX(kSMALL, fr.area);
+    lmd1 = 0.5 * (lmd1 + ipnrad);
+    lmd2 = 0.5 * (lmd2 + ipnrad);
+    lmd3 = 0.5 * (lmd3 + ipnrad);
+    const double x1 = (lmd1 + lmd1 - lmd2 - lmd3) / (c2 + c2);
+    const double x2 = (lmd2 - lmd3) / (cm + cm);
+    const double v2 = um * um + vm * vm + wm * wm;
+    const double gama1 = gama - 1.0;
+    const double cc1 = 0.5 * gama1 * v2;
+    const double enthalpy = (gama / gama1) * (pm / rm);
+    const double hm = enthalpy + 0.5 * v2;
+    double sr = 0.0;
+    if (K.viscous) {
+        const double tl = qc[4] / (K.R * qc[0]);
+        const double tr = qn[4] / (K.R * qn[0]);
+        const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
+                                 tr * sqrt(tr) * (1.0 + K.tsuth) / (tr + K.tsuth));
+        const double vt = 0.5 * (0.0 + 0.0);
+        const double rho = 0.5 * (qc[0] + qn[0]);
+        const double vis = vl + vt;
+        sr = 2.0 * vis / (rho * dist * K.reynolds + kSMALL);
+        sr *= 0.5 * fr.area;
+    }
+    co[0] = lmd1; co[1] = x1; co[2] = x2; co[3] = c2; co[4] = cits + vgn;
+    co[5] = um; co[6] = vm; co[7] = wm; co[8] = hm; co[9] = cc1;
+    co[10] = nx; co[11] = ny; co[12] = nz; co[13] = fr.area; co[14] = sr; co[15] = 0.0;
+}
+
+__device__ __forceinline__ void sweep_apply(const Consts &K, const double *co, const double *dq, double *f)
+{
+    const double gama1 = K.gamma - 1.0;
+    const double dc = co[4] * dq[0] - co[10] * dq[1] - co[11] * dq[2] - co[12] * dq[3];
+    const double c2dc = co[3] * dc;
+    double dh = -gama1 * (co[5] * dq[1] + co[6] * dq[2] + co[7] * dq[3] - dq[4]);
+    dh += co[9] * dq[0];
+    const double a = dh * co[1] + dc * co[2];
+    const double b = c2dc * co[1] + dh * co[2];
+    f[0] = co[0] * dq[0] - a;
+    f[1] = co[0] * dq[1] - co[5] * a + co[10] * b;
+    f[2] = co[0] * dq[2] - co[6] * a + co[11] * b;
+    f[3] = co[0] * dq[3] - co[7] * a + co[12] * b;
+    f[4] = co[0] * dq[4] - co[8] * a + co[4] * b;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) f[m] *= co[13];
+    if (K.viscous) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) f[m] -= co[14] * dq[m];
+    }
+}
+
 // ------------------------------------------------ gradients + limiter
 __global__ void __launch_bounds__(kBlock) k_grad_lim(DevMesh M, Consts K, const double *__restrict__ q,
                                                      double *__restrict__ grad, double *__restrict__ gradT,
-                                                     double *__restrict__ lim)
+                                                     double *__restrict__ lim, double *__restrict__ coef)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
@@ -261,6 +387,17 @@ __global__ void __launch_bounds__(kBlock) k_grad_lim(DevMesh M, Consts K, const 
         if (nb < nC) {
             rmin = dMIN(rmin, qn[0]); rmax = dMAX(rmax, qn[0]);
             pmin = dMIN(pmin, qn[4]); pmax = dMAX(pmax, qn[4]);
+        }
+        // LU-SGS coefficients of this face seen from cell c
+        {
+            const double dist = K.viscous ? dABS(fr.nx * (M.xcc[nb] - M.xcc[c]) + fr.ny * (M.ycc[nb] - M.ycc[c]) +
+                                                 fr.nz * (M.zcc[nb] - M.zcc[c]))
+                                          : 0.0;
+            double co[kCoef];
+            sweep_coef(K, qc, qn, fr, isL, dist, co);
+            double2 *dst = reinterpret_cast<double2 *>(coef + ((size_t)f * 2 + (isL ? 0 : 1)) * kCoef);
+#pragma unroll
+            for (int j = 0; j < kCoef / 2; ++j) dst[j] = make_double2(co[2 * j], co[2 * j + 1]);
         }
     }
     const double ovol = 1.0 / M.vol[c];
@@ -480,18 +617,36 @@ __device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const Fac
 
 // One thread per cell: inviscid (and viscous) residual, spectral radii and
 // the per-block partial sums of res^2.  SPEC_ONLY skips the fluxes.
-template <bool SPEC_ONLY>
+// LocalTimeStep + LimitCFL for one cell: CFL*vol/spectral radius, CFL lowered
+// in low-pressure cells and scaled by the precomputed skewness factor.
+__device__ __forceinline__ double local_dt(const DevMesh &M, const Consts &K, int c, double p, double spec,
+                                           double cfl)
+{
+    double cc = cfl;
+    if (p < K.pLower) cc = K.cflStart;
+    else if (p < K.pUpper) cc = (p - K.pUpper) * (cfl - K.cflStart) / (K.pUpper - K.pLower) + cfl;
+    cc *= M.cflSkew[c];
+    return cc * M.vol[c] / spec;
+}
+
+// One thread per cell: inviscid (and viscous) residual, spectral radii, and
+// per-block partials of res^2 and, with FUSED_DT, of the local time step
+// minimum (CFL of the current step read from cflArr[*stepCtr]).
+template <bool SPEC_ONLY, bool FUSED_DT>
 __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const double *__restrict__ q,
                                                  const double *__restrict__ grad,
                                                  const double *__restrict__ gradT,
                                                  const double *__restrict__ lim, double *__restrict__ res,
                                                  double *__restrict__ specInv, double *__restrict__ specVis,
-                                                 double *__restrict__ normPart)
+                                                 double *__restrict__ normPart, const double *__restrict__ cflArr,
+                                                 const int *__restrict__ stepCtr, double *__restrict__ dtLoc,
+                                                 double *__restrict__ dtMinPart)
 {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int c = lb * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
     double acc[5] = {-0.0, -0.0, -0.0, -0.0, -0.0};
+    double dtl = 1.0e300;
     if (c < nC) {
         const bool needG = !SPEC_ONLY && (K.limiter != 2 || K.viscous);
         CellSide own;
@@ -563,8 +718,8 @@ __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const doub
             sInv += 0.5 * fr.area * (dABS(vn) + cm);
         }
         specInv[c] = sInv;
+        double sVis = 0.0;
         if (K.viscous) {
-            double sVis = 0.0;
             for (int s = 0; s < M.ellW; ++s) {
                 const int enc = M.c2fEnc[(size_t)s * nC + c];
                 if (enc == 0) break;
@@ -608,95 +763,119 @@ __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const doub
             }
             specVis[c] = sVis;
         }
+        if (FUSED_DT) {
+            double spec = sInv;
+            if (K.viscous) spec += sVis;
+            dtl = local_dt(M, K, c, own.q[4], spec, cflArr[*stepCtr]);
+            dtLoc[c] = dtl;
+        }
         if (!SPEC_ONLY) {
 #pragma unroll
             for (int m = 0; m < 5; ++m) res[(size_t)m * nC + c] = acc[m];
         }
     }
     if (SPEC_ONLY) return;
-    // deterministic block partial sums of res^2
-    __shared__ double red[5][kBlock];
+    // deterministic block partials: sum of res^2 and min of the local dt
+    __shared__ double red[6][kBlock];
 #pragma unroll
     for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = (c < nC) ? acc[m] * acc[m] : 0.0;
+    red[5][threadIdx.x] = dtl;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
 #pragma unroll
             for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+            red[5][threadIdx.x] = dMIN(red[5][threadIdx.x], red[5][threadIdx.x + w]);
         }
         __syncthreads();
     }
     if (threadIdx.x == 0) {
 #pragma unroll
         for (int m = 0; m < 5; ++m) normPart[(size_t)lb * 5 + m] = red[m][0];
+        if (FUSED_DT) dtMinPart[lb] = red[5][0];
     }
 }
 
-__global__ void k_norm_final(const double *__restrict__ part, int nPart, double *__restrict__ out)
+// Residual sums of the step (row *stepCtr of normOut) and, optionally, the
+// global minimum of the local time step (ComputeMinTimeStep; a minimum is
+// independent of the reduction order).
+__global__ void k_norm_final(const double *__restrict__ part, const double *__restrict__ dtPart, int nPart,
+                             double *__restrict__ normOut, const int *__restrict__ stepCtr,
+                             double *__restrict__ dtMin, int withDt)
 {
-    __shared__ double red[5][kBlock];
+    __shared__ double red[6][kBlock];
     double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double mn = 1.0e300;
     for (int i = threadIdx.x; i < nPart; i += blockDim.x) {
 #pragma unroll
         for (int m = 0; m < 5; ++m) a[m] += part[(size_t)i * 5 + m];
+        if (withDt) mn = dMIN(mn, dtPart[i]);
     }
 #pragma unroll
     for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = a[m];
+    red[5][threadIdx.x] = mn;
     __syncthreads();
     for (int w = kBlock / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
 #pragma unroll
             for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+            red[5][threadIdx.x] = dMIN(red[5][threadIdx.x], red[5][threadIdx.x + w]);
         }
         __syncthreads();
     }
-    if (threadIdx.x < 5) out[threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x < 5) normOut[(size_t)(*stepCtr) * 5 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x == 5 && withDt) *dtMin = red[5][0];
 }
 
 // ---------------------------------------------------------- time step
-__global__ void k_reset(unsigned long long *dtMinBits, int *negCount)
-{
-    *dtMinBits = 0x7FF0000000000000ull;  // +inf
-    *negCount = 0;
-}
-
+// Standalone TimeStep hook: local dt and block minima.
 __global__ void __launch_bounds__(kBlock) k_time_step(DevMesh M, Consts K, const double *__restrict__ q,
                                                       const double *__restrict__ specInv,
                                                       const double *__restrict__ specVis, double cfl,
-                                                      double *__restrict__ dtLoc,
-                                                      unsigned long long *__restrict__ dtMinBits)
+                                                      double *__restrict__ dtLoc, double *__restrict__ dtMinPart)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    unsigned long long bits = 0x7FF0000000000000ull;
+    double dtl = 1.0e300;
     if (c < M.nCell) {
         double spec = specInv[c];
         if (K.viscous) spec += specVis[c];
-        double cc = cfl;
-        const double p = q[(size_t)4 * M.nTotal + c];
-        if (p < K.pLower) cc = K.cflStart;
-        else if (p < K.pUpper) cc = (p - K.pUpper) * (cfl - K.cflStart) / (K.pUpper - K.pLower) + cfl;
-        cc *= M.cflSkew[c];
-        const double dt = cc * M.vol[c] / spec;
-        dtLoc[c] = dt;
-        bits = (unsigned long long)__double_as_longlong(dt);
+        dtl = local_dt(M, K, c, q[(size_t)4 * M.nTotal + c], spec, cfl);
+        dtLoc[c] = dtl;
     }
-    // wave-level min, then one atomic per wave (min is order independent)
-    for (int off = 32; off > 0; off >>= 1) {
-        const unsigned long long o = __shfl_down(bits, off, 64);
-        bits = (o < bits) ? o : bits;
+    __shared__ double red[kBlock];
+    red[threadIdx.x] = dtl;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = dMIN(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
     }
-    if ((threadIdx.x & 63) == 0) atomicMin(dtMinBits, bits);
+    if (threadIdx.x == 0) dtMinPart[blockIdx.x] = red[0];
 }
 
+__global__ void k_min_final(const double *__restrict__ part, int nPart, double *__restrict__ out)
+{
+    __shared__ double red[kBlock];
+    double mn = 1.0e300;
+    for (int i = threadIdx.x; i < nPart; i += blockDim.x) mn = dMIN(mn, part[i]);
+    red[threadIdx.x] = mn;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] = dMIN(red[threadIdx.x], red[threadIdx.x + w]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
+}
+
+// ReduceMaxTimeStep + Diagonal: dt = min(dt, ktmax*dtmin)/vol; D = 1/dt + radii.
 __global__ void __launch_bounds__(kBlock) k_dt_diag(DevMesh M, Consts K, const double *__restrict__ dtLoc,
                                                     const double *__restrict__ specInv,
                                                     const double *__restrict__ specVis,
-                                                    const unsigned long long *__restrict__ dtMinBits,
+                                                    const double *__restrict__ dtMin,
                                                     double *__restrict__ dt, double *__restrict__ diag)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= M.nCell) return;
-    const double mn = __longlong_as_double((long long)*dtMinBits);
+    const double mn = *dtMin;
     double d = dtLoc[c];
     if (K.ktmax > 0) d = dMIN(d, K.ktmax * mn);
     d /= M.vol[c];
@@ -707,139 +886,133 @@ __global__ void __launch_bounds__(kBlock) k_dt_diag(DevMesh M, Consts K, const d
     diag[c] = D;
 }
 
-// --------------------------------------------------------------- LU-SGS
-__device__ void mxdq(const double *prim, double nxs, double nys, double nzs, double area, double gama,
-                     const double *dq, double radius, double *f)
-{
-    const double vgn = 0.0;
-    const double rm = prim[0], um = prim[1], vm = prim[2], wm = prim[3], pm = prim[4];
-    const double c2 = dABS(gama * pm / rm);
-    const double cm = sqrt(c2);
-    const double cits = nxs * um + nys * vm + nzs * wm - vgn;
-    double lmd1 = cits, lmd2 = cits + cm, lmd3 = cits - cm;
-    const double ipnrad = -1.0 * radius / dMAX(kSMALL, area);
-    lmd1 = 0.5 * (lmd1 + ipnrad);
-    lmd2 = 0.5 * (lmd2 + ipnrad);
-    lmd3 = 0.5 * (lmd3 + ipnrad);
-    const double x1 = (lmd1 + lmd1 - lmd2 - lmd3) / (c2 + c2);
-    const double x2 = (lmd2 - lmd3) / (cm + cm);
-    const double dc = (cits + vgn) * dq[0] - nxs * dq[1] - nys * dq[2] - nzs * dq[3];
-    const double c2dc = c2 * dc;
-    const double v2 = um * um + vm * vm + wm * wm;
-    const double gama1 = gama - 1.0;
-    const double cc1 = 0.5 * gama1 * v2;
-    double dh = -gama1 * (um * dq[1] + vm * dq[2] + wm * dq[3] - dq[4]);
-    const double enthalpy = (gama / gama1) * (pm / rm);
-    dh += cc1 * dq[0];
-    const double hm = enthalpy + 0.5 * v2;
-    const double a = dh * x1 + dc * x2;
-    const double b = c2dc * x1 + dh * x2;
-    f[0] = lmd1 * dq[0] - a;
-    f[1] = lmd1 * dq[1] - um * a + nxs * b;
-    f[2] = lmd1 * dq[2] - vm * a + nys * b;
-    f[3] = lmd1 * dq[3] - wm * a + nzs * b;
-    f[4] = lmd1 * dq[4] - hm * a + (cits + vgn) * b;
-#pragma unroll
-    for (int m = 0; m < 5; ++m) f[m] *= area;
-}
-
-__device__ __forceinline__ double lusgs_rad(const double *ql, const double *qr, double nx, double ny,
-                                            double nz, double area, double gama)
-{
-    const double gl = gama, gr = gama;
-    const double hl = gl / (gl - 1.0) * ql[4] / ql[0] + 0.5 * (ql[1] * ql[1] + ql[2] * ql[2] + ql[3] * ql[3]);
-    const double hr = gr / (gr - 1.0) * qr[4] / qr[0] + 0.5 * (qr[1] * qr[1] + qr[2] * qr[2] + qr[3] * qr[3]);
-    const double t0 = sqrt(qr[0] / ql[0]);
-    const double t1 = 1.0 / (1.0 + t0);
-    const double rm = sqrt(qr[0] * ql[0]);
-    const double um = (ql[1] + qr[1] * t0) * t1;
-    const double vm = (ql[2] + qr[2] * t0) * t1;
-    const double wm = (ql[3] + qr[3] * t0) * t1;
-    const double hm = (hl + hr * t0) * t1;
-    const double pm = rm * (hm - 0.5 * (um * um + vm * vm + wm * wm)) * (gl - 1.0) / gl;
-    const double vn = nx * um + ny * vm + nz * wm - 0.0;
-    const double eig = dABS(vn) + sqrt(gama * pm / rm);
-    return eig * area;
-}
-
-__device__ __forceinline__ void sweep_cell(const DevMesh &M, const Consts &K, int c, bool forward,
-                                           bool firstSweep, const double *__restrict__ q,
-                                           const double *__restrict__ res, const double *__restrict__ diag,
-                                           double *dq, double *dflux)
+__device__ __forceinline__ void sweep_finish(const DevMesh &M, int c, bool useU, const double *rhs0,
+                                             const double *__restrict__ res, const double *__restrict__ diag,
+                                             double *dq, double *dflux)
 {
     const int nC = M.nCell, nT = M.nTotal;
-    double qc[5];
-    load5(q, nT, c, qc);
-    double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int s = 0; s < M.ellW; ++s) {
-        const int enc = M.c2fEnc[(size_t)s * nC + c];
-        if (enc == 0) break;
-        const int nb = M.c2n[(size_t)s * nC + c];
-        if (forward ? (nb > c) : (nb < c)) continue;
-        const bool isL = enc > 0;
-        const int f = (isL ? enc : -enc) - 1;
-        const FaceRec fr = M.face[f];
-        double nx = fr.nx, ny = fr.ny, nz = fr.nz;
-        if (!isL) { nx = -nx; ny = -ny; nz = -nz; }
-        double qn[5], dqn[5], df[5];
-        load5(q, nT, nb, qn);
-        load5(dq, nT, nb, dqn);
-        const double rad = lusgs_rad(qc, qn, nx, ny, nz, fr.area, K.gamma);
-        mxdq(qn, nx, ny, nz, fr.area, K.gamma, dqn, rad, df);
-        if (K.viscous) {
-            const double dist = dABS(fr.nx * (M.xcc[nb] - M.xcc[c]) + fr.ny * (M.ycc[nb] - M.ycc[c]) +
-                                     fr.nz * (M.zcc[nb] - M.zcc[c]));
-            const double tl = qc[4] / (K.R * qc[0]);
-            const double tr = qn[4] / (K.R * qn[0]);
-            const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
-                                     tr * sqrt(tr) * (1.0 + K.tsuth) / (tr + K.tsuth));
-            const double vt = 0.5 * (0.0 + 0.0);
-            const double rho = 0.5 * (qc[0] + qn[0]);
-            const double vis = vl + vt;
-            double sr = 2.0 * vis / (rho * dist * K.reynolds + kSMALL);
-            sr *= 0.5 * fr.area;
-#pragma unroll
-            for (int m = 0; m < 5; ++m) df[m] -= sr * dqn[m];
-        }
-#pragma unroll
-        for (int m = 0; m < 5; ++m) rhs0[m] += df[m];
-    }
     const double D = diag[c];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const double U = (forward && firstSweep) ? 0.0 : dflux[(size_t)m * nC + c];
+        const double U = useU ? dflux[(size_t)m * nC + c] : 0.0;
         const double r = res[(size_t)m * nC + c];
         dq[(size_t)m * nT + c] = (r - U - rhs0[m]) / D;
         dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_sweep_level(DevMesh M, Consts K, const int *__restrict__ cells,
-                                                        int begin, int end, bool forward, bool firstSweep,
-                                                        const double *__restrict__ q,
-                                                        const double *__restrict__ res,
-                                                        const double *__restrict__ diag,
-                                                        double *dq, double *dflux)
+__device__ __forceinline__ void load_coef(const double *__restrict__ coef, int f, bool isL, double *co)
+{
+    const double *p = coef + ((size_t)f * 2 + (isL ? 0 : 1)) * kCoef;
+    const double2 *src = reinterpret_cast<const double2 *>(p);
+#pragma unroll
+    for (int j = 0; j < 7; ++j) {
+        const double2 v = src[j];
+        co[2 * j] = v.x;
+        co[2 * j + 1] = v.y;
+    }
+    co[14] = p[14];
+}
+
+// Sweep of one cell for meshes with at most kW faces per cell: the face list,
+// the coefficients and the neighbours' dq are all loaded before any is used,
+// so a level costs a few memory latencies.
+constexpr int kW = 6;
+constexpr int kSweepBlock = 512;
+template <bool FWD>
+__device__ __forceinline__ void sweep_cell_batched(const DevMesh &M, const Consts &K, int c, bool useU,
+                                                   const double *__restrict__ coef, const double *__restrict__ res,
+                                                   const double *__restrict__ diag, double *dq, double *dflux)
+{
+    const int nC = M.nCell, nT = M.nTotal;
+    int enc[kW], nb[kW];
+#pragma unroll
+    for (int s = 0; s < kW; ++s) {
+        enc[s] = (s < M.ellW) ? M.c2fEnc[(size_t)s * nC + c] : 0;
+        nb[s] = (s < M.ellW) ? M.c2n[(size_t)s * nC + c] : 0;
+    }
+    // issue every load first (inactive slots read face 0 / cell 0 and are
+    // masked), then accumulate in face order
+    bool act[kW];
+    double co[kW][15], dqn[kW][5];
+#pragma unroll
+    for (int s = 0; s < kW; ++s) {
+        act[s] = enc[s] != 0 && (FWD ? (nb[s] < c) : (nb[s] > c));
+        const bool isL = enc[s] > 0;
+        const int f = act[s] ? (isL ? enc[s] : -enc[s]) - 1 : 0;
+        load_coef(coef, f, isL, co[s]);
+        load5(dq, nT, act[s] ? nb[s] : 0, dqn[s]);
+    }
+    double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < kW; ++s) {
+        if (!act[s]) continue;
+        double df[5];
+        sweep_apply(K, co[s], dqn[s], df);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) rhs0[m] += df[m];
+    }
+    sweep_finish(M, c, useU, rhs0, res, diag, dq, dflux);
+}
+
+// Any number of faces per cell (polyhedral meshes): face by face.
+template <bool FWD>
+__device__ __forceinline__ void sweep_cell_generic(const DevMesh &M, const Consts &K, int c, bool useU,
+                                                   const double *__restrict__ coef, const double *__restrict__ res,
+                                                   const double *__restrict__ diag, double *dq, double *dflux)
+{
+    const int nC = M.nCell, nT = M.nTotal;
+    double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int s = 0; s < M.ellW; ++s) {
+        const int enc = M.c2fEnc[(size_t)s * nC + c];
+        if (enc == 0) break;
+        const int nb = M.c2n[(size_t)s * nC + c];
+        if (FWD ? (nb > c) : (nb < c)) continue;
+        const bool isL = enc > 0;
+        const int f = (isL ? enc : -enc) - 1;
+        double co[15], dqn[5], df[5];
+        load_coef(coef, f, isL, co);
+        load5(dq, nT, nb, dqn);
+        sweep_apply(K, co, dqn, df);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) rhs0[m] += df[m];
+    }
+    sweep_finish(M, c, useU, rhs0, res, diag, dq, dflux);
+}
+
+template <bool FWD>
+__device__ __forceinline__ void sweep_cell(const DevMesh &M, const Consts &K, int c, bool useU,
+                                           const double *__restrict__ coef, const double *__restrict__ res,
+                                           const double *__restrict__ diag, double *dq, double *dflux)
+{
+    if (M.ellW <= kW) sweep_cell_batched<FWD>(M, K, c, useU, coef, res, diag, dq, dflux);
+    else sweep_cell_generic<FWD>(M, K, c, useU, coef, res, diag, dq, dflux);
+}
+
+// One level spread over the whole GPU (64-thread workgroups).
+template <bool FWD>
+__global__ void __launch_bounds__(64) k_sweep_level(DevMesh M, Consts K, const int *__restrict__ cells, int begin,
+                                                    int end, bool useU, const double *__restrict__ coef,
+                                                    const double *__restrict__ res, const double *__restrict__ diag,
+                                                    double *dq, double *dflux)
 {
     const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= end) return;
-    sweep_cell(M, K, cells[i], forward, firstSweep, q, res, diag, dq, dflux);
+    sweep_cell<FWD>(M, K, cells[i], useU, coef, res, diag, dq, dflux);
 }
 
 // One workgroup walks a run of small levels; the barrier between levels
 // makes the previous level's dq visible to the whole workgroup.
-__global__ void __launch_bounds__(1024) k_sweep_block(DevMesh M, Consts K, const int *__restrict__ ptr,
-                                                      const int *__restrict__ cells, int l0, int l1,
-                                                      bool forward, bool firstSweep,
-                                                      const double *__restrict__ q,
-                                                      const double *__restrict__ res,
-                                                      const double *__restrict__ diag, double *dq,
-                                                      double *__restrict__ dflux)
+template <bool FWD>
+__global__ void __launch_bounds__(kSweepBlock) k_sweep_block(DevMesh M, Consts K, const int *__restrict__ ptr,
+                                                      const int *__restrict__ cells, int l0, int l1, bool useU,
+                                                      const double *__restrict__ coef, const double *__restrict__ res,
+                                                      const double *__restrict__ diag, double *dq, double *dflux)
 {
     for (int L = l0; L < l1; ++L) {
         const int b = ptr[L], e = ptr[L + 1];
         for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x)
-            sweep_cell(M, K, cells[i], forward, firstSweep, q, res, diag, dq, dflux);
+            sweep_cell<FWD>(M, K, cells[i], useU, coef, res, diag, dq, dflux);
         __syncthreads();
     }
 }
@@ -935,14 +1108,16 @@ __global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K, const do
 
 // Sequential SolutionFix over the flagged cells in increasing index: a
 // neighbour below the cell is read from the updated field, one above from
-// the old field, exactly as the reference's in-place loop sees them.
+// the old field, exactly as the reference's in-place loop sees them.  With
+// `advance` the step counter moves on and the fix count is accumulated.
 __global__ void k_fix(DevMesh M, Consts K, const double *__restrict__ qOld, double *__restrict__ qNew,
-                      unsigned char *__restrict__ negFlag, const int *__restrict__ negCount)
+                      unsigned char *__restrict__ negFlag, int *__restrict__ negCount, int *__restrict__ negTotal,
+                      int *__restrict__ stepCtr, int advance)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (*negCount == 0) return;
     const int nC = M.nCell, nT = M.nTotal;
-    for (int c = 0; c < nC; ++c) {
+    const int count = *negCount;
+    for (int c = 0; c < nC && count > 0; ++c) {
         if (!negFlag[c]) continue;
         negFlag[c] = 0;
         double prim[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
@@ -969,11 +1144,30 @@ __global__ void k_fix(DevMesh M, Consts K, const double *__restrict__ qOld, doub
         }
         for (int m = 0; m < 5; ++m) qNew[(size_t)m * nT + c] = prim[m];
     }
+    if (advance) {
+        *negTotal += count;
+        *negCount = 0;
+        *stepCtr += 1;
+    }
 }
 
 inline int nblk(int n, int b) { return (n + b - 1) / b; }
 
+// Streaming copy used to measure achievable HBM bandwidth.
+__global__ void __launch_bounds__(kBlock) k_copy(const double2 *__restrict__ src, double2 *__restrict__ dst,
+                                                 size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 }  // namespace
+
+void launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
+{
+    const size_t n2 = n / 2;
+    hipLaunchKernelGGL(k_copy, dim3(256 * 32), dim3(kBlock), 0, s, (const double2 *)src, (double2 *)dst, n2);
+}
 
 // ------------------------------------------------------------ launchers
 void launch_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
@@ -990,62 +1184,77 @@ void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
     hipLaunchKernelGGL(k_grad_lim, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
-                       f.lim);
+                       f.lim, f.coef);
 }
 
 int flux_blocks(const DevMesh &m) { return nblk(m.nCell, kBlock); }
 
-void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt)
 {
-    hipLaunchKernelGGL(k_flux<false>, dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
-                       f.lim, f.res, f.specInv, f.specVis, f.normPart);
+    if (fusedDt)
+        hipLaunchKernelGGL((k_flux<false, true>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad,
+                           f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
+                           f.dtMinPart);
+    else
+        hipLaunchKernelGGL((k_flux<false, false>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad,
+                           f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
+                           f.dtMinPart);
 }
 
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_flux<true>, dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
-                       f.lim, f.res, f.specInv, f.specVis, f.normPart);
+    hipLaunchKernelGGL((k_flux<true, false>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
+                       f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart);
 }
 
-void launch_norm_final(const DevMesh &m, DevFlow &f, int slot, hipStream_t s)
+void launch_norm_final(const DevMesh &m, DevFlow &f, bool withDt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, s, f.normPart, flux_blocks(m),
-                       f.normOut + (size_t)slot * 5);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, s, f.normPart, f.dtMinPart, flux_blocks(m), f.normOut,
+                       f.stepCtr, f.dtMin, withDt ? 1 : 0);
 }
 
 void launch_time_step(const DevMesh &m, const Consts &k, DevFlow &f, double cfl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reset, dim3(1), dim3(1), 0, s, f.dtMinBits, f.negCount);
-    hipLaunchKernelGGL(k_time_step, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.specInv,
-                       f.specVis, cfl, f.dtLoc, f.dtMinBits);
+    const int nb = nblk(m.nCell, kBlock);
+    hipLaunchKernelGGL(k_time_step, dim3(nb), dim3(kBlock), 0, s, m, k, f.q, f.specInv, f.specVis, cfl, f.dtLoc,
+                       f.dtMinPart);
+    hipLaunchKernelGGL(k_min_final, dim3(1), dim3(kBlock), 0, s, f.dtMinPart, nb, f.dtMin);
 }
 
 void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
     hipLaunchKernelGGL(k_dt_diag, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.dtLoc, f.specInv,
-                       f.specVis, f.dtMinBits, f.dt, f.diag);
+                       f.specVis, f.dtMin, f.dt, f.diag);
 }
 
-// Levels of at most kSmallLevel cells are grouped and walked by a single
-// workgroup; larger levels get one launch each.
-static const int kSmallLevel = 4096;
+// Runs of levels with at most kSmallLevel cells are walked by one workgroup
+// (barriers are cheap); larger levels are spread over the GPU.
+static const int kSmallLevel = kSweepBlock;
 
 void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,
                   bool firstSweep, hipStream_t s)
 {
+    const bool useU = !(forward && firstSweep);
     int L = 0;
     while (L < lv.nLevels) {
         const int sz = lv.hptr[L + 1] - lv.hptr[L];
         if (sz <= kSmallLevel) {
             int L1 = L + 1;
             while (L1 < lv.nLevels && lv.hptr[L1 + 1] - lv.hptr[L1] <= kSmallLevel) ++L1;
-            hipLaunchKernelGGL(k_sweep_block, dim3(1), dim3(1024), 0, s, m, k, lv.ptr, lv.cells, L, L1, forward,
-                               firstSweep, f.q, f.res, f.diag, f.dq, f.dflux);
+            if (forward)
+                hipLaunchKernelGGL(k_sweep_block<true>, dim3(1), dim3(kSweepBlock), 0, s, m, k, lv.ptr, lv.cells, L, L1,
+                                   useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+            else
+                hipLaunchKernelGGL(k_sweep_block<false>, dim3(1), dim3(kSweepBlock), 0, s, m, k, lv.ptr, lv.cells, L, L1,
+                                   useU, f.coef, f.res, f.diag, f.dq, f.dflux);
             L = L1;
         } else {
-            hipLaunchKernelGGL(k_sweep_level, dim3(nblk(sz, kBlock)), dim3(kBlock), 0, s, m, k, lv.cells,
-                               lv.hptr[L], lv.hptr[L + 1], forward, firstSweep, f.q, f.res, f.diag, f.dq,
-                               f.dflux);
+            if (forward)
+                hipLaunchKernelGGL(k_sweep_level<true>, dim3(nblk(sz, 64)), dim3(64), 0, s, m, k, lv.cells,
+                                   lv.hptr[L], lv.hptr[L + 1], useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+            else
+                hipLaunchKernelGGL(k_sweep_level<false>, dim3(nblk(sz, 64)), dim3(64), 0, s, m, k, lv.cells,
+                                   lv.hptr[L], lv.hptr[L + 1], useU, f.coef, f.res, f.diag, f.dq, f.dflux);
             ++L;
         }
     }
@@ -1057,11 +1266,12 @@ void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
     hipLaunchKernelGGL(k_ghost_dq, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.dq);
 }
 
-void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, hipStream_t s)
 {
     hipLaunchKernelGGL(k_update, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.dq, f.qAlt,
                        f.negFlag, f.negCount);
-    hipLaunchKernelGGL(k_fix, dim3(1), dim3(1), 0, s, m, k, f.q, f.qAlt, f.negFlag, f.negCount);
+    hipLaunchKernelGGL(k_fix, dim3(1), dim3(1), 0, s, m, k, f.q, f.qAlt, f.negFlag, f.negCount, f.negTotal,
+                       f.stepCtr, advance ? 1 : 0);
 }
 
 }  // namespace phl

@@ -427,4 +427,204 @@ bool makeBumpHex(int nx, int ny, int nz, double bump, Mesh &out, std::string &er
     return buildMesh(em, classifyBump, &g, out, err);
 }
 
+bool makeBumpHybrid(int nx, int nz, int nPrism, double h1, int nyTet, double jitter, unsigned seed,
+                    double bump, Mesh &out, std::string &err)
+{
+    if (nx < 1 || nz < 1 || nPrism < 1 || nyTet < 1 || !(h1 > 0.0) || jitter < 0.0 || jitter > 0.3) {
+        err = "makeBumpHybrid: bad arguments";
+        return false;
+    }
+    const double x0 = -1.5, x1 = 1.5, ytop = 0.8, zmax = 0.2;
+    const int NX = nx + 1, NZ = nz + 1;
+    // growth ratio so that the last prism layer is close to the tet row height
+    auto yw = [&](double xx) { return bump * std::exp(-25.0 * xx * xx); };
+    double r = 1.2;
+    {
+        // solve h1 * r^(nPrism-1) = (ytop - thickness(r)) / nyTet by bisection
+        double lo = 1.0 + 1e-9, hi = 3.0;
+        for (int it = 0; it < 200; ++it) {
+            const double mid = 0.5 * (lo + hi);
+            const double thick = h1 * (std::pow(mid, nPrism) - 1.0) / (mid - 1.0);
+            const double last = h1 * std::pow(mid, nPrism - 1);
+            const double target = (ytop - thick) / nyTet;
+            if (last < target) lo = mid; else hi = mid;
+        }
+        r = 0.5 * (lo + hi);
+    }
+    std::vector<double> d(nPrism + 1, 0.0);
+    for (int l = 1; l <= nPrism; ++l) d[l] = d[l - 1] + h1 * std::pow(r, l - 1);
+    if (!(d[nPrism] < 0.5 * (ytop - bump))) {
+        err = "makeBumpHybrid: prism layers too thick";
+        return false;
+    }
+    ElementMesh em;
+    auto nPrismNode = [&](int l, int i, int k) { return (l * NZ + k) * NX + i; };
+    const int prismNodes = (nPrism + 1) * NX * NZ;
+    auto nTetNode = [&](int j, int i, int k) {
+        return j == 0 ? nPrismNode(nPrism, i, k) : prismNodes + ((j - 1) * NZ + k) * NX + i;
+    };
+    const size_t nNode = (size_t)prismNodes + (size_t)nyTet * NX * NZ;
+    em.x.resize(nNode); em.y.resize(nNode); em.z.resize(nNode);
+    for (int l = 0; l <= nPrism; ++l)
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NX; ++i) {
+                const int n = nPrismNode(l, i, k);
+                const double xx = x0 + (x1 - x0) * i / nx;
+                em.x[n] = xx; em.y[n] = yw(xx) + d[l]; em.z[n] = zmax * k / nz;
+            }
+    uint64_t st = 0x9E3779B97F4A7C15ull ^ (uint64_t)seed;
+    auto urand = [&]() {  // xorshift64*, uniform in [-1, 1)
+        st ^= st >> 12; st ^= st << 25; st ^= st >> 27;
+        return (double)((st * 2685821657736338717ull) >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+    };
+    const double dx = (x1 - x0) / nx, dz = zmax / nz;
+    for (int j = 1; j <= nyTet; ++j)
+        for (int k = 0; k < NZ; ++k)
+            for (int i = 0; i < NX; ++i) {
+                const int n = nTetNode(j, i, k);
+                double xx = x0 + (x1 - x0) * i / nx;
+                const double yb = yw(xx) + d[nPrism];
+                const double dy = (ytop - yb) / nyTet;
+                double yy = yb + (ytop - yb) * j / nyTet;
+                double zz = zmax * k / nz;
+                const double jx = urand(), jy = urand(), jz = urand();
+                if (j < nyTet && i > 0 && i < nx && k > 0 && k < nz) {
+                    xx += jitter * dx * jx; yy += jitter * dy * jy; zz += jitter * dz * jz;
+                }
+                em.x[n] = xx; em.y[n] = yy; em.z[n] = zz;
+            }
+    // cells, numbered column by column (k outer, then i), from the wall up
+    em.cellNodePtr.assign(1, 0);
+    auto addCell = [&](int kind, std::initializer_list<int> nodes) {
+        em.kind.push_back(kind);
+        for (int n : nodes) em.cellNodes.push_back(n);
+        em.cellNodePtr.push_back((int)em.cellNodes.size());
+    };
+    for (int k = 0; k < nz; ++k)
+        for (int i = 0; i < nx; ++i) {
+            for (int l = 0; l < nPrism; ++l) {
+                // wall quad split along (i,k)-(i+1,k+1), matching the tet split above
+                const int a0 = nPrismNode(l, i, k), b0 = nPrismNode(l, i + 1, k);
+                const int c0 = nPrismNode(l, i + 1, k + 1), e0 = nPrismNode(l, i, k + 1);
+                const int a1 = nPrismNode(l + 1, i, k), b1 = nPrismNode(l + 1, i + 1, k);
+                const int c1 = nPrismNode(l + 1, i + 1, k + 1), e1 = nPrismNode(l + 1, i, k + 1);
+                addCell(PRISM, {a0, b0, c0, a1, b1, c1});
+                addCell(PRISM, {a0, c0, e0, a1, c1, e1});
+            }
+            for (int j = 0; j < nyTet; ++j) {
+                // lattice cell corners v[dx][dy][dz]; six tetrahedra along the
+                // (0,0,0)-(1,1,1) diagonal, one per ordering of the axes
+                int v[2][2][2];
+                for (int a = 0; a < 2; ++a)
+                    for (int b = 0; b < 2; ++b)
+                        for (int c = 0; c < 2; ++c) v[a][b][c] = nTetNode(j + b, i + a, k + c);
+                static const int perms[6][3] = {{0, 2, 1}, {2, 0, 1}, {0, 1, 2}, {1, 0, 2}, {1, 2, 0}, {2, 1, 0}};
+                for (const auto &pm : perms) {
+                    int p[3] = {0, 0, 0};
+                    int nodes[4];
+                    nodes[0] = v[0][0][0];
+                    for (int s2 = 0; s2 < 3; ++s2) {
+                        p[pm[s2]] = 1;
+                        nodes[s2 + 1] = v[p[0]][p[1]][p[2]];
+                    }
+                    addCell(TET, {nodes[0], nodes[1], nodes[2], nodes[3]});
+                }
+            }
+        }
+    BumpGeom g{x0, x1, ytop, zmax};
+    return buildMesh(em, classifyBump, &g, out, err);
+}
+
+bool renumberCells(const Mesh &in, const std::vector<int> &perm, Mesh &out, std::string &err)
+{
+    const int nC = in.nCell;
+    if ((int)perm.size() != nC) {
+        err = "renumberCells: permutation size";
+        return false;
+    }
+    std::vector<int> inv(nC, -1);
+    for (int i = 0; i < nC; ++i) {
+        if (perm[i] < 0 || perm[i] >= nC || inv[perm[i]] >= 0) {
+            err = "renumberCells: not a permutation";
+            return false;
+        }
+        inv[perm[i]] = i;
+    }
+    struct F { int le, re, bc, old; bool flip; };
+    std::vector<F> bf, inf;
+    for (int f = 0; f < in.nFace; ++f) {
+        if (f < in.nBFace) {
+            bf.push_back({inv[in.left[f]], -1, in.bcType[f], f, false});
+        } else {
+            int a = inv[in.left[f]], b = inv[in.right[f]];
+            const bool flip = a > b;
+            if (flip) std::swap(a, b);
+            inf.push_back({a, b, 0, f, flip});
+        }
+    }
+    std::stable_sort(bf.begin(), bf.end(), [](const F &x, const F &y) {
+        if (x.bc != y.bc) return x.bc < y.bc;
+        return x.le < y.le;
+    });
+    std::sort(inf.begin(), inf.end(), [](const F &x, const F &y) {
+        if (x.le != y.le) return x.le < y.le;
+        return x.re < y.re;
+    });
+    Mesh m;
+    m.nNode = in.nNode; m.nCell = nC; m.nBFace = in.nBFace; m.nFace = in.nFace;
+    m.x = in.x; m.y = in.y; m.z = in.z;
+    m.cellNodePtr.assign(1, 0);
+    for (int i = 0; i < nC; ++i) {
+        const int o = perm[i];
+        for (int k = in.cellNodePtr[o]; k < in.cellNodePtr[o + 1]; ++k) m.cellNodes.push_back(in.cellNodes[k]);
+        m.cellNodePtr.push_back((int)m.cellNodes.size());
+    }
+    m.faceNodePtr.assign(1, 0);
+    m.left.resize(m.nFace); m.right.resize(m.nFace); m.bcType.resize(m.nBFace);
+    for (int f = 0; f < m.nFace; ++f) {
+        const bool bnd = f < m.nBFace;
+        const F &fc = bnd ? bf[f] : inf[f - m.nBFace];
+        const int b = in.faceNodePtr[fc.old], e = in.faceNodePtr[fc.old + 1];
+        std::vector<int> nodes(in.faceNodes.begin() + b, in.faceNodes.begin() + e);
+        if (fc.flip) std::reverse(nodes.begin(), nodes.end());
+        for (int n : nodes) m.faceNodes.push_back(n);
+        m.faceNodePtr.push_back((int)m.faceNodes.size());
+        m.left[f] = fc.le;
+        m.right[f] = bnd ? nC + f : fc.re;
+        if (bnd) m.bcType[f] = fc.bc;
+    }
+    m.computeMetrics();
+    out = std::move(m);
+    return true;
+}
+
+std::vector<int> colourOrder(const Mesh &m, int &nColours)
+{
+    const int nC = m.nCell;
+    std::vector<int> ptr(nC + 1, 0);
+    for (int f = m.nBFace; f < m.nFace; ++f) { ptr[m.left[f] + 1]++; ptr[m.right[f] + 1]++; }
+    for (int i = 0; i < nC; ++i) ptr[i + 1] += ptr[i];
+    std::vector<int> nb(ptr[nC]), fill(ptr.begin(), ptr.end() - 1);
+    for (int f = m.nBFace; f < m.nFace; ++f) {
+        nb[fill[m.left[f]]++] = m.right[f];
+        nb[fill[m.right[f]]++] = m.left[f];
+    }
+    std::vector<int> colour(nC, -1);
+    nColours = 0;
+    std::vector<int> used;
+    for (int c = 0; c < nC; ++c) {
+        used.assign(nColours + 1, 0);
+        for (int k = ptr[c]; k < ptr[c + 1]; ++k)
+            if (colour[nb[k]] >= 0) used[colour[nb[k]]] = 1;
+        int col = 0;
+        while (used[col]) ++col;
+        colour[c] = col;
+        nColours = std::max(nColours, col + 1);
+    }
+    std::vector<int> perm(nC);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) { return colour[a] < colour[b]; });
+    return perm;
+}
+
 }  // namespace phl

@@ -46,4 +46,21 @@ bool buildMesh(const ElementMesh &em, BcClassifier classify, const void *user, M
 
 bool makeBumpHex(int nx, int ny, int nz, double bump, Mesh &out, std::string &err);
 
+// Hybrid bump channel: nPrism prism layers over the triangulated lower wall
+// (first height h1, geometric growth chosen so the last layer matches the tet
+// spacing), then a tetrahedral fill of nyTet rows: a seeded jittered point
+// lattice (interior points moved by up to `jitter` of the local spacing) split
+// into six tetrahedra per lattice cell.
+bool makeBumpHybrid(int nx, int nz, int nPrism, double h1, int nyTet, double jitter, unsigned seed,
+                    double bump, Mesh &out, std::string &err);
+
+// Renumber cells: new cell i is old cell perm[i].  Faces are re-derived in
+// the reference's order (boundary first, then interior by owner) and the
+// metrics recomputed.  The LU-SGS sweep order of the reference is the cell
+// order, so a renumbering is a choice of sweep order.
+bool renumberCells(const Mesh &in, const std::vector<int> &perm, Mesh &out, std::string &err);
+// Greedy face-neighbour colouring in cell order; returns the permutation that
+// lists cells by (colour, old index) and the number of colours.
+std::vector<int> colourOrder(const Mesh &m, int &nColours);
+
 }  // namespace phl

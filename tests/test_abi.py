@@ -1,0 +1,76 @@
+"""C-ABI and host-side checks that need no GPU: the library loads, exports every
+symbol include/phl.h declares, builds the synthetic meshes, and rejects
+unsupported options with a status instead of exiting."""
+import subprocess
+
+import numpy as np
+import pytest
+
+import phenlei_amd as pa
+
+
+def test_library_exports_every_declared_symbol():
+    lib = pa.LIB_PATH
+    out = subprocess.run(["nm", "-D", "--defined-only", lib], capture_output=True, text=True, check=True).stdout
+    exported = {line.split()[-1] for line in out.splitlines() if line.strip()}
+    declared = pa.exported_symbols()
+    assert len(declared) >= 30
+    missing = [s for s in declared if s not in exported]
+    assert not missing, missing
+
+
+def test_bump_channel_sizes_match_the_configuration():
+    m = pa.Mesh.bump_hex(64, 32, 8, 0.05)
+    assert (m.n_cell, m.n_face, m.n_bface) == (16384, 51968, 5632)
+    conn = m.connectivity()
+    counts = np.bincount(conn["bc_type"], minlength=5)
+    assert counts[4] == 2 * 32 * 8        # inlet + outlet farfield
+    assert counts[2] == 64 * 8            # lower wall
+    assert counts[3] == 64 * 8 + 2 * 64 * 32  # top + sides symmetry
+    # faces: boundary first, ghost index nCell+f, interior owner < neighbour
+    nb = m.n_bface
+    assert np.array_equal(conn["right"][:nb], m.n_cell + np.arange(nb))
+    assert np.all(conn["left"][nb:] < conn["right"][nb:])
+
+
+def test_face_normals_point_from_left_to_right():
+    m = pa.Mesh.bump_hex(12, 6, 4, 0.05)
+    conn, met = m.connectivity(), m.metrics()
+    n = np.stack([met["xfn"], met["yfn"], met["zfn"]])
+    d = np.stack([met["xcc"][conn["right"]] - met["xcc"][conn["left"]],
+                  met["ycc"][conn["right"]] - met["ycc"][conn["left"]],
+                  met["zcc"][conn["right"]] - met["zcc"][conn["left"]]])
+    assert np.all((n * d).sum(axis=0) > 0)
+
+
+def test_renumbering_is_a_permutation_of_the_same_grid():
+    m = pa.Mesh.bump_hex(8, 4, 4, 0.05)
+    r = m.renumbered("colour")
+    assert r.n_colours == 2
+    assert (r.n_cell, r.n_face, r.n_bface) == (m.n_cell, m.n_face, m.n_bface)
+    v0 = np.sort(m.metrics()["vol"][:m.n_cell])
+    v1 = np.sort(r.metrics()["vol"][:r.n_cell])
+    assert np.allclose(v0, v1, rtol=1e-13)
+    perm = np.arange(m.n_cell)[::-1].copy()
+    rr = m.renumbered("perm", perm)
+    assert np.allclose(rr.metrics()["vol"][:m.n_cell], m.metrics()["vol"][:m.n_cell][perm], rtol=1e-13)
+
+
+def test_mesh_create_rejects_bad_connectivity():
+    m = pa.Mesh.bump_hex(4, 2, 2, 0.05)
+    c = m.connectivity()
+    bad = c["right"].copy()
+    bad[0] = 0  # a boundary face must point at its ghost cell
+    with pytest.raises(pa.PhlError) as e:
+        pa.Mesh.from_arrays(c["x"], c["y"], c["z"], c["face_node_ptr"], c["face_nodes"], c["left"], bad,
+                            c["n_bface"], c["bc_type"], c["cell_node_ptr"], c["cell_nodes"])
+    assert e.value.status == pa.PHL_ERR_ARG
+
+
+@pytest.mark.parametrize("override", [dict(gradient=1), dict(n_lusgs_sweeps=2), dict(entropy_fix=2),
+                                      dict(t_wall=300.0), dict(viscous=3), dict(precision=7)])
+def test_unsupported_options_return_a_status(override):
+    m = pa.Mesh.bump_hex(4, 2, 2, 0.05)
+    with pytest.raises(pa.PhlError) as e:
+        pa.Solver(m, pa.default_params(**override))
+    assert e.value.status == pa.PHL_ERR_UNSUPPORTED

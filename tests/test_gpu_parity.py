@@ -113,3 +113,53 @@ def test_deterministic():
     q2 = gpu.get_state()
     assert np.array_equal(h1, h2)
     assert np.array_equal(q1, q2)
+
+
+def make_hybrid_pair(**pover):
+    m = pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7)
+    conn, met = m.connectivity(), m.metrics()
+    p = pa.default_params(**pover)
+    return m, pa.Solver(m, p), orc.OracleSolver(conn, met, p)
+
+
+@pytest.mark.parametrize("viscous", [0, 1])
+def test_hybrid_mesh_residual_parity(viscous):
+    m, gpu, cpu = make_hybrid_pair(mach=0.3, viscous=viscous, reynolds=5e5, cfl_start=10.0, cfl_end=10.0)
+    q = seeded(m, cpu, 21 + viscous)
+    gpu.set_state(q)
+    cpu.set("q", q)
+    gpu.update_residuals()
+    cpu.update_residuals()
+    cpu.spectrum()
+    nc = m.n_cell
+    rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
+    floor = 1e-3 * np.max(np.abs(rc), axis=1, keepdims=True)
+    assert rel_err(rg, rc, floor) <= TOL
+    for name in ("spec_inv", "spec_vis"):
+        g, c = gpu.get_field(name)[:nc], cpu.get(name)[:nc]
+        assert rel_err(g, c, 1e-300) <= TOL, name
+    if viscous:
+        for name in ("dtdx", "dtdy", "dtdz"):
+            g, c = gpu.get_field(name)[:nc], cpu.get(name)[:nc]
+            assert rel_err(g, c, 1e-12 * np.max(np.abs(c))) <= TOL, name
+    gpu.time_step(2)
+    cpu.time_step(2)
+    assert gpu.lusgs() == cpu.lusgs()
+    qg, qc = gpu.get_state()[:, :nc], cpu.get("q")[:, :nc]
+    assert rel_err(qg, qc, 1e-300) <= TOL
+
+
+@pytest.mark.parametrize("viscous", [0, 1])
+def test_hybrid_mesh_iterations_parity(viscous):
+    m, gpu, cpu = make_hybrid_pair(mach=0.3, viscous=viscous, reynolds=5e5, cfl_start=5.0, cfl_end=5.0)
+    gpu.init_freestream()
+    cpu.init_freestream()
+    hg = gpu.iterate(1, 8)
+    hc, _ = cpu.iterate(1, 8)
+    assert np.max(np.abs(hg - hc) / hc) <= 1e-10
+    nc = m.n_cell
+    qg, qc = gpu.get_state()[:, :nc], cpu.get("q")[:, :nc]
+    dq = qc - cpu.freestream()[:, None]
+    err = np.max(np.abs(qg - qc) / np.maximum(np.max(np.abs(dq), axis=1, keepdims=True), 1e-300))
+    print("hybrid viscous", viscous, "state rel err", err)
+    assert err <= 1e-9
