@@ -92,6 +92,8 @@ def main():
     ap.add_argument("--config", default=DEFAULT_CONFIG, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="also run the HBM copy probe")
+    ap.add_argument("--launch", default="graph", choices=["graph", "eager"],
+                    help="replay captured hipGraphs (default) or launch every kernel eagerly")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -102,6 +104,7 @@ def main():
     mesh = build_mesh(cfg["mesh"])
     params = pa.default_params(**cfg["params"])
     solver = pa.Solver(mesh, params, device=0)
+    solver.set_launch_mode(args.launch)
     solver.init_freestream()
     print(f"[bench] {mesh.n_cell} cells, {mesh.n_face} faces; warmup {args.warmup}", file=sys.stderr, flush=True)
     solver.iterate(1, args.warmup, history=False)
@@ -136,7 +139,8 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "cells": mesh.n_cell, "faces": mesh.n_face,
-                   "precision": "fp64", "parallelism": "single GPU"},
+                   "precision": "fp64", "parallelism": "single GPU",
+                   "launch": args.launch},
         "roofline": roof,
         "kernel_ms_per_step": per_class,
     }

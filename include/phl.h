@@ -60,7 +60,7 @@ typedef struct phl_params {
     int entropy_fix;        /* roeEntropyFixMethod (3) */
     int cfl_vary_step;      /* CFLVaryStep (500) */
     int n_lusgs_sweeps;     /* nLUSGSSweeps (1) */
-    int precision;          /* 0 FP64; 1 mixed (FP32 faces, FP64 accumulation/LU-SGS) */
+    int precision;          /* 0 FP64; 1 mixed (FP32 gradients and face fluxes, FP64 accumulation/LU-SGS) */
     int reserved;
 } phl_params;
 
@@ -171,13 +171,43 @@ phl_status phl_lusgs(phl_ctx *ctx, int *n_fixed);
  * sum over real cells of res_m^2. */
 phl_status phl_residual_norm(phl_ctx *ctx, double *sum5);
 
+/* Wall force and moment coefficients: NSSolverUnstruct::AirForceCoefParts
+ * over all solid-surface faces (:16026-16250) and Force::CompForce
+ * (API/Physics/src/Force.cpp:439-533).  Boundary states, temperature,
+ * viscosity and gradients are first refreshed from the current state.
+ * coef[PHL_N_FORCE]: the part sums scaled as CompForce scales them (CA, CN,
+ * CZ friction/pressure; pressure-only cpx/cpy/cpz; rolling, yawing and
+ * pitching moments, friction/pressure), then the totals. */
+enum {
+    PHL_FORCE_CA_F = 0, PHL_FORCE_CA_P = 1, PHL_FORCE_CN_F = 2, PHL_FORCE_CN_P = 3,
+    PHL_FORCE_CZ_F = 4, PHL_FORCE_CZ_P = 5, PHL_FORCE_CPX = 6, PHL_FORCE_CPY = 7, PHL_FORCE_CPZ = 8,
+    PHL_FORCE_CMX_F = 9, PHL_FORCE_CMX_P = 10, PHL_FORCE_CMY_F = 11, PHL_FORCE_CMY_P = 12,
+    PHL_FORCE_CMZ_F = 13, PHL_FORCE_CMZ_P = 14,
+    PHL_FORCE_CL = 15, PHL_FORCE_CD = 16, PHL_FORCE_SIDE = 17, PHL_FORCE_CD_PR = 18,
+    PHL_FORCE_CD_SF = 19, PHL_FORCE_CD_CL2PA = 20, PHL_FORCE_XCP = 21, PHL_N_FORCE = 22
+};
+phl_status phl_forces(phl_ctx *ctx, double *coef);
+
 /* ---------------- host driver (Controller::Relaxation loop) ----------- */
 /* n_steps outer iterations numbered first_step..; rms_hist[i] (may be NULL)
  * receives the density-residual RMS of each step (Residual.cpp:221-224). */
 phl_status phl_iterate(phl_ctx *ctx, int first_step, int n_steps, double *rms_hist);
-/* Iterate until R_k/R_1 <= drop or max_steps; returns the step count. */
+/* Iterate until R_k/R_1 <= drop or max_steps.  The residual is checked every
+ * 32 steps, so the run ends on the first multiple of 32 steps at or after the
+ * step that reached the drop; steps_done is the number of steps run (the
+ * state returned is after that many), rms_hist[0..steps_done-1] the history. */
 phl_status phl_solve(phl_ctx *ctx, int max_steps, double drop, int *steps_done, double *rms_hist);
 phl_status phl_synchronize(phl_ctx *ctx);
+/* How phl_iterate / phl_solve launch (no reference counterpart: the
+ * reference launches every kernel eagerly on the default stream).  mode 1
+ * (default) replays two-step hipGraphs; mode 0 launches eagerly.  Results are
+ * identical; kernel timing always launches eagerly. */
+phl_status phl_set_launch_mode(phl_ctx *ctx, int mode);
+/* Switch between the FP64 form (0) and the mixed form (1: FP32 gradients,
+ * reconstruction, Roe and viscous fluxes; FP64 residual accumulation,
+ * spectral radii, time step, LU-SGS and norms) on a live context, e.g. to
+ * finish a mixed-form solve in FP64.  The reference has FP64 only. */
+phl_status phl_set_precision(phl_ctx *ctx, int precision);
 
 /* ---------------- measurement (GPU/Manage/src/TimeMeasure.cu:15-67) ----
  * With timing on, phl_iterate records HIP events on the context's stream

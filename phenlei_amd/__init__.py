@@ -69,6 +69,12 @@ def default_params(**overrides):
     return p
 
 
+# Order of phl_forces' output (PHL_FORCE_* in include/phl.h).
+FORCE_NAMES = ("CA_f", "CA_p", "CN_f", "CN_p", "CZ_f", "CZ_p", "cpx", "cpy", "cpz",
+               "Cmx_f", "Cmx_p", "Cmy_f", "Cmy_p", "Cmz_f", "Cmz_p",
+               "CL", "CD", "side", "CD_pr", "CD_sf", "CD_CL2PA", "xcp")
+
+
 class PhlError(RuntimeError):
     def __init__(self, status, msg):
         super().__init__(f"phl status {status}: {msg}")
@@ -116,9 +122,12 @@ def _declare(lib):
         "phl_time_step": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_lusgs": (ctypes.c_int, [_P, _I]),
         "phl_residual_norm": (ctypes.c_int, [_P, _D]),
+        "phl_forces": (ctypes.c_int, [_P, _D]),
         "phl_iterate": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
         "phl_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
         "phl_synchronize": (ctypes.c_int, [_P]),
+        "phl_set_launch_mode": (ctypes.c_int, [_P, ctypes.c_int]),
+        "phl_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_get_kernel_timing": (ctypes.c_int, [_P, _D, _I]),
         "phl_hbm_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -332,6 +341,20 @@ class Solver:
         _check(_lib.phl_residual_norm(self._h, _dptr(out)))
         return out
 
+    def set_launch_mode(self, mode):
+        """'graph' (default) replays captured two-step hipGraphs; 'eager' launches each kernel."""
+        _check(_lib.phl_set_launch_mode(self._h, {"eager": 0, "graph": 1}[mode]))
+
+    def set_precision(self, precision):
+        """'fp64' or 'mixed' (FP32 gradients and face fluxes, FP64 accumulation)."""
+        _check(_lib.phl_set_precision(self._h, {"fp64": 0, "mixed": 1}[precision]))
+
+    def forces(self):
+        """Wall force coefficients (phl_forces) as a dict keyed by FORCE_NAMES."""
+        out = np.empty(len(FORCE_NAMES))
+        _check(_lib.phl_forces(self._h, _dptr(out)))
+        return dict(zip(FORCE_NAMES, out))
+
     def iterate(self, first_step, n_steps, history=True):
         hist = np.empty(max(n_steps, 1))
         _check(_lib.phl_iterate(self._h, first_step, n_steps, _dptr(hist) if history else None))
@@ -341,7 +364,7 @@ class Solver:
         hist = np.empty(max_steps)
         n = ctypes.c_int()
         _check(_lib.phl_solve(self._h, max_steps, drop, ctypes.byref(n), _dptr(hist)))
-        return n.value, hist
+        return n.value, hist[:n.value]
 
     def synchronize(self):
         _check(_lib.phl_synchronize(self._h))

@@ -38,6 +38,10 @@ struct phl_ctx {
     std::vector<hipEvent_t> events;
     double kernelMs[PHL_N_KERNEL_CLASSES] = {0};
     int timedSteps = 0;
+    // two-step hipGraphs, one per assignment of the q / qAlt buffers
+    int launchMode = 1;  // 0 eager, 1 graph
+    double *qBuf0 = nullptr;
+    hipGraphExec_t stepPair[2] = {nullptr, nullptr};
 };
 
 namespace {
@@ -126,6 +130,40 @@ void build_levels(const phl::Mesh &m, const std::vector<int> &c2fPtr, const std:
     for (int c = 0; c < nC; ++c) cells[fill[level[c]]++] = c;
 }
 
+// Position-major active-face lists of one sweep direction: for position p
+// (cell cells[p]) the faces whose neighbour precedes the cell in the sweep
+// (lower index forward; higher index or ghost backward), in the cell's face
+// order, as coefficient rows 2*f + side and neighbour indices.
+// The lists are padded to Wpad slots (3, 4 or 6: the widths the sweep
+// kernels are compiled for) with row -1.
+void build_sweep_order(int nC, const std::vector<int> &c2fPtr, const std::vector<int> &c2f,
+                       const std::vector<int> &c2n, const std::vector<char> &isLeft, const std::vector<int> &cells,
+                       bool forward, int &W, int &Wpad, std::vector<int> &row, std::vector<int> &nb)
+{
+    W = 0;
+    for (int c = 0; c < nC; ++c) {
+        int n = 0;
+        for (int k = c2fPtr[c]; k < c2fPtr[c + 1]; ++k) n += forward ? (c2n[k] < c) : (c2n[k] > c);
+        W = std::max(W, n);
+    }
+    Wpad = W <= 3 ? 3 : (W <= 4 ? 4 : (W <= 6 ? 6 : W));
+    const int Wa = Wpad;
+    row.assign((size_t)Wa * nC, -1);
+    nb.assign((size_t)Wa * nC, 0);
+    for (int p = 0; p < nC; ++p) {
+        const int c = cells[p];
+        int s = 0;
+        for (int k = c2fPtr[c]; k < c2fPtr[c + 1]; ++k) {
+            const int n = c2n[k];
+            if (forward ? (n < c) : (n > c)) {
+                row[(size_t)s * nC + p] = 2 * c2f[k] + (isLeft[k] ? 0 : 1);
+                nb[(size_t)s * nC + p] = n;
+                ++s;
+            }
+        }
+    }
+}
+
 double compute_cfl(const phl_params &p, int step)
 {
     // NSSolver::ComputeCFL (CFD/NSSolver/src/NSSolver.cpp:595-668), CFLMethod 0.
@@ -173,7 +211,8 @@ phl_status check_params(const phl_params &p)
     if (p.entropy_fix != 1 && p.entropy_fix != 3 && p.entropy_fix != 5)
         return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1, 3 or 5");
     if (p.n_lusgs_sweeps != 1) return fail(PHL_ERR_UNSUPPORTED, "nLUSGSSweeps must be 1");
-    if (p.precision != 0) return fail(PHL_ERR_UNSUPPORTED, "precision must be 0 (FP64)");
+    if (p.precision != 0 && p.precision != 1)
+        return fail(PHL_ERR_UNSUPPORTED, "precision must be 0 (FP64) or 1 (mixed)");
     if (p.t_wall > 0.0) return fail(PHL_ERR_UNSUPPORTED, "isothermal walls are not implemented (t_wall <= 0)");
     if (p.cfl_vary_step <= 0) return fail(PHL_ERR_ARG, "cfl_vary_step must be positive");
     return PHL_OK;
@@ -198,6 +237,19 @@ phl_status copy_field(phl_ctx *c, const double *dsrc, int rows, int rowLen, int 
         for (int i = rowLen; i < outRowLen; ++i) o[i] = 0.0;
     }
     HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
+}
+
+// FP32 rows (mixed-form gradients) widened to FP64 on the host.
+phl_status copy_field_f32(phl_ctx *c, const float *dsrc, int rows, int rowLen, int outRowLen, double *out)
+{
+    std::vector<float> tmp((size_t)rows * rowLen);
+    if (!tmp.empty())
+        HIPTRY(hipMemcpyAsync(tmp.data(), dsrc, tmp.size() * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    for (int r = 0; r < rows; ++r)
+        for (int i = 0; i < outRowLen; ++i)
+            out[(size_t)r * outRowLen + i] = i < rowLen ? (double)tmp[(size_t)r * rowLen + i] : 0.0;
     return PHL_OK;
 }
 
@@ -400,6 +452,8 @@ void phl_destroy(phl_ctx *c)
     if (c->hostCfl) hipHostFree(c->hostCfl);
     if (c->hostZero) hipHostFree(c->hostZero);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
+    for (hipGraphExec_t g : c->stepPair)
+        if (g) hipGraphExecDestroy(g);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -499,6 +553,12 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(upload(c, &dm.limZero, limZero));
     dm.limGhost = nullptr;
     TRY(upload(c, &dm.visFlag, visFlag));
+    std::vector<int> wall;
+    for (int f = 0; f < nB; ++f)
+        if (m.bcType[f] == PHL_BC_SOLID_SURFACE) wall.push_back(f);
+    dm.nWall = (int)wall.size();
+    if (wall.empty()) wall.push_back(0);  // keeps the device pointer valid
+    TRY(upload(c, &dm.wallFaces, wall));
 
     // LU-SGS level schedules
     std::vector<int> cellsF, cellsB;
@@ -511,6 +571,15 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(upload(c, &tmp, cellsB)); c->bwd.cells = const_cast<int *>(tmp);
     c->fwd.nLevels = (int)c->hfwdPtr.size() - 1; c->fwd.hptr = c->hfwdPtr.data();
     c->bwd.nLevels = (int)c->hbwdPtr.size() - 1; c->bwd.hptr = c->hbwdPtr.data();
+    for (int dir = 0; dir < 2; ++dir) {
+        phl::Levels &lv = dir == 0 ? c->fwd : c->bwd;
+        std::vector<int> row, nbs;
+        build_sweep_order(nC, c2fPtr, c2f, c2n, isLeft, dir == 0 ? cellsF : cellsB, dir == 0, lv.so.W, lv.so.Wpad,
+                          row, nbs);
+        lv.so.cells = lv.cells;
+        TRY(upload(c, &lv.so.row, row));
+        TRY(upload(c, &lv.so.nb, nbs));
+    }
 
     // constants
     phl::Consts &k = c->k;
@@ -550,14 +619,18 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.limiter = p.limiter;
     k.entropyFix = p.entropy_fix;
     k.nSweeps = p.n_lusgs_sweeps;
+    k.mixed = p.precision;
 
     // flow fields
     phl::DevFlow &fl = c->f;
     TRY(dalloc(c, &fl.q, (size_t)5 * nT));
     TRY(dalloc(c, &fl.qAlt, (size_t)5 * nT));
+    c->qBuf0 = fl.q;
     TRY(dalloc(c, &fl.res, (size_t)5 * nC));
     TRY(dalloc(c, &fl.grad, (size_t)15 * nC));
     TRY(dalloc(c, &fl.gradT, (size_t)3 * nC));
+    TRY(dalloc(c, &fl.gradF, (size_t)15 * nC));
+    TRY(dalloc(c, &fl.gradTF, (size_t)3 * nC));
     TRY(dalloc(c, &fl.lim, (size_t)nT));
     TRY(dalloc(c, &fl.dq, (size_t)5 * nT));
     TRY(dalloc(c, &fl.dflux, (size_t)5 * nC));
@@ -579,6 +652,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.coef, (size_t)nF * 2 * 16));
     TRY(dalloc(c, &fl.t, (size_t)nT));
     TRY(dalloc(c, &fl.visl, (size_t)nT));
+    TRY(dalloc(c, &fl.forceOut, 15));
     HIPTRY(hipMemset(fl.q, 0, sizeof(double) * 5 * nT));
     HIPTRY(hipMemset(fl.qAlt, 0, sizeof(double) * 5 * nT));
     HIPTRY(hipMemset(fl.res, 0, sizeof(double) * 5 * nC));
@@ -656,9 +730,13 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
     switch (field) {
     case PHL_FIELD_Q: return copy_field(c, f.q, 5, nT, nT, out);
     case PHL_FIELD_RES: return copy_field(c, f.res, 5, nC, nT, out);
-    case PHL_FIELD_DQDX: return copy_field(c, f.grad, 5, nC, nT, out);
-    case PHL_FIELD_DQDY: return copy_field(c, f.grad + (size_t)5 * nC, 5, nC, nT, out);
-    case PHL_FIELD_DQDZ: return copy_field(c, f.grad + (size_t)10 * nC, 5, nC, nT, out);
+    case PHL_FIELD_DQDX:
+    case PHL_FIELD_DQDY:
+    case PHL_FIELD_DQDZ: {
+        const size_t off = (size_t)5 * (field - PHL_FIELD_DQDX) * nC;
+        if (c->k.mixed) return copy_field_f32(c, f.gradF + off, 5, nC, nT, out);
+        return copy_field(c, f.grad + off, 5, nC, nT, out);
+    }
     case PHL_FIELD_LIMIT: return copy_field(c, f.lim, 1, nT, nT, out);
     case PHL_FIELD_DQ: return copy_field(c, f.dq, 5, nT, nT, out);
     case PHL_FIELD_SPEC_INV: return copy_field(c, f.specInv, 1, nC, nT, out);
@@ -670,9 +748,13 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
         phl::launch_gamma_t(c->m, c->k, f, c->stream);
         TRY(check_launch(c));
         return copy_field(c, field == PHL_FIELD_T ? f.t : f.visl, 1, nT, nT, out);
-    case PHL_FIELD_DTDX: return copy_field(c, f.gradT, 1, nC, nT, out);
-    case PHL_FIELD_DTDY: return copy_field(c, f.gradT + nC, 1, nC, nT, out);
-    case PHL_FIELD_DTDZ: return copy_field(c, f.gradT + (size_t)2 * nC, 1, nC, nT, out);
+    case PHL_FIELD_DTDX:
+    case PHL_FIELD_DTDY:
+    case PHL_FIELD_DTDZ: {
+        const size_t off = (size_t)(field - PHL_FIELD_DTDX) * nC;
+        if (c->k.mixed) return copy_field_f32(c, f.gradTF + off, 1, nC, nT, out);
+        return copy_field(c, f.gradT + off, 1, nC, nT, out);
+    }
     case PHL_FIELD_DELTA_FLUX: return copy_field(c, f.dflux, 5, nC, nT, out);
     default: return fail(PHL_ERR_ARG, "phl_get_field: unknown field");
     }
@@ -788,6 +870,54 @@ phl_status phl_residual_norm(phl_ctx *c, double *sum5)
     return PHL_OK;
 }
 
+// Force::CompForce (API/Physics/src/Force.cpp:439-533) for the all-wall
+// part: scale the sums by the reference area and lengths, then lift, drag,
+// side force, pressure and friction drag, induced-drag estimate and centre
+// of pressure.
+static void force_coefficients(const phl_params &p, const double *sums, double *out)
+{
+    const double kPiD = 3.141592653589793;
+    const double scf = 1.0 / p.force_ref_area;
+    const double vcm = 1.0 / (p.force_ref_area * p.force_ref_length);
+    const double sb = 1.0 / (p.force_ref_area * p.force_ref_span);
+    for (int i = 0; i < 9; ++i) out[i] = sums[i] * scf;
+    for (int i = 9; i < 13; ++i) out[i] = sums[i] * sb;
+    for (int i = 13; i < 15; ++i) out[i] = sums[i] * vcm;
+    const double attack = p.alpha_deg * kPiD / 180.0, sideslip = p.beta_deg * kPiD / 180.0;
+    const double sina = std::sin(attack), cosa = std::cos(attack);
+    const double sinb = std::sin(sideslip), cosb = std::cos(sideslip);
+    const double cfx = out[0] + out[1], cfy = out[2] + out[3], cfz = out[4] + out[5];
+    const double cmz = out[13] + out[14];
+    const double cl = -sina * cfx + cosa * cfy;
+    const double cd = cosa * cosb * cfx + sina * cosb * cfy + sinb * cfz;
+    const double cdp = cosa * cosb * out[6] + sina * cosb * out[7] + sinb * out[8];
+    const double AR = 9.5;
+    out[PHL_FORCE_CL] = cl;
+    out[PHL_FORCE_CD] = cd;
+    out[PHL_FORCE_SIDE] = -cosa * sinb * cfx - sina * sinb * cfy + cosb * cfz;
+    out[PHL_FORCE_CD_PR] = cdp;
+    out[PHL_FORCE_CD_SF] = cd - cdp;
+    out[PHL_FORCE_CD_CL2PA] = cd - (cl * cl) / (kPiD * AR);
+    out[PHL_FORCE_XCP] = (cfy < 0.0 ? -1.0 : 1.0) * cmz / (std::fabs(cfy) + 1.0e-40);
+}
+
+phl_status phl_forces(phl_ctx *c, double *coef)
+{
+    if (!c || !coef) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    hipStream_t s = c->stream;
+    phl::launch_boundary(c->m, c->k, c->f, s);
+    phl::launch_gamma_t(c->m, c->k, c->f, s);
+    phl::launch_grad_lim(c->m, c->k, c->f, s, true);
+    const double ref[3] = {c->p.force_ref_x, c->p.force_ref_y, c->p.force_ref_z};
+    phl::launch_forces(c->m, c->k, c->f, ref, s);
+    HIPTRY(hipGetLastError());
+    HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.forceOut, sizeof(double) * 15, hipMemcpyDeviceToHost, s));
+    HIPTRY(hipStreamSynchronize(s));
+    force_coefficients(c->p, c->hostNorm, coef);
+    return PHL_OK;
+}
+
 // ----------------------------------------------------------- host driver
 static const int kTimedChunk = 64;
 
@@ -815,6 +945,52 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     swap_state(c);
 }
 
+// Two consecutive steps captured as one hipGraph.  A step swaps q and qAlt,
+// so after two steps the buffers are back where they started and the graph
+// can be replayed; one graph is kept for each assignment of the buffers.
+// The captured sequence holds kernels only.
+static phl_status step_pair_graph(phl_ctx *c, hipGraphExec_t *out)
+{
+    const int parity = c->f.q == c->qBuf0 ? 0 : 1;
+    if (!c->stepPair[parity]) {
+        hipGraph_t g = nullptr;
+        HIPTRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        enqueue_step(c, nullptr);
+        enqueue_step(c, nullptr);
+        HIPTRY(hipStreamEndCapture(c->stream, &g));
+        const hipError_t e = hipGraphInstantiate(&c->stepPair[parity], g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        if (e != hipSuccess) {
+            c->stepPair[parity] = nullptr;
+            return fail(PHL_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+        }
+    }
+    *out = c->stepPair[parity];
+    return PHL_OK;
+}
+
+phl_status phl_set_precision(phl_ctx *c, int precision)
+{
+    if (!c || (precision != 0 && precision != 1)) return fail(PHL_ERR_ARG, "precision must be 0 or 1");
+    if (c->k.mixed == precision) return PHL_OK;
+    HIPTRY(hipSetDevice(c->device));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    c->k.mixed = precision;
+    c->p.precision = precision;
+    for (hipGraphExec_t &g : c->stepPair) {  // the graphs hold the other kernels
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    return PHL_OK;
+}
+
+phl_status phl_set_launch_mode(phl_ctx *c, int mode)
+{
+    if (!c || (mode != 0 && mode != 1)) return fail(PHL_ERR_ARG, "launch mode must be 0 (eager) or 1 (graph)");
+    c->launchMode = mode;
+    return PHL_OK;
+}
+
 phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist)
 {
     if (!c || n_steps < 0) return fail(PHL_ERR_ARG, "bad argument");
@@ -834,7 +1010,15 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         for (int i = 0; i < chunk; ++i) c->hostCfl[i] = compute_cfl(c->p, first_step + done + i);
         HIPTRY(hipMemcpyAsync(c->f.cflArr, c->hostCfl, sizeof(double) * chunk, hipMemcpyHostToDevice, c->stream));
         reset_step_counter(c);
-        for (int i = 0; i < chunk; ++i)
+        int i = 0;
+        if (!c->timing && c->launchMode == 1) {
+            for (; i + 2 <= chunk; i += 2) {
+                hipGraphExec_t g = nullptr;
+                TRY(step_pair_graph(c, &g));
+                HIPTRY(hipGraphLaunch(g, c->stream));
+            }
+        }
+        for (; i < chunk; ++i)
             enqueue_step(c, c->timing ? &c->events[(size_t)i * (K + 1)] : nullptr);
         TRY(check_launch(c));
         HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5 * chunk, hipMemcpyDeviceToHost,
@@ -914,7 +1098,7 @@ phl_status phl_solve(phl_ctx *c, int max_steps, double drop, int *steps_done, do
 {
     if (!c || max_steps <= 0 || !(drop > 0.0)) return fail(PHL_ERR_ARG, "bad argument");
     std::vector<double> hist(max_steps);
-    const int chunk = 25;
+    const int chunk = 32;  // steps between host checks of the residual history
     int done = 0;
     double r1 = 0.0;
     int hit = -1;
@@ -926,7 +1110,7 @@ phl_status phl_solve(phl_ctx *c, int max_steps, double drop, int *steps_done, do
             if (hist[i] <= drop * r1) { hit = i; break; }
         done += n;
     }
-    if (steps_done) *steps_done = (hit >= 0) ? hit + 1 : done;
+    if (steps_done) *steps_done = done;
     if (rms_hist) std::copy(hist.begin(), hist.begin() + done, rms_hist);
     return PHL_OK;
 }

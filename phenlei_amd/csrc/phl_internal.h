@@ -34,6 +34,8 @@ struct DevMesh {
     const unsigned char *limZero;  // [nCell] 1 if next to a farfield-type face
     const double *limGhost;     // [nBFace] limiter value of the ghost cell
     const unsigned char *visFlag;  // [nFace] bit0: skewness correction, bit1: normal derivative
+    const int *wallFaces;       // [nWall] solid-surface faces, increasing
+    int nWall;
 };
 
 // Scalars of the path, fixed per context.
@@ -49,6 +51,7 @@ struct Consts {
     double oprl, oprt;
     double densLimit, presLimit;  // 1e-6 freestream
     int viscous, limiter, entropyFix, nSweeps;
+    int mixed;              // 1: FP32 gradients and face fluxes, FP64 accumulation
 };
 
 // Mutable flow state on the device.
@@ -58,6 +61,8 @@ struct DevFlow {
     double *res;      // [5][nCell]
     double *grad;     // [15][nCell]: d/dx of m at [m], d/dy at [5+m], d/dz at [10+m]
     double *gradT;    // [3][nCell] (viscous)
+    float *gradF;     // [15][nCell] mixed form
+    float *gradTF;    // [3][nCell] mixed form
     double *lim;      // [nTotal]
     double *dq;       // [5][nTotal]
     double *dflux;    // [5][nCell]
@@ -73,20 +78,34 @@ struct DevFlow {
     int *negTotal;      // running SolutionFix count
     double *coef;       // [nFace][2][16] LU-SGS face coefficients per side
     double *t, *visl;   // [nTotal], only for the standalone gamma/T hook
+    double *forceOut;   // [15] wall force and moment sums
 };
 
 // LU-SGS level schedule.
+// Sweep order of one direction, position-major: position p of the level
+// order holds cell cells[p] and its active faces (lower neighbours forward,
+// higher ones and ghosts backward) in increasing face order.
+struct SweepOrder {
+    int W;              // most active faces of any cell
+    int Wpad;           // slots stored: 3, 4 or 6 (>= W), or W when W > 6
+    const int *cells;   // [nCell]
+    const int *row;     // [Wpad][nCell]: 2*face + coefficient side, -1 if unused
+    const int *nb;      // [Wpad][nCell]: neighbour cell (0 if unused)
+};
+
 struct Levels {
     int nLevels;
     int *ptr;    // device [nLevels+1]
     int *cells;  // device [nCell]
     int *hptr;   // host copy of ptr
+    SweepOrder so;
 };
 
 // Launchers (phl_kernels.hip).  All are asynchronous on `s`.
 void launch_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
-void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+// fp64Grad: store FP64 gradients even in the mixed form (forces, field reads).
+void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad = false);
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt);
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 int flux_blocks(const DevMesh &m);
@@ -98,5 +117,6 @@ void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &l
 void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, hipStream_t s);
 void launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
+void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *torqueRef, hipStream_t s);
 
 }  // namespace phl

@@ -33,6 +33,8 @@ constexpr int kBlock = 256;
 __device__ __forceinline__ double dMIN(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double dMAX(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double dABS(double a) { return (a < 0) ? -a : a; }
+template <typename T> __device__ __forceinline__ T tMAX(T a, T b) { return (a > b) ? a : b; }
+template <typename T> __device__ __forceinline__ T tABS(T a) { return (a < T(0)) ? -a : a; }
 
 // Blocks are dispatched round-robin over the 8 XCDs (block b lands on XCD
 // b % 8).  Give each XCD a contiguous range of cells so that neighbouring
@@ -44,88 +46,92 @@ __device__ __forceinline__ int xcd_block(int b, int nblk)
     return x * q + (x < r ? x : r) + i;
 }
 
-__device__ __forceinline__ double harten(double eig, double lim)
+template <typename T>
+__device__ __forceinline__ T harten(T eig, T lim)
 {
-    double e = dABS(eig);
-    if (e < lim) e = 0.5 * (eig * eig + lim * lim) / lim;
+    T e = tABS(eig);
+    if (e < lim) e = T(0.5) * (eig * eig + lim * lim) / lim;
     return e;
 }
 
 // Roe flux per unit area, conservative form, static mesh, perfect gas.
-__device__ void roe_flux(const double *qLin, const double *qRin, double xsn, double ysn, double zsn,
-                         double gama_, int fixMethod, double ent, double *flux)
+// T is double for the FP64 form and float for the mixed form; the double
+// instantiation performs exactly the operations of the reference.
+template <typename T>
+__device__ void roe_flux(const T *qLin, const T *qRin, T xsn, T ysn, T zsn, T gama_, int fixMethod, T ent,
+                         T *flux)
 {
-    const double LITTLE = 1.0E-10;
-    const double gmL = gama_, gmR = gama_;
-    double rL = qLin[0], uL = qLin[1], vL = qLin[2], wL = qLin[3], pL = qLin[4];
-    double rR = qRin[0], uR = qRin[1], vR = qRin[2], wR = qRin[3], pR = qRin[4];
-    rL = dMAX(rL, LITTLE); rR = dMAX(rR, LITTLE);
-    pL = dMAX(pL, LITTLE); pR = dMAX(pR, LITTLE);
-    const double v2L = uL * uL + vL * vL + wL * wL;
-    const double v2R = uR * uR + vR * vR + wR * wR;
-    const double hintL = (gmL / (gmL - 1.0)) * (pL / rL);
-    const double hintR = (gmR / (gmR - 1.0)) * (pR / rR);
-    const double hL = hintL + 0.5 * v2L;
-    const double hR = hintR + 0.5 * v2R;
-    const double vb = 0.0;
-    const double vnL = xsn * uL + ysn * vL + zsn * wL - vb;
-    const double vnR = xsn * uR + ysn * vR + zsn * wR - vb;
-    const double rvnL = rL * vnL, rvnR = rR * vnR;
-    flux[0] = 0.5 * (rvnL + rvnR);
-    flux[1] = 0.5 * (rvnL * uL + xsn * pL + rvnR * uR + xsn * pR);
-    flux[2] = 0.5 * (rvnL * vL + ysn * pL + rvnR * vR + ysn * pR);
-    flux[3] = 0.5 * (rvnL * wL + zsn * pL + rvnR * wR + zsn * pR);
-    flux[4] = 0.5 * (rvnL * hL + vb * pL + rvnR * hR + vb * pR);
+    const T LITTLE = T(1.0E-10);
+    const T gmL = gama_, gmR = gama_;
+    T rL = qLin[0], uL = qLin[1], vL = qLin[2], wL = qLin[3], pL = qLin[4];
+    T rR = qRin[0], uR = qRin[1], vR = qRin[2], wR = qRin[3], pR = qRin[4];
+    rL = tMAX(rL, LITTLE); rR = tMAX(rR, LITTLE);
+    pL = tMAX(pL, LITTLE); pR = tMAX(pR, LITTLE);
+    const T v2L = uL * uL + vL * vL + wL * wL;
+    const T v2R = uR * uR + vR * vR + wR * wR;
+    const T hintL = (gmL / (gmL - T(1.0))) * (pL / rL);
+    const T hintR = (gmR / (gmR - T(1.0))) * (pR / rR);
+    const T hL = hintL + T(0.5) * v2L;
+    const T hR = hintR + T(0.5) * v2R;
+    const T vb = T(0.0);
+    const T vnL = xsn * uL + ysn * vL + zsn * wL - vb;
+    const T vnR = xsn * uR + ysn * vR + zsn * wR - vb;
+    const T rvnL = rL * vnL, rvnR = rR * vnR;
+    flux[0] = T(0.5) * (rvnL + rvnR);
+    flux[1] = T(0.5) * (rvnL * uL + xsn * pL + rvnR * uR + xsn * pR);
+    flux[2] = T(0.5) * (rvnL * vL + ysn * pL + rvnR * vR + ysn * pR);
+    flux[3] = T(0.5) * (rvnL * wL + zsn * pL + rvnR * wR + zsn * pR);
+    flux[4] = T(0.5) * (rvnL * hL + vb * pL + rvnR * hR + vb * pR);
 
-    const double ratio = sqrt(rR / rL);
-    const double coef = 1.0 / (1.0 + ratio);
-    const double rRoe = sqrt(rL * rR);
-    const double uRoe = (uL + uR * ratio) * coef;
-    const double vRoe = (vL + vR * ratio) * coef;
-    const double wRoe = (wL + wR * ratio) * coef;
-    const double pRoe = (pL + pR * ratio) * coef;
-    const double gama = (gmL + gmR * ratio) * coef;
-    const double gamm1 = gama - 1.0;
-    const double v2Roe = uRoe * uRoe + vRoe * vRoe + wRoe * wRoe;
-    const double hRoe = gama / gamm1 * pRoe / rRoe + 0.5 * v2Roe;
-    const double vn = xsn * uRoe + ysn * vRoe + zsn * wRoe - vb;
-    const double theta = vn + vb;
-    const double c2 = gamm1 * (hRoe - 0.5 * v2Roe);
-    const double cm = sqrt(dABS(c2));
-    double e1 = dABS(vn), e2 = dABS(vn + cm), e3 = dABS(vn - cm);
-    const double EL = (1.0 / (gmL - 1.0)) * (pL / rL) + 0.5 * v2L;
-    const double ER = (1.0 / (gmR - 1.0)) * (pR / rR) + 0.5 * v2R;
-    const double dq0 = rR - rL;
-    const double dq1 = rR * uR - rL * uL;
-    const double dq2 = rR * vR - rL * vL;
-    const double dq3 = rR * wR - rL * wL;
-    const double dq4 = rR * ER - rL * EL;
+    const T ratio = sqrt(rR / rL);
+    const T coef = T(1.0) / (T(1.0) + ratio);
+    const T rRoe = sqrt(rL * rR);
+    const T uRoe = (uL + uR * ratio) * coef;
+    const T vRoe = (vL + vR * ratio) * coef;
+    const T wRoe = (wL + wR * ratio) * coef;
+    const T pRoe = (pL + pR * ratio) * coef;
+    const T gama = (gmL + gmR * ratio) * coef;
+    const T gamm1 = gama - T(1.0);
+    const T v2Roe = uRoe * uRoe + vRoe * vRoe + wRoe * wRoe;
+    const T hRoe = gama / gamm1 * pRoe / rRoe + T(0.5) * v2Roe;
+    const T vn = xsn * uRoe + ysn * vRoe + zsn * wRoe - vb;
+    const T theta = vn + vb;
+    const T c2 = gamm1 * (hRoe - T(0.5) * v2Roe);
+    const T cm = sqrt(tABS(c2));
+    T e1 = tABS(vn), e2 = tABS(vn + cm), e3 = tABS(vn - cm);
+    const T EL = (T(1.0) / (gmL - T(1.0))) * (pL / rL) + T(0.5) * v2L;
+    const T ER = (T(1.0) / (gmR - T(1.0))) * (pR / rR) + T(0.5) * v2R;
+    const T dq0 = rR - rL;
+    const T dq1 = rR * uR - rL * uL;
+    const T dq2 = rR * vR - rL * vL;
+    const T dq3 = rR * wR - rL * wL;
+    const T dq4 = rR * ER - rL * EL;
     if (fixMethod == 1) {
-        const double mx = dMAX(e2, e3);
-        const double t0 = mx * ent, t1 = mx * ent;
-        e1 = dMAX(t0, e1); e2 = dMAX(t1, e2); e3 = dMAX(t1, e3);
+        const T mx = tMAX(e2, e3);
+        const T t0 = mx * ent, t1 = mx * ent;
+        e1 = tMAX(t0, e1); e2 = tMAX(t1, e2); e3 = tMAX(t1, e3);
     } else if (fixMethod == 3) {
-        const double me = dABS(vn) + cm;
-        const double al = ent * me;
-        const double cl = ent * me;
+        const T me = tABS(vn) + cm;
+        const T al = ent * me;
+        const T cl = ent * me;
         e1 = harten(e1, cl);
         e2 = harten(e2, al);
         e3 = harten(e3, al);
     } else if (fixMethod == 5) {
-        const double mx = dMAX(e2, e3);
-        const double d = mx * ent;
+        const T mx = tMAX(e2, e3);
+        const T d = mx * ent;
         e1 += d; e2 += d; e3 += d;
     }
-    const double xi1 = (2.0 * e1 - e2 - e3) / (2.0 * c2);
-    const double xi2 = (e2 - e3) / (2.0 * cm);
-    const double dc = theta * dq0 - xsn * dq1 - ysn * dq2 - zsn * dq3;
-    const double c2dc = c2 * dc;
-    const double dh = -gamm1 * (uRoe * dq1 + vRoe * dq2 + wRoe * dq3 - dq4) + 0.5 * gamm1 * v2Roe * dq0;
-    flux[0] -= 0.5 * (e1 * dq0 - dh * xi1 - dc * xi2);
-    flux[1] -= 0.5 * (e1 * dq1 + (xsn * c2dc - uRoe * dh) * xi1 + (xsn * dh - uRoe * dc) * xi2);
-    flux[2] -= 0.5 * (e1 * dq2 + (ysn * c2dc - vRoe * dh) * xi1 + (ysn * dh - vRoe * dc) * xi2);
-    flux[3] -= 0.5 * (e1 * dq3 + (zsn * c2dc - wRoe * dh) * xi1 + (zsn * dh - wRoe * dc) * xi2);
-    flux[4] -= 0.5 * (e1 * dq4 + (theta * c2dc - hRoe * dh) * xi1 + (theta * dh - hRoe * dc) * xi2);
+    const T xi1 = (T(2.0) * e1 - e2 - e3) / (T(2.0) * c2);
+    const T xi2 = (e2 - e3) / (T(2.0) * cm);
+    const T dc = theta * dq0 - xsn * dq1 - ysn * dq2 - zsn * dq3;
+    const T c2dc = c2 * dc;
+    const T dh = -gamm1 * (uRoe * dq1 + vRoe * dq2 + wRoe * dq3 - dq4) + T(0.5) * gamm1 * v2Roe * dq0;
+    flux[0] -= T(0.5) * (e1 * dq0 - dh * xi1 - dc * xi2);
+    flux[1] -= T(0.5) * (e1 * dq1 + (xsn * c2dc - uRoe * dh) * xi1 + (xsn * dh - uRoe * dc) * xi2);
+    flux[2] -= T(0.5) * (e1 * dq2 + (ysn * c2dc - vRoe * dh) * xi1 + (ysn * dh - vRoe * dc) * xi2);
+    flux[3] -= T(0.5) * (e1 * dq3 + (zsn * c2dc - wRoe * dh) * xi1 + (zsn * dh - wRoe * dc) * xi2);
+    flux[4] -= T(0.5) * (e1 * dq4 + (theta * c2dc - hRoe * dh) * xi1 + (theta * dh - hRoe * dc) * xi2);
 }
 
 __device__ __forceinline__ double venfun(double x, double y, double eps)
@@ -324,6 +330,8 @@ __device__ __forceinline__ void sweep_coef(const Consts &K, const double *qc, co
     co[10] = nx; co[11] = ny; co[12] = nz; co[13] = fr.area; co[14] = sr; co[15] = 0.0;
 }
 
+// VIS: 1 viscous, 0 inviscid, -1 read from K (the generic sweep).
+template <int VIS = -1>
 __device__ __forceinline__ void sweep_apply(const Consts &K, const double *co, const double *dq, double *f)
 {
     const double gama1 = K.gamma - 1.0;
@@ -340,15 +348,18 @@ __device__ __forceinline__ void sweep_apply(const Consts &K, const double *co, c
     f[4] = co[0] * dq[4] - co[8] * a + co[4] * b;
 #pragma unroll
     for (int m = 0; m < 5; ++m) f[m] *= co[13];
-    if (K.viscous) {
+    if (VIS == 1 || (VIS < 0 && K.viscous)) {
 #pragma unroll
         for (int m = 0; m < 5; ++m) f[m] -= co[14] * dq[m];
     }
 }
 
 // ------------------------------------------------ gradients + limiter
+// GT is the storage type of the gradients: double, or float for the mixed
+// form (computed in FP64, stored rounded to FP32).
+template <typename GT>
 __global__ void __launch_bounds__(kBlock) k_grad_lim(DevMesh M, Consts K, const double *__restrict__ q,
-                                                     double *__restrict__ grad, double *__restrict__ gradT,
+                                                     GT *__restrict__ grad, GT *__restrict__ gradT,
                                                      double *__restrict__ lim, double *__restrict__ coef)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -404,13 +415,13 @@ __global__ void __launch_bounds__(kBlock) k_grad_lim(DevMesh M, Consts K, const 
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         gx[m] *= ovol; gy[m] *= ovol; gz[m] *= ovol;
-        grad[(size_t)m * nC + c] = gx[m];
-        grad[(size_t)(5 + m) * nC + c] = gy[m];
-        grad[(size_t)(10 + m) * nC + c] = gz[m];
+        grad[(size_t)m * nC + c] = GT(gx[m]);
+        grad[(size_t)(5 + m) * nC + c] = GT(gy[m]);
+        grad[(size_t)(10 + m) * nC + c] = GT(gz[m]);
     }
     if (K.viscous) {
         tgx *= ovol; tgy *= ovol; tgz *= ovol;
-        gradT[c] = tgx; gradT[(size_t)nC + c] = tgy; gradT[(size_t)2 * nC + c] = tgz;
+        gradT[c] = GT(tgx); gradT[(size_t)nC + c] = GT(tgy); gradT[(size_t)2 * nC + c] = GT(tgz);
     }
     double l;
     if (K.limiter == 1) {
@@ -446,13 +457,18 @@ __global__ void __launch_bounds__(kBlock) k_grad_lim(DevMesh M, Consts K, const 
 }
 
 // ------------------------------------------------- face fluxes (gather)
+// One side of a face: the cell state (FP64) and the gradients in the face
+// arithmetic type T (double, or float in the mixed form).
+template <typename T>
 struct CellSide {
-    double q[5], g[15], lim, x, y, z;
+    double q[5];
+    T g[15];
+    double lim, x, y, z;
 };
 
-__device__ __forceinline__ void load_side(const DevMesh &M, const double *__restrict__ q,
-                                          const double *__restrict__ grad, const double *__restrict__ lim,
-                                          int i, bool needGrad, CellSide &s)
+template <typename T>
+__device__ __forceinline__ void load_side(const DevMesh &M, const double *__restrict__ q, const T *__restrict__ grad,
+                                          const double *__restrict__ lim, int i, bool needGrad, CellSide<T> &s)
 {
     const int nT = M.nTotal, nC = M.nCell;
     load5(q, nT, i, s.q);
@@ -463,33 +479,46 @@ __device__ __forceinline__ void load_side(const DevMesh &M, const double *__rest
         for (int m = 0; m < 15; ++m) s.g[m] = grad[(size_t)m * nC + i];
     } else {
 #pragma unroll
-        for (int m = 0; m < 15; ++m) s.g[m] = 0.0;
+        for (int m = 0; m < 15; ++m) s.g[m] = T(0.0);
     }
 }
 
-__device__ __forceinline__ void reconstruct(const CellSide &s, double lmin, const FaceRec &fr, double *out)
+// MUSCL face value (reconmeth 1) with the first-order fallback on negative
+// density or pressure.  The offset face - centroid is formed in FP64.
+template <typename T>
+__device__ __forceinline__ void reconstruct(const CellSide<T> &s, double lmin, const FaceRec &fr, T *out)
 {
-    double qt[5];
-    const double dx = fr.xc - s.x, dy = fr.yc - s.y, dz = fr.zc - s.z;
+    T qt[5];
+    const T dx = T(fr.xc - s.x), dy = T(fr.yc - s.y), dz = T(fr.zc - s.z);
+    const T lm = T(lmin);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) qt[m] = s.q[m];
+    for (int m = 0; m < 5; ++m) qt[m] = T(s.q[m]);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) qt[m] += lmin * (s.g[m] * dx + s.g[5 + m] * dy + s.g[10 + m] * dz);
-    const bool ok = !(qt[0] <= 0.0 || qt[4] <= 0.0);
+    for (int m = 0; m < 5; ++m) qt[m] += lm * (s.g[m] * dx + s.g[5 + m] * dy + s.g[10 + m] * dz);
+    const bool ok = !(qt[0] <= T(0.0) || qt[4] <= T(0.0));
 #pragma unroll
-    for (int m = 0; m < 5; ++m) out[m] = ok ? qt[m] : s.q[m];
+    for (int m = 0; m < 5; ++m) out[m] = ok ? qt[m] : T(s.q[m]);
 }
 
 // Viscous face flux (flux = -(1/Re) * A * fvis), one face, both sides given.
-__device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const FaceRec &fr,
-                             const CellSide &L, const CellSide &R, const double *gTL, const double *gTR,
-                             bool bnd, bool wall, double *fl)
+// Temperatures, geometry and the two-point differences along the normal are
+// formed in FP64 (they are small differences of O(1) values); the Sutherland
+// viscosity, the face-gradient assembly, the stresses and the flux are in T.
+template <typename T>
+__device__ __forceinline__ T sutherland(T t, T tsuth)
 {
-    const double two3rd = 2.0 / 3.0;
+    return t * sqrt(t) * (T(1.0) + tsuth) / (t + tsuth);
+}
+
+template <typename T>
+__device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const FaceRec &fr, const CellSide<T> &L,
+                             const CellSide<T> &R, const T *gTL, const T *gTR, bool bnd, bool wall, double *fl)
+{
+    const T two3rd = T(2.0) / T(3.0);
     const double tLc = L.q[4] / (K.R * L.q[0]);
     const double tRc = R.q[4] / (K.R * R.q[0]);
-    const double vlL = tLc * sqrt(tLc) * (1.0 + K.tsuth) / (tLc + K.tsuth);
-    const double vlR = tRc * sqrt(tRc) * (1.0 + K.tsuth) / (tRc + K.tsuth);
+    const T vlL = sutherland(T(tLc), T(K.tsuth));
+    const T vlR = sutherland(T(tRc), T(K.tsuth));
     // FaceWeight (Geo_UnstructGrid.cpp:2420)
     const double dxL0 = fr.xc - L.x, dyL0 = fr.yc - L.y, dzL0 = fr.zc - L.z;
     const double dxR0 = fr.xc - R.x, dyR0 = fr.yc - R.y, dzR0 = fr.zc - R.z;
@@ -499,13 +528,13 @@ __device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const Fac
     double wL = d2 * dd, wR = d1 * dd;
     if (bnd) { wL = 1.0; wR = 0.0; }
     // GetVisFaceValue (:7126)
-    const double mul = 0.5 * (vlL + vlR);
-    const double mut = 0.5 * (0.0 + 0.0);
+    const T mul = T(0.5) * (vlL + vlR);
+    const T mut = T(0.5) * (T(0.0) + T(0.0));
     double primF[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) primF[m] = 0.5 * (L.q[m] + R.q[m]);
     const double tmF = 0.5 * (tLc + tRc);
-    const double kcp = (mul * K.oprl + mut * K.oprt) * K.cp;
+    const T kcp = (mul * T(K.oprl) + mut * T(K.oprt)) * T(K.cp);
     if (wall) { primF[1] = 0.0; primF[2] = 0.0; primF[3] = 0.0; }
     // ComputeVisflux (:12977)
     const double nxs = fr.nx, nys = fr.ny, nzs = fr.nz;
@@ -544,11 +573,11 @@ __device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const Fac
     if (flag & 1u) {
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
-            f1[m] += L.g[m] * dxnL + L.g[5 + m] * dynL + L.g[10 + m] * dznL;
-            f2[m] += R.g[m] * dxnR + R.g[5 + m] * dynR + R.g[10 + m] * dznR;
+            f1[m] += double(L.g[m]) * dxnL + double(L.g[5 + m]) * dynL + double(L.g[10 + m]) * dznL;
+            f2[m] += double(R.g[m]) * dxnR + double(R.g[5 + m]) * dynR + double(R.g[10 + m]) * dznR;
         }
-        tL += gTL[0] * dxnL + gTL[1] * dynL + gTL[2] * dznL;
-        tR += gTR[0] * dxnR + gTR[1] * dynR + gTR[2] * dznR;
+        tL += double(gTL[0]) * dxnL + double(gTL[1]) * dynL + double(gTL[2]) * dznL;
+        tR += double(gTR[0]) * dxnR + double(gTR[1]) * dynR + double(gTR[2]) * dznR;
         if (tL < kSMALL) tL = tLc;
         if (tR < kSMALL) tR = tRc;
 #pragma unroll
@@ -573,46 +602,51 @@ __device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const Fac
         for (int m = 0; m < 5; ++m) dfdn[m] = wl * dfd1[m] + wr * dfd2[m];
         dtdn = dtd1 * wl + dtd2 * wr;
     }
-    double dfdx[5], dfdy[5], dfdz[5], dft1[5], dft2[5];
+    const T tWL = T(wL), tWR = T(wR);
+    const T n1 = T(nxs), n2 = T(nys), n3 = T(nzs);
+    const T a1 = T(t1x), a2 = T(t1y), a3 = T(t1z), b1 = T(t2x), b2 = T(t2y), b3 = T(t2z);
+    T dfdx[5], dfdy[5], dfdz[5], dft1[5], dft2[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        dfdx[m] = wL * L.g[m] + wR * R.g[m];
-        dfdy[m] = wL * L.g[5 + m] + wR * R.g[5 + m];
-        dfdz[m] = wL * L.g[10 + m] + wR * R.g[10 + m];
+        dfdx[m] = tWL * L.g[m] + tWR * R.g[m];
+        dfdy[m] = tWL * L.g[5 + m] + tWR * R.g[5 + m];
+        dfdz[m] = tWL * L.g[10 + m] + tWR * R.g[10 + m];
     }
 #pragma unroll
     for (int m = 1; m <= 3; ++m) {
-        dft1[m] = t1x * dfdx[m] + t1y * dfdy[m] + t1z * dfdz[m];
-        dft2[m] = t2x * dfdx[m] + t2y * dfdy[m] + t2z * dfdz[m];
+        dft1[m] = a1 * dfdx[m] + a2 * dfdy[m] + a3 * dfdz[m];
+        dft2[m] = b1 * dfdx[m] + b2 * dfdy[m] + b3 * dfdz[m];
     }
 #pragma unroll
     for (int m = 1; m <= 3; ++m) {
-        dfdx[m] = nxs * dfdn[m] + t1x * dft1[m] + t2x * dft2[m];
-        dfdy[m] = nys * dfdn[m] + t1y * dft1[m] + t2y * dft2[m];
-        dfdz[m] = nzs * dfdn[m] + t1z * dft1[m] + t2z * dft2[m];
+        const T dn = T(dfdn[m]);
+        dfdx[m] = n1 * dn + a1 * dft1[m] + b1 * dft2[m];
+        dfdy[m] = n2 * dn + a2 * dft1[m] + b2 * dft2[m];
+        dfdz[m] = n3 * dn + a3 * dft1[m] + b3 * dft2[m];
     }
-    const double dudx = dfdx[1], dudy = dfdy[1], dudz = dfdz[1];
-    const double dvdx = dfdx[2], dvdy = dfdy[2], dvdz = dfdz[2];
-    const double dwdx = dfdx[3], dwdy = dfdy[3], dwdz = dfdz[3];
-    double qNorm = 0.0;
-    qNorm += kcp * dtdn;
-    const double divv2p3 = two3rd * (dudx + dvdy + dwdz);
-    const double vis = mul + mut;
-    const double um = fM[1], vm = fM[2], wm = fM[3];
-    const double txx = vis * (2.0 * dudx - divv2p3);
-    const double tyy = vis * (2.0 * dvdy - divv2p3);
-    const double tzz = vis * (2.0 * dwdz - divv2p3);
-    const double txy = vis * (dudy + dvdx);
-    const double txz = vis * (dudz + dwdx);
-    const double tyz = vis * (dvdz + dwdy);
-    double fv[5];
-    fv[0] = 0.0;
-    fv[1] = nxs * txx + nys * txy + nzs * txz;
-    fv[2] = nxs * txy + nys * tyy + nzs * tyz;
-    fv[3] = nxs * txz + nys * tyz + nzs * tzz;
+    const T dudx = dfdx[1], dudy = dfdy[1], dudz = dfdz[1];
+    const T dvdx = dfdx[2], dvdy = dfdy[2], dvdz = dfdz[2];
+    const T dwdx = dfdx[3], dwdy = dfdy[3], dwdz = dfdz[3];
+    T qNorm = T(0.0);
+    qNorm += kcp * T(dtdn);
+    const T divv2p3 = two3rd * (dudx + dvdy + dwdz);
+    const T vis = mul + mut;
+    const T um = T(fM[1]), vm = T(fM[2]), wm = T(fM[3]);
+    const T txx = vis * (T(2.0) * dudx - divv2p3);
+    const T tyy = vis * (T(2.0) * dvdy - divv2p3);
+    const T tzz = vis * (T(2.0) * dwdz - divv2p3);
+    const T txy = vis * (dudy + dvdx);
+    const T txz = vis * (dudz + dwdx);
+    const T tyz = vis * (dvdz + dwdy);
+    T fv[5];
+    fv[0] = T(0.0);
+    fv[1] = n1 * txx + n2 * txy + n3 * txz;
+    fv[2] = n1 * txy + n2 * tyy + n3 * tyz;
+    fv[3] = n1 * txz + n2 * tyz + n3 * tzz;
     fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + qNorm;
+    const T oRe = T(K.oRe), area = T(fr.area);
 #pragma unroll
-    for (int m = 0; m < 5; ++m) fl[m] = -K.oRe * fr.area * fv[m];
+    for (int m = 0; m < 5; ++m) fl[m] = double(-oRe * area * fv[m]);
 }
 
 // One thread per cell: inviscid (and viscous) residual, spectral radii and
@@ -632,10 +666,12 @@ __device__ __forceinline__ double local_dt(const DevMesh &M, const Consts &K, in
 // One thread per cell: inviscid (and viscous) residual, spectral radii, and
 // per-block partials of res^2 and, with FUSED_DT, of the local time step
 // minimum (CFL of the current step read from cflArr[*stepCtr]).
-template <bool SPEC_ONLY, bool FUSED_DT>
+// T is the face arithmetic type: double, or float in the mixed form (FP32
+// gradients, reconstruction, Roe and viscous fluxes; FP64 accumulation).
+template <bool SPEC_ONLY, bool FUSED_DT, typename T>
 __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const double *__restrict__ q,
-                                                 const double *__restrict__ grad,
-                                                 const double *__restrict__ gradT,
+                                                 const T *__restrict__ grad,
+                                                 const T *__restrict__ gradT,
                                                  const double *__restrict__ lim, double *__restrict__ res,
                                                  double *__restrict__ specInv, double *__restrict__ specVis,
                                                  double *__restrict__ normPart, const double *__restrict__ cflArr,
@@ -649,7 +685,7 @@ __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const doub
     double dtl = 1.0e300;
     if (c < nC) {
         const bool needG = !SPEC_ONLY && (K.limiter != 2 || K.viscous);
-        CellSide own;
+        CellSide<T> own;
         load_side(M, q, grad, lim, c, needG, own);
         double sInv = 0.0;
         for (int s = 0; s < M.ellW; ++s) {
@@ -659,52 +695,54 @@ __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const doub
             const int f = (isL ? enc : -enc) - 1;
             const int nb = M.c2n[(size_t)s * nC + c];
             const FaceRec fr = M.face[f];
-            CellSide oth;
+            CellSide<T> oth;
             load_side(M, q, grad, lim, nb, !SPEC_ONLY && K.limiter != 2, oth);
-            const CellSide &L = isL ? own : oth;
-            const CellSide &R = isL ? oth : own;
+            const CellSide<T> &L = isL ? own : oth;
+            const CellSide<T> &R = isL ? oth : own;
             if (!SPEC_ONLY) {
-                double qL[5], qR[5];
+                T qL[5], qR[5];
                 if (K.limiter != 2) {
                     const double lmin = dMIN(L.lim, R.lim);
                     reconstruct(L, lmin, fr, qL);
                     reconstruct(R, lmin, fr, qR);
                 } else {
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) { qL[m] = L.q[m]; qR[m] = R.q[m]; }
+                    for (int m = 0; m < 5; ++m) { qL[m] = T(L.q[m]); qR[m] = T(R.q[m]); }
                 }
+                const T nx = T(fr.nx), ny = T(fr.ny), nz = T(fr.nz);
                 if (f < M.nBFace) {
                     const int bc = M.bcType[f];
                     if (bc == kBC_WALL || bc == kBC_SYMM) {
 #pragma unroll
                         for (int m = 0; m < 5; ++m) {
-                            const double tmp = 0.5 * (L.q[m] + R.q[m]);
+                            const T tmp = T(0.5 * (L.q[m] + R.q[m]));
                             qL[m] = tmp; qR[m] = tmp;
                         }
                         if (bc == kBC_SYMM) {
-                            const double vn = 2.0 * (fr.nx * qL[1] + fr.ny * qL[2] + fr.nz * qL[3] - 0.0);
+                            const T vn = T(2.0) * (nx * qL[1] + ny * qL[2] + nz * qL[3] - T(0.0));
                             qR[0] = qL[0];
-                            qR[1] = qL[1] - vn * fr.nx;
-                            qR[2] = qL[2] - vn * fr.ny;
-                            qR[3] = qL[3] - vn * fr.nz;
+                            qR[1] = qL[1] - vn * nx;
+                            qR[2] = qL[2] - vn * ny;
+                            qR[3] = qL[3] - vn * nz;
                             qR[4] = qL[4];
                         }
                         if (bc == kBC_WALL && K.viscous) {
-                            qL[1] = 0.0; qL[2] = 0.0; qL[3] = 0.0;
-                            qR[1] = 0.0; qR[2] = 0.0; qR[3] = 0.0;
+                            qL[1] = T(0.0); qL[2] = T(0.0); qL[3] = T(0.0);
+                            qR[1] = T(0.0); qR[2] = T(0.0); qR[3] = T(0.0);
                         }
                     }
                 }
-                double fl[5];
-                roe_flux(qL, qR, fr.nx, fr.ny, fr.nz, K.gamma, K.entropyFix, K.entropyCoef, fl);
+                T fl[5];
+                roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
+                const T area = T(fr.area);
 #pragma unroll
-                for (int m = 0; m < 5; ++m) fl[m] *= fr.area;
+                for (int m = 0; m < 5; ++m) fl[m] *= area;
                 if (isL) {
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
+                    for (int m = 0; m < 5; ++m) acc[m] -= double(fl[m]);
                 } else {
 #pragma unroll
-                    for (int m = 0; m < 5; ++m) acc[m] += fl[m];
+                    for (int m = 0; m < 5; ++m) acc[m] += double(fl[m]);
                 }
             }
             // inviscid spectral radius at the face-averaged state
@@ -727,14 +765,14 @@ __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const doub
                 const int f = (isL ? enc : -enc) - 1;
                 const int nb = M.c2n[(size_t)s * nC + c];
                 const FaceRec fr = M.face[f];
-                CellSide oth;
+                CellSide<T> oth;
                 load_side(M, q, grad, lim, nb, !SPEC_ONLY, oth);
-                const CellSide &L = isL ? own : oth;
-                const CellSide &R = isL ? oth : own;
+                const CellSide<T> &L = isL ? own : oth;
+                const CellSide<T> &R = isL ? oth : own;
                 const bool bnd = f < M.nBFace;
                 if (!SPEC_ONLY) {
                     const int li = isL ? c : nb, ri = isL ? nb : c;
-                    double gTL[3] = {0.0, 0.0, 0.0}, gTR[3] = {0.0, 0.0, 0.0};
+                    T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
                     if (li < nC) { gTL[0] = gradT[li]; gTL[1] = gradT[(size_t)nC + li]; gTL[2] = gradT[(size_t)2 * nC + li]; }
                     if (ri < nC) { gTR[0] = gradT[ri]; gTR[1] = gradT[(size_t)nC + ri]; gTR[2] = gradT[(size_t)2 * nC + ri]; }
                     const bool wall = bnd && M.bcType[f] == kBC_WALL;
@@ -891,12 +929,17 @@ __device__ __forceinline__ void sweep_finish(const DevMesh &M, int c, bool useU,
                                              double *dq, double *dflux)
 {
     const int nC = M.nCell, nT = M.nTotal;
+    // every load before the first store, so they are issued together
     const double D = diag[c];
+    double r[5], U[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const double U = useU ? dflux[(size_t)m * nC + c] : 0.0;
-        const double r = res[(size_t)m * nC + c];
-        dq[(size_t)m * nT + c] = (r - U - rhs0[m]) / D;
+        r[m] = res[(size_t)m * nC + c];
+        U[m] = useU ? dflux[(size_t)m * nC + c] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        dq[(size_t)m * nT + c] = (r[m] - U[m] - rhs0[m]) / D;
         dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
     }
 }
@@ -1013,6 +1056,99 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_block(DevMesh M, Consts K
         const int b = ptr[L], e = ptr[L + 1];
         for (int i = b + (int)threadIdx.x; i < e; i += blockDim.x)
             sweep_cell<FWD>(M, K, cells[i], useU, coef, res, diag, dq, dflux);
+        __syncthreads();
+    }
+}
+
+// Sweep-ordered variant.  For each direction the host lists, per position
+// p of the level order, the cell and its active faces in face order (row =
+// 2*face + side of the coefficient block, nb = the neighbour), so the loads
+// of a position are independent of each other: one memory round for the
+// lists, one for coefficients and neighbour dq, then the store.
+// The whole 128-byte block is loaded so that no part of it is fetched later
+// inside a branch.
+__device__ __forceinline__ void load_coef_row(const double *__restrict__ coef, int row, double *co)
+{
+    const double2 *src = reinterpret_cast<const double2 *>(coef + (size_t)row * kCoef);
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const double2 v = src[j];
+        co[2 * j] = v.x;
+        co[2 * j + 1] = v.y;
+    }
+}
+
+template <int NW, int VIS>
+__device__ __forceinline__ void sweep_position(const DevMesh &M, const Consts &K, const SweepOrder &so, int p,
+                                               bool useU, const double *__restrict__ coef,
+                                               const double *__restrict__ res, const double *__restrict__ diag,
+                                               double *dq, double *dflux)
+{
+    const int nT = M.nTotal, P = M.nCell;
+    const int c = so.cells[p];
+    int row[NW], nb[NW];
+#pragma unroll
+    for (int s = 0; s < NW; ++s) {  // the lists are padded to NW slots
+        row[s] = so.row[(size_t)s * P + p];
+        nb[s] = so.nb[(size_t)s * P + p];
+    }
+    // the cell's own operands first: they depend on c only
+    const int nC = M.nCell;
+    const double D = diag[c];
+    double r[5], U[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        r[m] = res[(size_t)m * nC + c];
+        U[m] = useU ? dflux[(size_t)m * nC + c] : 0.0;
+    }
+    double co[NW][16], dqn[NW][5];
+#pragma unroll
+    for (int s = 0; s < NW; ++s) {
+        const bool act = row[s] >= 0;
+        load_coef_row(coef, act ? row[s] : 0, co[s]);
+        load5(dq, nT, act ? nb[s] : 0, dqn[s]);
+    }
+    // Unused slots add +0.0, which leaves the sum unchanged (it starts at
+    // +0.0, so it is never -0.0); no branch lets the loads of a slot sink.
+    double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NW; ++s) {
+        double df[5];
+        sweep_apply<VIS>(K, co[s], dqn[s], df);
+        const bool act = row[s] >= 0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) rhs0[m] += act ? df[m] : 0.0;
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        dq[(size_t)m * nT + c] = (r[m] - U[m] - rhs0[m]) / D;
+        dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
+    }
+}
+
+template <int NW, int VIS>
+__global__ void __launch_bounds__(64) k_sweep_level_so(DevMesh M, Consts K, SweepOrder so, int begin, int end,
+                                                       bool useU, const double *__restrict__ coef,
+                                                       const double *__restrict__ res,
+                                                       const double *__restrict__ diag, double *dq, double *dflux)
+{
+    const int p = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end) return;
+    sweep_position<NW, VIS>(M, K, so, p, useU, coef, res, diag, dq, dflux);
+}
+
+template <int NW, int VIS>
+__global__ void __launch_bounds__(kSweepBlock) k_sweep_block_so(DevMesh M, Consts K, SweepOrder so,
+                                                               const int *__restrict__ ptr, int l0, int l1,
+                                                               bool useU, const double *__restrict__ coef,
+                                                               const double *__restrict__ res,
+                                                               const double *__restrict__ diag, double *dq,
+                                                               double *dflux)
+{
+    for (int L = l0; L < l1; ++L) {
+        const int b = ptr[L], e = ptr[L + 1];
+        for (int p = b + (int)threadIdx.x; p < e; p += blockDim.x)
+            sweep_position<NW, VIS>(M, K, so, p, useU, coef, res, diag, dq, dflux);
         __syncthreads();
     }
 }
@@ -1161,6 +1297,101 @@ __global__ void __launch_bounds__(kBlock) k_copy(const double2 *__restrict__ src
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// Wall force and moment sums of AirForceCoefParts
+// (NSSolverUnstruct.cpp:16026-16250): pressure and viscous stress on every
+// solid-surface face, moments about the reference point.  One workgroup:
+// thread t sums faces t, t+256, ... in increasing order, then a fixed tree
+// reduction; the result does not depend on scheduling.  out[15] holds, in
+// order, CA_f CA_p CN_f CN_p CZ_f CZ_p cpx cpy cpz Cl_f Cl_p Cn_f Cn_p Cm_f
+// Cm_p, unscaled.
+constexpr int kForceSums = 15;
+
+__device__ __forceinline__ void correct_gradient(double fl, double fr, double &gx, double &gy, double &gz,
+                                                 double dx, double dy, double dz, double ods)
+{
+    const double fold = gx * dx + gy * dy + gz * dz;
+    const double fnew = (fr - fl) * ods;
+    const double df = fnew - fold;
+    gx += df * dx;
+    gy += df * dy;
+    gz += df * dz;
+}
+
+__global__ void __launch_bounds__(kBlock) k_forces(DevMesh M, Consts K, const double *__restrict__ q,
+                                                   const double *__restrict__ grad,
+                                                   const double *__restrict__ visl, double tx0, double ty0,
+                                                   double tz0, double *__restrict__ out)
+{
+    __shared__ double red[kForceSums][kBlock];
+    const int t = threadIdx.x;
+    const int nT = M.nTotal, nC = M.nCell;
+    const double poo = K.primInf[4];
+    double acc[kForceSums];
+#pragma unroll
+    for (int j = 0; j < kForceSums; ++j) acc[j] = 0.0;
+    for (int w = t; w < M.nWall; w += kBlock) {
+        const int f = M.wallFaces[w];
+        const int le = M.fle[f], re = nC + f;
+        const FaceRec fr = M.face[f];
+        const double nx = fr.nx * fr.area, ny = fr.ny * fr.area, nz = fr.nz * fr.area;
+        const double cp = 2.0 * (q[(size_t)4 * nT + le] - poo);
+        const double dpx = nx * cp, dpy = ny * cp, dpz = nz * cp;
+        acc[1] += dpx; acc[3] += dpy; acc[5] += dpz;
+        const double xc = fr.xc, yc = fr.yc, zc = fr.zc;
+        if (K.viscous) {
+            double g[9];  // du/d(x,y,z), dv/d(x,y,z), dw/d(x,y,z)
+#pragma unroll
+            for (int m = 0; m < 3; ++m) {
+                g[3 * m + 0] = grad[(size_t)(1 + m) * nC + le];
+                g[3 * m + 1] = grad[(size_t)(6 + m) * nC + le];
+                g[3 * m + 2] = grad[(size_t)(11 + m) * nC + le];
+            }
+            double dx = M.xcc[re] - M.xcc[le];
+            double dy = M.ycc[re] - M.ycc[le];
+            double dz = M.zcc[re] - M.zcc[le];
+            const double ods = 1.0 / sqrt(dx * dx + dy * dy + dz * dz);
+            dx = dx * ods; dy = dy * ods; dz = dz * ods;
+#pragma unroll
+            for (int m = 0; m < 3; ++m)
+                correct_gradient(q[(size_t)(1 + m) * nT + le], q[(size_t)(1 + m) * nT + re], g[3 * m], g[3 * m + 1],
+                                 g[3 * m + 2], dx, dy, dz, ods);
+            const double vis = visl[le];
+            const double c23 = 2.0 / 3.0;
+            const double dudx = g[0], dudy = g[1], dudz = g[2];
+            const double dvdx = g[3], dvdy = g[4], dvdz = g[5];
+            const double dwdx = g[6], dwdy = g[7], dwdz = g[8];
+            const double txx = vis * c23 * (2.0 * dudx - dvdy - dwdz);
+            const double tyy = vis * c23 * (2.0 * dvdy - dwdz - dudx);
+            const double tzz = vis * c23 * (2.0 * dwdz - dudx - dvdy);
+            const double txy = vis * (dudy + dvdx);
+            const double txz = vis * (dudz + dwdx);
+            const double tyz = vis * (dvdz + dwdy);
+            const double fvsx = -2.0 * (nx * txx + ny * txy + nz * txz) * K.oRe;
+            const double fvsy = -2.0 * (nx * txy + ny * tyy + nz * tyz) * K.oRe;
+            const double fvsz = -2.0 * (nx * txz + ny * tyz + nz * tzz) * K.oRe;
+            acc[0] += fvsx; acc[2] += fvsy; acc[4] += fvsz;
+            acc[9] += (yc - ty0) * fvsz - (zc - tz0) * fvsy;
+            acc[11] += (zc - tz0) * fvsx - (xc - tx0) * fvsz;
+            acc[13] += (xc - tx0) * fvsy - (yc - ty0) * fvsx;
+        }
+        acc[10] += (yc - ty0) * dpz - (zc - tz0) * dpy;
+        acc[12] += (zc - tz0) * dpx - (xc - tx0) * dpz;
+        acc[14] += (xc - tx0) * dpy - (yc - ty0) * dpx;
+        acc[6] += dpx; acc[7] += dpy; acc[8] += dpz;
+    }
+#pragma unroll
+    for (int j = 0; j < kForceSums; ++j) red[j][t] = acc[j];
+    __syncthreads();
+    for (int h = kBlock / 2; h > 0; h >>= 1) {
+        if (t < h) {
+#pragma unroll
+            for (int j = 0; j < kForceSums; ++j) red[j][t] += red[j][t + h];
+        }
+        __syncthreads();
+    }
+    if (t < kForceSums) out[t] = red[t][0];
+}
+
 }  // namespace
 
 void launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
@@ -1181,29 +1412,40 @@ void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
     hipLaunchKernelGGL(k_gamma_t, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.t, f.visl);
 }
 
-void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad)
 {
-    hipLaunchKernelGGL(k_grad_lim, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
-                       f.lim, f.coef);
+    if (k.mixed && !fp64Grad)
+        hipLaunchKernelGGL(k_grad_lim<float>, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.gradF,
+                           f.gradTF, f.lim, f.coef);
+    else
+        hipLaunchKernelGGL(k_grad_lim<double>, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.grad,
+                           f.gradT, f.lim, f.coef);
 }
 
 int flux_blocks(const DevMesh &m) { return nblk(m.nCell, kBlock); }
 
+template <bool FUSED_DT>
+static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    if (k.mixed)
+        hipLaunchKernelGGL((k_flux<false, FUSED_DT, float>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q,
+                           f.gradF, f.gradTF, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr,
+                           f.dtLoc, f.dtMinPart);
+    else
+        hipLaunchKernelGGL((k_flux<false, FUSED_DT, double>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q,
+                           f.grad, f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr,
+                           f.dtLoc, f.dtMinPart);
+}
+
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt)
 {
-    if (fusedDt)
-        hipLaunchKernelGGL((k_flux<false, true>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad,
-                           f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
-                           f.dtMinPart);
-    else
-        hipLaunchKernelGGL((k_flux<false, false>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad,
-                           f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
-                           f.dtMinPart);
+    if (fusedDt) launch_flux_t<true>(m, k, f, s);
+    else launch_flux_t<false>(m, k, f, s);
 }
 
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_flux<true, false>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
+    hipLaunchKernelGGL((k_flux<true, false, double>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
                        f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart);
 }
 
@@ -1231,10 +1473,36 @@ void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
 // (barriers are cheap); larger levels are spread over the GPU.
 static const int kSmallLevel = kSweepBlock;
 
+template <int NW, int VIS>
+static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool useU,
+                            hipStream_t s)
+{
+    int L = 0;
+    while (L < lv.nLevels) {
+        const int sz = lv.hptr[L + 1] - lv.hptr[L];
+        if (sz <= kSmallLevel) {
+            int L1 = L + 1;
+            while (L1 < lv.nLevels && lv.hptr[L1 + 1] - lv.hptr[L1] <= kSmallLevel) ++L1;
+            hipLaunchKernelGGL((k_sweep_block_so<NW, VIS>), dim3(1), dim3(kSweepBlock), 0, s, m, k, lv.so, lv.ptr, L, L1,
+                               useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+            L = L1;
+        } else {
+            hipLaunchKernelGGL((k_sweep_level_so<NW, VIS>), dim3(nblk(sz, 64)), dim3(64), 0, s, m, k, lv.so, lv.hptr[L],
+                               lv.hptr[L + 1], useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+            ++L;
+        }
+    }
+}
+
 void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,
                   bool firstSweep, hipStream_t s)
 {
     const bool useU = !(forward && firstSweep);
+    const bool v = k.viscous != 0;
+    if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1>(m, k, f, lv, useU, s) : launch_sweep_so<3, 0>(m, k, f, lv, useU, s);
+    if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1>(m, k, f, lv, useU, s) : launch_sweep_so<4, 0>(m, k, f, lv, useU, s);
+    if (lv.so.Wpad == 6) return v ? launch_sweep_so<6, 1>(m, k, f, lv, useU, s) : launch_sweep_so<6, 0>(m, k, f, lv, useU, s);
+    // wider polyhedra: face-by-face through the cell -> face lists
     int L = 0;
     while (L < lv.nLevels) {
         const int sz = lv.hptr[L + 1] - lv.hptr[L];
@@ -1272,6 +1540,12 @@ void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, 
                        f.negFlag, f.negCount);
     hipLaunchKernelGGL(k_fix, dim3(1), dim3(1), 0, s, m, k, f.q, f.qAlt, f.negFlag, f.negCount, f.negTotal,
                        f.stepCtr, advance ? 1 : 0);
+}
+
+void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *torqueRef, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_forces, dim3(1), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.visl, torqueRef[0], torqueRef[1],
+                       torqueRef[2], f.forceOut);
 }
 
 }  // namespace phl

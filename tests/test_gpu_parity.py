@@ -163,3 +163,95 @@ def test_hybrid_mesh_iterations_parity(viscous):
     err = np.max(np.abs(qg - qc) / np.maximum(np.max(np.abs(dq), axis=1, keepdims=True), 1e-300))
     print("hybrid viscous", viscous, "state rel err", err)
     assert err <= 1e-9
+
+
+@pytest.mark.parametrize("viscous", [0, 1])
+def test_forces_parity(viscous):
+    """Wall force and moment coefficients after a few iterations on the hybrid
+    bump, against the oracle's AirForceCoefParts + CompForce.  The GPU sums
+    the wall faces in a fixed tree order, the oracle in face order, so the
+    sums agree to rounding (1e-12 of the largest coefficient)."""
+    m, gpu, cpu = make_hybrid_pair(mach=0.3, viscous=viscous, reynolds=5e5, cfl_start=5.0, cfl_end=5.0,
+                                   alpha_deg=2.0, force_ref_x=0.25, force_ref_y=0.1, force_ref_z=0.05)
+    gpu.init_freestream()
+    cpu.init_freestream()
+    gpu.iterate(1, 5)
+    cpu.iterate(1, 5)
+    fg = gpu.forces()
+    fc = cpu.forces()
+    vals = np.array([fg[k] for k in pa.FORCE_NAMES])
+    scale = np.max(np.abs(fc[:15]))
+    assert scale > 0
+    print("forces viscous", viscous, "CL", fg["CL"], "CD", fg["CD"], "max diff", np.max(np.abs(vals - fc)))
+    assert np.max(np.abs(vals[:21] - fc[:21])) <= 1e-12 * scale
+    assert vals[21] == pytest.approx(fc[21], rel=1e-9, abs=1e-12)
+    # the forces call refreshes derived fields only: the state is unchanged and
+    # the next iterations still match
+    hg = gpu.iterate(6, 2)
+    hc, _ = cpu.iterate(6, 2)
+    assert np.max(np.abs(hg - hc) / hc) <= 1e-10
+
+
+def test_graph_replay_equals_eager_launch():
+    """phl_iterate replays captured two-step hipGraphs by default; an odd
+    number of steps exercises both buffer assignments and the eager tail.
+    Every replay must equal the eager launches bit for bit."""
+    m = pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7)
+    p = pa.default_params(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0)
+    a, b = pa.Solver(m, p), pa.Solver(m, p)
+    a.set_launch_mode("eager")
+    a.init_freestream()
+    b.init_freestream()
+    for first, n in ((1, 3), (4, 4), (8, 5)):
+        ha = a.iterate(first, n)
+        hb = b.iterate(first, n)
+        assert np.array_equal(ha, hb)
+        assert np.array_equal(a.get_state(), b.get_state())
+
+
+@pytest.mark.parametrize("viscous", [0, 1])
+def test_mixed_precision_residual(viscous):
+    """Mixed form (FP32 gradients, reconstruction, Roe and viscous fluxes,
+    FP64 accumulation) against the FP64 oracle on a perturbed state: the
+    residual agrees to FP32 rounding of the face fluxes, and the spectral
+    radii and LU-SGS inputs, which stay FP64, agree exactly as in FP64."""
+    m, gpu, cpu = make_hybrid_pair(mach=0.3, viscous=viscous, reynolds=5e5, precision=1)
+    q = seeded(m, cpu, 31 + viscous)
+    gpu.set_state(q)
+    cpu.set("q", q)
+    gpu.update_residuals()
+    cpu.update_residuals()
+    cpu.spectrum()
+    nc = m.n_cell
+    rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
+    scale = np.max(np.abs(rc), axis=1, keepdims=True)
+    err = np.max(np.abs(rg - rc) / scale)
+    print("mixed residual, viscous", viscous, "max err / max |res| =", err)
+    assert err <= 1e-4
+    assert err > 0.0  # the FP32 path really ran
+    sg, sc = gpu.get_field("spec_inv")[:nc], cpu.get("spec_inv")[:nc]
+    assert rel_err(sg, sc, 1e-300) <= TOL
+    gg, gc = gpu.get_field("dqdx")[:, :nc], cpu.get("dqdx")[:, :nc]
+    assert np.max(np.abs(gg - gc)) <= 1e-6 * np.max(np.abs(gc))  # FP32 storage
+
+
+def test_mixed_precision_forces_track_fp64():
+    """50 iterations from freestream in the FP64 and the mixed form: lift and
+    drag agree within 1e-4 (absolute coefficient), the density residual
+    histories within 1e-3 relative; switching precision on a live context
+    continues the run."""
+    m = pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7)
+    base = dict(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0)
+    a = pa.Solver(m, pa.default_params(**base))
+    b = pa.Solver(m, pa.default_params(precision=1, **base))
+    a.init_freestream()
+    b.init_freestream()
+    ha = a.iterate(1, 50)
+    hb = b.iterate(1, 50)
+    fa, fb = a.forces(), b.forces()
+    print("CL", fa["CL"], fb["CL"], "CD", fa["CD"], fb["CD"], "hist rel", np.max(np.abs(hb - ha) / ha))
+    assert abs(fa["CL"] - fb["CL"]) <= 1e-4 and abs(fa["CD"] - fb["CD"]) <= 1e-4
+    assert np.max(np.abs(hb - ha) / ha) <= 1e-3
+    b.set_precision("fp64")
+    hb2 = b.iterate(51, 4)
+    assert np.all(np.isfinite(hb2))

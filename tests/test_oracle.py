@@ -344,3 +344,77 @@ def test_viscous_couette_residual_closed_form():
     scale = fs[4] * dy * dz
     assert np.max(np.abs(res[:4, inner])) <= 1e-12 * scale
     assert np.allclose(res[4, inner], mu * a * a * vol / re_num, rtol=1e-9)
+
+
+def _wall_box(nx, ny, nz):
+    """Uniform box with the lower wall y=0 a solid surface and every other
+    boundary an extrapolation (copy) boundary."""
+    base = box_mesh(nx, ny, nz)
+    c = base.connectivity()
+    bc = np.where(c["bc_type"] == 2, 2, 1).astype(np.int32)
+    return pa.Mesh.from_arrays(c["x"], c["y"], c["z"], c["face_node_ptr"], c["face_nodes"], c["left"], c["right"],
+                               c["n_bface"], bc, c["cell_node_ptr"], c["cell_nodes"])
+
+
+def test_forces_closed_form_on_a_sheared_flat_wall():
+    """Uniform overpressure delta and linear shear u = a*y over the flat wall
+    y = 0 of a 3 x 0.8 x 0.2 box: the wall integrals of AirForceCoefParts have
+    closed forms (pressure -2*delta*A in y, friction 2*mu*a*A/Re in x with the
+    wall-corrected gradient, moment arms about a chosen reference point), and
+    CompForce's scalings and wind-axis projections are fixed by alpha."""
+    nx, ny, nz = 8, 4, 5
+    m = _wall_box(nx, ny, nz)
+    met = m.metrics()
+    re_num, a, delta, alpha = 2000.0, 0.6, 0.05, 30.0
+    x0, y0, z0 = 0.5, 1.0, 0.05
+    aref, lref, span = 2.0, 0.5, 4.0
+    s = oracle_for(m, mach=0.5, viscous=1, reynolds=re_num, alpha_deg=alpha, force_ref_x=x0, force_ref_y=y0,
+                   force_ref_z=z0, force_ref_area=aref, force_ref_length=lref, force_ref_span=span)
+    fs = s.freestream()
+    q = np.tile(fs[:, None], (1, m.n_total))
+    q[1] = a * met["ycc"]
+    q[2] = 0.0
+    q[3] = 0.0
+    q[4] = fs[4] + delta
+    s.set("q", q)
+    out = s.forces()
+    lx, lz, aw = 3.0, 0.2, 3.0 * 0.2
+    t = q[4, 0] / (q[0, 0] / (1.4 * 0.25))
+    ts = 110.4 / 288.15
+    mu = t * np.sqrt(t) * (1 + ts) / (t + ts)
+    fx_f = 2.0 * mu * a * aw / re_num          # raw sum of fvsx
+    fy_p = -2.0 * delta * aw                   # raw sum of n_y * area * cp
+    scf, sb, vcm = 1 / aref, 1 / (aref * span), 1 / (aref * lref)
+    expect = {
+        0: fx_f * scf, 1: 0.0, 2: 0.0, 3: fy_p * scf, 4: 0.0, 5: 0.0, 6: 0.0, 7: fy_p * scf, 8: 0.0,
+        9: 0.0,                                             # Cmx_f: fvsy = fvsz = 0
+        10: -(-2.0 * delta) * lx * (lz * lz / 2 - z0 * lz) * sb,   # -sum (z - z0) dpy
+        11: fx_f / aw * lx * (lz * lz / 2 - z0 * lz) * sb,        # sum (z - z0) fvsx
+        12: 0.0,
+        13: y0 * fx_f * vcm,                                 # -(0 - y0) fvsx
+        14: (-2.0 * delta) * lz * (0.0 - 3.0 * x0) * vcm,   # sum (x - x0) dpy
+    }
+    scale = max(abs(v) for v in expect.values())
+    for i, v in expect.items():
+        assert abs(out[i] - v) <= 1e-12 * scale, (i, out[i], v)
+    al = np.radians(alpha)
+    cfx, cfy = out[0] + out[1], out[2] + out[3]
+    assert out[15] == pytest.approx(-np.sin(al) * cfx + np.cos(al) * cfy, rel=1e-13)
+    assert out[16] == pytest.approx(np.cos(al) * cfx + np.sin(al) * cfy, rel=1e-13)
+    assert out[18] == pytest.approx(np.cos(al) * out[6] + np.sin(al) * out[7], rel=1e-13)
+    assert out[19] == pytest.approx(out[16] - out[18], rel=1e-13)
+    assert out[20] == pytest.approx(out[16] - out[15] ** 2 / (np.pi * 9.5), rel=1e-13)
+    assert out[21] == pytest.approx(np.sign(cfy) * (out[13] + out[14]) / abs(cfy), rel=1e-13)
+
+
+def test_forces_inviscid_ignore_shear():
+    m = _wall_box(6, 3, 3)
+    met = m.metrics()
+    s = oracle_for(m, mach=0.5, viscous=0)
+    fs = s.freestream()
+    q = np.tile(fs[:, None], (1, m.n_total))
+    q[1] = 0.8 * met["ycc"]
+    s.set("q", q)
+    out = s.forces()
+    assert np.all(out[[0, 2, 4, 9, 11, 13]] == 0.0)
+    assert np.max(np.abs(out[:15])) <= 1e-15  # p = p_inf: no pressure force either
