@@ -49,6 +49,39 @@ CONFIGS = {
 }
 DEFAULT_CONFIG = "c2"
 
+# Batched sweep: 64 inviscid bump-channel cases in one context, sizes and
+# freestream drawn from a fixed seed.
+C4_CASES, C4_SEED = 64, 20241202
+
+
+def c4_cases(n_cases=C4_CASES, seed=C4_SEED):
+    """(nx, ny, nz, bump, mach, alpha) per case: cells ~ U[50k, 400k] with the
+    channel's 8:4:1 aspect, Mach ~ U[0.3, 0.8], AoA ~ U[-2, 6] deg, bump
+    height ~ U[0.02, 0.08]."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_cases):
+        n = rng.uniform(50e3, 400e3)
+        nz = max(2, int(round((n / 32.0) ** (1.0 / 3.0))))
+        ny = 4 * nz
+        nx = max(4, int(round(n / (ny * nz))))
+        out.append((nx, ny, nz, float(rng.uniform(0.02, 0.08)), float(rng.uniform(0.3, 0.8)),
+                    float(rng.uniform(-2.0, 6.0))))
+    return out
+
+
+CONFIGS["c4"] = dict(workload="batched sweep: 64 inviscid bump-channel hex cases in one context, cells ~ U[50k,400k] "
+                              "(seeded), Mach ~ U[0.3,0.8], AoA ~ U[-2,6] deg, bump ~ U[0.02,0.08]; Roe + GG-cell + "
+                              "Venkat(K=5), LU-SGS CFL 10; FP32 face fluxes + FP64 accumulation; autotuned tiles; hipGraph",
+                     batch=True, params=dict(cfl_start=10.0, cfl_end=10.0, precision=1), cpu_steps=20)
+
+
+def build_batch(cfg):
+    cases = c4_cases()
+    meshes = [pa.Mesh.bump_hex(nx, ny, nz, h) for nx, ny, nz, h, _, _ in cases]
+    params = [pa.default_params(mach=mach, alpha_deg=alpha, **cfg["params"]) for *_, mach, alpha in cases]
+    return meshes, params
+
 
 def build_mesh(spec):
     kind = spec[0]
@@ -59,13 +92,16 @@ def build_mesh(spec):
     raise ValueError(kind)
 
 
-def flux_algorithmic_bytes(n_cell, n_face, viscous):
-    """Compulsory HBM bytes of one face-flux launch (FP64): read q 40N, grad
-    120N, limiter 8N, centroids 24N, face centres 24F, normal+area 32F,
-    left/right 8F; write residual 40N and spectral radius 8N."""
-    b = 240 * n_cell + 64 * n_face
+def flux_algorithmic_bytes(n_cell, n_face, viscous, mixed=False):
+    """Compulsory HBM bytes of one face-flux launch: read q 40N, gradients
+    120N (60N stored in FP32 by the mixed form), limiter 8N, centroids 24N,
+    face centres 24F, normal+area 32F, left/right 8F; write residual 40N and
+    spectral radius 8N; viscous adds the temperature gradient (24N, 12N in
+    FP32) and the viscous spectral radius 8N."""
+    g = 60 if mixed else 120
+    b = (240 - 120 + g) * n_cell + 64 * n_face
     if viscous:
-        b += 24 * n_cell + 8 * n_cell  # temperature gradient, viscous spectral radius
+        b += (12 if mixed else 24) * n_cell + 8 * n_cell
     return b
 
 
@@ -92,6 +128,7 @@ def main():
     ap.add_argument("--config", default=DEFAULT_CONFIG, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="also run the HBM copy probe")
+    ap.add_argument("--autotune", action="store_true", help="pick the flux/gradient tiles by timing first")
     ap.add_argument("--launch", default="graph", choices=["graph", "eager"],
                     help="replay captured hipGraphs (default) or launch every kernel eagerly")
     args = ap.parse_args()
@@ -101,11 +138,23 @@ def main():
 
     cfg = CONFIGS[args.config]
     print(f"[bench] config {args.config}: building mesh", file=sys.stderr, flush=True)
-    mesh = build_mesh(cfg["mesh"])
-    params = pa.default_params(**cfg["params"])
-    solver = pa.Solver(mesh, params, device=0)
+    if cfg.get("batch"):
+        parts, params_list = build_batch(cfg)
+        mesh = pa.Mesh.concat(parts)
+        params = params_list[0]
+        solver = pa.Solver(mesh, params_list, device=0)
+    else:
+        parts = [build_mesh(cfg["mesh"])]
+        mesh = parts[0]
+        params = pa.default_params(**cfg["params"])
+        solver = pa.Solver(mesh, params, device=0)
     solver.set_launch_mode(args.launch)
     solver.init_freestream()
+    tuned = None
+    if args.autotune or cfg.get("batch"):
+        tuned = solver.autotune()
+        solver.init_freestream()
+        print(f"[bench] autotuned tiles {tuned}", file=sys.stderr, flush=True)
     print(f"[bench] {mesh.n_cell} cells, {mesh.n_face} faces; warmup {args.warmup}", file=sys.stderr, flush=True)
     solver.iterate(1, args.warmup, history=False)
     solver.synchronize()
@@ -123,7 +172,8 @@ def main():
     times, n_timed = solver.kernel_timing()
     solver.set_kernel_timing(False)
     flux_ms = times["flux"] / max(n_timed, 1)
-    fb = flux_algorithmic_bytes(mesh.n_cell, mesh.n_face, params.viscous)
+    mixed = params.precision == 1
+    fb = flux_algorithmic_bytes(mesh.n_cell, mesh.n_face, params.viscous, mixed)
     achieved = fb / (flux_ms * 1e-3) / 1e9
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
@@ -136,17 +186,19 @@ def main():
     line = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32 face fluxes + f64" if mixed else "f64",
         "data": "synthetic",
         "config": {"workload": cfg["workload"], "cells": mesh.n_cell, "faces": mesh.n_face,
-                   "precision": "fp64", "parallelism": "single GPU",
-                   "launch": args.launch},
+                   "cases": mesh.n_cases, "precision": "mixed" if mixed else "fp64",
+                   "parallelism": "single GPU", "launch": args.launch,
+                   "tiles": None if tuned is None else {"flux": tuned["flux_tile"], "grad": tuned["grad_tile"]}},
         "roofline": roof,
         "kernel_ms_per_step": per_class,
     }
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(mesh, params, cfg["cpu_steps"])
+        line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_steps"])
     solver.close()
     print(json.dumps(line), flush=True)
 

@@ -123,6 +123,23 @@ phl_status phl_mesh_metrics(const phl_mesh *m, double *xfc, double *yfc, double 
                             double *xcc, double *ycc, double *zcc, double *vol, double *skew);
 void phl_mesh_destroy(phl_mesh *m);
 
+/* ---------------- batched form (no reference counterpart) ---------------
+ * A batch is a disjoint union of meshes, one steady case each (a Mach /
+ * angle / geometry sweep), solved together: every kernel launch of an
+ * iteration covers all cases, and the level schedules of the LU-SGS sweeps
+ * merge, so a level holds the cells of every case.  Each case keeps its own
+ * connectivity, metrics and sweep order, so its state evolves exactly as in
+ * a context of its own (its residual sums are accumulated in another order).
+ * The reference runs one case per process (Simulation.cpp:1190). */
+
+/* Concatenate n meshes into a batched mesh: nodes and cells in case order,
+ * all boundary faces first (case by case), then all interior faces. */
+phl_status phl_mesh_concat(const phl_mesh *const *parts, int n, phl_mesh **out);
+/* Case ranges of a mesh: cell_ptr[n+1] (cells of case k are
+ * cell_ptr[k]..cell_ptr[k+1]-1), bface_ptr[n+1] likewise for boundary faces
+ * (and so ghost cells).  A plain mesh is one case.  Either array may be NULL. */
+phl_status phl_mesh_cases(const phl_mesh *m, int *n_cases, int *cell_ptr, int *bface_ptr);
+
 /* ---------------- device and context lifecycle ------------------------- */
 
 /* GPUDeviceMultiGPU / GPUInfoQuery (GPU/Manage/src/GPUDeviceControl.cu:41-83). */
@@ -133,6 +150,12 @@ phl_status phl_device_info(int device, char *name, int name_len, int *n_cu, doub
 phl_status phl_create(const phl_mesh *mesh, const phl_params *p, int device, phl_ctx **out);
 /* GPUMemoryFree (GPU/Kernels/include/BasicDeviceVariables.h:689). */
 void phl_destroy(phl_ctx *ctx);
+/* Batched context over a mesh from phl_mesh_concat with params[n_cases]: the
+ * cases may differ only in mach, alpha_deg and beta_deg.  On a batched
+ * context phl_iterate writes rms_hist[step * n_cases + case]; the standalone
+ * phl_time_step and phl_lusgs hooks return PHL_ERR_UNSUPPORTED. */
+phl_status phl_create_batch(const phl_mesh *mesh, const phl_params *params, int device, phl_ctx **out);
+phl_status phl_batch_size(const phl_ctx *ctx, int *n_cases);
 
 /* ---------------- state transfer -------------------------------------- */
 phl_status phl_set_state(phl_ctx *ctx, const double *q);          /* [5][nTotal] primitive */
@@ -187,6 +210,8 @@ enum {
     PHL_FORCE_CD_SF = 19, PHL_FORCE_CD_CL2PA = 20, PHL_FORCE_XCP = 21, PHL_N_FORCE = 22
 };
 phl_status phl_forces(phl_ctx *ctx, double *coef);
+/* The same for case `c` of a batched context (phl_forces is case 0). */
+phl_status phl_forces_case(phl_ctx *ctx, int c, double *coef);
 
 /* ---------------- host driver (Controller::Relaxation loop) ----------- */
 /* n_steps outer iterations numbered first_step..; rms_hist[i] (may be NULL)
@@ -197,12 +222,24 @@ phl_status phl_iterate(phl_ctx *ctx, int first_step, int n_steps, double *rms_hi
  * step that reached the drop; steps_done is the number of steps run (the
  * state returned is after that many), rms_hist[0..steps_done-1] the history. */
 phl_status phl_solve(phl_ctx *ctx, int max_steps, double drop, int *steps_done, double *rms_hist);
+/* Batched solve: each case iterates until its density residual RMS has
+ * dropped by `drop` (then its state is frozen, from the end of the 32-step
+ * chunk in which it converged) or max_steps.  steps_done[n_cases]: the step
+ * at which each case reached the drop, or -(steps run) if it did not;
+ * rms_hist[max_steps * n_cases] as phl_iterate. */
+phl_status phl_solve_batch(phl_ctx *ctx, int max_steps, double drop, int *steps_done, double *rms_hist);
 phl_status phl_synchronize(phl_ctx *ctx);
 /* How phl_iterate / phl_solve launch (no reference counterpart: the
  * reference launches every kernel eagerly on the default stream).  mode 1
  * (default) replays two-step hipGraphs; mode 0 launches eagerly.  Results are
  * identical; kernel timing always launches eagerly. */
 phl_status phl_set_launch_mode(phl_ctx *ctx, int mode);
+/* Autotuner (no reference counterpart): times the flux and gradient kernels
+ * at workgroup tiles of 64, 128 and 256 cells on the current state and keeps
+ * the fastest for this context.  Tiles only regroup the residual partial
+ * sums.  ms_table (may be NULL) receives 6 mean times: flux at 64/128/256,
+ * then gradient at 64/128/256. */
+phl_status phl_autotune(phl_ctx *ctx, int *flux_tile, int *grad_tile, double *ms_table);
 /* Switch between the FP64 form (0) and the mixed form (1: FP32 gradients,
  * reconstruction, Roe and viscous fluxes; FP64 residual accumulation,
  * spectral radii, time step, LU-SGS and norms) on a live context, e.g. to

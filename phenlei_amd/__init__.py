@@ -123,11 +123,18 @@ def _declare(lib):
         "phl_lusgs": (ctypes.c_int, [_P, _I]),
         "phl_residual_norm": (ctypes.c_int, [_P, _D]),
         "phl_forces": (ctypes.c_int, [_P, _D]),
+        "phl_forces_case": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+        "phl_mesh_concat": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
+        "phl_mesh_cases": (ctypes.c_int, [_P, _I, _I, _I]),
+        "phl_create_batch": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_P)]),
+        "phl_batch_size": (ctypes.c_int, [_P, _I]),
+        "phl_solve_batch": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
         "phl_iterate": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
         "phl_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
         "phl_synchronize": (ctypes.c_int, [_P]),
         "phl_set_launch_mode": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
+        "phl_autotune": (ctypes.c_int, [_P, _I, _I, _D]),
         "phl_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_get_kernel_timing": (ctypes.c_int, [_P, _D, _I]),
         "phl_hbm_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int,
@@ -183,6 +190,12 @@ class Mesh:
         (self.n_node, self.n_cell, self.n_face, self.n_bface, self.n_face_nodes,
          self.n_cell_nodes) = [x.value for x in v]
         self.n_total = self.n_cell + self.n_bface
+        nc = ctypes.c_int()
+        _check(lib.phl_mesh_cases(self._h, ctypes.byref(nc), None, None))
+        self.n_cases = nc.value
+        self.case_cell_ptr = np.empty(self.n_cases + 1, np.int32)
+        self.case_bface_ptr = np.empty(self.n_cases + 1, np.int32)
+        _check(lib.phl_mesh_cases(self._h, ctypes.byref(nc), _iptr(self.case_cell_ptr), _iptr(self.case_bface_ptr)))
 
     @classmethod
     def bump_hex(cls, nx, ny, nz, bump=0.05):
@@ -209,6 +222,22 @@ class Mesh:
             _iptr(b[2]), _iptr(b[3]), n_bface, _iptr(b[4]), len(b[5]) - 1, _iptr(b[5]), _iptr(b[6]),
             ctypes.byref(h)))
         return cls(h)
+
+    @classmethod
+    def concat(cls, parts):
+        """Batched mesh: the disjoint union of `parts`, one case each (phl_mesh_concat)."""
+        arr = (_P * len(parts))(*[p._h for p in parts])
+        h = _P()
+        _check(load_library().phl_mesh_concat(arr, len(parts), ctypes.byref(h)))
+        out = cls(h)
+        out._parts = list(parts)
+        return out
+
+    def case_slices(self, k):
+        """(cell slice, ghost slice) of case k in [5][n_total] arrays."""
+        c0, c1 = self.case_cell_ptr[k], self.case_cell_ptr[k + 1]
+        b0, b1 = self.case_bface_ptr[k], self.case_bface_ptr[k + 1]
+        return slice(c0, c1), slice(self.n_cell + b0, self.n_cell + b1)
 
     def renumbered(self, method="colour", perm=None):
         """New mesh with cells renumbered ('colour' or an explicit permutation new->old)."""
@@ -279,10 +308,17 @@ class Solver:
     """Device context (phl_ctx).  Method names follow the reference hooks."""
 
     def __init__(self, mesh, params, device=0):
+        """params: one Params, or a list of Params (one per case) for a batched mesh."""
         self.mesh = mesh
         self.params = params
         h = _P()
-        _check(load_library().phl_create(mesh._h, ctypes.byref(params), device, ctypes.byref(h)))
+        if isinstance(params, (list, tuple)):
+            arr = (Params * len(params))(*params)
+            _check(load_library().phl_create_batch(mesh._h, arr, device, ctypes.byref(h)))
+            self.n_cases = len(params)
+        else:
+            _check(load_library().phl_create(mesh._h, ctypes.byref(params), device, ctypes.byref(h)))
+            self.n_cases = 1
         self._h = h
 
     # state ---------------------------------------------------------------
@@ -345,6 +381,15 @@ class Solver:
         """'graph' (default) replays captured two-step hipGraphs; 'eager' launches each kernel."""
         _check(_lib.phl_set_launch_mode(self._h, {"eager": 0, "graph": 1}[mode]))
 
+    def autotune(self):
+        """Pick the flux / gradient workgroup tiles by timing (phl_autotune)."""
+        ft, gt = ctypes.c_int(), ctypes.c_int()
+        ms = np.empty(6)
+        _check(_lib.phl_autotune(self._h, ctypes.byref(ft), ctypes.byref(gt), _dptr(ms)))
+        return dict(flux_tile=ft.value, grad_tile=gt.value,
+                    flux_ms=dict(zip((64, 128, 256), ms[:3].round(5).tolist())),
+                    grad_ms=dict(zip((64, 128, 256), ms[3:].round(5).tolist())))
+
     def set_precision(self, precision):
         """'fp64' or 'mixed' (FP32 gradients and face fluxes, FP64 accumulation)."""
         _check(_lib.phl_set_precision(self._h, {"fp64": 0, "mixed": 1}[precision]))
@@ -356,9 +401,25 @@ class Solver:
         return dict(zip(FORCE_NAMES, out))
 
     def iterate(self, first_step, n_steps, history=True):
-        hist = np.empty(max(n_steps, 1))
+        """Density-residual RMS per step ([n_steps], or [n_steps, n_cases] for a batch)."""
+        hist = np.empty((max(n_steps, 1), self.n_cases))
         _check(_lib.phl_iterate(self._h, first_step, n_steps, _dptr(hist) if history else None))
-        return hist[:n_steps] if history else None
+        if not history:
+            return None
+        return hist[:n_steps, 0] if self.n_cases == 1 else hist[:n_steps]
+
+    def solve_batch(self, max_steps, drop):
+        """Per-case steps to the drop (negative: not reached) and the [steps, n_cases] history."""
+        steps = np.empty(self.n_cases, np.int32)
+        hist = np.empty((max_steps, self.n_cases))
+        _check(_lib.phl_solve_batch(self._h, max_steps, drop, _iptr(steps), _dptr(hist)))
+        run = int(np.max(np.abs(steps)))
+        return steps, hist[:run]
+
+    def forces_case(self, k):
+        out = np.empty(len(FORCE_NAMES))
+        _check(_lib.phl_forces_case(self._h, k, _dptr(out)))
+        return dict(zip(FORCE_NAMES, out))
 
     def solve(self, max_steps, drop):
         hist = np.empty(max_steps)

@@ -33,6 +33,11 @@ struct phl_ctx {
     double *hostCfl = nullptr;   // pinned, CFL of each step of a chunk
     int *hostZero = nullptr;     // pinned constant 0
     double primInf[5];
+    // batched contexts: one parameter block and freestream per case
+    std::vector<phl_params> caseParams;
+    std::vector<phl::CaseConsts> hostCases;
+    std::vector<int> caseCellPtr, caseBFacePtr, caseWallPtr;
+    unsigned char *hostFrozen = nullptr;  // pinned [nCases]
     // kernel timing (HIP events on `stream`)
     bool timing = false;
     std::vector<hipEvent_t> events;
@@ -359,6 +364,7 @@ phl_status phl_mesh_bump_hybrid(int nx, int nz, int n_prism, double h1, int n_te
 phl_status phl_mesh_renumber(const phl_mesh *in, int method, const int *perm, int *n_colours, phl_mesh **out)
 {
     if (!in || !out) return fail(PHL_ERR_ARG, "null argument");
+    if (in->m.nCases() > 1) return fail(PHL_ERR_UNSUPPORTED, "phl_mesh_renumber: batched mesh");
     std::vector<int> pv;
     int nCol = 0;
     if (method == 0) {
@@ -378,6 +384,40 @@ phl_status phl_mesh_renumber(const phl_mesh *in, int method, const int *perm, in
     }
     if (n_colours) *n_colours = nCol;
     *out = pm;
+    return PHL_OK;
+}
+
+phl_status phl_mesh_concat(const phl_mesh *const *parts, int n, phl_mesh **out)
+{
+    if (!parts || n <= 0 || !out) return fail(PHL_ERR_ARG, "phl_mesh_concat: bad argument");
+    std::vector<const phl::Mesh *> pv;
+    for (int i = 0; i < n; ++i) {
+        if (!parts[i]) return fail(PHL_ERR_ARG, "phl_mesh_concat: null mesh");
+        pv.push_back(&parts[i]->m);
+    }
+    phl_mesh *pm = new (std::nothrow) phl_mesh;
+    if (!pm) return fail(PHL_ERR_ALLOC, "out of memory");
+    std::string err;
+    if (!phl::concatMeshes(pv, pm->m, err)) {
+        delete pm;
+        return fail(PHL_ERR_ARG, err);
+    }
+    *out = pm;
+    return PHL_OK;
+}
+
+phl_status phl_mesh_cases(const phl_mesh *pm, int *n_cases, int *cell_ptr, int *bface_ptr)
+{
+    if (!pm || !n_cases) return fail(PHL_ERR_ARG, "null argument");
+    const phl::Mesh &m = pm->m;
+    *n_cases = m.nCases();
+    if (m.nCases() == 1) {
+        if (cell_ptr) { cell_ptr[0] = 0; cell_ptr[1] = m.nCell; }
+        if (bface_ptr) { bface_ptr[0] = 0; bface_ptr[1] = m.nBFace; }
+        return PHL_OK;
+    }
+    if (cell_ptr) std::copy(m.caseCellPtr.begin(), m.caseCellPtr.end(), cell_ptr);
+    if (bface_ptr) std::copy(m.caseBFacePtr.begin(), m.caseBFacePtr.end(), bface_ptr);
     return PHL_OK;
 }
 
@@ -448,6 +488,7 @@ void phl_destroy(phl_ctx *c)
     if (c->stream) hipStreamSynchronize(c->stream);
     for (void *p : c->allocs) hipFree(p);
     if (c->hostNorm) hipHostFree(c->hostNorm);
+    if (c->hostFrozen) hipHostFree(c->hostFrozen);
     if (c->hostCount) hipHostFree(c->hostCount);
     if (c->hostCfl) hipHostFree(c->hostCfl);
     if (c->hostZero) hipHostFree(c->hostZero);
@@ -456,6 +497,33 @@ void phl_destroy(phl_ctx *c)
         if (g) hipGraphExecDestroy(g);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
+}
+
+// The freestream-dependent scalars of one case (Param_NSSolver and
+// PerfectGas nondimensionalisation: rho = 1, |V| = 1, p = 1/(gamma M^2)).
+static phl::CaseConsts case_consts(const phl_params &p, double avgVol)
+{
+    phl::CaseConsts cc{};
+    cc.R = 1.0 / (p.gamma * p.mach * p.mach);
+    cc.mach = p.mach;
+    const double kPiD = 3.141592653589793;
+    const double attack = p.alpha_deg * kPiD / 180.0, slide = p.beta_deg * kPiD / 180.0;
+    cc.primInf[0] = 1.0;
+    cc.primInf[1] = 1.0 * (std::cos(attack) * std::cos(slide));
+    cc.primInf[2] = 1.0 * (std::sin(attack) * std::cos(slide));
+    cc.primInf[3] = 1.0 * std::sin(slide);
+    cc.primInf[4] = cc.R * 1.0 * 1.0 / 1.0;
+    cc.entropyCoef = entropy_coef(p.mach, p.entropy_scale);
+    double e = p.venkat_k * p.venkat_k * p.venkat_k;
+    e *= 1.0 / std::pow(std::pow(avgVol, 1.0 / 3), 3.0);
+    cc.epsTmp = e;
+    const double pT = ptotal(p.mach, p.gamma);
+    cc.pLower = 0.001 * pT;
+    cc.pUpper = 0.02 * pT;
+    cc.cp = 1.0 / ((p.gamma - 1.0) * (p.mach * p.mach));
+    cc.densLimit = 1.0e-6 * cc.primInf[0];
+    cc.presLimit = 1.0e-6 * cc.primInf[4];
+    return cc;
 }
 
 static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
@@ -554,8 +622,13 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     dm.limGhost = nullptr;
     TRY(upload(c, &dm.visFlag, visFlag));
     std::vector<int> wall;
-    for (int f = 0; f < nB; ++f)
-        if (m.bcType[f] == PHL_BC_SOLID_SURFACE) wall.push_back(f);
+    c->caseWallPtr.assign(1, 0);
+    for (int i = 0; i < m.nCases(); ++i) {
+        const int b = m.nCases() > 1 ? m.caseBFacePtr[i] : 0, e = m.nCases() > 1 ? m.caseBFacePtr[i + 1] : nB;
+        for (int f = b; f < e; ++f)
+            if (m.bcType[f] == PHL_BC_SOLID_SURFACE) wall.push_back(f);
+        c->caseWallPtr.push_back((int)wall.size());
+    }
     dm.nWall = (int)wall.size();
     if (wall.empty()) wall.push_back(0);  // keeps the device pointer valid
     TRY(upload(c, &dm.wallFaces, wall));
@@ -581,40 +654,51 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         TRY(upload(c, &lv.so.nb, nbs));
     }
 
-    // constants
+    // constants: the scalars of every case, the first one also in k
     phl::Consts &k = c->k;
+    const int nCases = m.nCases();
+    if (c->caseParams.empty()) c->caseParams.assign(1, p);
+    if ((int)c->caseParams.size() != nCases) return fail(PHL_ERR_ARG, "one parameter block per case is required");
+    c->hostCases.resize(nCases);
+    for (int i = 0; i < nCases; ++i)
+        c->hostCases[i] = case_consts(c->caseParams[i], nCases > 1 ? m.caseAvgVol[i] : m.avgVol);
+    const phl::CaseConsts &c0 = c->hostCases[0];
     k.gamma = p.gamma;
-    k.R = 1.0 / (p.gamma * p.mach * p.mach);
-    k.mach = p.mach;
+    k.R = c0.R;
+    k.mach = c0.mach;
     k.reynolds = p.reynolds;
     k.oRe = p.reynolds > 0.0 ? 1.0 / p.reynolds : 0.0;
-    {
-        const double kPiD = 3.141592653589793;
-        const double attack = p.alpha_deg * kPiD / 180.0, slide = p.beta_deg * kPiD / 180.0;
-        c->primInf[0] = 1.0;
-        c->primInf[1] = 1.0 * (std::cos(attack) * std::cos(slide));
-        c->primInf[2] = 1.0 * (std::sin(attack) * std::cos(slide));
-        c->primInf[3] = 1.0 * std::sin(slide);
-        c->primInf[4] = k.R * 1.0 * 1.0 / 1.0;
-        for (int i = 0; i < 5; ++i) k.primInf[i] = c->primInf[i];
-    }
-    k.entropyCoef = entropy_coef(p.mach, p.entropy_scale);
-    {
-        double e = p.venkat_k * p.venkat_k * p.venkat_k;
-        e *= 1.0 / std::pow(std::pow(m.avgVol, 1.0 / 3), 3.0);
-        k.epsTmp = e;
-    }
-    const double pT = ptotal(p.mach, p.gamma);
-    k.pLower = 0.001 * pT;
-    k.pUpper = 0.02 * pT;
+    for (int i = 0; i < 5; ++i) k.primInf[i] = c->primInf[i] = c0.primInf[i];
+    k.entropyCoef = c0.entropyCoef;
+    k.epsTmp = c0.epsTmp;
+    k.pLower = c0.pLower;
+    k.pUpper = c0.pUpper;
     k.cflStart = p.cfl_start;
     k.ktmax = p.ktmax;
     k.tsuth = 110.4 / p.t_ref;
-    k.cp = 1.0 / ((p.gamma - 1.0) * (p.mach * p.mach));
+    k.cp = c0.cp;
     k.oprl = 1.0 / p.prandtl;
     k.oprt = 1.0 / p.prandtl_turb;
-    k.densLimit = 1.0e-6 * k.primInf[0];
-    k.presLimit = 1.0e-6 * k.primInf[4];
+    k.densLimit = c0.densLimit;
+    k.presLimit = c0.presLimit;
+    k.nCases = nCases;
+    k.cases = nullptr;
+    dm.caseOf = nullptr;
+    dm.caseCellPtr = nullptr;
+    if (nCases > 1) {
+        const phl::CaseConsts *dc = nullptr;
+        TRY(upload(c, &dc, c->hostCases));
+        k.cases = dc;
+        std::vector<unsigned short> caseOf(nT);
+        for (int i = 0; i < nCases; ++i) {
+            for (int cc = m.caseCellPtr[i]; cc < m.caseCellPtr[i + 1]; ++cc) caseOf[cc] = (unsigned short)i;
+            for (int f = m.caseBFacePtr[i]; f < m.caseBFacePtr[i + 1]; ++f) caseOf[nC + f] = (unsigned short)i;
+        }
+        TRY(upload(c, &dm.caseOf, caseOf));
+        TRY(upload(c, &dm.caseCellPtr, m.caseCellPtr));
+        c->caseCellPtr = m.caseCellPtr;
+        c->caseBFacePtr = m.caseBFacePtr;
+    }
     k.viscous = p.viscous;
     k.limiter = p.limiter;
     k.entropyFix = p.entropy_fix;
@@ -639,11 +723,16 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.dtLoc, (size_t)nC));
     TRY(dalloc(c, &fl.dt, (size_t)nC));
     TRY(dalloc(c, &fl.diag, (size_t)nC));
-    TRY(dalloc(c, &fl.normPart, (size_t)5 * phl::flux_blocks(dm)));
+    k.fluxTile = 256;
+    k.gradTile = 256;
+    const int maxFluxBlocks = (nC + 63) / 64;  // smallest tile
+    TRY(dalloc(c, &fl.normPart, (size_t)5 * maxFluxBlocks));
     c->normSlots = 1024;
-    TRY(dalloc(c, &fl.normOut, (size_t)5 * c->normSlots));
-    TRY(dalloc(c, &fl.dtMinPart, (size_t)std::max(phl::flux_blocks(dm), (nC + 255) / 256)));
-    TRY(dalloc(c, &fl.dtMin, 1));
+    TRY(dalloc(c, &fl.normOut, (size_t)5 * c->normSlots * nCases));
+    TRY(dalloc(c, &fl.dtMinPart, (size_t)maxFluxBlocks));
+    TRY(dalloc(c, &fl.dtMin, (size_t)nCases));
+    TRY(dalloc(c, &fl.frozen, (size_t)nCases));
+    HIPTRY(hipMemset(fl.frozen, 0, (size_t)nCases));
     TRY(dalloc(c, &fl.stepCtr, 1));
     TRY(dalloc(c, &fl.cflArr, (size_t)c->normSlots));
     TRY(dalloc(c, &fl.negFlag, (size_t)nC));
@@ -666,7 +755,9 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     HIPTRY(hipMemset(fl.negTotal, 0, sizeof(int)));
     HIPTRY(hipMemset(fl.stepCtr, 0, sizeof(int)));
     HIPTRY(hipMemcpy(fl.lim, limInit.data(), sizeof(double) * nT, hipMemcpyHostToDevice));
-    HIPTRY(hipHostMalloc((void **)&c->hostNorm, sizeof(double) * 5 * c->normSlots, hipHostMallocDefault));
+    HIPTRY(hipHostMalloc((void **)&c->hostNorm, sizeof(double) * 5 * c->normSlots * nCases, hipHostMallocDefault));
+    HIPTRY(hipHostMalloc((void **)&c->hostFrozen, (size_t)nCases, hipHostMallocDefault));
+    std::fill(c->hostFrozen, c->hostFrozen + nCases, (unsigned char)0);
     HIPTRY(hipHostMalloc((void **)&c->hostCount, sizeof(int), hipHostMallocDefault));
     HIPTRY(hipHostMalloc((void **)&c->hostCfl, sizeof(double) * c->normSlots, hipHostMallocDefault));
     HIPTRY(hipHostMalloc((void **)&c->hostZero, sizeof(int), hipHostMallocDefault));
@@ -678,6 +769,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
 phl_status phl_create(const phl_mesh *pm, const phl_params *p, int device, phl_ctx **out)
 {
     if (!pm || !p || !out) return fail(PHL_ERR_ARG, "phl_create: null argument");
+    if (pm->m.nCases() > 1) return fail(PHL_ERR_ARG, "phl_create: batched mesh, use phl_create_batch");
     TRY(check_params(*p));
     int nDev = 0;
     HIPTRY(hipGetDeviceCount(&nDev));
@@ -686,6 +778,39 @@ phl_status phl_create(const phl_mesh *pm, const phl_params *p, int device, phl_c
     if (!c) return fail(PHL_ERR_ALLOC, "phl_create: out of memory");
     c->device = device;
     c->p = *p;
+    phl_status s = create_impl(c, pm->m);
+    if (s != PHL_OK) {
+        std::string keep = g_err;
+        phl_destroy(c);
+        g_err = keep;
+        return s;
+    }
+    *out = c;
+    return PHL_OK;
+}
+
+phl_status phl_create_batch(const phl_mesh *pm, const phl_params *params, int device, phl_ctx **out)
+{
+    if (!pm || !params || !out) return fail(PHL_ERR_ARG, "phl_create_batch: null argument");
+    const int n = pm->m.nCases();
+    // the cases share every option; only the freestream (Mach, angles) varies
+    for (int i = 0; i < n; ++i) {
+        TRY(check_params(params[i]));
+        phl_params a = params[0], b = params[i];
+        a.mach = b.mach = 0.0;
+        a.alpha_deg = b.alpha_deg = 0.0;
+        a.beta_deg = b.beta_deg = 0.0;
+        if (std::memcmp(&a, &b, sizeof(phl_params)) != 0)
+            return fail(PHL_ERR_ARG, "phl_create_batch: cases may differ only in mach, alpha_deg, beta_deg");
+    }
+    int nDev = 0;
+    HIPTRY(hipGetDeviceCount(&nDev));
+    if (device < 0 || device >= nDev) return fail(PHL_ERR_ARG, "phl_create_batch: no such device");
+    phl_ctx *c = new (std::nothrow) phl_ctx;
+    if (!c) return fail(PHL_ERR_ALLOC, "phl_create_batch: out of memory");
+    c->device = device;
+    c->p = params[0];
+    c->caseParams.assign(params, params + n);
     phl_status s = create_impl(c, pm->m);
     if (s != PHL_OK) {
         std::string keep = g_err;
@@ -764,9 +889,20 @@ phl_status phl_init_freestream(phl_ctx *c)
 {
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
-    const int nT = c->m.nTotal;
+    const int nT = c->m.nTotal, nC = c->m.nCell;
     std::vector<double> h((size_t)5 * nT);
-    for (int m = 0; m < 5; ++m) std::fill(h.begin() + (size_t)m * nT, h.begin() + (size_t)(m + 1) * nT, c->primInf[m]);
+    if (c->k.nCases <= 1) {
+        for (int m = 0; m < 5; ++m)
+            std::fill(h.begin() + (size_t)m * nT, h.begin() + (size_t)(m + 1) * nT, c->primInf[m]);
+    } else {
+        for (int i = 0; i < c->k.nCases; ++i)
+            for (int m = 0; m < 5; ++m) {
+                double *row = h.data() + (size_t)m * nT;
+                const double v = c->hostCases[i].primInf[m];
+                std::fill(row + c->caseCellPtr[i], row + c->caseCellPtr[i + 1], v);
+                std::fill(row + nC + c->caseBFacePtr[i], row + nC + c->caseBFacePtr[i + 1], v);
+            }
+    }
     return phl_set_state(c, h.data());
 }
 
@@ -808,7 +944,7 @@ static void enqueue_residual(phl_ctx *c)
     phl::launch_boundary(c->m, c->k, c->f, c->stream);
     phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
     phl::launch_flux(c->m, c->k, c->f, c->stream, false);
-    phl::launch_norm_final(c->m, c->f, false, c->stream);
+    phl::launch_norm_final(c->m, c->k, c->f, false, c->stream);
 }
 
 phl_status phl_update_residuals(phl_ctx *c)
@@ -829,6 +965,7 @@ phl_status phl_spectrum_radius(phl_ctx *c)
 
 phl_status phl_time_step(phl_ctx *c, int step)
 {
+    if (c && c->k.nCases > 1) return fail(PHL_ERR_UNSUPPORTED, "not available on a batched context");
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     phl::launch_time_step(c->m, c->k, c->f, compute_cfl(c->p, step), c->stream);
@@ -847,6 +984,7 @@ static void enqueue_lusgs(phl_ctx *c)
 
 phl_status phl_lusgs(phl_ctx *c, int *n_fixed)
 {
+    if (c && c->k.nCases > 1) return fail(PHL_ERR_UNSUPPORTED, "not available on a batched context");
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     HIPTRY(hipMemsetAsync(c->f.negCount, 0, sizeof(int), c->stream));
@@ -901,22 +1039,34 @@ static void force_coefficients(const phl_params &p, const double *sums, double *
     out[PHL_FORCE_XCP] = (cfy < 0.0 ? -1.0 : 1.0) * cmz / (std::fabs(cfy) + 1.0e-40);
 }
 
-phl_status phl_forces(phl_ctx *c, double *coef)
+phl_status phl_forces_case(phl_ctx *c, int cs, double *coef)
 {
     if (!c || !coef) return fail(PHL_ERR_ARG, "null argument");
+    if (cs < 0 || cs >= c->k.nCases) return fail(PHL_ERR_ARG, "phl_forces_case: no such case");
     HIPTRY(hipSetDevice(c->device));
     hipStream_t s = c->stream;
     phl::launch_boundary(c->m, c->k, c->f, s);
     phl::launch_gamma_t(c->m, c->k, c->f, s);
     phl::launch_grad_lim(c->m, c->k, c->f, s, true);
-    const double ref[3] = {c->p.force_ref_x, c->p.force_ref_y, c->p.force_ref_z};
-    phl::launch_forces(c->m, c->k, c->f, ref, s);
+    // the case's wall faces are a contiguous run of the sorted wall list
+    phl::DevMesh dm = c->m;
+    phl::Consts k = c->k;
+    const phl_params &pc = c->caseParams.empty() ? c->p : c->caseParams[cs];
+    if (c->k.nCases > 1) {
+        dm.wallFaces = c->m.wallFaces + c->caseWallPtr[cs];
+        dm.nWall = c->caseWallPtr[cs + 1] - c->caseWallPtr[cs];
+        for (int i = 0; i < 5; ++i) k.primInf[i] = c->hostCases[cs].primInf[i];
+    }
+    const double ref[3] = {pc.force_ref_x, pc.force_ref_y, pc.force_ref_z};
+    phl::launch_forces(dm, k, c->f, ref, s);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.forceOut, sizeof(double) * 15, hipMemcpyDeviceToHost, s));
     HIPTRY(hipStreamSynchronize(s));
-    force_coefficients(c->p, c->hostNorm, coef);
+    force_coefficients(pc, c->hostNorm, coef);
     return PHL_OK;
 }
+
+phl_status phl_forces(phl_ctx *c, double *coef) { return phl_forces_case(c, 0, coef); }
 
 // ----------------------------------------------------------- host driver
 static const int kTimedChunk = 64;
@@ -936,7 +1086,7 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     phl::launch_boundary(c->m, c->k, c->f, s); mark();
     phl::launch_grad_lim(c->m, c->k, c->f, s); mark();
     phl::launch_flux(c->m, c->k, c->f, s, true); mark();
-    phl::launch_norm_final(c->m, c->f, true, s); mark();
+    phl::launch_norm_final(c->m, c->k, c->f, true, s); mark();
     phl::launch_dt_diag(c->m, c->k, c->f, s); mark();
     phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, s); mark();
     phl::launch_ghost_dq(c->m, c->k, c->f, s); mark();
@@ -984,6 +1134,58 @@ phl_status phl_set_precision(phl_ctx *c, int precision)
     return PHL_OK;
 }
 
+// Autotuner: time the fused flux kernel and the gradient kernel at each
+// workgroup tile on the current state (a few launches each, HIP events on
+// the context's stream) and keep the fastest.  The tile changes only the
+// grouping of the residual partial sums, never a cell's arithmetic.
+phl_status phl_autotune(phl_ctx *c, int *flux_tile, int *grad_tile, double *ms_table)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    static const int tiles[3] = {64, 128, 256};
+    hipEvent_t e0, e1;
+    HIPTRY(hipEventCreate(&e0));
+    HIPTRY(hipEventCreate(&e1));
+    const int reps = 5;
+    double best[2] = {1e300, 1e300};
+    int pick[2] = {256, 256};
+    const phl::Consts keep = c->k;
+    // the flux kernel reads the step's CFL from cflArr[*stepCtr]
+    c->hostCfl[0] = compute_cfl(c->p, 1);
+    HIPTRY(hipMemcpyAsync(c->f.cflArr, c->hostCfl, sizeof(double), hipMemcpyHostToDevice, c->stream));
+    reset_step_counter(c);
+    phl::launch_boundary(c->m, c->k, c->f, c->stream);
+    for (int kind = 0; kind < 2; ++kind)
+        for (int t = 0; t < 3; ++t) {
+            phl::Consts k = keep;
+            if (kind == 0) k.fluxTile = tiles[t]; else k.gradTile = tiles[t];
+            float ms = 0.f;
+            for (int r = 0; r <= reps; ++r) {  // the first launch warms up
+                if (r == 1) HIPTRY(hipEventRecord(e0, c->stream));
+                if (kind == 0) phl::launch_flux(c->m, k, c->f, c->stream, true);
+                else phl::launch_grad_lim(c->m, k, c->f, c->stream);
+            }
+            HIPTRY(hipEventRecord(e1, c->stream));
+            HIPTRY(hipEventSynchronize(e1));
+            HIPTRY(hipEventElapsedTime(&ms, e0, e1));
+            const double per = ms / reps;
+            if (ms_table) ms_table[kind * 3 + t] = per;
+            if (per < best[kind]) { best[kind] = per; pick[kind] = tiles[t]; }
+        }
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    TRY(check_launch(c));
+    c->k.fluxTile = pick[0];
+    c->k.gradTile = pick[1];
+    for (hipGraphExec_t &g : c->stepPair) {  // the graphs hold the old tiles
+        if (g) hipGraphExecDestroy(g);
+        g = nullptr;
+    }
+    if (flux_tile) *flux_tile = pick[0];
+    if (grad_tile) *grad_tile = pick[1];
+    return PHL_OK;
+}
+
 phl_status phl_set_launch_mode(phl_ctx *c, int mode)
 {
     if (!c || (mode != 0 && mode != 1)) return fail(PHL_ERR_ARG, "launch mode must be 0 (eager) or 1 (graph)");
@@ -1009,6 +1211,8 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         HIPTRY(hipStreamSynchronize(c->stream));  // hostCfl may still be in use
         for (int i = 0; i < chunk; ++i) c->hostCfl[i] = compute_cfl(c->p, first_step + done + i);
         HIPTRY(hipMemcpyAsync(c->f.cflArr, c->hostCfl, sizeof(double) * chunk, hipMemcpyHostToDevice, c->stream));
+        if (c->k.nCases > 1)
+            HIPTRY(hipMemcpyAsync(c->f.frozen, c->hostFrozen, (size_t)c->k.nCases, hipMemcpyHostToDevice, c->stream));
         reset_step_counter(c);
         int i = 0;
         if (!c->timing && c->launchMode == 1) {
@@ -1021,7 +1225,8 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         for (; i < chunk; ++i)
             enqueue_step(c, c->timing ? &c->events[(size_t)i * (K + 1)] : nullptr);
         TRY(check_launch(c));
-        HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5 * chunk, hipMemcpyDeviceToHost,
+        const int nCases = c->k.nCases;
+        HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5 * chunk * nCases, hipMemcpyDeviceToHost,
                               c->stream));
         HIPTRY(hipStreamSynchronize(c->stream));
         if (c->timing) {
@@ -1034,12 +1239,16 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
                 }
             c->timedSteps += chunk;
         }
-        for (int i = 0; i < chunk; ++i) {
-            const double s0 = c->hostNorm[(size_t)i * 5];
-            if (!std::isfinite(s0))
-                return fail(PHL_ERR_NUMERIC, "residual is not finite at step " + std::to_string(first_step + done + i));
-            if (rms_hist) rms_hist[done + i] = std::sqrt(s0 / c->m.nCell);
-        }
+        for (int i = 0; i < chunk; ++i)
+            for (int k = 0; k < nCases; ++k) {
+                const double s0 = c->hostNorm[((size_t)i * nCases + k) * 5];
+                if (!std::isfinite(s0))
+                    return fail(PHL_ERR_NUMERIC, "residual is not finite at step " +
+                                                     std::to_string(first_step + done + i) + " of case " +
+                                                     std::to_string(k));
+                const int nCk = nCases > 1 ? c->caseCellPtr[k + 1] - c->caseCellPtr[k] : c->m.nCell;
+                if (rms_hist) rms_hist[(size_t)(done + i) * nCases + k] = std::sqrt(s0 / nCk);
+            }
         done += chunk;
     }
     return PHL_OK;
@@ -1091,6 +1300,47 @@ phl_status phl_hbm_copy(int device, double bytes, int iters, double *gbps)
     hipEventDestroy(e0); hipEventDestroy(e1); hipStreamDestroy(s);
     hipFree(a); hipFree(b);
     *gbps = best;
+    return PHL_OK;
+}
+
+// Batched solve: every case runs until its own density residual has dropped
+// by `drop`; a converged case is frozen (its state is no longer updated) at
+// the end of the 32-step chunk in which it converged.
+phl_status phl_solve_batch(phl_ctx *c, int max_steps, double drop, int *steps_done, double *rms_hist)
+{
+    if (!c || max_steps <= 0 || !(drop > 0.0) || !steps_done) return fail(PHL_ERR_ARG, "bad argument");
+    const int n = c->k.nCases;
+    std::vector<double> hist((size_t)max_steps * n);
+    std::vector<double> r1(n, 0.0);
+    std::vector<int> hit(n, -1);
+    if (n > 1) std::fill(c->hostFrozen, c->hostFrozen + n, (unsigned char)0);
+    const int chunk = 32;
+    int done = 0, left = n;
+    while (done < max_steps && left > 0) {
+        const int m = std::min(chunk, max_steps - done);
+        TRY(phl_iterate(c, 1 + done, m, hist.data() + (size_t)done * n));
+        for (int k = 0; k < n; ++k) {
+            if (done == 0) r1[k] = hist[k];
+            if (hit[k] >= 0) continue;
+            for (int i = done; i < done + m; ++i)
+                if (hist[(size_t)i * n + k] <= drop * r1[k]) { hit[k] = i; break; }
+            if (hit[k] >= 0) {
+                --left;
+                if (n > 1) c->hostFrozen[k] = 1;
+            }
+        }
+        done += m;
+    }
+    for (int k = 0; k < n; ++k) steps_done[k] = hit[k] >= 0 ? hit[k] + 1 : -done;
+    if (rms_hist) std::copy(hist.begin(), hist.begin() + (size_t)done * n, rms_hist);
+    if (n > 1) std::fill(c->hostFrozen, c->hostFrozen + n, (unsigned char)0);
+    return PHL_OK;
+}
+
+phl_status phl_batch_size(const phl_ctx *c, int *n_cases)
+{
+    if (!c || !n_cases) return fail(PHL_ERR_ARG, "null argument");
+    *n_cases = c->k.nCases;
     return PHL_OK;
 }
 

@@ -36,6 +36,16 @@ struct DevMesh {
     const unsigned char *visFlag;  // [nFace] bit0: skewness correction, bit1: normal derivative
     const int *wallFaces;       // [nWall] solid-surface faces, increasing
     int nWall;
+    const unsigned short *caseOf;  // [nTotal] case of each cell (batched contexts)
+    const int *caseCellPtr;        // [nCases+1] cell range of each case
+};
+
+// The scalars that depend on a case's freestream (Mach, angles) and mesh
+// (mean volume).  A batched context holds one per case.
+struct CaseConsts {
+    double R, mach;
+    double primInf[5];
+    double entropyCoef, epsTmp, pLower, pUpper, cp, densLimit, presLimit;
 };
 
 // Scalars of the path, fixed per context.
@@ -52,6 +62,9 @@ struct Consts {
     double densLimit, presLimit;  // 1e-6 freestream
     int viscous, limiter, entropyFix, nSweeps;
     int mixed;              // 1: FP32 gradients and face fluxes, FP64 accumulation
+    int nCases;             // > 1: batched context, per-case scalars in cases[caseOf[cell]]
+    int fluxTile, gradTile; // cells per workgroup of the flux and gradient kernels (64, 128, 256)
+    const CaseConsts *cases;
 };
 
 // Mutable flow state on the device.
@@ -68,9 +81,9 @@ struct DevFlow {
     double *dflux;    // [5][nCell]
     double *specInv, *specVis, *dtLoc, *dt, *diag;  // [nCell]
     double *normPart;   // [nBlocksFlux][5]
-    double *normOut;    // [maxSteps][5], row = *stepCtr
+    double *normOut;    // [maxSteps][nCases][5], row = *stepCtr
     double *dtMinPart;  // [nBlocksFlux] block minima of the local time step
-    double *dtMin;      // global minimum (ComputeMinTimeStep)
+    double *dtMin;      // [nCases] minimum local time step (ComputeMinTimeStep)
     int *stepCtr;       // step index within the current chunk (device side)
     double *cflArr;     // [maxSteps] CFL of each step of the chunk
     unsigned char *negFlag;  // [nCell]
@@ -79,6 +92,7 @@ struct DevFlow {
     double *coef;       // [nFace][2][16] LU-SGS face coefficients per side
     double *t, *visl;   // [nTotal], only for the standalone gamma/T hook
     double *forceOut;   // [15] wall force and moment sums
+    unsigned char *frozen;  // [nCases] batched contexts: 1 once a case has converged
 };
 
 // LU-SGS level schedule.
@@ -108,8 +122,9 @@ void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad = false);
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt);
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
-int flux_blocks(const DevMesh &m);
-void launch_norm_final(const DevMesh &m, DevFlow &f, bool withDt, hipStream_t s);
+// Workgroups of the fused flux launch (one partial-sum row each).
+int flux_blocks(const DevMesh &m, const Consts &k);
+void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s);
 void launch_time_step(const DevMesh &m, const Consts &k, DevFlow &f, double cfl, hipStream_t s);
 void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,

@@ -36,6 +36,27 @@ __device__ __forceinline__ double dABS(double a) { return (a < 0) ? -a : a; }
 template <typename T> __device__ __forceinline__ T tMAX(T a, T b) { return (a > b) ? a : b; }
 template <typename T> __device__ __forceinline__ T tABS(T a) { return (a < T(0)) ? -a : a; }
 
+// The context scalars as seen from one cell: in a batched context the
+// freestream-dependent ones come from the cell's case.
+__device__ __forceinline__ Consts case_k(const Consts &K, const DevMesh &M, int cell)
+{
+    if (K.nCases <= 1) return K;
+    Consts k = K;
+    const CaseConsts &cc = K.cases[M.caseOf[cell]];
+    k.R = cc.R;
+    k.mach = cc.mach;
+#pragma unroll
+    for (int m = 0; m < 5; ++m) k.primInf[m] = cc.primInf[m];
+    k.entropyCoef = cc.entropyCoef;
+    k.epsTmp = cc.epsTmp;
+    k.pLower = cc.pLower;
+    k.pUpper = cc.pUpper;
+    k.cp = cc.cp;
+    k.densLimit = cc.densLimit;
+    k.presLimit = cc.presLimit;
+    return k;
+}
+
 // Blocks are dispatched round-robin over the 8 XCDs (block b lands on XCD
 // b % 8).  Give each XCD a contiguous range of cells so that neighbouring
 // cells share an L2.
@@ -149,12 +170,13 @@ __device__ __forceinline__ void load5(const double *__restrict__ a, int n, int i
 }
 
 // ------------------------------------------------------------------ BCs
-__global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K, double *__restrict__ q)
+__global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K0, double *__restrict__ q)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= M.nBFace) return;
     const int nT = M.nTotal;
     const int le = M.fle[f], re = M.nCell + f;
+    const Consts K = case_k(K0, M, le);
     double ql[5], qr[5];
     load5(q, nT, le, ql);
 #pragma unroll
@@ -215,11 +237,12 @@ __global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K, double
 }
 
 // Standalone gamma/T/viscosity hook (the fused kernels recompute T and mu).
-__global__ void __launch_bounds__(kBlock) k_gamma_t(DevMesh M, Consts K, const double *__restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_gamma_t(DevMesh M, Consts K0, const double *__restrict__ q,
                                                     double *__restrict__ t, double *__restrict__ visl)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= M.nTotal) return;
+    const Consts K = case_k(K0, M, i);
     const int nT = M.nTotal;
     const double tt = q[(size_t)4 * nT + i] / (K.R * q[i]);
     t[i] = tt;
@@ -357,14 +380,15 @@ __device__ __forceinline__ void sweep_apply(const Consts &K, const double *co, c
 // ------------------------------------------------ gradients + limiter
 // GT is the storage type of the gradients: double, or float for the mixed
 // form (computed in FP64, stored rounded to FP32).
-template <typename GT>
-__global__ void __launch_bounds__(kBlock) k_grad_lim(DevMesh M, Consts K, const double *__restrict__ q,
+template <typename GT, int BS>
+__global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const double *__restrict__ q,
                                                      GT *__restrict__ grad, GT *__restrict__ gradT,
                                                      double *__restrict__ lim, double *__restrict__ coef)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
     if (c >= nC) return;
+    const Consts K = case_k(K0, M, c);
     double qc[5];
     load5(q, nT, c, qc);
     double gx[5], gy[5], gz[5];
@@ -668,8 +692,8 @@ __device__ __forceinline__ double local_dt(const DevMesh &M, const Consts &K, in
 // minimum (CFL of the current step read from cflArr[*stepCtr]).
 // T is the face arithmetic type: double, or float in the mixed form (FP32
 // gradients, reconstruction, Roe and viscous fluxes; FP64 accumulation).
-template <bool SPEC_ONLY, bool FUSED_DT, typename T>
-__global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const double *__restrict__ q,
+template <bool SPEC_ONLY, bool FUSED_DT, typename T, int BS>
+__global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double *__restrict__ q,
                                                  const T *__restrict__ grad,
                                                  const T *__restrict__ gradT,
                                                  const double *__restrict__ lim, double *__restrict__ res,
@@ -684,6 +708,7 @@ __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const doub
     double acc[5] = {-0.0, -0.0, -0.0, -0.0, -0.0};
     double dtl = 1.0e300;
     if (c < nC) {
+        const Consts K = case_k(K0, M, c);
         const bool needG = !SPEC_ONLY && (K.limiter != 2 || K.viscous);
         CellSide<T> own;
         load_side(M, q, grad, lim, c, needG, own);
@@ -814,12 +839,12 @@ __global__ void __launch_bounds__(kBlock) k_flux(DevMesh M, Consts K, const doub
     }
     if (SPEC_ONLY) return;
     // deterministic block partials: sum of res^2 and min of the local dt
-    __shared__ double red[6][kBlock];
+    __shared__ double red[6][BS];
 #pragma unroll
     for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = (c < nC) ? acc[m] * acc[m] : 0.0;
     red[5][threadIdx.x] = dtl;
     __syncthreads();
-    for (int w = kBlock / 2; w > 0; w >>= 1) {
+    for (int w = BS / 2; w > 0; w >>= 1) {
         if ((int)threadIdx.x < w) {
 #pragma unroll
             for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
@@ -865,9 +890,46 @@ __global__ void k_norm_final(const double *__restrict__ part, const double *__re
     if (threadIdx.x == 5 && withDt) *dtMin = red[5][0];
 }
 
+// Batched contexts: per-case residual sums of the step and minimum local time
+// step, one workgroup per case, in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_case_reduce(DevMesh M, const double *__restrict__ res,
+                                                        const double *__restrict__ dtLoc, int nCases,
+                                                        double *__restrict__ normOut, const int *__restrict__ stepCtr,
+                                                        double *__restrict__ dtMin)
+{
+    __shared__ double red[6][kBlock];
+    const int cs = blockIdx.x;
+    const int b = M.caseCellPtr[cs], e = M.caseCellPtr[cs + 1];
+    const int nC = M.nCell;
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double mn = 1.0e300;
+    for (int c = b + (int)threadIdx.x; c < e; c += blockDim.x) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            const double r = res[(size_t)m * nC + c];
+            a[m] += r * r;
+        }
+        mn = dMIN(mn, dtLoc[c]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = a[m];
+    red[5][threadIdx.x] = mn;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+            red[5][threadIdx.x] = dMIN(red[5][threadIdx.x], red[5][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x < 5) normOut[((size_t)(*stepCtr) * nCases + cs) * 5 + threadIdx.x] = red[threadIdx.x][0];
+    if (threadIdx.x == 5) dtMin[cs] = red[5][0];
+}
+
 // ---------------------------------------------------------- time step
 // Standalone TimeStep hook: local dt and block minima.
-__global__ void __launch_bounds__(kBlock) k_time_step(DevMesh M, Consts K, const double *__restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_time_step(DevMesh M, Consts K0, const double *__restrict__ q,
                                                       const double *__restrict__ specInv,
                                                       const double *__restrict__ specVis, double cfl,
                                                       double *__restrict__ dtLoc, double *__restrict__ dtMinPart)
@@ -875,6 +937,7 @@ __global__ void __launch_bounds__(kBlock) k_time_step(DevMesh M, Consts K, const
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     double dtl = 1.0e300;
     if (c < M.nCell) {
+        const Consts K = case_k(K0, M, c);
         double spec = specInv[c];
         if (K.viscous) spec += specVis[c];
         dtl = local_dt(M, K, c, q[(size_t)4 * M.nTotal + c], spec, cfl);
@@ -913,7 +976,7 @@ __global__ void __launch_bounds__(kBlock) k_dt_diag(DevMesh M, Consts K, const d
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= M.nCell) return;
-    const double mn = *dtMin;
+    const double mn = dtMin[K.nCases > 1 ? M.caseOf[c] : 0];
     double d = dtLoc[c];
     if (K.ktmax > 0) d = dMIN(d, K.ktmax * mn);
     d /= M.vol[c];
@@ -1206,16 +1269,20 @@ __global__ void __launch_bounds__(kBlock) k_ghost_dq(DevMesh M, Consts K, const 
 }
 
 // ------------------------------------------------------------- update
-__global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K, const double *__restrict__ q,
+__global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K0, const double *__restrict__ q,
                                                    const double *__restrict__ dq, double *__restrict__ qNew,
-                                                   unsigned char *__restrict__ negFlag, int *__restrict__ negCount)
+                                                   unsigned char *__restrict__ negFlag, int *__restrict__ negCount,
+                                                   const unsigned char *__restrict__ frozen)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nT = M.nTotal;
     if (i >= nT) return;
+    const Consts K = case_k(K0, M, i);
     double prim[5];
     load5(q, nT, i, prim);
-    if (i < M.nCell) {
+    // a converged case of a batch keeps its state
+    const bool hold = K.nCases > 1 && frozen[M.caseOf[i]];
+    if (i < M.nCell && !hold) {
         const double g = K.gamma;
         const double v2 = prim[1] * prim[1] + prim[2] * prim[2] + prim[3] * prim[3];
         const double em = (prim[4] / prim[0]) / (g - 1.0) + 0.5 * v2;
@@ -1246,7 +1313,7 @@ __global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K, const do
 // neighbour below the cell is read from the updated field, one above from
 // the old field, exactly as the reference's in-place loop sees them.  With
 // `advance` the step counter moves on and the fix count is accumulated.
-__global__ void k_fix(DevMesh M, Consts K, const double *__restrict__ qOld, double *__restrict__ qNew,
+__global__ void k_fix(DevMesh M, Consts K0, const double *__restrict__ qOld, double *__restrict__ qNew,
                       unsigned char *__restrict__ negFlag, int *__restrict__ negCount, int *__restrict__ negTotal,
                       int *__restrict__ stepCtr, int advance)
 {
@@ -1255,6 +1322,7 @@ __global__ void k_fix(DevMesh M, Consts K, const double *__restrict__ qOld, doub
     const int count = *negCount;
     for (int c = 0; c < nC && count > 0; ++c) {
         if (!negFlag[c]) continue;
+        const Consts K = case_k(K0, M, c);
         negFlag[c] = 0;
         double prim[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
         double vs = 0.0;
@@ -1412,46 +1480,68 @@ void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
     hipLaunchKernelGGL(k_gamma_t, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.t, f.visl);
 }
 
-void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad)
+template <int BS>
+static void launch_grad_lim_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp32)
 {
-    if (k.mixed && !fp64Grad)
-        hipLaunchKernelGGL(k_grad_lim<float>, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.gradF,
+    if (fp32)
+        hipLaunchKernelGGL((k_grad_lim<float, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.gradF,
                            f.gradTF, f.lim, f.coef);
     else
-        hipLaunchKernelGGL(k_grad_lim<double>, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.grad,
+        hipLaunchKernelGGL((k_grad_lim<double, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.grad,
                            f.gradT, f.lim, f.coef);
 }
 
-int flux_blocks(const DevMesh &m) { return nblk(m.nCell, kBlock); }
+void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad)
+{
+    const bool fp32 = k.mixed && !fp64Grad;
+    switch (k.gradTile) {
+    case 64: launch_grad_lim_t<64>(m, k, f, s, fp32); break;
+    case 128: launch_grad_lim_t<128>(m, k, f, s, fp32); break;
+    default: launch_grad_lim_t<256>(m, k, f, s, fp32); break;
+    }
+}
 
-template <bool FUSED_DT>
+int flux_blocks(const DevMesh &m, const Consts &k) { return nblk(m.nCell, k.fluxTile); }
+
+template <bool FUSED_DT, int BS>
 static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
+    const int nb = nblk(m.nCell, BS);
     if (k.mixed)
-        hipLaunchKernelGGL((k_flux<false, FUSED_DT, float>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q,
-                           f.gradF, f.gradTF, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr,
-                           f.dtLoc, f.dtMinPart);
+        hipLaunchKernelGGL((k_flux<false, FUSED_DT, float, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, f.gradF,
+                           f.gradTF, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
+                           f.dtMinPart);
     else
-        hipLaunchKernelGGL((k_flux<false, FUSED_DT, double>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q,
-                           f.grad, f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr,
-                           f.dtLoc, f.dtMinPart);
+        hipLaunchKernelGGL((k_flux<false, FUSED_DT, double, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, f.grad,
+                           f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
+                           f.dtMinPart);
 }
 
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt)
 {
-    if (fusedDt) launch_flux_t<true>(m, k, f, s);
-    else launch_flux_t<false>(m, k, f, s);
+    if (!fusedDt) return launch_flux_t<false, 256>(m, k, f, s);  // the standalone residual hook
+    switch (k.fluxTile) {
+    case 64: launch_flux_t<true, 64>(m, k, f, s); break;
+    case 128: launch_flux_t<true, 128>(m, k, f, s); break;
+    default: launch_flux_t<true, 256>(m, k, f, s); break;
+    }
 }
 
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_flux<true, false, double>), dim3(flux_blocks(m)), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.gradT,
+    hipLaunchKernelGGL((k_flux<true, false, double, 256>), dim3(nblk(m.nCell, 256)), dim3(256), 0, s, m, k, f.q, f.grad, f.gradT,
                        f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart);
 }
 
-void launch_norm_final(const DevMesh &m, DevFlow &f, bool withDt, hipStream_t s)
+void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, s, f.normPart, f.dtMinPart, flux_blocks(m), f.normOut,
+    if (k.nCases > 1) {
+        hipLaunchKernelGGL(k_case_reduce, dim3(k.nCases), dim3(kBlock), 0, s, m, f.res, f.dtLoc, k.nCases, f.normOut,
+                           f.stepCtr, f.dtMin);
+        return;
+    }
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, s, f.normPart, f.dtMinPart,
+                       withDt ? flux_blocks(m, k) : nblk(m.nCell, 256), f.normOut,
                        f.stepCtr, f.dtMin, withDt ? 1 : 0);
 }
 
@@ -1537,7 +1627,7 @@ void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, hipStream_t s)
 {
     hipLaunchKernelGGL(k_update, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.dq, f.qAlt,
-                       f.negFlag, f.negCount);
+                       f.negFlag, f.negCount, f.frozen);
     hipLaunchKernelGGL(k_fix, dim3(1), dim3(1), 0, s, m, k, f.q, f.qAlt, f.negFlag, f.negCount, f.negTotal,
                        f.stepCtr, advance ? 1 : 0);
 }

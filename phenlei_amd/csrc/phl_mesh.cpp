@@ -627,4 +627,80 @@ std::vector<int> colourOrder(const Mesh &m, int &nColours)
     return perm;
 }
 
+bool concatMeshes(const std::vector<const Mesh *> &parts, Mesh &out, std::string &err)
+{
+    if (parts.empty()) { err = "no meshes to concatenate"; return false; }
+    Mesh m;
+    long nNode = 0, nCell = 0, nFace = 0, nB = 0;
+    for (const Mesh *p : parts) {
+        if (!p->caseCellPtr.empty()) { err = "cannot concatenate batched meshes"; return false; }
+        nNode += p->nNode; nCell += p->nCell; nFace += p->nFace; nB += p->nBFace;
+    }
+    if (nCell + nB > 2147483000L || parts.size() > 65535) { err = "batch too large"; return false; }
+    m.nNode = (int)nNode; m.nCell = (int)nCell; m.nFace = (int)nFace; m.nBFace = (int)nB;
+    const int nT = m.nCell + m.nBFace;
+    m.left.resize(nFace); m.right.resize(nFace); m.bcType.resize(nB);
+    m.faceNodePtr.assign(nFace + 1, 0);
+    m.cellNodePtr.assign(m.nCell + 1, 0);
+    m.xfc.resize(nFace); m.yfc.resize(nFace); m.zfc.resize(nFace);
+    m.xfn.resize(nFace); m.yfn.resize(nFace); m.zfn.resize(nFace); m.area.resize(nFace);
+    m.xcc.resize(nT); m.ycc.resize(nT); m.zcc.resize(nT); m.vol.resize(nT);
+    m.skew.resize(m.nCell);
+    // face index in the union of each part's face f
+    std::vector<std::vector<int>> fmap(parts.size());
+    int bOff = 0, iOff = m.nBFace, cOff = 0, nOff = 0;
+    m.caseCellPtr.push_back(0);
+    m.caseBFacePtr.push_back(0);
+    for (size_t k = 0; k < parts.size(); ++k) {
+        const Mesh &p = *parts[k];
+        fmap[k].resize(p.nFace);
+        for (int f = 0; f < p.nFace; ++f) fmap[k][f] = f < p.nBFace ? bOff + f : iOff + (f - p.nBFace);
+        bOff += p.nBFace; iOff += p.nFace - p.nBFace;
+        cOff += p.nCell; nOff += p.nNode;
+        m.caseCellPtr.push_back(cOff);
+        m.caseBFacePtr.push_back(bOff);
+        m.caseAvgVol.push_back(p.avgVol);
+    }
+    // face node counts, then nodes
+    for (size_t k = 0; k < parts.size(); ++k) {
+        const Mesh &p = *parts[k];
+        for (int f = 0; f < p.nFace; ++f) m.faceNodePtr[fmap[k][f] + 1] = p.faceNodePtr[f + 1] - p.faceNodePtr[f];
+    }
+    for (long f = 0; f < nFace; ++f) m.faceNodePtr[f + 1] += m.faceNodePtr[f];
+    m.faceNodes.resize(m.faceNodePtr[nFace]);
+    cOff = 0; nOff = 0;
+    int cnOff = 0;
+    for (size_t k = 0; k < parts.size(); ++k) {
+        const Mesh &p = *parts[k];
+        for (int i = 0; i < p.nNode; ++i) { m.x.push_back(p.x[i]); m.y.push_back(p.y[i]); m.z.push_back(p.z[i]); }
+        for (int f = 0; f < p.nFace; ++f) {
+            const int g = fmap[k][f];
+            int o = m.faceNodePtr[g];
+            for (int j = p.faceNodePtr[f]; j < p.faceNodePtr[f + 1]; ++j) m.faceNodes[o++] = p.faceNodes[j] + nOff;
+            m.left[g] = p.left[f] + cOff;
+            m.right[g] = f < p.nBFace ? m.nCell + g : p.right[f] + cOff;
+            if (f < p.nBFace) m.bcType[g] = p.bcType[f];
+            m.xfc[g] = p.xfc[f]; m.yfc[g] = p.yfc[f]; m.zfc[g] = p.zfc[f];
+            m.xfn[g] = p.xfn[f]; m.yfn[g] = p.yfn[f]; m.zfn[g] = p.zfn[f]; m.area[g] = p.area[f];
+            if (f < p.nBFace) {  // ghost cell metrics
+                const int gs = p.nCell + f, gd = m.nCell + g;
+                m.xcc[gd] = p.xcc[gs]; m.ycc[gd] = p.ycc[gs]; m.zcc[gd] = p.zcc[gs]; m.vol[gd] = p.vol[gs];
+            }
+        }
+        for (int c = 0; c < p.nCell; ++c) {
+            m.xcc[cOff + c] = p.xcc[c]; m.ycc[cOff + c] = p.ycc[c]; m.zcc[cOff + c] = p.zcc[c];
+            m.vol[cOff + c] = p.vol[c]; m.skew[cOff + c] = p.skew[c];
+            m.cellNodePtr[cOff + c + 1] = p.cellNodePtr[c + 1] - p.cellNodePtr[c];
+        }
+        for (int j = 0; j < p.cellNodePtr[p.nCell]; ++j) m.cellNodes.push_back(p.cellNodes[j] + nOff);
+        cOff += p.nCell; nOff += p.nNode; cnOff += p.cellNodePtr[p.nCell];
+    }
+    for (int c = 0; c < m.nCell; ++c) m.cellNodePtr[c + 1] += m.cellNodePtr[c];
+    double vs = 0.0;
+    for (int c = 0; c < m.nCell; ++c) vs += m.vol[c];
+    m.avgVol = vs / m.nCell;
+    out = std::move(m);
+    return true;
+}
+
 }  // namespace phl

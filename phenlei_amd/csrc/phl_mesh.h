@@ -23,9 +23,24 @@ struct Mesh {
     std::vector<double> skew;                                // [nCell], degrees
     double avgVol = 0.0;
 
+    // Cases of a concatenated (batched) mesh: case k owns cells
+    // [caseCellPtr[k], caseCellPtr[k+1]) and boundary faces
+    // [caseBFacePtr[k], caseBFacePtr[k+1]); avgVol of each case.  Empty for
+    // a single-case mesh.
+    std::vector<int> caseCellPtr, caseBFacePtr;
+    std::vector<double> caseAvgVol;
+
     int nTotal() const { return nCell + nBFace; }
+    int nCases() const { return caseCellPtr.empty() ? 1 : (int)caseCellPtr.size() - 1; }
     void computeMetrics();
 };
+
+// Disjoint union of meshes, one case each: cells and nodes concatenated in
+// case order, all boundary faces first (case by case), then all interior
+// faces (case by case).  Each cell keeps its faces in the same relative
+// order, and the metrics are copied, so every case sees exactly the
+// connectivity, metrics and sweep order of its own mesh.
+bool concatMeshes(const std::vector<const Mesh *> &parts, Mesh &out, std::string &err);
 
 // One cell of a generic element mesh, before faces are derived.
 enum CellKind { TET = 0, PYRAMID = 1, PRISM = 2, HEX = 3 };

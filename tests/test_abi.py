@@ -74,3 +74,30 @@ def test_unsupported_options_return_a_status(override):
     with pytest.raises(pa.PhlError) as e:
         pa.Solver(m, pa.default_params(**override))
     assert e.value.status == pa.PHL_ERR_UNSUPPORTED
+
+
+def test_concat_keeps_each_case_intact():
+    a = pa.Mesh.bump_hex(6, 3, 2, 0.05)
+    b = pa.Mesh.bump_hex(4, 5, 3, 0.02)
+    u = pa.Mesh.concat([a, b])
+    assert u.n_cases == 2
+    assert (u.n_cell, u.n_face, u.n_bface) == (a.n_cell + b.n_cell, a.n_face + b.n_face, a.n_bface + b.n_bface)
+    cu, mu = u.connectivity(), u.metrics()
+    nb = u.n_bface
+    assert np.array_equal(cu["right"][:nb], u.n_cell + np.arange(nb))
+    for k, part in enumerate((a, b)):
+        cs, gs = u.case_slices(k)
+        mp = part.metrics()
+        assert np.array_equal(mu["vol"][cs], mp["vol"][:part.n_cell])
+        assert np.array_equal(mu["vol"][gs], mp["vol"][part.n_cell:])
+        assert np.array_equal(mu["xcc"][gs], mp["xcc"][part.n_cell:])
+    # faces of case 1 map onto its own faces, in order, with shifted cells
+    cb = b.connectivity()
+    b0 = u.case_bface_ptr[1]
+    assert np.array_equal(cu["left"][b0:b0 + b.n_bface], cb["left"][:b.n_bface] + a.n_cell)
+    assert np.array_equal(cu["bc_type"][b0:b0 + b.n_bface], cb["bc_type"])
+    i0 = nb + (a.n_face - a.n_bface)
+    assert np.array_equal(cu["left"][i0:], cb["left"][b.n_bface:] + a.n_cell)
+    assert np.array_equal(cu["right"][i0:], cb["right"][b.n_bface:] + a.n_cell)
+    with pytest.raises(pa.PhlError):
+        u.renumbered("colour")

@@ -255,3 +255,72 @@ def test_mixed_precision_forces_track_fp64():
     b.set_precision("fp64")
     hb2 = b.iterate(51, 4)
     assert np.all(np.isfinite(hb2))
+
+
+def _batch_cases():
+    meshes = [pa.Mesh.bump_hex(12, 6, 4, 0.05), pa.Mesh.bump_hex(16, 5, 3, 0.03), pa.Mesh.bump_hex(8, 8, 2, 0.07)]
+    params = [pa.default_params(mach=0.5, alpha_deg=0.0, cfl_start=5.0, cfl_end=5.0),
+              pa.default_params(mach=0.35, alpha_deg=3.0, cfl_start=5.0, cfl_end=5.0),
+              pa.default_params(mach=0.75, alpha_deg=-1.5, cfl_start=5.0, cfl_end=5.0)]
+    return meshes, params
+
+
+def test_batch_equals_separate_contexts():
+    """Three cases of different size, Mach, angle and bump height in one
+    batched context: each case's state after 12 iterations equals its own
+    context's bit for bit, residual histories and forces agree to rounding."""
+    meshes, params = _batch_cases()
+    u = pa.Mesh.concat(meshes)
+    batch = pa.Solver(u, params)
+    batch.init_freestream()
+    hb = batch.iterate(1, 12)
+    qb = batch.get_state()
+    for k, (m, p) in enumerate(zip(meshes, params)):
+        s = pa.Solver(m, p)
+        s.init_freestream()
+        hs = s.iterate(1, 12)
+        q = s.get_state()
+        cs, gs = u.case_slices(k)
+        assert np.array_equal(qb[:, cs], q[:, :m.n_cell]), k
+        assert np.array_equal(qb[:, gs], q[:, m.n_cell:]), k
+        assert np.max(np.abs(hb[:, k] - hs) / hs) <= 1e-12, k
+        fb, fs = batch.forces_case(k), s.forces()
+        assert abs(fb["CL"] - fs["CL"]) <= 1e-13 and abs(fb["CD"] - fs["CD"]) <= 1e-13, k
+
+
+def test_batch_solve_freezes_converged_cases():
+    meshes, params = _batch_cases()
+    u = pa.Mesh.concat(meshes)
+    batch = pa.Solver(u, params)
+    batch.init_freestream()
+    steps, hist = batch.solve_batch(320, 0.2)
+    print("steps to 0.2 drop per case", steps)
+    assert np.all(steps > 0)
+    qb = batch.get_state()
+    for k, (m, p) in enumerate(zip(meshes, params)):
+        # frozen at the end of the 32-step chunk in which it converged
+        n_run = int(np.ceil(steps[k] / 32.0)) * 32
+        s = pa.Solver(m, p)
+        s.init_freestream()
+        h = s.iterate(1, n_run)
+        assert np.argmax(h / h[0] <= 0.2) + 1 == steps[k]
+        cs, _ = u.case_slices(k)
+        assert np.array_equal(qb[:, cs], s.get_state()[:, :m.n_cell]), k
+
+
+def test_autotuned_tiles_keep_the_state():
+    """The autotuner's workgroup tiles regroup only the residual partial sums:
+    with the tiles it picked, the state is bit-identical to the default
+    tiles and the history agrees to rounding."""
+    m = pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7)
+    p = pa.default_params(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0)
+    a, b = pa.Solver(m, p), pa.Solver(m, p)
+    a.init_freestream()
+    b.init_freestream()
+    tuned = b.autotune()
+    print("autotune", tuned)
+    assert tuned["flux_tile"] in (64, 128, 256) and tuned["grad_tile"] in (64, 128, 256)
+    b.init_freestream()
+    ha, hb = a.iterate(1, 6), b.iterate(1, 6)
+    assert np.array_equal(a.get_state(), b.get_state())
+    assert np.max(np.abs(ha - hb) / ha) <= 1e-12
