@@ -246,6 +246,32 @@ phl_status phl_autotune(phl_ctx *ctx, int *flux_tile, int *grad_tile, double *ms
  * finish a mixed-form solve in FP64.  The reference has FP64 only. */
 phl_status phl_set_precision(phl_ctx *ctx, int precision);
 
+/* ---------------- learned warm start (no reference counterpart) -------
+ * A mesh message-passing network predicting a case's flow field from cell
+ * features (position and case parameters), trained on converged fields the
+ * solver generates; the prediction is loaded with phl_set_state and the
+ * solver's own residual judges it.  Hidden width 128, FP32, dense layers on
+ * MFMA.  Parameters, flat: W0 [8][128], b0 [128], A [L][128][128],
+ * B [L][128][128], c [L][128], Wo [128][5], bo [5].  Features X [N][8],
+ * targets / outputs [N][5], row-major. */
+typedef struct phl_mpnn phl_mpnn;
+phl_status phl_mpnn_create(int layers, unsigned seed, int device, phl_mpnn **out);
+void phl_mpnn_destroy(phl_mpnn *net);
+phl_status phl_mpnn_n_params(const phl_mpnn *net, int *count);
+phl_status phl_mpnn_set_params(phl_mpnn *net, const float *flat);   /* also resets Adam */
+phl_status phl_mpnn_get_params(phl_mpnn *net, float *flat);
+/* Graph: nbr[n_nodes][width] neighbour lists, -1 padded; or the interior
+ * face neighbours of a mesh's cells. */
+phl_status phl_mpnn_set_graph(phl_mpnn *net, int n_nodes, int width, const int *nbr);
+phl_status phl_mpnn_set_graph_mesh(phl_mpnn *net, const phl_mesh *mesh);
+/* Features (and targets, may be NULL) of the current graph, kept on the device. */
+phl_status phl_mpnn_set_data(phl_mpnn *net, const float *X, const float *T);
+phl_status phl_mpnn_forward(phl_mpnn *net, float *Y);
+/* Loss (mean of squared errors) and its gradient, flat like the parameters. */
+phl_status phl_mpnn_gradient(phl_mpnn *net, double *loss, float *grad);
+/* steps of Adam (lr, beta 0.9 / 0.999, eps 1e-8); loss_hist[steps] may be NULL. */
+phl_status phl_mpnn_train(phl_mpnn *net, int steps, float lr, double *loss_hist);
+
 /* ---------------- measurement (GPU/Manage/src/TimeMeasure.cu:15-67) ----
  * With timing on, phl_iterate records HIP events on the context's stream
  * around each kernel class of the iteration.  Classes, in launch order:

@@ -86,6 +86,7 @@ _lib = None
 _P = ctypes.c_void_p
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
+_F = ctypes.POINTER(ctypes.c_float)
 
 
 def _declare(lib):
@@ -139,6 +140,17 @@ def _declare(lib):
         "phl_get_kernel_timing": (ctypes.c_int, [_P, _D, _I]),
         "phl_hbm_copy": (ctypes.c_int, [ctypes.c_int, ctypes.c_double, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_double)]),
+        "phl_mpnn_create": (ctypes.c_int, [ctypes.c_int, ctypes.c_uint, ctypes.c_int, ctypes.POINTER(_P)]),
+        "phl_mpnn_destroy": (None, [_P]),
+        "phl_mpnn_n_params": (ctypes.c_int, [_P, _I]),
+        "phl_mpnn_set_params": (ctypes.c_int, [_P, _F]),
+        "phl_mpnn_get_params": (ctypes.c_int, [_P, _F]),
+        "phl_mpnn_set_graph": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _I]),
+        "phl_mpnn_set_graph_mesh": (ctypes.c_int, [_P, _P]),
+        "phl_mpnn_set_data": (ctypes.c_int, [_P, _F, _F]),
+        "phl_mpnn_forward": (ctypes.c_int, [_P, _F]),
+        "phl_mpnn_gradient": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), _F]),
+        "phl_mpnn_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, _D]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -447,3 +459,93 @@ class Solver:
 
     def __del__(self):
         self.close()
+
+
+def _fptr(a):
+    return a.ctypes.data_as(_F)
+
+
+class Mpnn:
+    """Warm-start message-passing network on the device (phl_mpnn): hidden
+    width 128, FP32, dense layers on MFMA, Adam training."""
+
+    N_IN, N_OUT, HIDDEN = 8, 5, 128
+
+    def __init__(self, layers=6, seed=0, device=0):
+        self.layers = layers
+        h = _P()
+        _check(load_library().phl_mpnn_create(layers, seed, device, ctypes.byref(h)))
+        self._h = h
+        n = ctypes.c_int()
+        _check(_lib.phl_mpnn_n_params(self._h, ctypes.byref(n)))
+        self.n_params = n.value
+        self.n_nodes = 0
+
+    def layout(self):
+        """(name, shape) of the flat parameter vector, in order."""
+        L, Hd = self.layers, self.HIDDEN
+        return [("W0", (self.N_IN, Hd)), ("b0", (Hd,)), ("A", (L, Hd, Hd)), ("B", (L, Hd, Hd)), ("c", (L, Hd)),
+                ("Wo", (Hd, self.N_OUT)), ("bo", (self.N_OUT,))]
+
+    def get_params(self):
+        flat = np.empty(self.n_params, np.float32)
+        _check(_lib.phl_mpnn_get_params(self._h, _fptr(flat)))
+        return self.unflatten(flat)
+
+    def set_params(self, params):
+        flat = np.concatenate([np.asarray(params[k], np.float32).reshape(-1) for k, _ in self.layout()])
+        assert flat.size == self.n_params
+        _check(_lib.phl_mpnn_set_params(self._h, _fptr(flat)))
+
+    def unflatten(self, flat):
+        out, o = {}, 0
+        for k, shp in self.layout():
+            n = int(np.prod(shp))
+            out[k] = flat[o:o + n].reshape(shp).copy()
+            o += n
+        return out
+
+    def set_graph(self, nbr=None, mesh=None):
+        if mesh is not None:
+            _check(_lib.phl_mpnn_set_graph_mesh(self._h, mesh._h))
+            self.n_nodes = mesh.n_cell
+        else:
+            nbr = np.ascontiguousarray(nbr, np.int32)
+            _check(_lib.phl_mpnn_set_graph(self._h, nbr.shape[0], nbr.shape[1], _iptr(nbr)))
+            self.n_nodes = nbr.shape[0]
+
+    def set_data(self, X, T=None):
+        X = np.ascontiguousarray(X, np.float32).reshape(self.n_nodes, self.N_IN)
+        Tp = None
+        if T is not None:
+            T = np.ascontiguousarray(T, np.float32).reshape(self.n_nodes, self.N_OUT)
+            Tp = _fptr(T)
+        _check(_lib.phl_mpnn_set_data(self._h, _fptr(X), Tp))
+
+    def forward(self):
+        y = np.empty((self.n_nodes, self.N_OUT), np.float32)
+        _check(_lib.phl_mpnn_forward(self._h, _fptr(y)))
+        return y
+
+    def gradient(self):
+        loss = ctypes.c_double()
+        g = np.empty(self.n_params, np.float32)
+        _check(_lib.phl_mpnn_gradient(self._h, ctypes.byref(loss), _fptr(g)))
+        return loss.value, self.unflatten(g)
+
+    def train(self, steps, lr):
+        hist = np.empty(max(steps, 1))
+        _check(_lib.phl_mpnn_train(self._h, steps, lr, _dptr(hist)))
+        return hist[:steps]
+
+    def close(self):
+        if getattr(self, "_h", None):
+            _lib.phl_mpnn_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+

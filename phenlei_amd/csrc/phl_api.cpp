@@ -13,10 +13,6 @@
 #include <string>
 #include <vector>
 
-struct phl_mesh {
-    phl::Mesh m;
-};
-
 struct phl_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
@@ -261,6 +257,8 @@ phl_status copy_field_f32(phl_ctx *c, const float *dsrc, int rows, int rowLen, i
 void swap_state(phl_ctx *c) { std::swap(c->f.q, c->f.qAlt); }
 
 }  // namespace
+
+void phl::set_last_error(const std::string &msg) { g_err = msg; }
 
 extern "C" {
 

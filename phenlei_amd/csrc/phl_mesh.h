@@ -79,3 +79,13 @@ bool renumberCells(const Mesh &in, const std::vector<int> &perm, Mesh &out, std:
 std::vector<int> colourOrder(const Mesh &m, int &nColours);
 
 }  // namespace phl
+
+// The C-ABI's opaque mesh handle.
+struct phl_mesh {
+    phl::Mesh m;
+};
+
+namespace phl {
+// Message returned by phl_last_error (thread-local).
+void set_last_error(const std::string &msg);
+}  // namespace phl

@@ -1,0 +1,377 @@
+// phl_mpnn.hip -- the warm-start message-passing network on gfx950: forward,
+// backward and Adam, hand-written, FP32 with the dense layers on MFMA
+// (v_mfma_f32_32x32x2_f32).  The network (no reference counterpart):
+//
+//     h0 = relu(X W0 + b0)                     X [N, 8] cell features
+//     l = 1..L:  m = mean_{j in N(i)} h_j       (face neighbours)
+//                z = h A_l + m B_l + c_l ;  h = h + relu(z)
+//     y  = h Wo + bo                           y [N, 5]
+//
+// Rows are cells, columns features, all row-major.  The hidden width is 128.
+// Every reduction (weight gradients, bias gradients, loss) is a fixed-order
+// two-stage sum, so training is deterministic.
+#include "phl_mpnn_internal.h"
+
+#include <hip/hip_runtime.h>
+
+namespace phl {
+namespace nn {
+
+namespace {
+
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+
+constexpr int H = kHidden;    // 128
+constexpr int KC = 32;        // K chunk staged in LDS
+constexpr int TR = 128;       // rows per workgroup tile
+constexpr int kThreads = 256; // 4 waves
+constexpr int kRowChunk = 2048;  // rows per partial of a weight gradient
+
+// D(32x32) += A(32x2) * B(2x32).  Lane l holds A[l % 32][l / 32] and
+// B[l / 32][l % 32]; accumulator register r of lane l holds
+// D[(r % 4) + 4 * (l / 32) + 8 * (r / 4)][l % 32].
+__device__ __forceinline__ int acc_row(int r, int lane) { return (r & 3) + 4 * (lane >> 5) + 8 * (r >> 2); }
+
+// C[N, 128] = A1[N, K1] W1[K1, 128] (+ A2[N, 128] W2[128, 128]) + bias,
+// W given row-major [K, 128], or transposed ([128, K], TW) for the backward
+// products dZ W^T.  Epilogue: z = C stored to Z (if non-null); out = z
+// (mode 0), relu(z) (mode 1), Hres + relu(z) (mode 2) or Hres + z (mode 3).
+template <bool TW>
+__device__ __forceinline__ void stage_w(float (*Wl)[H + 4], const float *__restrict__ W, int K, int k0)
+{
+    for (int e = threadIdx.x; e < KC * H; e += kThreads) {
+        const int kk = e / H, c = e % H;
+        const int k = k0 + kk;
+        float v = 0.f;
+        if (k < K) v = TW ? W[(size_t)c * K + k] : W[(size_t)k * H + c];
+        Wl[kk][c] = v;
+    }
+}
+
+__device__ __forceinline__ void stage_a(float (*Al)[KC + 1], const float *__restrict__ A, int N, int K, int row0,
+                                        int k0)
+{
+    for (int e = threadIdx.x; e < TR * KC; e += kThreads) {
+        const int r = e / KC, kk = e % KC;
+        const int row = row0 + r, k = k0 + kk;
+        Al[r][kk] = (row < N && k < K) ? A[(size_t)row * K + k] : 0.f;
+    }
+}
+
+template <bool TW>
+__device__ __forceinline__ void accumulate(f32x16 (&acc)[4], float (*Al)[KC + 1], float (*Wl)[H + 4],
+                                           const float *__restrict__ A, const float *__restrict__ W, int N, int K,
+                                           int row0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (int k0 = 0; k0 < K; k0 += KC) {
+        __syncthreads();
+        stage_a(Al, A, N, K, row0, k0);
+        stage_w<TW>(Wl, W, K, k0);
+        __syncthreads();
+        const int kmax = (K - k0 < KC) ? (K - k0) : KC;
+        for (int kk = 0; kk < kmax; kk += 2) {
+            const float a = Al[wave * 32 + (lane & 31)][kk + (lane >> 5)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float w = Wl[kk + (lane >> 5)][b * 32 + (lane & 31)];
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, w, acc[b], 0, 0, 0);
+            }
+        }
+    }
+}
+
+template <bool TW>
+__global__ void __launch_bounds__(kThreads) k_dense(const float *__restrict__ A1, const float *__restrict__ W1, int K1,
+                                                    const float *__restrict__ A2, const float *__restrict__ W2,
+                                                    const float *__restrict__ bias, int N, int mode,
+                                                    const float *__restrict__ Hres, float *__restrict__ Z,
+                                                    float *__restrict__ out)
+{
+    __shared__ float Al[TR][KC + 1];
+    __shared__ float Wl[KC][H + 4];
+    const int row0 = blockIdx.x * TR;
+    f32x16 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
+    accumulate<TW>(acc, Al, Wl, A1, W1, N, K1, row0);
+    if (A2) accumulate<TW>(acc, Al, Wl, A2, W2, N, H, row0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = row0 + wave * 32 + acc_row(r, lane);
+            const int col = b * 32 + (lane & 31);
+            if (row >= N) continue;
+            const size_t o = (size_t)row * H + col;
+            const float z = acc[b][r] + (bias ? bias[col] : 0.f);
+            if (Z) Z[o] = z;
+            const float rz = z > 0.f ? z : 0.f;
+            out[o] = mode == 0 ? z : (mode == 1 ? rz : (mode == 2 ? Hres[o] + rz : Hres[o] + z));
+        }
+}
+
+// Partial of G[K, 128] = A[rows, K]^T D[rows, 128] over one chunk of
+// kRowChunk rows; K = 128.  Wave w owns G rows 32w..32w+31.
+__global__ void __launch_bounds__(kThreads) k_wgrad_part(const float *__restrict__ A, const float *__restrict__ D,
+                                                         int N, float *__restrict__ part)
+{
+    __shared__ float Al[32][H + 4];
+    __shared__ float Dl[32][H + 4];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = blockIdx.x * kRowChunk;
+    f32x16 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
+    for (int s = 0; s < kRowChunk; s += 32) {
+        __syncthreads();
+        for (int e = threadIdx.x; e < 32 * H; e += kThreads) {
+            const int n = e / H, c = e % H;
+            const int row = r0 + s + n;
+            const bool in = row < N;
+            Al[n][c] = in ? A[(size_t)row * H + c] : 0.f;
+            Dl[n][c] = in ? D[(size_t)row * H + c] : 0.f;
+        }
+        __syncthreads();
+        for (int n = 0; n < 32; n += 2) {
+            const float a = Al[n + (lane >> 5)][wave * 32 + (lane & 31)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const float d = Dl[n + (lane >> 5)][b * 32 + (lane & 31)];
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x2f32(a, d, acc[b], 0, 0, 0);
+            }
+        }
+    }
+    float *p = part + (size_t)blockIdx.x * H * H;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) p[(size_t)(wave * 32 + acc_row(r, lane)) * H + b * 32 + (lane & 31)] = acc[b][r];
+}
+
+// Small products on the vector ALUs: partial of G[K, C] = A[rows, K]^T D[rows, C]
+// for K, C <= 128 (input layer, output layer, bias rows with A = 1).
+__global__ void __launch_bounds__(kThreads) k_small_grad_part(const float *__restrict__ A, int K,
+                                                              const float *__restrict__ D, int C, int N,
+                                                              float *__restrict__ part)
+{
+    const int r0 = blockIdx.x * kRowChunk;
+    const int r1 = (r0 + kRowChunk < N) ? r0 + kRowChunk : N;
+    float *p = part + (size_t)blockIdx.x * K * C;
+    for (int e = threadIdx.x; e < K * C; e += kThreads) {
+        const int k = e / C, c = e % C;
+        float s = 0.f;
+        for (int r = r0; r < r1; ++r) s += (A ? A[(size_t)r * K + k] : 1.f) * D[(size_t)r * C + c];
+        p[e] = s;
+    }
+}
+
+// out = sum of nPart partials of size n, added in partial order.
+__global__ void __launch_bounds__(kThreads) k_sum_parts(const float *__restrict__ part, int nPart, int n,
+                                                        float *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    float s = 0.f;
+    for (int i = 0; i < nPart; ++i) s += part[(size_t)i * n + e];
+    out[e] = s;
+}
+
+// m_i = mean of h_j over the neighbours of i (scale = 1/deg_i), or, for the
+// transpose in the backward pass, out_j += sum over neighbours i of
+// w_i * h_i with w_i = 1/deg_i (TRANSPOSE).
+template <bool TRANSPOSE>
+__global__ void __launch_bounds__(kThreads) k_neighbour(const int *__restrict__ nbr, int W, int N,
+                                                        const float *__restrict__ invDeg,
+                                                        const float *__restrict__ h, float *__restrict__ out)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * H) return;
+    const int i = (int)(e / H), c = (int)(e % H);
+    float s = 0.f;
+    for (int k = 0; k < W; ++k) {
+        const int j = nbr[(size_t)k * N + i];
+        if (j < 0) break;
+        s += TRANSPOSE ? h[(size_t)j * H + c] * invDeg[j] : h[(size_t)j * H + c];
+    }
+    if (TRANSPOSE) out[e] += s;
+    else out[e] = s * invDeg[i];
+}
+
+// dz = dh * (z > 0)
+__global__ void __launch_bounds__(kThreads) k_relu_back(const float *__restrict__ dh, const float *__restrict__ z,
+                                                        size_t n, float *__restrict__ dz)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) dz[e] = z[e] > 0.f ? dh[e] : 0.f;
+}
+
+__global__ void __launch_bounds__(kThreads) k_add(float *__restrict__ a, const float *__restrict__ b, size_t n)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e < n) a[e] += b[e];
+}
+
+// Output layer: y = h Wo + bo, one thread per (row, output).
+__global__ void __launch_bounds__(kThreads) k_out(const float *__restrict__ h, const float *__restrict__ Wo,
+                                                  const float *__restrict__ bo, int N, float *__restrict__ y)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= N * kOut) return;
+    const int i = e / kOut, o = e % kOut;
+    float s = bo[o];
+    for (int k = 0; k < H; ++k) s += h[(size_t)i * H + k] * Wo[k * kOut + o];
+    y[e] = s;
+}
+
+// dh = dy Wo^T, one thread per (row, hidden column).
+__global__ void __launch_bounds__(kThreads) k_out_back(const float *__restrict__ dy, const float *__restrict__ Wo,
+                                                       int N, float *__restrict__ dh)
+{
+    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= (size_t)N * H) return;
+    const int i = (int)(e / H), k = (int)(e % H);
+    float s = 0.f;
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) s += dy[(size_t)i * kOut + o] * Wo[k * kOut + o];
+    dh[e] = s;
+}
+
+// dy = 2 (y - t) / (N * 5) and per-block partial sums of (y - t)^2.
+__global__ void __launch_bounds__(kThreads) k_loss(const float *__restrict__ y, const float *__restrict__ t, int n,
+                                                   float scale, float *__restrict__ dy, double *__restrict__ part)
+{
+    __shared__ double red[kThreads];
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double s = 0.0;
+    if (e < n) {
+        const float d = y[e] - t[e];
+        dy[e] = 2.f * d * scale;
+        s = (double)d * (double)d;
+    }
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = red[0];
+}
+
+__global__ void k_loss_final(const double *__restrict__ part, int nPart, double scale, double *__restrict__ out)
+{
+    __shared__ double red[kThreads];
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nPart; i += blockDim.x) s += part[i];
+    red[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = kThreads / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0] * scale;
+}
+
+// Adam with bias correction (Kingma & Ba 2015, Algorithm 1).
+__global__ void __launch_bounds__(kThreads) k_adam(float *__restrict__ p, const float *__restrict__ g,
+                                                   float *__restrict__ m, float *__restrict__ v, int n, float lr,
+                                                   float b1, float b2, float c1, float c2, float eps)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n) return;
+    const float gi = g[e];
+    const float mi = b1 * m[e] + (1.f - b1) * gi;
+    const float vi = b2 * v[e] + (1.f - b2) * gi * gi;
+    m[e] = mi;
+    v[e] = vi;
+    p[e] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+inline int nb(size_t n, int b) { return (int)((n + b - 1) / b); }
+
+}  // namespace
+
+// ---------------------------------------------------------------- launchers
+void dense(const float *A1, const float *W1, int K1, const float *A2, const float *W2, const float *bias, int N,
+           int mode, const float *Hres, float *Z, float *out, bool transW, hipStream_t s)
+{
+    if (transW)
+        hipLaunchKernelGGL(k_dense<true>, dim3(nb(N, TR)), dim3(kThreads), 0, s, A1, W1, K1, A2, W2, bias, N, mode,
+                           Hres, Z, out);
+    else
+        hipLaunchKernelGGL(k_dense<false>, dim3(nb(N, TR)), dim3(kThreads), 0, s, A1, W1, K1, A2, W2, bias, N, mode,
+                           Hres, Z, out);
+}
+
+int grad_parts(int N) { return nb(N, kRowChunk); }
+
+void wgrad(const float *A, const float *D, int N, float *part, float *out, hipStream_t s)
+{
+    const int np = grad_parts(N);
+    hipLaunchKernelGGL(k_wgrad_part, dim3(np), dim3(kThreads), 0, s, A, D, N, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nb(H * H, kThreads)), dim3(kThreads), 0, s, part, np, H * H, out);
+}
+
+void small_grad(const float *A, int K, const float *D, int C, int N, float *part, float *out, hipStream_t s)
+{
+    const int np = grad_parts(N);
+    hipLaunchKernelGGL(k_small_grad_part, dim3(np), dim3(kThreads), 0, s, A, K, D, C, N, part);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nb(K * C, kThreads)), dim3(kThreads), 0, s, part, np, K * C, out);
+}
+
+void neighbour_mean(const int *nbr, int W, int N, const float *invDeg, const float *h, float *m, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neighbour<false>, dim3(nb((size_t)N * H, kThreads)), dim3(kThreads), 0, s, nbr, W, N,
+                       invDeg, h, m);
+}
+
+void neighbour_mean_transpose_add(const int *nbr, int W, int N, const float *invDeg, const float *dm, float *dh,
+                                  hipStream_t s)
+{
+    hipLaunchKernelGGL(k_neighbour<true>, dim3(nb((size_t)N * H, kThreads)), dim3(kThreads), 0, s, nbr, W, N,
+                       invDeg, dm, dh);
+}
+
+void relu_back(const float *dh, const float *z, size_t n, float *dz, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_relu_back, dim3(nb(n, kThreads)), dim3(kThreads), 0, s, dh, z, n, dz);
+}
+
+void add(float *a, const float *b, size_t n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_add, dim3(nb(n, kThreads)), dim3(kThreads), 0, s, a, b, n);
+}
+
+void out_layer(const float *h, const float *Wo, const float *bo, int N, float *y, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_out, dim3(nb((size_t)N * kOut, kThreads)), dim3(kThreads), 0, s, h, Wo, bo, N, y);
+}
+
+void out_layer_back(const float *dy, const float *Wo, int N, float *dh, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_out_back, dim3(nb((size_t)N * H, kThreads)), dim3(kThreads), 0, s, dy, Wo, N, dh);
+}
+
+int loss_parts(int n) { return nb(n, kThreads); }
+
+void loss(const float *y, const float *t, int n, float *dy, double *part, double *out, hipStream_t s)
+{
+    const int np = loss_parts(n);
+    hipLaunchKernelGGL(k_loss, dim3(np), dim3(kThreads), 0, s, y, t, n, 1.f / (float)n, dy, part);
+    hipLaunchKernelGGL(k_loss_final, dim3(1), dim3(kThreads), 0, s, part, np, 1.0 / (double)n, out);
+}
+
+void adam(float *p, const float *g, float *m, float *v, int n, float lr, int step, hipStream_t s)
+{
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)step), c2 = 1.f - powf(b2, (float)step);
+    hipLaunchKernelGGL(k_adam, dim3(nb(n, kThreads)), dim3(kThreads), 0, s, p, g, m, v, n, lr, b1, b2, c1, c2, 1e-8f);
+}
+
+}  // namespace nn
+}  // namespace phl
