@@ -30,6 +30,11 @@ import phenlei_amd as pa  # noqa: E402
 METRIC = "cell-updates/s and time-to-1e-8 residual drop: mixed+warm-start vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 
+# HBM bytes per k_flux launch measured with rocprofv3 FETCH_SIZE (x2.0, the
+# factor a 1 GiB copy kernel calibrates) + WRITE_SIZE (x1.0), per workload:
+# profiles/k_flux_bump_hybrid_1m.md.  None where not measured.
+MEASURED_FLUX_TRAFFIC = {"c2": 2649.0e6}
+
 # Workloads.  'c1' is the reference's CPU-runnable case (64x32x8 hex bump
 # channel, inviscid, M0.5, CFL 10).
 CONFIGS = {
@@ -175,8 +180,9 @@ def main():
     mixed = params.precision == 1
     fb = flux_algorithmic_bytes(mesh.n_cell, mesh.n_face, params.viscous, mixed)
     achieved = fb / (flux_ms * 1e-3) / 1e9
+    traffic = MEASURED_FLUX_TRAFFIC.get(args.config) if not mixed else None
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_flux", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb)
     per_class = {k: round(v / max(n_timed, 1), 5) for k, v in times.items()}
     print(f"[bench] mean ms per step by kernel class: {per_class}", file=sys.stderr, flush=True)

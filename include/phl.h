@@ -54,7 +54,11 @@ typedef struct phl_params {
     double force_ref_area;  /* forceReferenceArea (1.0) */
     double force_ref_x, force_ref_y, force_ref_z; /* TorqueRefX/Y/Z (0) */
     double force_ref_span;  /* forceReferenceLengthSpanWise (1.0) */
-    int viscous;            /* viscousType: 0 inviscid, 1 laminar */
+    double sa_nu_inf;       /* muoo: freestream nu~ of the SA model (3.0) */
+    double sa_vist_inf;     /* freeStreamViscosity: eddy viscosity of inflow ghosts (1e-3) */
+    double turb_cfl_scale;  /* turbCFLScale (1.0) */
+    double turb_skew_angle; /* turbSkewnessAngle, degrees (0.0) */
+    int viscous;            /* viscousType: 0 inviscid, 1 laminar, 3 Spalart-Allmaras (1eq-sa) */
     int gradient;           /* gradientName: 0 "ggcell" */
     int limiter;            /* uns_limiter_name: 0 "vencat" (ivencat=7), 1 "nolim", 2 "1st" */
     int entropy_fix;        /* roeEntropyFixMethod (3) */

@@ -48,6 +48,9 @@ struct orc_solver {
     double *q[5], *res[5], *dqdx[5], *dqdy[5], *dqdz[5], *dq[5], *dflux[5];
     double *lim, *specInv, *specVis, *dt, *diag, *gama, *t, *visl, *vist;
     double *dtdx, *dtdy, *dtdz;
+    /* Spalart-Allmaras (viscous == 3): nu~ and its residual, spectrum,
+     * face matrices, increments, gradient and the wall distance */
+    double *nut, *resT, *specT, *matL, *matR, *dqT, *dfluxT, *gnx, *gny, *gnz, *wdist;
     /* derived constants */
     double primInf[5];
     double R;           /* coefficientOfStateEquation, PerfectGas.cpp:1401 */
@@ -363,6 +366,9 @@ orc_solver *orc_create(const orc_params *p, int nCell, int nBFace, int nFace,
     s->lim = zeros(nT); s->specInv = zeros(nT); s->specVis = zeros(nT);
     s->dt = zeros(nT); s->diag = zeros(nT); s->gama = zeros(nT); s->t = zeros(nT);
     s->visl = zeros(nT); s->vist = zeros(nT);
+    s->nut = zeros(nT); s->resT = zeros(nT); s->specT = zeros(nT);
+    s->matL = zeros(nFace); s->matR = zeros(nFace); s->dqT = zeros(nT); s->dfluxT = zeros(nT);
+    s->gnx = zeros(nT); s->gny = zeros(nT); s->gnz = zeros(nT); s->wdist = zeros(nT);
     s->dtdx = zeros(nT); s->dtdy = zeros(nT); s->dtdz = zeros(nT);
     s->R = 1.0 / (p->gamma * p->mach * p->mach);
     s->pTotal = orc_ptotal(p->mach, p->gamma);
@@ -386,6 +392,8 @@ void orc_destroy(orc_solver *s)
     }
     free(s->lim); free(s->specInv); free(s->specVis); free(s->dt); free(s->diag);
     free(s->gama); free(s->t); free(s->visl); free(s->vist);
+    free(s->nut); free(s->resT); free(s->specT); free(s->matL); free(s->matR); free(s->dqT); free(s->dfluxT);
+    free(s->gnx); free(s->gny); free(s->gnz); free(s->wdist);
     free(s->dtdx); free(s->dtdy); free(s->dtdz);
     free(s);
 }
@@ -416,6 +424,15 @@ static double *field1(orc_solver *s, int field)
     case ORC_DTDX: return s->dtdx;
     case ORC_DTDY: return s->dtdy;
     case ORC_DTDZ: return s->dtdz;
+    case ORC_VIST: return s->vist;
+    case ORC_NUT: return s->nut;
+    case ORC_RES_T: return s->resT;
+    case ORC_SPEC_T: return s->specT;
+    case ORC_DQ_T: return s->dqT;
+    case ORC_WDIST: return s->wdist;
+    case ORC_DNUDX: return s->gnx;
+    case ORC_DNUDY: return s->gny;
+    case ORC_DNUDZ: return s->gnz;
     default: return NULL;
     }
 }
@@ -450,6 +467,10 @@ void orc_init_freestream(orc_solver *s)
 {
     for (int m = 0; m < 5; ++m)
         for (int c = 0; c < s->nTotal; ++c) s->q[m][c] = s->primInf[m];
+    /* TurbSolverUnstr::InitFlowAsRestart (:910-934): nu~ = freestream value;
+     * the eddy viscosity starts at zero (NSSolverUnstruct.cpp:688-692) */
+    if (s->p.viscous == 3)
+        for (int c = 0; c < s->nTotal; ++c) { s->nut[c] = s->p.sa_nu_inf; s->vist[c] = 0.0; }
 }
 
 /* ------------------------------------------------------------------ */
@@ -562,7 +583,6 @@ void orc_viscosity(orc_solver *s)
     for (int c = 0; c < s->nTotal; ++c) {
         double tt = s->t[c];
         s->visl[c] = tt * sqrt(tt) * (1.0 + ts) / (tt + ts);
-        s->vist[c] = 0.0;
     }
 }
 
@@ -1428,6 +1448,7 @@ int orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist)
             rms_hist[i] = sqrt(sums[0] / s->nCell);
         }
         nNeg += orc_lusgs(s, NULL);
+        if (s->p.viscous == 3) orc_sa_relax(s);
     }
     return nNeg;
 }
@@ -1543,4 +1564,463 @@ void orc_force_coefficients(const double *raw, double alpha_deg, double beta_deg
     out[20] = cd - (cl * cl) / (PI_ * AR);
     /* SIGN(one, cfy), Math_BasisFunction.h:150-155 */
     out[21] = (cfy < 0.0 ? -1.0 : 1.0) * cmz / (ABS(cfy) + SMALL);
+}
+
+/* ================================================================== */
+/* Spalart-Allmaras one-equation model, NSMB formation (viscous == 3)  */
+/* CFD/TurbSolver/src/TurbSolverUnstr.cpp, TurbSolver.cpp,             */
+/* Param_TurbSolver.cpp; non-conservative variable (rhoFlag = 0).      */
+/* ================================================================== */
+
+/* PositiveSA, CFD/TurbSolver/include/TurbSolver.h:103-115 (Allmaras 2012). */
+static double positive_sa(double nue, double visl, double rho)
+{
+    if (nue < 0.0) {
+        double xsi = nue / (visl / rho);
+        double xsi3 = xsi * xsi * xsi;
+        return nue * (16.0 + xsi3) / (16.0 - xsi3);
+    }
+    return nue;
+}
+
+void orc_set_wall_distance(orc_solver *s, const double *wd)
+{
+    for (int c = 0; c < s->nCell; ++c) s->wdist[c] = wd[c];
+}
+
+/* Cell-centre to wall distance: the minimum over solid-surface faces of the
+ * distance to the face, each face split into triangles (face centre, node i,
+ * node i+1), the point-triangle distance by closest-point projection
+ * (Ericson, Real-Time Collision Detection, 5.1.5).  Restates the geometry of
+ * Pre_WalldistCompute::ComputeWallDist3DByProjectingAccurate
+ * (API/PreProcess/src/Pre_WalldistCompute.cpp:613-728, 4355). */
+static double tri_dist2(const double *p, const double *a, const double *b, const double *c)
+{
+    double ab[3], ac[3], ap[3], bp[3], cp[3], q[3];
+    for (int i = 0; i < 3; ++i) { ab[i] = b[i] - a[i]; ac[i] = c[i] - a[i]; ap[i] = p[i] - a[i]; }
+    double d1 = ab[0] * ap[0] + ab[1] * ap[1] + ab[2] * ap[2];
+    double d2 = ac[0] * ap[0] + ac[1] * ap[1] + ac[2] * ap[2];
+    if (d1 <= 0.0 && d2 <= 0.0) { for (int i = 0; i < 3; ++i) q[i] = a[i]; goto done; }
+    for (int i = 0; i < 3; ++i) bp[i] = p[i] - b[i];
+    double d3 = ab[0] * bp[0] + ab[1] * bp[1] + ab[2] * bp[2];
+    double d4 = ac[0] * bp[0] + ac[1] * bp[1] + ac[2] * bp[2];
+    if (d3 >= 0.0 && d4 <= d3) { for (int i = 0; i < 3; ++i) q[i] = b[i]; goto done; }
+    double vc = d1 * d4 - d3 * d2;
+    if (vc <= 0.0 && d1 >= 0.0 && d3 <= 0.0) {
+        double v = d1 / (d1 - d3);
+        for (int i = 0; i < 3; ++i) q[i] = a[i] + v * ab[i];
+        goto done;
+    }
+    for (int i = 0; i < 3; ++i) cp[i] = p[i] - c[i];
+    double d5 = ab[0] * cp[0] + ab[1] * cp[1] + ab[2] * cp[2];
+    double d6 = ac[0] * cp[0] + ac[1] * cp[1] + ac[2] * cp[2];
+    if (d6 >= 0.0 && d5 <= d6) { for (int i = 0; i < 3; ++i) q[i] = c[i]; goto done; }
+    double vb = d5 * d2 - d1 * d6;
+    if (vb <= 0.0 && d2 >= 0.0 && d6 <= 0.0) {
+        double w = d2 / (d2 - d6);
+        for (int i = 0; i < 3; ++i) q[i] = a[i] + w * ac[i];
+        goto done;
+    }
+    double va = d3 * d6 - d5 * d4;
+    if (va <= 0.0 && (d4 - d3) >= 0.0 && (d5 - d6) >= 0.0) {
+        double w = (d4 - d3) / ((d4 - d3) + (d5 - d6));
+        for (int i = 0; i < 3; ++i) q[i] = b[i] + w * (c[i] - b[i]);
+        goto done;
+    }
+    {
+        double den = 1.0 / (va + vb + vc);
+        double v = vb * den, w = vc * den;
+        for (int i = 0; i < 3; ++i) q[i] = a[i] + ab[i] * v + ac[i] * w;
+    }
+done:
+    {
+        double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+        return dx * dx + dy * dy + dz * dz;
+    }
+}
+
+void orc_wall_distance(int nCell, const double *xcc, const double *ycc, const double *zcc, int nWall,
+                       const int *wallFaceNodePtr, const int *wallFaceNodes, const double *wxfc,
+                       const double *wyfc, const double *wzfc, const double *x, const double *y,
+                       const double *z, double *out)
+{
+    for (int c = 0; c < nCell; ++c) {
+        double p[3] = {xcc[c], ycc[c], zcc[c]};
+        double best = LARGE;
+        for (int f = 0; f < nWall; ++f) {
+            double a[3] = {wxfc[f], wyfc[f], wzfc[f]};
+            int b0 = wallFaceNodePtr[f], n = wallFaceNodePtr[f + 1] - b0;
+            for (int i = 0; i < n; ++i) {
+                int n1 = wallFaceNodes[b0 + i], n2 = wallFaceNodes[b0 + (i + 1) % n];
+                double b[3] = {x[n1], y[n1], z[n1]}, cc[3] = {x[n2], y[n2], z[n2]};
+                double d2 = tri_dist2(p, a, b, cc);
+                if (d2 < best) best = d2;
+            }
+        }
+        out[c] = sqrt(best);
+    }
+}
+
+/* TurbSolverUnstr::Boundary (:5919): wall nu~_g = -nu~ (WallBCRegion :6116),
+ * farfield by the sign of the Riemann normal velocity (FarfieldBCRegion
+ * :6546), symmetry / outflow / extrapolation copy (OutflowBCRegion :6021).
+ * The eddy viscosity of the ghost follows the same rule. */
+void orc_sa_boundary(orc_solver *s)
+{
+    const double nuInf = s->p.sa_nu_inf, vistInf = s->p.sa_vist_inf;
+    const double roo = s->primInf[IR], uoo = s->primInf[IU], voo = s->primInf[IV], woo = s->primInf[IW];
+    const double poo = s->primInf[IP], g = s->p.gamma;
+    for (int f = 0; f < s->nBFace; ++f) {
+        int le = s->left[f], re = s->right[f], bc = s->bcType[f];
+        if (is_wall(bc)) {
+            s->vist[re] = -s->vist[le];
+            s->nut[re] = -s->nut[le];
+        } else if (bc == ORC_BC_FARFIELD) {
+            double rin = s->q[IR][le], uin = s->q[IU][le], vin = s->q[IV][le], win = s->q[IW][le];
+            double pin = s->q[IP][le];
+            double vno = s->xfn[f] * uoo + s->yfn[f] * voo + s->zfn[f] * woo - 0.0;
+            double vni = s->xfn[f] * uin + s->yfn[f] * vin + s->zfn[f] * win - 0.0;
+            double vei = sqrt(uin * uin + vin * vin + win * win);
+            double coo = sqrt(ABS(g * poo / roo));
+            double cin = sqrt(ABS(g * pin / rin));
+            int outflow;
+            if (vei > cin) {
+                outflow = vni >= 0.0;
+            } else {
+                double riemp = vni + 2.0 * cin / (g - 1.0);
+                double riemm = vno - 2.0 * coo / (g - 1.0);
+                double vnb = 0.5 * (riemp + riemm);
+                outflow = vnb >= 0.0;
+            }
+            if (outflow) {
+                s->vist[re] = s->vist[le];
+                s->nut[re] = s->nut[le];
+            } else {
+                s->vist[re] = vistInf;
+                s->nut[re] = nuInf;
+            }
+        } else {
+            s->vist[re] = s->vist[le];
+            s->nut[re] = s->nut[le];
+        }
+    }
+}
+
+/* TurbSolverUnstr::Viscosity, "1eq-sa" branch (:6910-6960): real cells. */
+void orc_sa_viscosity(orc_solver *s)
+{
+    const double cv13 = 7.1 * 7.1 * 7.1;
+    for (int c = 0; c < s->nCell; ++c) {
+        double nue = s->nut[c];
+        if (nue > 0.0) {
+            double ld = nue * s->q[IR][c] / (s->visl[c] + SMALL);
+            double ld3 = ld * ld * ld;
+            double fv1 = ld3 / (ld3 + cv13);
+            s->vist[c] = s->q[IR][c] * fv1 * nue;
+        } else {
+            s->vist[c] = 0.0;
+        }
+    }
+}
+
+/* GG-cell gradient of nu~ (TurbSolverUnstr::GetGradientField :8652 with
+ * gradientName "ggcell"). */
+void orc_sa_gradient(orc_solver *s)
+{
+    grad_ggcell(s, s->nut, s->gnx, s->gny, s->gnz);
+}
+
+/* InitSpectrum (:1025-1071, steady): spec = 1/(turbCFLScale dt + SMALL). */
+static void sa_init_spectrum(orc_solver *s)
+{
+    for (int c = 0; c < s->nTotal; ++c) s->specT[c] = 0.0;
+    for (int c = 0; c < s->nCell; ++c) s->specT[c] = 1.0 / (s->p.turb_cfl_scale * s->dt[c] + SMALL);
+}
+
+/* SourceFluxOneEquationOriginal (:2702-2916), NSMB formation (the
+ * diffusion source cb2 |grad nu~|^2 is not added), no transition (ft2 = 0),
+ * length scale = wall distance (RANS). */
+static void sa_source(orc_solver *s)
+{
+    const double karm = 0.41, cb1 = 0.1355, cb2 = 0.622, cv1 = 7.1, cw2 = 0.3, cw3 = 2.0;
+    const double sigma = 2.0 / 3.0;
+    const double kap2 = karm * karm, oKap2 = 1.0 / kap2, osigma = 1.0 / sigma;
+    const double cw1 = cb1 * oKap2 + (1.0 + cb2) / sigma;
+    const double cv13 = cv1 * cv1 * cv1, cw36 = pow(cw3, 6), or6 = 1.0 / 6.0, ct3 = 1.2;
+    const double saC2 = 0.7, saC3 = 0.9, cw1Kap2 = cw1 * kap2;
+    const double oRe = 1.0 / s->p.reynolds;
+    (void)osigma;
+    for (int c = 0; c < s->nCell; ++c) {
+        double nueSA = s->nut[c];
+        double dudy = s->dqdy[IU][c], dudz = s->dqdz[IU][c];
+        double dvdx = s->dqdx[IV][c], dvdz = s->dqdz[IV][c];
+        double dwdx = s->dqdx[IW][c], dwdy = s->dqdy[IW][c];
+        double a1 = dwdy - dvdz, a2 = dudz - dwdx, a3 = dvdx - dudy;
+        double omega = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
+        double volume = s->vol[c];
+        double olam = s->q[IR][c] / (s->visl[c] + SMALL);
+        double distance2 = s->wdist[c] * s->wdist[c];
+        double oDistance2 = 1.0 / distance2;
+        double xsi = nueSA * olam + SMALL;
+        double xsi3 = xsi * xsi * xsi;
+        double fv1 = xsi3 / (xsi3 + cv13);
+        double fv2 = 1.0 - xsi / (1.0 + xsi * fv1);
+        double rs = nueSA * oKap2 * oDistance2 * oRe;
+        double nuetRs = nueSA * rs;
+        double sBar = rs * fv2;
+        double std_;
+        if (sBar >= -saC2 * omega) std_ = omega + sBar;
+        else std_ = omega + omega * (saC2 * saC2 * omega + saC3 * sBar) / ((saC3 - 2 * saC2) * omega - sBar);
+        double ostd = 1.0 / (std_ + SMALL);
+        double r = rs * ostd;
+        r = MIN(r, 10.0);
+        double r5 = pow(r, 5);
+        double gg = r + cw2 * (r5 * r - r);
+        double g6 = pow(gg, 6);
+        double fw = gg * pow((1.0 + cw36) / (g6 + cw36), or6);
+        double ft2 = 0.0;
+        double production = cb1 * (1.0 - ft2) * std_ * nueSA;
+        double wallDest = (cw1Kap2 * fw - cb1 * ft2) * nuetRs;
+        const double destrAlpha = 50.0;
+        if (nueSA < 0.0) {
+            production = cb1 * (1.0 - ct3) * omega * nueSA;
+            wallDest = -destrAlpha * cw1Kap2 * nuetRs;
+        }
+        double source = production - wallDest;
+        s->resT[c] += (source) * volume;
+        double dfv2dk = (3.0 * cv13 * pow(fv1 / (xsi + SMALL), 2) - 1.0) / pow(1.0 + xsi * fv1, 2);
+        double dsdnu = oRe * oKap2 * oDistance2 * (fv2 + xsi * dfv2dk);
+        double dfwdg = fw / (gg + SMALL) * (1.0 - g6 / (g6 + cw36));
+        double dgdr = 1.0 + cw2 * (6.0 * r5 - 1.0);
+        double drdnu = oRe * oKap2 * oDistance2 * ostd * (1.0 - nueSA * ostd * dsdnu);
+        double dfwdnu = dfwdg * dgdr * drdnu;
+        double prod = cb1 * (1.0 - ft2) * std_;
+        double dest = (cw1 * fw - cb1 * ft2 * oKap2) * nueSA * oDistance2 * oRe;
+        double prodp = cb1 * ((1.0 - ft2) * dsdnu);
+        double destp = (cw1 * (fw + dfwdnu * nueSA) - cb1 * ft2 * oKap2) * oDistance2 * oRe;
+        if (nueSA < 0.0) {
+            prod = cb1 * (1.0 - ct3) * omega;
+            dest = -destrAlpha * cw1 * nueSA * oDistance2 * oRe;
+            prodp = cb1 * ((1.0 - ct3) * dsdnu);
+            destp = -destrAlpha * cw1 * oDistance2 * oRe;
+        }
+        double prde = prod - dest;
+        double prdep = prodp - destp;
+        double part1 = -0.5 * (prde - ABS(prde));
+        double part2 = -0.5 * (prdep - ABS(prdep)) * nueSA;
+        s->specT[c] += (part1 + part2) * volume;
+    }
+}
+
+/* InviscidFlux with first-order face values (GetQlQrTurbulence :4978,
+ * GetQlQrNS :5065: boundary faces averaged, symmetry copies the left value)
+ * and NNDFlux (:5721) in the non-conservative form; LoadFlux (:2205). */
+static void sa_inviscid(orc_solver *s)
+{
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double qLv[5], qRv[5];
+        for (int m = 0; m < 5; ++m) { qLv[m] = s->q[m][le]; qRv[m] = s->q[m][re]; }
+        double tl = s->nut[le], tr = s->nut[re];
+        if (f < s->nBFace) {
+            for (int m = 0; m < 5; ++m) {
+                double t = 0.5 * (qRv[m] + qLv[m]);
+                qRv[m] = t; qLv[m] = t;
+            }
+            if (s->bcType[f] == ORC_BC_SYMMETRY) {
+                tr = tl;
+            } else {
+                double t = 0.5 * (tr + tl);
+                tr = t; tl = t;
+            }
+        }
+        double vnL = s->xfn[f] * qLv[IU] + s->yfn[f] * qLv[IV] + s->zfn[f] * qLv[IW] - 0.0;
+        double vnR = s->xfn[f] * qRv[IU] + s->yfn[f] * qRv[IV] + s->zfn[f] * qRv[IW] - 0.0;
+        double vnLL = 0.5 * (vnL + ABS(vnL));
+        double vnRR = 0.5 * (vnR - ABS(vnR));
+        double fl = (vnLL * (1.0 * tl) + vnRR * (1.0 * tr)) * s->area[f];
+        s->resT[le] -= fl;
+        if (f >= s->nBFace) s->resT[re] += fl;
+    }
+}
+
+/* ViscousFlux: GetVisFaceValue mlt (:3893-4000, ONE_EQU) and ComputeVisflux
+ * NSMB branch (:4438-4670) with turbSkewnessAngle. */
+static void sa_viscous(orc_solver *s)
+{
+    const double cb2 = 0.622, sigma = 2.0 / 3.0, osigma = 1.0 / sigma;
+    const double oRe = 1.0 / s->p.reynolds;
+    const double turbSkew = s->p.turb_skew_angle;
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double nueLeft = s->nut[le], visLeft = s->visl[le], rhoLeft = s->q[IR][le];
+        double nueRight = s->nut[re], visRight = s->visl[re], rhoRight = s->q[IR][re];
+        if (f < s->nBFace) {
+            if (nueLeft < 0.0) {
+                nueLeft = positive_sa(nueLeft, visLeft, rhoLeft);
+                if (is_wall(s->bcType[f])) nueRight = -nueLeft;
+            }
+        } else {
+            nueLeft = positive_sa(nueLeft, visLeft, rhoLeft);
+            nueRight = positive_sa(nueRight, visRight, rhoRight);
+        }
+        double mlt = osigma * 0.5 * (visLeft / rhoLeft + visRight / rhoRight + nueLeft + nueRight);
+
+        double nxs = s->xfn[f], nys = s->yfn[f], nzs = s->zfn[f];
+        double dxL = s->xcc[le] - s->xfc[f], dyL = s->ycc[le] - s->yfc[f], dzL = s->zcc[le] - s->zfc[f];
+        double dxR = s->xcc[re] - s->xfc[f], dyR = s->ycc[re] - s->yfc[f], dzR = s->zcc[re] - s->zfc[f];
+        double dL = nxs * dxL + nys * dyL + nzs * dzL;
+        double dR = nxs * dxR + nys * dyR + nzs * dzR;
+        double t = -dL / (sqrt(dxL * dxL + dyL * dyL + dzL * dzL) + SMALL);
+        t = MIN(t, 1.0); t = MAX(t, -1.0);
+        double angle1 = asin(t) * 180.0 / PI_;
+        t = dR / (sqrt(dxR * dxR + dyR * dyR + dzR * dzR) + SMALL);
+        t = MIN(t, 1.0); t = MAX(t, -1.0);
+        double angle2 = asin(t) * 180.0 / PI_;
+        double f1 = s->nut[le], f2 = s->nut[re];
+        double fMid = 0.5 * (f1 + f2);
+        double dfdn = 0.0;
+        if (angle1 > turbSkew && angle2 > turbSkew && ABS(dL) > TINY && ABS(dR) > TINY) {
+            double dfd1 = (f1 - fMid) / dL;
+            double dfd2 = (f2 - fMid) / dR;
+            double dtmp = dL * dL + dR * dR;
+            double weightL = dL * dL / dtmp;
+            double weightR = 1.0 - weightL;
+            dfdn = weightL * dfd1 + weightR * dfd2;
+            if (f < s->nBFace && s->bcType[f] != ORC_BC_SYMMETRY) {
+                double normalDist = (s->xcc[re] - s->xcc[le]) * nxs + (s->ycc[re] - s->ycc[le]) * nys +
+                                    (s->zcc[re] - s->zcc[le]) * nzs;
+                dfdn = (f2 - f1) / normalDist;
+            }
+        }
+        double flux1 = -oRe * (1.0 + cb2) * mlt * dfdn * s->area[f];
+        double flux2 = oRe * cb2 * osigma * dfdn * s->area[f];
+        s->resT[le] -= flux1;
+        s->resT[le] -= flux2 * (s->visl[le] / s->q[IR][le] + nueLeft);
+        if (f >= s->nBFace) {
+            s->resT[re] += flux1;
+            s->resT[re] += flux2 * (s->visl[re] / s->q[IR][re] + nueRight);
+        }
+    }
+}
+
+/* SpectrumRadiusOfOneEquation (:4690-4855), NSMB formation: convective
+ * upwind Jacobians (rhoFlag = 0: density-weighted) and the diffusion
+ * radius, accumulated into the spectrum and the face matrices. */
+void orc_sa_spectrum(orc_solver *s)
+{
+    const double oRe = 1.0 / s->p.reynolds;
+    const double sigma = 2.0 / 3.0, osigma = 1.0 / sigma, cb2 = 0.622;
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double rhoL = s->q[IR][le], uL = s->q[IU][le], vL = s->q[IV][le], wL = s->q[IW][le];
+        double vnL = s->xfn[f] * uL + s->yfn[f] * vL + s->zfn[f] * wL - 0.0;
+        double rcL = 0 * 1.0 + (1 - 0) * rhoL;
+        double invJacobianL = vnL > 0.0 ? rcL * vnL * s->area[f] : 0.0;
+        double rhoR = s->q[IR][re], uR = s->q[IU][re], vR = s->q[IV][re], wR = s->q[IW][re];
+        double vnR = s->xfn[f] * uR + s->yfn[f] * vR + s->zfn[f] * wR - 0.0;
+        double rcR = 0 * 1.0 + (1 - 0) * rhoR;
+        double invJacobianR = vnR < 0.0 ? rcR * vnR * s->area[f] : 0.0;
+        s->specT[le] += invJacobianL;
+        s->specT[re] -= invJacobianR;
+        s->matR[f] = invJacobianR;
+        s->matL[f] = -invJacobianL;
+        double orL = 1.0 / (rhoL + SMALL), orR = 1.0 / (rhoR + SMALL);
+        double nueL = positive_sa(s->nut[le], s->visl[le], rhoL);
+        double nueR = positive_sa(s->nut[re], s->visl[re], rhoR);
+        if (f < s->nBFace && is_wall(s->bcType[f])) nueR = -nueL;
+        double nueffL = s->visl[le] * orL + ABS(nueL);
+        double nueffR = s->visl[re] * orR + ABS(nueR);
+        double nueff = 0.5 * (nueffL + nueffR);
+        double volume = 0.5 * (s->vol[le] + s->vol[re]);
+        double oVolume = 1.0 / volume;
+        double ns2 = s->area[f] * s->area[f];
+        double term = oRe * oVolume * (1.0 + cb2) * (osigma * nueff * ns2);
+        s->specT[le] += term;
+        s->specT[re] += term;
+        s->matL[f] += -term;
+        s->matR[f] += -term;
+        s->matL[f] += oRe * oVolume * ns2 * (cb2 * osigma * nueffL);
+        s->matR[f] += oRe * oVolume * ns2 * (cb2 * osigma * nueffR);
+    }
+}
+
+/* TurbSolver::UpdateResiduals (TurbSolver.cpp:316-347) for the steady
+ * one-equation case: boundary, spectrum initialisation, source, inviscid
+ * and viscous fluxes (mod_turb_res = 0).  LoadResiduals leaves -0.0. */
+void orc_sa_update_residuals(orc_solver *s)
+{
+    for (int c = 0; c < s->nTotal; ++c) s->resT[c] = -0.0;
+    orc_sa_boundary(s);
+    sa_init_spectrum(s);
+    sa_source(s);
+    sa_inviscid(s);
+    sa_viscous(s);
+}
+
+/* Scalar LU-SGS: SolveLUSGSForward (:1400-1519), SetGhostDQLUSGS
+ * (:1639-1731: wall 0; farfield, symmetry, inflow, outflow copy; others 0),
+ * SolveLUSGSBackward (:1521-1637), UpdateFlowField (:1180-1255, RANS with
+ * qTurbLimit = 0: no clipping).  One sweep pair (dq initialised to the
+ * residual, LUplusDQ to zero). */
+void orc_sa_lusgs(orc_solver *s)
+{
+    const int nC = s->nCell;
+    for (int c = 0; c < s->nTotal; ++c) { s->dqT[c] = s->resT[c]; s->dfluxT[c] = 0.0; }
+    for (int c = 0; c < nC; ++c) {
+        double rhs0 = 0.0;
+        double ux = s->dfluxT[c];
+        s->dfluxT[c] = 0.0;
+        double dqc = s->resT[c];
+        for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
+            int f = s->c2f[k], le = s->left[f], re = s->right[f];
+            if (le > c || re > c) continue;
+            double df;
+            if (re == c) { int t = le; le = re; re = t; df = s->matL[f] * s->dqT[re]; }
+            else df = s->matR[f] * s->dqT[re];
+            rhs0 += df;
+        }
+        s->dqT[c] = (dqc - ux - rhs0) / s->specT[c];
+        s->dfluxT[c] += rhs0;
+    }
+    for (int f = 0; f < s->nBFace; ++f) {
+        int bc = s->bcType[f], le = s->left[f], re = s->right[f];
+        if (is_wall(bc)) s->dqT[re] = 0.0;
+        else if (bc == ORC_BC_FARFIELD || bc == ORC_BC_SYMMETRY || bc == ORC_BC_INFLOW || bc == ORC_BC_OUTFLOW)
+            s->dqT[re] = s->dqT[le];
+        else s->dqT[re] = 0.0;
+    }
+    for (int c = nC - 1; c >= 0; --c) {
+        double rhs0 = 0.0;
+        double lx = s->dfluxT[c];
+        s->dfluxT[c] = 0.0;
+        double dqc = s->resT[c];
+        for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
+            int f = s->c2f[k], le = s->left[f], re = s->right[f];
+            if (le < c || re < c) continue;
+            double df;
+            if (re == c) { int t = le; le = re; re = t; df = s->matL[f] * s->dqT[re]; }
+            else df = s->matR[f] * s->dqT[re];
+            rhs0 += df;
+        }
+        s->dqT[c] = (dqc - lx - rhs0) / s->specT[c];
+        s->dfluxT[c] += rhs0;
+    }
+    for (int c = 0; c < nC; ++c) s->nut[c] += s->dqT[c];
+}
+
+/* One relaxation of the turbulence solver after the NS one
+ * (Controller::Relaxation with iSolver = TURB): SetBoundaryValueOfField,
+ * UpdateIndependentField (eddy viscosity, gradient of nu~), the residual,
+ * then LUSGS (Diagonal = SpectrumRadiusOfOneEquation, sweeps, update).
+ * The NS gradients, laminar viscosity and time step are those of the NS
+ * relaxation of the same step; q is the updated NS state. */
+void orc_sa_relax(orc_solver *s)
+{
+    orc_sa_boundary(s);
+    orc_sa_viscosity(s);
+    orc_sa_gradient(s);
+    orc_sa_update_residuals(s);
+    orc_sa_spectrum(s);
+    orc_sa_lusgs(s);
 }

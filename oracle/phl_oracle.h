@@ -41,7 +41,8 @@ typedef struct orc_params {
     double force_ref_area;
     double force_ref_x, force_ref_y, force_ref_z;
     double force_ref_span;
-    int viscous;            /* 0 inviscid, 1 laminar */
+    double sa_nu_inf, sa_vist_inf, turb_cfl_scale, turb_skew_angle;
+    int viscous;            /* 0 inviscid, 1 laminar, 3 Spalart-Allmaras */
     int gradient;           /* 0 ggcell, 1 lsq */
     int limiter;            /* 0 vencat (ivencat=7), 1 nolim, 2 first order */
     int entropy_fix;        /* roeEntropyFixMethod (1..6) */
@@ -59,7 +60,10 @@ enum { ORC_BC_EXTRAPOLATION = 1, ORC_BC_SOLID_SURFACE = 2, ORC_BC_SYMMETRY = 3,
 enum { ORC_Q = 0, ORC_RES = 1, ORC_DQDX = 2, ORC_DQDY = 3, ORC_DQDZ = 4,
        ORC_LIMIT = 5, ORC_DQ = 6, ORC_SPEC_INV = 7, ORC_SPEC_VIS = 8,
        ORC_DT = 9, ORC_DIAG = 10, ORC_T = 11, ORC_VISL = 12, ORC_DTDX = 13,
-       ORC_DTDY = 14, ORC_DTDZ = 15, ORC_DELTA_FLUX = 16 };
+       ORC_DTDY = 14, ORC_DTDZ = 15, ORC_DELTA_FLUX = 16,
+       /* Spalart-Allmaras (viscous == 3) */
+       ORC_VIST = 17, ORC_NUT = 18, ORC_RES_T = 19, ORC_SPEC_T = 20, ORC_DQ_T = 21,
+       ORC_WDIST = 22, ORC_DNUDX = 23, ORC_DNUDY = 24, ORC_DNUDZ = 25 };
 
 typedef struct orc_solver orc_solver;
 
@@ -109,6 +113,19 @@ void orc_time_step(orc_solver *s, int step);
 int  orc_lusgs(orc_solver *s, double *sweep_normals); /* returns #negative cells fixed */
 void orc_residual_norm(const orc_solver *s, double *sum5);
 int  orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist);
+/* ---- Spalart-Allmaras (TurbSolverUnstr) ---- */
+void orc_set_wall_distance(orc_solver *s, const double *wd);
+void orc_wall_distance(int nCell, const double *xcc, const double *ycc, const double *zcc, int nWall,
+                       const int *wallFaceNodePtr, const int *wallFaceNodes, const double *wxfc,
+                       const double *wyfc, const double *wzfc, const double *x, const double *y,
+                       const double *z, double *out);
+void orc_sa_boundary(orc_solver *s);
+void orc_sa_viscosity(orc_solver *s);
+void orc_sa_gradient(orc_solver *s);
+void orc_sa_update_residuals(orc_solver *s);
+void orc_sa_spectrum(orc_solver *s);
+void orc_sa_lusgs(orc_solver *s);
+void orc_sa_relax(orc_solver *s);   /* one turbulence relaxation after the NS one */
 #define ORC_N_FORCE 22
 void orc_forces(orc_solver *s, double *out);        /* AirForceCoefParts + CompForce */
 void orc_force_coefficients(const double *raw, double alpha_deg, double beta_deg, double *out);

@@ -42,7 +42,8 @@ class Params(ctypes.Structure):
         ("lusgs_tol", ctypes.c_double), ("force_ref_length", ctypes.c_double),
         ("force_ref_area", ctypes.c_double), ("force_ref_x", ctypes.c_double),
         ("force_ref_y", ctypes.c_double), ("force_ref_z", ctypes.c_double),
-        ("force_ref_span", ctypes.c_double),
+        ("force_ref_span", ctypes.c_double), ("sa_nu_inf", ctypes.c_double), ("sa_vist_inf", ctypes.c_double),
+        ("turb_cfl_scale", ctypes.c_double), ("turb_skew_angle", ctypes.c_double),
         ("viscous", ctypes.c_int), ("gradient", ctypes.c_int), ("limiter", ctypes.c_int),
         ("entropy_fix", ctypes.c_int), ("cfl_vary_step", ctypes.c_int),
         ("n_lusgs_sweeps", ctypes.c_int), ("precision", ctypes.c_int), ("reserved", ctypes.c_int),
@@ -60,6 +61,7 @@ def default_params(**overrides):
     p.venkat_k, p.entropy_scale = 5.0, 1.0
     p.cfl_start, p.cfl_end, p.ktmax, p.lusgs_tol = 0.01, 10.0, 1.0e10, 0.01
     p.force_ref_length, p.force_ref_area, p.force_ref_span = 1.0, 1.0, 1.0
+    p.sa_nu_inf, p.sa_vist_inf, p.turb_cfl_scale, p.turb_skew_angle = 3.0, 1.0e-3, 1.0, 0.0
     p.viscous, p.gradient, p.limiter, p.entropy_fix = 0, 0, 0, 3
     p.cfl_vary_step, p.n_lusgs_sweeps, p.precision = 500, 1, 0
     for k, v in overrides.items():

@@ -285,6 +285,10 @@ void phl_params_default(phl_params *p)
     p->force_ref_length = 1.0;
     p->force_ref_area = 1.0;
     p->force_ref_span = 1.0;
+    p->sa_nu_inf = 3.0;
+    p->sa_vist_inf = 1.0e-3;
+    p->turb_cfl_scale = 1.0;
+    p->turb_skew_angle = 0.0;
     p->viscous = 0;
     p->gradient = 0;
     p->limiter = 0;
