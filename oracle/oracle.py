@@ -12,9 +12,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
 
 FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
-             diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16)
+             diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16,
+             vist=17, nut=18, res_t=19, spec_t=20, dq_t=21, wdist=22, dnudx=23, dnudy=24, dnudz=25)
 FIELD_ROWS = dict(q=5, res=5, dqdx=5, dqdy=5, dqdz=5, limit=1, dq=5, spec_inv=1, spec_vis=1, dt=1,
-                  diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5)
+                  diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5,
+                  vist=1, nut=1, res_t=1, spec_t=1, dq_t=1, wdist=1, dnudx=1, dnudy=1, dnudz=1)
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -47,7 +49,9 @@ def load():
         lib.orc_freestream.argtypes = [_P, _D]
         for name in ("orc_init_freestream", "orc_boundary", "orc_gamma_t", "orc_viscosity", "orc_gradient",
                      "orc_limiter", "orc_zero_residual", "orc_inviscid_flux", "orc_viscous_flux",
-                     "orc_update_residuals", "orc_spectrum"):
+                     "orc_update_residuals", "orc_spectrum", "orc_sa_boundary", "orc_sa_viscosity",
+                     "orc_sa_gradient", "orc_sa_update_residuals", "orc_sa_spectrum", "orc_sa_lusgs",
+                     "orc_sa_relax"):
             getattr(lib, name).argtypes = [_P]
         lib.orc_time_step.argtypes = [_P, ctypes.c_int]
         lib.orc_lusgs.argtypes = [_P, _D]
@@ -56,6 +60,9 @@ def load():
         lib.orc_iterate.argtypes = [_P, ctypes.c_int, ctypes.c_int, _D]
         lib.orc_iterate.restype = ctypes.c_int
         lib.orc_forces.argtypes = [_P, _D]
+        lib.orc_set_wall_distance.argtypes = [_P, _D]
+        lib.orc_sa_face_matrices.argtypes = [_P, _D, _D]
+        lib.orc_wall_distance.argtypes = [ctypes.c_int, _D, _D, _D, ctypes.c_int, _I, _I] + [_D] * 6 + [_D]
         lib.orc_force_coefficients.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
         lib.orc_roe_flux.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]
@@ -169,6 +176,15 @@ class OracleSolver:
             raise AttributeError(name)
         return lambda *a: fn(self._h, *a)
 
+    def set_wall_distance(self, wd):
+        wd = np.ascontiguousarray(wd, dtype=np.float64)
+        load().orc_set_wall_distance(self._h, _dp(wd))
+
+    def sa_face_matrices(self, n_face):
+        a, b = np.empty(n_face), np.empty(n_face)
+        load().orc_sa_face_matrices(self._h, _dp(a), _dp(b))
+        return a, b
+
     def lusgs(self):
         sweeps = np.empty(8)
         return load().orc_lusgs(self._h, _dp(sweeps))
@@ -196,3 +212,28 @@ class OracleSolver:
 
     def __del__(self):
         self.close()
+
+
+def wall_distance(mesh_conn, metrics, bc_wall=2):
+    """Cell-centre wall distance over the solid-surface faces (orc_wall_distance)."""
+    n_cell = len(mesh_conn["cell_node_ptr"]) - 1
+    nb = mesh_conn["n_bface"]
+    walls = [f for f in range(nb) if mesh_conn["bc_type"][f] == bc_wall]
+    fp = mesh_conn["face_node_ptr"]
+    ptr = [0]
+    nodes = []
+    for f in walls:
+        nodes.extend(mesh_conn["face_nodes"][fp[f]:fp[f + 1]])
+        ptr.append(len(nodes))
+    ptr = np.asarray(ptr, np.int32)
+    nodes = np.asarray(nodes if nodes else [0], np.int32)
+    w = np.asarray(walls if walls else [0], np.int64)
+    wx = np.ascontiguousarray(metrics["xfc"][w])
+    wy = np.ascontiguousarray(metrics["yfc"][w])
+    wz = np.ascontiguousarray(metrics["zfc"][w])
+    out = np.empty(n_cell)
+    c = [np.ascontiguousarray(metrics[k][:n_cell]) for k in ("xcc", "ycc", "zcc")]
+    xyz = [np.ascontiguousarray(mesh_conn[k], dtype=np.float64) for k in ("x", "y", "z")]
+    load().orc_wall_distance(n_cell, _dp(c[0]), _dp(c[1]), _dp(c[2]), len(walls), _ip(ptr), _ip(nodes),
+                             _dp(wx), _dp(wy), _dp(wz), _dp(xyz[0]), _dp(xyz[1]), _dp(xyz[2]), _dp(out))
+    return out

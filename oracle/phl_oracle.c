@@ -2024,3 +2024,8 @@ void orc_sa_relax(orc_solver *s)
     orc_sa_spectrum(s);
     orc_sa_lusgs(s);
 }
+
+void orc_sa_face_matrices(const orc_solver *s, double *matL, double *matR)
+{
+    for (int f = 0; f < s->nFace; ++f) { matL[f] = s->matL[f]; matR[f] = s->matR[f]; }
+}

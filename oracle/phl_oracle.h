@@ -126,6 +126,7 @@ void orc_sa_update_residuals(orc_solver *s);
 void orc_sa_spectrum(orc_solver *s);
 void orc_sa_lusgs(orc_solver *s);
 void orc_sa_relax(orc_solver *s);   /* one turbulence relaxation after the NS one */
+void orc_sa_face_matrices(const orc_solver *s, double *matL, double *matR);
 #define ORC_N_FORCE 22
 void orc_forces(orc_solver *s, double *out);        /* AirForceCoefParts + CompForce */
 void orc_force_coefficients(const double *raw, double alpha_deg, double beta_deg, double *out);
