@@ -74,7 +74,12 @@ enum {
     PHL_FIELD_DQDZ = 4, PHL_FIELD_LIMIT = 5, PHL_FIELD_DQ = 6, PHL_FIELD_SPEC_INV = 7,
     PHL_FIELD_SPEC_VIS = 8, PHL_FIELD_DT = 9, PHL_FIELD_DIAG = 10, PHL_FIELD_T = 11,
     PHL_FIELD_VISL = 12, PHL_FIELD_DTDX = 13, PHL_FIELD_DTDY = 14, PHL_FIELD_DTDZ = 15,
-    PHL_FIELD_DELTA_FLUX = 16
+    PHL_FIELD_DELTA_FLUX = 16,
+    /* Spalart-Allmaras (viscous = 3): eddy viscosity, nu~, its residual,
+     * LU-SGS diagonal and increment, wall distance, gradient of nu~ */
+    PHL_FIELD_VIST = 17, PHL_FIELD_NUT = 18, PHL_FIELD_RES_T = 19, PHL_FIELD_SPEC_T = 20,
+    PHL_FIELD_DQ_T = 21, PHL_FIELD_WDIST = 22, PHL_FIELD_DNUDX = 23, PHL_FIELD_DNUDY = 24,
+    PHL_FIELD_DNUDZ = 25
 };
 
 /* Boundary-condition codes (API/Geometry/include/Geo_SimpleBC.h:32-39). */
@@ -169,6 +174,11 @@ phl_status phl_get_state(phl_ctx *ctx, double *q);
 phl_status phl_get_residual(phl_ctx *ctx, double *res);
 /* Any field of the path, laid out like the oracle's orc_get. */
 phl_status phl_get_field(phl_ctx *ctx, int field, double *out);
+/* Overwrite a field: Q ([5][nTotal], as phl_set_state), DT ([nCell]), VIST
+ * ([nTotal]), NUT ([nTotal]) or WDIST ([nCell], e.g. a wall distance the
+ * preprocessor computed; by default the context computes the exact distance
+ * to the solid-surface faces when it is created). */
+phl_status phl_set_field(phl_ctx *ctx, int field, const double *in);
 /* Controller::InitFlow freestream fill (CFD/Solver/src/Controller.cpp:81-140). */
 phl_status phl_init_freestream(phl_ctx *ctx);
 
@@ -216,6 +226,31 @@ enum {
 phl_status phl_forces(phl_ctx *ctx, double *coef);
 /* The same for case `c` of a batched context (phl_forces is case 0). */
 phl_status phl_forces_case(phl_ctx *ctx, int c, double *coef);
+
+/* ---------------- Spalart-Allmaras (viscous = 3) -----------------------
+ * One-equation model, NSMB formation, non-conservative nu~, first-order
+ * convection (CFD/TurbSolver/src/TurbSolverUnstr.cpp).  phl_iterate runs one
+ * turbulence relaxation after each NS update, as Controller::Relaxation does
+ * for the turbulence solver; the hooks below are its steps. */
+/* TurbSolverUnstr::Boundary (:5919): ghost nu~ and eddy viscosity. */
+phl_status phl_turb_boundary(phl_ctx *ctx);
+/* TurbSolverUnstr::Viscosity (:6910), "1eq-sa": eddy viscosity of real cells. */
+phl_status phl_turb_viscosity(phl_ctx *ctx);
+/* TurbSolverUnstr::GetGradientField (:8652), ggcell gradient of nu~. */
+phl_status phl_turb_gradient(phl_ctx *ctx);
+/* TurbSolver::UpdateResiduals (TurbSolver.cpp:316): boundary, spectrum
+ * initialisation from DT, SourceFluxOneEquationOriginal (:2702), first-order
+ * NNDFlux (:5721) and the NSMB diffusion (ComputeVisflux :4438).  Reads the
+ * NS gradients, the laminar viscosity (phl_gamma_t) and DT. */
+phl_status phl_turb_update_residuals(phl_ctx *ctx);
+/* SpectrumRadiusOfOneEquation (:4690): diagonal and face matrices. */
+phl_status phl_turb_spectrum(phl_ctx *ctx);
+/* SolveLUSGSForward / SetGhostDQLUSGS / SolveLUSGSBackward / UpdateFlowField
+ * (:1400,1639,1521,1180): one sweep pair and the nu~ update. */
+phl_status phl_turb_lusgs(phl_ctx *ctx);
+/* One turbulence relaxation: boundary, viscosity, gradient, residuals,
+ * spectrum, LU-SGS. */
+phl_status phl_turb_relax(phl_ctx *ctx);
 
 /* ---------------- host driver (Controller::Relaxation loop) ----------- */
 /* n_steps outer iterations numbered first_step..; rms_hist[i] (may be NULL)
@@ -281,9 +316,10 @@ phl_status phl_mpnn_train(phl_mpnn *net, int steps, float lr, double *loss_hist)
  * around each kernel class of the iteration.  Classes, in launch order:
  * 0 boundary, 1 gradient+limiter, 2 face flux (+spectral radius, local time
  * step), 3 residual norm + minimum time step, 4 dt/diagonal, 5 forward sweep,
- * 6 ghost dq, 7 backward sweep, 8 update.  ms_total[k] is the summed time of
- * class k over n_steps[0] timed steps. */
-#define PHL_N_KERNEL_CLASSES 9
+ * 6 ghost dq, 7 backward sweep, 8 update, 9 turbulence relaxation (SA; empty
+ * otherwise).  ms_total[k] is the summed time of class k over n_steps[0]
+ * timed steps. */
+#define PHL_N_KERNEL_CLASSES 10
 phl_status phl_set_kernel_timing(phl_ctx *ctx, int enable);
 phl_status phl_get_kernel_timing(phl_ctx *ctx, double *ms_total, int *n_steps);
 /* Achievable HBM bandwidth: a streaming copy of `bytes` bytes, best of

@@ -24,9 +24,12 @@ PHL_ERR_ALLOC = 5
 
 # Field identifiers (include/phl.h), shared with the oracle.
 FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
-             diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16)
+             diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16,
+             vist=17, nut=18, res_t=19, spec_t=20, dq_t=21, wdist=22, dnudx=23, dnudy=24, dnudz=25)
 FIELD_ROWS = dict(q=5, res=5, dqdx=5, dqdy=5, dqdz=5, limit=1, dq=5, spec_inv=1, spec_vis=1, dt=1,
-                  diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5)
+                  diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5,
+                  vist=1, nut=1, res_t=1, spec_t=1, dq_t=1, wdist=1, dnudx=1, dnudy=1, dnudz=1)
+SETTABLE_ROWS_LEN = dict(q="total", dt="cell", vist="total", nut="total", wdist="cell")
 
 BC = dict(extrapolation=1, solid_surface=2, symmetry=3, farfield=4, inflow=5, outflow=6)
 
@@ -115,6 +118,14 @@ def _declare(lib):
         "phl_get_state": (ctypes.c_int, [_P, _D]),
         "phl_get_residual": (ctypes.c_int, [_P, _D]),
         "phl_get_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+        "phl_set_field": (ctypes.c_int, [_P, ctypes.c_int, _D]),
+        "phl_turb_boundary": (ctypes.c_int, [_P]),
+        "phl_turb_viscosity": (ctypes.c_int, [_P]),
+        "phl_turb_gradient": (ctypes.c_int, [_P]),
+        "phl_turb_update_residuals": (ctypes.c_int, [_P]),
+        "phl_turb_spectrum": (ctypes.c_int, [_P]),
+        "phl_turb_lusgs": (ctypes.c_int, [_P]),
+        "phl_turb_relax": (ctypes.c_int, [_P]),
         "phl_init_freestream": (ctypes.c_int, [_P]),
         "phl_boundary": (ctypes.c_int, [_P]),
         "phl_gamma_t": (ctypes.c_int, [_P]),
@@ -300,7 +311,7 @@ class Mesh:
 
 
 KERNEL_CLASSES = ["boundary", "grad_lim", "flux", "norm", "dt_diag", "sweep_fwd", "ghost_dq",
-                  "sweep_bwd", "update"]
+                  "sweep_bwd", "update", "turb"]
 
 
 def hbm_copy_gbps(device=0, nbytes=2 * 1024**3, iters=10):
@@ -356,8 +367,37 @@ class Solver:
         _check(_lib.phl_get_field(self._h, FIELD[name], _dptr(out)))
         return out if rows > 1 else out[0]
 
+    def set_field(self, name, value):
+        """Overwrite q, dt, vist, nut or wdist (phl_set_field)."""
+        n = self.mesh.n_total if SETTABLE_ROWS_LEN[name] == "total" else self.mesh.n_cell
+        v = np.ascontiguousarray(value, dtype=np.float64).reshape(FIELD_ROWS[name], -1)[:, :n]
+        v = np.ascontiguousarray(v)
+        _check(_lib.phl_set_field(self._h, FIELD[name], _dptr(v)))
+
     def init_freestream(self):
         _check(_lib.phl_init_freestream(self._h))
+
+    # Spalart-Allmaras hooks (viscous = 3) -----------------------------------
+    def turb_boundary(self):
+        _check(_lib.phl_turb_boundary(self._h))
+
+    def turb_viscosity(self):
+        _check(_lib.phl_turb_viscosity(self._h))
+
+    def turb_gradient(self):
+        _check(_lib.phl_turb_gradient(self._h))
+
+    def turb_update_residuals(self):
+        _check(_lib.phl_turb_update_residuals(self._h))
+
+    def turb_spectrum(self):
+        _check(_lib.phl_turb_spectrum(self._h))
+
+    def turb_lusgs(self):
+        _check(_lib.phl_turb_lusgs(self._h))
+
+    def turb_relax(self):
+        _check(_lib.phl_turb_relax(self._h))
 
     # the path ------------------------------------------------------------
     def boundary(self):

@@ -205,7 +205,9 @@ double entropy_coef(double mach, double scale)
 phl_status check_params(const phl_params &p)
 {
     if (!(p.mach > 0.0) || !(p.gamma > 1.0)) return fail(PHL_ERR_ARG, "mach and gamma must be positive (gamma > 1)");
-    if (p.viscous != 0 && p.viscous != 1) return fail(PHL_ERR_UNSUPPORTED, "viscous must be 0 or 1");
+    if (p.viscous != 0 && p.viscous != 1 && p.viscous != 3)
+        return fail(PHL_ERR_UNSUPPORTED, "viscous must be 0 (inviscid), 1 (laminar) or 3 (Spalart-Allmaras)");
+    if (p.viscous == 3 && !(p.turb_cfl_scale > 0.0)) return fail(PHL_ERR_ARG, "turb_cfl_scale must be positive");
     if (p.viscous && !(p.reynolds > 0.0)) return fail(PHL_ERR_ARG, "reynolds must be positive for viscous flow");
     if (p.gradient != 0) return fail(PHL_ERR_UNSUPPORTED, "only the ggcell gradient is implemented");
     if (p.limiter < 0 || p.limiter > 2) return fail(PHL_ERR_UNSUPPORTED, "limiter must be 0 (vencat), 1 (nolim) or 2 (1st)");
@@ -604,6 +606,8 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         unsigned char fl = 0;
         if (a1 > 60.0 && a2 > 60.0) fl |= 1;
         if (a1 > 0.0 && a2 > 0.0) fl |= 2;
+        // turbulence diffusion (TurbSolverUnstr.cpp ComputeVisflux, turbSkewnessAngle)
+        if (a1 > p.turb_skew_angle && a2 > p.turb_skew_angle) fl |= 4;
         visFlag[f] = fl;
     }
 
@@ -706,6 +710,9 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.entropyFix = p.entropy_fix;
     k.nSweeps = p.n_lusgs_sweeps;
     k.mixed = p.precision;
+    k.saNuInf = p.sa_nu_inf;
+    k.saVistInf = p.sa_vist_inf;
+    k.turbCflScale = p.turb_cfl_scale;
 
     // flow fields
     phl::DevFlow &fl = c->f;
@@ -744,6 +751,48 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.t, (size_t)nT));
     TRY(dalloc(c, &fl.visl, (size_t)nT));
     TRY(dalloc(c, &fl.forceOut, 15));
+    TRY(dalloc(c, &fl.vist, (size_t)nT));
+    HIPTRY(hipMemset(fl.vist, 0, sizeof(double) * nT));
+    dm.wallDist = nullptr;
+    if (p.viscous == 3) {
+        TRY(dalloc(c, &fl.nut, (size_t)nT));
+        TRY(dalloc(c, &fl.resT, (size_t)nT));
+        TRY(dalloc(c, &fl.specT, (size_t)nC));
+        TRY(dalloc(c, &fl.saMat, (size_t)2 * nF));
+        TRY(dalloc(c, &fl.dqT, (size_t)nT));
+        TRY(dalloc(c, &fl.dfluxT, (size_t)nC));
+        TRY(dalloc(c, &fl.gradNut, (size_t)3 * nC));
+        HIPTRY(hipMemset(fl.nut, 0, sizeof(double) * nT));
+        HIPTRY(hipMemset(fl.resT, 0, sizeof(double) * nT));
+        HIPTRY(hipMemset(fl.specT, 0, sizeof(double) * nC));
+        HIPTRY(hipMemset(fl.saMat, 0, sizeof(double) * 2 * nF));
+        HIPTRY(hipMemset(fl.dqT, 0, sizeof(double) * nT));
+        HIPTRY(hipMemset(fl.dfluxT, 0, sizeof(double) * nC));
+        HIPTRY(hipMemset(fl.gradNut, 0, sizeof(double) * 3 * nC));
+        // wall distance: exact point-to-surface distance over the wall faces,
+        // each split into (centre, node i, node i+1) triangles
+        std::vector<double> tri;
+        for (int i = 0; i < dm.nWall; ++i) {
+            const int f = wall[i];
+            const int b = m.faceNodePtr[f], n = m.faceNodePtr[f + 1] - b;
+            for (int j = 0; j < n; ++j) {
+                const int n1 = m.faceNodes[b + j], n2 = m.faceNodes[b + (j + 1) % n];
+                const double t9[9] = {m.xfc[f], m.yfc[f], m.zfc[f], m.x[n1], m.y[n1], m.z[n1],
+                                      m.x[n2], m.y[n2], m.z[n2]};
+                tri.insert(tri.end(), t9, t9 + 9);
+            }
+        }
+        const int nTri = (int)(tri.size() / 9);
+        if (tri.empty()) tri.assign(9, 0.0);
+        const double *dTri = nullptr;
+        TRY(upload(c, &dTri, tri));
+        double *wd = nullptr;
+        TRY(dalloc(c, &wd, (size_t)nC));
+        phl::launch_wall_distance(dm.xcc, dm.ycc, dm.zcc, nC, dTri, nTri, wd, c->stream);
+        HIPTRY(hipGetLastError());
+        HIPTRY(hipStreamSynchronize(c->stream));
+        dm.wallDist = wd;
+    }
     HIPTRY(hipMemset(fl.q, 0, sizeof(double) * 5 * nT));
     HIPTRY(hipMemset(fl.qAlt, 0, sizeof(double) * 5 * nT));
     HIPTRY(hipMemset(fl.res, 0, sizeof(double) * 5 * nC));
@@ -798,6 +847,8 @@ phl_status phl_create_batch(const phl_mesh *pm, const phl_params *params, int de
     // the cases share every option; only the freestream (Mach, angles) varies
     for (int i = 0; i < n; ++i) {
         TRY(check_params(params[i]));
+        if (params[i].viscous == 3)
+            return fail(PHL_ERR_UNSUPPORTED, "phl_create_batch: the Spalart-Allmaras model runs on single-case contexts");
         phl_params a = params[0], b = params[i];
         a.mach = b.mach = 0.0;
         a.alpha_deg = b.alpha_deg = 0.0;
@@ -883,8 +934,45 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
         return copy_field(c, f.gradT + off, 1, nC, nT, out);
     }
     case PHL_FIELD_DELTA_FLUX: return copy_field(c, f.dflux, 5, nC, nT, out);
-    default: return fail(PHL_ERR_ARG, "phl_get_field: unknown field");
+    case PHL_FIELD_VIST: return copy_field(c, f.vist, 1, nT, nT, out);
+    default: break;
     }
+    if (field >= PHL_FIELD_NUT && field <= PHL_FIELD_DNUDZ) {
+        if (c->k.viscous != 3) return fail(PHL_ERR_ARG, "phl_get_field: turbulence fields need viscous = 3");
+        switch (field) {
+        case PHL_FIELD_NUT: return copy_field(c, f.nut, 1, nT, nT, out);
+        case PHL_FIELD_RES_T: return copy_field(c, f.resT, 1, nC, nT, out);
+        case PHL_FIELD_SPEC_T: return copy_field(c, f.specT, 1, nC, nT, out);
+        case PHL_FIELD_DQ_T: return copy_field(c, f.dqT, 1, nT, nT, out);
+        case PHL_FIELD_WDIST: return copy_field(c, c->m.wallDist, 1, nC, nT, out);
+        default: return copy_field(c, f.gradNut + (size_t)(field - PHL_FIELD_DNUDX) * nC, 1, nC, nT, out);
+        }
+    }
+    return fail(PHL_ERR_ARG, "phl_get_field: unknown field");
+}
+
+phl_status phl_set_field(phl_ctx *c, int field, const double *in)
+{
+    if (!c || !in) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    const int nC = c->m.nCell, nT = c->m.nTotal;
+    double *dst = nullptr;
+    size_t n = 0;
+    switch (field) {
+    case PHL_FIELD_Q: return phl_set_state(c, in);
+    case PHL_FIELD_DT: dst = c->f.dt; n = (size_t)nC; break;
+    case PHL_FIELD_VIST: dst = c->f.vist; n = (size_t)nT; break;
+    case PHL_FIELD_NUT:
+    case PHL_FIELD_WDIST:
+        if (c->k.viscous != 3) return fail(PHL_ERR_ARG, "phl_set_field: turbulence fields need viscous = 3");
+        dst = field == PHL_FIELD_NUT ? c->f.nut : const_cast<double *>(c->m.wallDist);
+        n = field == PHL_FIELD_NUT ? (size_t)nT : (size_t)nC;
+        break;
+    default: return fail(PHL_ERR_ARG, "phl_set_field: field is not settable");
+    }
+    HIPTRY(hipMemcpyAsync(dst, in, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
 }
 
 phl_status phl_init_freestream(phl_ctx *c)
@@ -904,6 +992,12 @@ phl_status phl_init_freestream(phl_ctx *c)
                 std::fill(row + c->caseCellPtr[i], row + c->caseCellPtr[i + 1], v);
                 std::fill(row + nC + c->caseBFacePtr[i], row + nC + c->caseBFacePtr[i + 1], v);
             }
+    }
+    HIPTRY(hipMemsetAsync(c->f.vist, 0, sizeof(double) * nT, c->stream));
+    if (c->k.viscous == 3) {
+        std::vector<double> nut((size_t)nT, c->k.saNuInf);
+        HIPTRY(hipMemcpyAsync(c->f.nut, nut.data(), sizeof(double) * nT, hipMemcpyHostToDevice, c->stream));
+        HIPTRY(hipStreamSynchronize(c->stream));
     }
     return phl_set_state(c, h.data());
 }
@@ -1000,6 +1094,64 @@ phl_status phl_lusgs(phl_ctx *c, int *n_fixed)
     return PHL_OK;
 }
 
+// ----------------------------------------------- Spalart-Allmaras hooks
+static phl_status sa_ctx(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    if (c->k.viscous != 3) return fail(PHL_ERR_ARG, "the turbulence hooks need viscous = 3");
+    HIPTRY(hipSetDevice(c->device));
+    return PHL_OK;
+}
+
+phl_status phl_turb_boundary(phl_ctx *c)
+{
+    TRY(sa_ctx(c));
+    phl::launch_sa_boundary(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_turb_viscosity(phl_ctx *c)
+{
+    TRY(sa_ctx(c));
+    phl::launch_sa_viscosity(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_turb_gradient(phl_ctx *c)
+{
+    TRY(sa_ctx(c));
+    phl::launch_sa_gradient(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_turb_update_residuals(phl_ctx *c)
+{
+    TRY(sa_ctx(c));
+    phl::launch_sa_residual(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_turb_spectrum(phl_ctx *c)
+{
+    TRY(sa_ctx(c));
+    phl::launch_sa_spectrum(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_turb_lusgs(phl_ctx *c)
+{
+    TRY(sa_ctx(c));
+    phl::launch_sa_lusgs(c->m, c->k, c->f, c->fwd, c->bwd, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_turb_relax(phl_ctx *c)
+{
+    TRY(sa_ctx(c));
+    phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, c->stream);
+    return check_launch(c);
+}
+
 phl_status phl_residual_norm(phl_ctx *c, double *sum5)
 {
     if (!c || !sum5) return fail(PHL_ERR_ARG, "null argument");
@@ -1085,7 +1237,10 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
         if (ev) hipEventRecord(ev[e++], s);
     };
     mark();
-    phl::launch_boundary(c->m, c->k, c->f, s); mark();
+    phl::launch_boundary(c->m, c->k, c->f, s);
+    // the turbulence model reads the laminar viscosity of the step's state
+    if (c->k.viscous == 3) phl::launch_gamma_t(c->m, c->k, c->f, s);
+    mark();
     phl::launch_grad_lim(c->m, c->k, c->f, s); mark();
     phl::launch_flux(c->m, c->k, c->f, s, true); mark();
     phl::launch_norm_final(c->m, c->k, c->f, true, s); mark();
@@ -1095,6 +1250,9 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, s); mark();
     phl::launch_update(c->m, c->k, c->f, true, s); mark();
     swap_state(c);
+    // Spalart-Allmaras relaxation on the updated state
+    if (c->k.viscous == 3) phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, s);
+    mark();
 }
 
 // Two consecutive steps captured as one hipGraph.  A step swaps q and qAlt,

@@ -38,6 +38,7 @@ struct DevMesh {
     int nWall;
     const unsigned short *caseOf;  // [nTotal] case of each cell (batched contexts)
     const int *caseCellPtr;        // [nCases+1] cell range of each case
+    const double *wallDist;        // [nCell] distance to the nearest solid surface (SA)
 };
 
 // The scalars that depend on a case's freestream (Mach, angles) and mesh
@@ -64,6 +65,7 @@ struct Consts {
     int mixed;              // 1: FP32 gradients and face fluxes, FP64 accumulation
     int nCases;             // > 1: batched context, per-case scalars in cases[caseOf[cell]]
     int fluxTile, gradTile; // cells per workgroup of the flux and gradient kernels (64, 128, 256)
+    double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     const CaseConsts *cases;
 };
 
@@ -93,6 +95,15 @@ struct DevFlow {
     double *t, *visl;   // [nTotal], only for the standalone gamma/T hook
     double *forceOut;   // [15] wall force and moment sums
     unsigned char *frozen;  // [nCases] batched contexts: 1 once a case has converged
+    double *vist;     // [nTotal] eddy viscosity (0 unless SA)
+    // Spalart-Allmaras (viscous == 3), allocated only then
+    double *nut;      // [nTotal] nu~ (non-conservative)
+    double *resT;     // [nTotal] residual of nu~ (real cells written)
+    double *specT;    // [nCell] diagonal of the scalar LU-SGS
+    double *saMat;    // [2*nFace] face matrices: [2f] used by the left cell, [2f+1] by the right
+    double *dqT;      // [nTotal]
+    double *dfluxT;   // [nCell]
+    double *gradNut;  // [3][nCell]
 };
 
 // LU-SGS level schedule.
@@ -133,5 +144,19 @@ void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, hipStream_t s);
 void launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
 void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *torqueRef, hipStream_t s);
+
+// Spalart-Allmaras (phl_sa.hip).  tri: [nTri][9] wall triangles (face centre
+// and two consecutive face nodes).
+void launch_wall_distance(const double *xcc, const double *ycc, const double *zcc, int nCell, const double *tri,
+                          int nTri, double *wd, hipStream_t s);
+void launch_sa_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_sa_viscosity(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_sa_gradient(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_sa_residual(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_sa_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
+                     hipStream_t s);
+void launch_sa_relax(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
+                     hipStream_t s);
 
 }  // namespace phl

@@ -313,7 +313,7 @@ __device__ __forceinline__ double lusgs_rad(const double *ql, const double *qr, 
 
 
 __device__ __forceinline__ void sweep_coef(const Consts &K, const double *qc, const double *qn, const FaceRec &fr,
-                                           bool isL, double dist, double *co)
+                                           bool isL, double dist, double vistC, double vistN, double *co)
 {
     double nx = fr.nx, ny = fr.ny, nz = fr.nz;
     if (!isL) { nx = -nx; ny = -ny; nz = -nz; }
@@ -342,7 +342,7 @@ __device__ __forceinline__ void sweep_coef(const Consts &K, const double *qc, co
         const double tr = qn[4] / (K.R * qn[0]);
         const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
                                  tr * sqrt(tr) * (1.0 + K.tsuth) / (tr + K.tsuth));
-        const double vt = 0.5 * (0.0 + 0.0);
+        const double vt = 0.5 * (vistC + vistN);
         const double rho = 0.5 * (qc[0] + qn[0]);
         const double vis = vl + vt;
         sr = 2.0 * vis / (rho * dist * K.reynolds + kSMALL);
@@ -383,7 +383,8 @@ __device__ __forceinline__ void sweep_apply(const Consts &K, const double *co, c
 template <typename GT, int BS>
 __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const double *__restrict__ q,
                                                      GT *__restrict__ grad, GT *__restrict__ gradT,
-                                                     double *__restrict__ lim, double *__restrict__ coef)
+                                                     double *__restrict__ lim, double *__restrict__ coef,
+                                                     const double *__restrict__ vist)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
@@ -429,7 +430,7 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
                                                  fr.nz * (M.zcc[nb] - M.zcc[c]))
                                           : 0.0;
             double co[kCoef];
-            sweep_coef(K, qc, qn, fr, isL, dist, co);
+            sweep_coef(K, qc, qn, fr, isL, dist, vist[c], vist[nb], co);
             double2 *dst = reinterpret_cast<double2 *>(coef + ((size_t)f * 2 + (isL ? 0 : 1)) * kCoef);
 #pragma unroll
             for (int j = 0; j < kCoef / 2; ++j) dst[j] = make_double2(co[2 * j], co[2 * j + 1]);
@@ -536,7 +537,8 @@ __device__ __forceinline__ T sutherland(T t, T tsuth)
 
 template <typename T>
 __device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const FaceRec &fr, const CellSide<T> &L,
-                             const CellSide<T> &R, const T *gTL, const T *gTR, bool bnd, bool wall, double *fl)
+                             const CellSide<T> &R, const T *gTL, const T *gTR, bool bnd, bool wall, double vistL,
+                             double vistR, double *fl)
 {
     const T two3rd = T(2.0) / T(3.0);
     const double tLc = L.q[4] / (K.R * L.q[0]);
@@ -553,7 +555,7 @@ __device__ void viscous_face(const DevMesh &M, const Consts &K, int f, const Fac
     if (bnd) { wL = 1.0; wR = 0.0; }
     // GetVisFaceValue (:7126)
     const T mul = T(0.5) * (vlL + vlR);
-    const T mut = T(0.5) * (T(0.0) + T(0.0));
+    const T mut = T(0.5) * (T(vistL) + T(vistR));
     double primF[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) primF[m] = 0.5 * (L.q[m] + R.q[m]);
@@ -700,7 +702,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                                                  double *__restrict__ specInv, double *__restrict__ specVis,
                                                  double *__restrict__ normPart, const double *__restrict__ cflArr,
                                                  const int *__restrict__ stepCtr, double *__restrict__ dtLoc,
-                                                 double *__restrict__ dtMinPart)
+                                                 double *__restrict__ dtMinPart, const double *__restrict__ vist)
 {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int c = lb * blockDim.x + threadIdx.x;
@@ -802,7 +804,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                     if (ri < nC) { gTR[0] = gradT[ri]; gTR[1] = gradT[(size_t)nC + ri]; gTR[2] = gradT[(size_t)2 * nC + ri]; }
                     const bool wall = bnd && M.bcType[f] == kBC_WALL;
                     double fl[5];
-                    viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, fl);
+                    viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[li], vist[ri], fl);
                     if (isL) {
 #pragma unroll
                         for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
@@ -815,7 +817,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                 const double tr = R.q[4] / (K.R * R.q[0]);
                 const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
                                          tr * sqrt(tr) * (1.0 + K.tsuth) / (tr + K.tsuth));
-                const double vt = 0.5 * (0.0 + 0.0);
+                const double vt = 0.5 * (vist[isL ? c : nb] + vist[isL ? nb : c]);
                 const double dx = R.x - L.x, dy = R.y - L.y, dz = R.z - L.z;
                 const double ds = dABS(fr.nx * dx + fr.ny * dy + fr.nz * dz);
                 const double rho = 0.5 * (L.q[0] + R.q[0]);
@@ -1485,10 +1487,10 @@ static void launch_grad_lim_t(const DevMesh &m, const Consts &k, DevFlow &f, hip
 {
     if (fp32)
         hipLaunchKernelGGL((k_grad_lim<float, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.gradF,
-                           f.gradTF, f.lim, f.coef);
+                           f.gradTF, f.lim, f.coef, f.vist);
     else
         hipLaunchKernelGGL((k_grad_lim<double, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.grad,
-                           f.gradT, f.lim, f.coef);
+                           f.gradT, f.lim, f.coef, f.vist);
 }
 
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad)
@@ -1510,11 +1512,11 @@ static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStre
     if (k.mixed)
         hipLaunchKernelGGL((k_flux<false, FUSED_DT, float, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, f.gradF,
                            f.gradTF, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
-                           f.dtMinPart);
+                           f.dtMinPart, f.vist);
     else
         hipLaunchKernelGGL((k_flux<false, FUSED_DT, double, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, f.grad,
                            f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
-                           f.dtMinPart);
+                           f.dtMinPart, f.vist);
 }
 
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt)
@@ -1530,7 +1532,8 @@ void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, b
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
     hipLaunchKernelGGL((k_flux<true, false, double, 256>), dim3(nblk(m.nCell, 256)), dim3(256), 0, s, m, k, f.q, f.grad, f.gradT,
-                       f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart);
+                       f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart,
+                       f.vist);
 }
 
 void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s)

@@ -68,11 +68,19 @@ def test_mesh_create_rejects_bad_connectivity():
 
 
 @pytest.mark.parametrize("override", [dict(gradient=1), dict(n_lusgs_sweeps=2), dict(entropy_fix=2),
-                                      dict(t_wall=300.0), dict(viscous=3), dict(precision=7)])
+                                      dict(t_wall=300.0), dict(viscous=2), dict(precision=7)])
 def test_unsupported_options_return_a_status(override):
     m = pa.Mesh.bump_hex(4, 2, 2, 0.05)
     with pytest.raises(pa.PhlError) as e:
         pa.Solver(m, pa.default_params(**override))
+    assert e.value.status == pa.PHL_ERR_UNSUPPORTED
+
+
+def test_turbulence_batch_is_rejected_before_any_device_call():
+    a, b = pa.Mesh.bump_hex(4, 2, 2, 0.05), pa.Mesh.bump_hex(4, 2, 2, 0.03)
+    u = pa.Mesh.concat([a, b])
+    with pytest.raises(pa.PhlError) as e:
+        pa.Solver(u, [pa.default_params(viscous=3), pa.default_params(viscous=3, mach=0.6)])
     assert e.value.status == pa.PHL_ERR_UNSUPPORTED
 
 
