@@ -52,6 +52,15 @@ CONFIGS = {
                   mesh=("bump_hex", 256, 64, 64, 0.05),
                   params=dict(mach=0.5, cfl_start=10.0, cfl_end=10.0), cpu_steps=8),
 }
+CONFIGS["c3"] = dict(workload="swept wing (root chord 1, taper 0.5, AR 4, quarter-chord sweep 25 deg, symmetric "
+                              "cubic 12% section), hybrid 3,983,616 cells (30 prism layers each side of the wing plane, "
+                              "first height 1e-5, growth 1.2; tetrahedral rows to a farfield at 20 root chords), "
+                              "Spalart-Allmaras RANS, M0.6, AoA 2 deg, Re 3e6 per root chord, Roe + GG-cell + "
+                              "Venkat(K=5), LU-SGS CFL 10",
+                     mesh=("swept_wing", 80, 40, 30, 1e-5, 29),
+                     params=dict(mach=0.6, alpha_deg=2.0, viscous=3, reynolds=3e6, cfl_start=10.0, cfl_end=10.0,
+                                 force_ref_area=1.125, force_ref_length=0.7778, force_ref_span=1.5),
+                     cpu_steps=1)
 DEFAULT_CONFIG = "c2"
 
 # Batched sweep: 64 inviscid bump-channel cases in one context, sizes and
@@ -94,6 +103,8 @@ def build_mesh(spec):
         return pa.Mesh.bump_hex(*spec[1:])
     if kind == "bump_hybrid":
         return pa.Mesh.bump_hybrid(*spec[1:])
+    if kind == "swept_wing":
+        return pa.Mesh.swept_wing(*spec[1:])
     raise ValueError(kind)
 
 
@@ -110,11 +121,13 @@ def flux_algorithmic_bytes(n_cell, n_face, viscous, mixed=False):
     return b
 
 
-def cpu_baseline(mesh, params, steps):
+def cpu_baseline(mesh, params, steps, wall_distance=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # the checker, timed here as the CPU baseline only
     conn, met = mesh.connectivity(), mesh.metrics()
     cpu = orc.OracleSolver(conn, met, params)
+    if wall_distance is not None:
+        cpu.set_wall_distance(wall_distance)  # preprocessing, outside the timed iterations
     cpu.init_freestream()
     t0 = time.perf_counter()
     cpu.iterate(1, steps)
@@ -136,6 +149,8 @@ def main():
     ap.add_argument("--autotune", action="store_true", help="pick the flux/gradient tiles by timing first")
     ap.add_argument("--launch", default="graph", choices=["graph", "eager"],
                     help="replay captured hipGraphs (default) or launch every kernel eagerly")
+    ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
+                    help="override the configuration's precision")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -151,7 +166,11 @@ def main():
     else:
         parts = [build_mesh(cfg["mesh"])]
         mesh = parts[0]
-        params = pa.default_params(**cfg["params"])
+        pover = dict(cfg["params"])
+        if args.precision:
+            pover["precision"] = 1 if args.precision == "mixed" else 0
+        params = pa.default_params(**pover)
+        print(f"[bench] {mesh.n_cell} cells: creating the context", file=sys.stderr, flush=True)
         solver = pa.Solver(mesh, params, device=0)
     solver.set_launch_mode(args.launch)
     solver.init_freestream()
@@ -204,7 +223,8 @@ def main():
     }
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_steps"])
+        wd = solver.get_field("wdist")[:mesh.n_cell] if params.viscous == 3 else None
+        line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_steps"], wd)
     solver.close()
     print(json.dumps(line), flush=True)
 

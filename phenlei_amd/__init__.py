@@ -107,6 +107,8 @@ def _declare(lib):
                                                 ctypes.c_int, ctypes.c_double, ctypes.c_uint, ctypes.c_double,
                                                 ctypes.POINTER(_P)]),
         "phl_mesh_renumber": (ctypes.c_int, [_P, ctypes.c_int, _I, _I, ctypes.POINTER(_P)]),
+        "phl_mesh_swept_wing": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_double,
+                                               ctypes.c_int, ctypes.POINTER(_P)]),
         "phl_mesh_connectivity": (ctypes.c_int, [_P, _D, _D, _D, _I, _I, _I, _I, _I, _I, _I]),
         "phl_mesh_metrics": (ctypes.c_int, [_P] + [_D] * 12),
         "phl_mesh_destroy": (None, [_P]),
@@ -226,6 +228,13 @@ class Mesh:
     def bump_hex(cls, nx, ny, nz, bump=0.05):
         h = _P()
         _check(load_library().phl_mesh_bump_hex(nx, ny, nz, bump, ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def swept_wing(cls, n_chord, n_span, n_prism=30, h1=1e-5, n_tet=29):
+        """Semi-span swept wing (phl_mesh_swept_wing); (80, 40, 30, 1e-5, 29) is 3.98M cells."""
+        h = _P()
+        _check(load_library().phl_mesh_swept_wing(n_chord, n_span, n_prism, h1, n_tet, ctypes.byref(h)))
         return cls(h)
 
     @classmethod

@@ -365,6 +365,20 @@ phl_status phl_mesh_bump_hybrid(int nx, int nz, int n_prism, double h1, int n_te
     return PHL_OK;
 }
 
+phl_status phl_mesh_swept_wing(int n_chord, int n_span, int n_prism, double h1, int n_tet, phl_mesh **out)
+{
+    if (!out) return fail(PHL_ERR_ARG, "null out");
+    phl_mesh *pm = new (std::nothrow) phl_mesh;
+    if (!pm) return fail(PHL_ERR_ALLOC, "out of memory");
+    std::string err;
+    if (!phl::makeSweptWing(n_chord, n_span, n_prism, h1, n_tet, pm->m, err)) {
+        delete pm;
+        return fail(PHL_ERR_ARG, err);
+    }
+    *out = pm;
+    return PHL_OK;
+}
+
 phl_status phl_mesh_renumber(const phl_mesh *in, int method, const int *perm, int *n_colours, phl_mesh **out)
 {
     if (!in || !out) return fail(PHL_ERR_ARG, "null argument");

@@ -535,6 +535,198 @@ bool makeBumpHybrid(int nx, int nz, int nPrism, double h1, int nyTet, double jit
     return buildMesh(em, classifyBump, &g, out, err);
 }
 
+namespace {
+// Planform of the swept wing: root chord cr at z = 0 (a symmetry plane),
+// linear taper to the tip at z = b, quarter-chord line swept by tan25.
+struct WingGeom {
+    double b, cr, taper, tanLE, far;
+    double zc(double z) const { return z < b ? z : b; }
+    double xLE(double z) const { return tanLE * zc(z); }
+    double chord(double z) const { return cr * (1.0 - (1.0 - taper) * zc(z) / b); }
+};
+
+int classifyWing(double xc, double yc, double zc, const void *user)
+{
+    const WingGeom *g = (const WingGeom *)user;
+    const double tol = 1e-9;
+    if (std::fabs(zc) < tol) return 3;  // SYMMETRY: the root plane
+    if (std::fabs(yc) < 0.5 && zc < g->b + tol && xc > g->xLE(zc) - tol && xc < g->xLE(zc) + g->chord(zc) + tol)
+        return 2;  // SOLID_SURFACE
+    return 4;      // FARFIELD
+}
+
+// Ratio r of a geometric sequence of n spacings, the first s1 * r, summing
+// to `total` (bisection; 1 < r < 4).
+double growthFor(double s1, int n, double total)
+{
+    double lo = 1.0 + 1e-12, hi = 4.0;
+    for (int it = 0; it < 200; ++it) {
+        const double r = 0.5 * (lo + hi);
+        double sum = 0.0, sp = s1;
+        for (int m = 0; m < n; ++m) { sp *= r; sum += sp; }
+        if (sum < total) lo = r; else hi = r;
+    }
+    return 0.5 * (lo + hi);
+}
+}  // namespace
+
+bool makeSweptWing(int nChord, int nSpan, int nPrism, double h1, int nTet, Mesh &out, std::string &err)
+{
+    if (nChord < 4 || nSpan < 2 || nPrism < 1 || nTet < 1 || !(h1 > 0.0)) {
+        err = "makeSweptWing: bad arguments";
+        return false;
+    }
+    // root chord 1, taper 0.5, aspect ratio 4 (semi-span (1 + taper) = 1.5),
+    // quarter-chord sweep 25 degrees, farfield 20 root chords away
+    WingGeom g;
+    g.cr = 1.0;
+    g.taper = 0.5;
+    g.b = 0.25 * 4.0 * g.cr * (1.0 + g.taper);
+    const double kPi = 3.141592653589793;
+    g.tanLE = std::tan(25.0 * kPi / 180.0) + 0.25 * g.cr * (1.0 - g.taper) / g.b;
+    g.far = 20.0;
+    const double growth = 1.2, tau = 0.12, tipRound = 0.05 * g.b;
+    const int nUp = std::max(2, (3 * nChord + 5) / 10), nDown = std::max(2, (6 * nChord + 5) / 10);
+    const int nTip = std::max(2, (4 * nSpan + 5) / 10);
+    const int NI = nUp + nChord + nDown, NK = nSpan + nTip, NJ = 2 * (nPrism + nTet), jw = nPrism + nTet;
+
+    // spanwise stations: sine clustering towards the tip, then geometric
+    std::vector<double> zs(NK + 1);
+    for (int k = 0; k <= nSpan; ++k) zs[k] = g.b * std::sin(0.5 * kPi * k / nSpan);
+    {
+        const double s1 = zs[nSpan] - zs[nSpan - 1];
+        const double r = growthFor(s1, nTip, g.far - g.b);
+        double sp = s1, z = g.b;
+        for (int m = 1; m <= nTip; ++m) { sp *= r; z += sp; zs[nSpan + m] = z; }
+        zs[NK] = g.far;
+    }
+    // wall-normal distances: nPrism layers growing by 1.2 from h1, then
+    // nTet rows growing geometrically to the farfield
+    std::vector<double> eta(nPrism + nTet + 1, 0.0);
+    {
+        double sp = h1;
+        for (int l = 1; l <= nPrism; ++l) { eta[l] = eta[l - 1] + sp; sp *= growth; }
+        const double last = eta[nPrism] - eta[nPrism - 1];
+        if (!(eta[nPrism] < 0.1 * g.far)) {
+            err = "makeSweptWing: prism layers too thick";
+            return false;
+        }
+        const double r = growthFor(last, nTet, g.far - eta[nPrism]);
+        sp = last;
+        for (int l = nPrism + 1; l <= nPrism + nTet; ++l) { sp *= r; eta[l] = eta[l - 1] + sp; }
+        eta[nPrism + nTet] = g.far;
+    }
+    // chordwise stations per span station: upstream geometric, cosine on the
+    // chord, downstream geometric; xi[i] is the chord fraction (-1 off the wing)
+    std::vector<double> xs((size_t)(NI + 1) * (NK + 1)), xiOf(NI + 1, -1.0);
+    for (int i = 0; i <= nChord; ++i) xiOf[nUp + i] = 0.5 * (1.0 - std::cos(kPi * i / nChord));
+    for (int k = 0; k <= NK; ++k) {
+        const double c = g.chord(zs[k]), xle = g.xLE(zs[k]);
+        double *row = &xs[(size_t)k * (NI + 1)];
+        for (int i = 0; i <= nChord; ++i) row[nUp + i] = xle + c * xiOf[nUp + i];
+        const double s1 = c * xiOf[nUp + 1];
+        const double r1 = growthFor(s1, nUp - 1, g.far - s1);
+        double sp = s1, d = s1;
+        for (int m = 2; m <= nUp; ++m) { sp *= r1; d += sp; row[nUp - m] = xle - d; }
+        row[nUp - 1] = xle - s1;
+        row[0] = xle - g.far;
+        const double s2 = c * (1.0 - xiOf[nUp + nChord - 1]);
+        const double r2 = growthFor(s2, nDown - 1, g.far - s2);
+        sp = s2; d = s2;
+        row[nUp + nChord + 1] = xle + c + s2;
+        for (int m = 2; m <= nDown; ++m) { sp *= r2; d += sp; row[nUp + nChord + m] = xle + c + d; }
+        row[NI] = xle + c + g.far;
+    }
+    // half-thickness: cubic section t/c = 0.12 (maximum at 1/3 chord,
+    // t = 6.75 * 0.06 * xi (1 - xi)^2), closed parabolically over the last
+    // 5% of the span so that the surface ends in the tip edge
+    auto thickness = [&](int i, int k) {
+        const double xi = xiOf[i], z = zs[k];
+        if (!(xi > 0.0 && xi < 1.0) || !(z < g.b)) return 0.0;
+        double t = g.chord(z) * 6.75 * (0.5 * tau) * xi * (1.0 - xi) * (1.0 - xi);
+        if (z > g.b - tipRound) {
+            const double s = (z - (g.b - tipRound)) / tipRound;
+            t *= 1.0 - s * s;
+        }
+        return t;
+    };
+
+    // nodes: the (i, j, k) lattice; on the wing the j = jw plane has a
+    // second (lower-surface) copy of every node where the thickness is > 0
+    ElementMesh em;
+    const size_t nBase = (size_t)(NI + 1) * (NJ + 1) * (NK + 1);
+    auto base = [&](int i, int j, int k) { return (int)(((size_t)k * (NJ + 1) + j) * (NI + 1) + i); };
+    std::vector<int> lower((size_t)(NI + 1) * (NK + 1));
+    em.x.resize(nBase); em.y.resize(nBase); em.z.resize(nBase);
+    for (int k = 0; k <= NK; ++k)
+        for (int i = 0; i <= NI; ++i) {
+            const double t = thickness(i, k), x = xs[(size_t)k * (NI + 1) + i];
+            for (int j = 0; j <= NJ; ++j) {
+                const int n = base(i, j, k);
+                const double e = eta[j >= jw ? j - jw : jw - j];
+                const double yy = e + t * (1.0 - e / g.far);
+                em.x[n] = x;
+                em.y[n] = j >= jw ? yy : -yy;
+                em.z[n] = zs[k];
+            }
+            int lo = base(i, jw, k);
+            if (t > 0.0) {
+                lo = (int)em.x.size();
+                em.x.push_back(x);
+                em.y.push_back(-t);
+                em.z.push_back(zs[k]);
+            }
+            lower[(size_t)k * (NI + 1) + i] = lo;
+        }
+    if (em.x.size() > (size_t)0x7fffffff) {
+        err = "makeSweptWing: too many nodes";
+        return false;
+    }
+    // cells column by column (k outer, then i), bottom to top; near the
+    // slit plane two prisms per lattice cell, beyond it six tetrahedra along
+    // the (0,0,0)-(1,1,1) diagonal (both split every shared face alike)
+    em.cellNodePtr.assign(1, 0);
+    const size_t nCells = (size_t)NI * NK * 2 * (2 * nPrism + 6 * nTet);
+    em.kind.reserve(nCells);
+    em.cellNodes.reserve(nCells * 5);
+    auto addCell = [&](int kind, std::initializer_list<int> nodes) {
+        em.kind.push_back(kind);
+        for (int n : nodes) em.cellNodes.push_back(n);
+        em.cellNodePtr.push_back((int)em.cellNodes.size());
+    };
+    for (int k = 0; k < NK; ++k)
+        for (int i = 0; i < NI; ++i)
+            for (int j = 0; j < NJ; ++j) {
+                const bool below = j < jw;
+                int v[2][2][2];
+                for (int a = 0; a < 2; ++a)
+                    for (int b2 = 0; b2 < 2; ++b2)
+                        for (int c = 0; c < 2; ++c) {
+                            const int jj = j + b2;
+                            v[a][b2][c] = (jj == jw && below) ? lower[(size_t)(k + c) * (NI + 1) + i + a]
+                                                               : base(i + a, jj, k + c);
+                        }
+                const int layer = below ? jw - 1 - j : j - jw;
+                if (layer < nPrism) {
+                    addCell(PRISM, {v[0][0][0], v[1][0][0], v[1][0][1], v[0][1][0], v[1][1][0], v[1][1][1]});
+                    addCell(PRISM, {v[0][0][0], v[1][0][1], v[0][0][1], v[0][1][0], v[1][1][1], v[0][1][1]});
+                } else {
+                    static const int perms[6][3] = {{0, 2, 1}, {2, 0, 1}, {0, 1, 2}, {1, 0, 2}, {1, 2, 0}, {2, 1, 0}};
+                    for (const auto &pm : perms) {
+                        int p[3] = {0, 0, 0};
+                        int nodes[4];
+                        nodes[0] = v[0][0][0];
+                        for (int s2 = 0; s2 < 3; ++s2) {
+                            p[pm[s2]] = 1;
+                            nodes[s2 + 1] = v[p[0]][p[1]][p[2]];
+                        }
+                        addCell(TET, {nodes[0], nodes[1], nodes[2], nodes[3]});
+                    }
+                }
+            }
+    return buildMesh(em, classifyWing, &g, out, err);
+}
+
 bool renumberCells(const Mesh &in, const std::vector<int> &perm, Mesh &out, std::string &err)
 {
     const int nC = in.nCell;

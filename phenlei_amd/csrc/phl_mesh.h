@@ -69,6 +69,15 @@ bool makeBumpHex(int nx, int ny, int nz, double bump, Mesh &out, std::string &er
 bool makeBumpHybrid(int nx, int nz, int nPrism, double h1, int nyTet, double jitter, unsigned seed,
                     double bump, Mesh &out, std::string &err);
 
+// Semi-span swept wing in a box: root chord 1 on the symmetry plane z = 0,
+// taper 0.5, aspect ratio 4, quarter-chord sweep 25 degrees, symmetric cubic
+// section of 12% thickness; farfield 20 root chords away.  A lattice of
+// nChord cells on the chord (plus 0.3 nChord upstream and 0.6 nChord
+// downstream) by nSpan cells on the span (plus 0.4 nSpan beyond the tip);
+// on each side of the wing plane nPrism prism layers (first height h1,
+// growth 1.2) and nTet rows of tetrahedra.
+bool makeSweptWing(int nChord, int nSpan, int nPrism, double h1, int nTet, Mesh &out, std::string &err);
+
 // Renumber cells: new cell i is old cell perm[i].  Faces are re-derived in
 // the reference's order (boundary first, then interior by owner) and the
 // metrics recomputed.  The LU-SGS sweep order of the reference is the cell

@@ -24,6 +24,8 @@ def rel_err(a, b, floor):
 def mesh_of(kind):
     if kind == "hex":
         return pa.Mesh.bump_hex(24, 12, 4, 0.08)
+    if kind == "wing":
+        return pa.Mesh.swept_wing(10, 5, 4, 1e-3, 3)
     return pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7)
 
 
@@ -37,7 +39,7 @@ def pair(m, **pover):
     return gpu, cpu, wd
 
 
-@pytest.mark.parametrize("kind", ["hex", "hybrid"])
+@pytest.mark.parametrize("kind", ["hex", "hybrid", "wing"])
 def test_wall_distance_matches_oracle(kind):
     m = mesh_of(kind)
     gpu, cpu, wd = pair(m)
@@ -46,7 +48,7 @@ def test_wall_distance_matches_oracle(kind):
     assert np.max(np.abs(g - wd) / wd) <= 1e-14
 
 
-@pytest.mark.parametrize("kind", ["hex", "hybrid"])
+@pytest.mark.parametrize("kind", ["hex", "hybrid", "wing"])
 def test_turbulence_hooks_match_oracle(kind):
     m = mesh_of(kind)
     nc, nt = m.n_cell, m.n_total
@@ -113,7 +115,7 @@ def test_turbulence_hooks_match_oracle(kind):
     assert rel_err(ng, nc_, 1e-300) <= 1e-12
 
 
-@pytest.mark.parametrize("kind", ["hex", "hybrid"])
+@pytest.mark.parametrize("kind", ["hex", "hybrid", "wing"])
 def test_sa_iterations_match_oracle(kind):
     m = mesh_of(kind)
     nc = m.n_cell
