@@ -138,6 +138,83 @@ def cpu_baseline(mesh, params, steps, wall_distance=None):
                        f"({mesh.n_cell} cells, {dt:.1f} s, sequential C oracle)")
 
 
+def time_to_drop(mesh, pover, drop, max_steps, chunk=256, stall=2048):
+    """Wall time and iterations to reach drop * R_1 (R_1: the first density-
+    residual RMS of the FP64 run from freestream) for (a) the plain FP64 form
+    and (b) the mixed form, switched to FP64 on the live context once its
+    residual has not halved for `stall` iterations.  Both start from
+    freestream; the clock covers the iterations and the host's residual checks
+    every `chunk` iterations."""
+    out = {}
+    target = None
+    for form in ("fp64", "mixed_then_fp64"):
+        s = pa.Solver(mesh, pa.default_params(**dict(pover, precision=0 if form == "fp64" else 1)))
+        s.init_freestream()
+        s.synchronize()
+        t0 = time.perf_counter()
+        n, hit, best, since, switched = 0, -1, np.inf, 0, None
+        while n < max_steps and hit < 0:
+            h = s.iterate(1 + n, min(chunk, max_steps - n))
+            if target is None:
+                target = drop * float(h[0])
+            below = np.nonzero(h <= target)[0]
+            if below.size:
+                hit = n + int(below[0]) + 1
+            if form != "fp64" and switched is None:
+                if h.min() < 0.5 * best:
+                    best, since = float(h.min()), 0
+                else:
+                    since += h.size
+                if since >= stall:
+                    s.set_precision("fp64")
+                    switched = n + h.size
+            n += h.size
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        out[form] = dict(iterations=hit if hit > 0 else -n, seconds=round(wall, 3),
+                         reached=hit > 0, switched_to_fp64_at=switched)
+        print(f"[bench] time to {drop:g} drop, {form}: {out[form]}", file=sys.stderr, flush=True)
+        s.close()
+    a, b = out["fp64"], out["mixed_then_fp64"]
+    out["speedup"] = round(a["seconds"] / b["seconds"], 3) if a["reached"] and b["reached"] else None
+    out["drop"] = drop
+    out["target_abs"] = target
+    out["warm_start"] = None  # the warm-start network is trained per family of cases (tests, warmstart.py)
+    return out
+
+
+def batch_time_to_drop(mesh, params_list, drop, max_steps, stall=2048):
+    """Batched form: every case of the batch to drop * its own R_1, (a) FP64
+    throughout and (b) mixed with the switch to FP64 once every unconverged
+    case has stalled (phl_set_precision_switch).  Converged cases freeze."""
+    out = {}
+    for form in ("fp64", "mixed_then_fp64"):
+        plist = []
+        for p in params_list:
+            q = pa.Params.from_buffer_copy(p)
+            q.precision = 0 if form == "fp64" else 1
+            plist.append(q)
+        s = pa.Solver(mesh, plist, device=0)
+        s.autotune()
+        s.init_freestream()
+        if form != "fp64":
+            s.set_precision_switch(stall)
+        s.synchronize()
+        t0 = time.perf_counter()
+        steps, _ = s.solve_batch(max_steps, drop)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        out[form] = dict(seconds=round(wall, 3), cases_reached=int(np.sum(steps > 0)),
+                         max_iterations=int(np.max(np.abs(steps))), median_iterations=int(np.median(np.abs(steps))),
+                         final_precision=s.precision())
+        print(f"[bench] batch time to {drop:g} drop, {form}: {out[form]}", file=sys.stderr, flush=True)
+        s.close()
+    a, b = out["fp64"], out["mixed_then_fp64"]
+    out["speedup"] = round(a["seconds"] / b["seconds"], 3)
+    out["drop"] = drop
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -147,10 +224,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--copy-probe", action="store_true", help="also run the HBM copy probe")
     ap.add_argument("--autotune", action="store_true", help="pick the flux/gradient tiles by timing first")
-    ap.add_argument("--launch", default="graph", choices=["graph", "eager"],
-                    help="replay captured hipGraphs (default) or launch every kernel eagerly")
+    ap.add_argument("--launch", default="graph", choices=["graph", "eager", "eager_overlap"],
+                    help="replay captured hipGraphs (default), launch every kernel eagerly on one stream, or "
+                         "eagerly with the turbulence sweeps on a second stream")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="override the configuration's precision")
+    ap.add_argument("--converge", type=float, default=None, metavar="DROP",
+                    help="also time the FP64 and mixed-then-FP64 forms to this residual drop from freestream")
+    ap.add_argument("--max-steps", type=int, default=20000, help="iteration cap of --converge")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -221,9 +302,19 @@ def main():
         "roofline": roof,
         "kernel_ms_per_step": per_class,
     }
+    if args.converge is not None:
+        solver.close()
+        if cfg.get("batch"):
+            line["time_to_drop"] = batch_time_to_drop(mesh, params_list, args.converge, args.max_steps)
+        else:
+            line["time_to_drop"] = time_to_drop(mesh, pover, args.converge, args.max_steps)
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        wd = solver.get_field("wdist")[:mesh.n_cell] if params.viscous == 3 else None
+        wd = None
+        if params.viscous == 3:
+            if solver._h is None:
+                solver = pa.Solver(mesh, params, device=0)
+            wd = solver.get_field("wdist")[:mesh.n_cell]
         line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_steps"], wd)
     solver.close()
     print(json.dumps(line), flush=True)

@@ -65,7 +65,10 @@ typedef struct phl_params {
     int cfl_vary_step;      /* CFLVaryStep (500) */
     int n_lusgs_sweeps;     /* nLUSGSSweeps (1) */
     int precision;          /* 0 FP64; 1 mixed (FP32 gradients and face fluxes, FP64 accumulation/LU-SGS) */
-    int reserved;
+    int storage_order;      /* device storage of the cells: 0 locality order (cells sorted along a
+                             * Morton curve of their centroids, per case; default), 1 the mesh's order.
+                             * Results do not depend on it: every sweep and accumulation follows the
+                             * mesh's cell and face numbering either way. */
 } phl_params;
 
 /* Field identifiers for phl_get_field (same numbering as the oracle). */
@@ -279,6 +282,9 @@ phl_status phl_solve(phl_ctx *ctx, int max_steps, double drop, int *steps_done, 
  * rms_hist[max_steps * n_cases] as phl_iterate. */
 phl_status phl_solve_batch(phl_ctx *ctx, int max_steps, double drop, int *steps_done, double *rms_hist);
 phl_status phl_synchronize(phl_ctx *ctx);
+/* Number of dependency levels of the forward and backward LU-SGS sweeps
+ * (cells of a level are updated in parallel; the levels run in order). */
+phl_status phl_sweep_levels(const phl_ctx *ctx, int *fwd_levels, int *bwd_levels);
 /* How phl_iterate / phl_solve launch (no reference counterpart: the
  * reference launches every kernel eagerly on the default stream).  mode 1
  * (default) replays two-step hipGraphs; mode 0 launches eagerly.  Results are
@@ -295,6 +301,12 @@ phl_status phl_autotune(phl_ctx *ctx, int *flux_tile, int *grad_tile, double *ms
  * spectral radii, time step, LU-SGS and norms) on a live context, e.g. to
  * finish a mixed-form solve in FP64.  The reference has FP64 only. */
 phl_status phl_set_precision(phl_ctx *ctx, int precision);
+phl_status phl_get_precision(const phl_ctx *ctx, int *precision);
+/* Precision switch of phl_solve / phl_solve_batch (no reference counterpart):
+ * a mixed context whose unconverged cases have all gone stall_steps steps
+ * without halving their residual (the FP32 flux round-off floor, near a
+ * 1e-5 drop) switches to FP64 and continues.  0 (default) never switches. */
+phl_status phl_set_precision_switch(phl_ctx *ctx, int stall_steps);
 
 /* ---------------- learned warm start (no reference counterpart) -------
  * A mesh message-passing network predicting a case's flow field from cell

@@ -49,7 +49,7 @@ typedef struct orc_params {
     int cfl_vary_step;      /* CFLVaryStep */
     int n_lusgs_sweeps;     /* nLUSGSSweeps */
     int precision;          /* 0 fp64, 1 mixed (ignored by the oracle) */
-    int reserved;
+    int storage_order;  /* device storage order of the product; the oracle ignores it */
 } orc_params;
 
 /* Boundary-condition codes (API/Geometry/include/Geo_SimpleBC.h:32-39). */

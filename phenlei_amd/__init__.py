@@ -49,7 +49,7 @@ class Params(ctypes.Structure):
         ("turb_cfl_scale", ctypes.c_double), ("turb_skew_angle", ctypes.c_double),
         ("viscous", ctypes.c_int), ("gradient", ctypes.c_int), ("limiter", ctypes.c_int),
         ("entropy_fix", ctypes.c_int), ("cfl_vary_step", ctypes.c_int),
-        ("n_lusgs_sweeps", ctypes.c_int), ("precision", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("n_lusgs_sweeps", ctypes.c_int), ("precision", ctypes.c_int), ("storage_order", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -149,7 +149,10 @@ def _declare(lib):
         "phl_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
         "phl_synchronize": (ctypes.c_int, [_P]),
         "phl_set_launch_mode": (ctypes.c_int, [_P, ctypes.c_int]),
+        "phl_sweep_levels": (ctypes.c_int, [_P, _I, _I]),
         "phl_set_precision": (ctypes.c_int, [_P, ctypes.c_int]),
+        "phl_get_precision": (ctypes.c_int, [_P, _I]),
+        "phl_set_precision_switch": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_autotune": (ctypes.c_int, [_P, _I, _I, _D]),
         "phl_set_kernel_timing": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_get_kernel_timing": (ctypes.c_int, [_P, _D, _I]),
@@ -441,8 +444,10 @@ class Solver:
         return out
 
     def set_launch_mode(self, mode):
-        """'graph' (default) replays captured two-step hipGraphs; 'eager' launches each kernel."""
-        _check(_lib.phl_set_launch_mode(self._h, {"eager": 0, "graph": 1}[mode]))
+        """'graph' (default) replays captured hipGraphs of 8 steps; 'eager' launches each kernel
+        in order on one stream; 'eager_overlap' launches eagerly with the turbulence sweeps on a
+        second stream (as the graphs do)."""
+        _check(_lib.phl_set_launch_mode(self._h, {"eager": 0, "graph": 1, "eager_overlap": 2}[mode]))
 
     def autotune(self):
         """Pick the flux / gradient workgroup tiles by timing (phl_autotune)."""
@@ -456,6 +461,16 @@ class Solver:
     def set_precision(self, precision):
         """'fp64' or 'mixed' (FP32 gradients and face fluxes, FP64 accumulation)."""
         _check(_lib.phl_set_precision(self._h, {"fp64": 0, "mixed": 1}[precision]))
+
+    def precision(self):
+        """'fp64' or 'mixed' (the current form of the context)."""
+        v = ctypes.c_int()
+        _check(_lib.phl_get_precision(self._h, ctypes.byref(v)))
+        return "mixed" if v.value else "fp64"
+
+    def set_precision_switch(self, stall_steps):
+        """solve / solve_batch switch a mixed context to FP64 after stall_steps without halving."""
+        _check(_lib.phl_set_precision_switch(self._h, stall_steps))
 
     def forces(self):
         """Wall force coefficients (phl_forces) as a dict keyed by FORCE_NAMES."""
@@ -492,6 +507,12 @@ class Solver:
 
     def synchronize(self):
         _check(_lib.phl_synchronize(self._h))
+
+    def sweep_levels(self):
+        """(forward, backward) dependency levels of the LU-SGS sweeps."""
+        a, b = ctypes.c_int(), ctypes.c_int()
+        _check(_lib.phl_sweep_levels(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
 
     def set_kernel_timing(self, enable=True):
         _check(_lib.phl_set_kernel_timing(self._h, 1 if enable else 0))

@@ -33,16 +33,28 @@ struct phl_ctx {
     std::vector<phl_params> caseParams;
     std::vector<phl::CaseConsts> hostCases;
     std::vector<int> caseCellPtr, caseBFacePtr, caseWallPtr;
+    // storage position of each mesh cell and the mesh cell at each position
+    // (real cells only; ghosts keep their index)
+    std::vector<int> slotOf, cellAt;
     unsigned char *hostFrozen = nullptr;  // pinned [nCases]
     // kernel timing (HIP events on `stream`)
     bool timing = false;
     std::vector<hipEvent_t> events;
     double kernelMs[PHL_N_KERNEL_CLASSES] = {0};
     int timedSteps = 0;
-    // two-step hipGraphs, one per assignment of the q / qAlt buffers
-    int launchMode = 1;  // 0 eager, 1 graph
+    // hipGraphs of kGraphSteps steps, one per assignment of the q / qAlt buffers
+    int launchMode = 1;  // 0 eager, 1 graph, 2 eager with the turbulence sweeps on the side stream
     double *qBuf0 = nullptr;
-    hipGraphExec_t stepPair[2] = {nullptr, nullptr};
+    hipGraphExec_t stepGraph[2] = {nullptr, nullptr};
+    // Spalart-Allmaras: the scalar sweeps of a step touch only nu~ and the
+    // turbulence LU-SGS arrays, which the next step's NS kernels do not use,
+    // so they run on `side` beside them; evJoin orders the next relaxation
+    hipStream_t side = nullptr;
+    hipEvent_t evFork = nullptr, evJoin = nullptr;
+    bool turbPending = false;
+    // phl_solve / phl_solve_batch: switch a mixed context to FP64 once no
+    // unconverged case has halved its residual for this many steps (0: never)
+    int precisionSwitchStall = 0;
 };
 
 namespace {
@@ -229,9 +241,18 @@ phl_status check_launch(phl_ctx *c)
     return PHL_OK;
 }
 
+// Real cells of one cell-indexed row from storage order to mesh order (in place).
+void row_to_mesh(const phl_ctx *c, double *row, std::vector<double> &tmp)
+{
+    const int nC = (int)c->cellAt.size();
+    tmp.assign(row, row + nC);
+    for (int i = 0; i < nC; ++i) row[c->cellAt[i]] = tmp[i];
+}
+
 phl_status copy_field(phl_ctx *c, const double *dsrc, int rows, int rowLen, int outRowLen, double *out)
 {
-    // rows of rowLen device values into rows of outRowLen host values (zero padded)
+    // rows of rowLen device values (cell-indexed, storage order) into rows of
+    // outRowLen host values in mesh order (zero padded)
     for (int r = 0; r < rows; ++r) {
         double *o = out + (size_t)r * outRowLen;
         if (rowLen > 0)
@@ -239,6 +260,23 @@ phl_status copy_field(phl_ctx *c, const double *dsrc, int rows, int rowLen, int 
                                   hipMemcpyDeviceToHost, c->stream));
         for (int i = rowLen; i < outRowLen; ++i) o[i] = 0.0;
     }
+    HIPTRY(hipStreamSynchronize(c->stream));
+    std::vector<double> tmp;
+    for (int r = 0; r < rows && rowLen >= c->m.nCell; ++r) row_to_mesh(c, out + (size_t)r * outRowLen, tmp);
+    return PHL_OK;
+}
+
+// Host rows of rowLen cell-indexed values (mesh order) to the device (storage order).
+phl_status upload_cells(phl_ctx *c, double *ddst, int rows, int rowLen, const double *in)
+{
+    const int nC = c->m.nCell;
+    std::vector<double> h(in, in + (size_t)rows * rowLen);
+    for (int r = 0; r < rows; ++r) {
+        double *o = h.data() + (size_t)r * rowLen;
+        const double *src = in + (size_t)r * rowLen;
+        for (int i = 0; i < nC && i < rowLen; ++i) o[c->slotOf[i]] = src[i];
+    }
+    HIPTRY(hipMemcpyAsync(ddst, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     return PHL_OK;
 }
@@ -253,6 +291,8 @@ phl_status copy_field_f32(phl_ctx *c, const float *dsrc, int rows, int rowLen, i
     for (int r = 0; r < rows; ++r)
         for (int i = 0; i < outRowLen; ++i)
             out[(size_t)r * outRowLen + i] = i < rowLen ? (double)tmp[(size_t)r * rowLen + i] : 0.0;
+    std::vector<double> t2;
+    for (int r = 0; r < rows && rowLen >= c->m.nCell; ++r) row_to_mesh(c, out + (size_t)r * outRowLen, t2);
     return PHL_OK;
 }
 
@@ -504,6 +544,7 @@ void phl_destroy(phl_ctx *c)
     if (!c) return;
     hipSetDevice(c->device);
     if (c->stream) hipStreamSynchronize(c->stream);
+    if (c->side) hipStreamSynchronize(c->side);
     for (void *p : c->allocs) hipFree(p);
     if (c->hostNorm) hipHostFree(c->hostNorm);
     if (c->hostFrozen) hipHostFree(c->hostFrozen);
@@ -511,8 +552,14 @@ void phl_destroy(phl_ctx *c)
     if (c->hostCfl) hipHostFree(c->hostCfl);
     if (c->hostZero) hipHostFree(c->hostZero);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
-    for (hipGraphExec_t g : c->stepPair)
+    for (hipGraphExec_t g : c->stepGraph)
         if (g) hipGraphExecDestroy(g);
+    if (c->side) {
+        hipStreamSynchronize(c->side);
+        hipStreamDestroy(c->side);
+    }
+    if (c->evFork) hipEventDestroy(c->evFork);
+    if (c->evJoin) hipEventDestroy(c->evJoin);
     if (c->stream) hipStreamDestroy(c->stream);
     delete c;
 }
@@ -544,12 +591,68 @@ static phl::CaseConsts case_consts(const phl_params &p, double avgVol)
     return cc;
 }
 
+// Storage order of the real cells: within each case, cells sorted by the
+// Morton key of their centroid, each coordinate replaced by its rank among
+// the case's cells (so thin wall layers and stretched farfield cells get as
+// many key bits as any other region); ties keep the mesh order.  Contiguous
+// blocks of this order are compact in space, so a workgroup's neighbour
+// gathers mostly hit data its own XCD's L2 already holds.
+static void locality_order(const phl::Mesh &m, std::vector<int> &cellAt)
+{
+    const int nC = m.nCell;
+    cellAt.resize(nC);
+    std::vector<int> ranges = m.caseCellPtr;
+    if (ranges.empty()) ranges = {0, nC};
+    std::vector<uint64_t> key(nC, 0);
+    std::vector<int> idx;
+    for (size_t r = 0; r + 1 < ranges.size(); ++r) {
+        const int b = ranges[r], e = ranges[r + 1], n = e - b;
+        if (n <= 0) continue;
+        std::vector<uint32_t> q[3];
+        const double *xyz[3] = {m.xcc.data(), m.ycc.data(), m.zcc.data()};
+        for (int a = 0; a < 3; ++a) {
+            idx.resize(n);
+            for (int i = 0; i < n; ++i) idx[i] = b + i;
+            std::stable_sort(idx.begin(), idx.end(), [&](int u, int v) { return xyz[a][u] < xyz[a][v]; });
+            q[a].assign(n, 0);
+            for (int i = 0; i < n; ++i)
+                q[a][idx[i] - b] = (uint32_t)(((uint64_t)i << 21) / (uint64_t)n);  // 21-bit rank
+        }
+        for (int i = 0; i < n; ++i) {
+            uint64_t k = 0;
+            for (int bit = 20; bit >= 0; --bit)
+                for (int a = 0; a < 3; ++a) k = (k << 1) | ((q[a][i] >> bit) & 1u);
+            key[b + i] = k;
+        }
+        for (int i = 0; i < n; ++i) cellAt[b + i] = b + i;
+        std::stable_sort(cellAt.begin() + b, cellAt.begin() + e, [&](int u, int v) { return key[u] < key[v]; });
+    }
+}
+
 static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
 {
     const phl_params &p = c->p;
     const int nC = m.nCell, nB = m.nBFace, nF = m.nFace, nT = m.nTotal();
     HIPTRY(hipSetDevice(c->device));
     HIPTRY(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+
+    // storage order of the real cells; S() maps a mesh index (real or ghost)
+    // to its storage index
+    if (p.storage_order == 0) {
+        locality_order(m, c->cellAt);
+    } else {
+        c->cellAt.resize(nC);
+        for (int i = 0; i < nC; ++i) c->cellAt[i] = i;
+    }
+    c->slotOf.assign(nC, 0);
+    for (int i = 0; i < nC; ++i) c->slotOf[c->cellAt[i]] = i;
+    const std::vector<int> &slotOf = c->slotOf, &cellAt = c->cellAt;
+    auto S = [&](int i) { return i < nC ? slotOf[i] : i; };
+    auto permuteCells = [&](const std::vector<double> &v) {  // [nTotal] or [nCell], mesh -> storage order
+        std::vector<double> o(v);
+        for (int i = 0; i < nC; ++i) o[slotOf[i]] = v[i];
+        return o;
+    };
 
     // cell -> face (increasing face order) and cell -> neighbour
     std::vector<int> c2fPtr(nC + 1, 0);
@@ -573,12 +676,14 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     int W = 0;
     for (int i = 0; i < nC; ++i) W = std::max(W, c2fPtr[i + 1] - c2fPtr[i]);
     std::vector<int> ellEnc((size_t)W * nC, 0), ellNb((size_t)W * nC, 0);
-    for (int i = 0; i < nC; ++i)
+    for (int i = 0; i < nC; ++i) {
+        const int st = slotOf[i];
         for (int k = c2fPtr[i]; k < c2fPtr[i + 1]; ++k) {
             const int s = k - c2fPtr[i];
-            ellEnc[(size_t)s * nC + i] = isLeft[k] ? (c2f[k] + 1) : -(c2f[k] + 1);
-            ellNb[(size_t)s * nC + i] = c2n[k];
+            ellEnc[(size_t)s * nC + st] = isLeft[k] ? (c2f[k] + 1) : -(c2f[k] + 1);
+            ellNb[(size_t)s * nC + st] = S(c2n[k]);
         }
+    }
 
     std::vector<phl::FaceRec> faces(nF);
     for (int f = 0; f < nF; ++f)
@@ -587,9 +692,9 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     // Venkatakrishnan scale and CFL skewness factor (geometry only).
     std::vector<double> vf(nC), cflSkew(nC);
     for (int i = 0; i < nC; ++i) {
-        vf[i] = std::pow(std::pow(m.vol[i], 1.0 / 3), 3.0);
+        vf[slotOf[i]] = std::pow(std::pow(m.vol[i], 1.0 / 3), 3.0);
         const double sk = m.skew[i];
-        cflSkew[i] = (sk < 0.1) ? 0.1 : ((sk < 10.0) ? (1.0 - std::exp(0.0 - sk)) : 1.0);
+        cflSkew[slotOf[i]] = (sk < 0.1) ? 0.1 : ((sk < 10.0) ? (1.0 - std::exp(0.0 - sk)) : 1.0);
     }
     std::vector<unsigned char> limZero(nC, 0);
     std::vector<double> limInit(nT, 0.0);
@@ -598,7 +703,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         const int bc = m.bcType[f];
         const bool keep = bc == PHL_BC_SYMMETRY || bc == PHL_BC_SOLID_SURFACE || bc == PHL_BC_OUTFLOW;
         if (!keep && p.limiter == 0) {
-            limZero[m.left[f]] = 1;
+            limZero[slotOf[m.left[f]]] = 1;
             limInit[nC + f] = 0.0;
         }
     }
@@ -629,13 +734,19 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     dm.nCell = nC; dm.nBFace = nB; dm.nFace = nF; dm.nTotal = nT; dm.ellW = W;
     TRY(upload(c, &dm.face, faces));
     TRY(upload(c, &dm.bcType, m.bcType));
-    TRY(upload(c, &dm.fle, m.left));
+    {
+        std::vector<int> fle(nF);
+        for (int f = 0; f < nF; ++f) fle[f] = S(m.left[f]);
+        TRY(upload(c, &dm.fle, fle));
+    }
     TRY(upload(c, &dm.c2fEnc, ellEnc));
     TRY(upload(c, &dm.c2n, ellNb));
-    TRY(upload(c, &dm.xcc, m.xcc));
-    TRY(upload(c, &dm.ycc, m.ycc));
-    TRY(upload(c, &dm.zcc, m.zcc));
-    TRY(upload(c, &dm.vol, m.vol));
+    TRY(upload(c, &dm.xcc, permuteCells(m.xcc)));
+    TRY(upload(c, &dm.ycc, permuteCells(m.ycc)));
+    TRY(upload(c, &dm.zcc, permuteCells(m.zcc)));
+    TRY(upload(c, &dm.vol, permuteCells(m.vol)));
+    TRY(upload(c, &dm.logicalOf, cellAt));
+    TRY(upload(c, &dm.cellAt, slotOf));
     TRY(upload(c, &dm.venkatVf, vf));
     TRY(upload(c, &dm.cflSkew, cflSkew));
     TRY(upload(c, &dm.limZero, limZero));
@@ -657,11 +768,13 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     std::vector<int> cellsF, cellsB;
     build_levels(m, c2fPtr, c2n, true, c->hfwdPtr, cellsF);
     build_levels(m, c2fPtr, c2n, false, c->hbwdPtr, cellsB);
+    std::vector<int> cellsFs(nC), cellsBs(nC);
+    for (int i = 0; i < nC; ++i) { cellsFs[i] = slotOf[cellsF[i]]; cellsBs[i] = slotOf[cellsB[i]]; }
     const int *tmp;
     TRY(upload(c, &tmp, c->hfwdPtr)); c->fwd.ptr = const_cast<int *>(tmp);
-    TRY(upload(c, &tmp, cellsF)); c->fwd.cells = const_cast<int *>(tmp);
+    TRY(upload(c, &tmp, cellsFs)); c->fwd.cells = const_cast<int *>(tmp);
     TRY(upload(c, &tmp, c->hbwdPtr)); c->bwd.ptr = const_cast<int *>(tmp);
-    TRY(upload(c, &tmp, cellsB)); c->bwd.cells = const_cast<int *>(tmp);
+    TRY(upload(c, &tmp, cellsBs)); c->bwd.cells = const_cast<int *>(tmp);
     c->fwd.nLevels = (int)c->hfwdPtr.size() - 1; c->fwd.hptr = c->hfwdPtr.data();
     c->bwd.nLevels = (int)c->hbwdPtr.size() - 1; c->bwd.hptr = c->hbwdPtr.data();
     for (int dir = 0; dir < 2; ++dir) {
@@ -669,6 +782,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         std::vector<int> row, nbs;
         build_sweep_order(nC, c2fPtr, c2f, c2n, isLeft, dir == 0 ? cellsF : cellsB, dir == 0, lv.so.W, lv.so.Wpad,
                           row, nbs);
+        for (int &v : nbs) v = S(v);  // unused slots hold 0, a valid storage index
         lv.so.cells = lv.cells;
         TRY(upload(c, &lv.so.row, row));
         TRY(upload(c, &lv.so.nb, nbs));
@@ -711,6 +825,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         k.cases = dc;
         std::vector<unsigned short> caseOf(nT);
         for (int i = 0; i < nCases; ++i) {
+            // the storage order keeps every case inside its own range
             for (int cc = m.caseCellPtr[i]; cc < m.caseCellPtr[i + 1]; ++cc) caseOf[cc] = (unsigned short)i;
             for (int f = m.caseBFacePtr[i]; f < m.caseBFacePtr[i + 1]; ++f) caseOf[nC + f] = (unsigned short)i;
         }
@@ -769,6 +884,9 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     HIPTRY(hipMemset(fl.vist, 0, sizeof(double) * nT));
     dm.wallDist = nullptr;
     if (p.viscous == 3) {
+        HIPTRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
+        HIPTRY(hipEventCreateWithFlags(&c->evFork, hipEventDisableTiming));
+        HIPTRY(hipEventCreateWithFlags(&c->evJoin, hipEventDisableTiming));
         TRY(dalloc(c, &fl.nut, (size_t)nT));
         TRY(dalloc(c, &fl.resT, (size_t)nT));
         TRY(dalloc(c, &fl.specT, (size_t)nC));
@@ -819,7 +937,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     HIPTRY(hipMemset(fl.negCount, 0, sizeof(int)));
     HIPTRY(hipMemset(fl.negTotal, 0, sizeof(int)));
     HIPTRY(hipMemset(fl.stepCtr, 0, sizeof(int)));
-    HIPTRY(hipMemcpy(fl.lim, limInit.data(), sizeof(double) * nT, hipMemcpyHostToDevice));
+    HIPTRY(hipMemcpy(fl.lim, limInit.data(), sizeof(double) * nT, hipMemcpyHostToDevice));  // ghosts only differ
     HIPTRY(hipHostMalloc((void **)&c->hostNorm, sizeof(double) * 5 * c->normSlots * nCases, hipHostMallocDefault));
     HIPTRY(hipHostMalloc((void **)&c->hostFrozen, (size_t)nCases, hipHostMallocDefault));
     std::fill(c->hostFrozen, c->hostFrozen + nCases, (unsigned char)0);
@@ -894,9 +1012,7 @@ phl_status phl_set_state(phl_ctx *c, const double *q)
 {
     if (!c || !q) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
-    HIPTRY(hipMemcpyAsync(c->f.q, q, sizeof(double) * 5 * c->m.nTotal, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipStreamSynchronize(c->stream));
-    return PHL_OK;
+    return upload_cells(c, c->f.q, 5, c->m.nTotal, q);
 }
 
 phl_status phl_get_state(phl_ctx *c, double *q)
@@ -984,9 +1100,7 @@ phl_status phl_set_field(phl_ctx *c, int field, const double *in)
         break;
     default: return fail(PHL_ERR_ARG, "phl_set_field: field is not settable");
     }
-    HIPTRY(hipMemcpyAsync(dst, in, sizeof(double) * n, hipMemcpyHostToDevice, c->stream));
-    HIPTRY(hipStreamSynchronize(c->stream));
-    return PHL_OK;
+    return upload_cells(c, dst, 1, (int)n, in);
 }
 
 phl_status phl_init_freestream(phl_ctx *c)
@@ -1239,10 +1353,20 @@ phl_status phl_forces(phl_ctx *c, double *coef) { return phl_forces_case(c, 0, c
 // ----------------------------------------------------------- host driver
 static const int kTimedChunk = 64;
 
+// Make the solver stream wait for turbulence sweeps still running on the
+// side stream.
+static void join_turb(phl_ctx *c)
+{
+    if (c->turbPending) {
+        hipStreamWaitEvent(c->stream, c->evJoin, 0);
+        c->turbPending = false;
+    }
+}
+
 // One outer iteration: LoadResiduals+UpdateResiduals, SpectrumRadius,
-// TimeStep, LUSGS (Controller::Relaxation, Controller.cpp:567-608).  With
-// `ev` non-null an event is recorded before the first and after every
-// kernel class.
+// TimeStep, LUSGS (Controller::Relaxation, Controller.cpp:567-608), then the
+// turbulence relaxation.  With `ev` non-null an event is recorded before the
+// first and after every kernel class (everything on the solver stream).
 static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
 {
     hipStream_t s = c->stream;
@@ -1265,31 +1389,55 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     phl::launch_update(c->m, c->k, c->f, true, s); mark();
     swap_state(c);
     // Spalart-Allmaras relaxation on the updated state
-    if (c->k.viscous == 3) phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, s);
+    if (c->k.viscous == 3) {
+        if (!ev && c->launchMode != 0) {
+            join_turb(c);  // the previous step's sweeps wrote nu~
+            phl::launch_sa_pre(c->m, c->k, c->f, s);
+            hipEventRecord(c->evFork, s);
+            hipStreamWaitEvent(c->side, c->evFork, 0);
+            phl::launch_sa_lusgs(c->m, c->k, c->f, c->fwd, c->bwd, c->side);
+            hipEventRecord(c->evJoin, c->side);
+            c->turbPending = true;
+        } else {
+            phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, s);
+        }
+    }
     mark();
 }
 
-// Two consecutive steps captured as one hipGraph.  A step swaps q and qAlt,
-// so after two steps the buffers are back where they started and the graph
-// can be replayed; one graph is kept for each assignment of the buffers.
-// The captured sequence holds kernels only.
-static phl_status step_pair_graph(phl_ctx *c, hipGraphExec_t *out)
+// kGraphSteps consecutive steps captured as one hipGraph.  A step swaps q
+// and qAlt, so after an even number of steps the buffers are back where they
+// started and the graph can be replayed; one graph is kept for each
+// assignment of the buffers.  The captured sequence holds kernels and the
+// event edges between the solver and side streams; it starts and ends with
+// the side stream joined.
+static const int kGraphSteps = 8;
+static phl_status step_graph(phl_ctx *c, hipGraphExec_t *out)
 {
     const int parity = c->f.q == c->qBuf0 ? 0 : 1;
-    if (!c->stepPair[parity]) {
+    if (!c->stepGraph[parity]) {
         hipGraph_t g = nullptr;
         HIPTRY(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        enqueue_step(c, nullptr);
-        enqueue_step(c, nullptr);
+        for (int i = 0; i < kGraphSteps; ++i) enqueue_step(c, nullptr);
+        join_turb(c);
         HIPTRY(hipStreamEndCapture(c->stream, &g));
-        const hipError_t e = hipGraphInstantiate(&c->stepPair[parity], g, nullptr, nullptr, 0);
+        const hipError_t e = hipGraphInstantiate(&c->stepGraph[parity], g, nullptr, nullptr, 0);
         hipGraphDestroy(g);
         if (e != hipSuccess) {
-            c->stepPair[parity] = nullptr;
+            c->stepGraph[parity] = nullptr;
             return fail(PHL_ERR_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
         }
+        // capturing swapped the buffers kGraphSteps times: back at the start
     }
-    *out = c->stepPair[parity];
+    *out = c->stepGraph[parity];
+    return PHL_OK;
+}
+
+phl_status phl_sweep_levels(const phl_ctx *c, int *fwd_levels, int *bwd_levels)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    if (fwd_levels) *fwd_levels = c->fwd.nLevels;
+    if (bwd_levels) *bwd_levels = c->bwd.nLevels;
     return PHL_OK;
 }
 
@@ -1301,7 +1449,7 @@ phl_status phl_set_precision(phl_ctx *c, int precision)
     HIPTRY(hipStreamSynchronize(c->stream));
     c->k.mixed = precision;
     c->p.precision = precision;
-    for (hipGraphExec_t &g : c->stepPair) {  // the graphs hold the other kernels
+    for (hipGraphExec_t &g : c->stepGraph) {  // the graphs hold the other kernels
         if (g) hipGraphExecDestroy(g);
         g = nullptr;
     }
@@ -1351,7 +1499,7 @@ phl_status phl_autotune(phl_ctx *c, int *flux_tile, int *grad_tile, double *ms_t
     TRY(check_launch(c));
     c->k.fluxTile = pick[0];
     c->k.gradTile = pick[1];
-    for (hipGraphExec_t &g : c->stepPair) {  // the graphs hold the old tiles
+    for (hipGraphExec_t &g : c->stepGraph) {  // the graphs hold the old tiles
         if (g) hipGraphExecDestroy(g);
         g = nullptr;
     }
@@ -1362,7 +1510,8 @@ phl_status phl_autotune(phl_ctx *c, int *flux_tile, int *grad_tile, double *ms_t
 
 phl_status phl_set_launch_mode(phl_ctx *c, int mode)
 {
-    if (!c || (mode != 0 && mode != 1)) return fail(PHL_ERR_ARG, "launch mode must be 0 (eager) or 1 (graph)");
+    if (!c || mode < 0 || mode > 2)
+        return fail(PHL_ERR_ARG, "launch mode must be 0 (eager), 1 (graph) or 2 (eager, turbulence sweeps overlapped)");
     c->launchMode = mode;
     return PHL_OK;
 }
@@ -1390,9 +1539,10 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         reset_step_counter(c);
         int i = 0;
         if (!c->timing && c->launchMode == 1) {
-            for (; i + 2 <= chunk; i += 2) {
+            for (; i + kGraphSteps <= chunk; i += kGraphSteps) {
                 hipGraphExec_t g = nullptr;
-                TRY(step_pair_graph(c, &g));
+                join_turb(c);  // a graph starts with the side stream joined
+                TRY(step_graph(c, &g));
                 HIPTRY(hipGraphLaunch(g, c->stream));
             }
         }
@@ -1425,6 +1575,7 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
             }
         done += chunk;
     }
+    join_turb(c);  // later work on the solver stream sees the final nu~
     return PHL_OK;
 }
 
@@ -1480,6 +1631,35 @@ phl_status phl_hbm_copy(int device, double bytes, int iters, double *gbps)
 // Batched solve: every case runs until its own density residual has dropped
 // by `drop`; a converged case is frozen (its state is no longer updated) at
 // the end of the 32-step chunk in which it converged.
+// Progress of one case for the precision switch: the best residual so far
+// and the step at which it last halved.
+struct Progress {
+    double mark = INFINITY, best = INFINITY;
+    int since = 0;
+    void add(double r, int step)
+    {
+        if (r < best) best = r;
+        if (best < 0.5 * mark) {
+            mark = best;
+            since = step;
+        }
+    }
+};
+
+phl_status phl_set_precision_switch(phl_ctx *c, int stall_steps)
+{
+    if (!c || stall_steps < 0) return fail(PHL_ERR_ARG, "bad argument");
+    c->precisionSwitchStall = stall_steps;
+    return PHL_OK;
+}
+
+phl_status phl_get_precision(const phl_ctx *c, int *precision)
+{
+    if (!c || !precision) return fail(PHL_ERR_ARG, "null argument");
+    *precision = c->k.mixed;
+    return PHL_OK;
+}
+
 phl_status phl_solve_batch(phl_ctx *c, int max_steps, double drop, int *steps_done, double *rms_hist)
 {
     if (!c || max_steps <= 0 || !(drop > 0.0) || !steps_done) return fail(PHL_ERR_ARG, "bad argument");
@@ -1490,20 +1670,27 @@ phl_status phl_solve_batch(phl_ctx *c, int max_steps, double drop, int *steps_do
     if (n > 1) std::fill(c->hostFrozen, c->hostFrozen + n, (unsigned char)0);
     const int chunk = 32;
     int done = 0, left = n;
+    std::vector<Progress> prog(n);
     while (done < max_steps && left > 0) {
         const int m = std::min(chunk, max_steps - done);
         TRY(phl_iterate(c, 1 + done, m, hist.data() + (size_t)done * n));
+        bool stalled = true;
         for (int k = 0; k < n; ++k) {
             if (done == 0) r1[k] = hist[k];
             if (hit[k] >= 0) continue;
-            for (int i = done; i < done + m; ++i)
+            for (int i = done; i < done + m; ++i) {
+                prog[k].add(hist[(size_t)i * n + k], i);
                 if (hist[(size_t)i * n + k] <= drop * r1[k]) { hit[k] = i; break; }
+            }
             if (hit[k] >= 0) {
                 --left;
                 if (n > 1) c->hostFrozen[k] = 1;
+            } else if (done + m - prog[k].since < c->precisionSwitchStall) {
+                stalled = false;
             }
         }
         done += m;
+        if (c->k.mixed && c->precisionSwitchStall > 0 && left > 0 && stalled) TRY(phl_set_precision(c, 0));
     }
     for (int k = 0; k < n; ++k) steps_done[k] = hit[k] >= 0 ? hit[k] + 1 : -done;
     if (rms_hist) std::copy(hist.begin(), hist.begin() + (size_t)done * n, rms_hist);
@@ -1526,13 +1713,18 @@ phl_status phl_solve(phl_ctx *c, int max_steps, double drop, int *steps_done, do
     int done = 0;
     double r1 = 0.0;
     int hit = -1;
+    Progress prog;
     while (done < max_steps && hit < 0) {
         const int n = std::min(chunk, max_steps - done);
         TRY(phl_iterate(c, 1 + done, n, hist.data() + done));
         if (done == 0) r1 = hist[0];
-        for (int i = done; i < done + n; ++i)
+        for (int i = done; i < done + n; ++i) {
+            prog.add(hist[i], i);
             if (hist[i] <= drop * r1) { hit = i; break; }
+        }
         done += n;
+        if (hit < 0 && c->k.mixed && c->precisionSwitchStall > 0 && done - prog.since >= c->precisionSwitchStall)
+            TRY(phl_set_precision(c, 0));
     }
     if (steps_done) *steps_done = done;
     if (rms_hist) std::copy(hist.begin(), hist.begin() + done, rms_hist);

@@ -1,6 +1,10 @@
 // phl_internal.h -- device data layout shared by the kernels and the host
 // driver.  Not part of the C-ABI.
 //
+// Cell-indexed arrays are in storage order (a locality order of the real
+// cells, then the ghosts of boundary faces 0..nBFace-1); faces keep the
+// mesh's numbering.
+//
 // Layout in HBM (FP64 plain form):
 //   cell fields     SoA [m][nCell] or [m][nTotal] (ghosts after real cells)
 //   face geometry   AoS FaceRec, 64 B per face (one gather touches one line)
@@ -39,7 +43,14 @@ struct DevMesh {
     const unsigned short *caseOf;  // [nTotal] case of each cell (batched contexts)
     const int *caseCellPtr;        // [nCases+1] cell range of each case
     const double *wallDist;        // [nCell] distance to the nearest solid surface (SA)
+    // Cells are stored in a locality order; the sweeps and SolutionFix follow
+    // the mesh's own numbering through these maps (ghost cells are not moved).
+    const int *logicalOf;          // [nCell] mesh index of the cell stored at each position
+    const int *cellAt;             // [nCell] storage position of each mesh cell
 };
+
+// Mesh index of a stored cell (ghost indices are their own: above every real cell).
+__device__ __forceinline__ int mesh_index(const DevMesh &M, int i) { return i < M.nCell ? M.logicalOf[i] : i; }
 
 // The scalars that depend on a case's freestream (Mach, angles) and mesh
 // (mean volume).  A batched context holds one per case.
@@ -156,6 +167,9 @@ void launch_sa_residual(const DevMesh &m, const Consts &k, DevFlow &f, hipStream
 void launch_sa_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
                      hipStream_t s);
+// Everything of a relaxation before the sweeps: boundary, eddy viscosity,
+// gradient, residual, spectrum and face matrices.  launch_sa_lusgs is the rest.
+void launch_sa_pre(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sa_relax(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
                      hipStream_t s);
 

@@ -707,6 +707,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int c = lb * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
+    (void)nT;
     double acc[5] = {-0.0, -0.0, -0.0, -0.0, -0.0};
     double dtl = 1.0e300;
     if (c < nC) {
@@ -1045,7 +1046,8 @@ __device__ __forceinline__ void sweep_cell_batched(const DevMesh &M, const Const
     double co[kW][15], dqn[kW][5];
 #pragma unroll
     for (int s = 0; s < kW; ++s) {
-        act[s] = enc[s] != 0 && (FWD ? (nb[s] < c) : (nb[s] > c));
+        act[s] = enc[s] != 0 && (FWD ? (mesh_index(M, nb[s]) < mesh_index(M, c))
+                                     : (mesh_index(M, nb[s]) > mesh_index(M, c)));
         const bool isL = enc[s] > 0;
         const int f = act[s] ? (isL ? enc[s] : -enc[s]) - 1 : 0;
         load_coef(coef, f, isL, co[s]);
@@ -1075,7 +1077,7 @@ __device__ __forceinline__ void sweep_cell_generic(const DevMesh &M, const Const
         const int enc = M.c2fEnc[(size_t)s * nC + c];
         if (enc == 0) break;
         const int nb = M.c2n[(size_t)s * nC + c];
-        if (FWD ? (nb > c) : (nb < c)) continue;
+        if (FWD ? (mesh_index(M, nb) > mesh_index(M, c)) : (mesh_index(M, nb) < mesh_index(M, c))) continue;
         const bool isL = enc > 0;
         const int f = (isL ? enc : -enc) - 1;
         double co[15], dqn[5], df[5];
@@ -1322,7 +1324,9 @@ __global__ void k_fix(DevMesh M, Consts K0, const double *__restrict__ qOld, dou
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int nC = M.nCell, nT = M.nTotal;
     const int count = *negCount;
-    for (int c = 0; c < nC && count > 0; ++c) {
+    // the reference's loop order: increasing mesh index
+    for (int lc = 0; lc < nC && count > 0; ++lc) {
+        const int c = M.cellAt[lc];
         if (!negFlag[c]) continue;
         const Consts K = case_k(K0, M, c);
         negFlag[c] = 0;
@@ -1332,7 +1336,7 @@ __global__ void k_fix(DevMesh M, Consts K0, const double *__restrict__ qOld, dou
             const int enc = M.c2fEnc[(size_t)s * nC + c];
             if (enc == 0) break;
             const int nb = M.c2n[(size_t)s * nC + c];
-            const double *src = (nb < c) ? qNew : qOld;
+            const double *src = (mesh_index(M, nb) < lc) ? qNew : qOld;
             const double rn = src[nb], pn = src[(size_t)4 * nT + nb];
             if (rn < K.densLimit || pn < K.presLimit) continue;
             vs += 1.0;
@@ -1505,18 +1509,20 @@ void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 
 int flux_blocks(const DevMesh &m, const Consts &k) { return nblk(m.nCell, k.fluxTile); }
 
+template <bool SPEC_ONLY, bool FUSED_DT, typename T, int BS>
+static void launch_flux_passes(const DevMesh &m, const Consts &k, DevFlow &f, const T *grad, const T *gradT,
+                               hipStream_t s)
+{
+    const int nb = nblk(m.nCell, BS);
+    hipLaunchKernelGGL((k_flux<SPEC_ONLY, FUSED_DT, T, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, grad, gradT, f.lim,
+                       f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart, f.vist);
+}
+
 template <bool FUSED_DT, int BS>
 static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
-    const int nb = nblk(m.nCell, BS);
-    if (k.mixed)
-        hipLaunchKernelGGL((k_flux<false, FUSED_DT, float, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, f.gradF,
-                           f.gradTF, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
-                           f.dtMinPart, f.vist);
-    else
-        hipLaunchKernelGGL((k_flux<false, FUSED_DT, double, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, f.grad,
-                           f.gradT, f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
-                           f.dtMinPart, f.vist);
+    if (k.mixed) launch_flux_passes<false, FUSED_DT, float, BS>(m, k, f, f.gradF, f.gradTF, s);
+    else launch_flux_passes<false, FUSED_DT, double, BS>(m, k, f, f.grad, f.gradT, s);
 }
 
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt)
@@ -1531,9 +1537,7 @@ void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, b
 
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
-    hipLaunchKernelGGL((k_flux<true, false, double, 256>), dim3(nblk(m.nCell, 256)), dim3(256), 0, s, m, k, f.q, f.grad, f.gradT,
-                       f.lim, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart,
-                       f.vist);
+    launch_flux_passes<true, false, double, 256>(m, k, f, f.grad, f.gradT, s);
 }
 
 void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s)

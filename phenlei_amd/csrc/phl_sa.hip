@@ -644,14 +644,19 @@ void launch_sa_lusgs(const DevMesh &m, const Consts &, DevFlow &f, const Levels 
 // gradient of nu~, the residual (TurbSolver::UpdateResiduals, which applies
 // the boundary again), spectrum and face matrices (Diagonal), forward sweep,
 // ghost dq, backward sweep, update.
-void launch_sa_relax(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
-                     hipStream_t s)
+void launch_sa_pre(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
     launch_sa_boundary(m, k, f, s);
     launch_sa_viscosity(m, k, f, s);
     launch_sa_gradient(m, k, f, s);
     launch_sa_residual(m, k, f, s);
     launch_sa_spectrum(m, k, f, s);
+}
+
+void launch_sa_relax(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
+                     hipStream_t s)
+{
+    launch_sa_pre(m, k, f, s);
     launch_sa_lusgs(m, k, f, fwd, bwd, s);
 }
 
