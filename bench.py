@@ -183,12 +183,47 @@ def time_to_drop(mesh, pover, drop, max_steps, chunk=256, stall=2048):
     return out
 
 
-def batch_time_to_drop(mesh, params_list, drop, max_steps, stall=2048):
+def batch_solve_logged(s, max_steps, drop, stall, label, chunk=256):
+    """phl_solve_batch's loop (per-case drop, freezing, precision switch)
+    driven from here so that each chunk of iterations reports its progress."""
+    n = s.n_cases
+    r1 = None
+    hit = np.full(n, -1)
+    best = np.full(n, np.inf)
+    mark = np.full(n, np.inf)
+    since = np.zeros(n, int)
+    done = 0
+    while done < max_steps and np.any(hit < 0):
+        h = s.iterate(1 + done, min(chunk, max_steps - done))
+        if r1 is None:
+            r1 = h[0].copy()
+        for k in np.nonzero(hit < 0)[0]:
+            for i in range(h.shape[0]):
+                v = h[i, k]
+                best[k] = min(best[k], v)
+                if best[k] < 0.5 * mark[k]:
+                    mark[k], since[k] = best[k], done + i
+                if v <= drop * r1[k]:
+                    hit[k] = done + i + 1
+                    break
+        done += h.shape[0]
+        s.set_frozen(hit > 0)
+        left = hit < 0
+        if stall and s.precision() == "mixed" and left.any() and np.all(done - since[left] >= stall):
+            s.set_precision("fp64")
+            print(f"[bench]   {label}: switched to fp64 at step {done}", file=sys.stderr, flush=True)
+        print(f"[bench]   {label}: step {done}, {int(np.sum(hit > 0))}/{n} cases at the drop, "
+              f"worst remaining drop {np.max((best / r1)[left]) if left.any() else 0:.2e}",
+              file=sys.stderr, flush=True)
+    return np.where(hit > 0, hit, -done)
+
+
+def batch_time_to_drop(mesh, params_list, drop, max_steps, stall=2048, forms=("fp64", "mixed_then_fp64")):
     """Batched form: every case of the batch to drop * its own R_1, (a) FP64
     throughout and (b) mixed with the switch to FP64 once every unconverged
     case has stalled (phl_set_precision_switch).  Converged cases freeze."""
     out = {}
-    for form in ("fp64", "mixed_then_fp64"):
+    for form in forms:
         plist = []
         for p in params_list:
             q = pa.Params.from_buffer_copy(p)
@@ -201,7 +236,7 @@ def batch_time_to_drop(mesh, params_list, drop, max_steps, stall=2048):
             s.set_precision_switch(stall)
         s.synchronize()
         t0 = time.perf_counter()
-        steps, _ = s.solve_batch(max_steps, drop)
+        steps = batch_solve_logged(s, max_steps, drop, stall if form != "fp64" else 0, form)
         s.synchronize()
         wall = time.perf_counter() - t0
         out[form] = dict(seconds=round(wall, 3), cases_reached=int(np.sum(steps > 0)),
@@ -209,8 +244,8 @@ def batch_time_to_drop(mesh, params_list, drop, max_steps, stall=2048):
                          final_precision=s.precision())
         print(f"[bench] batch time to {drop:g} drop, {form}: {out[form]}", file=sys.stderr, flush=True)
         s.close()
-    a, b = out["fp64"], out["mixed_then_fp64"]
-    out["speedup"] = round(a["seconds"] / b["seconds"], 3)
+    if "fp64" in out and "mixed_then_fp64" in out:
+        out["speedup"] = round(out["fp64"]["seconds"] / out["mixed_then_fp64"]["seconds"], 3)
     out["drop"] = drop
     return out
 
@@ -229,9 +264,13 @@ def main():
                          "eagerly with the turbulence sweeps on a second stream")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="override the configuration's precision")
+    ap.add_argument("--storage-order", default=None, choices=["locality", "mesh"],
+                    help="device storage order of the cells (default: the library's, locality)")
     ap.add_argument("--converge", type=float, default=None, metavar="DROP",
                     help="also time the FP64 and mixed-then-FP64 forms to this residual drop from freestream")
     ap.add_argument("--max-steps", type=int, default=20000, help="iteration cap of --converge")
+    ap.add_argument("--forms", default="fp64,mixed_then_fp64",
+                    help="forms timed by --converge on a batch (comma separated)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -239,8 +278,12 @@ def main():
 
     cfg = CONFIGS[args.config]
     print(f"[bench] config {args.config}: building mesh", file=sys.stderr, flush=True)
+    so = {"locality": 0, "mesh": 1}.get(args.storage_order)
     if cfg.get("batch"):
         parts, params_list = build_batch(cfg)
+        if so is not None:
+            for p in params_list:
+                p.storage_order = so
         mesh = pa.Mesh.concat(parts)
         params = params_list[0]
         solver = pa.Solver(mesh, params_list, device=0)
@@ -250,6 +293,8 @@ def main():
         pover = dict(cfg["params"])
         if args.precision:
             pover["precision"] = 1 if args.precision == "mixed" else 0
+        if so is not None:
+            pover["storage_order"] = so
         params = pa.default_params(**pover)
         print(f"[bench] {mesh.n_cell} cells: creating the context", file=sys.stderr, flush=True)
         solver = pa.Solver(mesh, params, device=0)
@@ -305,7 +350,8 @@ def main():
     if args.converge is not None:
         solver.close()
         if cfg.get("batch"):
-            line["time_to_drop"] = batch_time_to_drop(mesh, params_list, args.converge, args.max_steps)
+            line["time_to_drop"] = batch_time_to_drop(mesh, params_list, args.converge, args.max_steps,
+                                                      forms=tuple(args.forms.split(",")))
         else:
             line["time_to_drop"] = time_to_drop(mesh, pover, args.converge, args.max_steps)
     if not args.no_cpu:

@@ -179,6 +179,10 @@ void phl_destroy(phl_ctx *ctx);
  * phl_time_step and phl_lusgs hooks return PHL_ERR_UNSUPPORTED. */
 phl_status phl_create_batch(const phl_mesh *mesh, const phl_params *params, int device, phl_ctx **out);
 phl_status phl_batch_size(const phl_ctx *ctx, int *n_cases);
+/* Freeze cases of a batched context (frozen[c] = 1: the update keeps case c's
+ * state) from the next phl_iterate on; for drivers that judge convergence
+ * themselves.  phl_solve_batch manages these flags on its own. */
+phl_status phl_set_frozen(phl_ctx *ctx, const unsigned char *frozen);
 
 /* ---------------- state transfer -------------------------------------- */
 phl_status phl_set_state(phl_ctx *ctx, const double *q);          /* [5][nTotal] primitive */

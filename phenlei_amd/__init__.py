@@ -144,6 +144,7 @@ def _declare(lib):
         "phl_mesh_cases": (ctypes.c_int, [_P, _I, _I, _I]),
         "phl_create_batch": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_P)]),
         "phl_batch_size": (ctypes.c_int, [_P, _I]),
+        "phl_set_frozen": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_ubyte)]),
         "phl_solve_batch": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
         "phl_iterate": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_int, _D]),
         "phl_solve": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_double, _I, _D]),
@@ -493,6 +494,11 @@ class Solver:
         _check(_lib.phl_solve_batch(self._h, max_steps, drop, _iptr(steps), _dptr(hist)))
         run = int(np.max(np.abs(steps)))
         return steps, hist[:run]
+
+    def set_frozen(self, frozen):
+        """Freeze cases of a batch (bool per case) from the next iterate on."""
+        f = np.ascontiguousarray(np.asarray(frozen, dtype=np.uint8))
+        _check(_lib.phl_set_frozen(self._h, f.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))))
 
     def forces_case(self, k):
         out = np.empty(len(FORCE_NAMES))

@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <utility>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -673,6 +674,25 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
             c2f[k] = f; c2n[k] = le; isLeft[k] = 0;
         }
     }
+    // Storage order of the interior faces: sorted by the storage positions of
+    // their two cells, so that a run of faces reads a compact run of cells
+    // and a cell's faces sit close together; boundary faces keep their
+    // index (their ghost cell is nCell+f).  Each cell still accumulates its
+    // faces in the mesh's face order: fslot only maps where a face's data live.
+    std::vector<int> fslot(nF);
+    for (int f = 0; f < nF; ++f) fslot[f] = f;
+    if (p.storage_order == 0 && nF > nB) {
+        std::vector<int> ord(nF - nB);
+        for (int f = nB; f < nF; ++f) ord[f - nB] = f;
+        auto key = [&](int f) {
+            const int a = slotOf[m.left[f]], b = slotOf[m.right[f]];
+            return std::make_pair(std::min(a, b), std::max(a, b));
+        };
+        std::stable_sort(ord.begin(), ord.end(), [&](int u, int v) { return key(u) < key(v); });
+        for (int i = 0; i < nF - nB; ++i) fslot[ord[i]] = nB + i;
+    }
+    std::vector<int> c2fS(c2f.size());
+    for (size_t k = 0; k < c2f.size(); ++k) c2fS[k] = fslot[c2f[k]];
     int W = 0;
     for (int i = 0; i < nC; ++i) W = std::max(W, c2fPtr[i + 1] - c2fPtr[i]);
     std::vector<int> ellEnc((size_t)W * nC, 0), ellNb((size_t)W * nC, 0);
@@ -680,14 +700,14 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         const int st = slotOf[i];
         for (int k = c2fPtr[i]; k < c2fPtr[i + 1]; ++k) {
             const int s = k - c2fPtr[i];
-            ellEnc[(size_t)s * nC + st] = isLeft[k] ? (c2f[k] + 1) : -(c2f[k] + 1);
+            ellEnc[(size_t)s * nC + st] = isLeft[k] ? (c2fS[k] + 1) : -(c2fS[k] + 1);
             ellNb[(size_t)s * nC + st] = S(c2n[k]);
         }
     }
 
     std::vector<phl::FaceRec> faces(nF);
     for (int f = 0; f < nF; ++f)
-        faces[f] = {m.xfn[f], m.yfn[f], m.zfn[f], m.area[f], m.xfc[f], m.yfc[f], m.zfc[f], 0.0};
+        faces[fslot[f]] = {m.xfn[f], m.yfn[f], m.zfn[f], m.area[f], m.xfc[f], m.yfc[f], m.zfc[f], 0.0};
 
     // Venkatakrishnan scale and CFL skewness factor (geometry only).
     std::vector<double> vf(nC), cflSkew(nC);
@@ -727,7 +747,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         if (a1 > 0.0 && a2 > 0.0) fl |= 2;
         // turbulence diffusion (TurbSolverUnstr.cpp ComputeVisflux, turbSkewnessAngle)
         if (a1 > p.turb_skew_angle && a2 > p.turb_skew_angle) fl |= 4;
-        visFlag[f] = fl;
+        visFlag[fslot[f]] = fl;
     }
 
     phl::DevMesh &dm = c->m;
@@ -735,9 +755,10 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(upload(c, &dm.face, faces));
     TRY(upload(c, &dm.bcType, m.bcType));
     {
-        std::vector<int> fle(nF);
-        for (int f = 0; f < nF; ++f) fle[f] = S(m.left[f]);
+        std::vector<int> fle(nF), fre(nF);
+        for (int f = 0; f < nF; ++f) { fle[fslot[f]] = S(m.left[f]); fre[fslot[f]] = S(m.right[f]); }
         TRY(upload(c, &dm.fle, fle));
+        TRY(upload(c, &dm.fre, fre));
     }
     TRY(upload(c, &dm.c2fEnc, ellEnc));
     TRY(upload(c, &dm.c2n, ellNb));
@@ -780,7 +801,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     for (int dir = 0; dir < 2; ++dir) {
         phl::Levels &lv = dir == 0 ? c->fwd : c->bwd;
         std::vector<int> row, nbs;
-        build_sweep_order(nC, c2fPtr, c2f, c2n, isLeft, dir == 0 ? cellsF : cellsB, dir == 0, lv.so.W, lv.so.Wpad,
+        build_sweep_order(nC, c2fPtr, c2fS, c2n, isLeft, dir == 0 ? cellsF : cellsB, dir == 0, lv.so.W, lv.so.Wpad,
                           row, nbs);
         for (int &v : nbs) v = S(v);  // unused slots hold 0, a valid storage index
         lv.so.cells = lv.cells;
@@ -880,8 +901,25 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.t, (size_t)nT));
     TRY(dalloc(c, &fl.visl, (size_t)nT));
     TRY(dalloc(c, &fl.forceOut, 15));
+    TRY(dalloc(c, &fl.faceInv, (size_t)6 * nF));
+    TRY(dalloc(c, &fl.faceVis, p.viscous ? (size_t)6 * nF : 1));
     TRY(dalloc(c, &fl.vist, (size_t)nT));
     HIPTRY(hipMemset(fl.vist, 0, sizeof(double) * nT));
+    {
+        // gather records: centroids and limiter start values; states and
+        // gradients are written by the boundary and gradient kernels
+        std::vector<phl::GatherRec> rec((size_t)nT);
+        const std::vector<double> xs = permuteCells(m.xcc), ys = permuteCells(m.ycc), zs = permuteCells(m.zcc);
+        for (int i = 0; i < nT; ++i) {
+            phl::GatherRec &r = rec[i];
+            std::memset(&r, 0, sizeof(r));
+            r.x = xs[i]; r.y = ys[i]; r.z = zs[i];
+            r.lim = limInit[i];
+        }
+        const phl::GatherRec *d = nullptr;
+        TRY(upload(c, &d, rec));
+        fl.rec = const_cast<phl::GatherRec *>(d);
+    }
     dm.wallDist = nullptr;
     if (p.viscous == 3) {
         HIPTRY(hipStreamCreateWithFlags(&c->side, hipStreamNonBlocking));
@@ -1695,6 +1733,14 @@ phl_status phl_solve_batch(phl_ctx *c, int max_steps, double drop, int *steps_do
     for (int k = 0; k < n; ++k) steps_done[k] = hit[k] >= 0 ? hit[k] + 1 : -done;
     if (rms_hist) std::copy(hist.begin(), hist.begin() + (size_t)done * n, rms_hist);
     if (n > 1) std::fill(c->hostFrozen, c->hostFrozen + n, (unsigned char)0);
+    return PHL_OK;
+}
+
+phl_status phl_set_frozen(phl_ctx *c, const unsigned char *frozen)
+{
+    if (!c || !frozen) return fail(PHL_ERR_ARG, "null argument");
+    if (c->k.nCases <= 1) return fail(PHL_ERR_UNSUPPORTED, "phl_set_frozen: not a batched context");
+    std::copy(frozen, frozen + c->k.nCases, c->hostFrozen);  // uploaded by the next phl_iterate
     return PHL_OK;
 }
 

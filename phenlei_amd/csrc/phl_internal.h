@@ -29,6 +29,7 @@ struct DevMesh {
     const FaceRec *face;        // [nFace]
     const int *bcType;          // [nBFace]
     const int *fle;             // [nFace] left cell
+    const int *fre;             // [nFace] right cell (the ghost nCell+f of a boundary face)
     const int *c2fEnc;          // [W][nCell]: f+1 if the cell is left, -(f+1) if right, 0 pad
     const int *c2n;             // [W][nCell]: the other cell (ghost index nCell+f on boundaries)
     const double *xcc, *ycc, *zcc;  // [nTotal]
@@ -80,6 +81,20 @@ struct Consts {
     const CaseConsts *cases;
 };
 
+// What the face loop gathers of one cell, in one 192-byte record (two cache
+// lines instead of one per scattered SoA value): state, centroid, then the
+// gradients (FP64, or FP32 in the mixed form, filling the first 60 bytes of
+// the slot), then the limiter.  Real cells are refreshed by the gradient
+// kernel, ghost states by the boundary kernel; centroids, ghost gradients
+// (zero) and ghost limiters never change.
+struct alignas(64) GatherRec {
+    double q[5];
+    double x, y, z;
+    double g[15];
+    double lim;
+};
+static_assert(sizeof(GatherRec) == 192, "GatherRec is three 64-byte sectors");
+
 // Mutable flow state on the device.
 struct DevFlow {
     double *q;        // [5][nTotal] primitive variables (current)
@@ -107,6 +122,9 @@ struct DevFlow {
     double *forceOut;   // [15] wall force and moment sums
     unsigned char *frozen;  // [nCases] batched contexts: 1 once a case has converged
     double *vist;     // [nTotal] eddy viscosity (0 unless SA)
+    GatherRec *rec;   // [nTotal] face-loop gather records
+    double *faceInv;  // [nFace][6] inviscid flux (area-scaled) and spectral-radius term of each face
+    double *faceVis;  // [nFace][6] viscous flux and spectral-radius term (viscous flow only)
     // Spalart-Allmaras (viscous == 3), allocated only then
     double *nut;      // [nTotal] nu~ (non-conservative)
     double *resT;     // [nTotal] residual of nu~ (real cells written)

@@ -170,7 +170,8 @@ __device__ __forceinline__ void load5(const double *__restrict__ a, int n, int i
 }
 
 // ------------------------------------------------------------------ BCs
-__global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K0, double *__restrict__ q)
+__global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K0, double *__restrict__ q,
+                                                     GatherRec *__restrict__ rec)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= M.nBFace) return;
@@ -233,7 +234,10 @@ __global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K0, doubl
     }
     // extrapolation / outflow: copy (already in qr)
 #pragma unroll
-    for (int m = 0; m < 5; ++m) q[(size_t)m * nT + re] = qr[m];
+    for (int m = 0; m < 5; ++m) {
+        q[(size_t)m * nT + re] = qr[m];
+        rec[re].q[m] = qr[m];
+    }
 }
 
 // Standalone gamma/T/viscosity hook (the fused kernels recompute T and mu).
@@ -384,7 +388,7 @@ template <typename GT, int BS>
 __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const double *__restrict__ q,
                                                      GT *__restrict__ grad, GT *__restrict__ gradT,
                                                      double *__restrict__ lim, double *__restrict__ coef,
-                                                     const double *__restrict__ vist)
+                                                     const double *__restrict__ vist, GatherRec *__restrict__ rec)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
@@ -479,6 +483,17 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
         if (M.limZero[c]) l = 0.0;
     }
     lim[c] = l;
+    if (rec) {  // the face loop's gather record of this cell (null: gradients for other readers only)
+        GatherRec &r = rec[c];
+        double2 *dq2 = reinterpret_cast<double2 *>(r.q);
+        dq2[0] = make_double2(qc[0], qc[1]);
+        dq2[1] = make_double2(qc[2], qc[3]);
+        r.q[4] = qc[4];
+        GT *g = reinterpret_cast<GT *>(r.g);
+#pragma unroll
+        for (int m = 0; m < 5; ++m) { g[m] = GT(gx[m]); g[5 + m] = GT(gy[m]); g[10 + m] = GT(gz[m]); }
+        r.lim = l;
+    }
 }
 
 // ------------------------------------------------- face fluxes (gather)
@@ -506,6 +521,20 @@ __device__ __forceinline__ void load_side(const DevMesh &M, const double *__rest
 #pragma unroll
         for (int m = 0; m < 15; ++m) s.g[m] = T(0.0);
     }
+}
+
+template <typename T>
+__device__ __forceinline__ void load_side_rec(const GatherRec *__restrict__ rec, int i, bool needGrad, CellSide<T> &s)
+{
+    const GatherRec &r = rec[i];
+    const double2 *v = reinterpret_cast<const double2 *>(&r);
+    const double2 a = v[0], b = v[1], c = v[2], d = v[3];
+    s.q[0] = a.x; s.q[1] = a.y; s.q[2] = b.x; s.q[3] = b.y; s.q[4] = c.x;
+    s.x = c.y; s.y = d.x; s.z = d.y;
+    s.lim = r.lim;
+    const T *g = reinterpret_cast<const T *>(r.g);
+#pragma unroll
+    for (int m = 0; m < 15; ++m) s.g[m] = needGrad ? g[m] : T(0.0);
 }
 
 // MUSCL face value (reconmeth 1) with the first-order fallback on negative
@@ -702,7 +731,8 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                                                  double *__restrict__ specInv, double *__restrict__ specVis,
                                                  double *__restrict__ normPart, const double *__restrict__ cflArr,
                                                  const int *__restrict__ stepCtr, double *__restrict__ dtLoc,
-                                                 double *__restrict__ dtMinPart, const double *__restrict__ vist)
+                                                 double *__restrict__ dtMinPart, const double *__restrict__ vist,
+                                                 const GatherRec *__restrict__ rec)
 {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
     const int c = lb * blockDim.x + threadIdx.x;
@@ -714,7 +744,10 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
         const Consts K = case_k(K0, M, c);
         const bool needG = !SPEC_ONLY && (K.limiter != 2 || K.viscous);
         CellSide<T> own;
-        load_side(M, q, grad, lim, c, needG, own);
+        // the spectrum hook reads the SoA state directly (records are
+        // refreshed by the boundary and gradient kernels of a residual)
+        if (SPEC_ONLY) load_side(M, q, grad, lim, c, needG, own);
+        else load_side_rec(rec, c, needG, own);
         double sInv = 0.0;
         for (int s = 0; s < M.ellW; ++s) {
             const int enc = M.c2fEnc[(size_t)s * nC + c];
@@ -724,7 +757,8 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
             const int nb = M.c2n[(size_t)s * nC + c];
             const FaceRec fr = M.face[f];
             CellSide<T> oth;
-            load_side(M, q, grad, lim, nb, !SPEC_ONLY && K.limiter != 2, oth);
+            if (SPEC_ONLY) load_side(M, q, grad, lim, nb, false, oth);
+            else load_side_rec(rec, nb, K.limiter != 2, oth);
             const CellSide<T> &L = isL ? own : oth;
             const CellSide<T> &R = isL ? oth : own;
             if (!SPEC_ONLY) {
@@ -794,7 +828,8 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                 const int nb = M.c2n[(size_t)s * nC + c];
                 const FaceRec fr = M.face[f];
                 CellSide<T> oth;
-                load_side(M, q, grad, lim, nb, !SPEC_ONLY, oth);
+                if (SPEC_ONLY) load_side(M, q, grad, lim, nb, false, oth);
+                else load_side_rec(rec, nb, true, oth);
                 const CellSide<T> &L = isL ? own : oth;
                 const CellSide<T> &R = isL ? oth : own;
                 const bool bnd = f < M.nBFace;
@@ -859,6 +894,193 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
 #pragma unroll
         for (int m = 0; m < 5; ++m) normPart[(size_t)lb * 5 + m] = red[m][0];
         if (FUSED_DT) dtMinPart[lb] = red[5][0];
+    }
+}
+
+// Face-parallel form of the face loop: one thread per face evaluates the
+// face's inviscid flux and spectral-radius term (and the viscous ones) once,
+// from the two cells' gather records, and stores them; k_cell_gather then
+// sums them per cell.  The per-face values are exactly what each owner
+// computes in k_flux (they depend only on the face's left/right states), and
+// the gather adds them in the same order, so the residuals are identical
+// while every interior face is evaluated once instead of twice.
+template <typename T>
+__global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const GatherRec *__restrict__ rec,
+                                                   const T *__restrict__ gradT, const double *__restrict__ vist,
+                                                   double *__restrict__ faceInv, double *__restrict__ faceVis)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= M.nFace) return;
+    const int nC = M.nCell;
+    const int le = M.fle[f], re = M.fre[f];
+    const Consts K = case_k(K0, M, le);
+    const FaceRec fr = M.face[f];
+    const bool needG = K.limiter != 2 || K.viscous;
+    CellSide<T> L, R;
+    load_side_rec(rec, le, needG, L);
+    load_side_rec(rec, re, needG, R);
+    const bool bnd = f < M.nBFace;
+    const int bc = bnd ? M.bcType[f] : 0;
+    {
+        T qL[5], qR[5];
+        if (K.limiter != 2) {
+            const double lmin = dMIN(L.lim, R.lim);
+            reconstruct(L, lmin, fr, qL);
+            reconstruct(R, lmin, fr, qR);
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) { qL[m] = T(L.q[m]); qR[m] = T(R.q[m]); }
+        }
+        const T nx = T(fr.nx), ny = T(fr.ny), nz = T(fr.nz);
+        if (bnd && (bc == kBC_WALL || bc == kBC_SYMM)) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const T tmp = T(0.5 * (L.q[m] + R.q[m]));
+                qL[m] = tmp; qR[m] = tmp;
+            }
+            if (bc == kBC_SYMM) {
+                const T vn = T(2.0) * (nx * qL[1] + ny * qL[2] + nz * qL[3] - T(0.0));
+                qR[0] = qL[0];
+                qR[1] = qL[1] - vn * nx;
+                qR[2] = qL[2] - vn * ny;
+                qR[3] = qL[3] - vn * nz;
+                qR[4] = qL[4];
+            }
+            if (bc == kBC_WALL && K.viscous) {
+                qL[1] = T(0.0); qL[2] = T(0.0); qL[3] = T(0.0);
+                qR[1] = T(0.0); qR[2] = T(0.0); qR[3] = T(0.0);
+            }
+        }
+        T fl[5];
+        roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
+        const T area = T(fr.area);
+        // inviscid spectral radius term at the face-averaged state
+        const double vnl = fr.nx * L.q[1] + fr.ny * L.q[2] + fr.nz * L.q[3] - 0.0;
+        const double vnr = fr.nx * R.q[1] + fr.ny * R.q[2] + fr.nz * R.q[3] - 0.0;
+        const double g = 0.5 * (K.gamma + K.gamma);
+        const double pm = 0.5 * (L.q[4] + R.q[4]);
+        const double rm = 0.5 * (L.q[0] + R.q[0]);
+        const double vn = 0.5 * (vnl + vnr);
+        const double cm = sqrt(g * pm / rm);
+        double out[6];
+#pragma unroll
+        for (int m = 0; m < 5; ++m) out[m] = double(fl[m] * area);
+        out[5] = 0.5 * fr.area * (dABS(vn) + cm);
+        double2 *dst = reinterpret_cast<double2 *>(faceInv + (size_t)f * 6);
+        dst[0] = make_double2(out[0], out[1]);
+        dst[1] = make_double2(out[2], out[3]);
+        dst[2] = make_double2(out[4], out[5]);
+    }
+    if (K.viscous) {
+        T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
+        if (le < nC) { gTL[0] = gradT[le]; gTL[1] = gradT[(size_t)nC + le]; gTL[2] = gradT[(size_t)2 * nC + le]; }
+        if (re < nC) { gTR[0] = gradT[re]; gTR[1] = gradT[(size_t)nC + re]; gTR[2] = gradT[(size_t)2 * nC + re]; }
+        const bool wall = bnd && bc == kBC_WALL;
+        double out[6];
+        viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[le], vist[re], out);
+        const double tl = L.q[4] / (K.R * L.q[0]);
+        const double tr = R.q[4] / (K.R * R.q[0]);
+        const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
+                                 tr * sqrt(tr) * (1.0 + K.tsuth) / (tr + K.tsuth));
+        const double vt = 0.5 * (vist[le] + vist[re]);
+        const double dx = R.x - L.x, dy = R.y - L.y, dz = R.z - L.z;
+        const double ds = dABS(fr.nx * dx + fr.ny * dy + fr.nz * dz);
+        const double rho = 0.5 * (L.q[0] + R.q[0]);
+        const double vis = vl + vt;
+        double sr = 2.0 * vis / (rho * ds * K.reynolds + kSMALL);
+        sr *= 0.5 * fr.area;
+        out[5] = sr;
+        double2 *dst = reinterpret_cast<double2 *>(faceVis + (size_t)f * 6);
+        dst[0] = make_double2(out[0], out[1]);
+        dst[1] = make_double2(out[2], out[3]);
+        dst[2] = make_double2(out[4], out[5]);
+    }
+}
+
+// Per cell: the face values summed in the cell's face order (every inviscid
+// contribution, then every viscous one: the reference's LoadFlux order),
+// spectral radii, local time step and the block partials of res^2 and min dt.
+template <int BS>
+__global__ void __launch_bounds__(BS) k_cell_gather(DevMesh M, Consts K0, const double *__restrict__ q,
+                                                    const double *__restrict__ faceInv,
+                                                    const double *__restrict__ faceVis, double *__restrict__ res,
+                                                    double *__restrict__ specInv, double *__restrict__ specVis,
+                                                    double *__restrict__ normPart, const double *__restrict__ cflArr,
+                                                    const int *__restrict__ stepCtr, double *__restrict__ dtLoc,
+                                                    double *__restrict__ dtMinPart, bool fusedDt)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int c = lb * blockDim.x + threadIdx.x;
+    const int nC = M.nCell;
+    double acc[5] = {-0.0, -0.0, -0.0, -0.0, -0.0};
+    double dtl = 1.0e300;
+    if (c < nC) {
+        const Consts K = case_k(K0, M, c);
+        double sInv = 0.0, sVis = 0.0;
+        for (int s = 0; s < M.ellW; ++s) {
+            const int enc = M.c2fEnc[(size_t)s * nC + c];
+            if (enc == 0) break;
+            const bool isL = enc > 0;
+            const int f = (isL ? enc : -enc) - 1;
+            const double2 *src = reinterpret_cast<const double2 *>(faceInv + (size_t)f * 6);
+            const double2 a = src[0], b = src[1], e = src[2];
+            const double fl[5] = {a.x, a.y, b.x, b.y, e.x};
+            if (isL) {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) acc[m] += fl[m];
+            }
+            sInv += e.y;
+        }
+        specInv[c] = sInv;
+        if (K.viscous) {
+            for (int s = 0; s < M.ellW; ++s) {
+                const int enc = M.c2fEnc[(size_t)s * nC + c];
+                if (enc == 0) break;
+                const bool isL = enc > 0;
+                const int f = (isL ? enc : -enc) - 1;
+                const double2 *src = reinterpret_cast<const double2 *>(faceVis + (size_t)f * 6);
+                const double2 a = src[0], b = src[1], e = src[2];
+                const double fl[5] = {a.x, a.y, b.x, b.y, e.x};
+                if (isL) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) acc[m] += fl[m];
+                }
+                sVis += e.y;
+            }
+            specVis[c] = sVis;
+        }
+        if (fusedDt) {
+            double spec = sInv;
+            if (K.viscous) spec += sVis;
+            dtl = local_dt(M, K, c, q[(size_t)4 * M.nTotal + c], spec, cflArr[*stepCtr]);
+            dtLoc[c] = dtl;
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) res[(size_t)m * nC + c] = acc[m];
+    }
+    __shared__ double red[6][BS];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) red[m][threadIdx.x] = (c < nC) ? acc[m] * acc[m] : 0.0;
+    red[5][threadIdx.x] = dtl;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+            red[5][threadIdx.x] = dMIN(red[5][threadIdx.x], red[5][threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) normPart[(size_t)lb * 5 + m] = red[m][0];
+        if (fusedDt) dtMinPart[lb] = red[5][0];
     }
 }
 
@@ -1478,7 +1700,7 @@ void launch_copy(const double *src, double *dst, size_t n, hipStream_t s)
 void launch_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
     if (m.nBFace == 0) return;
-    hipLaunchKernelGGL(k_boundary, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, k, f.q);
+    hipLaunchKernelGGL(k_boundary, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.rec);
 }
 
 void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
@@ -1489,12 +1711,14 @@ void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
 template <int BS>
 static void launch_grad_lim_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp32)
 {
+    // the gather records hold the gradient type of the context's face loop
+    GatherRec *rec = (fp32 == (k.mixed != 0)) ? f.rec : nullptr;
     if (fp32)
         hipLaunchKernelGGL((k_grad_lim<float, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.gradF,
-                           f.gradTF, f.lim, f.coef, f.vist);
+                           f.gradTF, f.lim, f.coef, f.vist, rec);
     else
         hipLaunchKernelGGL((k_grad_lim<double, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.grad,
-                           f.gradT, f.lim, f.coef, f.vist);
+                           f.gradT, f.lim, f.coef, f.vist, rec);
 }
 
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad)
@@ -1515,14 +1739,22 @@ static void launch_flux_passes(const DevMesh &m, const Consts &k, DevFlow &f, co
 {
     const int nb = nblk(m.nCell, BS);
     hipLaunchKernelGGL((k_flux<SPEC_ONLY, FUSED_DT, T, BS>), dim3(nb), dim3(BS), 0, s, m, k, f.q, grad, gradT, f.lim,
-                       f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart, f.vist);
+                       f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart, f.vist,
+                       f.rec);
 }
 
 template <bool FUSED_DT, int BS>
 static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
-    if (k.mixed) launch_flux_passes<false, FUSED_DT, float, BS>(m, k, f, f.gradF, f.gradTF, s);
-    else launch_flux_passes<false, FUSED_DT, double, BS>(m, k, f, f.grad, f.gradT, s);
+    const int nbf = nblk(m.nFace, 256);
+    if (k.mixed)
+        hipLaunchKernelGGL(k_face_flux<float>, dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradTF, f.vist, f.faceInv,
+                           f.faceVis);
+    else
+        hipLaunchKernelGGL(k_face_flux<double>, dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradT, f.vist, f.faceInv,
+                           f.faceVis);
+    hipLaunchKernelGGL((k_cell_gather<BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.faceInv, f.faceVis,
+                       f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart, FUSED_DT);
 }
 
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt)
