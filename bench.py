@@ -30,10 +30,11 @@ import phenlei_amd as pa  # noqa: E402
 METRIC = "cell-updates/s and time-to-1e-8 residual drop: mixed+warm-start vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes per k_flux launch measured with rocprofv3 FETCH_SIZE (x2.0, the
-# factor a 1 GiB copy kernel calibrates) + WRITE_SIZE (x1.0), per workload:
-# profiles/k_flux_bump_hybrid_1m.md.  None where not measured.
-MEASURED_FLUX_TRAFFIC = {"c2": 2649.0e6}
+# HBM bytes of one face loop (k_face_flux + k_cell_gather) measured with
+# rocprofv3 FETCH_SIZE (x2.0, the factor a 1 GiB copy kernel calibrates) +
+# WRITE_SIZE (x1.0), per workload: profiles/k_flux_bump_hybrid_1m.md.  None
+# where not measured.
+MEASURED_FLUX_TRAFFIC = {"c2": 1380.6e6}
 
 # Workloads.  'c1' is the reference's CPU-runnable case (64x32x8 hex bump
 # channel, inviscid, M0.5, CFL 10).
@@ -56,10 +57,12 @@ CONFIGS["c3"] = dict(workload="swept wing (root chord 1, taper 0.5, AR 4, quarte
                               "cubic 12% section), hybrid 3,983,616 cells (30 prism layers each side of the wing plane, "
                               "first height 1e-5, growth 1.2; tetrahedral rows to a farfield at 20 root chords), "
                               "Spalart-Allmaras RANS, M0.6, AoA 2 deg, Re 3e6 per root chord, Roe + GG-cell + "
-                              "Venkat(K=5), LU-SGS CFL 10",
+                              "Venkat(K=5), LU-SGS with a CFL ramp 0.01 -> 5 over 500 steps (the reference's ramp "
+                              "to 10 diverges behind the tip near step 450)",
                      mesh=("swept_wing", 80, 40, 30, 1e-5, 29),
-                     params=dict(mach=0.6, alpha_deg=2.0, viscous=3, reynolds=3e6, cfl_start=10.0, cfl_end=10.0,
-                                 force_ref_area=1.125, force_ref_length=0.7778, force_ref_span=1.5),
+                     params=dict(mach=0.6, alpha_deg=2.0, viscous=3, reynolds=3e6, cfl_start=0.01, cfl_end=5.0,
+                                 cfl_vary_step=500, force_ref_area=1.125, force_ref_length=0.7778,
+                                 force_ref_span=1.5),
                      cpu_steps=1)
 DEFAULT_CONFIG = "c2"
 
@@ -183,6 +186,43 @@ def time_to_drop(mesh, pover, drop, max_steps, chunk=256, stall=2048):
     return out
 
 
+def precision_compare(mesh, pover, steps, chunk=250):
+    """The same number of iterations from freestream in the FP64 and the
+    mixed form: lift and drag of both, and the relative L2 difference of the
+    fields (each primitive variable's deviation from freestream, over real
+    cells)."""
+    out = {"steps": steps}
+    fields = {}
+    for form in ("fp64", "mixed"):
+        s = pa.Solver(mesh, pa.default_params(**dict(pover, precision=0 if form == "fp64" else 1)))
+        s.init_freestream()
+        done, last = 0, None
+        while done < steps:
+            h = s.iterate(1 + done, min(chunk, steps - done))
+            done += h.size
+            last = float(h[-1])
+            f = s.forces()
+            print(f"[bench]   {form}: step {done}, density residual {last:.3e}, CL {f['CL']:.6f} CD {f['CD']:.6f}",
+                  file=sys.stderr, flush=True)
+        f = s.forces()
+        out[f"cl_{form}"], out[f"cd_{form}"], out[f"rms_{form}"] = f["CL"], f["CD"], last
+        q = s.get_state()[:, :mesh.n_cell]
+        if pover.get("viscous") == 3:
+            fields[form + "_nut"] = s.get_field("nut")[:mesh.n_cell]
+        fields[form] = q
+        s.close()
+    out["dcl"] = abs(out["cl_fp64"] - out["cl_mixed"])
+    out["dcd"] = abs(out["cd_fp64"] - out["cd_mixed"])
+    a, b = fields["fp64"], fields["mixed"]
+    dev = a - np.mean(a, axis=1, keepdims=True)
+    out["field_l2_rel"] = [float(np.linalg.norm(a[m] - b[m]) / max(np.linalg.norm(dev[m]), 1e-300)) for m in range(5)]
+    if "fp64_nut" in fields:
+        na, nb = fields["fp64_nut"], fields["mixed_nut"]
+        out["nut_l2_rel"] = float(np.linalg.norm(na - nb) / max(np.linalg.norm(na - np.mean(na)), 1e-300))
+    print(f"[bench] precision comparison: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def batch_solve_logged(s, max_steps, drop, stall, label, chunk=256):
     """phl_solve_batch's loop (per-case drop, freezing, precision switch)
     driven from here so that each chunk of iterations reports its progress."""
@@ -264,6 +304,8 @@ def main():
                          "eagerly with the turbulence sweeps on a second stream")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="override the configuration's precision")
+    ap.add_argument("--precision-compare", type=int, default=None, metavar="STEPS",
+                    help="also run STEPS iterations in FP64 and in the mixed form and compare CL/CD and fields")
     ap.add_argument("--storage-order", default=None, choices=["locality", "mesh"],
                     help="device storage order of the cells (default: the library's, locality)")
     ap.add_argument("--converge", type=float, default=None, metavar="DROP",
@@ -328,7 +370,7 @@ def main():
     traffic = MEASURED_FLUX_TRAFFIC.get(args.config) if not mixed else None
     roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="k_flux", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb)
+                kernel="k_face_flux + k_cell_gather", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb)
     per_class = {k: round(v / max(n_timed, 1), 5) for k, v in times.items()}
     print(f"[bench] mean ms per step by kernel class: {per_class}", file=sys.stderr, flush=True)
     if args.copy_probe:
@@ -347,6 +389,9 @@ def main():
         "roofline": roof,
         "kernel_ms_per_step": per_class,
     }
+    if args.precision_compare is not None and not cfg.get("batch"):
+        solver.close()
+        line["precision_compare"] = precision_compare(mesh, pover, args.precision_compare)
     if args.converge is not None:
         solver.close()
         if cfg.get("batch"):

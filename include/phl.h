@@ -120,10 +120,12 @@ phl_status phl_mesh_bump_hybrid(int nx, int nz, int n_prism, double h1, int n_te
  * taper 0.5, aspect ratio 4, quarter-chord sweep 25 deg, symmetric cubic
  * section (t/c 0.12, half-thickness 0.405 xi (1-xi)^2, closed over the last
  * 5% of the span), farfield 20 root chords away.  Lattice: n_chord cells on
- * the chord (cosine), 0.3 n_chord upstream, 0.6 n_chord downstream; n_span
- * on the span (sine, clustered at the tip), 0.4 n_span beyond it; on each
+ * the chord, 0.3 n_chord upstream, 0.6 n_chord downstream; n_span on the
+ * span, 0.4 n_span beyond the tip; on each
  * side of the wing plane n_prism prism layers (first height h1, growth 1.2)
- * and n_tet rows of tetrahedra.  Cells: (1.9 n_chord)(1.4 n_span) x
+ * and n_tet rows of tetrahedra.  The chord and span stations are half
+ * clustered (cosine at both chord edges, sine at the tip), half uniform: the
+ * H-type lattice lines run to the farfield.  Cells: (1.9 n_chord)(1.4 n_span) x
  * 2 (2 n_prism + 6 n_tet); (80, 40, 30, 1e-5, 29) gives 3.98M.  Wall
  * SOLID_SURFACE, root plane SYMMETRY, outer boundary FARFIELD. */
 phl_status phl_mesh_swept_wing(int n_chord, int n_span, int n_prism, double h1, int n_tet, phl_mesh **out);

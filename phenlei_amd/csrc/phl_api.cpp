@@ -56,6 +56,7 @@ struct phl_ctx {
     // phl_solve / phl_solve_batch: switch a mixed context to FP64 once no
     // unconverged case has halved its residual for this many steps (0: never)
     int precisionSwitchStall = 0;
+    bool coefF32Ok = false;  // the sweep-ordered kernels (at most 6 faces a side) can read FP32 coefficients
 };
 
 namespace {
@@ -808,6 +809,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         TRY(upload(c, &lv.so.row, row));
         TRY(upload(c, &lv.so.nb, nbs));
     }
+    c->coefF32Ok = c->fwd.so.Wpad <= 6 && c->bwd.so.Wpad <= 6;
 
     // constants: the scalars of every case, the first one also in k
     phl::Consts &k = c->k;
@@ -860,6 +862,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.entropyFix = p.entropy_fix;
     k.nSweeps = p.n_lusgs_sweeps;
     k.mixed = p.precision;
+    k.coefF32 = (k.mixed && c->coefF32Ok) ? 1 : 0;
     k.saNuInf = p.sa_nu_inf;
     k.saVistInf = p.sa_vist_inf;
     k.turbCflScale = p.turb_cfl_scale;
@@ -1486,6 +1489,7 @@ phl_status phl_set_precision(phl_ctx *c, int precision)
     HIPTRY(hipSetDevice(c->device));
     HIPTRY(hipStreamSynchronize(c->stream));
     c->k.mixed = precision;
+    c->k.coefF32 = (precision && c->coefF32Ok) ? 1 : 0;
     c->p.precision = precision;
     for (hipGraphExec_t &g : c->stepGraph) {  // the graphs hold the other kernels
         if (g) hipGraphExecDestroy(g);

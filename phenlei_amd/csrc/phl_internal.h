@@ -77,6 +77,7 @@ struct Consts {
     int mixed;              // 1: FP32 gradients and face fluxes, FP64 accumulation
     int nCases;             // > 1: batched context, per-case scalars in cases[caseOf[cell]]
     int fluxTile, gradTile; // cells per workgroup of the flux and gradient kernels (64, 128, 256)
+    int coefF32;            // mixed form: LU-SGS face coefficients stored in FP32 (sweep arithmetic stays FP64)
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     const CaseConsts *cases;
 };

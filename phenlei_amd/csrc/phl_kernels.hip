@@ -435,9 +435,18 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
                                           : 0.0;
             double co[kCoef];
             sweep_coef(K, qc, qn, fr, isL, dist, vist[c], vist[nb], co);
-            double2 *dst = reinterpret_cast<double2 *>(coef + ((size_t)f * 2 + (isL ? 0 : 1)) * kCoef);
+            const size_t row = (size_t)f * 2 + (isL ? 0 : 1);
+            if (K.coefF32) {
+                float4 *dst = reinterpret_cast<float4 *>(reinterpret_cast<float *>(coef) + row * kCoef);
 #pragma unroll
-            for (int j = 0; j < kCoef / 2; ++j) dst[j] = make_double2(co[2 * j], co[2 * j + 1]);
+                for (int j = 0; j < kCoef / 4; ++j)
+                    dst[j] = make_float4(float(co[4 * j]), float(co[4 * j + 1]), float(co[4 * j + 2]),
+                                         float(co[4 * j + 3]));
+            } else {
+                double2 *dst = reinterpret_cast<double2 *>(coef + row * kCoef);
+#pragma unroll
+                for (int j = 0; j < kCoef / 2; ++j) dst[j] = make_double2(co[2 * j], co[2 * j + 1]);
+            }
         }
     }
     const double ovol = 1.0 / M.vol[c];
@@ -1356,18 +1365,28 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_block(DevMesh M, Consts K
 // lists, one for coefficients and neighbour dq, then the store.
 // The whole 128-byte block is loaded so that no part of it is fetched later
 // inside a branch.
+template <typename CT>
 __device__ __forceinline__ void load_coef_row(const double *__restrict__ coef, int row, double *co)
 {
-    const double2 *src = reinterpret_cast<const double2 *>(coef + (size_t)row * kCoef);
+    if constexpr (sizeof(CT) == 4) {  // FP32 rows (mixed form), 64 bytes
+        const float4 *src = reinterpret_cast<const float4 *>(reinterpret_cast<const float *>(coef) + (size_t)row * kCoef);
 #pragma unroll
-    for (int j = 0; j < 8; ++j) {
-        const double2 v = src[j];
-        co[2 * j] = v.x;
-        co[2 * j + 1] = v.y;
+        for (int j = 0; j < 4; ++j) {
+            const float4 v = src[j];
+            co[4 * j] = v.x; co[4 * j + 1] = v.y; co[4 * j + 2] = v.z; co[4 * j + 3] = v.w;
+        }
+    } else {
+        const double2 *src = reinterpret_cast<const double2 *>(coef + (size_t)row * kCoef);
+#pragma unroll
+        for (int j = 0; j < 8; ++j) {
+            const double2 v = src[j];
+            co[2 * j] = v.x;
+            co[2 * j + 1] = v.y;
+        }
     }
 }
 
-template <int NW, int VIS>
+template <int NW, int VIS, typename CT>
 __device__ __forceinline__ void sweep_position(const DevMesh &M, const Consts &K, const SweepOrder &so, int p,
                                                bool useU, const double *__restrict__ coef,
                                                const double *__restrict__ res, const double *__restrict__ diag,
@@ -1394,7 +1413,7 @@ __device__ __forceinline__ void sweep_position(const DevMesh &M, const Consts &K
 #pragma unroll
     for (int s = 0; s < NW; ++s) {
         const bool act = row[s] >= 0;
-        load_coef_row(coef, act ? row[s] : 0, co[s]);
+        load_coef_row<CT>(coef, act ? row[s] : 0, co[s]);
         load5(dq, nT, act ? nb[s] : 0, dqn[s]);
     }
     // Unused slots add +0.0, which leaves the sum unchanged (it starts at
@@ -1415,7 +1434,7 @@ __device__ __forceinline__ void sweep_position(const DevMesh &M, const Consts &K
     }
 }
 
-template <int NW, int VIS>
+template <int NW, int VIS, typename CT>
 __global__ void __launch_bounds__(64) k_sweep_level_so(DevMesh M, Consts K, SweepOrder so, int begin, int end,
                                                        bool useU, const double *__restrict__ coef,
                                                        const double *__restrict__ res,
@@ -1423,10 +1442,10 @@ __global__ void __launch_bounds__(64) k_sweep_level_so(DevMesh M, Consts K, Swee
 {
     const int p = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= end) return;
-    sweep_position<NW, VIS>(M, K, so, p, useU, coef, res, diag, dq, dflux);
+    sweep_position<NW, VIS, CT>(M, K, so, p, useU, coef, res, diag, dq, dflux);
 }
 
-template <int NW, int VIS>
+template <int NW, int VIS, typename CT>
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_block_so(DevMesh M, Consts K, SweepOrder so,
                                                                const int *__restrict__ ptr, int l0, int l1,
                                                                bool useU, const double *__restrict__ coef,
@@ -1437,7 +1456,7 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_block_so(DevMesh M, Const
     for (int L = l0; L < l1; ++L) {
         const int b = ptr[L], e = ptr[L + 1];
         for (int p = b + (int)threadIdx.x; p < e; p += blockDim.x)
-            sweep_position<NW, VIS>(M, K, so, p, useU, coef, res, diag, dq, dflux);
+            sweep_position<NW, VIS, CT>(M, K, so, p, useU, coef, res, diag, dq, dflux);
         __syncthreads();
     }
 }
@@ -1802,7 +1821,7 @@ void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
 // (barriers are cheap); larger levels are spread over the GPU.
 static const int kSmallLevel = kSweepBlock;
 
-template <int NW, int VIS>
+template <int NW, int VIS, typename CT>
 static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool useU,
                             hipStream_t s)
 {
@@ -1812,12 +1831,12 @@ static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const
         if (sz <= kSmallLevel) {
             int L1 = L + 1;
             while (L1 < lv.nLevels && lv.hptr[L1 + 1] - lv.hptr[L1] <= kSmallLevel) ++L1;
-            hipLaunchKernelGGL((k_sweep_block_so<NW, VIS>), dim3(1), dim3(kSweepBlock), 0, s, m, k, lv.so, lv.ptr, L, L1,
-                               useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+            hipLaunchKernelGGL((k_sweep_block_so<NW, VIS, CT>), dim3(1), dim3(kSweepBlock), 0, s, m, k, lv.so, lv.ptr, L,
+                               L1, useU, f.coef, f.res, f.diag, f.dq, f.dflux);
             L = L1;
         } else {
-            hipLaunchKernelGGL((k_sweep_level_so<NW, VIS>), dim3(nblk(sz, 64)), dim3(64), 0, s, m, k, lv.so, lv.hptr[L],
-                               lv.hptr[L + 1], useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+            hipLaunchKernelGGL((k_sweep_level_so<NW, VIS, CT>), dim3(nblk(sz, 64)), dim3(64), 0, s, m, k, lv.so,
+                               lv.hptr[L], lv.hptr[L + 1], useU, f.coef, f.res, f.diag, f.dq, f.dflux);
             ++L;
         }
     }
@@ -1828,9 +1847,14 @@ void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &l
 {
     const bool useU = !(forward && firstSweep);
     const bool v = k.viscous != 0;
-    if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1>(m, k, f, lv, useU, s) : launch_sweep_so<3, 0>(m, k, f, lv, useU, s);
-    if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1>(m, k, f, lv, useU, s) : launch_sweep_so<4, 0>(m, k, f, lv, useU, s);
-    if (lv.so.Wpad == 6) return v ? launch_sweep_so<6, 1>(m, k, f, lv, useU, s) : launch_sweep_so<6, 0>(m, k, f, lv, useU, s);
+    if (k.coefF32) {  // only set when both directions have at most 6 slots
+        if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1, float>(m, k, f, lv, useU, s) : launch_sweep_so<3, 0, float>(m, k, f, lv, useU, s);
+        if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1, float>(m, k, f, lv, useU, s) : launch_sweep_so<4, 0, float>(m, k, f, lv, useU, s);
+        return v ? launch_sweep_so<6, 1, float>(m, k, f, lv, useU, s) : launch_sweep_so<6, 0, float>(m, k, f, lv, useU, s);
+    }
+    if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1, double>(m, k, f, lv, useU, s) : launch_sweep_so<3, 0, double>(m, k, f, lv, useU, s);
+    if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1, double>(m, k, f, lv, useU, s) : launch_sweep_so<4, 0, double>(m, k, f, lv, useU, s);
+    if (lv.so.Wpad == 6) return v ? launch_sweep_so<6, 1, double>(m, k, f, lv, useU, s) : launch_sweep_so<6, 0, double>(m, k, f, lv, useU, s);
     // wider polyhedra: face-by-face through the cell -> face lists
     int L = 0;
     while (L < lv.nLevels) {

@@ -590,9 +590,13 @@ bool makeSweptWing(int nChord, int nSpan, int nPrism, double h1, int nTet, Mesh 
     const int nTip = std::max(2, (4 * nSpan + 5) / 10);
     const int NI = nUp + nChord + nDown, NK = nSpan + nTip, NJ = 2 * (nPrism + nTet), jw = nPrism + nTet;
 
-    // spanwise stations: sine clustering towards the tip, then geometric
+    // Spanwise stations: half sine clustering towards the tip, half uniform,
+    // then geometric.  The lattice lines run to the farfield (an H-type
+    // grid), so a strong clustering at the wing's edges would leave slivers of
+    // extreme aspect ratio in the tetrahedral rows far from the wing.
     std::vector<double> zs(NK + 1);
-    for (int k = 0; k <= nSpan; ++k) zs[k] = g.b * std::sin(0.5 * kPi * k / nSpan);
+    for (int k = 0; k <= nSpan; ++k)
+        zs[k] = g.b * (0.5 * std::sin(0.5 * kPi * k / nSpan) + 0.5 * (double)k / nSpan);
     {
         const double s1 = zs[nSpan] - zs[nSpan - 1];
         const double r = growthFor(s1, nTip, g.far - g.b);
@@ -619,7 +623,9 @@ bool makeSweptWing(int nChord, int nSpan, int nPrism, double h1, int nTet, Mesh 
     // chordwise stations per span station: upstream geometric, cosine on the
     // chord, downstream geometric; xi[i] is the chord fraction (-1 off the wing)
     std::vector<double> xs((size_t)(NI + 1) * (NK + 1)), xiOf(NI + 1, -1.0);
-    for (int i = 0; i <= nChord; ++i) xiOf[nUp + i] = 0.5 * (1.0 - std::cos(kPi * i / nChord));
+    // chord fractions: half cosine clustering at both edges, half uniform
+    for (int i = 0; i <= nChord; ++i)
+        xiOf[nUp + i] = 0.25 * (1.0 - std::cos(kPi * i / nChord)) + 0.5 * (double)i / nChord;
     for (int k = 0; k <= NK; ++k) {
         const double c = g.chord(zs[k]), xle = g.xLE(zs[k]);
         double *row = &xs[(size_t)k * (NI + 1)];
