@@ -790,6 +790,16 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     std::vector<int> cellsF, cellsB;
     build_levels(m, c2fPtr, c2n, true, c->hfwdPtr, cellsF);
     build_levels(m, c2fPtr, c2n, false, c->hbwdPtr, cellsB);
+    // Within a level, cells in storage order: the level kernels hand the
+    // k-th eighth of a level to the k-th XCD, which then also holds (in its
+    // L2) the same region's lower neighbours from the previous levels and the
+    // residuals and diagonals its gather and diagonal kernels wrote.
+    for (int dir = 0; dir < 2; ++dir) {
+        std::vector<int> &cl = dir == 0 ? cellsF : cellsB;
+        const std::vector<int> &ptr = dir == 0 ? c->hfwdPtr : c->hbwdPtr;
+        for (size_t L = 0; L + 1 < ptr.size(); ++L)
+            std::sort(cl.begin() + ptr[L], cl.begin() + ptr[L + 1], [&](int a, int b) { return slotOf[a] < slotOf[b]; });
+    }
     std::vector<int> cellsFs(nC), cellsBs(nC);
     for (int i = 0; i < nC; ++i) { cellsFs[i] = slotOf[cellsF[i]]; cellsBs[i] = slotOf[cellsB[i]]; }
     const int *tmp;

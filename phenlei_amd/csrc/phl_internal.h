@@ -50,6 +50,17 @@ struct DevMesh {
     const int *cellAt;             // [nCell] storage position of each mesh cell
 };
 
+// Logical block of hardware block b of nblk: workgroups are dispatched to
+// the 8 XCDs round-robin (b % 8), so this gives XCD x the contiguous logical
+// range of the x-th eighth; kernels that use it on storage-ordered cells keep
+// a region of the mesh in one XCD's L2 from kernel to kernel.
+__device__ __forceinline__ int xcd_block(int b, int nblk)
+{
+    const int q = nblk >> 3, r = nblk & 7;
+    const int x = b & 7, i = b >> 3;
+    return x * q + (x < r ? x : r) + i;
+}
+
 // Mesh index of a stored cell (ghost indices are their own: above every real cell).
 __device__ __forceinline__ int mesh_index(const DevMesh &M, int i) { return i < M.nCell ? M.logicalOf[i] : i; }
 

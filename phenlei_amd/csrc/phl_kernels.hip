@@ -57,15 +57,6 @@ __device__ __forceinline__ Consts case_k(const Consts &K, const DevMesh &M, int 
     return k;
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs (block b lands on XCD
-// b % 8).  Give each XCD a contiguous range of cells so that neighbouring
-// cells share an L2.
-__device__ __forceinline__ int xcd_block(int b, int nblk)
-{
-    const int q = nblk >> 3, r = nblk & 7;
-    const int x = b & 7, i = b >> 3;
-    return x * q + (x < r ? x : r) + i;
-}
 
 template <typename T>
 __device__ __forceinline__ T harten(T eig, T lim)
@@ -1208,7 +1199,7 @@ __global__ void __launch_bounds__(kBlock) k_dt_diag(DevMesh M, Consts K, const d
                                                     const double *__restrict__ dtMin,
                                                     double *__restrict__ dt, double *__restrict__ diag)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= M.nCell) return;
     const double mn = dtMin[K.nCases > 1 ? M.caseOf[c] : 0];
     double d = dtLoc[c];
@@ -1337,7 +1328,7 @@ __global__ void __launch_bounds__(64) k_sweep_level(DevMesh M, Consts K, const i
                                                     const double *__restrict__ res, const double *__restrict__ diag,
                                                     double *dq, double *dflux)
 {
-    const int i = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = begin + xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (i >= end) return;
     sweep_cell<FWD>(M, K, cells[i], useU, coef, res, diag, dq, dflux);
 }
@@ -1440,7 +1431,7 @@ __global__ void __launch_bounds__(64) k_sweep_level_so(DevMesh M, Consts K, Swee
                                                        const double *__restrict__ res,
                                                        const double *__restrict__ diag, double *dq, double *dflux)
 {
-    const int p = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = begin + xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= end) return;
     sweep_position<NW, VIS, CT>(M, K, so, p, useU, coef, res, diag, dq, dflux);
 }

@@ -501,7 +501,7 @@ __global__ void __launch_bounds__(64) k_sa_sweep_level(DevMesh M, SweepOrder so,
                                                       const double *__restrict__ resT,
                                                       const double *__restrict__ specT, double *dq, double *dflux)
 {
-    const int p = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    const int p = begin + xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= end) return;
     sa_sweep_position<NW>(M, so, p, useU, mat, resT, specT, dq, dflux);
 }
