@@ -304,6 +304,9 @@ def main():
                          "eagerly with the turbulence sweeps on a second stream")
     ap.add_argument("--precision", default=None, choices=["fp64", "mixed"],
                     help="override the configuration's precision")
+    ap.add_argument("--warm-start", action="store_true",
+                    help="also run the learned warm start end to end on bump-channel cases (32 training cases, "
+                         "4 held out) and time mixed + warm start against plain FP64 to a 1e-8 drop")
     ap.add_argument("--precision-compare", type=int, default=None, metavar="STEPS",
                     help="also run STEPS iterations in FP64 and in the mixed form and compare CL/CD and fields")
     ap.add_argument("--storage-order", default=None, choices=["locality", "mesh"],
@@ -389,6 +392,9 @@ def main():
         "roofline": roof,
         "kernel_ms_per_step": per_class,
     }
+    if args.warm_start:
+        from phenlei_amd import warmstart as ws
+        line["warm_start"] = ws.compare(log=lambda m: print("[bench] warm start: " + m, file=sys.stderr, flush=True))
     if args.precision_compare is not None and not cfg.get("batch"):
         solver.close()
         line["precision_compare"] = precision_compare(mesh, pover, args.precision_compare)

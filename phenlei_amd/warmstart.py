@@ -93,7 +93,9 @@ def solve_cases(cases, dims, drop, max_steps, cfl=10.0, device=0):
 
 
 def train_network(meshes, cases, fields, steps, lr=1e-3, layers=6, seed=0, device=0):
-    """Train the device network on the union graph of the training cases."""
+    """Train the device network on the union graph of the training cases.
+    steps/lr: one Adam run, or lists of equal length (a step-wise learning-rate
+    schedule; the optimiser state carries over)."""
     union = Mesh.concat(meshes)
     X = np.concatenate([features(m, c["bump"], c["mach"], c["alpha"]) for m, c in zip(meshes, cases)])
     T = np.concatenate([targets(f, freestream(c["mach"], c["alpha"]), m.n_cell)
@@ -101,7 +103,9 @@ def train_network(meshes, cases, fields, steps, lr=1e-3, layers=6, seed=0, devic
     net = Mpnn(layers=layers, seed=seed, device=device)
     net.set_graph(mesh=union)
     net.set_data(X, T)
-    hist = net.train(steps, lr)
+    steps = list(np.atleast_1d(steps))
+    lrs = list(np.atleast_1d(lr)) if np.ndim(lr) else [lr] * len(steps)
+    hist = np.concatenate([net.train(int(n), float(r)) for n, r in zip(steps, lrs)])
     return net, hist
 
 
@@ -126,3 +130,78 @@ def steps_to(solver, target, max_steps, chunk=64):
             return done + int(hit[0]) + 1, r1
         done += h.size
     return -done, r1
+
+
+def time_to_target(solver, target, max_steps, switch_stall=0, chunk=64):
+    """Iterations and wall seconds until the density-residual RMS is at or
+    below `target`; with switch_stall > 0 a mixed-precision solver switches
+    to FP64 once its residual has not halved for that many steps."""
+    import time
+    solver.synchronize()
+    t0 = time.perf_counter()
+    done, best, mark, since = 0, np.inf, np.inf, 0
+    while done < max_steps:
+        h = solver.iterate(1 + done, min(chunk, max_steps - done))
+        hit = np.nonzero(h <= target)[0]
+        if hit.size:
+            solver.synchronize()
+            return done + int(hit[0]) + 1, time.perf_counter() - t0
+        for i, v in enumerate(h):
+            best = min(best, v)
+            if best < 0.5 * mark:
+                mark, since = best, done + i
+        done += h.size
+        if switch_stall and solver.precision() == "mixed" and done - since >= switch_stall:
+            solver.set_precision("fp64")
+    solver.synchronize()
+    return -done, time.perf_counter() - t0
+
+
+def compare(n_train=32, n_held=4, dims=(24, 12, 4), drop=1e-8, max_steps=12000,
+            train_steps=(1500, 1500, 1500, 1500), train_lr=(2e-3, 1e-3, 3e-4, 1e-4), log=None):
+    """The learned warm start end to end on bump-channel cases: converged
+    fields of n_train seeded cases (one batched solve), the network trained
+    on them, then for n_held held-out cases the wall time to drop * R_1 (R_1:
+    the first residual from freestream) of (a) the plain FP64 form from
+    freestream and (b) the mixed form started from the prediction, switched
+    to FP64 when it stalls.  Returns a dict of per-case and total figures."""
+    import time
+    say = log or (lambda *a: None)
+    t0 = time.perf_counter()
+    train = sample_cases(n_train, seed=11)
+    meshes, fields, steps = solve_cases(train, dims, drop=drop, max_steps=max_steps)
+    t_data = time.perf_counter() - t0
+    say(f"training data: {n_train} cases of {meshes[0].n_cell} cells, median {int(np.median(np.abs(steps)))} "
+        f"iterations, {t_data:.1f} s")
+    t0 = time.perf_counter()
+    net, hist = train_network(meshes, train, fields, list(train_steps), list(train_lr))
+    t_train = time.perf_counter() - t0
+    say(f"training: loss {hist[0]:.3e} -> {hist[-1]:.3e} in {len(hist)} Adam steps, {t_train:.1f} s")
+    out = dict(n_train=n_train, cells_per_case=meshes[0].n_cell, drop=drop, data_seconds=round(t_data, 2),
+               train_seconds=round(t_train, 2), final_loss=float(hist[-1]), cases=[])
+    tot_a = tot_b = 0.0
+    for case in sample_cases(n_held, seed=99):
+        mesh = Mesh.bump_hex(*dims, case["bump"])
+        base = dict(mach=case["mach"], alpha_deg=case["alpha"], cfl_start=10.0, cfl_end=10.0)
+        a = Solver(mesh, default_params(**base))
+        a.init_freestream()
+        r1 = float(a.iterate(1, 1)[0])
+        a.init_freestream()
+        na, ta = time_to_target(a, drop * r1, max_steps)
+        b = Solver(mesh, default_params(precision=1, **base))
+        t1 = time.perf_counter()
+        b.set_state(predict_state(net, mesh, case))
+        t_pred = time.perf_counter() - t1
+        nb, tb = time_to_target(b, drop * r1, max_steps, switch_stall=512)
+        tb += t_pred
+        a.close()
+        b.close()
+        rec = dict(case, fp64_iterations=na, fp64_seconds=round(ta, 3), warm_mixed_iterations=nb,
+                   warm_mixed_seconds=round(tb, 3), predict_seconds=round(t_pred, 3))
+        say(f"held-out {rec}")
+        out["cases"].append(rec)
+        if na > 0 and nb > 0:
+            tot_a += ta
+            tot_b += tb
+    out["speedup"] = round(tot_a / tot_b, 3) if tot_b > 0 else None
+    return out
