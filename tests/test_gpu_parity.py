@@ -344,3 +344,33 @@ def test_storage_order_does_not_change_results(viscous):
     if viscous == 3:
         assert np.array_equal(a.get_field("nut"), b.get_field("nut"))
         assert np.array_equal(a.get_field("wdist"), b.get_field("wdist"))
+
+
+def test_solve_switches_a_stalled_mixed_context_to_fp64():
+    """phl_set_precision_switch: the mixed form stalls near its FP32 floor;
+    phl_solve then switches the context to FP64, which carries the residual
+    below that floor."""
+    m = pa.Mesh.bump_hex(24, 12, 4, 0.05)
+    s = pa.Solver(m, pa.default_params(precision=1, cfl_start=10.0, cfl_end=10.0))
+    s.set_precision_switch(256)
+    s.init_freestream()
+    n, h = s.solve(6000, 1e-8)
+    print("steps", n, "final drop", h[-1] / h[0], "precision", s.precision())
+    assert s.precision() == "fp64"
+    assert h[-1] / h[0] <= 1e-7
+
+
+def test_frozen_case_keeps_its_state():
+    a, b = pa.Mesh.bump_hex(8, 4, 2, 0.05), pa.Mesh.bump_hex(6, 4, 2, 0.03)
+    u = pa.Mesh.concat([a, b])
+    s = pa.Solver(u, [pa.default_params(mach=0.5), pa.default_params(mach=0.6)])
+    s.init_freestream()
+    s.iterate(1, 5)
+    q0 = s.get_state()
+    s.set_frozen([True, False])
+    s.iterate(6, 5)
+    q1 = s.get_state()
+    cs0, gs0 = u.case_slices(0)
+    cs1, _ = u.case_slices(1)
+    assert np.array_equal(q0[:, cs0], q1[:, cs0])
+    assert not np.array_equal(q0[:, cs1], q1[:, cs1])
