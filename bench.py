@@ -124,6 +124,16 @@ def flux_algorithmic_bytes(n_cell, n_face, viscous, mixed=False):
     return b
 
 
+def sweep_algorithmic_bytes(n_cell, n_face, mixed=False):
+    """Compulsory HBM bytes of one LU-SGS sweep pair: every face side is used
+    once over the two directions (its coefficient row, 128 B, or 64 B in the
+    mixed form, and the neighbour's dq, 40 B); per cell, forward reads res and
+    the diagonal and writes dq and the off-diagonal sum (128 B), backward also
+    reads that sum (168 B)."""
+    row = 64 if mixed else 128
+    return 296 * n_cell + 2 * n_face * (row + 40)
+
+
 def cpu_baseline(mesh, params, steps, wall_distance=None):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # the checker, timed here as the CPU baseline only
@@ -309,6 +319,8 @@ def main():
                          "4 held out) and time mixed + warm start against plain FP64 to a 1e-8 drop")
     ap.add_argument("--precision-compare", type=int, default=None, metavar="STEPS",
                     help="also run STEPS iterations in FP64 and in the mixed form and compare CL/CD and fields")
+    ap.add_argument("--face-loop", default=None, choices=["tile", "split"],
+                    help="face-loop form (default: the library's, split)")
     ap.add_argument("--storage-order", default=None, choices=["locality", "mesh"],
                     help="device storage order of the cells (default: the library's, locality)")
     ap.add_argument("--converge", type=float, default=None, metavar="DROP",
@@ -324,11 +336,14 @@ def main():
     cfg = CONFIGS[args.config]
     print(f"[bench] config {args.config}: building mesh", file=sys.stderr, flush=True)
     so = {"locality": 0, "mesh": 1}.get(args.storage_order)
+    fl = {"split": 0, "tile": 1}.get(args.face_loop)
     if cfg.get("batch"):
         parts, params_list = build_batch(cfg)
-        if so is not None:
-            for p in params_list:
+        for p in params_list:
+            if so is not None:
                 p.storage_order = so
+            if fl is not None:
+                p.face_loop = fl
         mesh = pa.Mesh.concat(parts)
         params = params_list[0]
         solver = pa.Solver(mesh, params_list, device=0)
@@ -340,6 +355,8 @@ def main():
             pover["precision"] = 1 if args.precision == "mixed" else 0
         if so is not None:
             pover["storage_order"] = so
+        if fl is not None:
+            pover["face_loop"] = fl
         params = pa.default_params(**pover)
         print(f"[bench] {mesh.n_cell} cells: creating the context", file=sys.stderr, flush=True)
         solver = pa.Solver(mesh, params, device=0)
@@ -375,6 +392,13 @@ def main():
                 frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
                 kernel="k_face_flux + k_cell_gather", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb)
     per_class = {k: round(v / max(n_timed, 1), 5) for k, v in times.items()}
+    sweep_ms = (times["sweep_fwd"] + times["sweep_bwd"]) / max(n_timed, 1)
+    sb = sweep_algorithmic_bytes(mesh.n_cell, mesh.n_face, mixed)
+    sweep_gbs = sb / (sweep_ms * 1e-3) / 1e9
+    roof_sweep = dict(bound="hbm", achieved=round(sweep_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                      frac=round(sweep_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="k_sweep_level_so + k_sweep_block_so",
+                      kernel_ms=round(sweep_ms, 5), algorithmic_bytes=sb,
+                      note="latency bound: each dependency level is a launch with two dependent memory rounds")
     print(f"[bench] mean ms per step by kernel class: {per_class}", file=sys.stderr, flush=True)
     if args.copy_probe:
         roof["copy_probe_gbs"] = round(pa.hbm_copy_gbps(0, 2 * 1024**3, 10), 1)
@@ -390,6 +414,7 @@ def main():
                    "parallelism": "single GPU", "launch": args.launch,
                    "tiles": None if tuned is None else {"flux": tuned["flux_tile"], "grad": tuned["grad_tile"]}},
         "roofline": roof,
+        "roofline_sweep": roof_sweep,
         "kernel_ms_per_step": per_class,
     }
     if args.warm_start:

@@ -69,6 +69,11 @@ typedef struct phl_params {
                              * Morton curve of their centroids, per case; default), 1 the mesh's order.
                              * Results do not depend on it: every sweep and accumulation follows the
                              * mesh's cell and face numbering either way. */
+    int face_loop;          /* 0 (default): face kernel + cell gather, face values through HBM; 1:
+                             * tile-fused (a workgroup evaluates the faces of its cells into LDS and sums
+                             * them; faces between tiles are evaluated twice).  Identical results; the
+                             * tile form measured slower on the hybrid meshes (C2 0.39 vs 0.32 ms). */
+    int reserved;
 } phl_params;
 
 /* Field identifiers for phl_get_field (same numbering as the oracle). */

@@ -753,6 +753,49 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
 
     phl::DevMesh &dm = c->m;
     dm.nCell = nC; dm.nBFace = nB; dm.nFace = nF; dm.nTotal = nT; dm.ellW = W;
+    // Tile-fused face loop (face_loop 1): the faces touching each tile of
+    // kTileCells storage-ordered cells, and each cell's faces as slots of that
+    // list.  Off when a tile's face values would not fit 64 KB of LDS.
+    dm.tileStride = 0;
+    dm.tileFacePtr = nullptr; dm.tileFaces = nullptr; dm.c2loc = nullptr;
+    if (p.face_loop == 1) {
+        const int TB = phl::kTileCells, nTiles = (nC + TB - 1) / TB;
+        std::vector<int> tptr(nTiles + 1, 0), tfaces;
+        std::vector<unsigned short> c2loc((size_t)W * nC, 0);
+        std::vector<int> list;
+        int maxF = 0;
+        for (int t = 0; t < nTiles; ++t) {
+            list.clear();
+            const int c0 = t * TB, c1 = std::min(nC, c0 + TB);
+            for (int i = c0; i < c1; ++i)
+                for (int sl = 0; sl < W; ++sl) {
+                    const int enc = ellEnc[(size_t)sl * nC + i];
+                    if (enc == 0) break;
+                    list.push_back((enc > 0 ? enc : -enc) - 1);
+                }
+            std::sort(list.begin(), list.end());
+            list.erase(std::unique(list.begin(), list.end()), list.end());
+            maxF = std::max(maxF, (int)list.size());
+            for (int i = c0; i < c1; ++i)
+                for (int sl = 0; sl < W; ++sl) {
+                    const int enc = ellEnc[(size_t)sl * nC + i];
+                    if (enc == 0) break;
+                    const int f = (enc > 0 ? enc : -enc) - 1;
+                    c2loc[(size_t)sl * nC + i] =
+                        (unsigned short)(std::lower_bound(list.begin(), list.end(), f) - list.begin());
+                }
+            tfaces.insert(tfaces.end(), list.begin(), list.end());
+            tptr[t + 1] = (int)tfaces.size();
+        }
+        const size_t ldsBytes = (size_t)maxF * 6 * sizeof(double) * (p.viscous ? 2 : 1);
+        dm.tileStride = (p.face_loop == 1 && maxF < 65536 && ldsBytes <= 65536) ? maxF : 0;
+        dm.tileFacePtr = nullptr; dm.tileFaces = nullptr; dm.c2loc = nullptr;
+        if (dm.tileStride > 0) {
+            TRY(upload(c, &dm.tileFacePtr, tptr));
+            TRY(upload(c, &dm.tileFaces, tfaces));
+            TRY(upload(c, &dm.c2loc, c2loc));
+        }
+    }
     TRY(upload(c, &dm.face, faces));
     TRY(upload(c, &dm.bcType, m.bcType));
     {

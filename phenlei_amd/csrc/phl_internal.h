@@ -48,7 +48,14 @@ struct DevMesh {
     // the mesh's own numbering through these maps (ghost cells are not moved).
     const int *logicalOf;          // [nCell] mesh index of the cell stored at each position
     const int *cellAt;             // [nCell] storage position of each mesh cell
+    // Tile-fused face loop (kTileCells cells per workgroup): the faces touching
+    // each tile, and each cell's faces as slots of its tile's list (ELL order).
+    int tileStride;                // most faces of any tile (0: the tile form is off)
+    const int *tileFacePtr;        // [nTiles+1]
+    const int *tileFaces;          // storage face ids, ascending within a tile
+    const unsigned short *c2loc;   // [W][nCell]
 };
+constexpr int kTileCells = 128;
 
 // Logical block of hardware block b of nblk: workgroups are dispatched to
 // the 8 XCDs round-robin (b % 8), so this gives XCD x the contiguous logical

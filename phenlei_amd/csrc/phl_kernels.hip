@@ -21,6 +21,8 @@
 // UpdateFlowField :14587; SolutionFix :14509.
 #include "phl_internal.h"
 
+#include <algorithm>
+
 namespace phl {
 
 namespace {
@@ -904,13 +906,22 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
 // computes in k_flux (they depend only on the face's left/right states), and
 // the gather adds them in the same order, so the residuals are identical
 // while every interior face is evaluated once instead of twice.
-template <typename T>
-__global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const GatherRec *__restrict__ rec,
-                                                   const T *__restrict__ gradT, const double *__restrict__ vist,
-                                                   double *__restrict__ faceInv, double *__restrict__ faceVis)
+__device__ __forceinline__ void store6(double *dst, const double *v)
 {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= M.nFace) return;
+    double2 *d = reinterpret_cast<double2 *>(dst);
+    d[0] = make_double2(v[0], v[1]);
+    d[1] = make_double2(v[2], v[3]);
+    d[2] = make_double2(v[4], v[5]);
+}
+
+// The stored values of face f: invOut[0..4] the area-scaled inviscid flux,
+// invOut[5] the inviscid spectral-radius term; visOut[] likewise (viscous
+// flow only).  Each group is stored as soon as it is known.
+template <typename T>
+__device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, const GatherRec *__restrict__ rec,
+                                            const T *__restrict__ gradT, const double *__restrict__ vist, int f,
+                                            double *invOut, double *visOut)
+{
     const int nC = M.nCell;
     const int le = M.fle[f], re = M.fre[f];
     const Consts K = case_k(K0, M, le);
@@ -962,22 +973,19 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const G
         const double rm = 0.5 * (L.q[0] + R.q[0]);
         const double vn = 0.5 * (vnl + vnr);
         const double cm = sqrt(g * pm / rm);
-        double out[6];
+        double inv[6];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) out[m] = double(fl[m] * area);
-        out[5] = 0.5 * fr.area * (dABS(vn) + cm);
-        double2 *dst = reinterpret_cast<double2 *>(faceInv + (size_t)f * 6);
-        dst[0] = make_double2(out[0], out[1]);
-        dst[1] = make_double2(out[2], out[3]);
-        dst[2] = make_double2(out[4], out[5]);
+        for (int m = 0; m < 5; ++m) inv[m] = double(fl[m] * area);
+        inv[5] = 0.5 * fr.area * (dABS(vn) + cm);
+        store6(invOut, inv);
     }
     if (K.viscous) {
         T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
         if (le < nC) { gTL[0] = gradT[le]; gTL[1] = gradT[(size_t)nC + le]; gTL[2] = gradT[(size_t)2 * nC + le]; }
         if (re < nC) { gTR[0] = gradT[re]; gTR[1] = gradT[(size_t)nC + re]; gTR[2] = gradT[(size_t)2 * nC + re]; }
         const bool wall = bnd && bc == kBC_WALL;
-        double out[6];
-        viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[le], vist[re], out);
+        double vis[6];
+        viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[le], vist[re], vis);
         const double tl = L.q[4] / (K.R * L.q[0]);
         const double tr = R.q[4] / (K.R * R.q[0]);
         const double vl = 0.5 * (tl * sqrt(tl) * (1.0 + K.tsuth) / (tl + K.tsuth) +
@@ -986,15 +994,22 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const G
         const double dx = R.x - L.x, dy = R.y - L.y, dz = R.z - L.z;
         const double ds = dABS(fr.nx * dx + fr.ny * dy + fr.nz * dz);
         const double rho = 0.5 * (L.q[0] + R.q[0]);
-        const double vis = vl + vt;
-        double sr = 2.0 * vis / (rho * ds * K.reynolds + kSMALL);
+        const double mu = vl + vt;
+        double sr = 2.0 * mu / (rho * ds * K.reynolds + kSMALL);
         sr *= 0.5 * fr.area;
-        out[5] = sr;
-        double2 *dst = reinterpret_cast<double2 *>(faceVis + (size_t)f * 6);
-        dst[0] = make_double2(out[0], out[1]);
-        dst[1] = make_double2(out[2], out[3]);
-        dst[2] = make_double2(out[4], out[5]);
+        vis[5] = sr;
+        store6(visOut, vis);
     }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const GatherRec *__restrict__ rec,
+                                                   const T *__restrict__ gradT, const double *__restrict__ vist,
+                                                   double *__restrict__ faceInv, double *__restrict__ faceVis)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= M.nFace) return;
+    face_values<T>(M, K0, rec, gradT, vist, f, faceInv + (size_t)f * 6, faceVis + (size_t)f * 6);
 }
 
 // Per cell: the face values summed in the cell's face order (every inviscid
@@ -1081,6 +1096,97 @@ __global__ void __launch_bounds__(BS) k_cell_gather(DevMesh M, Consts K0, const 
 #pragma unroll
         for (int m = 0; m < 5; ++m) normPart[(size_t)lb * 5 + m] = red[m][0];
         if (fusedDt) dtMinPart[lb] = red[5][0];
+    }
+}
+
+// Tile-fused face loop: workgroup t owns the cells [t*BS, (t+1)*BS) of the
+// storage order.  It evaluates every face touching them (host-built list)
+// into LDS, then each thread sums its cell's faces from LDS in the cell's
+// face order, exactly like k_cell_gather sums the stored face values.  Faces
+// between two tiles are evaluated by both (identical values); the face values
+// never travel through HBM.
+template <typename T, int BS>
+__global__ void __launch_bounds__(BS) k_flux_tile(DevMesh M, Consts K0, const GatherRec *__restrict__ rec,
+                                                  const T *__restrict__ gradT, const double *__restrict__ vist,
+                                                  const int *__restrict__ tileFacePtr, const int *__restrict__ tileFaces,
+                                                  const unsigned short *__restrict__ c2loc, int ldsStride,
+                                                  const double *__restrict__ q, double *__restrict__ res,
+                                                  double *__restrict__ specInv, double *__restrict__ specVis,
+                                                  double *__restrict__ normPart, const double *__restrict__ cflArr,
+                                                  const int *__restrict__ stepCtr, double *__restrict__ dtLoc,
+                                                  double *__restrict__ dtMinPart, bool fusedDt)
+{
+    extern __shared__ double lds[];  // [ldsStride][6] inviscid, then [ldsStride][6] viscous
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int fb = tileFacePtr[lb], nf = tileFacePtr[lb + 1] - fb;
+    double *li = lds, *lv = lds + (size_t)ldsStride * 6;
+    for (int i = threadIdx.x; i < nf; i += BS)
+        face_values<T>(M, K0, rec, gradT, vist, tileFaces[fb + i], li + (size_t)i * 6, lv + (size_t)i * 6);
+    __syncthreads();
+    const int c = lb * BS + threadIdx.x;
+    const int nC = M.nCell;
+    double acc[5] = {-0.0, -0.0, -0.0, -0.0, -0.0};
+    double dtl = 1.0e300;
+    if (c < nC) {
+        const Consts K = case_k(K0, M, c);
+        double sInv = 0.0, sVis = 0.0;
+        for (int s = 0; s < M.ellW; ++s) {
+            const int enc = M.c2fEnc[(size_t)s * nC + c];
+            if (enc == 0) break;
+            const double *v = li + (size_t)c2loc[(size_t)s * nC + c] * 6;
+            if (enc > 0) {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) acc[m] -= v[m];
+            } else {
+#pragma unroll
+                for (int m = 0; m < 5; ++m) acc[m] += v[m];
+            }
+            sInv += v[5];
+        }
+        specInv[c] = sInv;
+        if (K.viscous) {
+            for (int s = 0; s < M.ellW; ++s) {
+                const int enc = M.c2fEnc[(size_t)s * nC + c];
+                if (enc == 0) break;
+                const double *v = lv + (size_t)c2loc[(size_t)s * nC + c] * 6;
+                if (enc > 0) {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) acc[m] -= v[m];
+                } else {
+#pragma unroll
+                    for (int m = 0; m < 5; ++m) acc[m] += v[m];
+                }
+                sVis += v[5];
+            }
+            specVis[c] = sVis;
+        }
+        if (fusedDt) {
+            double spec = sInv;
+            if (K.viscous) spec += sVis;
+            dtl = local_dt(M, K, c, q[(size_t)4 * M.nTotal + c], spec, cflArr[*stepCtr]);
+            dtLoc[c] = dtl;
+        }
+#pragma unroll
+        for (int m = 0; m < 5; ++m) res[(size_t)m * nC + c] = acc[m];
+    }
+    __syncthreads();  // the reduction below reuses the start of the LDS
+    double *red = lds;  // [6][BS]
+#pragma unroll
+    for (int m = 0; m < 5; ++m) red[m * BS + threadIdx.x] = (c < nC) ? acc[m] * acc[m] : 0.0;
+    red[5 * BS + threadIdx.x] = dtl;
+    __syncthreads();
+    for (int w = BS / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) red[m * BS + threadIdx.x] += red[m * BS + threadIdx.x + w];
+            red[5 * BS + threadIdx.x] = dMIN(red[5 * BS + threadIdx.x], red[5 * BS + threadIdx.x + w]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) normPart[(size_t)lb * 5 + m] = red[m * BS];
+        if (fusedDt) dtMinPart[lb] = red[5 * BS];
     }
 }
 
@@ -1741,7 +1847,10 @@ void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
     }
 }
 
-int flux_blocks(const DevMesh &m, const Consts &k) { return nblk(m.nCell, k.fluxTile); }
+int flux_blocks(const DevMesh &m, const Consts &k)
+{
+    return nblk(m.nCell, m.tileStride > 0 ? kTileCells : k.fluxTile);
+}
 
 template <bool SPEC_ONLY, bool FUSED_DT, typename T, int BS>
 static void launch_flux_passes(const DevMesh &m, const Consts &k, DevFlow &f, const T *grad, const T *gradT,
@@ -1756,6 +1865,20 @@ static void launch_flux_passes(const DevMesh &m, const Consts &k, DevFlow &f, co
 template <bool FUSED_DT, int BS>
 static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
+    if (m.tileStride > 0) {
+        const size_t lds = std::max((size_t)m.tileStride * 6 * sizeof(double) * (k.viscous ? 2 : 1),
+                                    (size_t)6 * kTileCells * sizeof(double));
+        const int nb = nblk(m.nCell, kTileCells);
+        if (k.mixed)
+            hipLaunchKernelGGL((k_flux_tile<float, kTileCells>), dim3(nb), dim3(kTileCells), lds, s, m, k, f.rec,
+                               f.gradTF, f.vist, m.tileFacePtr, m.tileFaces, m.c2loc, m.tileStride, f.q, f.res,
+                               f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart, FUSED_DT);
+        else
+            hipLaunchKernelGGL((k_flux_tile<double, kTileCells>), dim3(nb), dim3(kTileCells), lds, s, m, k, f.rec,
+                               f.gradT, f.vist, m.tileFacePtr, m.tileFaces, m.c2loc, m.tileStride, f.q, f.res,
+                               f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart, FUSED_DT);
+        return;
+    }
     const int nbf = nblk(m.nFace, 256);
     if (k.mixed)
         hipLaunchKernelGGL(k_face_flux<float>, dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradTF, f.vist, f.faceInv,
