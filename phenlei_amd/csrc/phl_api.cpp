@@ -1169,7 +1169,10 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
         case PHL_FIELD_SPEC_T: return copy_field(c, f.specT, 1, nC, nT, out);
         case PHL_FIELD_DQ_T: return copy_field(c, f.dqT, 1, nT, nT, out);
         case PHL_FIELD_WDIST: return copy_field(c, c->m.wallDist, 1, nC, nT, out);
-        default: return copy_field(c, f.gradNut + (size_t)(field - PHL_FIELD_DNUDX) * nC, 1, nC, nT, out);
+        default:  // a relaxation does not need the gradient of nu~: compute it from the current field
+            phl::launch_sa_gradient(c->m, c->k, f, c->stream);
+            TRY(check_launch(c));
+            return copy_field(c, f.gradNut + (size_t)(field - PHL_FIELD_DNUDX) * nC, 1, nC, nT, out);
         }
     }
     return fail(PHL_ERR_ARG, "phl_get_field: unknown field");

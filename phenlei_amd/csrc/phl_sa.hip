@@ -209,16 +209,12 @@ __global__ void __launch_bounds__(kBlock) k_sa_gradient(DevMesh M, const double 
 // Per real cell: spectrum initialisation, source (and its implicit part),
 // convective and diffusive faces.  GT: storage type of the NS gradients.
 template <typename GT>
-__global__ void __launch_bounds__(kBlock) k_sa_residual(DevMesh M, Consts K, const double *__restrict__ q,
-                                                        const double *__restrict__ visl,
-                                                        const GT *__restrict__ grad,
-                                                        const double *__restrict__ nut,
-                                                        const double *__restrict__ dt,
-                                                        double *__restrict__ resT, double *__restrict__ specT)
+__device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts &K, const double *__restrict__ q,
+                                                 const double *__restrict__ visl, const GT *__restrict__ grad,
+                                                 const double *__restrict__ nut, const double *__restrict__ dt,
+                                                 int c, double &resOut, double &specOut)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
-    if (c >= nC) return;
     const double oRe = K.oRe;
     // --- source (SourceFluxOneEquationOriginal, NSMB: no cb2 |grad|^2 term)
     const double kap2 = kKarm * kKarm, oKap2 = 1.0 / kap2;
@@ -377,8 +373,24 @@ __global__ void __launch_bounds__(kBlock) k_sa_residual(DevMesh M, Consts K, con
             acc += flux2 * (visl[re] / q[re] + nueRight);
         }
     }
-    resT[c] = acc;
-    specT[c] = spec;
+    resOut = acc;
+    specOut = spec;
+}
+
+template <typename GT>
+__global__ void __launch_bounds__(kBlock) k_sa_residual(DevMesh M, Consts K, const double *__restrict__ q,
+                                                        const double *__restrict__ visl,
+                                                        const GT *__restrict__ grad,
+                                                        const double *__restrict__ nut,
+                                                        const double *__restrict__ dt,
+                                                        double *__restrict__ resT, double *__restrict__ specT)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (c >= M.nCell) return;
+    double r, sp;
+    sa_residual_cell<GT>(M, K, q, visl, grad, nut, dt, c, r, sp);
+    resT[c] = r;
+    specT[c] = sp;
 }
 
 // ------------------------------------------------------- spectrum, mats
@@ -416,15 +428,11 @@ __device__ __forceinline__ void sa_face_spectrum(const DevMesh &M, const Consts 
     coeffR = oRe * oVolume * ns2 * (kCb2 * osigma * nueffR);
 }
 
-__global__ void __launch_bounds__(kBlock) k_sa_spectrum(DevMesh M, Consts K, const double *__restrict__ q,
-                                                        const double *__restrict__ visl,
-                                                        const double *__restrict__ nut, double *__restrict__ specT,
-                                                        double *__restrict__ mat)
+__device__ __forceinline__ void sa_spectrum_cell(const DevMesh &M, const Consts &K, const double *__restrict__ q,
+                                                 const double *__restrict__ visl, const double *__restrict__ nut,
+                                                 int c, double &spec, double *__restrict__ mat)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int nC = M.nCell;
-    if (c >= nC) return;
-    double spec = specT[c];
     for (int s = 0; s < M.ellW; ++s) {
         const int enc = M.c2fEnc[(size_t)s * nC + c];
         if (enc == 0) break;
@@ -449,6 +457,38 @@ __global__ void __launch_bounds__(kBlock) k_sa_spectrum(DevMesh M, Consts K, con
             spec += term;
         }
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_sa_spectrum(DevMesh M, Consts K, const double *__restrict__ q,
+                                                        const double *__restrict__ visl,
+                                                        const double *__restrict__ nut, double *__restrict__ specT,
+                                                        double *__restrict__ mat)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (c >= M.nCell) return;
+    double spec = specT[c];
+    sa_spectrum_cell(M, K, q, visl, nut, c, spec, mat);
+    specT[c] = spec;
+}
+
+// Residual and spectrum of one relaxation in one pass over a cell's faces
+// data (the neighbours loaded for the residual are still cached for the
+// spectrum); the values are those of k_sa_residual followed by k_sa_spectrum.
+template <typename GT>
+__global__ void __launch_bounds__(kBlock) k_sa_residual_spectrum(DevMesh M, Consts K, const double *__restrict__ q,
+                                                                 const double *__restrict__ visl,
+                                                                 const GT *__restrict__ grad,
+                                                                 const double *__restrict__ nut,
+                                                                 const double *__restrict__ dt,
+                                                                 double *__restrict__ resT,
+                                                                 double *__restrict__ specT, double *__restrict__ mat)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    if (c >= M.nCell) return;
+    double r, spec;
+    sa_residual_cell<GT>(M, K, q, visl, grad, nut, dt, c, r, spec);
+    resT[c] = r;
+    sa_spectrum_cell(M, K, q, visl, nut, c, spec, mat);
     specT[c] = spec;
 }
 
@@ -644,13 +684,20 @@ void launch_sa_lusgs(const DevMesh &m, const Consts &, DevFlow &f, const Levels 
 // gradient of nu~, the residual (TurbSolver::UpdateResiduals, which applies
 // the boundary again), spectrum and face matrices (Diagonal), forward sweep,
 // ghost dq, backward sweep, update.
+// The gradient of nu~ feeds no term of the NSMB formation, so a relaxation
+// leaves it out; the field read (and the standalone hook) computes it.
 void launch_sa_pre(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 {
     launch_sa_boundary(m, k, f, s);
     launch_sa_viscosity(m, k, f, s);
-    launch_sa_gradient(m, k, f, s);
-    launch_sa_residual(m, k, f, s);
-    launch_sa_spectrum(m, k, f, s);
+    launch_sa_boundary(m, k, f, s);
+    const int nbc = nblk(m.nCell, kBlock);
+    if (k.mixed)
+        hipLaunchKernelGGL(k_sa_residual_spectrum<float>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.gradF,
+                           f.nut, f.dt, f.resT, f.specT, f.saMat);
+    else
+        hipLaunchKernelGGL(k_sa_residual_spectrum<double>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.grad,
+                           f.nut, f.dt, f.resT, f.specT, f.saMat);
 }
 
 void launch_sa_relax(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
