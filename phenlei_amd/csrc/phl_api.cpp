@@ -1136,6 +1136,7 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
     case PHL_FIELD_DQDY:
     case PHL_FIELD_DQDZ: {
         const size_t off = (size_t)5 * (field - PHL_FIELD_DQDX) * nC;
+        phl::launch_rec_grad(c->m, c->k, f, c->stream);
         if (c->k.mixed) return copy_field_f32(c, f.gradF + off, 5, nC, nT, out);
         return copy_field(c, f.grad + off, 5, nC, nT, out);
     }

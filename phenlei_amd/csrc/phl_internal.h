@@ -177,8 +177,13 @@ struct Levels {
 // Launchers (phl_kernels.hip).  All are asynchronous on `s`.
 void launch_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
-// fp64Grad: store FP64 gradients even in the mixed form (forces, field reads).
+// The gradients go to the gather records (the face loop and the SA source
+// read them there).  fp64Grad: also store FP64 SoA gradients, even in the
+// mixed form (the force integration).
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad = false);
+// Copy the gradients of the gather records to the SoA arrays of the
+// context's precision (grad, or gradF in the mixed form).
+void launch_rec_grad(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt);
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 // Workgroups of the fused flux launch (one partial-sum row each).

@@ -381,7 +381,8 @@ template <typename GT, int BS>
 __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const double *__restrict__ q,
                                                      GT *__restrict__ grad, GT *__restrict__ gradT,
                                                      double *__restrict__ lim, double *__restrict__ coef,
-                                                     const double *__restrict__ vist, GatherRec *__restrict__ rec)
+                                                     const double *__restrict__ vist, GatherRec *__restrict__ rec,
+                                                     bool gradSoA)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
@@ -446,9 +447,14 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
         gx[m] *= ovol; gy[m] *= ovol; gz[m] *= ovol;
-        grad[(size_t)m * nC + c] = GT(gx[m]);
-        grad[(size_t)(5 + m) * nC + c] = GT(gy[m]);
-        grad[(size_t)(10 + m) * nC + c] = GT(gz[m]);
+    }
+    if (gradSoA) {  // for the force integration (the iteration reads the gather records)
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            grad[(size_t)m * nC + c] = GT(gx[m]);
+            grad[(size_t)(5 + m) * nC + c] = GT(gy[m]);
+            grad[(size_t)(10 + m) * nC + c] = GT(gz[m]);
+        }
     }
     if (K.viscous) {
         tgx *= ovol; tgy *= ovol; tgz *= ovol;
@@ -1825,26 +1831,46 @@ void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
 }
 
 template <int BS>
-static void launch_grad_lim_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp32)
+static void launch_grad_lim_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp32, bool soa)
 {
     // the gather records hold the gradient type of the context's face loop
     GatherRec *rec = (fp32 == (k.mixed != 0)) ? f.rec : nullptr;
     if (fp32)
         hipLaunchKernelGGL((k_grad_lim<float, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.gradF,
-                           f.gradTF, f.lim, f.coef, f.vist, rec);
+                           f.gradTF, f.lim, f.coef, f.vist, rec, soa);
     else
         hipLaunchKernelGGL((k_grad_lim<double, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.grad,
-                           f.gradT, f.lim, f.coef, f.vist, rec);
+                           f.gradT, f.lim, f.coef, f.vist, rec, soa);
 }
 
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad)
 {
     const bool fp32 = k.mixed && !fp64Grad;
     switch (k.gradTile) {
-    case 64: launch_grad_lim_t<64>(m, k, f, s, fp32); break;
-    case 128: launch_grad_lim_t<128>(m, k, f, s, fp32); break;
-    default: launch_grad_lim_t<256>(m, k, f, s, fp32); break;
+    case 64: launch_grad_lim_t<64>(m, k, f, s, fp32, fp64Grad); break;
+    case 128: launch_grad_lim_t<128>(m, k, f, s, fp32, fp64Grad); break;
+    default: launch_grad_lim_t<256>(m, k, f, s, fp32, fp64Grad); break;
     }
+}
+
+// The gradients of the last residual evaluation, copied out of the gather
+// records into the SoA arrays (grad[m * nC + c], m = 0..14) for a field read.
+template <typename GT>
+__global__ void __launch_bounds__(kBlock) k_rec_grad(int nC, const GatherRec *__restrict__ rec, GT *__restrict__ grad)
+{
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= nC) return;
+    const GT *g = reinterpret_cast<const GT *>(rec[c].g);
+#pragma unroll
+    for (int m = 0; m < 15; ++m) grad[(size_t)m * nC + c] = g[m];
+}
+
+void launch_rec_grad(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    if (k.mixed)
+        hipLaunchKernelGGL(k_rec_grad<float>, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m.nCell, f.rec, f.gradF);
+    else
+        hipLaunchKernelGGL(k_rec_grad<double>, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m.nCell, f.rec, f.grad);
 }
 
 int flux_blocks(const DevMesh &m, const Consts &k)

@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(kBlock) k_sa_gradient(DevMesh M, const double 
 // convective and diffusive faces.  GT: storage type of the NS gradients.
 template <typename GT>
 __device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts &K, const double *__restrict__ q,
-                                                 const double *__restrict__ visl, const GT *__restrict__ grad,
+                                                 const double *__restrict__ visl, const GatherRec *__restrict__ rec,
                                                  const double *__restrict__ nut, const double *__restrict__ dt,
                                                  int c, double &resOut, double &specOut)
 {
@@ -222,9 +222,11 @@ __device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts 
     const double cv13 = kCv1 * kCv1 * kCv1, cw36 = pow(kCw3, 6), or6 = 1.0 / 6.0, ct3 = 1.2;
     const double saC2 = 0.7, saC3 = 0.9, cw1Kap2 = cw1 * kap2;
     const double nueSA = nut[c];
-    const double dudy = double(grad[(size_t)6 * nC + c]), dudz = double(grad[(size_t)11 * nC + c]);
-    const double dvdx = double(grad[(size_t)2 * nC + c]), dvdz = double(grad[(size_t)12 * nC + c]);
-    const double dwdx = double(grad[(size_t)3 * nC + c]), dwdy = double(grad[(size_t)8 * nC + c]);
+    // velocity gradients of the NS step (d/dx of variable m at [m], d/dy at [5+m], d/dz at [10+m])
+    const GT *grad = reinterpret_cast<const GT *>(rec[c].g);
+    const double dudy = double(grad[6]), dudz = double(grad[11]);
+    const double dvdx = double(grad[2]), dvdz = double(grad[12]);
+    const double dwdx = double(grad[3]), dwdy = double(grad[8]);
     const double a1 = dwdy - dvdz, a2 = dudz - dwdx, a3 = dvdx - dudy;
     const double omega = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
     const double volume = M.vol[c];
@@ -380,7 +382,7 @@ __device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts 
 template <typename GT>
 __global__ void __launch_bounds__(kBlock) k_sa_residual(DevMesh M, Consts K, const double *__restrict__ q,
                                                         const double *__restrict__ visl,
-                                                        const GT *__restrict__ grad,
+                                                        const GatherRec *__restrict__ grad,
                                                         const double *__restrict__ nut,
                                                         const double *__restrict__ dt,
                                                         double *__restrict__ resT, double *__restrict__ specT)
@@ -477,7 +479,7 @@ __global__ void __launch_bounds__(kBlock) k_sa_spectrum(DevMesh M, Consts K, con
 template <typename GT>
 __global__ void __launch_bounds__(kBlock) k_sa_residual_spectrum(DevMesh M, Consts K, const double *__restrict__ q,
                                                                  const double *__restrict__ visl,
-                                                                 const GT *__restrict__ grad,
+                                                                 const GatherRec *__restrict__ grad,
                                                                  const double *__restrict__ nut,
                                                                  const double *__restrict__ dt,
                                                                  double *__restrict__ resT,
@@ -654,10 +656,10 @@ void launch_sa_residual(const DevMesh &m, const Consts &k, DevFlow &f, hipStream
     const int nbc = nblk(m.nCell, kBlock);
     launch_sa_boundary(m, k, f, s);
     if (k.mixed)
-        hipLaunchKernelGGL(k_sa_residual<float>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.gradF, f.nut,
+        hipLaunchKernelGGL(k_sa_residual<float>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.rec, f.nut,
                            f.dt, f.resT, f.specT);
     else
-        hipLaunchKernelGGL(k_sa_residual<double>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.grad, f.nut,
+        hipLaunchKernelGGL(k_sa_residual<double>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.rec, f.nut,
                            f.dt, f.resT, f.specT);
 }
 
@@ -693,10 +695,10 @@ void launch_sa_pre(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
     launch_sa_boundary(m, k, f, s);
     const int nbc = nblk(m.nCell, kBlock);
     if (k.mixed)
-        hipLaunchKernelGGL(k_sa_residual_spectrum<float>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.gradF,
+        hipLaunchKernelGGL(k_sa_residual_spectrum<float>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.rec,
                            f.nut, f.dt, f.resT, f.specT, f.saMat);
     else
-        hipLaunchKernelGGL(k_sa_residual_spectrum<double>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.grad,
+        hipLaunchKernelGGL(k_sa_residual_spectrum<double>, dim3(nbc), dim3(kBlock), 0, s, m, k, f.q, f.visl, f.rec,
                            f.nut, f.dt, f.resT, f.specT, f.saMat);
 }
 
