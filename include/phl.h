@@ -72,8 +72,17 @@ typedef struct phl_params {
     int face_loop;          /* 0 (default): face kernel + cell gather, face values through HBM; 1:
                              * tile-fused (a workgroup evaluates the faces of its cells into LDS and sums
                              * them; faces between tiles are evaluated twice).  Identical results; the
-                             * tile form measured slower on the hybrid meshes (C2 0.39 vs 0.32 ms). */
-    int reserved;
+                             * tile form measured slower on the hybrid meshes (1M-cell hybrid bump:
+                             * 0.39 vs 0.32 ms). */
+    int sweep_form;         /* LU-SGS sweeps (identical results: same arithmetic and summation order):
+                             * 0 (default) dataflow, one launch per direction: chunks of 64 cells of
+                             * the level order, each wavefront waiting on the flags of the chunks that
+                             * hold its cells' swept neighbours, chunks taken in launch order;
+                             * 1 one launch per dependency level;
+                             * 2 dataflow with chunks taken from a ticket counter (independent of the
+                             * workgroup dispatch order; slower on large meshes).
+                             * The dataflow waits are bounded (0.2 s): a wait that runs out makes
+                             * phl_iterate/phl_lusgs fail with PHL_ERR_NUMERIC. */
 } phl_params;
 
 /* Field identifiers for phl_get_field (same numbering as the oracle). */

@@ -51,7 +51,7 @@ typedef struct orc_params {
     int precision;          /* 0 fp64, 1 mixed (ignored by the oracle) */
     int storage_order;  /* device storage order of the product; the oracle ignores it */
     int face_loop;      /* face-loop form of the product; the oracle ignores it */
-    int reserved;
+    int sweep_form;     /* sweep launch form of the product; the oracle ignores it */
 } orc_params;
 
 /* Boundary-condition codes (API/Geometry/include/Geo_SimpleBC.h:32-39). */

@@ -50,7 +50,7 @@ class Params(ctypes.Structure):
         ("viscous", ctypes.c_int), ("gradient", ctypes.c_int), ("limiter", ctypes.c_int),
         ("entropy_fix", ctypes.c_int), ("cfl_vary_step", ctypes.c_int),
         ("n_lusgs_sweeps", ctypes.c_int), ("precision", ctypes.c_int), ("storage_order", ctypes.c_int),
-        ("face_loop", ctypes.c_int), ("reserved", ctypes.c_int),
+        ("face_loop", ctypes.c_int), ("sweep_form", ctypes.c_int),
     ]
 
     def as_dict(self):

@@ -179,6 +179,58 @@ void build_sweep_order(int nC, const std::vector<int> &c2fPtr, const std::vector
     }
 }
 
+// The dataflow sweep's order (phl::FlowOrder) for one direction: each level
+// of `cells` (mesh numbers, level order) padded to whole chunks of 64
+// positions, the slot lists of build_sweep_order moved along (row, nb in mesh
+// numbers; converted to storage here), and per chunk the distinct chunks
+// holding its real neighbours, all of earlier levels.
+phl_status build_flow_order(phl_ctx *c, int nC, const std::vector<int> &cells, const std::vector<int> &lptr,
+                            const std::vector<int> &row, const std::vector<int> &nb, int Wa,
+                            const std::vector<int> &slotOf, phl::FlowOrder &fo)
+{
+    int nCh = 0;
+    for (size_t L = 0; L + 1 < lptr.size(); ++L) nCh += (lptr[L + 1] - lptr[L] + 63) / 64;
+    const size_t Pp = (size_t)64 * nCh;
+    std::vector<int> fc(Pp, -1), frow((size_t)Wa * Pp, -1), fnb((size_t)Wa * Pp, 0), chunkOf(nC, -1);
+    std::vector<int> depPtr(1, 0), dep, d;
+    int ch = 0;
+    for (size_t L = 0; L + 1 < lptr.size(); ++L) {
+        const int levelChunk0 = ch;
+        for (int b = lptr[L]; b < lptr[L + 1]; b += 64, ++ch) {
+            d.clear();
+            for (int j = 0; j < 64 && b + j < lptr[L + 1]; ++j) {
+                const int pos = b + j;
+                const size_t q = (size_t)ch * 64 + j;
+                fc[q] = slotOf[cells[pos]];
+                chunkOf[cells[pos]] = ch;
+                for (int s = 0; s < Wa; ++s) {
+                    const int r = row[(size_t)s * nC + pos], n = nb[(size_t)s * nC + pos];
+                    frow[(size_t)s * Pp + q] = r;
+                    fnb[(size_t)s * Pp + q] = n < nC ? slotOf[n] : n;
+                    if (r < 0 || n >= nC) continue;
+                    // a neighbour swept earlier lies in a chunk of an earlier level
+                    if (chunkOf[n] < 0 || chunkOf[n] >= levelChunk0)
+                        return fail(PHL_ERR_ARG, "dataflow sweep order: a neighbour is not in an earlier level");
+                    d.push_back(chunkOf[n]);
+                }
+            }
+            std::sort(d.begin(), d.end());
+            d.erase(std::unique(d.begin(), d.end()), d.end());
+            dep.insert(dep.end(), d.begin(), d.end());
+            depPtr.push_back((int)dep.size());
+        }
+    }
+    if (dep.empty()) dep.push_back(0);  // keeps the device pointer valid
+    const int *t;
+    TRY(upload(c, &t, fc)); fo.cells = t;
+    TRY(upload(c, &t, frow)); fo.row = t;
+    TRY(upload(c, &t, fnb)); fo.nb = t;
+    TRY(upload(c, &t, depPtr)); fo.depPtr = t;
+    TRY(upload(c, &t, dep)); fo.dep = t;
+    fo.nChunks = nCh;
+    return PHL_OK;
+}
+
 double compute_cfl(const phl_params &p, int step)
 {
     // NSSolver::ComputeCFL (CFD/NSSolver/src/NSSolver.cpp:595-668), CFLMethod 0.
@@ -230,6 +282,9 @@ phl_status check_params(const phl_params &p)
     if (p.n_lusgs_sweeps != 1) return fail(PHL_ERR_UNSUPPORTED, "nLUSGSSweeps must be 1");
     if (p.precision != 0 && p.precision != 1)
         return fail(PHL_ERR_UNSUPPORTED, "precision must be 0 (FP64) or 1 (mixed)");
+    if (p.sweep_form < 0 || p.sweep_form > 2)
+        return fail(PHL_ERR_ARG, "sweep_form must be 0 (dataflow, launch order), 1 (one launch per level) or 2 "
+                                 "(dataflow, ticket order)");
     if (p.t_wall > 0.0) return fail(PHL_ERR_UNSUPPORTED, "isothermal walls are not implemented (t_wall <= 0)");
     if (p.cfl_vary_step <= 0) return fail(PHL_ERR_ARG, "cfl_vary_step must be positive");
     return PHL_OK;
@@ -240,6 +295,21 @@ phl_status check_launch(phl_ctx *c)
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return fail(PHL_ERR_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
     (void)c;
+    return PHL_OK;
+}
+
+// Dataflow sweeps bound every wait; a wave that gave up leaves its cells'
+// dq unfinished and is counted in sweepCtl[2].  Checked after the work is
+// done (synchronises the stream).
+phl_status check_sweep_waits(phl_ctx *c)
+{
+    if (c->k.sweepForm == 1) return PHL_OK;
+    int gaveUp = 0;
+    HIPTRY(hipMemcpyAsync(&gaveUp, c->f.sweepCtl + 2, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    if (gaveUp != 0)
+        return fail(PHL_ERR_NUMERIC, "dataflow sweep: " + std::to_string(gaveUp) +
+                                         " wavefronts exceeded the wait bound; the state is invalid");
     return PHL_OK;
 }
 
@@ -857,6 +927,9 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         std::vector<int> row, nbs;
         build_sweep_order(nC, c2fPtr, c2fS, c2n, isLeft, dir == 0 ? cellsF : cellsB, dir == 0, lv.so.W, lv.so.Wpad,
                           row, nbs);
+        if (p.sweep_form != 1 && lv.so.Wpad <= 6)
+            TRY(build_flow_order(c, nC, dir == 0 ? cellsF : cellsB, dir == 0 ? c->hfwdPtr : c->hbwdPtr, row, nbs,
+                                 lv.so.Wpad, slotOf, lv.flow));
         for (int &v : nbs) v = S(v);  // unused slots hold 0, a valid storage index
         lv.so.cells = lv.cells;
         TRY(upload(c, &lv.so.row, row));
@@ -916,6 +989,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.nSweeps = p.n_lusgs_sweeps;
     k.mixed = p.precision;
     k.coefF32 = (k.mixed && c->coefF32Ok) ? 1 : 0;
+    k.sweepForm = p.sweep_form;
     k.saNuInf = p.sa_nu_inf;
     k.saVistInf = p.sa_vist_inf;
     k.turbCflScale = p.turb_cfl_scale;
@@ -954,6 +1028,11 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.negCount, 1));
     TRY(dalloc(c, &fl.negTotal, 1));
     TRY(dalloc(c, &fl.coef, (size_t)nF * 2 * 16));
+    const int nChunks = std::max({c->fwd.flow.nChunks, c->bwd.flow.nChunks, 1});
+    TRY(dalloc(c, &fl.sweepFlag, (size_t)nChunks));
+    HIPTRY(hipMemset(fl.sweepFlag, 0, sizeof(int) * nChunks));
+    TRY(dalloc(c, &fl.sweepCtl, 4));
+    HIPTRY(hipMemset(fl.sweepCtl, 0, sizeof(int) * 4));
     TRY(dalloc(c, &fl.t, (size_t)nT));
     TRY(dalloc(c, &fl.visl, (size_t)nT));
     TRY(dalloc(c, &fl.forceOut, 15));
@@ -1312,6 +1391,7 @@ phl_status phl_lusgs(phl_ctx *c, int *n_fixed)
     HIPTRY(hipMemsetAsync(c->f.negCount, 0, sizeof(int), c->stream));
     enqueue_lusgs(c);
     TRY(check_launch(c));
+    TRY(check_sweep_waits(c));
     if (n_fixed) {
         HIPTRY(hipMemcpyAsync(c->hostCount, c->f.negCount, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPTRY(hipStreamSynchronize(c->stream));
@@ -1652,6 +1732,7 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5 * chunk * nCases, hipMemcpyDeviceToHost,
                               c->stream));
         HIPTRY(hipStreamSynchronize(c->stream));
+        TRY(check_sweep_waits(c));
         if (c->timing) {
             for (int i = 0; i < chunk; ++i)
                 for (int k = 0; k < K; ++k) {

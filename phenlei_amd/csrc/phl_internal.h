@@ -96,6 +96,7 @@ struct Consts {
     int nCases;             // > 1: batched context, per-case scalars in cases[caseOf[cell]]
     int fluxTile, gradTile; // cells per workgroup of the flux and gradient kernels (64, 128, 256)
     int coefF32;            // mixed form: LU-SGS face coefficients stored in FP32 (sweep arithmetic stays FP64)
+    int sweepForm;          // 0, 2: dataflow sweep (launch order, ticket order); 1: one launch per level
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     const CaseConsts *cases;
 };
@@ -126,6 +127,8 @@ struct DevFlow {
     double *lim;      // [nTotal]
     double *dq;       // [5][nTotal]
     double *dflux;    // [5][nCell]
+    int *sweepFlag;   // [chunks] dataflow sweep: epoch of the sweep that last published the chunk's dq
+    int *sweepCtl;    // [4] dataflow sweep: ticket, epoch, waves that gave up waiting
     double *specInv, *specVis, *dtLoc, *dt, *diag;  // [nCell]
     double *normPart;   // [nBlocksFlux][5]
     double *normOut;    // [maxSteps][nCases][5], row = *stepCtr
@@ -166,12 +169,25 @@ struct SweepOrder {
     const int *nb;      // [Wpad][nCell]: neighbour cell (0 if unused)
 };
 
+// Dataflow sweep order: the level order with every level padded to whole
+// chunks of 64 positions (a chunk never spans two levels), so a chunk
+// depends only on chunks of earlier levels.
+struct FlowOrder {
+    int nChunks;        // 0: not built
+    const int *cells;   // [64*nChunks] storage index, -1 for padding
+    const int *row;     // [Wpad][64*nChunks] as SweepOrder::row
+    const int *nb;      // [Wpad][64*nChunks] as SweepOrder::nb
+    const int *depPtr;  // [nChunks+1]
+    const int *dep;     // the distinct earlier chunks holding a chunk's real neighbours
+};
+
 struct Levels {
     int nLevels;
     int *ptr;    // device [nLevels+1]
     int *cells;  // device [nCell]
     int *hptr;   // host copy of ptr
     SweepOrder so;
+    FlowOrder flow;
 };
 
 // Launchers (phl_kernels.hip).  All are asynchronous on `s`.

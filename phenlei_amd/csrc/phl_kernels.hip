@@ -1564,6 +1564,147 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_block_so(DevMesh M, Const
     }
 }
 
+// ------------------------------------------------- dataflow sweep
+// One launch per direction instead of one per level (sweep_form 0 and 2).  Each
+// 64-thread block (one wavefront) takes the next chunk of 64 positions of
+// the padded level order from a ticket counter.  A chunk depends only on
+// chunks of earlier levels, which were taken earlier by blocks that are
+// running or done, so every wait ends.  The wave polls the flags of the
+// chunks it depends on (one per lane), then sweeps its 64 cells exactly as
+// sweep_position does and publishes its chunk.  dq and the flags are stored
+// and loaded with device-scope (sc1) accesses, which are coherent across the
+// XCDs' L2 caches, and the wave's dq stores complete (vmcnt 0) before its
+// flag store is issued.
+// ctl: [0] ticket, [1] epoch of the current sweep, [2] waves that gave up.
+constexpr long long kFlowWaitTicks = 20000000;  // 0.2 s of the 100 MHz wall clock
+
+__global__ void k_sweep_flow_prep(int *ctl)
+{
+    ctl[0] = 0;
+    ctl[1] += 1;
+}
+
+__device__ __forceinline__ int flag_load(const int *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ double dq_load(const double *p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// One chunk of the dataflow sweep, swept by one wavefront: false if the wait
+// bound was exceeded (the chunk is then neither swept nor published).
+template <int NW, int VIS, typename CT>
+__device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, const FlowOrder &fo, int *ctl,
+                                           int *chunkFlag, bool useU, const double *__restrict__ coef,
+                                           const double *__restrict__ res, const double *__restrict__ diag,
+                                           double *dq, double *dflux, int tk, int lane, int epoch, long long t0)
+{
+    const int nT = M.nTotal, nC = M.nCell, P = 64 * fo.nChunks;
+    const int p = tk * 64 + lane;
+    const int c = fo.cells[p];
+    const bool live = c >= 0;
+    const int cc = live ? c : 0;
+    // everything that does not depend on the neighbours' dq is loaded first
+    int row[NW], nb[NW];
+#pragma unroll
+    for (int s = 0; s < NW; ++s) {
+        row[s] = fo.row[(size_t)s * P + p];
+        nb[s] = fo.nb[(size_t)s * P + p];
+    }
+    const double D = diag[cc];
+    double r[5], U[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        r[m] = res[(size_t)m * nC + cc];
+        U[m] = useU ? dflux[(size_t)m * nC + cc] : 0.0;
+    }
+    double co[NW][16];
+#pragma unroll
+    for (int s = 0; s < NW; ++s) load_coef_row<CT>(coef, row[s] >= 0 ? row[s] : 0, co[s]);
+
+    // wait for the chunks this one depends on, one flag per lane
+    const int d0 = fo.depPtr[tk], d1 = fo.depPtr[tk + 1];
+    for (int i0 = d0; i0 < d1; i0 += 64) {
+        const int i = i0 + lane;
+        bool ok = i >= d1;
+        const int dc = ok ? 0 : fo.dep[i];
+        for (int it = 1;; ++it) {
+            if (!ok) ok = flag_load(&chunkFlag[dc]) == epoch;
+            if (__all(ok)) break;
+            // bounded wait: never reached by a correct schedule; a wave that
+            // gives up makes every later waiter give up too
+            if (wall_clock64() - t0 > kFlowWaitTicks || ((it & 63) == 0 && flag_load(&ctl[2]) != 0)) return false;
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the dq loads stay after the flag loads
+    double dqn[NW][5];
+#pragma unroll
+    for (int s = 0; s < NW; ++s) {
+        const int i = row[s] >= 0 ? nb[s] : 0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dqn[s][m] = dq_load(&dq[(size_t)m * nT + i]);
+    }
+    double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int s = 0; s < NW; ++s) {
+        double df[5];
+        sweep_apply<VIS>(K, co[s], dqn[s], df);
+        const bool act = row[s] >= 0;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) rhs0[m] += act ? df[m] : 0.0;
+    }
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            __hip_atomic_store(&dq[(size_t)m * nT + c], (r[m] - U[m] - rhs0[m]) / D, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_AGENT);
+            dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
+        }
+    }
+    // the wave's dq stores are complete before the flag that publishes them
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    if (lane == 0) __hip_atomic_store(&chunkFlag[tk], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// TICKET false (sweep_form 0): block b sweeps chunk b.  Workgroups of a
+// launch are dispatched to each XCD in increasing order, so when block b is
+// resident every smaller block of its XCD has been dispatched; the smallest
+// unfinished chunk then always has its dependencies done and proceeds.
+// TICKET true (sweep_form 2): a block of 4 wavefronts takes 4 consecutive
+// chunks from a ticket counter, so a chunk is only ever claimed by a running
+// block after every smaller chunk, whatever the dispatch order (the counter
+// is one device-scope atomic per block, which serialises large launches).
+template <int NW, int VIS, typename CT, bool TICKET>
+__global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Consts K, FlowOrder fo, int *ctl,
+                                                                  int *chunkFlag, bool useU,
+                                                                  const double *__restrict__ coef,
+                                                                  const double *__restrict__ res,
+                                                                  const double *__restrict__ diag, double *dq,
+                                                                  double *dflux)
+{
+    const int lane = (int)threadIdx.x & 63;
+    int tk;
+    if constexpr (TICKET) {
+        __shared__ int ticket;
+        if (threadIdx.x == 0) ticket = __hip_atomic_fetch_add(&ctl[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __syncthreads();
+        tk = __builtin_amdgcn_readfirstlane(ticket * 4 + ((int)threadIdx.x >> 6));
+        if (tk >= fo.nChunks) return;
+    } else {
+        tk = blockIdx.x;
+    }
+    if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, tk, lane, ctl[1],
+                                 wall_clock64()))
+        if (lane == 0) atomicAdd(&ctl[2], 1);
+}
+
 __global__ void __launch_bounds__(kBlock) k_ghost_dq(DevMesh M, Consts K, const double *__restrict__ q,
                                                      double *__restrict__ dq)
 {
@@ -1965,6 +2106,16 @@ template <int NW, int VIS, typename CT>
 static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool useU,
                             hipStream_t s)
 {
+    if (lv.flow.nChunks > 0) {  // built unless sweep_form is 1
+        hipLaunchKernelGGL(k_sweep_flow_prep, dim3(1), dim3(1), 0, s, f.sweepCtl);
+        if (k.sweepForm == 2)
+            hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, true>), dim3(nblk(lv.flow.nChunks, 4)), dim3(256), 0, s, m,
+                               k, lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+        else
+            hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, false>), dim3(lv.flow.nChunks), dim3(64), 0, s, m, k,
+                               lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+        return;
+    }
     int L = 0;
     while (L < lv.nLevels) {
         const int sz = lv.hptr[L + 1] - lv.hptr[L];

@@ -391,3 +391,45 @@ def test_face_loop_forms_agree(viscous):
     assert np.max(np.abs(ha - hb) / hb) <= 1e-14  # the norm's block partials group 128 vs 256 cells
     assert np.array_equal(a.get_state(), b.get_state())
     assert np.array_equal(a.get_residual(), b.get_residual())
+
+
+@pytest.mark.parametrize("kind,precision,form", [("hex", 0, 0), ("hybrid", 0, 0), ("hybrid", 1, 0), ("wing", 0, 0),
+                                                 ("hybrid", 0, 2), ("wing", 1, 2)])
+def test_dataflow_sweep_equals_level_sweep(kind, precision, form):
+    """The dataflow sweeps (one launch per direction, per-chunk flags; chunks
+    in launch order, form 0, or ticket order, form 2) and the level-by-level
+    launches (form 1) apply the same arithmetic in the same order: states,
+    residual histories and dq are bitwise equal, over several calls (the
+    flags carry a new epoch each sweep) and in both launch modes."""
+    if kind == "hex":
+        m = pa.Mesh.bump_hex(48, 16, 6, 0.08)
+    elif kind == "wing":
+        m = pa.Mesh.swept_wing(12, 6, 5, 1e-3, 3)
+    else:
+        m = pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7)
+    base = dict(mach=0.3, viscous=0 if kind == "hex" else 1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0,
+                precision=precision)
+    a = pa.Solver(m, pa.default_params(sweep_form=form, **base))
+    b = pa.Solver(m, pa.default_params(sweep_form=1, **base))
+    for s in (a, b):
+        s.init_freestream()
+    for mode, first, n in (("eager", 1, 5), ("graph", 6, 17)):
+        a.set_launch_mode(mode)
+        b.set_launch_mode(mode)
+        ha, hb = a.iterate(first, n), b.iterate(first, n)
+        assert np.array_equal(ha, hb), mode
+        assert np.array_equal(a.get_state(), b.get_state()), mode
+        assert np.array_equal(a.get_field("dq"), b.get_field("dq")), mode
+
+
+def test_dataflow_sweep_batch_equals_level_sweep():
+    """A batched context (levels merged over the cases) in the dataflow form."""
+    parts = [pa.Mesh.bump_hex(24 + 8 * i, 8, 4, 0.08) for i in range(3)]
+    u = pa.Mesh.concat(parts)
+    out = []
+    for form in (0, 1):
+        s = pa.Solver(u, [pa.default_params(cfl_start=5.0, cfl_end=5.0, sweep_form=form) for _ in parts])
+        s.init_freestream()
+        out.append((s.iterate(1, 12), s.get_state()))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1])
