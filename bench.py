@@ -402,10 +402,14 @@ def main():
     sweep_ms = (times["sweep_fwd"] + times["sweep_bwd"]) / max(n_timed, 1)
     sb = sweep_algorithmic_bytes(mesh.n_cell, mesh.n_face, mixed)
     sweep_gbs = sb / (sweep_ms * 1e-3) / 1e9
+    levels_form = params.sweep_form == 1
     roof_sweep = dict(bound="hbm", achieved=round(sweep_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                      frac=round(sweep_gbs / HBM_PEAK_GBS, 4), traffic=None, kernel="k_sweep_level_so + k_sweep_block_so",
+                      frac=round(sweep_gbs / HBM_PEAK_GBS, 4), traffic=None,
+                      kernel="k_sweep_level_so + k_sweep_block_so" if levels_form else "k_sweep_flow",
                       kernel_ms=round(sweep_ms, 5), algorithmic_bytes=sb,
-                      note="latency bound: each dependency level is a launch with two dependent memory rounds")
+                      note="latency bound: each dependency level is a launch with two dependent memory rounds"
+                      if levels_form else "latency bound: one launch per direction; each dependency level costs "
+                      "a flag round trip, the neighbours' dq loads and the store acknowledgement")
     print(f"[bench] mean ms per step by kernel class: {per_class}", file=sys.stderr, flush=True)
     if args.copy_probe:
         roof["copy_probe_gbs"] = round(pa.hbm_copy_gbps(0, 2 * 1024**3, 10), 1)

@@ -1031,6 +1031,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     const int nChunks = std::max({c->fwd.flow.nChunks, c->bwd.flow.nChunks, 1});
     TRY(dalloc(c, &fl.sweepFlag, (size_t)nChunks));
     HIPTRY(hipMemset(fl.sweepFlag, 0, sizeof(int) * nChunks));
+    TRY(dalloc(c, &fl.casePart, (size_t)nCases * 32 * 6));
     TRY(dalloc(c, &fl.sweepCtl, 4));
     HIPTRY(hipMemset(fl.sweepCtl, 0, sizeof(int) * 4));
     TRY(dalloc(c, &fl.t, (size_t)nT));

@@ -127,6 +127,7 @@ struct DevFlow {
     double *lim;      // [nTotal]
     double *dq;       // [5][nTotal]
     double *dflux;    // [5][nCell]
+    double *casePart; // batched contexts: [nCases][32][6] partial residual sums and dt minima
     int *sweepFlag;   // [chunks] dataflow sweep: epoch of the sweep that last published the chunk's dq
     int *sweepCtl;    // [4] dataflow sweep: ticket, epoch, waves that gave up waiting
     double *specInv, *specVis, *dtLoc, *dt, *diag;  // [nCell]

@@ -1228,15 +1228,19 @@ __global__ void k_norm_final(const double *__restrict__ part, const double *__re
 }
 
 // Batched contexts: per-case residual sums of the step and minimum local time
-// step, one workgroup per case, in a fixed order.
+// step in a fixed order.  Workgroup (case cs, slice j) reduces the j-th of
+// kCaseSlices contiguous parts of the case's cells into casePart; then one
+// thread per case adds its slices in order.
+constexpr int kCaseSlices = 32;
 __global__ void __launch_bounds__(kBlock) k_case_reduce(DevMesh M, const double *__restrict__ res,
-                                                        const double *__restrict__ dtLoc, int nCases,
-                                                        double *__restrict__ normOut, const int *__restrict__ stepCtr,
-                                                        double *__restrict__ dtMin)
+                                                        const double *__restrict__ dtLoc,
+                                                        double *__restrict__ casePart)
 {
     __shared__ double red[6][kBlock];
-    const int cs = blockIdx.x;
-    const int b = M.caseCellPtr[cs], e = M.caseCellPtr[cs + 1];
+    const int cs = blockIdx.x / kCaseSlices, j = blockIdx.x % kCaseSlices;
+    const int cb = M.caseCellPtr[cs], ce = M.caseCellPtr[cs + 1];
+    const int per = (ce - cb + kCaseSlices - 1) / kCaseSlices;
+    const int b = min(ce, cb + j * per), e = min(ce, b + per);
     const int nC = M.nCell;
     double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double mn = 1.0e300;
@@ -1260,8 +1264,26 @@ __global__ void __launch_bounds__(kBlock) k_case_reduce(DevMesh M, const double 
         }
         __syncthreads();
     }
-    if (threadIdx.x < 5) normOut[((size_t)(*stepCtr) * nCases + cs) * 5 + threadIdx.x] = red[threadIdx.x][0];
-    if (threadIdx.x == 5) dtMin[cs] = red[5][0];
+    if (threadIdx.x < 6) casePart[(size_t)blockIdx.x * 6 + threadIdx.x] = red[threadIdx.x][0];
+}
+
+__global__ void __launch_bounds__(64) k_case_final(const double *__restrict__ casePart, int nCases,
+                                                   double *__restrict__ normOut, const int *__restrict__ stepCtr,
+                                                   double *__restrict__ dtMin)
+{
+    const int cs = blockIdx.x * blockDim.x + threadIdx.x;
+    if (cs >= nCases) return;
+    double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double mn = 1.0e300;
+    for (int j = 0; j < kCaseSlices; ++j) {
+        const double *p = casePart + ((size_t)cs * kCaseSlices + j) * 6;
+#pragma unroll
+        for (int m = 0; m < 5; ++m) a[m] += p[m];
+        mn = dMIN(mn, p[5]);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) normOut[((size_t)(*stepCtr) * nCases + cs) * 5 + m] = a[m];
+    dtMin[cs] = mn;
 }
 
 // ---------------------------------------------------------- time step
@@ -2075,7 +2097,9 @@ void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s)
 {
     if (k.nCases > 1) {
-        hipLaunchKernelGGL(k_case_reduce, dim3(k.nCases), dim3(kBlock), 0, s, m, f.res, f.dtLoc, k.nCases, f.normOut,
+        hipLaunchKernelGGL(k_case_reduce, dim3(k.nCases * kCaseSlices), dim3(kBlock), 0, s, m, f.res, f.dtLoc,
+                           f.casePart);
+        hipLaunchKernelGGL(k_case_final, dim3(nblk(k.nCases, 64)), dim3(64), 0, s, f.casePart, k.nCases, f.normOut,
                            f.stepCtr, f.dtMin);
         return;
     }

@@ -176,3 +176,25 @@ def test_sa_mixed_precision_tracks_fp64():
     print("mixed vs fp64: hist", err_h, "nut", err_n)
     assert err_h <= 1e-3
     assert err_n <= 1e-3
+
+
+@pytest.mark.parametrize("kind,mode", [("hybrid", "graph"), ("wing", "eager_overlap"), ("wing", "eager")])
+def test_sa_dataflow_sweeps_equal_level_sweeps(kind, mode):
+    """NS sweeps in the dataflow form (the default) against one launch per
+    level, with the SA model: bitwise equal states, nu~ and mu_t, also with
+    the (level-launched) SA sweeps beside the next NS step's dataflow sweeps
+    on the second stream (graph and eager_overlap)."""
+    m = mesh_of(kind)
+    base = dict(viscous=SA, reynolds=1e5, cfl_start=5.0, cfl_end=5.0)
+    a = pa.Solver(m, pa.default_params(sweep_form=0, **base))
+    b = pa.Solver(m, pa.default_params(sweep_form=1, **base))
+    a.set_launch_mode(mode)
+    b.set_launch_mode("eager")
+    for s in (a, b):
+        s.init_freestream()
+    for first, n in ((1, 11), (12, 9)):
+        ha, hb = a.iterate(first, n), b.iterate(first, n)
+        assert np.array_equal(ha, hb)
+        assert np.array_equal(a.get_state(), b.get_state())
+        assert np.array_equal(a.get_field("nut"), b.get_field("nut"))
+        assert np.array_equal(a.get_field("vist"), b.get_field("vist"))
