@@ -1681,17 +1681,20 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     }
     if (live) {
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
+        for (int m = 0; m < 5; ++m)
             __hip_atomic_store(&dq[(size_t)m * nT + c], (r[m] - U[m] - rhs0[m]) / D, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
-            dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
-        }
     }
     // the wave's dq stores are complete before the flag that publishes them
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt(0)
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
     if (lane == 0) __hip_atomic_store(&chunkFlag[tk], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    // the off-diagonal sums are read only by the next sweep (another launch)
+    if (live) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
+    }
     return true;
 }
 
