@@ -53,6 +53,11 @@ struct phl_ctx {
     hipStream_t side = nullptr;
     hipEvent_t evFork = nullptr, evJoin = nullptr;
     bool turbPending = false;
+    // Dataflow sweeps: the scalar sweeps of a step's turbulence relaxation
+    // ride along with the next step's NS sweeps (same lists, same order);
+    // saDeferred: a relaxation is waiting for its sweeps and update.
+    bool saFused = false;
+    bool saDeferred = false;
     // phl_solve / phl_solve_batch: switch a mixed context to FP64 once no
     // unconverged case has halved its residual for this many steps (0: never)
     int precisionSwitchStall = 0;
@@ -936,6 +941,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         TRY(upload(c, &lv.so.nb, nbs));
     }
     c->coefF32Ok = c->fwd.so.Wpad <= 6 && c->bwd.so.Wpad <= 6;
+    c->saFused = p.viscous == 3 && c->fwd.flow.nChunks > 0 && c->bwd.flow.nChunks > 0;
 
     // constants: the scalars of every case, the first one also in k
     phl::Consts &k = c->k;
@@ -1542,6 +1548,16 @@ static void join_turb(phl_ctx *c)
     }
 }
 
+// Finish a turbulence relaxation whose sweeps were left for the next step
+// (one launch per level, on the solver stream).
+static void flush_sa(phl_ctx *c)
+{
+    if (c->saDeferred) {
+        phl::launch_sa_lusgs(c->m, c->k, c->f, c->fwd, c->bwd, c->stream);
+        c->saDeferred = false;
+    }
+}
+
 // One outer iteration: LoadResiduals+UpdateResiduals, SpectrumRadius,
 // TimeStep, LUSGS (Controller::Relaxation, Controller.cpp:567-608), then the
 // turbulence relaxation.  With `ev` non-null an event is recorded before the
@@ -1562,14 +1578,29 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     phl::launch_flux(c->m, c->k, c->f, s, true); mark();
     phl::launch_norm_final(c->m, c->k, c->f, true, s); mark();
     phl::launch_dt_diag(c->m, c->k, c->f, s); mark();
-    phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, s); mark();
-    phl::launch_ghost_dq(c->m, c->k, c->f, s); mark();
-    phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, s); mark();
-    phl::launch_update(c->m, c->k, c->f, true, s); mark();
+    // the previous step's turbulence sweeps, carried by this step's sweeps:
+    // they read only the turbulence arrays, which this step's NS kernels so
+    // far have not touched, and write only nu~'s increment
+    const bool carry = c->saFused && c->saDeferred;
+    if (carry) phl::launch_sa_sweep_init(c->m, c->f, s);
+    phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, s, carry); mark();
+    phl::launch_ghost_dq(c->m, c->k, c->f, s);
+    if (carry) phl::launch_sa_ghost_dq(c->m, c->f, s);
+    mark();
+    phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, s, carry); mark();
+    phl::launch_update(c->m, c->k, c->f, true, s);
+    if (carry) {
+        phl::launch_sa_update(c->m, c->f, s);
+        c->saDeferred = false;
+    }
+    mark();
     swap_state(c);
     // Spalart-Allmaras relaxation on the updated state
     if (c->k.viscous == 3) {
-        if (!ev && c->launchMode != 0) {
+        if (c->saFused) {
+            phl::launch_sa_pre(c->m, c->k, c->f, s);
+            c->saDeferred = true;
+        } else if (!ev && c->launchMode != 0) {
             join_turb(c);  // the previous step's sweeps wrote nu~
             phl::launch_sa_pre(c->m, c->k, c->f, s);
             hipEventRecord(c->evFork, s);
@@ -1719,7 +1750,9 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         reset_step_counter(c);
         int i = 0;
         if (!c->timing && c->launchMode == 1) {
-            for (; i + kGraphSteps <= chunk; i += kGraphSteps) {
+            // graphs are captured with a turbulence relaxation deferred at entry
+            if (c->saFused && !c->saDeferred && i + kGraphSteps < chunk) enqueue_step(c, nullptr), ++i;
+            for (; i + kGraphSteps <= chunk && c->saFused == c->saDeferred; i += kGraphSteps) {
                 hipGraphExec_t g = nullptr;
                 join_turb(c);  // a graph starts with the side stream joined
                 TRY(step_graph(c, &g));
@@ -1756,7 +1789,9 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
             }
         done += chunk;
     }
+    flush_sa(c);
     join_turb(c);  // later work on the solver stream sees the final nu~
+    TRY(check_launch(c));
     return PHL_OK;
 }
 

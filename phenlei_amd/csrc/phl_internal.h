@@ -182,6 +182,13 @@ struct FlowOrder {
     const int *dep;     // the distinct earlier chunks holding a chunk's real neighbours
 };
 
+// The scalar Spalart-Allmaras sweep carried along by a dataflow NS sweep.
+struct SaSweep {
+    bool on;
+    const double *mat, *res, *spec;
+    double *dq, *dflux;
+};
+
 struct Levels {
     int nLevels;
     int *ptr;    // device [nLevels+1]
@@ -208,8 +215,11 @@ int flux_blocks(const DevMesh &m, const Consts &k);
 void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s);
 void launch_time_step(const DevMesh &m, const Consts &k, DevFlow &f, double cfl, hipStream_t s);
 void launch_dt_diag(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+// withSA: a dataflow sweep (flow order built) also takes the same
+// direction of the pending Spalart-Allmaras sweep (launch_sa_sweep_init
+// before the forward one, launch_sa_ghost_dq between, launch_sa_update after).
 void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,
-                  bool firstSweep, hipStream_t s);
+                  bool firstSweep, hipStream_t s, bool withSA = false);
 void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, hipStream_t s);
 void launch_copy(const double *src, double *dst, size_t n, hipStream_t s);
@@ -224,6 +234,9 @@ void launch_sa_viscosity(const DevMesh &m, const Consts &k, DevFlow &f, hipStrea
 void launch_sa_gradient(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sa_residual(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sa_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+void launch_sa_sweep_init(const DevMesh &m, DevFlow &f, hipStream_t s);
+void launch_sa_ghost_dq(const DevMesh &m, DevFlow &f, hipStream_t s);
+void launch_sa_update(const DevMesh &m, DevFlow &f, hipStream_t s);
 void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
                      hipStream_t s);
 // Everything of a relaxation before the sweeps: boundary, eddy viscosity,

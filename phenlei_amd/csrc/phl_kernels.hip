@@ -1618,11 +1618,15 @@ __device__ __forceinline__ double dq_load(const double *p)
 
 // One chunk of the dataflow sweep, swept by one wavefront: false if the wait
 // bound was exceeded (the chunk is then neither swept nor published).
+// sa.on: the chunk's cells also take the same-direction step of the scalar
+// Spalart-Allmaras sweep (the same lists and order as sa_sweep_position; its
+// dq is published under the same flag).
 template <int NW, int VIS, typename CT>
 __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, const FlowOrder &fo, int *ctl,
                                            int *chunkFlag, bool useU, const double *__restrict__ coef,
                                            const double *__restrict__ res, const double *__restrict__ diag,
-                                           double *dq, double *dflux, int tk, int lane, int epoch, long long t0)
+                                           double *dq, double *dflux, const SaSweep &sa, int tk, int lane,
+                                           int epoch, long long t0)
 {
     const int nT = M.nTotal, nC = M.nCell, P = 64 * fo.nChunks;
     const int p = tk * 64 + lane;
@@ -1646,6 +1650,14 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     double co[NW][16];
 #pragma unroll
     for (int s = 0; s < NW; ++s) load_coef_row<CT>(coef, row[s] >= 0 ? row[s] : 0, co[s]);
+    double aT[NW], rT = 0.0, DT = 1.0, UT = 0.0;
+    if (sa.on) {
+#pragma unroll
+        for (int s = 0; s < NW; ++s) aT[s] = sa.mat[row[s] >= 0 ? row[s] : 0];
+        rT = sa.res[cc];
+        DT = sa.spec[cc];
+        UT = sa.dflux[cc];
+    }
 
     // wait for the chunks this one depends on, one flag per lane
     const int d0 = fo.depPtr[tk], d1 = fo.depPtr[tk + 1];
@@ -1670,6 +1682,11 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
 #pragma unroll
         for (int m = 0; m < 5; ++m) dqn[s][m] = dq_load(&dq[(size_t)m * nT + i]);
     }
+    double dT[NW];
+    if (sa.on) {
+#pragma unroll
+        for (int s = 0; s < NW; ++s) dT[s] = dq_load(&sa.dq[row[s] >= 0 ? nb[s] : 0]);
+    }
     double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < NW; ++s) {
@@ -1679,11 +1696,17 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
 #pragma unroll
         for (int m = 0; m < 5; ++m) rhs0[m] += act ? df[m] : 0.0;
     }
+    double rhsT = 0.0;
+    if (sa.on) {
+#pragma unroll
+        for (int s = 0; s < NW; ++s) rhsT += (row[s] >= 0) ? aT[s] * dT[s] : 0.0;
+    }
     if (live) {
 #pragma unroll
         for (int m = 0; m < 5; ++m)
             __hip_atomic_store(&dq[(size_t)m * nT + c], (r[m] - U[m] - rhs0[m]) / D, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_AGENT);
+        if (sa.on) __hip_atomic_store(&sa.dq[c], (rT - UT - rhsT) / DT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // the wave's dq stores are complete before the flag that publishes them
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -1694,6 +1717,7 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     if (live) {
 #pragma unroll
         for (int m = 0; m < 5; ++m) dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
+        if (sa.on) sa.dflux[c] = 0.0 + rhsT;
     }
     return true;
 }
@@ -1712,7 +1736,7 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
                                                                   const double *__restrict__ coef,
                                                                   const double *__restrict__ res,
                                                                   const double *__restrict__ diag, double *dq,
-                                                                  double *dflux)
+                                                                  double *dflux, SaSweep sa)
 {
     const int lane = (int)threadIdx.x & 63;
     int tk;
@@ -1725,7 +1749,7 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
     } else {
         tk = blockIdx.x;
     }
-    if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, tk, lane, ctl[1],
+    if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, sa, tk, lane, ctl[1],
                                  wall_clock64()))
         if (lane == 0) atomicAdd(&ctl[2], 1);
 }
@@ -2131,16 +2155,18 @@ static const int kSmallLevel = kSweepBlock;
 
 template <int NW, int VIS, typename CT>
 static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool useU,
-                            hipStream_t s)
+                            bool withSA, hipStream_t s)
 {
     if (lv.flow.nChunks > 0) {  // built unless sweep_form is 1
+        SaSweep sa{};
+        if (withSA) sa = SaSweep{true, f.saMat, f.resT, f.specT, f.dqT, f.dfluxT};
         hipLaunchKernelGGL(k_sweep_flow_prep, dim3(1), dim3(1), 0, s, f.sweepCtl);
         if (k.sweepForm == 2)
             hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, true>), dim3(nblk(lv.flow.nChunks, 4)), dim3(256), 0, s, m,
-                               k, lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+                               k, lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa);
         else
             hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, false>), dim3(lv.flow.nChunks), dim3(64), 0, s, m, k,
-                               lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux);
+                               lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa);
         return;
     }
     int L = 0;
@@ -2161,18 +2187,18 @@ static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const
 }
 
 void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &lv, bool forward,
-                  bool firstSweep, hipStream_t s)
+                  bool firstSweep, hipStream_t s, bool withSA)
 {
     const bool useU = !(forward && firstSweep);
     const bool v = k.viscous != 0;
     if (k.coefF32) {  // only set when both directions have at most 6 slots
-        if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1, float>(m, k, f, lv, useU, s) : launch_sweep_so<3, 0, float>(m, k, f, lv, useU, s);
-        if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1, float>(m, k, f, lv, useU, s) : launch_sweep_so<4, 0, float>(m, k, f, lv, useU, s);
-        return v ? launch_sweep_so<6, 1, float>(m, k, f, lv, useU, s) : launch_sweep_so<6, 0, float>(m, k, f, lv, useU, s);
+        if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1, float>(m, k, f, lv, useU, withSA, s) : launch_sweep_so<3, 0, float>(m, k, f, lv, useU, withSA, s);
+        if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1, float>(m, k, f, lv, useU, withSA, s) : launch_sweep_so<4, 0, float>(m, k, f, lv, useU, withSA, s);
+        return v ? launch_sweep_so<6, 1, float>(m, k, f, lv, useU, withSA, s) : launch_sweep_so<6, 0, float>(m, k, f, lv, useU, withSA, s);
     }
-    if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1, double>(m, k, f, lv, useU, s) : launch_sweep_so<3, 0, double>(m, k, f, lv, useU, s);
-    if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1, double>(m, k, f, lv, useU, s) : launch_sweep_so<4, 0, double>(m, k, f, lv, useU, s);
-    if (lv.so.Wpad == 6) return v ? launch_sweep_so<6, 1, double>(m, k, f, lv, useU, s) : launch_sweep_so<6, 0, double>(m, k, f, lv, useU, s);
+    if (lv.so.Wpad == 3) return v ? launch_sweep_so<3, 1, double>(m, k, f, lv, useU, withSA, s) : launch_sweep_so<3, 0, double>(m, k, f, lv, useU, withSA, s);
+    if (lv.so.Wpad == 4) return v ? launch_sweep_so<4, 1, double>(m, k, f, lv, useU, withSA, s) : launch_sweep_so<4, 0, double>(m, k, f, lv, useU, withSA, s);
+    if (lv.so.Wpad == 6) return v ? launch_sweep_so<6, 1, double>(m, k, f, lv, useU, withSA, s) : launch_sweep_so<6, 0, double>(m, k, f, lv, useU, withSA, s);
     // wider polyhedra: face-by-face through the cell -> face lists
     int L = 0;
     while (L < lv.nLevels) {

@@ -669,16 +669,31 @@ void launch_sa_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream
                        f.specT, f.saMat);
 }
 
-void launch_sa_lusgs(const DevMesh &m, const Consts &, DevFlow &f, const Levels &fwd, const Levels &bwd,
-                     hipStream_t s)
+void launch_sa_sweep_init(const DevMesh &m, DevFlow &f, hipStream_t s)
 {
     hipLaunchKernelGGL(k_sa_sweep_init, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m.nTotal, m.nCell, f.resT,
                        f.dqT, f.dfluxT);
-    sa_sweep_dispatch(m, f, fwd, true, s);
+}
+
+void launch_sa_ghost_dq(const DevMesh &m, DevFlow &f, hipStream_t s)
+{
     if (m.nBFace > 0)
         hipLaunchKernelGGL(k_sa_ghost_dq, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, f.dqT);
-    sa_sweep_dispatch(m, f, bwd, true, s);
+}
+
+void launch_sa_update(const DevMesh &m, DevFlow &f, hipStream_t s)
+{
     hipLaunchKernelGGL(k_sa_update, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, f.nut, f.dqT);
+}
+
+void launch_sa_lusgs(const DevMesh &m, const Consts &, DevFlow &f, const Levels &fwd, const Levels &bwd,
+                     hipStream_t s)
+{
+    launch_sa_sweep_init(m, f, s);
+    sa_sweep_dispatch(m, f, fwd, true, s);
+    launch_sa_ghost_dq(m, f, s);
+    sa_sweep_dispatch(m, f, bwd, true, s);
+    launch_sa_update(m, f, s);
 }
 
 // One relaxation of the turbulence model after the NS update (the turbulence
