@@ -76,6 +76,15 @@ def test_unsupported_options_return_a_status(override):
     assert e.value.status == pa.PHL_ERR_UNSUPPORTED
 
 
+@pytest.mark.parametrize("override", [dict(sweep_form=-1), dict(sweep_form=3)])
+def test_bad_sweep_form_is_an_argument_error(override):
+    m = pa.Mesh.bump_hex(4, 2, 2, 0.05)
+    with pytest.raises(pa.PhlError) as e:
+        pa.Solver(m, pa.default_params(**override))
+    assert e.value.status == pa.PHL_ERR_ARG
+    assert "sweep_form" in str(e.value)
+
+
 def test_turbulence_batch_is_rejected_before_any_device_call():
     a, b = pa.Mesh.bump_hex(4, 2, 2, 0.05), pa.Mesh.bump_hex(4, 2, 2, 0.03)
     u = pa.Mesh.concat([a, b])
