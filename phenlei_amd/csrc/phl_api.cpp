@@ -996,6 +996,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.mixed = p.precision;
     k.coefF32 = (k.mixed && c->coefF32Ok) ? 1 : 0;
     k.sweepForm = p.sweep_form;
+    k.coefInSweep = (c->fwd.flow.nChunks > 0 && c->bwd.flow.nChunks > 0) ? 1 : 0;
     k.saNuInf = p.sa_nu_inf;
     k.saVistInf = p.sa_vist_inf;
     k.turbCflScale = p.turb_cfl_scale;
@@ -1033,7 +1034,9 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.negFlag, (size_t)nC));
     TRY(dalloc(c, &fl.negCount, 1));
     TRY(dalloc(c, &fl.negTotal, 1));
-    TRY(dalloc(c, &fl.coef, (size_t)nF * 2 * 16));
+    // LU-SGS face coefficients, stored only for the level form (the dataflow
+    // sweeps evaluate them)
+    TRY(dalloc(c, &fl.coef, k.coefInSweep ? (size_t)16 : (size_t)nF * 2 * 16));
     const int nChunks = std::max({c->fwd.flow.nChunks, c->bwd.flow.nChunks, 1});
     TRY(dalloc(c, &fl.sweepFlag, (size_t)nChunks));
     HIPTRY(hipMemset(fl.sweepFlag, 0, sizeof(int) * nChunks));

@@ -96,6 +96,7 @@ struct Consts {
     int nCases;             // > 1: batched context, per-case scalars in cases[caseOf[cell]]
     int fluxTile, gradTile; // cells per workgroup of the flux and gradient kernels (64, 128, 256)
     int coefF32;            // mixed form: LU-SGS face coefficients stored in FP32 (sweep arithmetic stays FP64)
+    int coefInSweep;        // dataflow sweeps evaluate the face coefficients themselves (none stored)
     int sweepForm;          // 0, 2: dataflow sweep (launch order, ticket order); 1: one launch per level
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     const CaseConsts *cases;

@@ -422,8 +422,9 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
             rmin = dMIN(rmin, qn[0]); rmax = dMAX(rmax, qn[0]);
             pmin = dMIN(pmin, qn[4]); pmax = dMAX(pmax, qn[4]);
         }
-        // LU-SGS coefficients of this face seen from cell c
-        {
+        // LU-SGS coefficients of this face seen from cell c (the dataflow
+        // sweeps evaluate them themselves)
+        if (!K.coefInSweep) {
             const double dist = K.viscous ? dABS(fr.nx * (M.xcc[nb] - M.xcc[c]) + fr.ny * (M.ycc[nb] - M.ycc[c]) +
                                                  fr.nz * (M.zcc[nb] - M.zcc[c]))
                                           : 0.0;
@@ -1625,8 +1626,9 @@ template <int NW, int VIS, typename CT>
 __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, const FlowOrder &fo, int *ctl,
                                            int *chunkFlag, bool useU, const double *__restrict__ coef,
                                            const double *__restrict__ res, const double *__restrict__ diag,
-                                           double *dq, double *dflux, const SaSweep &sa, int tk, int lane,
-                                           int epoch, long long t0)
+                                           double *dq, double *dflux, const SaSweep &sa,
+                                           const double *__restrict__ q, const double *__restrict__ vist, int tk,
+                                           int lane, int epoch, long long t0)
 {
     const int nT = M.nTotal, nC = M.nCell, P = 64 * fo.nChunks;
     const int p = tk * 64 + lane;
@@ -1648,8 +1650,33 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
         U[m] = useU ? dflux[(size_t)m * nC + cc] : 0.0;
     }
     double co[NW][16];
+    if (K.coefInSweep) {
+        // the face terms of k_grad_lim, evaluated here while the chunk waits:
+        // sweep_coef on the same operands, rounded to CT as a stored row is
+        const Consts Kc = case_k(K, M, cc);
+        double qc[5];
+        load5(q, nT, cc, qc);
+        const double vc = vist[cc];
 #pragma unroll
-    for (int s = 0; s < NW; ++s) load_coef_row<CT>(coef, row[s] >= 0 ? row[s] : 0, co[s]);
+        for (int s = 0; s < NW; ++s) {
+            const int rw = row[s] >= 0 ? row[s] : 0, n = row[s] >= 0 ? nb[s] : cc;
+            const FaceRec fr = M.face[rw >> 1];
+            const bool isL = (rw & 1) == 0;
+            double qn[5];
+            load5(q, nT, n, qn);
+            const double dist = Kc.viscous ? dABS(fr.nx * (M.xcc[n] - M.xcc[cc]) + fr.ny * (M.ycc[n] - M.ycc[cc]) +
+                                                  fr.nz * (M.zcc[n] - M.zcc[cc]))
+                                           : 0.0;
+            sweep_coef(Kc, qc, qn, fr, isL, dist, vc, vist[n], co[s]);
+            if constexpr (sizeof(CT) == 4) {
+#pragma unroll
+                for (int j = 0; j < 16; ++j) co[s][j] = double(float(co[s][j]));
+            }
+        }
+    } else {
+#pragma unroll
+        for (int s = 0; s < NW; ++s) load_coef_row<CT>(coef, row[s] >= 0 ? row[s] : 0, co[s]);
+    }
     double aT[NW], rT = 0.0, DT = 1.0, UT = 0.0;
     if (sa.on) {
 #pragma unroll
@@ -1736,7 +1763,9 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
                                                                   const double *__restrict__ coef,
                                                                   const double *__restrict__ res,
                                                                   const double *__restrict__ diag, double *dq,
-                                                                  double *dflux, SaSweep sa)
+                                                                  double *dflux, SaSweep sa,
+                                                                  const double *__restrict__ q,
+                                                                  const double *__restrict__ vist)
 {
     const int lane = (int)threadIdx.x & 63;
     int tk;
@@ -1749,8 +1778,8 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
     } else {
         tk = blockIdx.x;
     }
-    if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, sa, tk, lane, ctl[1],
-                                 wall_clock64()))
+    if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, sa, q, vist, tk, lane,
+                                 ctl[1], wall_clock64()))
         if (lane == 0) atomicAdd(&ctl[2], 1);
 }
 
@@ -2163,10 +2192,12 @@ static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const
         hipLaunchKernelGGL(k_sweep_flow_prep, dim3(1), dim3(1), 0, s, f.sweepCtl);
         if (k.sweepForm == 2)
             hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, true>), dim3(nblk(lv.flow.nChunks, 4)), dim3(256), 0, s, m,
-                               k, lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa);
+                               k, lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa,
+                               f.q, f.vist);
         else
             hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, false>), dim3(lv.flow.nChunks), dim3(64), 0, s, m, k,
-                               lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa);
+                               lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa,
+                               f.q, f.vist);
         return;
     }
     int L = 0;
