@@ -124,14 +124,22 @@ def flux_algorithmic_bytes(n_cell, n_face, viscous, mixed=False):
     return b
 
 
-def sweep_algorithmic_bytes(n_cell, n_face, mixed=False):
-    """Compulsory HBM bytes of one LU-SGS sweep pair: every face side is used
-    once over the two directions (its coefficient row, 128 B, or 64 B in the
-    mixed form, and the neighbour's dq, 40 B); per cell, forward reads res and
-    the diagonal and writes dq and the off-diagonal sum (128 B), backward also
-    reads that sum (168 B)."""
-    row = 64 if mixed else 128
-    return 296 * n_cell + 2 * n_face * (row + 40)
+def sweep_algorithmic_bytes(n_cell, n_face, mixed=False, viscous=False, levels_form=False):
+    """Compulsory HBM bytes of one LU-SGS sweep pair.  Every face side is used
+    once over the two directions.  Level form: the side's stored coefficient
+    row (128 B, 64 B in the mixed form) and the neighbour's dq (40 B); per
+    cell, forward reads res and the diagonal and writes dq and the
+    off-diagonal sum (128 B), backward also reads that sum (168 B).  Dataflow
+    form (the default): the sweep evaluates the coefficients, so a face side
+    reads the face record (64 B), the neighbour's state and dq (80 B), and
+    when viscous its eddy viscosity and centroid (32 B); a cell also reads
+    its own state (40 B; viscous +32 B) in each direction."""
+    if levels_form:
+        row = 64 if mixed else 128
+        return 296 * n_cell + 2 * n_face * (row + 40)
+    side = 144 + (32 if viscous else 0)
+    cell = 376 + (64 if viscous else 0)
+    return cell * n_cell + 2 * n_face * side
 
 
 def cpu_baseline(mesh, params, steps, wall_distance=None):
@@ -400,7 +408,7 @@ def main():
                 kernel="k_face_flux + k_cell_gather", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb)
     per_class = {k: round(v / max(n_timed, 1), 5) for k, v in times.items()}
     sweep_ms = (times["sweep_fwd"] + times["sweep_bwd"]) / max(n_timed, 1)
-    sb = sweep_algorithmic_bytes(mesh.n_cell, mesh.n_face, mixed)
+    sb = sweep_algorithmic_bytes(mesh.n_cell, mesh.n_face, mixed, params.viscous != 0, params.sweep_form == 1)
     sweep_gbs = sb / (sweep_ms * 1e-3) / 1e9
     levels_form = params.sweep_form == 1
     roof_sweep = dict(bound="hbm", achieved=round(sweep_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
