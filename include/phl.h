@@ -59,7 +59,8 @@ typedef struct phl_params {
     double turb_cfl_scale;  /* turbCFLScale (1.0) */
     double turb_skew_angle; /* turbSkewnessAngle, degrees (0.0) */
     int viscous;            /* viscousType: 0 inviscid, 1 laminar, 3 Spalart-Allmaras (1eq-sa) */
-    int gradient;           /* gradientName: 0 "ggcell" */
+    int gradient;           /* gradientName: 0 "ggcell" (ghost gradient 0), 1 "lsq" (weights 1/|dx|^2,
+                             * ghost takes its owner's gradient; NS only, viscous 0 or 1) */
     int limiter;            /* uns_limiter_name: 0 "vencat" (ivencat=7), 1 "nolim", 2 "1st" */
     int entropy_fix;        /* roeEntropyFixMethod (3) */
     int cfl_vary_step;      /* CFLVaryStep (500) */

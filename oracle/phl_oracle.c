@@ -48,6 +48,9 @@ struct orc_solver {
     double *q[5], *res[5], *dqdx[5], *dqdy[5], *dqdz[5], *dq[5], *dflux[5];
     double *lim, *specInv, *specVis, *dt, *diag, *gama, *t, *visl, *vist;
     double *dtdx, *dtdy, *dtdz;
+    /* least-squares gradient (gradient == 1): face weights and the inverse
+     * normal-matrix entries per cell */
+    double *iwt, *ixx, *iyy, *izz, *ixy, *ixz, *iyz;
     /* Spalart-Allmaras (viscous == 3): nu~ and its residual, spectrum,
      * face matrices, increments, gradient and the wall distance */
     double *nut, *resT, *specT, *matL, *matR, *dqT, *dfluxT, *gnx, *gny, *gnz, *wdist;
@@ -321,6 +324,51 @@ void orc_freestream(const orc_solver *s, double *prim)
     prim[IP] = s->R * 1.0 * 1.0 / 1.0;
 }
 
+/* UnstructGrid::ComputeWeight, Geo_UnstructGrid.cpp:3761-3940, 3-D branch
+ * with every face marked (fMark = 1): face weight 1/(|dx|^2 + SMALL) over
+ * the centroid difference right - left, the 3x3 normal matrix of each cell
+ * summed in face order, then its inverse (zero if the determinant is below
+ * SMALL). */
+static void lsq_weights(orc_solver *s)
+{
+    int nC = s->nCell, nF = s->nFace;
+    s->iwt = zeros(nF);
+    s->ixx = zeros(nC); s->iyy = zeros(nC); s->izz = zeros(nC);
+    s->ixy = zeros(nC); s->ixz = zeros(nC); s->iyz = zeros(nC);
+    for (int f = 0; f < nF; ++f) {
+        int le = s->left[f], re = f < s->nBFace ? f + nC : s->right[f];
+        double dx = (s->xcc[re] - s->xcc[le]) * 1.0;
+        double dy = (s->ycc[re] - s->ycc[le]) * 1.0;
+        double dz = (s->zcc[re] - s->zcc[le]) * 1.0;
+        double ods = 1.0 / (dx * dx + dy * dy + dz * dz + SMALL);
+        double txx = ods * dx * dx, tyy = ods * dy * dy, tzz = ods * dz * dz;
+        double txy = ods * dx * dy, txz = ods * dx * dz, tyz = ods * dy * dz;
+        s->iwt[f] = ods;
+        s->ixx[le] += txx; s->iyy[le] += tyy; s->izz[le] += tzz;
+        s->ixy[le] += txy; s->ixz[le] += txz; s->iyz[le] += tyz;
+        if (f < s->nBFace) continue;
+        s->ixx[re] += txx; s->iyy[re] += tyy; s->izz[re] += tzz;
+        s->ixy[re] += txy; s->ixz[re] += txz; s->iyz[re] += tyz;
+    }
+    for (int c = 0; c < nC; ++c) {
+        double txx = s->ixx[c], tyy = s->iyy[c], tzz = s->izz[c];
+        double txy = s->ixy[c], txz = s->ixz[c], tyz = s->iyz[c];
+        double delt = (txx * tyy * tzz + 2.0 * txy * txz * tyz - txx * tyz * tyz - tyy * txz * txz - tzz * txy * txy);
+        if (ABS(delt) < SMALL) {
+            s->ixx[c] = 0.0; s->iyy[c] = 0.0; s->izz[c] = 0.0;
+            s->ixy[c] = 0.0; s->ixz[c] = 0.0; s->iyz[c] = 0.0;
+        } else {
+            double odelt = 1.0 / delt;
+            s->ixx[c] = (tyy * tzz - tyz * tyz) * odelt;
+            s->iyy[c] = (txx * tzz - txz * txz) * odelt;
+            s->izz[c] = (txx * tyy - txy * txy) * odelt;
+            s->ixy[c] = (txz * tyz - txy * tzz) * odelt;
+            s->ixz[c] = (txy * tyz - txz * tyy) * odelt;
+            s->iyz[c] = (txy * txz - tyz * txx) * odelt;
+        }
+    }
+}
+
 orc_solver *orc_create(const orc_params *p, int nCell, int nBFace, int nFace,
                        const int *left, const int *right, const int *bcType,
                        const double *xfc, const double *yfc, const double *zfc,
@@ -376,6 +424,7 @@ orc_solver *orc_create(const orc_params *p, int nCell, int nBFace, int nFace,
     s->tsuth = 110.4 / p->t_ref;
     orc_freestream(s, s->primInf);
     for (int c = 0; c < nT; ++c) s->gama[c] = p->gamma;
+    lsq_weights(s);
     return s;
 }
 
@@ -395,6 +444,7 @@ void orc_destroy(orc_solver *s)
     free(s->nut); free(s->resT); free(s->specT); free(s->matL); free(s->matR); free(s->dqT); free(s->dfluxT);
     free(s->gnx); free(s->gny); free(s->gnz); free(s->wdist);
     free(s->dtdx); free(s->dtdy); free(s->dtdz);
+    free(s->iwt); free(s->ixx); free(s->iyy); free(s->izz); free(s->ixy); free(s->ixz); free(s->iyz);
     free(s);
 }
 
@@ -613,10 +663,44 @@ static void grad_ggcell(const orc_solver *s, const double *q, double *gx, double
     }
 }
 
+/* UnstructGrid::CompGradientLSQ, Geo_UnstructGrid.cpp:3995-4085: the
+ * weighted differences right - left summed into both cells in face order,
+ * times the cell's inverse normal matrix; a ghost takes its owner's gradient. */
+static void grad_lsq(const orc_solver *s, const double *q, double *gx, double *gy, double *gz)
+{
+    int nT = s->nTotal, nC = s->nCell;
+    for (int c = 0; c < nT; ++c) { gx[c] = 0.0; gy[c] = 0.0; gz[c] = 0.0; }
+    for (int f = 0; f < s->nFace; ++f) {
+        int le = s->left[f], re = f < s->nBFace ? f + nC : s->right[f];
+        double dx = s->xcc[re] - s->xcc[le], dy = s->ycc[re] - s->ycc[le], dz = s->zcc[re] - s->zcc[le];
+        double dq = s->iwt[f] * (q[re] - q[le]) * 1.0;
+        double tmpx = dq * dx, tmpy = dq * dy, tmpz = dq * dz;
+        gx[le] += tmpx; gy[le] += tmpy; gz[le] += tmpz;
+        if (f < s->nBFace) continue;
+        gx[re] += tmpx; gy[re] += tmpy; gz[re] += tmpz;
+    }
+    for (int c = 0; c < nC; ++c) {
+        double tx = gx[c], ty = gy[c], tz = gz[c];
+        gx[c] = s->ixx[c] * tx + s->ixy[c] * ty + s->ixz[c] * tz;
+        gy[c] = s->ixy[c] * tx + s->iyy[c] * ty + s->iyz[c] * tz;
+        gz[c] = s->ixz[c] * tx + s->iyz[c] * ty + s->izz[c] * tz;
+    }
+    for (int f = 0; f < s->nBFace; ++f) {
+        int le = s->left[f], re = f + nC;
+        gx[re] = gx[le]; gy[re] = gy[le]; gz[re] = gz[le];
+    }
+}
+
 /* NSSolverUnstruct::GetGradientField, NSSolverUnstruct.cpp:18244-18338:
- * gradients of the five primitives and of the temperature. */
+ * gradients of the five primitives and of the temperature (gradientName
+ * "ggcell" or "lsq", :18266-18272, :18308-18314). */
 void orc_gradient(orc_solver *s)
 {
+    if (s->p.gradient == 1) {
+        for (int m = 0; m < 5; ++m) grad_lsq(s, s->q[m], s->dqdx[m], s->dqdy[m], s->dqdz[m]);
+        grad_lsq(s, s->t, s->dtdx, s->dtdy, s->dtdz);
+        return;
+    }
     for (int m = 0; m < 5; ++m) grad_ggcell(s, s->q[m], s->dqdx[m], s->dqdy[m], s->dqdz[m]);
     grad_ggcell(s, s->t, s->dtdx, s->dtdy, s->dtdz);
 }

@@ -280,7 +280,10 @@ phl_status check_params(const phl_params &p)
         return fail(PHL_ERR_UNSUPPORTED, "viscous must be 0 (inviscid), 1 (laminar) or 3 (Spalart-Allmaras)");
     if (p.viscous == 3 && !(p.turb_cfl_scale > 0.0)) return fail(PHL_ERR_ARG, "turb_cfl_scale must be positive");
     if (p.viscous && !(p.reynolds > 0.0)) return fail(PHL_ERR_ARG, "reynolds must be positive for viscous flow");
-    if (p.gradient != 0) return fail(PHL_ERR_UNSUPPORTED, "only the ggcell gradient is implemented");
+    if (p.gradient != 0 && p.gradient != 1)
+        return fail(PHL_ERR_UNSUPPORTED, "gradient must be 0 (ggcell) or 1 (lsq)");
+    if (p.gradient == 1 && p.viscous == 3)
+        return fail(PHL_ERR_UNSUPPORTED, "the lsq gradient is implemented for the NS equations only (viscous 0, 1)");
     if (p.limiter < 0 || p.limiter > 2) return fail(PHL_ERR_UNSUPPORTED, "limiter must be 0 (vencat), 1 (nolim) or 2 (1st)");
     if (p.entropy_fix != 1 && p.entropy_fix != 3 && p.entropy_fix != 5)
         return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1, 3 or 5");
@@ -784,6 +787,45 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     std::vector<phl::FaceRec> faces(nF);
     for (int f = 0; f < nF; ++f)
         faces[fslot[f]] = {m.xfn[f], m.yfn[f], m.zfn[f], m.area[f], m.xfc[f], m.yfc[f], m.zfc[f], 0.0};
+    // Least-squares weights and inverse normal matrices (UnstructGrid::ComputeWeight,
+    // Geo_UnstructGrid.cpp:3761-3940, 3-D, every face marked): per face
+    // 1/(|x_R - x_L|^2 + SMALL), per cell the 3x3 normal matrix summed in
+    // face order, inverted by cofactors (zero below SMALL).
+    std::vector<double> lsqInvS;  // uploaded with the other cell arrays
+    if (p.gradient == 1) {
+        std::vector<double> inv((size_t)6 * nC, 0.0);
+        double *ixx = inv.data(), *iyy = ixx + nC, *izz = iyy + nC, *ixy = izz + nC, *ixz = ixy + nC, *iyz = ixz + nC;
+        for (int f = 0; f < nF; ++f) {
+            const int le = m.left[f], re = m.right[f];
+            const double dx = (m.xcc[re] - m.xcc[le]) * 1.0, dy = (m.ycc[re] - m.ycc[le]) * 1.0,
+                         dz = (m.zcc[re] - m.zcc[le]) * 1.0;
+            const double ods = 1.0 / (dx * dx + dy * dy + dz * dz + 1.0e-40);
+            const double txx = ods * dx * dx, tyy = ods * dy * dy, tzz = ods * dz * dz;
+            const double txy = ods * dx * dy, txz = ods * dx * dz, tyz = ods * dy * dz;
+            faces[fslot[f]].lsqW = ods;
+            ixx[le] += txx; iyy[le] += tyy; izz[le] += tzz; ixy[le] += txy; ixz[le] += txz; iyz[le] += tyz;
+            if (f < nB) continue;
+            ixx[re] += txx; iyy[re] += tyy; izz[re] += tzz; ixy[re] += txy; ixz[re] += txz; iyz[re] += tyz;
+        }
+        std::vector<double> invS((size_t)6 * nC);
+        for (int i = 0; i < nC; ++i) {
+            const double txx = ixx[i], tyy = iyy[i], tzz = izz[i], txy = ixy[i], txz = ixz[i], tyz = iyz[i];
+            const double delt = (txx * tyy * tzz + 2.0 * txy * txz * tyz - txx * tyz * tyz - tyy * txz * txz -
+                                 tzz * txy * txy);
+            double o[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+            if (!(std::fabs(delt) < 1.0e-40)) {
+                const double odelt = 1.0 / delt;
+                o[0] = (tyy * tzz - tyz * tyz) * odelt;
+                o[1] = (txx * tzz - txz * txz) * odelt;
+                o[2] = (txx * tyy - txy * txy) * odelt;
+                o[3] = (txz * tyz - txy * tzz) * odelt;
+                o[4] = (txy * tyz - txz * tyy) * odelt;
+                o[5] = (txy * txz - tyz * txx) * odelt;
+            }
+            for (int k = 0; k < 6; ++k) invS[(size_t)k * nC + slotOf[i]] = o[k];
+        }
+        lsqInvS = std::move(invS);
+    }
 
     // Venkatakrishnan scale and CFL skewness factor (geometry only).
     std::vector<double> vf(nC), cflSkew(nC);
@@ -888,6 +930,8 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(upload(c, &dm.logicalOf, cellAt));
     TRY(upload(c, &dm.cellAt, slotOf));
     TRY(upload(c, &dm.venkatVf, vf));
+    dm.lsqInv = nullptr;
+    if (!lsqInvS.empty()) TRY(upload(c, &dm.lsqInv, lsqInvS));
     TRY(upload(c, &dm.cflSkew, cflSkew));
     TRY(upload(c, &dm.limZero, limZero));
     dm.limGhost = nullptr;
@@ -995,6 +1039,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.nSweeps = p.n_lusgs_sweeps;
     k.mixed = p.precision;
     k.coefF32 = (k.mixed && c->coefF32Ok) ? 1 : 0;
+    k.gradient = p.gradient;
     k.sweepForm = p.sweep_form;
     k.coefInSweep = (c->fwd.flow.nChunks > 0 && c->bwd.flow.nChunks > 0) ? 1 : 0;
     k.saNuInf = p.sa_nu_inf;

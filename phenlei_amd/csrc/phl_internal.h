@@ -20,7 +20,8 @@
 namespace phl {
 
 struct FaceRec {
-    double nx, ny, nz, area, xc, yc, zc, pad;
+    double nx, ny, nz, area, xc, yc, zc;
+    double lsqW;  // least-squares weight 1/(|x_R - x_L|^2 + SMALL) (gradient 1), else 0
 };
 
 // Read-only mesh data on the device.
@@ -35,6 +36,7 @@ struct DevMesh {
     const double *xcc, *ycc, *zcc;  // [nTotal]
     const double *vol;          // [nTotal]
     const double *venkatVf;     // [nCell] (vol^(1/3))^3, Limiter.cpp:1126-1128
+    const double *lsqInv;       // [6][nCell] least-squares inverse normal matrix xx, yy, zz, xy, xz, yz (gradient 1)
     const double *cflSkew;      // [nCell] skewness CFL factor, NSSolverUnstruct.cpp:6106-6116
     const unsigned char *limZero;  // [nCell] 1 if next to a farfield-type face
     const double *limGhost;     // [nBFace] limiter value of the ghost cell
@@ -97,6 +99,7 @@ struct Consts {
     int fluxTile, gradTile; // cells per workgroup of the flux and gradient kernels (64, 128, 256)
     int coefF32;            // mixed form: LU-SGS face coefficients stored in FP32 (sweep arithmetic stays FP64)
     int coefInSweep;        // dataflow sweeps evaluate the face coefficients themselves (none stored)
+    int gradient;           // 0 Green-Gauss cell, 1 least squares
     int sweepForm;          // 0, 2: dataflow sweep (launch order, ticket order); 1: one launch per level
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     const CaseConsts *cases;

@@ -403,20 +403,38 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
         const int f = (isL ? enc : -enc) - 1;
         const int nb = M.c2n[(size_t)s * nC + c];
         const FaceRec fr = M.face[f];
-        const double nx = fr.nx * fr.area, ny = fr.ny * fr.area, nz = fr.nz * fr.area;
         double qn[5];
         load5(q, nT, nb, qn);
+        if (K.gradient == 1) {
+            // CompGradientLSQ (Geo_UnstructGrid.cpp:4027-4064): w (q_R - q_L) (x_R - x_L)
+            // in the face's own orientation, summed in the cell's face order
+            const double dx = isL ? M.xcc[nb] - M.xcc[c] : M.xcc[c] - M.xcc[nb];
+            const double dy = isL ? M.ycc[nb] - M.ycc[c] : M.ycc[c] - M.ycc[nb];
+            const double dz = isL ? M.zcc[nb] - M.zcc[c] : M.zcc[c] - M.zcc[nb];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) {
-            const double qfc = 0.5 * (qc[m] + qn[m]);
-            if (isL) { gx[m] += qfc * nx; gy[m] += qfc * ny; gz[m] += qfc * nz; }
-            else     { gx[m] -= qfc * nx; gy[m] -= qfc * ny; gz[m] -= qfc * nz; }
-        }
-        if (K.viscous) {
-            const double tn = qn[4] / (K.R * qn[0]);
-            const double tfc = 0.5 * (tc + tn);
-            if (isL) { tgx += tfc * nx; tgy += tfc * ny; tgz += tfc * nz; }
-            else     { tgx -= tfc * nx; tgy -= tfc * ny; tgz -= tfc * nz; }
+            for (int m = 0; m < 5; ++m) {
+                const double dq = fr.lsqW * (isL ? qn[m] - qc[m] : qc[m] - qn[m]);
+                gx[m] += dq * dx; gy[m] += dq * dy; gz[m] += dq * dz;
+            }
+            if (K.viscous) {
+                const double tn = qn[4] / (K.R * qn[0]);
+                const double dq = fr.lsqW * (isL ? tn - tc : tc - tn);
+                tgx += dq * dx; tgy += dq * dy; tgz += dq * dz;
+            }
+        } else {
+            const double nx = fr.nx * fr.area, ny = fr.ny * fr.area, nz = fr.nz * fr.area;
+#pragma unroll
+            for (int m = 0; m < 5; ++m) {
+                const double qfc = 0.5 * (qc[m] + qn[m]);
+                if (isL) { gx[m] += qfc * nx; gy[m] += qfc * ny; gz[m] += qfc * nz; }
+                else     { gx[m] -= qfc * nx; gy[m] -= qfc * ny; gz[m] -= qfc * nz; }
+            }
+            if (K.viscous) {
+                const double tn = qn[4] / (K.R * qn[0]);
+                const double tfc = 0.5 * (tc + tn);
+                if (isL) { tgx += tfc * nx; tgy += tfc * ny; tgz += tfc * nz; }
+                else     { tgx -= tfc * nx; tgy -= tfc * ny; tgz -= tfc * nz; }
+            }
         }
         if (nb < nC) {
             rmin = dMIN(rmin, qn[0]); rmax = dMAX(rmax, qn[0]);
@@ -445,9 +463,22 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
         }
     }
     const double ovol = 1.0 / M.vol[c];
+    double lxx = 0.0, lyy = 0.0, lzz = 0.0, lxy = 0.0, lxz = 0.0, lyz = 0.0;
+    if (K.gradient == 1) {
+        lxx = M.lsqInv[c]; lyy = M.lsqInv[(size_t)nC + c]; lzz = M.lsqInv[(size_t)2 * nC + c];
+        lxy = M.lsqInv[(size_t)3 * nC + c]; lxz = M.lsqInv[(size_t)4 * nC + c]; lyz = M.lsqInv[(size_t)5 * nC + c];
 #pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        gx[m] *= ovol; gy[m] *= ovol; gz[m] *= ovol;
+        for (int m = 0; m < 5; ++m) {
+            const double tx = gx[m], ty = gy[m], tz = gz[m];
+            gx[m] = lxx * tx + lxy * ty + lxz * tz;
+            gy[m] = lxy * tx + lyy * ty + lyz * tz;
+            gz[m] = lxz * tx + lyz * ty + lzz * tz;
+        }
+    } else {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            gx[m] *= ovol; gy[m] *= ovol; gz[m] *= ovol;
+        }
     }
     if (gradSoA) {  // for the force integration (the iteration reads the gather records)
 #pragma unroll
@@ -458,7 +489,14 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
         }
     }
     if (K.viscous) {
-        tgx *= ovol; tgy *= ovol; tgz *= ovol;
+        if (K.gradient == 1) {
+            const double tx = tgx, ty = tgy, tz = tgz;
+            tgx = lxx * tx + lxy * ty + lxz * tz;
+            tgy = lxy * tx + lyy * ty + lyz * tz;
+            tgz = lxz * tx + lyz * ty + lzz * tz;
+        } else {
+            tgx *= ovol; tgy *= ovol; tgz *= ovol;
+        }
         gradT[c] = GT(tgx); gradT[(size_t)nC + c] = GT(tgy); gradT[(size_t)2 * nC + c] = GT(tgz);
     }
     double l;
@@ -502,6 +540,17 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
 #pragma unroll
         for (int m = 0; m < 5; ++m) { g[m] = GT(gx[m]); g[5 + m] = GT(gy[m]); g[10 + m] = GT(gz[m]); }
         r.lim = l;
+        if (K.gradient == 1) {  // a ghost takes its owner's least-squares gradient (Geo_UnstructGrid.cpp:4077-4084)
+            for (int s = 0; s < M.ellW; ++s) {
+                const int enc = M.c2fEnc[(size_t)s * nC + c];
+                if (enc == 0) break;
+                const int nb = M.c2n[(size_t)s * nC + c];
+                if (nb < nC) continue;
+                GT *gg = reinterpret_cast<GT *>(rec[nb].g);
+#pragma unroll
+                for (int m = 0; m < 15; ++m) gg[m] = g[m];
+            }
+        }
     }
 }
 
@@ -847,6 +896,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                     T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
                     if (li < nC) { gTL[0] = gradT[li]; gTL[1] = gradT[(size_t)nC + li]; gTL[2] = gradT[(size_t)2 * nC + li]; }
                     if (ri < nC) { gTR[0] = gradT[ri]; gTR[1] = gradT[(size_t)nC + ri]; gTR[2] = gradT[(size_t)2 * nC + ri]; }
+                    else if (K.gradient == 1) { gTR[0] = gTL[0]; gTR[1] = gTL[1]; gTR[2] = gTL[2]; }  // ghost: owner's
                     const bool wall = bnd && M.bcType[f] == kBC_WALL;
                     double fl[5];
                     viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[li], vist[ri], fl);
@@ -990,6 +1040,7 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
         T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
         if (le < nC) { gTL[0] = gradT[le]; gTL[1] = gradT[(size_t)nC + le]; gTL[2] = gradT[(size_t)2 * nC + le]; }
         if (re < nC) { gTR[0] = gradT[re]; gTR[1] = gradT[(size_t)nC + re]; gTR[2] = gradT[(size_t)2 * nC + re]; }
+        else if (K.gradient == 1) { gTR[0] = gTL[0]; gTR[1] = gTL[1]; gTR[2] = gTL[2]; }  // ghost: owner's lsq gradient
         const bool wall = bnd && bc == kBC_WALL;
         double vis[6];
         viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[le], vist[re], vis);

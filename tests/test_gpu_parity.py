@@ -433,3 +433,46 @@ def test_dataflow_sweep_batch_equals_level_sweep():
         out.append((s.iterate(1, 12), s.get_state()))
     assert np.array_equal(out[0][0], out[1][0])
     assert np.array_equal(out[0][1], out[1][1])
+
+
+@pytest.mark.parametrize("kind,viscous", [("hex", 0), ("hybrid", 0), ("hybrid", 1)])
+def test_least_squares_gradient_parity(kind, viscous):
+    """gradientName "lsq": per-call gradients, temperature gradients,
+    limiter, residual and one LU-SGS step against the oracle (ghosts take
+    their owner's gradient, which the boundary faces' reconstruction and
+    viscous terms read); then iterations."""
+    pover = dict(mach=0.3, viscous=viscous, reynolds=5e5, cfl_start=5.0, cfl_end=5.0, gradient=1)
+    if kind == "hex":
+        m, gpu, cpu = make_pair(16, 8, 4, bump=0.08, **pover)
+    else:
+        m, gpu, cpu = make_hybrid_pair(**pover)
+    q = seeded(m, cpu, 41 + viscous)
+    gpu.set_state(q)
+    cpu.set("q", q)
+    gpu.update_residuals()
+    cpu.update_residuals()
+    nc = m.n_cell
+    for name in ("dqdx", "dqdy", "dqdz"):
+        g, c = gpu.get_field(name)[:, :nc], cpu.get(name)[:, :nc]
+        assert rel_err(g, c, 1e-12 * np.max(np.abs(c))) <= TOL, name
+    if viscous:
+        for name in ("dtdx", "dtdy", "dtdz"):
+            g, c = gpu.get_field(name)[:nc], cpu.get(name)[:nc]
+            assert rel_err(g, c, 1e-12 * np.max(np.abs(c))) <= TOL, name
+    assert np.max(np.abs(gpu.get_field("limit") - cpu.get("limit"))) <= 1e-13
+    rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
+    floor = 1e-3 * np.max(np.abs(rc), axis=1, keepdims=True)
+    err_r = rel_err(rg, rc, floor)
+    print(kind, viscous, "lsq residual rel err", err_r, "bitwise:", np.array_equal(rg, rc))
+    assert err_r <= TOL
+    gpu.time_step(2)
+    cpu.spectrum()
+    cpu.time_step(2)
+    assert gpu.lusgs() == cpu.lusgs()
+    qg, qc = gpu.get_state()[:, :nc], cpu.get("q")[:, :nc]
+    assert rel_err(qg, qc, 1e-300) <= TOL
+    gpu.init_freestream()
+    cpu.init_freestream()
+    hg = gpu.iterate(1, 8)
+    hc, _ = cpu.iterate(1, 8)
+    assert np.max(np.abs(hg - hc) / hc) <= 1e-10

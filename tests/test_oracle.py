@@ -199,6 +199,58 @@ def test_green_gauss_exact_for_linear_fields():
         assert np.max(np.abs(g - coef[:, col:col + 1])) <= 1e-12, name
 
 
+@pytest.mark.parametrize("kind", ["bump", "hybrid"])
+def test_least_squares_exact_for_linear_fields_in_every_cell(kind):
+    """Least squares (gradientName "lsq") recovers a linear field's gradient in
+    every real cell of a distorted mesh, boundary cells included, when the
+    ghosts carry the same linear field; ghosts take their owner's gradient."""
+    m = pa.Mesh.bump_hex(12, 6, 4, 0.08) if kind == "bump" else pa.Mesh.bump_hybrid(8, 3, 6, 1e-3, 6, 0.08, 5)
+    met = m.metrics()
+    s = oracle_for(m, gradient=1)
+    coef = np.array([[1.0, 0.3, -0.2, 0.1], [0.2, -0.5, 0.7, 0.3], [0.1, 0.2, 0.4, -0.6],
+                     [-0.3, 0.1, 0.2, 0.9], [2.0, 0.4, -0.1, 0.25]])
+    q = coef[:, :1] + coef[:, 1:2] * met["xcc"] + coef[:, 2:3] * met["ycc"] + coef[:, 3:4] * met["zcc"]
+    s.set("q", q)
+    s.gradient()
+    nc = m.n_cell
+    owner = m.connectivity()["left"][:m.n_bface]
+    for name, col in (("dqdx", 1), ("dqdy", 2), ("dqdz", 3)):
+        g = s.get(name)
+        assert np.max(np.abs(g[:, :nc] - coef[:, col:col + 1])) <= 1e-11, name
+        assert np.array_equal(g[:, nc:], g[:, owner]), name
+
+
+def test_least_squares_solves_the_weighted_normal_equations():
+    """Each cell's lsq gradient minimises sum_f w_f (g . d_f - dq_f)^2 over its
+    faces, w_f = 1/|d_f|^2 (d_f: centroid difference to the neighbour or
+    ghost): checked against numpy's solve of the 3x3 normal equations on a
+    random field."""
+    m = pa.Mesh.bump_hybrid(8, 3, 6, 1e-3, 6, 0.08, 5)
+    conn, met = m.connectivity(), m.metrics()
+    s = oracle_for(m, gradient=1)
+    rng = np.random.default_rng(3)
+    q = 1.0 + 0.1 * rng.random((5, m.n_total))
+    s.set("q", q)
+    s.gradient()
+    g = np.stack([s.get("dqdx")[0], s.get("dqdy")[0], s.get("dqdz")[0]])
+    xc = np.stack([met["xcc"], met["ycc"], met["zcc"]])
+    left, right = conn["left"], conn["right"]
+    A = np.zeros((m.n_cell, 3, 3))
+    b = np.zeros((m.n_cell, 3))
+    for f in range(m.n_face):
+        le, re = left[f], right[f]
+        d = xc[:, re] - xc[:, le]
+        w = 1.0 / (d @ d)
+        for c, sgn in ((le, 1.0), (re, 1.0)):
+            if c >= m.n_cell:
+                continue
+            A[c] += w * np.outer(d, d)
+            b[c] += w * d * (q[0, re] - q[0, le]) * sgn
+    ref = np.linalg.solve(A, b[..., None])[..., 0].T
+    err = np.max(np.abs(g[:, :m.n_cell] - ref)) / np.max(np.abs(ref))
+    assert err <= 1e-10, err
+
+
 def test_metrics_closure_and_volume():
     m = pa.Mesh.bump_hex(16, 8, 4, 0.05)
     conn, met = m.connectivity(), m.metrics()
