@@ -3,7 +3,7 @@ compressible residual + LU-SGS path, behind the C-ABI declared in include/phl.h.
 
 This module is a thin ctypes binding over libphl.so.  It is the host-side
 mirror used by tests and bench.py; a C++ host solver calls the same C-ABI
-directly (see INTEGRATION.md).  There is no CPU fallback: if libphl.so is
+directly.  There is no CPU fallback: if libphl.so is
 missing or does not load, importing the solver raises.
 """
 import ctypes

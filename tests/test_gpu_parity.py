@@ -1,6 +1,6 @@
 """GPU parity: every hook of the path against the CPU oracle on identical
 meshes and states.  FP64 bar: per-call residuals within 1e-12 relative to the
-local flux magnitude (the north-star bar); the kernels are written to round
+local flux magnitude; the kernels are written to round
 like the oracle, so most fields agree exactly."""
 import numpy as np
 import pytest
