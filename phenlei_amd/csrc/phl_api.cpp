@@ -226,7 +226,13 @@ phl_status build_flow_order(phl_ctx *c, int nC, const std::vector<int> &cells, c
         }
     }
     if (dep.empty()) dep.push_back(0);  // keeps the device pointer valid
+    std::vector<int> width(std::max(nCh, 1), 0);
+    for (int k = 0; k < nCh; ++k)
+        for (int j = 0; j < 64; ++j)
+            for (int s = 0; s < Wa; ++s)
+                if (frow[(size_t)s * Pp + (size_t)k * 64 + j] >= 0) width[k] = std::max(width[k], s + 1);
     const int *t;
+    TRY(upload(c, &t, width)); fo.width = t;
     TRY(upload(c, &t, fc)); fo.cells = t;
     TRY(upload(c, &t, frow)); fo.row = t;
     TRY(upload(c, &t, fnb)); fo.nb = t;

@@ -184,6 +184,7 @@ struct FlowOrder {
     const int *nb;      // [Wpad][64*nChunks] as SweepOrder::nb
     const int *depPtr;  // [nChunks+1]
     const int *dep;     // the distinct earlier chunks holding a chunk's real neighbours
+    const int *width;   // [nChunks] most active slots of any cell of the chunk (slots are packed first)
 };
 
 // The scalar Spalart-Allmaras sweep carried along by a dataflow NS sweep.

@@ -1686,12 +1686,20 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     const int c = fo.cells[p];
     const bool live = c >= 0;
     const int cc = live ? c : 0;
+    // Active slots come first in every list, so slots at or beyond the
+    // chunk's widest cell are unused by all 64 cells: they are skipped
+    // (a skipped slot would add +0.0 to a sum that is never -0.0).
+    const int width = __builtin_amdgcn_readfirstlane(fo.width[tk]);
     // everything that does not depend on the neighbours' dq is loaded first
     int row[NW], nb[NW];
 #pragma unroll
     for (int s = 0; s < NW; ++s) {
-        row[s] = fo.row[(size_t)s * P + p];
-        nb[s] = fo.nb[(size_t)s * P + p];
+        row[s] = -1;
+        nb[s] = 0;
+        if (s < width) {
+            row[s] = fo.row[(size_t)s * P + p];
+            nb[s] = fo.nb[(size_t)s * P + p];
+        }
     }
     const double D = diag[cc];
     double r[5], U[5];
@@ -1710,6 +1718,7 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
         const double vc = vist[cc];
 #pragma unroll
         for (int s = 0; s < NW; ++s) {
+            if (s >= width) break;
             const int rw = row[s] >= 0 ? row[s] : 0, n = row[s] >= 0 ? nb[s] : cc;
             const FaceRec fr = M.face[rw >> 1];
             const bool isL = (rw & 1) == 0;
@@ -1726,12 +1735,13 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
         }
     } else {
 #pragma unroll
-        for (int s = 0; s < NW; ++s) load_coef_row<CT>(coef, row[s] >= 0 ? row[s] : 0, co[s]);
+        for (int s = 0; s < NW; ++s)
+            if (s < width) load_coef_row<CT>(coef, row[s] >= 0 ? row[s] : 0, co[s]);
     }
     double aT[NW], rT = 0.0, DT = 1.0, UT = 0.0;
     if (sa.on) {
 #pragma unroll
-        for (int s = 0; s < NW; ++s) aT[s] = sa.mat[row[s] >= 0 ? row[s] : 0];
+        for (int s = 0; s < NW; ++s) aT[s] = s < width ? sa.mat[row[s] >= 0 ? row[s] : 0] : 0.0;
         rT = sa.res[cc];
         DT = sa.spec[cc];
         UT = sa.dflux[cc];
@@ -1756,6 +1766,7 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     double dqn[NW][5];
 #pragma unroll
     for (int s = 0; s < NW; ++s) {
+        if (s >= width) break;
         const int i = row[s] >= 0 ? nb[s] : 0;
 #pragma unroll
         for (int m = 0; m < 5; ++m) dqn[s][m] = dq_load(&dq[(size_t)m * nT + i]);
@@ -1763,11 +1774,12 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     double dT[NW];
     if (sa.on) {
 #pragma unroll
-        for (int s = 0; s < NW; ++s) dT[s] = dq_load(&sa.dq[row[s] >= 0 ? nb[s] : 0]);
+        for (int s = 0; s < NW; ++s) dT[s] = s < width ? dq_load(&sa.dq[row[s] >= 0 ? nb[s] : 0]) : 0.0;
     }
     double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
     for (int s = 0; s < NW; ++s) {
+        if (s >= width) break;
         double df[5];
         sweep_apply<VIS>(K, co[s], dqn[s], df);
         const bool act = row[s] >= 0;
@@ -1777,7 +1789,8 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     double rhsT = 0.0;
     if (sa.on) {
 #pragma unroll
-        for (int s = 0; s < NW; ++s) rhsT += (row[s] >= 0) ? aT[s] * dT[s] : 0.0;
+        for (int s = 0; s < NW; ++s)
+            if (s < width) rhsT += (row[s] >= 0) ? aT[s] * dT[s] : 0.0;
     }
     if (live) {
 #pragma unroll
