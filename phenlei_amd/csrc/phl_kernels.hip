@@ -971,10 +971,35 @@ __device__ __forceinline__ void store6(double *dst, const double *v)
     d[2] = make_double2(v[4], v[5]);
 }
 
+// Compact face value (mixed form, 32 bytes): the five flux components are
+// FP32 results widened to FP64, so they are stored as FP32 without loss; the
+// spectral-radius term is FP64.
+struct alignas(32) FaceValF32 {
+    float f[6];  // f[5] unused
+    double s;
+};
+__device__ __forceinline__ void store_compact(double *dst, const double *v)
+{
+    FaceValF32 *d = reinterpret_cast<FaceValF32 *>(dst);
+    float4 *a = reinterpret_cast<float4 *>(d->f);
+    a[0] = make_float4(float(v[0]), float(v[1]), float(v[2]), float(v[3]));
+    reinterpret_cast<double2 *>(d)[1] = make_double2(__longlong_as_double((long long)__float_as_uint(float(v[4]))),
+                                                     v[5]);
+}
+__device__ __forceinline__ void load_compact(const double *src, double *fl, double &spec)
+{
+    const float4 a = reinterpret_cast<const float4 *>(src)[0];
+    const double2 b = reinterpret_cast<const double2 *>(src)[1];
+    fl[0] = double(a.x); fl[1] = double(a.y); fl[2] = double(a.z); fl[3] = double(a.w);
+    fl[4] = double(__uint_as_float((unsigned)__double_as_longlong(b.x)));
+    spec = b.y;
+}
+
 // The stored values of face f: invOut[0..4] the area-scaled inviscid flux,
 // invOut[5] the inviscid spectral-radius term; visOut[] likewise (viscous
-// flow only).  Each group is stored as soon as it is known.
-template <typename T>
+// flow only).  Each group is stored as soon as it is known, as six doubles
+// or (COMPACT, FP32 faces) as a FaceValF32.
+template <typename T, bool COMPACT = false>
 __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, const GatherRec *__restrict__ rec,
                                             const T *__restrict__ gradT, const double *__restrict__ vist, int f,
                                             double *invOut, double *visOut)
@@ -1034,7 +1059,8 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
 #pragma unroll
         for (int m = 0; m < 5; ++m) inv[m] = double(fl[m] * area);
         inv[5] = 0.5 * fr.area * (dABS(vn) + cm);
-        store6(invOut, inv);
+        if constexpr (COMPACT) store_compact(invOut, inv);
+        else store6(invOut, inv);
     }
     if (K.viscous) {
         T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
@@ -1056,7 +1082,8 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
         double sr = 2.0 * mu / (rho * ds * K.reynolds + kSMALL);
         sr *= 0.5 * fr.area;
         vis[5] = sr;
-        store6(visOut, vis);
+        if constexpr (COMPACT) store_compact(visOut, vis);
+        else store6(visOut, vis);
     }
 }
 
@@ -1067,13 +1094,16 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const G
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= M.nFace) return;
-    face_values<T>(M, K0, rec, gradT, vist, f, faceInv + (size_t)f * 6, faceVis + (size_t)f * 6);
+    // FP32 faces are stored compact (32 bytes a face and group)
+    constexpr bool C = sizeof(T) == 4;
+    constexpr int stride = C ? 4 : 6;
+    face_values<T, C>(M, K0, rec, gradT, vist, f, faceInv + (size_t)f * stride, faceVis + (size_t)f * stride);
 }
 
 // Per cell: the face values summed in the cell's face order (every inviscid
 // contribution, then every viscous one: the reference's LoadFlux order),
 // spectral radii, local time step and the block partials of res^2 and min dt.
-template <int BS>
+template <int BS, bool COMPACT>
 __global__ void __launch_bounds__(BS) k_cell_gather(DevMesh M, Consts K0, const double *__restrict__ q,
                                                     const double *__restrict__ faceInv,
                                                     const double *__restrict__ faceVis, double *__restrict__ res,
@@ -1095,9 +1125,15 @@ __global__ void __launch_bounds__(BS) k_cell_gather(DevMesh M, Consts K0, const 
             if (enc == 0) break;
             const bool isL = enc > 0;
             const int f = (isL ? enc : -enc) - 1;
-            const double2 *src = reinterpret_cast<const double2 *>(faceInv + (size_t)f * 6);
-            const double2 a = src[0], b = src[1], e = src[2];
-            const double fl[5] = {a.x, a.y, b.x, b.y, e.x};
+            double fl[5], sp;
+            if constexpr (COMPACT) {
+                load_compact(faceInv + (size_t)f * 4, fl, sp);
+            } else {
+                const double2 *src = reinterpret_cast<const double2 *>(faceInv + (size_t)f * 6);
+                const double2 a = src[0], b = src[1], e = src[2];
+                fl[0] = a.x; fl[1] = a.y; fl[2] = b.x; fl[3] = b.y; fl[4] = e.x;
+                sp = e.y;
+            }
             if (isL) {
 #pragma unroll
                 for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
@@ -1105,7 +1141,7 @@ __global__ void __launch_bounds__(BS) k_cell_gather(DevMesh M, Consts K0, const 
 #pragma unroll
                 for (int m = 0; m < 5; ++m) acc[m] += fl[m];
             }
-            sInv += e.y;
+            sInv += sp;
         }
         specInv[c] = sInv;
         if (K.viscous) {
@@ -1114,9 +1150,15 @@ __global__ void __launch_bounds__(BS) k_cell_gather(DevMesh M, Consts K0, const 
                 if (enc == 0) break;
                 const bool isL = enc > 0;
                 const int f = (isL ? enc : -enc) - 1;
-                const double2 *src = reinterpret_cast<const double2 *>(faceVis + (size_t)f * 6);
-                const double2 a = src[0], b = src[1], e = src[2];
-                const double fl[5] = {a.x, a.y, b.x, b.y, e.x};
+                double fl[5], sp;
+                if constexpr (COMPACT) {
+                    load_compact(faceVis + (size_t)f * 4, fl, sp);
+                } else {
+                    const double2 *src = reinterpret_cast<const double2 *>(faceVis + (size_t)f * 6);
+                    const double2 a = src[0], b = src[1], e = src[2];
+                    fl[0] = a.x; fl[1] = a.y; fl[2] = b.x; fl[3] = b.y; fl[4] = e.x;
+                    sp = e.y;
+                }
                 if (isL) {
 #pragma unroll
                     for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
@@ -1124,7 +1166,7 @@ __global__ void __launch_bounds__(BS) k_cell_gather(DevMesh M, Consts K0, const 
 #pragma unroll
                     for (int m = 0; m < 5; ++m) acc[m] += fl[m];
                 }
-                sVis += e.y;
+                sVis += sp;
             }
             specVis[c] = sVis;
         }
@@ -2195,8 +2237,14 @@ static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStre
     else
         hipLaunchKernelGGL(k_face_flux<double>, dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradT, f.vist, f.faceInv,
                            f.faceVis);
-    hipLaunchKernelGGL((k_cell_gather<BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.faceInv, f.faceVis,
-                       f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc, f.dtMinPart, FUSED_DT);
+    if (k.mixed)  // the FP32 face kernel stored compact values
+        hipLaunchKernelGGL((k_cell_gather<BS, true>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.faceInv,
+                           f.faceVis, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
+                           f.dtMinPart, FUSED_DT);
+    else
+        hipLaunchKernelGGL((k_cell_gather<BS, false>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.faceInv,
+                           f.faceVis, f.res, f.specInv, f.specVis, f.normPart, f.cflArr, f.stepCtr, f.dtLoc,
+                           f.dtMinPart, FUSED_DT);
 }
 
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt)
