@@ -476,3 +476,25 @@ def test_least_squares_gradient_parity(kind, viscous):
     hg = gpu.iterate(1, 8)
     hc, _ = cpu.iterate(1, 8)
     assert np.max(np.abs(hg - hc) / hc) <= 1e-10
+
+
+@pytest.mark.parametrize("dims,viscous,form", [((1, 1, 1), 0, 0), ((2, 1, 1), 1, 0), ((1, 2, 3), 0, 0),
+                                               ((1, 1, 1), 0, 1), ((1, 2, 3), 1, 2)])
+def test_tiny_meshes_match_oracle(dims, viscous, form):
+    """Edge sizes: one cell with only boundary faces (no interior face, one
+    level, one chunk), two and six cells; every sweep form; against the
+    oracle over a few iterations."""
+    m, gpu, cpu = make_pair(*dims, mach=0.4, viscous=viscous, reynolds=1e4, cfl_start=2.0, cfl_end=2.0,
+                            sweep_form=form)
+    gpu.init_freestream()
+    cpu.init_freestream()
+    q = seeded(m, cpu, 7)
+    gpu.set_state(q)
+    cpu.set("q", q)
+    hg = gpu.iterate(1, 4)
+    hc, _ = cpu.iterate(1, 4)
+    nc = m.n_cell
+    assert np.all(np.isfinite(hg))
+    assert np.max(np.abs(hg - hc) / np.maximum(hc, 1e-300)) <= 1e-10
+    qg, qc = gpu.get_state()[:, :nc], cpu.get("q")[:, :nc]
+    assert np.max(np.abs(qg - qc) / np.maximum(np.abs(qc), 1e-300)) <= 1e-12
