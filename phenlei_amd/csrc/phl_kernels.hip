@@ -581,6 +581,16 @@ __device__ __forceinline__ void load_side(const DevMesh &M, const double *__rest
     }
 }
 
+// State and centroid of cell or ghost i from its gather record (written for
+// the step by k_grad_lim and k_boundary): the record's first 64 bytes.
+__device__ __forceinline__ void load_rec_state(const GatherRec *__restrict__ rec, int i, double *qv, double *xv)
+{
+    const double2 *v = reinterpret_cast<const double2 *>(&rec[i]);
+    const double2 a = v[0], b = v[1], c = v[2], d = v[3];
+    qv[0] = a.x; qv[1] = a.y; qv[2] = b.x; qv[3] = b.y; qv[4] = c.x;
+    xv[0] = c.y; xv[1] = d.x; xv[2] = d.y;
+}
+
 template <typename T>
 __device__ __forceinline__ void load_side_rec(const GatherRec *__restrict__ rec, int i, bool needGrad, CellSide<T> &s)
 {
@@ -1720,7 +1730,7 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
                                            int *chunkFlag, bool useU, const double *__restrict__ coef,
                                            const double *__restrict__ res, const double *__restrict__ diag,
                                            double *dq, double *dflux, const SaSweep &sa,
-                                           const double *__restrict__ q, const double *__restrict__ vist, int tk,
+                                           const GatherRec *__restrict__ rec, const double *__restrict__ vist, int tk,
                                            int lane, int epoch, long long t0)
 {
     const int nT = M.nTotal, nC = M.nCell, P = 64 * fo.nChunks;
@@ -1753,10 +1763,12 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     double co[NW][16];
     if (K.coefInSweep) {
         // the face terms of k_grad_lim, evaluated here while the chunk waits:
-        // sweep_coef on the same operands, rounded to CT as a stored row is
+        // sweep_coef on the same operands, rounded to CT as a stored row is.
+        // States and centroids come from the gather records (the first 64
+        // bytes of a record: one line per neighbour instead of eight).
         const Consts Kc = case_k(K, M, cc);
-        double qc[5];
-        load5(q, nT, cc, qc);
+        double qc[5], xc[3];
+        load_rec_state(rec, cc, qc, xc);
         const double vc = vist[cc];
 #pragma unroll
         for (int s = 0; s < NW; ++s) {
@@ -1764,10 +1776,10 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
             const int rw = row[s] >= 0 ? row[s] : 0, n = row[s] >= 0 ? nb[s] : cc;
             const FaceRec fr = M.face[rw >> 1];
             const bool isL = (rw & 1) == 0;
-            double qn[5];
-            load5(q, nT, n, qn);
-            const double dist = Kc.viscous ? dABS(fr.nx * (M.xcc[n] - M.xcc[cc]) + fr.ny * (M.ycc[n] - M.ycc[cc]) +
-                                                  fr.nz * (M.zcc[n] - M.zcc[cc]))
+            double qn[5], xn[3];
+            load_rec_state(rec, n, qn, xn);
+            const double dist = Kc.viscous ? dABS(fr.nx * (xn[0] - xc[0]) + fr.ny * (xn[1] - xc[1]) +
+                                                  fr.nz * (xn[2] - xc[2]))
                                            : 0.0;
             sweep_coef(Kc, qc, qn, fr, isL, dist, vc, vist[n], co[s]);
             if constexpr (sizeof(CT) == 4) {
@@ -1870,7 +1882,7 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
                                                                   const double *__restrict__ res,
                                                                   const double *__restrict__ diag, double *dq,
                                                                   double *dflux, SaSweep sa,
-                                                                  const double *__restrict__ q,
+                                                                  const GatherRec *__restrict__ rec,
                                                                   const double *__restrict__ vist)
 {
     const int lane = (int)threadIdx.x & 63;
@@ -1884,7 +1896,7 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
     } else {
         tk = blockIdx.x;
     }
-    if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, sa, q, vist, tk, lane,
+    if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, sa, rec, vist, tk, lane,
                                  ctl[1], wall_clock64()))
         if (lane == 0) atomicAdd(&ctl[2], 1);
 }
@@ -2305,11 +2317,11 @@ static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const
         if (k.sweepForm == 2)
             hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, true>), dim3(nblk(lv.flow.nChunks, 4)), dim3(256), 0, s, m,
                                k, lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa,
-                               f.q, f.vist);
+                               f.rec, f.vist);
         else
             hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, false>), dim3(lv.flow.nChunks), dim3(64), 0, s, m, k,
                                lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa,
-                               f.q, f.vist);
+                               f.rec, f.vist);
         return;
     }
     int L = 0;
