@@ -325,6 +325,12 @@ def main():
     ap.add_argument("--warm-start", action="store_true",
                     help="also run the learned warm start end to end on bump-channel cases (32 training cases, "
                          "4 held out) and time mixed + warm start against plain FP64 to a 1e-8 drop")
+    ap.add_argument("--ws-cases", default="32,4", metavar="TRAIN,HELD",
+                    help="warm start: training and held-out case counts")
+    ap.add_argument("--ws-dims", default="24,12,4", metavar="NX,NY,NZ",
+                    help="warm start: bump-channel grid of every case")
+    ap.add_argument("--ws-steps", default="1500,1500,1500,1500", metavar="N,...",
+                    help="warm start: Adam steps per learning-rate stage (2e-3, 1e-3, 3e-4, 1e-4)")
     ap.add_argument("--precision-compare", type=int, default=None, metavar="STEPS",
                     help="also run STEPS iterations in FP64 and in the mixed form and compare CL/CD and fields")
     ap.add_argument("--face-loop", default=None, choices=["tile", "split"],
@@ -439,7 +445,12 @@ def main():
     }
     if args.warm_start:
         from phenlei_amd import warmstart as ws
-        line["warm_start"] = ws.compare(log=lambda m: print("[bench] warm start: " + m, file=sys.stderr, flush=True))
+        n_train, n_held = (int(v) for v in args.ws_cases.split(","))
+        dims = tuple(int(v) for v in args.ws_dims.split(","))
+        stages = [int(v) for v in args.ws_steps.split(",")]
+        lrs = [2e-3, 1e-3, 3e-4, 1e-4][:len(stages)]
+        line["warm_start"] = ws.compare(n_train=n_train, n_held=n_held, dims=dims, train_steps=stages, train_lr=lrs,
+                                        log=lambda m: print("[bench] warm start: " + m, file=sys.stderr, flush=True))
     if args.precision_compare is not None and not cfg.get("batch"):
         solver.close()
         line["precision_compare"] = precision_compare(mesh, pover, args.precision_compare)
