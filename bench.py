@@ -308,6 +308,61 @@ def batch_time_to_drop(mesh, params_list, drop, max_steps, stall=2048, forms=("f
     return out
 
 
+def mpnn_dense_flops(layers, train):
+    """Dense-layer FLOPs per cell of the warm-start network (hidden width
+    128, 8 inputs, 5 outputs): the input layer, per layer h A + m B, the
+    output layer; a training step adds, per layer, the weight gradients of A
+    and B and the products dZ A^T and dZ B^T, plus the input- and
+    output-layer gradients.  Neighbour means, ReLU, loss and Adam are not
+    counted (they are HBM-bound and small)."""
+    H, I, O = 128, 8, 5
+    fwd = 2 * I * H + layers * 2 * (2 * H * H) + 2 * H * O
+    if not train:
+        return fwd
+    return fwd + layers * 4 * (2 * H * H) + 2 * I * H + 2 * (2 * H * O)
+
+
+def mpnn_throughput(n_cases, dims, steps, layers=6):
+    """The warm-start network at inference and training scale: n_cases
+    bump-channel hex cases of dims cells as one union graph, synthetic
+    features and targets.  Times `steps` forward passes (each copies the
+    prediction y [cells, 5] to the host) and `steps` Adam steps, and prices
+    both by their dense-layer FLOPs against the FP32 MFMA peak."""
+    print(f"[bench] mpnn: building {n_cases} cases of {dims}", file=sys.stderr, flush=True)
+    meshes = [pa.Mesh.bump_hex(*dims, 0.02 + 0.06 * i / max(n_cases - 1, 1)) for i in range(n_cases)]
+    union = pa.Mesh.concat(meshes)
+    n = union.n_cell
+    rng = np.random.default_rng(5)
+    net = pa.Mpnn(layers=layers, seed=0, device=0)
+    net.set_graph(mesh=union)
+    net.set_data(rng.standard_normal((n, net.N_IN)).astype(np.float32),
+                 0.1 * rng.standard_normal((n, net.N_OUT)).astype(np.float32))
+    print(f"[bench] mpnn: {n} cells; forward x {steps}", file=sys.stderr, flush=True)
+    net.forward()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward()
+    t_fwd = (time.perf_counter() - t0) / steps
+    print(f"[bench] mpnn: forward {t_fwd * 1e3:.1f} ms; training x {steps}", file=sys.stderr, flush=True)
+    net.train(1, 1e-4)
+    t0 = time.perf_counter()
+    net.train(steps, 1e-4)
+    t_train = (time.perf_counter() - t0) / steps
+    print(f"[bench] mpnn: training step {t_train * 1e3:.1f} ms", file=sys.stderr, flush=True)
+    net.close()
+    peak = 157.3e12
+    out = dict(workload=f"{n_cases} bump-channel hex cases of {dims[0]}x{dims[1]}x{dims[2]} as one graph, "
+                        f"{layers} layers, width 128, FP32 (MFMA 32x32x2 f32), synthetic features",
+               cells=n, steps=steps)
+    for key, t, train in (("forward", t_fwd, False), ("train_step", t_train, True)):
+        fl = mpnn_dense_flops(layers, train) * n
+        out[key] = dict(ms=round(t * 1e3, 3), cells_per_s=n / t, dense_tflop=fl / 1e12,
+                        achieved_tflops=round(fl / t / 1e12, 2), peak_tflops=peak / 1e12,
+                        frac=round(fl / t / peak, 4))
+    out["forward"]["note"] = "wall time per call, including the copy of y to the host"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -331,6 +386,9 @@ def main():
                     help="warm start: bump-channel grid of every case")
     ap.add_argument("--ws-steps", default="1500,1500,1500,1500", metavar="N,...",
                     help="warm start: Adam steps per learning-rate stage (2e-3, 1e-3, 3e-4, 1e-4)")
+    ap.add_argument("--mpnn", default=None, metavar="CASES,NX,NY,NZ",
+                    help="also time the warm-start network at this scale (e.g. 64,64,56,56: 64 x 200,704 cells)")
+    ap.add_argument("--mpnn-steps", type=int, default=5, help="forward passes and Adam steps timed by --mpnn")
     ap.add_argument("--precision-compare", type=int, default=None, metavar="STEPS",
                     help="also run STEPS iterations in FP64 and in the mixed form and compare CL/CD and fields")
     ap.add_argument("--face-loop", default=None, choices=["tile", "split"],
@@ -451,6 +509,10 @@ def main():
         lrs = [2e-3, 1e-3, 3e-4, 1e-4][:len(stages)]
         line["warm_start"] = ws.compare(n_train=n_train, n_held=n_held, dims=dims, train_steps=stages, train_lr=lrs,
                                         log=lambda m: print("[bench] warm start: " + m, file=sys.stderr, flush=True))
+    if args.mpnn:
+        solver.close()
+        c, nx, ny, nz = (int(v) for v in args.mpnn.split(","))
+        line["mpnn"] = mpnn_throughput(c, (nx, ny, nz), args.mpnn_steps)
     if args.precision_compare is not None and not cfg.get("batch"):
         solver.close()
         line["precision_compare"] = precision_compare(mesh, pover, args.precision_compare)
