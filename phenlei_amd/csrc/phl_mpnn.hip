@@ -26,6 +26,7 @@ constexpr int KC = 32;        // K chunk staged in LDS
 constexpr int TR = 128;       // rows per workgroup tile
 constexpr int kThreads = 256; // 4 waves
 constexpr int kRowChunk = 2048;  // rows per partial of a weight gradient
+constexpr int kMaxDeg = 8;       // neighbour lists up to this width take the batched-load path
 
 // D(32x32) += A(32x2) * B(2x32).  Lane l holds A[l % 32][l / 32] and
 // B[l / 32][l % 32]; accumulator register r of lane l holds
@@ -36,25 +37,45 @@ __device__ __forceinline__ int acc_row(int r, int lane) { return (r & 3) + 4 * (
 // W given row-major [K, 128], or transposed ([128, K], TW) for the backward
 // products dZ W^T.  Epilogue: z = C stored to Z (if non-null); out = z
 // (mode 0), relu(z) (mode 1), Hres + relu(z) (mode 2) or Hres + z (mode 3).
+// Each thread stages kPer elements of a chunk: all its global loads are
+// issued before the first LDS store, so they are in flight together.
+constexpr int kPer = TR * KC / kThreads;  // 16, and KC * H / kThreads too
+static_assert(TR * KC == KC * H, "A and W chunks have the same size");
+
 template <bool TW>
 __device__ __forceinline__ void stage_w(float (*Wl)[H + 4], const float *__restrict__ W, int K, int k0)
 {
-    for (int e = threadIdx.x; e < KC * H; e += kThreads) {
+    float v[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int e = threadIdx.x + i * kThreads;
         const int kk = e / H, c = e % H;
         const int k = k0 + kk;
-        float v = 0.f;
-        if (k < K) v = TW ? W[(size_t)c * K + k] : W[(size_t)k * H + c];
-        Wl[kk][c] = v;
+        v[i] = 0.f;
+        if (k < K) v[i] = TW ? W[(size_t)c * K + k] : W[(size_t)k * H + c];
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int e = threadIdx.x + i * kThreads;
+        Wl[e / H][e % H] = v[i];
     }
 }
 
 __device__ __forceinline__ void stage_a(float (*Al)[KC + 1], const float *__restrict__ A, int N, int K, int row0,
                                         int k0)
 {
-    for (int e = threadIdx.x; e < TR * KC; e += kThreads) {
+    float v[kPer];
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int e = threadIdx.x + i * kThreads;
         const int r = e / KC, kk = e % KC;
         const int row = row0 + r, k = k0 + kk;
-        Al[r][kk] = (row < N && k < K) ? A[(size_t)row * K + k] : 0.f;
+        v[i] = (row < N && k < K) ? A[(size_t)row * K + k] : 0.f;
+    }
+#pragma unroll
+    for (int i = 0; i < kPer; ++i) {
+        const int e = threadIdx.x + i * kThreads;
+        Al[e / KC][e % KC] = v[i];
     }
 }
 
@@ -82,7 +103,7 @@ __device__ __forceinline__ void accumulate(f32x16 (&acc)[4], float (*Al)[KC + 1]
 }
 
 template <bool TW>
-__global__ void __launch_bounds__(kThreads) k_dense(const float *__restrict__ A1, const float *__restrict__ W1, int K1,
+__global__ void __launch_bounds__(kThreads, 2) k_dense(const float *__restrict__ A1, const float *__restrict__ W1, int K1,
                                                     const float *__restrict__ A2, const float *__restrict__ W2,
                                                     const float *__restrict__ bias, int N, int mode,
                                                     const float *__restrict__ Hres, float *__restrict__ Z,
@@ -129,13 +150,21 @@ __global__ void __launch_bounds__(kThreads) k_wgrad_part(const float *__restrict
 #pragma unroll
         for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
     for (int s = 0; s < kRowChunk; s += 32) {
-        __syncthreads();
-        for (int e = threadIdx.x; e < 32 * H; e += kThreads) {
-            const int n = e / H, c = e % H;
-            const int row = r0 + s + n;
+        float va[kPer], vd[kPer];
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = threadIdx.x + i * kThreads;
+            const int row = r0 + s + e / H;
             const bool in = row < N;
-            Al[n][c] = in ? A[(size_t)row * H + c] : 0.f;
-            Dl[n][c] = in ? D[(size_t)row * H + c] : 0.f;
+            va[i] = in ? A[(size_t)row * H + e % H] : 0.f;
+            vd[i] = in ? D[(size_t)row * H + e % H] : 0.f;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < kPer; ++i) {
+            const int e = threadIdx.x + i * kThreads;
+            Al[e / H][e % H] = va[i];
+            Dl[e / H][e % H] = vd[i];
         }
         __syncthreads();
         for (int n = 0; n < 32; n += 2) {
@@ -194,10 +223,26 @@ __global__ void __launch_bounds__(kThreads) k_neighbour(const int *__restrict__ 
     if (e >= (size_t)N * H) return;
     const int i = (int)(e / H), c = (int)(e % H);
     float s = 0.f;
-    for (int k = 0; k < W; ++k) {
-        const int j = nbr[(size_t)k * N + i];
-        if (j < 0) break;
-        s += TRANSPOSE ? h[(size_t)j * H + c] * invDeg[j] : h[(size_t)j * H + c];
+    if (W <= kMaxDeg) {
+        // every index, then every row value, in flight together; the lists
+        // are compacted (valid entries first), so the sum order is k = 0, 1, ..
+        int j[kMaxDeg];
+        float x[kMaxDeg];
+#pragma unroll
+        for (int k = 0; k < kMaxDeg; ++k) j[k] = k < W ? nbr[(size_t)k * N + i] : -1;
+#pragma unroll
+        for (int k = 0; k < kMaxDeg; ++k) x[k] = j[k] >= 0 ? h[(size_t)j[k] * H + c] : 0.f;
+#pragma unroll
+        for (int k = 0; k < kMaxDeg; ++k) {
+            if (j[k] < 0) break;
+            s += TRANSPOSE ? x[k] * invDeg[j[k]] : x[k];
+        }
+    } else {
+        for (int k = 0; k < W; ++k) {
+            const int jk = nbr[(size_t)k * N + i];
+            if (jk < 0) break;
+            s += TRANSPOSE ? h[(size_t)jk * H + c] * invDeg[jk] : h[(size_t)jk * H + c];
+        }
     }
     if (TRANSPOSE) out[e] += s;
     else out[e] = s * invDeg[i];

@@ -62,13 +62,15 @@ def test_gradient_matches_oracle():
         assert err <= 1e-4, k
 
 
-def test_ragged_explicit_graph():
-    """Explicit neighbour lists with isolated cells and uneven degree."""
+@pytest.mark.parametrize("w,pairs", [(5, 500), (12, 1200)])
+def test_ragged_explicit_graph(w, pairs):
+    """Explicit neighbour lists with isolated cells and uneven degree; width
+    12 takes the neighbour kernels' path for lists wider than 8."""
     rng = np.random.default_rng(4)
-    n, w = 300, 5
+    n = 300
     nbr = -np.ones((n, w), np.int64)
     deg = np.zeros(n, int)
-    for _ in range(500):
+    for _ in range(pairs):
         i, j = rng.integers(0, n, 2)
         if i != j and deg[i] < w and deg[j] < w and j not in nbr[i]:
             nbr[i, deg[i]] = j
