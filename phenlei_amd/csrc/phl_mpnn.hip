@@ -37,59 +37,80 @@ __device__ __forceinline__ int acc_row(int r, int lane) { return (r & 3) + 4 * (
 // W given row-major [K, 128], or transposed ([128, K], TW) for the backward
 // products dZ W^T.  Epilogue: z = C stored to Z (if non-null); out = z
 // (mode 0), relu(z) (mode 1), Hres + relu(z) (mode 2) or Hres + z (mode 3).
-// Each thread stages kPer elements of a chunk: all its global loads are
-// issued before the first LDS store, so they are in flight together.
-constexpr int kPer = TR * KC / kThreads;  // 16, and KC * H / kThreads too
-static_assert(TR * KC == KC * H, "A and W chunks have the same size");
+// One thread's share of a K chunk, as float4: A [TR][KC] and W [KC][H] are
+// 1024 float4 each, 4 per thread.  K is a multiple of 4 (kIn or kHidden), so
+// a float4 is either wholly inside K or wholly past it; rows and weight
+// offsets are multiples of 4 floats, so every float4 load is aligned.
+static_assert(kIn % 4 == 0 && H % 4 == 0, "float4 staging needs K % 4 == 0");
+constexpr int kVec = TR * KC / 4 / kThreads;  // 4
+static_assert(TR * KC == KC * H && kVec * 4 * kThreads == TR * KC, "A and W chunks have the same size");
+
+struct Chunk {
+    float4 a[kVec], w[kVec];
+};
 
 template <bool TW>
-__device__ __forceinline__ void stage_w(float (*Wl)[H + 4], const float *__restrict__ W, int K, int k0)
+__device__ __forceinline__ void load_chunk(Chunk &st, const float *__restrict__ A, const float *__restrict__ W,
+                                           int N, int K, int row0, int k0)
 {
-    float v[kPer];
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int e = threadIdx.x + i * kThreads;
-        const int kk = e / H, c = e % H;
-        const int k = k0 + kk;
-        v[i] = 0.f;
-        if (k < K) v[i] = TW ? W[(size_t)c * K + k] : W[(size_t)k * H + c];
-    }
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int e = threadIdx.x + i * kThreads;
-        Wl[e / H][e % H] = v[i];
+    for (int i = 0; i < kVec; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        const int row = row0 + (q >> 3), k = k0 + 4 * (q & 7);  // A: row q/8, k 4(q%8)..+3
+        st.a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < N && k < K) st.a[i] = *(const float4 *)(A + (size_t)row * K + k);
+        st.w[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (TW) {  // W^T stored [H][K]: column q/8, k 4(q%8)..+3
+            if (k < K) st.w[i] = *(const float4 *)(W + (size_t)(q >> 3) * K + k);
+        } else {   // W stored [K][H]: k q/32, columns 4(q%32)..+3
+            const int kw = k0 + (q >> 5);
+            if (kw < K) st.w[i] = *(const float4 *)(W + (size_t)kw * H + 4 * (q & 31));
+        }
     }
 }
 
-__device__ __forceinline__ void stage_a(float (*Al)[KC + 1], const float *__restrict__ A, int N, int K, int row0,
-                                        int k0)
+template <bool TW>
+__device__ __forceinline__ void store_chunk(const Chunk &st, float (*Al)[KC + 1], float (*Wl)[H + 4])
 {
-    float v[kPer];
 #pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int e = threadIdx.x + i * kThreads;
-        const int r = e / KC, kk = e % KC;
-        const int row = row0 + r, k = k0 + kk;
-        v[i] = (row < N && k < K) ? A[(size_t)row * K + k] : 0.f;
-    }
-#pragma unroll
-    for (int i = 0; i < kPer; ++i) {
-        const int e = threadIdx.x + i * kThreads;
-        Al[e / KC][e % KC] = v[i];
+    for (int i = 0; i < kVec; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        const int r = q >> 3, kk = 4 * (q & 7);
+        Al[r][kk] = st.a[i].x;
+        Al[r][kk + 1] = st.a[i].y;
+        Al[r][kk + 2] = st.a[i].z;
+        Al[r][kk + 3] = st.a[i].w;
+        if (TW) {
+            Wl[kk][r] = st.w[i].x;
+            Wl[kk + 1][r] = st.w[i].y;
+            Wl[kk + 2][r] = st.w[i].z;
+            Wl[kk + 3][r] = st.w[i].w;
+        } else {
+            const int kw = q >> 5, c = 4 * (q & 31);
+            Wl[kw][c] = st.w[i].x;
+            Wl[kw][c + 1] = st.w[i].y;
+            Wl[kw][c + 2] = st.w[i].z;
+            Wl[kw][c + 3] = st.w[i].w;
+        }
     }
 }
 
+// The next chunk's loads are issued before the MFMAs of the current one and
+// land in registers while they run; the LDS tiles are rewritten between two
+// barriers.
 template <bool TW>
 __device__ __forceinline__ void accumulate(f32x16 (&acc)[4], float (*Al)[KC + 1], float (*Wl)[H + 4],
                                            const float *__restrict__ A, const float *__restrict__ W, int N, int K,
                                            int row0)
 {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    Chunk st;
+    load_chunk<TW>(st, A, W, N, K, row0, 0);
     for (int k0 = 0; k0 < K; k0 += KC) {
         __syncthreads();
-        stage_a(Al, A, N, K, row0, k0);
-        stage_w<TW>(Wl, W, K, k0);
+        store_chunk<TW>(st, Al, Wl);
         __syncthreads();
+        load_chunk<TW>(st, A, W, N, K, row0, k0 + KC);  // past K: zeros, no loads
         const int kmax = (K - k0 < KC) ? (K - k0) : KC;
         for (int kk = 0; kk < kmax; kk += 2) {
             const float a = Al[wave * 32 + (lane & 31)][kk + (lane >> 5)];
@@ -134,6 +155,8 @@ __global__ void __launch_bounds__(kThreads, 2) k_dense(const float *__restrict__
             out[o] = mode == 0 ? z : (mode == 1 ? rz : (mode == 2 ? Hres[o] + rz : Hres[o] + z));
         }
 }
+
+constexpr int kPer = 32 * H / kThreads;  // elements of a 32-row slab staged per thread
 
 // Partial of G[K, 128] = A[rows, K]^T D[rows, 128] over one chunk of
 // kRowChunk rows; K = 128.  Wave w owns G rows 32w..32w+31.
