@@ -237,38 +237,71 @@ __global__ void __launch_bounds__(kThreads) k_sum_parts(const float *__restrict_
 // m_i = mean of h_j over the neighbours of i (scale = 1/deg_i), or, for the
 // transpose in the backward pass, out_j += sum over neighbours i of
 // w_i * h_i with w_i = 1/deg_i (TRANSPOSE).
+// One thread per (cell, 4 columns), float4 rows; each column is summed in
+// list order, as a scalar loop over the neighbours would sum it.
+template <bool TRANSPOSE>
+__device__ __forceinline__ void add_row(float4 &s, const float4 x, const float *__restrict__ invDeg, int j)
+{
+    if (TRANSPOSE) {
+        const float w = invDeg[j];
+        s.x += x.x * w;
+        s.y += x.y * w;
+        s.z += x.z * w;
+        s.w += x.w * w;
+    } else {
+        s.x += x.x;
+        s.y += x.y;
+        s.z += x.z;
+        s.w += x.w;
+    }
+}
+
 template <bool TRANSPOSE>
 __global__ void __launch_bounds__(kThreads) k_neighbour(const int *__restrict__ nbr, int W, int N,
                                                         const float *__restrict__ invDeg,
                                                         const float *__restrict__ h, float *__restrict__ out)
 {
+    constexpr int Q = H / 4;
     const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= (size_t)N * H) return;
-    const int i = (int)(e / H), c = (int)(e % H);
-    float s = 0.f;
+    if (e >= (size_t)N * Q) return;
+    const int i = (int)(e / Q), c = 4 * (int)(e % Q);
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
     if (W <= kMaxDeg) {
-        // every index, then every row value, in flight together; the lists
-        // are compacted (valid entries first), so the sum order is k = 0, 1, ..
+        // every index, then every row, in flight together; the lists are
+        // compacted (valid entries first), so the sum order is k = 0, 1, ..
         int j[kMaxDeg];
-        float x[kMaxDeg];
+        float4 x[kMaxDeg];
 #pragma unroll
         for (int k = 0; k < kMaxDeg; ++k) j[k] = k < W ? nbr[(size_t)k * N + i] : -1;
 #pragma unroll
-        for (int k = 0; k < kMaxDeg; ++k) x[k] = j[k] >= 0 ? h[(size_t)j[k] * H + c] : 0.f;
+        for (int k = 0; k < kMaxDeg; ++k) {
+            x[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (j[k] >= 0) x[k] = *(const float4 *)(h + (size_t)j[k] * H + c);
+        }
 #pragma unroll
         for (int k = 0; k < kMaxDeg; ++k) {
             if (j[k] < 0) break;
-            s += TRANSPOSE ? x[k] * invDeg[j[k]] : x[k];
+            add_row<TRANSPOSE>(s, x[k], invDeg, j[k]);
         }
     } else {
         for (int k = 0; k < W; ++k) {
             const int jk = nbr[(size_t)k * N + i];
             if (jk < 0) break;
-            s += TRANSPOSE ? h[(size_t)jk * H + c] * invDeg[jk] : h[(size_t)jk * H + c];
+            add_row<TRANSPOSE>(s, *(const float4 *)(h + (size_t)jk * H + c), invDeg, jk);
         }
     }
-    if (TRANSPOSE) out[e] += s;
-    else out[e] = s * invDeg[i];
+    float4 *o = (float4 *)(out + (size_t)i * H + c);
+    if (TRANSPOSE) {
+        float4 v = *o;
+        v.x += s.x;
+        v.y += s.y;
+        v.z += s.z;
+        v.w += s.w;
+        *o = v;
+    } else {
+        const float w = invDeg[i];
+        *o = make_float4(s.x * w, s.y * w, s.z * w, s.w * w);
+    }
 }
 
 // dz = dh * (z > 0)
@@ -394,14 +427,14 @@ void small_grad(const float *A, int K, const float *D, int C, int N, float *part
 
 void neighbour_mean(const int *nbr, int W, int N, const float *invDeg, const float *h, float *m, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_neighbour<false>, dim3(nb((size_t)N * H, kThreads)), dim3(kThreads), 0, s, nbr, W, N,
+    hipLaunchKernelGGL(k_neighbour<false>, dim3(nb((size_t)N * (H / 4), kThreads)), dim3(kThreads), 0, s, nbr, W, N,
                        invDeg, h, m);
 }
 
 void neighbour_mean_transpose_add(const int *nbr, int W, int N, const float *invDeg, const float *dm, float *dh,
                                   hipStream_t s)
 {
-    hipLaunchKernelGGL(k_neighbour<true>, dim3(nb((size_t)N * H, kThreads)), dim3(kThreads), 0, s, nbr, W, N,
+    hipLaunchKernelGGL(k_neighbour<true>, dim3(nb((size_t)N * (H / 4), kThreads)), dim3(kThreads), 0, s, nbr, W, N,
                        invDeg, dm, dh);
 }
 
