@@ -8,8 +8,8 @@
 //     y  = h Wo + bo                           y [N, 5]
 //
 // Rows are cells, columns features, all row-major.  The hidden width is 128.
-// Every reduction (weight gradients, bias gradients, loss) is a fixed-order
-// two-stage sum, so training is deterministic.
+// Every reduction (weight gradients, bias gradients, loss) is a sum in a
+// fixed order (no atomics), so training is deterministic.
 #include "phl_mpnn_internal.h"
 
 #include <hip/hip_runtime.h>
@@ -223,15 +223,27 @@ __global__ void __launch_bounds__(kThreads) k_small_grad_part(const float *__res
     }
 }
 
-// out = sum of nPart partials of size n, added in partial order.
+// out = sum of nPart partials of size n.  A workgroup owns 64 consecutive
+// outputs (lane = output); wave w adds partials w, w + 4, w + 8, .. in order
+// and the four wave sums are added in wave order: a fixed order, so the
+// result is deterministic.
+constexpr int kSumWaves = kThreads / 64;
 __global__ void __launch_bounds__(kThreads) k_sum_parts(const float *__restrict__ part, int nPart, int n,
                                                         float *__restrict__ out)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n) return;
+    __shared__ float red[kSumWaves][64];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int e = blockIdx.x * 64 + lane;
     float s = 0.f;
-    for (int i = 0; i < nPart; ++i) s += part[(size_t)i * n + e];
-    out[e] = s;
+    if (e < n)
+        for (int i = wave; i < nPart; i += kSumWaves) s += part[(size_t)i * n + e];
+    red[wave][lane] = s;
+    __syncthreads();
+    if (wave == 0 && e < n) {
+        float t = red[0][lane];
+        for (int w = 1; w < kSumWaves; ++w) t += red[w][lane];
+        out[e] = t;
+    }
 }
 
 // m_i = mean of h_j over the neighbours of i (scale = 1/deg_i), or, for the
@@ -415,14 +427,14 @@ void wgrad(const float *A, const float *D, int N, float *part, float *out, hipSt
 {
     const int np = grad_parts(N);
     hipLaunchKernelGGL(k_wgrad_part, dim3(np), dim3(kThreads), 0, s, A, D, N, part);
-    hipLaunchKernelGGL(k_sum_parts, dim3(nb(H * H, kThreads)), dim3(kThreads), 0, s, part, np, H * H, out);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nb(H * H, 64)), dim3(kThreads), 0, s, part, np, H * H, out);
 }
 
 void small_grad(const float *A, int K, const float *D, int C, int N, float *part, float *out, hipStream_t s)
 {
     const int np = grad_parts(N);
     hipLaunchKernelGGL(k_small_grad_part, dim3(np), dim3(kThreads), 0, s, A, K, D, C, N, part);
-    hipLaunchKernelGGL(k_sum_parts, dim3(nb(K * C, kThreads)), dim3(kThreads), 0, s, part, np, K * C, out);
+    hipLaunchKernelGGL(k_sum_parts, dim3(nb(K * C, 64)), dim3(kThreads), 0, s, part, np, K * C, out);
 }
 
 void neighbour_mean(const int *nbr, int W, int N, const float *invDeg, const float *h, float *m, hipStream_t s)
