@@ -218,6 +218,7 @@ __global__ void __launch_bounds__(kThreads) k_small_grad_part(const float *__res
     for (int e = threadIdx.x; e < K * C; e += kThreads) {
         const int k = e / C, c = e % C;
         float s = 0.f;
+#pragma unroll 8
         for (int r = r0; r < r1; ++r) s += (A ? A[(size_t)r * K + k] : 1.f) * D[(size_t)r * C + c];
         p[e] = s;
     }
