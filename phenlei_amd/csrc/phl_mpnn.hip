@@ -317,12 +317,20 @@ __global__ void __launch_bounds__(kThreads) k_neighbour(const int *__restrict__ 
     }
 }
 
-// dz = dh * (z > 0)
+// dz = dh * (z > 0): thread t does elements 4t..4t+3 (float4 when all four
+// are in range; the arrays are rows of 128 floats, so only a tail past n
+// would not be), scalar for a tail.
 __global__ void __launch_bounds__(kThreads) k_relu_back(const float *__restrict__ dh, const float *__restrict__ z,
                                                         size_t n, float *__restrict__ dz)
 {
-    const size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e < n) dz[e] = z[e] > 0.f ? dh[e] : 0.f;
+    const size_t e = 4 * ((size_t)blockIdx.x * blockDim.x + threadIdx.x);
+    if (e + 4 <= n) {
+        const float4 zv = *(const float4 *)(z + e), hv = *(const float4 *)(dh + e);
+        *(float4 *)(dz + e) = make_float4(zv.x > 0.f ? hv.x : 0.f, zv.y > 0.f ? hv.y : 0.f,
+                                          zv.z > 0.f ? hv.z : 0.f, zv.w > 0.f ? hv.w : 0.f);
+    } else {
+        for (size_t k = e; k < n; ++k) dz[k] = z[k] > 0.f ? dh[k] : 0.f;
+    }
 }
 
 __global__ void __launch_bounds__(kThreads) k_add(float *__restrict__ a, const float *__restrict__ b, size_t n)
@@ -331,16 +339,31 @@ __global__ void __launch_bounds__(kThreads) k_add(float *__restrict__ a, const f
     if (e < n) a[e] += b[e];
 }
 
-// Output layer: y = h Wo + bo, one thread per (row, output).
+// Output layer: y = h Wo + bo, one thread per row (float4 reads of h), each
+// output summed over k = 0..127 in order.
 __global__ void __launch_bounds__(kThreads) k_out(const float *__restrict__ h, const float *__restrict__ Wo,
                                                   const float *__restrict__ bo, int N, float *__restrict__ y)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= N * kOut) return;
-    const int i = e / kOut, o = e % kOut;
-    float s = bo[o];
-    for (int k = 0; k < H; ++k) s += h[(size_t)i * H + k] * Wo[k * kOut + o];
-    y[e] = s;
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N) return;
+    float s[kOut];
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) s[o] = bo[o];
+    const float4 *hr = (const float4 *)(h + (size_t)i * H);
+    for (int q = 0; q < H / 4; ++q) {
+        const float4 x = hr[q];
+        const float *w = Wo + 4 * q * kOut;
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) s[o] += x.x * w[o];
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) s[o] += x.y * w[kOut + o];
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) s[o] += x.z * w[2 * kOut + o];
+#pragma unroll
+        for (int o = 0; o < kOut; ++o) s[o] += x.w * w[3 * kOut + o];
+    }
+#pragma unroll
+    for (int o = 0; o < kOut; ++o) y[(size_t)i * kOut + o] = s[o];
 }
 
 // dh = dy Wo^T, one thread per (row, hidden column).
@@ -453,7 +476,7 @@ void neighbour_mean_transpose_add(const int *nbr, int W, int N, const float *inv
 
 void relu_back(const float *dh, const float *z, size_t n, float *dz, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_relu_back, dim3(nb(n, kThreads)), dim3(kThreads), 0, s, dh, z, n, dz);
+    hipLaunchKernelGGL(k_relu_back, dim3(nb((n + 3) / 4, kThreads)), dim3(kThreads), 0, s, dh, z, n, dz);
 }
 
 void add(float *a, const float *b, size_t n, hipStream_t s)
@@ -463,7 +486,7 @@ void add(float *a, const float *b, size_t n, hipStream_t s)
 
 void out_layer(const float *h, const float *Wo, const float *bo, int N, float *y, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_out, dim3(nb((size_t)N * kOut, kThreads)), dim3(kThreads), 0, s, h, Wo, bo, N, y);
+    hipLaunchKernelGGL(k_out, dim3(nb((size_t)N, kThreads)), dim3(kThreads), 0, s, h, Wo, bo, N, y);
 }
 
 void out_layer_back(const float *dy, const float *Wo, int N, float *dh, hipStream_t s)
