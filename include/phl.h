@@ -82,8 +82,8 @@ typedef struct phl_params {
                              * 1 one launch per dependency level;
                              * 2 dataflow with chunks taken from a ticket counter (independent of the
                              * workgroup dispatch order; slower on large meshes).
-                             * The dataflow waits are bounded (0.2 s): a wait that runs out makes
-                             * phl_iterate/phl_lusgs fail with PHL_ERR_NUMERIC. */
+                             * The dataflow waits are bounded (0.2 s, phl_set_sweep_wait): a wait that
+                             * runs out makes phl_iterate/phl_lusgs fail with PHL_ERR_NUMERIC. */
 } phl_params;
 
 /* Field identifiers for phl_get_field (same numbering as the oracle). */
@@ -223,14 +223,38 @@ phl_status phl_boundary(phl_ctx *ctx);
 /* CallGPUComputeGamaAndTemperature (:4668) and
  * CallGPUComputeViscousCoefficientWithoutChemical (:6724). */
 phl_status phl_gamma_t(phl_ctx *ctx);
-/* NSSolverUnstruct::ComputeGradient -> GetGradientField (:18171,18244). */
+/* NSSolverUnstruct::ComputeGradient -> GetGradientField (:18171,18244).
+ * The same kernel also evaluates the limiter (both walk a cell's faces and
+ * neighbours once) and refreshes the face loops' gather records. */
 phl_status phl_gradient(phl_ctx *ctx);
-/* NSSolverUnstruct::ComputeLimiter (:18189). */
+/* NSSolverUnstruct::ComputeLimiter (:18189).  Right after phl_gradient on an
+ * unchanged state the limiter is already in place and this returns at once;
+ * otherwise it runs the gradient kernel. */
 phl_status phl_limiter(phl_ctx *ctx);
-/* LoadResiduals + NSSolver::UpdateResiduals (NSSolverUnstruct.cpp:4118;
- * NSSolver.cpp:71): boundary, gradients, limiter, inviscid (CallGPUGetQlQr ..
- * CallGPULoadFlux) and viscous (CallGPUComputeVisflux) face loops. */
+/* The steps of Controller::UpdateResiduals (CFD/Solver/src/Controller.cpp:
+ * 1002-1083), one call per reference hook.  A host driver calls, per
+ * iteration: phl_load_residuals, phl_boundary, phl_gamma_t (viscous),
+ * phl_gradient, phl_limiter, then phl_update_residuals (or its two halves
+ * phl_inviscid_flux and phl_viscous_flux), then phl_spectrum_radius,
+ * phl_time_step and phl_lusgs.  The face passes read the states, gradients
+ * and limiter that phl_boundary and phl_gradient left. */
+/* NSSolverUnstruct::LoadResiduals (:4116, CallGPULoadResiduals :4118):
+ * res = -rhs, with rhs = 0 on the single grid level. */
+phl_status phl_load_residuals(phl_ctx *ctx);
+/* NSSolverUnstruct::InviscidFlux (:6912): MUSCL reconstruction, BoundaryQlQrFix,
+ * Roe, x area (CallGPUGetQlQr .. CallGPUinviscidScheme :13955-14466) and
+ * LoadFlux (CallGPULoadFlux :6437), added to res. */
+phl_status phl_inviscid_flux(phl_ctx *ctx);
+/* NSSolverUnstruct::ViscousFlux (:7058; CallGPUGetVisFaceValue :7128,
+ * CallGPUComputeVisflux :12979), added to res.  No-op for inviscid flow. */
+phl_status phl_viscous_flux(phl_ctx *ctx);
+/* NSSolver::UpdateResiduals (CFD/NSSolver/src/NSSolver.cpp:71-82): the
+ * inviscid and viscous face loops (one face kernel for both), added to res. */
 phl_status phl_update_residuals(phl_ctx *ctx);
+/* Controller::LoadResiduals + Controller::UpdateResiduals fused
+ * (Controller.cpp:1176,1002): res = -0.0, Boundary, the gradient and limiter
+ * pass, both face loops, and the residual sums of the result. */
+phl_status phl_residual(phl_ctx *ctx);
 /* CallGPUSpectrumRadiusInviscid / Viscous (:5751,5854). */
 phl_status phl_spectrum_radius(phl_ctx *ctx);
 /* CallGPULocalTimeStep + CallGPUComputeMinTimeStep + CallGPUReduceMaxTimeStep
@@ -239,8 +263,9 @@ phl_status phl_time_step(phl_ctx *ctx, int step);
 /* Controller::LUSGS (CFD/Solver/src/Controller.cpp:1878): Diagonal, sweeps,
  * CallGPUUpdateFlowField.  n_fixed receives the SolutionFix count. */
 phl_status phl_lusgs(phl_ctx *ctx, int *n_fixed);
-/* ComputeResidualonGrid sums (API/Physics/src/Residual.cpp:534): sum5[m] =
- * sum over real cells of res_m^2. */
+/* ComputeResidualonGrid sums (API/Physics/src/Residual.cpp:534) of the
+ * residual currently in res: sum5[m] = sum over real cells of res_m^2 (case 0
+ * of a batched context).  Deterministic (fixed blocks, fixed order). */
 phl_status phl_residual_norm(phl_ctx *ctx, double *sum5);
 
 /* Wall force and moment coefficients: NSSolverUnstruct::AirForceCoefParts
@@ -308,7 +333,9 @@ phl_status phl_synchronize(phl_ctx *ctx);
 phl_status phl_sweep_levels(const phl_ctx *ctx, int *fwd_levels, int *bwd_levels);
 /* How phl_iterate / phl_solve launch (no reference counterpart: the
  * reference launches every kernel eagerly on the default stream).  mode 1
- * (default) replays two-step hipGraphs; mode 0 launches eagerly.  Results are
+ * (default) replays hipGraphs of 8 captured steps (the steps of a call beyond
+ * its last whole 8 launch eagerly); mode 0 launches eagerly; mode 2 launches
+ * eagerly with the turbulence sweeps on a second stream.  Results are
  * identical; kernel timing always launches eagerly. */
 phl_status phl_set_launch_mode(phl_ctx *ctx, int mode);
 /* Autotuner (no reference counterpart): times the flux and gradient kernels
@@ -344,7 +371,10 @@ phl_status phl_mpnn_n_params(const phl_mpnn *net, int *count);
 phl_status phl_mpnn_set_params(phl_mpnn *net, const float *flat);   /* also resets Adam */
 phl_status phl_mpnn_get_params(phl_mpnn *net, float *flat);
 /* Graph: nbr[n_nodes][width] neighbour lists, -1 padded; or the interior
- * face neighbours of a mesh's cells. */
+ * face neighbours of a mesh's cells.  The lists must be symmetric with equal
+ * multiplicity (j appears in i's list as often as i in j's, as face relations
+ * are): the backward pass applies the transpose of the neighbour mean as a
+ * gather over each node's own list.  Other lists return PHL_ERR_ARG. */
 phl_status phl_mpnn_set_graph(phl_mpnn *net, int n_nodes, int width, const int *nbr);
 phl_status phl_mpnn_set_graph_mesh(phl_mpnn *net, const phl_mesh *mesh);
 /* Features (and targets, may be NULL) of the current graph, kept on the device. */
@@ -369,6 +399,16 @@ phl_status phl_get_kernel_timing(phl_ctx *ctx, double *ms_total, int *n_steps);
 /* Achievable HBM bandwidth: a streaming copy of `bytes` bytes, best of
  * `iters` launches, in GB/s (reported beside the vendor peak). */
 phl_status phl_hbm_copy(int device, double bytes, int iters, double *gbps);
+/* Launch statistics (no reference counterpart): graph replays and eagerly
+ * launched steps of phl_iterate since the context was created. */
+phl_status phl_launch_counts(const phl_ctx *ctx, long long *graph_replays, long long *eager_steps);
+/* Wait bound of the dataflow sweeps, in ticks of the 100 MHz wall clock (no
+ * reference counterpart; default 20,000,000 = 0.2 s).  A wave that waits
+ * longer gives up, every later waiter gives up at once, every kernel still
+ * exits, and phl_iterate / phl_lusgs return PHL_ERR_NUMERIC; the state is then
+ * invalid until phl_set_state or phl_init_freestream replaces it (which also
+ * clears the give-up count).  0 makes every wave that would wait give up. */
+phl_status phl_set_sweep_wait(phl_ctx *ctx, int ticks);
 
 #ifdef __cplusplus
 }

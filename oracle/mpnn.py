@@ -48,6 +48,18 @@ def neighbour_mean(h, nbr):
     return s / np.maximum(deg, 1)[:, None], deg
 
 
+def neighbour_mean_transpose(g, nbr):
+    """Transpose of neighbour_mean applied to g: cell j receives g_i / deg_i
+    from every cell i that lists j.  Written as a gather over j's own list,
+    as the device kernel does, which equals the scatter only when the lists
+    are symmetric with equal multiplicity (every face relation is)."""
+    valid = nbr >= 0
+    deg = np.maximum(valid.sum(axis=1), 1)
+    idx = np.where(valid, nbr, 0)
+    w = g / deg[:, None]
+    return (w[idx] * valid[..., None]).sum(axis=1)
+
+
 def forward(p, X, nbr, keep=False):
     """Returns y (and the activations needed by backward when keep)."""
     z0 = X @ p["W0"] + p["b0"]
@@ -82,9 +94,6 @@ def backward(p, X, nbr, t):
     dh = dy @ p["Wo"].T
     L = p["A"].shape[0]
     gA, gB, gc = np.zeros_like(p["A"]), np.zeros_like(p["B"]), np.zeros_like(p["c"])
-    valid = nbr >= 0
-    deg = np.maximum(valid.sum(axis=1), 1)
-    idx = np.where(valid, nbr, 0)
     for l in reversed(range(L)):
         z, m, hin = a["zs"][l], a["ms"][l], a["hs"][l]
         dz = dh * (z > 0)
@@ -92,12 +101,7 @@ def backward(p, X, nbr, t):
         gB[l] = m.T @ dz
         gc[l] = dz.sum(axis=0)
         dm = dz @ p["B"][l].T
-        # transpose of the mean: cell j receives dm_i / deg_i from every i
-        # that lists j; the face relation is symmetric, so it is a gather
-        # over j's own neighbours
-        w = dm / deg[:, None]
-        back = (w[idx] * valid[..., None]).sum(axis=1)
-        dh = dh + dz @ p["A"][l].T + back
+        dh = dh + dz @ p["A"][l].T + neighbour_mean_transpose(dm, nbr)
     g["A"], g["B"], g["c"] = gA, gB, gc
     dz0 = dh * (a["z0"] > 0)
     g["W0"] = X.T @ dz0

@@ -49,7 +49,7 @@ def load():
         lib.orc_freestream.argtypes = [_P, _D]
         for name in ("orc_init_freestream", "orc_boundary", "orc_gamma_t", "orc_viscosity", "orc_gradient",
                      "orc_limiter", "orc_zero_residual", "orc_inviscid_flux", "orc_viscous_flux",
-                     "orc_update_residuals", "orc_spectrum", "orc_sa_boundary", "orc_sa_viscosity",
+                     "orc_update_residuals", "orc_residual", "orc_spectrum", "orc_sa_boundary", "orc_sa_viscosity",
                      "orc_sa_gradient", "orc_sa_update_residuals", "orc_sa_spectrum", "orc_sa_lusgs",
                      "orc_sa_relax"):
             getattr(lib, name).argtypes = [_P]

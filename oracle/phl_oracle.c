@@ -1106,16 +1106,27 @@ void orc_viscous_flux(orc_solver *s)
 
 /* Controller::UpdateResiduals (Controller.cpp:1002-1061) + LoadResiduals:
  * boundary, independent fields, then the inviscid and viscous face loops. */
+/* NSSolver::UpdateResiduals (CFD/NSSolver/src/NSSolver.cpp:71-82): the
+ * inviscid and viscous face loops, added to res. */
 void orc_update_residuals(orc_solver *s)
 {
+    orc_inviscid_flux(s);
+    orc_viscous_flux(s);
+}
+
+/* Controller::LoadResiduals + Controller::UpdateResiduals
+ * (CFD/Solver/src/Controller.cpp:1176,1002-1083): res = -rhs, Boundary
+ * (SetBoundaryValueOfField :1467), UpdateIndependentField (:1715: gamma/T,
+ * viscosity, gradient, limiter), then NSSolver::UpdateResiduals. */
+void orc_residual(orc_solver *s)
+{
+    orc_zero_residual(s);
     orc_boundary(s);
     orc_gamma_t(s);
     if (s->p.viscous) orc_viscosity(s);
     orc_gradient(s);
     orc_limiter(s);
-    orc_zero_residual(s);
-    orc_inviscid_flux(s);
-    orc_viscous_flux(s);
+    orc_update_residuals(s);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1523,7 +1534,7 @@ int orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist)
 {
     int nNeg = 0;
     for (int i = 0; i < n_steps; ++i) {
-        orc_update_residuals(s);
+        orc_residual(s);
         orc_spectrum(s);
         orc_time_step(s, first_step + i);
         if (rms_hist) {

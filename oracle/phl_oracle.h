@@ -109,7 +109,8 @@ void orc_limiter(orc_solver *s);
 void orc_zero_residual(orc_solver *s);
 void orc_inviscid_flux(orc_solver *s);
 void orc_viscous_flux(orc_solver *s);
-void orc_update_residuals(orc_solver *s);          /* LoadResiduals + UpdateResiduals */
+void orc_update_residuals(orc_solver *s);          /* NSSolver::UpdateResiduals: both face loops */
+void orc_residual(orc_solver *s);                  /* LoadResiduals + Controller::UpdateResiduals */
 void orc_spectrum(orc_solver *s);
 void orc_time_step(orc_solver *s, int step);
 int  orc_lusgs(orc_solver *s, double *sweep_normals); /* returns #negative cells fixed */

@@ -135,6 +135,12 @@ def _declare(lib):
         "phl_gradient": (ctypes.c_int, [_P]),
         "phl_limiter": (ctypes.c_int, [_P]),
         "phl_update_residuals": (ctypes.c_int, [_P]),
+        "phl_load_residuals": (ctypes.c_int, [_P]),
+        "phl_inviscid_flux": (ctypes.c_int, [_P]),
+        "phl_viscous_flux": (ctypes.c_int, [_P]),
+        "phl_residual": (ctypes.c_int, [_P]),
+        "phl_launch_counts": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong)]),
+        "phl_set_sweep_wait": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_spectrum_radius": (ctypes.c_int, [_P]),
         "phl_time_step": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_lusgs": (ctypes.c_int, [_P, _I]),
@@ -426,8 +432,32 @@ class Solver:
     def limiter(self):
         _check(_lib.phl_limiter(self._h))
 
+    def load_residuals(self):
+        _check(_lib.phl_load_residuals(self._h))
+
+    def inviscid_flux(self):
+        _check(_lib.phl_inviscid_flux(self._h))
+
+    def viscous_flux(self):
+        _check(_lib.phl_viscous_flux(self._h))
+
     def update_residuals(self):
+        """NSSolver::UpdateResiduals: both face loops, added to the residual."""
         _check(_lib.phl_update_residuals(self._h))
+
+    def residual(self):
+        """LoadResiduals + Controller::UpdateResiduals fused: boundary, gradient, limiter, face loops."""
+        _check(_lib.phl_residual(self._h))
+
+    def launch_counts(self):
+        """(graph replays, eagerly launched steps) of iterate since creation."""
+        a, b = ctypes.c_longlong(), ctypes.c_longlong()
+        _check(_lib.phl_launch_counts(self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def set_sweep_wait(self, ticks):
+        """Wait bound of the dataflow sweeps in 100 MHz ticks (default 20,000,000)."""
+        _check(_lib.phl_set_sweep_wait(self._h, int(ticks)))
 
     def spectrum_radius(self):
         _check(_lib.phl_spectrum_radius(self._h))

@@ -62,6 +62,12 @@ struct phl_ctx {
     // unconverged case has halved its residual for this many steps (0: never)
     int precisionSwitchStall = 0;
     bool coefF32Ok = false;  // the sweep-ordered kernels (at most 6 faces a side) can read FP32 coefficients
+    // hook form: the gradient kernel also evaluates the limiter, so
+    // phl_limiter right after phl_gradient on an unchanged state has nothing
+    // left to do.  stateVersion counts the calls that change q.
+    unsigned long long stateVersion = 0, gradVersion = ~0ull;
+    // launch statistics: graph replays and eagerly launched steps
+    long long graphReplays = 0, eagerSteps = 0;
 };
 
 namespace {
@@ -311,6 +317,9 @@ phl_status check_launch(phl_ctx *c)
     (void)c;
     return PHL_OK;
 }
+
+// Default bound of a dataflow sweep's waits: 0.2 s of the 100 MHz wall clock.
+constexpr int kSweepWaitTicks = 20000000;
 
 // Dataflow sweeps bound every wait; a wave that gave up leaves its cells'
 // dq unfinished and is counted in sweepCtl[2].  Checked after the work is
@@ -1093,7 +1102,11 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     HIPTRY(hipMemset(fl.sweepFlag, 0, sizeof(int) * nChunks));
     TRY(dalloc(c, &fl.casePart, (size_t)nCases * 32 * 6));
     TRY(dalloc(c, &fl.sweepCtl, 4));
-    HIPTRY(hipMemset(fl.sweepCtl, 0, sizeof(int) * 4));
+    {
+        const int ctl[4] = {0, 0, 0, kSweepWaitTicks};
+        HIPTRY(hipMemcpy(fl.sweepCtl, ctl, sizeof(ctl), hipMemcpyHostToDevice));
+    }
+    TRY(dalloc(c, &fl.negList, (size_t)phl::kFixList));
     TRY(dalloc(c, &fl.t, (size_t)nT));
     TRY(dalloc(c, &fl.visl, (size_t)nT));
     TRY(dalloc(c, &fl.forceOut, 15));
@@ -1242,11 +1255,22 @@ phl_status phl_create_batch(const phl_mesh *pm, const phl_params *params, int de
 }
 
 // ---------------------------------------------------------------- state
+// A new state makes the context valid again after a sweep gave up waiting:
+// the count of waves that gave up is cleared with the state.
+static phl_status clear_gave_up(phl_ctx *c)
+{
+    HIPTRY(hipMemsetAsync(c->f.sweepCtl + 2, 0, sizeof(int), c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
+}
+
 phl_status phl_set_state(phl_ctx *c, const double *q)
 {
     if (!c || !q) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
-    return upload_cells(c, c->f.q, 5, c->m.nTotal, q);
+    ++c->stateVersion;
+    TRY(upload_cells(c, c->f.q, 5, c->m.nTotal, q));
+    return clear_gave_up(c);
 }
 
 phl_status phl_get_state(phl_ctx *c, double *q)
@@ -1329,7 +1353,7 @@ phl_status phl_set_field(phl_ctx *c, int field, const double *in)
     switch (field) {
     case PHL_FIELD_Q: return phl_set_state(c, in);
     case PHL_FIELD_DT: dst = c->f.dt; n = (size_t)nC; break;
-    case PHL_FIELD_VIST: dst = c->f.vist; n = (size_t)nT; break;
+    case PHL_FIELD_VIST: dst = c->f.vist; n = (size_t)nT; ++c->stateVersion; break;
     case PHL_FIELD_NUT:
     case PHL_FIELD_WDIST:
         if (c->k.viscous != 3) return fail(PHL_ERR_ARG, "phl_set_field: turbulence fields need viscous = 3");
@@ -1373,6 +1397,7 @@ phl_status phl_boundary(phl_ctx *c)
 {
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
+    ++c->stateVersion;  // ghost states
     phl::launch_boundary(c->m, c->k, c->f, c->stream);
     return check_launch(c);
 }
@@ -1390,16 +1415,50 @@ phl_status phl_gradient(phl_ctx *c)
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
+    c->gradVersion = c->stateVersion;
     return check_launch(c);
 }
 
-phl_status phl_limiter(phl_ctx *c) { return phl_gradient(c); }
+// The gradient kernel evaluates the limiter in the same pass (both read the
+// cell's faces and neighbours once); on a state phl_gradient has just seen,
+// the limiter is already in place.
+phl_status phl_limiter(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    if (c->gradVersion == c->stateVersion) return PHL_OK;
+    return phl_gradient(c);
+}
 
 static void reset_step_counter(phl_ctx *c)
 {
     hipMemcpyAsync(c->f.stepCtr, c->hostZero, sizeof(int), hipMemcpyHostToDevice, c->stream);
 }
 
+phl_status phl_load_residuals(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_load_residuals(c->m, c->f, c->stream);
+    return check_launch(c);
+}
+
+// Face passes of the hook form: they read the gather records that the
+// boundary and gradient kernels refreshed and add to the residual in res.
+static phl_status face_pass(phl_ctx *c, int parts)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_face_pass(c->m, c->k, c->f, parts, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_inviscid_flux(phl_ctx *c) { return face_pass(c, 1); }
+phl_status phl_viscous_flux(phl_ctx *c) { return face_pass(c, 2); }
+phl_status phl_update_residuals(phl_ctx *c) { return face_pass(c, 3); }
+
+// The fused residual evaluation: LoadResiduals, Boundary, the gradient and
+// limiter pass, and both face loops in one face kernel + one gather, with the
+// residual sums of the result.
 static void enqueue_residual(phl_ctx *c)
 {
     reset_step_counter(c);
@@ -1409,11 +1468,12 @@ static void enqueue_residual(phl_ctx *c)
     phl::launch_norm_final(c->m, c->k, c->f, false, c->stream);
 }
 
-phl_status phl_update_residuals(phl_ctx *c)
+phl_status phl_residual(phl_ctx *c)
 {
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     enqueue_residual(c);
+    c->gradVersion = c->stateVersion;
     return check_launch(c);
 }
 
@@ -1450,6 +1510,7 @@ phl_status phl_lusgs(phl_ctx *c, int *n_fixed)
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     HIPTRY(hipMemsetAsync(c->f.negCount, 0, sizeof(int), c->stream));
+    ++c->stateVersion;
     enqueue_lusgs(c);
     TRY(check_launch(c));
     TRY(check_sweep_waits(c));
@@ -1523,6 +1584,9 @@ phl_status phl_residual_norm(phl_ctx *c, double *sum5)
 {
     if (!c || !sum5) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
+    reset_step_counter(c);
+    phl::launch_residual_norm(c->m, c->k, c->f, c->stream);
+    TRY(check_launch(c));
     HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5, hipMemcpyDeviceToHost, c->stream));
     HIPTRY(hipStreamSynchronize(c->stream));
     for (int m = 0; m < 5; ++m) sum5[m] = c->hostNorm[m];
@@ -1785,6 +1849,7 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
 {
     if (!c || n_steps < 0) return fail(PHL_ERR_ARG, "bad argument");
     HIPTRY(hipSetDevice(c->device));
+    ++c->stateVersion;
     const int K = PHL_N_KERNEL_CLASSES;
     if (c->timing && c->events.empty()) {
         c->events.resize((size_t)kTimedChunk * (K + 1));
@@ -1811,8 +1876,10 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
                 join_turb(c);  // a graph starts with the side stream joined
                 TRY(step_graph(c, &g));
                 HIPTRY(hipGraphLaunch(g, c->stream));
+                ++c->graphReplays;
             }
         }
+        c->eagerSteps += chunk - i;
         for (; i < chunk; ++i)
             enqueue_step(c, c->timing ? &c->events[(size_t)i * (K + 1)] : nullptr);
         TRY(check_launch(c));
@@ -2014,6 +2081,24 @@ phl_status phl_synchronize(phl_ctx *c)
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
+}
+
+phl_status phl_set_sweep_wait(phl_ctx *c, int ticks)
+{
+    if (!c || ticks < 0) return fail(PHL_ERR_ARG, "phl_set_sweep_wait: bad argument");
+    HIPTRY(hipSetDevice(c->device));
+    const int v = ticks > 0 ? ticks : 0;
+    HIPTRY(hipMemcpyAsync(c->f.sweepCtl + 3, &v, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
+}
+
+phl_status phl_launch_counts(const phl_ctx *c, long long *graph_replays, long long *eager_steps)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    if (graph_replays) *graph_replays = c->graphReplays;
+    if (eager_steps) *eager_steps = c->eagerSteps;
     return PHL_OK;
 }
 

@@ -58,6 +58,9 @@ struct DevMesh {
     const unsigned short *c2loc;   // [W][nCell]
 };
 constexpr int kTileCells = 128;
+// Flagged cells of one update that SolutionFix sorts in LDS (more fall back
+// to a walk over the mesh order).
+constexpr int kFixList = 4096;
 
 // Logical block of hardware block b of nblk: workgroups are dispatched to
 // the 8 XCDs round-robin (b % 8), so this gives XCD x the contiguous logical
@@ -143,6 +146,7 @@ struct DevFlow {
     double *cflArr;     // [maxSteps] CFL of each step of the chunk
     unsigned char *negFlag;  // [nCell]
     int *negCount;      // SolutionFix count of the last update
+    int *negList;       // [kFixList] mesh indices of the flagged cells (the first kFixList of them)
     int *negTotal;      // running SolutionFix count
     double *coef;       // [nFace][2][16] LU-SGS face coefficients per side
     double *t, *visl;   // [nTotal], only for the standalone gamma/T hook
@@ -215,6 +219,13 @@ void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 void launch_rec_grad(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fusedDt);
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+// Hook form of the residual (Controller::UpdateResiduals split as the
+// reference calls it): res = -0.0 (LoadResiduals), then face passes that add
+// their face values to res (parts bit 0: inviscid, bit 1: viscous), then the
+// residual sums of the current res into normOut row *stepCtr.
+void launch_load_residuals(const DevMesh &m, DevFlow &f, hipStream_t s);
+void launch_face_pass(const DevMesh &m, const Consts &k, DevFlow &f, int parts, hipStream_t s);
+void launch_residual_norm(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 // Workgroups of the fused flux launch (one partial-sum row each).
 int flux_blocks(const DevMesh &m, const Consts &k);
 void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s);

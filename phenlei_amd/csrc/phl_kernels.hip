@@ -1008,8 +1008,9 @@ __device__ __forceinline__ void load_compact(const double *src, double *fl, doub
 // The stored values of face f: invOut[0..4] the area-scaled inviscid flux,
 // invOut[5] the inviscid spectral-radius term; visOut[] likewise (viscous
 // flow only).  Each group is stored as soon as it is known, as six doubles
-// or (COMPACT, FP32 faces) as a FaceValF32.
-template <typename T, bool COMPACT = false>
+// or (COMPACT, FP32 faces) as a FaceValF32.  PARTS selects the groups
+// evaluated: bit 0 inviscid (InviscidFlux), bit 1 viscous (ViscousFlux).
+template <typename T, bool COMPACT = false, int PARTS = 3>
 __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, const GatherRec *__restrict__ rec,
                                             const T *__restrict__ gradT, const double *__restrict__ vist, int f,
                                             double *invOut, double *visOut)
@@ -1024,7 +1025,7 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
     load_side_rec(rec, re, needG, R);
     const bool bnd = f < M.nBFace;
     const int bc = bnd ? M.bcType[f] : 0;
-    {
+    if constexpr ((PARTS & 1) != 0) {
         T qL[5], qR[5];
         if (K.limiter != 2) {
             const double lmin = dMIN(L.lim, R.lim);
@@ -1072,7 +1073,7 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
         if constexpr (COMPACT) store_compact(invOut, inv);
         else store6(invOut, inv);
     }
-    if (K.viscous) {
+    if ((PARTS & 2) != 0 && K.viscous) {
         T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
         if (le < nC) { gTL[0] = gradT[le]; gTL[1] = gradT[(size_t)nC + le]; gTL[2] = gradT[(size_t)2 * nC + le]; }
         if (re < nC) { gTR[0] = gradT[re]; gTR[1] = gradT[(size_t)nC + re]; gTR[2] = gradT[(size_t)2 * nC + re]; }
@@ -1097,7 +1098,7 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
     }
 }
 
-template <typename T>
+template <typename T, int PARTS>
 __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const GatherRec *__restrict__ rec,
                                                    const T *__restrict__ gradT, const double *__restrict__ vist,
                                                    double *__restrict__ faceInv, double *__restrict__ faceVis)
@@ -1107,7 +1108,85 @@ __global__ void __launch_bounds__(256) k_face_flux(DevMesh M, Consts K0, const G
     // FP32 faces are stored compact (32 bytes a face and group)
     constexpr bool C = sizeof(T) == 4;
     constexpr int stride = C ? 4 : 6;
-    face_values<T, C>(M, K0, rec, gradT, vist, f, faceInv + (size_t)f * stride, faceVis + (size_t)f * stride);
+    face_values<T, C, PARTS>(M, K0, rec, gradT, vist, f, faceInv + (size_t)f * stride,
+                             faceVis + (size_t)f * stride);
+}
+
+// The hook form of a face loop's LoadFlux (NSSolverUnstruct.cpp:6435): one
+// thread per cell adds one group of stored face values (the inviscid ones of
+// InviscidFlux, or the viscous ones of ViscousFlux) to the residual already
+// in res, in the cell's face order.  LoadResiduals (res = -0.0), then the
+// inviscid pass, then the viscous pass round exactly like k_cell_gather's
+// single accumulation.
+template <bool COMPACT>
+__global__ void __launch_bounds__(kBlock) k_cell_add(DevMesh M, const double *__restrict__ fv,
+                                                     double *__restrict__ res)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int nC = M.nCell;
+    if (c >= nC) return;
+    double acc[5];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) acc[m] = res[(size_t)m * nC + c];
+    for (int s = 0; s < M.ellW; ++s) {
+        const int enc = M.c2fEnc[(size_t)s * nC + c];
+        if (enc == 0) break;
+        const bool isL = enc > 0;
+        const int f = (isL ? enc : -enc) - 1;
+        double fl[5], sp;
+        if constexpr (COMPACT) {
+            load_compact(fv + (size_t)f * 4, fl, sp);
+        } else {
+            const double2 *src = reinterpret_cast<const double2 *>(fv + (size_t)f * 6);
+            const double2 a = src[0], b = src[1], e = src[2];
+            fl[0] = a.x; fl[1] = a.y; fl[2] = b.x; fl[3] = b.y; fl[4] = e.x;
+        }
+        if (isL) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) acc[m] -= fl[m];
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) acc[m] += fl[m];
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) res[(size_t)m * nC + c] = acc[m];
+}
+
+// LoadResiduals (NSSolverUnstruct.cpp:4116): res = -rhs, and rhs is zero on
+// the single grid level, so every entry becomes -0.0.
+__global__ void __launch_bounds__(kBlock) k_load_res(double *__restrict__ res, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) res[i] = -0.0;
+}
+
+// Block partials of res^2 over 256-cell blocks of the storage order, grouped
+// and summed exactly as k_cell_gather<256> groups them, so a norm taken after
+// the hook sequence equals the one of the fused residual evaluation.
+__global__ void __launch_bounds__(256) k_res_partial(int nC, const double *__restrict__ res,
+                                                     double *__restrict__ normPart)
+{
+    const int lb = xcd_block(blockIdx.x, gridDim.x);
+    const int c = lb * 256 + threadIdx.x;
+    __shared__ double red[5][256];
+#pragma unroll
+    for (int m = 0; m < 5; ++m) {
+        const double v = c < nC ? res[(size_t)m * nC + c] : 0.0;
+        red[m][threadIdx.x] = c < nC ? v * v : 0.0;
+    }
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) red[m][threadIdx.x] += red[m][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int m = 0; m < 5; ++m) normPart[(size_t)lb * 5 + m] = red[m][0];
+    }
 }
 
 // Per cell: the face values summed in the cell's face order (every inviscid
@@ -1701,8 +1780,9 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_block_so(DevMesh M, Const
 // and loaded with device-scope (sc1) accesses, which are coherent across the
 // XCDs' L2 caches, and the wave's dq stores complete (vmcnt 0) before its
 // flag store is issued.
-// ctl: [0] ticket, [1] epoch of the current sweep, [2] waves that gave up.
-constexpr long long kFlowWaitTicks = 20000000;  // 0.2 s of the 100 MHz wall clock
+// ctl: [0] ticket, [1] epoch of the current sweep, [2] waves that gave up,
+// [3] the wait bound in ticks of the 100 MHz wall clock (set by the host;
+// 20,000,000 = 0.2 s by default).
 
 __global__ void k_sweep_flow_prep(int *ctl)
 {
@@ -1731,7 +1811,7 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
                                            const double *__restrict__ res, const double *__restrict__ diag,
                                            double *dq, double *dflux, const SaSweep &sa,
                                            const GatherRec *__restrict__ rec, const double *__restrict__ vist, int tk,
-                                           int lane, int epoch, long long t0)
+                                           int lane, int epoch, long long t0, long long waitTicks)
 {
     const int nT = M.nTotal, nC = M.nCell, P = 64 * fo.nChunks;
     const int p = tk * 64 + lane;
@@ -1812,7 +1892,7 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
             if (__all(ok)) break;
             // bounded wait: never reached by a correct schedule; a wave that
             // gives up makes every later waiter give up too
-            if (wall_clock64() - t0 > kFlowWaitTicks || ((it & 63) == 0 && flag_load(&ctl[2]) != 0)) return false;
+            if (wall_clock64() - t0 > waitTicks || ((it & 63) == 0 && flag_load(&ctl[2]) != 0)) return false;
             __builtin_amdgcn_s_sleep(1);
         }
     }
@@ -1897,7 +1977,7 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
         tk = blockIdx.x;
     }
     if (!flow_chunk<NW, VIS, CT>(M, K, fo, ctl, chunkFlag, useU, coef, res, diag, dq, dflux, sa, rec, vist, tk, lane,
-                                 ctl[1], wall_clock64()))
+                                 ctl[1], wall_clock64(), (long long)ctl[3]))
         if (lane == 0) atomicAdd(&ctl[2], 1);
 }
 
@@ -1957,7 +2037,7 @@ __global__ void __launch_bounds__(kBlock) k_ghost_dq(DevMesh M, Consts K, const 
 __global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K0, const double *__restrict__ q,
                                                    const double *__restrict__ dq, double *__restrict__ qNew,
                                                    unsigned char *__restrict__ negFlag, int *__restrict__ negCount,
-                                                   const unsigned char *__restrict__ frozen)
+                                                   int *__restrict__ negList, const unsigned char *__restrict__ frozen)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int nT = M.nTotal;
@@ -1987,53 +2067,90 @@ __global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K0, const d
         }
         if (prim[0] < K.densLimit || prim[4] < K.presLimit) {
             negFlag[i] = 1;
-            atomicAdd(negCount, 1);
+            const int slot = atomicAdd(negCount, 1);
+            if (slot < kFixList) negList[slot] = M.logicalOf[i];
         }
     }
 #pragma unroll
     for (int m = 0; m < 5; ++m) qNew[(size_t)m * nT + i] = prim[m];
 }
 
-// Sequential SolutionFix over the flagged cells in increasing index: a
-// neighbour below the cell is read from the updated field, one above from
-// the old field, exactly as the reference's in-place loop sees them.  With
-// `advance` the step counter moves on and the fix count is accumulated.
-__global__ void k_fix(DevMesh M, Consts K0, const double *__restrict__ qOld, double *__restrict__ qNew,
-                      unsigned char *__restrict__ negFlag, int *__restrict__ negCount, int *__restrict__ negTotal,
-                      int *__restrict__ stepCtr, int advance)
+// SolutionFix (:14509-14585) of stored cell c (mesh index lc): the average of
+// its admissible neighbours.  A neighbour below the cell is read from the
+// updated field, one above from the old field, exactly as the reference's
+// in-place update loop sees them.
+__device__ void fix_cell(const DevMesh &M, const Consts &K0, const double *__restrict__ qOld, double *qNew, int c,
+                         int lc)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const int nC = M.nCell, nT = M.nTotal;
-    const int count = *negCount;
-    // the reference's loop order: increasing mesh index
-    for (int lc = 0; lc < nC && count > 0; ++lc) {
-        const int c = M.cellAt[lc];
-        if (!negFlag[c]) continue;
-        const Consts K = case_k(K0, M, c);
-        negFlag[c] = 0;
-        double prim[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
-        double vs = 0.0;
-        for (int s = 0; s < M.ellW; ++s) {
-            const int enc = M.c2fEnc[(size_t)s * nC + c];
-            if (enc == 0) break;
-            const int nb = M.c2n[(size_t)s * nC + c];
-            const double *src = (mesh_index(M, nb) < lc) ? qNew : qOld;
-            const double rn = src[nb], pn = src[(size_t)4 * nT + nb];
-            if (rn < K.densLimit || pn < K.presLimit) continue;
-            vs += 1.0;
-            for (int m = 0; m < 5; ++m) {
-                double v = src[(size_t)m * nT + nb];
-                if (m == 0 || m == 4) v = dABS(v);
-                prim[m] += v * 1.0;
-            }
+    const Consts K = case_k(K0, M, c);
+    double prim[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    double vs = 0.0;
+    for (int s = 0; s < M.ellW; ++s) {
+        const int enc = M.c2fEnc[(size_t)s * nC + c];
+        if (enc == 0) break;
+        const int nb = M.c2n[(size_t)s * nC + c];
+        const double *src = (mesh_index(M, nb) < lc) ? qNew : qOld;
+        const double rn = src[nb], pn = src[(size_t)4 * nT + nb];
+        if (rn < K.densLimit || pn < K.presLimit) continue;
+        vs += 1.0;
+        for (int m = 0; m < 5; ++m) {
+            double v = src[(size_t)m * nT + nb];
+            if (m == 0 || m == 4) v = dABS(v);
+            prim[m] += v * 1.0;
         }
-        if (vs > 0.0) {
-            const double o = 1.0 / vs;
-            for (int m = 0; m < 5; ++m) prim[m] *= o;
-        } else {
-            for (int m = 0; m < 5; ++m) prim[m] = K.primInf[m];
+    }
+    if (vs > 0.0) {
+        const double o = 1.0 / vs;
+        for (int m = 0; m < 5; ++m) prim[m] *= o;
+    } else {
+        for (int m = 0; m < 5; ++m) prim[m] = K.primInf[m];
+    }
+    for (int m = 0; m < 5; ++m) qNew[(size_t)m * nT + c] = prim[m];
+}
+
+// Sequential SolutionFix of the flagged cells in increasing mesh index (the
+// reference's loop order).  k_update listed up to kFixList flagged cells by
+// mesh index; one workgroup sorts that list in LDS (ranks of distinct keys)
+// and its first thread fixes the cells in order.  Beyond kFixList flagged
+// cells the first thread walks the mesh order instead and stops after the
+// last flagged cell.  With `advance` the step counter moves on and the fix
+// count is accumulated.
+__global__ void __launch_bounds__(1024) k_fix(DevMesh M, Consts K0, const double *__restrict__ qOld,
+                                              double *__restrict__ qNew, unsigned char *__restrict__ negFlag,
+                                              int *__restrict__ negCount, const int *__restrict__ negList,
+                                              int *__restrict__ negTotal, int *__restrict__ stepCtr, int advance)
+{
+    __shared__ int keys[kFixList], order[kFixList];
+    const int count = *negCount;  // the same value in every thread
+    const bool listed = count > 0 && count <= kFixList;
+    if (listed) {
+        for (int i = threadIdx.x; i < count; i += blockDim.x) keys[i] = negList[i];
+        __syncthreads();
+        for (int i = threadIdx.x; i < count; i += blockDim.x) {
+            const int v = keys[i];
+            int r = 0;
+            for (int j = 0; j < count; ++j) r += keys[j] < v;
+            order[r] = v;
         }
-        for (int m = 0; m < 5; ++m) qNew[(size_t)m * nT + c] = prim[m];
+        __syncthreads();
+    }
+    if (threadIdx.x != 0) return;
+    if (listed) {
+        for (int k = 0; k < count; ++k) {
+            const int lc = order[k], c = M.cellAt[lc];
+            negFlag[c] = 0;
+            fix_cell(M, K0, qOld, qNew, c, lc);
+        }
+    } else if (count > 0) {
+        int left = count;
+        for (int lc = 0; lc < M.nCell && left > 0; ++lc) {
+            const int c = M.cellAt[lc];
+            if (!negFlag[c]) continue;
+            negFlag[c] = 0;
+            --left;
+            fix_cell(M, K0, qOld, qNew, c, lc);
+        }
     }
     if (advance) {
         *negTotal += count;
@@ -2244,10 +2361,10 @@ static void launch_flux_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStre
     }
     const int nbf = nblk(m.nFace, 256);
     if (k.mixed)
-        hipLaunchKernelGGL(k_face_flux<float>, dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradTF, f.vist, f.faceInv,
+        hipLaunchKernelGGL((k_face_flux<float, 3>), dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradTF, f.vist, f.faceInv,
                            f.faceVis);
     else
-        hipLaunchKernelGGL(k_face_flux<double>, dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradT, f.vist, f.faceInv,
+        hipLaunchKernelGGL((k_face_flux<double, 3>), dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradT, f.vist, f.faceInv,
                            f.faceVis);
     if (k.mixed)  // the FP32 face kernel stored compact values
         hipLaunchKernelGGL((k_cell_gather<BS, true>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.faceInv,
@@ -2267,6 +2384,43 @@ void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, b
     case 128: launch_flux_t<true, 128>(m, k, f, s); break;
     default: launch_flux_t<true, 256>(m, k, f, s); break;
     }
+}
+
+void launch_load_residuals(const DevMesh &m, DevFlow &f, hipStream_t s)
+{
+    const int n = 5 * m.nCell;
+    hipLaunchKernelGGL(k_load_res, dim3(nblk(n, kBlock)), dim3(kBlock), 0, s, f.res, n);
+}
+
+template <int PARTS>
+static void launch_face_pass_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    const int nbf = nblk(m.nFace, 256), nbc = nblk(m.nCell, kBlock);
+    double *fv = PARTS == 1 ? f.faceInv : f.faceVis;
+    if (k.mixed) {
+        hipLaunchKernelGGL((k_face_flux<float, PARTS>), dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradTF, f.vist,
+                           f.faceInv, f.faceVis);
+        hipLaunchKernelGGL(k_cell_add<true>, dim3(nbc), dim3(kBlock), 0, s, m, fv, f.res);
+    } else {
+        hipLaunchKernelGGL((k_face_flux<double, PARTS>), dim3(nbf), dim3(256), 0, s, m, k, f.rec, f.gradT, f.vist,
+                           f.faceInv, f.faceVis);
+        hipLaunchKernelGGL(k_cell_add<false>, dim3(nbc), dim3(kBlock), 0, s, m, fv, f.res);
+    }
+}
+
+void launch_face_pass(const DevMesh &m, const Consts &k, DevFlow &f, int parts, hipStream_t s)
+{
+    if (parts & 1) launch_face_pass_t<1>(m, k, f, s);
+    if ((parts & 2) && k.viscous) launch_face_pass_t<2>(m, k, f, s);
+}
+
+void launch_residual_norm(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    if (k.nCases > 1) return launch_norm_final(m, k, f, false, s);  // per-case sums read res directly
+    const int nb = nblk(m.nCell, 256);
+    hipLaunchKernelGGL(k_res_partial, dim3(nb), dim3(256), 0, s, m.nCell, f.res, f.normPart);
+    hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, s, f.normPart, f.dtMinPart, nb, f.normOut, f.stepCtr,
+                       f.dtMin, 0);
 }
 
 void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
@@ -2389,9 +2543,9 @@ void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 void launch_update(const DevMesh &m, const Consts &k, DevFlow &f, bool advance, hipStream_t s)
 {
     hipLaunchKernelGGL(k_update, dim3(nblk(m.nTotal, kBlock)), dim3(kBlock), 0, s, m, k, f.q, f.dq, f.qAlt,
-                       f.negFlag, f.negCount, f.frozen);
-    hipLaunchKernelGGL(k_fix, dim3(1), dim3(1), 0, s, m, k, f.q, f.qAlt, f.negFlag, f.negCount, f.negTotal,
-                       f.stepCtr, advance ? 1 : 0);
+                       f.negFlag, f.negCount, f.negList, f.frozen);
+    hipLaunchKernelGGL(k_fix, dim3(1), dim3(1024), 0, s, m, k, f.q, f.qAlt, f.negFlag, f.negCount, f.negList,
+                       f.negTotal, f.stepCtr, advance ? 1 : 0);
 }
 
 void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *torqueRef, hipStream_t s)

@@ -244,6 +244,23 @@ phl_status phl_mpnn_set_graph(phl_mpnn *n, int nNodes, int width, const int *nbr
         }
         inv[i] = 1.f / (float)std::max(d, 1);
     }
+    // The backward pass applies the transpose of the neighbour mean as a
+    // gather over each node's own list, which is the transpose only if the
+    // lists are symmetric with equal multiplicity: j appears in i's list as
+    // often as i appears in j's.
+    for (int i = 0; i < nNodes; ++i)
+        for (int k = 0; k < width; ++k) {
+            const int j = nbr[(size_t)i * width + k];
+            if (j < 0) continue;
+            int cij = 0, cji = 0;
+            for (int t = 0; t < width; ++t) {
+                cij += nbr[(size_t)i * width + t] == j;
+                cji += nbr[(size_t)j * width + t] == i;
+            }
+            if (cij != cji)
+                return fail(PHL_ERR_ARG, "phl_mpnn_set_graph: neighbour lists are not symmetric (node " +
+                                             std::to_string(i) + " lists " + std::to_string(j) + ")");
+        }
     HIPTRY(hipSetDevice(n->device));
     HIPTRY(hipStreamSynchronize(n->stream));
     if (n->nbr) hipFree(n->nbr);

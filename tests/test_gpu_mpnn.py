@@ -112,3 +112,18 @@ def test_adam_training_tracks_oracle_and_learns():
     more = net.train(300, 1e-3)
     print("loss", hist[0], "->", more[-1])
     assert more[-1] < 0.3 * hist[0]
+
+
+def test_set_graph_rejects_asymmetric_lists():
+    """The backward pass transposes the neighbour mean as a gather over each
+    node's own list, so lists that are not symmetric are an argument error."""
+    net = pa.Mpnn(layers=1, seed=0, device=0)
+    nbr = -np.ones((6, 2), np.int32)
+    for i in range(1, 5):
+        nbr[i, 0], nbr[i + 1, 1] = i + 1, i
+    net.set_graph(nbr)  # a symmetric chain
+    nbr[0, 0] = 5  # 0 lists 5, 5 does not list 0
+    with pytest.raises(pa.PhlError) as e:
+        net.set_graph(nbr)
+    assert e.value.status == pa.PHL_ERR_ARG
+    net.close()

@@ -38,8 +38,8 @@ def test_residual_parity(limiter):
     q = seeded(m, cpu, 1 + limiter)
     gpu.set_state(q)
     cpu.set("q", q)
-    gpu.update_residuals()
-    cpu.update_residuals()
+    gpu.residual()
+    cpu.residual()
     nc = m.n_cell
     qg, qc = gpu.get_state(), cpu.get("q")
     assert np.max(np.abs(qg - qc)) <= 1e-14  # ghost cells from the BCs
@@ -64,8 +64,8 @@ def test_time_step_and_lusgs_parity():
     q = seeded(m, cpu, 7)
     gpu.set_state(q)
     cpu.set("q", q)
-    gpu.update_residuals()
-    cpu.update_residuals()
+    gpu.residual()
+    cpu.residual()
     cpu.spectrum()
     gpu.time_step(3)
     cpu.time_step(3)
@@ -128,8 +128,8 @@ def test_hybrid_mesh_residual_parity(viscous):
     q = seeded(m, cpu, 21 + viscous)
     gpu.set_state(q)
     cpu.set("q", q)
-    gpu.update_residuals()
-    cpu.update_residuals()
+    gpu.residual()
+    cpu.residual()
     cpu.spectrum()
     nc = m.n_cell
     rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
@@ -193,20 +193,24 @@ def test_forces_parity(viscous):
 
 
 def test_graph_replay_equals_eager_launch():
-    """phl_iterate replays captured two-step hipGraphs by default; an odd
-    number of steps exercises both buffer assignments and the eager tail.
-    Every replay must equal the eager launches bit for bit."""
+    """phl_iterate replays captured hipGraphs of 8 steps by default, one per
+    assignment of the double-buffered state.  Calls of 9, 8 and 11 steps start
+    on both buffer parities, so both graphs are captured and replayed and the
+    eager tails run too.  Every replay equals the eager launches bit for bit,
+    and the launch counters show the graphs really ran."""
     m = pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7)
     p = pa.default_params(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0)
     a, b = pa.Solver(m, p), pa.Solver(m, p)
     a.set_launch_mode("eager")
     a.init_freestream()
     b.init_freestream()
-    for first, n in ((1, 3), (4, 4), (8, 5)):
+    for first, n in ((1, 9), (10, 8), (18, 11)):
         ha = a.iterate(first, n)
         hb = b.iterate(first, n)
         assert np.array_equal(ha, hb)
         assert np.array_equal(a.get_state(), b.get_state())
+    assert a.launch_counts() == (0, 28)
+    assert b.launch_counts() == (3, 4)  # 8+1, 8, 8+3 steps
 
 
 @pytest.mark.parametrize("viscous", [0, 1])
@@ -219,8 +223,8 @@ def test_mixed_precision_residual(viscous):
     q = seeded(m, cpu, 31 + viscous)
     gpu.set_state(q)
     cpu.set("q", q)
-    gpu.update_residuals()
-    cpu.update_residuals()
+    gpu.residual()
+    cpu.residual()
     cpu.spectrum()
     nc = m.n_cell
     rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
@@ -449,8 +453,8 @@ def test_least_squares_gradient_parity(kind, viscous):
     q = seeded(m, cpu, 41 + viscous)
     gpu.set_state(q)
     cpu.set("q", q)
-    gpu.update_residuals()
-    cpu.update_residuals()
+    gpu.residual()
+    cpu.residual()
     nc = m.n_cell
     for name in ("dqdx", "dqdy", "dqdz"):
         g, c = gpu.get_field(name)[:, :nc], cpu.get(name)[:, :nc]

@@ -72,3 +72,41 @@ def test_adam_reduces_the_loss():
         mpnn.adam_step(p, g, st, 1e-2)
     l1, _ = mpnn.backward(p, X, nbr, t)
     assert l1 < 0.5 * l0
+
+
+def _scatter_transpose(g, nbr):
+    """Explicit transpose of the neighbour mean: out[j] += g[i] / deg[i] for
+    every j in i's list (any lists, symmetric or not)."""
+    valid = nbr >= 0
+    deg = np.maximum(valid.sum(axis=1), 1)
+    out = np.zeros_like(g)
+    for i in range(nbr.shape[0]):
+        for j in nbr[i]:
+            if j >= 0:
+                out[j] += g[i] / deg[i]
+    return out
+
+
+def test_mean_transpose_is_the_adjoint_on_symmetric_lists():
+    """<mean(h), g> == <h, mean^T(g)>, and the gather form of the transpose
+    (the one the device kernel uses) equals the explicit scatter."""
+    _, X, nbr, _ = small_problem()
+    rng = np.random.default_rng(4)
+    h = rng.standard_normal((X.shape[0], 7))
+    g = rng.standard_normal((X.shape[0], 7))
+    m, _ = mpnn.neighbour_mean(h, nbr)
+    mt = mpnn.neighbour_mean_transpose(g, nbr)
+    assert np.allclose(mt, _scatter_transpose(g, nbr), rtol=0, atol=1e-13)
+    assert abs(np.sum(m * g) - np.sum(h * mt)) <= 1e-12 * np.sum(np.abs(m * g))
+
+
+def test_gather_transpose_needs_symmetric_lists():
+    """With one asymmetric entry (cell 0 lists 5, cell 5 does not list 0) the
+    gather form is no longer the transpose: the reason phl_mpnn_set_graph
+    rejects lists that are not symmetric."""
+    nbr = -np.ones((6, 2), np.int64)
+    nbr[0, 0] = 5
+    for i in range(1, 5):
+        nbr[i, 0], nbr[i + 1, 1] = i + 1, i
+    g = np.random.default_rng(2).standard_normal((6, 3))
+    assert not np.allclose(mpnn.neighbour_mean_transpose(g, nbr), _scatter_transpose(g, nbr))

@@ -279,7 +279,7 @@ def test_freestream_preserved_in_flat_channel():
     m = box_mesh(8, 4, 3)
     s = oracle_for(m, mach=0.5)
     s.init_freestream()
-    s.update_residuals()
+    s.residual()
     res = s.get("res")[:, :m.n_cell]
     pinf = s.freestream()[4]
     assert np.max(np.abs(res)) <= 1e-13 * pinf * m.metrics()["area"].max()
@@ -296,7 +296,7 @@ def test_spectral_radius_and_time_step_on_box():
     m = box_mesh(nx, ny, nz)
     s = oracle_for(m, mach=0.5, cfl_start=10.0, cfl_end=10.0)
     s.init_freestream()
-    s.update_residuals()
+    s.residual()
     s.spectrum()
     s.time_step(1)
     dx, dy, dz = 3.0 / nx, 0.8 / ny, 0.2 / nz
@@ -333,7 +333,7 @@ def test_lusgs_equals_dense_factorised_solve():
     q = np.tile(s.freestream()[:, None], (1, m.n_total))
     q[:, :m.n_cell] *= 1 + rng.uniform(-0.05, 0.05, size=(5, m.n_cell))
     s.set("q", q)
-    s.update_residuals()
+    s.residual()
     s.spectrum()
     s.time_step(1)
     res = s.get("res")[:, :m.n_cell].copy()
