@@ -75,6 +75,8 @@ def load():
         lib.orc_lusgs_rad.restype = ctypes.c_double
         lib.orc_entropy_coef.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.orc_entropy_coef.restype = ctypes.c_double
+        lib.orc_sa_source_point.argtypes = [ctypes.c_double] * 6 + [_D]
+        lib.orc_face_states.argtypes = [_P, ctypes.c_int, _D, _D]
         lib.orc_ptotal.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.orc_ptotal.restype = ctypes.c_double
         _lib = lib
@@ -185,6 +187,12 @@ class OracleSolver:
         load().orc_sa_face_matrices(self._h, _dp(a), _dp(b))
         return a, b
 
+    def face_states(self, f):
+        """(qL, qR) of face f: MUSCL with the positivity fallback and BoundaryQlQrFix."""
+        a, b = np.empty(5), np.empty(5)
+        load().orc_face_states(self._h, int(f), _dp(a), _dp(b))
+        return a, b
+
     def lusgs(self):
         sweeps = np.empty(8)
         return load().orc_lusgs(self._h, _dp(sweeps))
@@ -212,6 +220,14 @@ class OracleSolver:
 
     def __del__(self):
         self.close()
+
+
+def sa_source_point(nut, omega, wdist, rho, visl, reynolds):
+    """SA source at one point: (source per volume, (P - D)/nu~, its
+    linearisation, implicit diagonal term per volume)."""
+    out = np.empty(4)
+    load().orc_sa_source_point(nut, omega, wdist, rho, visl, reynolds, _dp(out))
+    return out
 
 
 def wall_distance(mesh_conn, metrics, bc_wall=2):

@@ -944,6 +944,11 @@ static void face_values(const orc_solver *s, int f, double *qL, double *qR)
     }
 }
 
+void orc_face_states(const orc_solver *s, int f, double *qL, double *qR)
+{
+    face_values(s, f, qL, qR);
+}
+
 void orc_zero_residual(orc_solver *s)
 {
     /* ZeroResiduals (:5642) then LoadResiduals res = -rhs with rhs = 0 (:4116). */
@@ -1832,78 +1837,96 @@ static void sa_init_spectrum(orc_solver *s)
     for (int c = 0; c < s->nCell; ++c) s->specT[c] = 1.0 / (s->p.turb_cfl_scale * s->dt[c] + SMALL);
 }
 
-/* SourceFluxOneEquationOriginal (:2702-2916), NSMB formation (the
- * diffusion source cb2 |grad nu~|^2 is not added), no transition (ft2 = 0),
- * length scale = wall distance (RANS). */
-static void sa_source(orc_solver *s)
+/* SourceFluxOneEquationOriginal (:2702-2916) at one cell, NSMB formation
+ * (the diffusion source cb2 |grad nu~|^2 is not added), no transition
+ * (ft2 = 0), length scale = wall distance (RANS).  Inputs: nu~, vorticity
+ * magnitude, wall distance, density, laminar viscosity, 1/Re.  Outputs:
+ * out[0] the source per unit volume, out[1] prde = (production - destruction)
+ * / nu~, out[2] its linearisation prdep as the reference forms it, out[3] the
+ * implicit diagonal term per unit volume (part1 + part2). */
+static void sa_source_cell(double nueSA, double omega, double wdist, double rho, double visl, double oRe,
+                           double *out)
 {
     const double karm = 0.41, cb1 = 0.1355, cb2 = 0.622, cv1 = 7.1, cw2 = 0.3, cw3 = 2.0;
     const double sigma = 2.0 / 3.0;
-    const double kap2 = karm * karm, oKap2 = 1.0 / kap2, osigma = 1.0 / sigma;
+    const double kap2 = karm * karm, oKap2 = 1.0 / kap2;
     const double cw1 = cb1 * oKap2 + (1.0 + cb2) / sigma;
     const double cv13 = cv1 * cv1 * cv1, cw36 = pow(cw3, 6), or6 = 1.0 / 6.0, ct3 = 1.2;
     const double saC2 = 0.7, saC3 = 0.9, cw1Kap2 = cw1 * kap2;
+    double olam = rho / (visl + SMALL);
+    double distance2 = wdist * wdist;
+    double oDistance2 = 1.0 / distance2;
+    double xsi = nueSA * olam + SMALL;
+    double xsi3 = xsi * xsi * xsi;
+    double fv1 = xsi3 / (xsi3 + cv13);
+    double fv2 = 1.0 - xsi / (1.0 + xsi * fv1);
+    double rs = nueSA * oKap2 * oDistance2 * oRe;
+    double nuetRs = nueSA * rs;
+    double sBar = rs * fv2;
+    double std_;
+    if (sBar >= -saC2 * omega) std_ = omega + sBar;
+    else std_ = omega + omega * (saC2 * saC2 * omega + saC3 * sBar) / ((saC3 - 2 * saC2) * omega - sBar);
+    double ostd = 1.0 / (std_ + SMALL);
+    double r = rs * ostd;
+    r = MIN(r, 10.0);
+    double r5 = pow(r, 5);
+    double gg = r + cw2 * (r5 * r - r);
+    double g6 = pow(gg, 6);
+    double fw = gg * pow((1.0 + cw36) / (g6 + cw36), or6);
+    double ft2 = 0.0;
+    double production = cb1 * (1.0 - ft2) * std_ * nueSA;
+    double wallDest = (cw1Kap2 * fw - cb1 * ft2) * nuetRs;
+    const double destrAlpha = 50.0;
+    if (nueSA < 0.0) {
+        production = cb1 * (1.0 - ct3) * omega * nueSA;
+        wallDest = -destrAlpha * cw1Kap2 * nuetRs;
+    }
+    out[0] = production - wallDest;
+    double dfv2dk = (3.0 * cv13 * pow(fv1 / (xsi + SMALL), 2) - 1.0) / pow(1.0 + xsi * fv1, 2);
+    double dsdnu = oRe * oKap2 * oDistance2 * (fv2 + xsi * dfv2dk);
+    double dfwdg = fw / (gg + SMALL) * (1.0 - g6 / (g6 + cw36));
+    double dgdr = 1.0 + cw2 * (6.0 * r5 - 1.0);
+    double drdnu = oRe * oKap2 * oDistance2 * ostd * (1.0 - nueSA * ostd * dsdnu);
+    double dfwdnu = dfwdg * dgdr * drdnu;
+    double prod = cb1 * (1.0 - ft2) * std_;
+    double dest = (cw1 * fw - cb1 * ft2 * oKap2) * nueSA * oDistance2 * oRe;
+    double prodp = cb1 * ((1.0 - ft2) * dsdnu);
+    double destp = (cw1 * (fw + dfwdnu * nueSA) - cb1 * ft2 * oKap2) * oDistance2 * oRe;
+    if (nueSA < 0.0) {
+        prod = cb1 * (1.0 - ct3) * omega;
+        dest = -destrAlpha * cw1 * nueSA * oDistance2 * oRe;
+        prodp = cb1 * ((1.0 - ct3) * dsdnu);
+        destp = -destrAlpha * cw1 * oDistance2 * oRe;
+    }
+    double prde = prod - dest;
+    double prdep = prodp - destp;
+    double part1 = -0.5 * (prde - ABS(prde));
+    double part2 = -0.5 * (prdep - ABS(prdep)) * nueSA;
+    out[1] = prde;
+    out[2] = prdep;
+    out[3] = part1 + part2;
+}
+
+void orc_sa_source_point(double nut, double omega, double wdist, double rho, double visl, double reynolds,
+                         double *out)
+{
+    sa_source_cell(nut, omega, wdist, rho, visl, 1.0 / reynolds, out);
+}
+
+static void sa_source(orc_solver *s)
+{
     const double oRe = 1.0 / s->p.reynolds;
-    (void)osigma;
     for (int c = 0; c < s->nCell; ++c) {
-        double nueSA = s->nut[c];
         double dudy = s->dqdy[IU][c], dudz = s->dqdz[IU][c];
         double dvdx = s->dqdx[IV][c], dvdz = s->dqdz[IV][c];
         double dwdx = s->dqdx[IW][c], dwdy = s->dqdy[IW][c];
         double a1 = dwdy - dvdz, a2 = dudz - dwdx, a3 = dvdx - dudy;
         double omega = sqrt(a1 * a1 + a2 * a2 + a3 * a3);
         double volume = s->vol[c];
-        double olam = s->q[IR][c] / (s->visl[c] + SMALL);
-        double distance2 = s->wdist[c] * s->wdist[c];
-        double oDistance2 = 1.0 / distance2;
-        double xsi = nueSA * olam + SMALL;
-        double xsi3 = xsi * xsi * xsi;
-        double fv1 = xsi3 / (xsi3 + cv13);
-        double fv2 = 1.0 - xsi / (1.0 + xsi * fv1);
-        double rs = nueSA * oKap2 * oDistance2 * oRe;
-        double nuetRs = nueSA * rs;
-        double sBar = rs * fv2;
-        double std_;
-        if (sBar >= -saC2 * omega) std_ = omega + sBar;
-        else std_ = omega + omega * (saC2 * saC2 * omega + saC3 * sBar) / ((saC3 - 2 * saC2) * omega - sBar);
-        double ostd = 1.0 / (std_ + SMALL);
-        double r = rs * ostd;
-        r = MIN(r, 10.0);
-        double r5 = pow(r, 5);
-        double gg = r + cw2 * (r5 * r - r);
-        double g6 = pow(gg, 6);
-        double fw = gg * pow((1.0 + cw36) / (g6 + cw36), or6);
-        double ft2 = 0.0;
-        double production = cb1 * (1.0 - ft2) * std_ * nueSA;
-        double wallDest = (cw1Kap2 * fw - cb1 * ft2) * nuetRs;
-        const double destrAlpha = 50.0;
-        if (nueSA < 0.0) {
-            production = cb1 * (1.0 - ct3) * omega * nueSA;
-            wallDest = -destrAlpha * cw1Kap2 * nuetRs;
-        }
-        double source = production - wallDest;
-        s->resT[c] += (source) * volume;
-        double dfv2dk = (3.0 * cv13 * pow(fv1 / (xsi + SMALL), 2) - 1.0) / pow(1.0 + xsi * fv1, 2);
-        double dsdnu = oRe * oKap2 * oDistance2 * (fv2 + xsi * dfv2dk);
-        double dfwdg = fw / (gg + SMALL) * (1.0 - g6 / (g6 + cw36));
-        double dgdr = 1.0 + cw2 * (6.0 * r5 - 1.0);
-        double drdnu = oRe * oKap2 * oDistance2 * ostd * (1.0 - nueSA * ostd * dsdnu);
-        double dfwdnu = dfwdg * dgdr * drdnu;
-        double prod = cb1 * (1.0 - ft2) * std_;
-        double dest = (cw1 * fw - cb1 * ft2 * oKap2) * nueSA * oDistance2 * oRe;
-        double prodp = cb1 * ((1.0 - ft2) * dsdnu);
-        double destp = (cw1 * (fw + dfwdnu * nueSA) - cb1 * ft2 * oKap2) * oDistance2 * oRe;
-        if (nueSA < 0.0) {
-            prod = cb1 * (1.0 - ct3) * omega;
-            dest = -destrAlpha * cw1 * nueSA * oDistance2 * oRe;
-            prodp = cb1 * ((1.0 - ct3) * dsdnu);
-            destp = -destrAlpha * cw1 * oDistance2 * oRe;
-        }
-        double prde = prod - dest;
-        double prdep = prodp - destp;
-        double part1 = -0.5 * (prde - ABS(prde));
-        double part2 = -0.5 * (prdep - ABS(prdep)) * nueSA;
-        s->specT[c] += (part1 + part2) * volume;
+        double o[4];
+        sa_source_cell(s->nut[c], omega, s->wdist[c], s->q[IR][c], s->visl[c], oRe, o);
+        s->resT[c] += (o[0]) * volume;
+        s->specT[c] += o[3] * volume;
     }
 }
 

@@ -130,6 +130,13 @@ void orc_sa_spectrum(orc_solver *s);
 void orc_sa_lusgs(orc_solver *s);
 void orc_sa_relax(orc_solver *s);   /* one turbulence relaxation after the NS one */
 void orc_sa_face_matrices(const orc_solver *s, double *matL, double *matR);
+/* SA source at one point: out[0] source per volume, out[1] (P - D)/nu~,
+ * out[2] its linearisation, out[3] the implicit diagonal term per volume. */
+void orc_sa_source_point(double nut, double omega, double wdist, double rho, double visl, double reynolds,
+                         double *out);
+/* MUSCL face states of face f (after orc_gradient / orc_limiter), with the
+ * positivity fallback and BoundaryQlQrFix applied. */
+void orc_face_states(const orc_solver *s, int f, double *qL, double *qR);
 #define ORC_N_FORCE 22
 void orc_forces(orc_solver *s, double *out);        /* AirForceCoefParts + CompForce */
 void orc_force_coefficients(const double *raw, double alpha_deg, double beta_deg, double *out);
