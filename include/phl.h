@@ -60,7 +60,9 @@ typedef struct phl_params {
     double turb_skew_angle; /* turbSkewnessAngle, degrees (0.0) */
     int viscous;            /* viscousType: 0 inviscid, 1 laminar, 3 Spalart-Allmaras (1eq-sa) */
     int gradient;           /* gradientName: 0 "ggcell" (ghost gradient 0), 1 "lsq" (weights 1/|dx|^2,
-                             * ghost takes its owner's gradient; NS only, viscous 0 or 1) */
+                             * ghost takes its owner's gradient), 2 "ggnode" (the reference default,
+                             * cfd_para.hypara:494: face values from inverse-distance node averages,
+                             * ghost takes its owner's gradient); lsq and ggnode: NS only, viscous 0 or 1 */
     int limiter;            /* uns_limiter_name: 0 "vencat" (ivencat=7), 1 "nolim", 2 "1st" */
     int entropy_fix;        /* roeEntropyFixMethod (3) */
     int cfl_vary_step;      /* CFLVaryStep (500) */
@@ -163,6 +165,29 @@ phl_status phl_mesh_metrics(const phl_mesh *m, double *xfc, double *yfc, double 
                             double *xcc, double *ycc, double *zcc, double *vol, double *skew);
 void phl_mesh_destroy(phl_mesh *m);
 
+/* ---------------- output and restart files (host) ----------------------
+ * Tecplot ASCII field file of a state, laid out like the reference's block
+ * dump (Post_WriteTecplot::StoreUnsFieldVisualData, API/PostProcess/src/
+ * Post_WriteTecplot.cpp:398-456, written by DumpToTecplotASCII::
+ * DumpFlowByFaceTopo, API/PostProcess/src/Post_Visual.cpp:1801-1940): one
+ * FEPolyhedron zone with node values of the reference's default visual
+ * variables (density, u, v, w, pressure, temperature, Mach number, Cp;
+ * cfd_para.hypara:790-791), interpolated from the cells as CompNodeVar does
+ * (API/Geometry/src/Geo_UnstructGrid.cpp:8527-8603).  q is the primitive
+ * state [5][nTotal] (phl_get_state), mach and gamma the freestream's. */
+phl_status phl_write_tecplot(const phl_mesh *m, const double *q, double mach, double gamma, const char *title,
+                             const char *path);
+/* One zone's restart block of a steady run: int outnstep, then q[n_eq][n_total]
+ * doubles, as NSSolverUnstruct::DumpRestart (CFD/NSSolver/src/
+ * NSSolverUnstruct.cpp:1179-1197; n_eq = 5 primitives) and
+ * TurbSolverUnstr::DumpRestart (CFD/TurbSolver/src/TurbSolverUnstr.cpp:
+ * 435-456; n_eq = 1, nu~) write it.  phl_read_restart checks the file size
+ * against n_eq and n_total (PHL_ERR_ARG when it does not match) and is the
+ * InitFlowAsRestart counterpart (NSSolverUnstruct.cpp:1832): load q with
+ * phl_set_state / phl_set_field. */
+phl_status phl_write_restart(const char *path, int outnstep, int n_eq, int n_total, const double *q);
+phl_status phl_read_restart(const char *path, int n_eq, int n_total, int *outnstep, double *q);
+
 /* ---------------- batched form (no reference counterpart) ---------------
  * A batch is a disjoint union of meshes, one steady case each (a Mach /
  * angle / geometry sweep), solved together: every kernel launch of an
@@ -227,6 +252,18 @@ phl_status phl_gamma_t(phl_ctx *ctx);
  * The same kernel also evaluates the limiter (both walk a cell's faces and
  * neighbours once) and refreshes the face loops' gather records. */
 phl_status phl_gradient(phl_ctx *ctx);
+/* gradient 2 ("ggnode"): NSSolverUnstruct::ComputeNodeValue + ModifyNodeValue
+ * (:4734-5117; CallGPUComputeNodeValue :4738, CallGPUModifyNodeValue :5087),
+ * which the reference runs in Controller::PostSolve after each update
+ * (Controller.cpp:5151-5198): inverse-distance node averages of the state
+ * and of the temperature array as the last phl_gamma_t left it.  phl_gradient
+ * reads the node values; phl_iterate and phl_residual compute them right
+ * after the boundary conditions, before the step's own phl_gamma_t.  After
+ * phl_set_state / phl_init_freestream the first node pass takes the state's
+ * own temperature (InitDependentVariables, NSSolver.cpp:761-794). */
+phl_status phl_compute_node_value(phl_ctx *ctx);
+/* The node values [6][nNode] (five primitives, then T) of the last node pass. */
+phl_status phl_get_node_values(phl_ctx *ctx, double *out);
 /* NSSolverUnstruct::ComputeLimiter (:18189).  Right after phl_gradient on an
  * unchanged state the limiter is already in place and this returns at once;
  * otherwise it runs the gradient kernel. */

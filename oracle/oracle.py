@@ -77,6 +77,9 @@ def load():
         lib.orc_entropy_coef.restype = ctypes.c_double
         lib.orc_sa_source_point.argtypes = [ctypes.c_double] * 6 + [_D]
         lib.orc_face_states.argtypes = [_P, ctypes.c_int, _D, _D]
+        lib.orc_set_nodes.argtypes = [_P, ctypes.c_int, _D, _D, _D, _I, _I, _I, _I]
+        lib.orc_node_values.argtypes = [_P]
+        lib.orc_get_node_values.argtypes = [_P, _D]
         lib.orc_ptotal.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.orc_ptotal.restype = ctypes.c_double
         _lib = lib
@@ -156,6 +159,22 @@ class OracleSolver:
         self.params = params
         self._h = lib.orc_create(ctypes.byref(params), self.n_cell, self.n_bface, len(conn["left"]),
                                  _ip(self._keep[0]), _ip(self._keep[1]), _ip(self._keep[2]), *[_dp(a) for a in m])
+        self.n_node = len(conn["x"])
+        # node lists: the node-based gradient (gradient 2) needs them
+        nodes = [np.ascontiguousarray(conn[k], dtype=np.float64) for k in ("x", "y", "z")]
+        lists = [np.ascontiguousarray(conn[k], dtype=np.int32) for k in
+                 ("face_node_ptr", "face_nodes", "cell_node_ptr", "cell_nodes")]
+        lib.orc_set_nodes(self._h, self.n_node, *[_dp(a) for a in nodes], *[_ip(a) for a in lists])
+
+    def compute_node_value(self):
+        """ComputeNodeValue + ModifyNodeValue on the current state."""
+        load().orc_node_values(self._h)
+
+    def node_values(self):
+        """Node values [6][nNode] (five primitives, then T) of the last node pass."""
+        out = np.empty((6, self.n_node))
+        load().orc_get_node_values(self._h, _dp(out))
+        return out
 
     def get(self, name):
         rows = FIELD_ROWS[name]

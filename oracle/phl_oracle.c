@@ -51,6 +51,15 @@ struct orc_solver {
     /* least-squares gradient (gradient == 1): face weights and the inverse
      * normal-matrix entries per cell */
     double *iwt, *ixx, *iyy, *izz, *ixy, *ixz, *iyz;
+    /* node-based Green-Gauss gradient (gradient == 2): node coordinates,
+     * face and cell node lists, node values of the primitives and of T,
+     * and whether the next node-value pass first makes t match q */
+    int nNode;
+    double *xn, *yn, *zn;
+    int *fnPtr, *fn, *cnPtr, *cn;
+    double *qNode[5], *tNode, *nodeWeight;
+    int *nodeBC;
+    int tFresh;
     /* Spalart-Allmaras (viscous == 3): nu~ and its residual, spectrum,
      * face matrices, increments, gradient and the wall distance */
     double *nut, *resT, *specT, *matL, *matR, *dqT, *dfluxT, *gnx, *gny, *gnz, *wdist;
@@ -445,6 +454,9 @@ void orc_destroy(orc_solver *s)
     free(s->gnx); free(s->gny); free(s->gnz); free(s->wdist);
     free(s->dtdx); free(s->dtdy); free(s->dtdz);
     free(s->iwt); free(s->ixx); free(s->iyy); free(s->izz); free(s->ixy); free(s->ixz); free(s->iyz);
+    free(s->xn); free(s->yn); free(s->zn); free(s->fnPtr); free(s->fn); free(s->cnPtr); free(s->cn);
+    for (int m = 0; m < 5; ++m) free(s->qNode[m]);
+    free(s->tNode); free(s->nodeWeight); free(s->nodeBC);
     free(s);
 }
 
@@ -503,6 +515,7 @@ int orc_get(orc_solver *s, int field, double *out)
 int orc_set(orc_solver *s, int field, const double *in)
 {
     double **f5 = field5(s, field);
+    if (field == ORC_Q) s->tFresh = 1;  /* a new state: its own temperature for the node values */
     if (f5) {
         for (int m = 0; m < 5; ++m) memcpy(f5[m], in + (size_t)m * s->nTotal, (size_t)s->nTotal * sizeof(double));
         return 5;
@@ -515,6 +528,7 @@ int orc_set(orc_solver *s, int field, const double *in)
 /* Controller::InitFlow: uniform freestream on every cell and ghost. */
 void orc_init_freestream(orc_solver *s)
 {
+    s->tFresh = 1;
     for (int m = 0; m < 5; ++m)
         for (int c = 0; c < s->nTotal; ++c) s->q[m][c] = s->primInf[m];
     /* TurbSolverUnstr::InitFlowAsRestart (:910-934): nu~ = freestream value;
@@ -691,6 +705,167 @@ static void grad_lsq(const orc_solver *s, const double *q, double *gx, double *g
     }
 }
 
+/* ------------------------------------------------------------------ */
+/* Node-based Green-Gauss gradient ("ggnode", the reference default)    */
+/* ------------------------------------------------------------------ */
+
+/* The node lists the node-based gradient needs (orc_create has only the
+ * face and cell metrics). */
+void orc_set_nodes(orc_solver *s, int nNode, const double *x, const double *y, const double *z,
+                   const int *faceNodePtr, const int *faceNodes, const int *cellNodePtr, const int *cellNodes)
+{
+    s->nNode = nNode;
+    s->xn = dup_d(x, nNode); s->yn = dup_d(y, nNode); s->zn = dup_d(z, nNode);
+    s->fnPtr = dup_i(faceNodePtr, s->nFace + 1);
+    s->fn = dup_i(faceNodes, faceNodePtr[s->nFace]);
+    s->cnPtr = dup_i(cellNodePtr, s->nCell + 1);
+    s->cn = dup_i(cellNodes, cellNodePtr[s->nCell]);
+    for (int m = 0; m < 5; ++m) s->qNode[m] = zeros(nNode);
+    s->tNode = zeros(nNode);
+    s->nodeWeight = zeros(nNode);
+    s->nodeBC = (int *)calloc((size_t)nNode, sizeof(int));
+    s->tFresh = 1;
+}
+
+/* Boundary regions: the reference walks its BC regions in order and each
+ * region's faces in order; a mesh here has one region per BC code, taken in
+ * increasing code order, each with its faces in increasing index. */
+#define FOR_BFACES_OF(s, code, f) \
+    for (int f = 0; f < (s)->nBFace; ++f) if ((s)->bcType[f] == (code))
+
+static void node_add(orc_solver *s, int point, double px, double py, double pz, const double *qv, double tv)
+{
+    double dx = s->xn[point] - px, dy = s->yn[point] - py, dz = s->zn[point] - pz;
+    double dist = sqrt(dx * dx + dy * dy + dz * dz);
+    double w = 1.0 / dist;
+    for (int m = 0; m < 5; ++m) s->qNode[m][point] += qv[m] * w;
+    s->tNode[point] += tv * w;
+    s->nodeWeight[point] += w;
+}
+
+/* NSSolverUnstruct::ComputeNodeValue (NSSolverUnstruct.cpp:4734-5081) and
+ * ModifyNodeValue (:5083-5117): inverse-distance node averages of q and t.
+ * A node on a solid surface takes only the solid faces around it (their
+ * face averages of the cell and ghost), a farfield node the farfield faces,
+ * a node of another non-symmetry boundary those faces; every other node
+ * (symmetry-plane nodes included) takes the real cells around it. */
+void orc_node_values(orc_solver *s)
+{
+    const int nN = s->nNode, nC = s->nCell;
+    if (s->tFresh) {  /* InitDependentVariables (NSSolver.cpp:761-794): t of the state itself */
+        orc_gamma_t(s);
+        s->tFresh = 0;
+    }
+    for (int i = 0; i < nN; ++i) {
+        s->nodeWeight[i] = 0.0;
+        for (int m = 0; m < 5; ++m) s->qNode[m][i] = 0;
+        s->tNode[i] = 0;
+        s->nodeBC[i] = 0;  /* NO_BOUNDARY_CONDITION */
+    }
+    /* node BC priority: solid surface, then farfield, then the other codes */
+    FOR_BFACES_OF(s, ORC_BC_SOLID_SURFACE, f)
+        for (int j = s->fnPtr[f]; j < s->fnPtr[f + 1]; ++j) s->nodeBC[s->fn[j]] = ORC_BC_SOLID_SURFACE;
+    FOR_BFACES_OF(s, ORC_BC_FARFIELD, f)
+        for (int j = s->fnPtr[f]; j < s->fnPtr[f + 1]; ++j)
+            if (s->nodeBC[s->fn[j]] != ORC_BC_SOLID_SURFACE) s->nodeBC[s->fn[j]] = ORC_BC_FARFIELD;
+    for (int code = 1; code <= 6; ++code) {
+        if (code == ORC_BC_SOLID_SURFACE || code == ORC_BC_FARFIELD || code == ORC_BC_SYMMETRY) continue;
+        FOR_BFACES_OF(s, code, f)
+            for (int j = s->fnPtr[f]; j < s->fnPtr[f + 1]; ++j) {
+                int point = s->fn[j];
+                if (s->nodeBC[point] == ORC_BC_SOLID_SURFACE || s->nodeBC[point] == ORC_BC_FARFIELD) continue;
+                s->nodeBC[point] = code;
+            }
+    }
+    /* Step 1-3: boundary faces in region order, each face's average of its
+     * cell and ghost */
+    for (int step = 0; step < 3; ++step)
+        for (int code = 1; code <= 6; ++code) {
+            if (code == ORC_BC_SYMMETRY) continue;
+            if (step == 0 && code != ORC_BC_SOLID_SURFACE) continue;
+            if (step == 1 && code != ORC_BC_FARFIELD) continue;
+            if (step == 2 && (code == ORC_BC_SOLID_SURFACE || code == ORC_BC_FARFIELD)) continue;
+            FOR_BFACES_OF(s, code, f) {
+                int le = s->left[f], re = s->right[f];
+                double qf[5], tf;
+                for (int m = 0; m < 5; ++m) qf[m] = 0.5 * (s->q[m][le] + s->q[m][re]);
+                tf = 0.5 * (s->t[le] + s->t[re]);
+                for (int j = s->fnPtr[f]; j < s->fnPtr[f + 1]; ++j) {
+                    int point = s->fn[j];
+                    if (step == 1 && s->nodeBC[point] == ORC_BC_SOLID_SURFACE) continue;
+                    if (step == 2 && (s->nodeBC[point] == ORC_BC_SOLID_SURFACE ||
+                                      s->nodeBC[point] == ORC_BC_FARFIELD)) continue;
+                    node_add(s, point, s->xfc[f], s->yfc[f], s->zfc[f], qf, tf);
+                }
+            }
+        }
+    /* Step 4: interior (and symmetry) nodes from the real cells around them */
+    for (int c = 0; c < nC; ++c) {
+        double qc[5];
+        for (int m = 0; m < 5; ++m) qc[m] = s->q[m][c];
+        for (int j = s->cnPtr[c]; j < s->cnPtr[c + 1]; ++j) {
+            int point = s->cn[j];
+            if (s->nodeBC[point] != 0) continue;
+            node_add(s, point, s->xcc[c], s->ycc[c], s->zcc[c], qc, s->t[c]);
+        }
+    }
+    /* ModifyNodeValue */
+    for (int i = 0; i < nN; ++i) {
+        if (s->nodeWeight[i] > 1.0e-40) {  /* SMALL, Precision.h:27 */
+            for (int m = 0; m < 5; ++m) s->qNode[m][i] /= s->nodeWeight[i];
+            s->tNode[i] /= s->nodeWeight[i];
+        } else {
+            for (int m = 0; m < 5; ++m) s->qNode[m][i] = 0.0;
+            s->tNode[i] = 0.0;
+        }
+    }
+}
+
+/* Node values [6][nNode]: the five primitives, then T. */
+void orc_get_node_values(const orc_solver *s, double *out)
+{
+    for (int m = 0; m < 5; ++m)
+        for (int i = 0; i < s->nNode; ++i) out[(size_t)m * s->nNode + i] = s->qNode[m][i];
+    for (int i = 0; i < s->nNode; ++i) out[(size_t)5 * s->nNode + i] = s->tNode[i];
+}
+
+static double face_node_mean(const orc_solver *s, int f, const double *qnode)
+{
+    double qf = 0.0;
+    for (int j = s->fnPtr[f]; j < s->fnPtr[f + 1]; ++j) qf += qnode[s->fn[j]];
+    qf /= (s->fnPtr[f + 1] - s->fnPtr[f]);
+    return qf;
+}
+
+/* UnstructGrid::CompGradientGGNode_NEW, Geo_UnstructGrid.cpp:4202-4315:
+ * face values are node means (interior and symmetry faces) or the average
+ * of the cell and its ghost (other boundary faces); the sums are divided by
+ * the volume, and every ghost takes its owner's gradient. */
+static void grad_ggnode(const orc_solver *s, const double *q, const double *qnode, double *gx, double *gy,
+                        double *gz)
+{
+    int nT = s->nTotal;
+    for (int c = 0; c < nT; ++c) { gx[c] = 0.0; gy[c] = 0.0; gz[c] = 0.0; }
+    for (int f = 0; f < s->nBFace; ++f) {
+        int le = s->left[f], re = f + s->nCell;
+        double qf = s->bcType[f] == ORC_BC_SYMMETRY ? face_node_mean(s, f, qnode) : 0.5 * (q[le] + q[re]);
+        double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
+        gx[le] += qf * nx; gy[le] += qf * ny; gz[le] += qf * nz;
+    }
+    for (int f = s->nBFace; f < s->nFace; ++f) {
+        int le = s->left[f], re = s->right[f];
+        double qf = face_node_mean(s, f, qnode);
+        double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
+        gx[le] += qf * nx; gy[le] += qf * ny; gz[le] += qf * nz;
+        gx[re] -= qf * nx; gy[re] -= qf * ny; gz[re] -= qf * nz;
+    }
+    for (int c = 0; c < s->nCell; ++c) { gx[c] /= s->vol[c]; gy[c] /= s->vol[c]; gz[c] /= s->vol[c]; }
+    for (int f = 0; f < s->nBFace; ++f) {
+        int le = s->left[f], re = f + s->nCell;
+        gx[re] = gx[le]; gy[re] = gy[le]; gz[re] = gz[le];
+    }
+}
+
 /* NSSolverUnstruct::GetGradientField, NSSolverUnstruct.cpp:18244-18338:
  * gradients of the five primitives and of the temperature (gradientName
  * "ggcell" or "lsq", :18266-18272, :18308-18314). */
@@ -699,6 +874,11 @@ void orc_gradient(orc_solver *s)
     if (s->p.gradient == 1) {
         for (int m = 0; m < 5; ++m) grad_lsq(s, s->q[m], s->dqdx[m], s->dqdy[m], s->dqdz[m]);
         grad_lsq(s, s->t, s->dtdx, s->dtdy, s->dtdz);
+        return;
+    }
+    if (s->p.gradient == 2) {
+        for (int m = 0; m < 5; ++m) grad_ggnode(s, s->q[m], s->qNode[m], s->dqdx[m], s->dqdy[m], s->dqdz[m]);
+        grad_ggnode(s, s->t, s->tNode, s->dtdx, s->dtdy, s->dtdz);
         return;
     }
     for (int m = 0; m < 5; ++m) grad_ggcell(s, s->q[m], s->dqdx[m], s->dqdy[m], s->dqdz[m]);
@@ -1127,6 +1307,10 @@ void orc_residual(orc_solver *s)
 {
     orc_zero_residual(s);
     orc_boundary(s);
+    /* ggnode: the node values the reference's PostSolve of the previous
+     * iteration computed (Controller.cpp:5151-5198): the state with fresh
+     * ghosts, and the temperature of that iteration's UpdateIndependentField */
+    if (s->p.gradient == 2) orc_node_values(s);
     orc_gamma_t(s);
     if (s->p.viscous) orc_viscosity(s);
     orc_gradient(s);
@@ -1579,6 +1763,10 @@ void orc_forces(orc_solver *s, double *out)
     orc_boundary(s);
     orc_gamma_t(s);
     if (s->p.viscous) orc_viscosity(s);
+    if (s->p.gradient == 2) {  /* node values of the current state and its temperature */
+        s->tFresh = 0;
+        orc_node_values(s);
+    }
     orc_gradient(s);
 
     const double poo = s->primInf[IP];

@@ -43,7 +43,7 @@ typedef struct orc_params {
     double force_ref_span;
     double sa_nu_inf, sa_vist_inf, turb_cfl_scale, turb_skew_angle;
     int viscous;            /* 0 inviscid, 1 laminar, 3 Spalart-Allmaras */
-    int gradient;           /* 0 ggcell, 1 lsq */
+    int gradient;           /* 0 ggcell, 1 lsq, 2 ggnode */
     int limiter;            /* 0 vencat (ivencat=7), 1 nolim, 2 first order */
     int entropy_fix;        /* roeEntropyFixMethod (1..6) */
     int cfl_vary_step;      /* CFLVaryStep */
@@ -99,6 +99,15 @@ int  orc_get(orc_solver *s, int field, double *out);
 int  orc_set(orc_solver *s, int field, const double *in);
 void orc_freestream(const orc_solver *s, double *prim5);
 void orc_init_freestream(orc_solver *s);
+
+/* ---- node-based Green-Gauss gradient (gradient == 2, "ggnode") ---- */
+/* Node coordinates and the face/cell node lists (required for gradient 2). */
+void orc_set_nodes(orc_solver *s, int nNode, const double *x, const double *y, const double *z,
+                   const int *faceNodePtr, const int *faceNodes, const int *cellNodePtr, const int *cellNodes);
+/* ComputeNodeValue + ModifyNodeValue (NSSolverUnstruct.cpp:4734-5117); the
+ * residual evaluation calls it after the boundary conditions. */
+void orc_node_values(orc_solver *s);
+void orc_get_node_values(const orc_solver *s, double *out6);  /* [6][nNode]: 5 primitives, T */
 
 /* ---- one iteration, piece by piece (NSSolverUnstruct / Controller) ---- */
 void orc_boundary(orc_solver *s);

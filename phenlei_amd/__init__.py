@@ -113,6 +113,10 @@ def _declare(lib):
         "phl_mesh_connectivity": (ctypes.c_int, [_P, _D, _D, _D, _I, _I, _I, _I, _I, _I, _I]),
         "phl_mesh_metrics": (ctypes.c_int, [_P] + [_D] * 12),
         "phl_mesh_destroy": (None, [_P]),
+        "phl_write_tecplot": (ctypes.c_int, [_P, _D, ctypes.c_double, ctypes.c_double, ctypes.c_char_p,
+                                             ctypes.c_char_p]),
+        "phl_write_restart": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
+        "phl_read_restart": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _I, _D]),
         "phl_device_info": (ctypes.c_int, [ctypes.c_int, ctypes.c_char_p, ctypes.c_int, _I,
                                            ctypes.POINTER(ctypes.c_double)]),
         "phl_create": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_P)]),
@@ -133,6 +137,8 @@ def _declare(lib):
         "phl_boundary": (ctypes.c_int, [_P]),
         "phl_gamma_t": (ctypes.c_int, [_P]),
         "phl_gradient": (ctypes.c_int, [_P]),
+        "phl_compute_node_value": (ctypes.c_int, [_P]),
+        "phl_get_node_values": (ctypes.c_int, [_P, _D]),
         "phl_limiter": (ctypes.c_int, [_P]),
         "phl_update_residuals": (ctypes.c_int, [_P]),
         "phl_load_residuals": (ctypes.c_int, [_P]),
@@ -324,6 +330,16 @@ class Mesh:
         _check(load_library().phl_mesh_metrics(self._h, *[_dptr(out[k]) for k in names_f + names_c + ["skew"]]))
         return out
 
+    def write_tecplot(self, path, q, mach, gamma=1.4, title="phenlei_amd"):
+        """Tecplot ASCII file of the primitive state q [5][nTotal]: one
+        FEPolyhedron zone, node values of density, u, v, w, pressure,
+        temperature, Mach number and Cp."""
+        q = np.ascontiguousarray(q, dtype=np.float64)
+        if q.shape != (5, self.n_total):
+            raise ValueError("q must be [5][nTotal]")
+        _check(load_library().phl_write_tecplot(self._h, _dptr(q), float(mach), float(gamma), title.encode(),
+                                                os.fsencode(path)))
+
     def __del__(self):
         if getattr(self, "_h", None) and _lib is not None:
             _lib.phl_mesh_destroy(self._h)
@@ -339,6 +355,21 @@ def hbm_copy_gbps(device=0, nbytes=2 * 1024**3, iters=10):
     g = ctypes.c_double()
     _check(load_library().phl_hbm_copy(device, float(nbytes), iters, ctypes.byref(g)))
     return g.value
+
+
+def write_restart(path, q, outnstep=0):
+    """One zone's restart block: int outnstep, then q[n_eq][n_total] doubles
+    (q [5][nTotal] primitives for the NS solver, [nTotal] nu~ for SA)."""
+    q = np.ascontiguousarray(np.atleast_2d(q), dtype=np.float64)
+    _check(load_library().phl_write_restart(os.fsencode(path), int(outnstep), q.shape[0], q.shape[1], _dptr(q)))
+
+
+def read_restart(path, n_eq, n_total):
+    """(outnstep, q[n_eq][n_total]) of a restart block."""
+    q = np.empty((n_eq, n_total))
+    step = ctypes.c_int()
+    _check(load_library().phl_read_restart(os.fsencode(path), n_eq, n_total, ctypes.byref(step), _dptr(q)))
+    return step.value, q
 
 
 def device_info(device=0):
@@ -428,6 +459,16 @@ class Solver:
 
     def gradient(self):
         _check(_lib.phl_gradient(self._h))
+
+    def compute_node_value(self):
+        """ComputeNodeValue + ModifyNodeValue (gradient 2, "ggnode")."""
+        _check(_lib.phl_compute_node_value(self._h))
+
+    def node_values(self):
+        """Node values [6][nNode] (five primitives, then T) of the last node pass (ggnode)."""
+        out = np.empty((6, self.mesh.n_node))
+        _check(_lib.phl_get_node_values(self._h, _dptr(out)))
+        return out
 
     def limiter(self):
         _check(_lib.phl_limiter(self._h))

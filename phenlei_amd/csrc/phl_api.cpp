@@ -68,6 +68,11 @@ struct phl_ctx {
     unsigned long long stateVersion = 0, gradVersion = ~0ull;
     // launch statistics: graph replays and eagerly launched steps
     long long graphReplays = 0, eagerSteps = 0;
+    // ggnode: the node temperatures of an iteration come from the previous
+    // iteration's t (the reference computes node values in PostSolve, before
+    // the next ComputeGamaAndTemperature); after a new state the first node
+    // pass takes the state's own t, as InitDependentVariables does
+    bool tFresh = true;
 };
 
 namespace {
@@ -292,10 +297,11 @@ phl_status check_params(const phl_params &p)
         return fail(PHL_ERR_UNSUPPORTED, "viscous must be 0 (inviscid), 1 (laminar) or 3 (Spalart-Allmaras)");
     if (p.viscous == 3 && !(p.turb_cfl_scale > 0.0)) return fail(PHL_ERR_ARG, "turb_cfl_scale must be positive");
     if (p.viscous && !(p.reynolds > 0.0)) return fail(PHL_ERR_ARG, "reynolds must be positive for viscous flow");
-    if (p.gradient != 0 && p.gradient != 1)
-        return fail(PHL_ERR_UNSUPPORTED, "gradient must be 0 (ggcell) or 1 (lsq)");
-    if (p.gradient == 1 && p.viscous == 3)
-        return fail(PHL_ERR_UNSUPPORTED, "the lsq gradient is implemented for the NS equations only (viscous 0, 1)");
+    if (p.gradient < 0 || p.gradient > 2)
+        return fail(PHL_ERR_UNSUPPORTED, "gradient must be 0 (ggcell), 1 (lsq) or 2 (ggnode)");
+    if (p.gradient != 0 && p.viscous == 3)
+        return fail(PHL_ERR_UNSUPPORTED,
+                    "the lsq and ggnode gradients are implemented for the NS equations only (viscous 0, 1)");
     if (p.limiter < 0 || p.limiter > 2) return fail(PHL_ERR_UNSUPPORTED, "limiter must be 0 (vencat), 1 (nolim) or 2 (1st)");
     if (p.entropy_fix != 1 && p.entropy_fix != 3 && p.entropy_fix != 5)
         return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1, 3 or 5");
@@ -724,6 +730,102 @@ static void locality_order(const phl::Mesh &m, std::vector<int> &cellAt)
     }
 }
 
+// Node-based Green-Gauss gradient (gradient 2): each node's contributors
+// in the order ComputeNodeValue adds them (NSSolverUnstruct.cpp:4734-5081).
+// One BC region per code, in increasing code order, faces in increasing
+// index: solid-surface faces (for solid nodes), farfield faces (for farfield
+// nodes that are not solid), the other non-symmetry codes (for nodes that
+// are neither), then the real cells in order for every other node
+// (symmetry-plane nodes included).  Weights are 1/distance from the node to
+// the face centre or cell centre, and their sums are added in the same
+// order (the reference's nodeWeight), so the device divides exactly as the
+// reference does.  The face node lists follow the stored face order.
+static phl_status upload_node_lists(phl_ctx *c, const phl::Mesh &m, const std::vector<int> &fslot)
+{
+    phl::DevMesh &dm = c->m;
+    dm.nNode = 0;
+    dm.nodeSrcPtr = nullptr; dm.nodeSrc = nullptr; dm.nodeW = nullptr; dm.nodeWsum = nullptr;
+    dm.fnPtr = nullptr; dm.fnList = nullptr;
+    if (c->p.gradient != 2) return PHL_OK;
+    const int nN = m.nNode, nC = m.nCell, nB = m.nBFace, nF = m.nFace;
+    const std::vector<int> &slotOf = c->slotOf;
+    std::vector<int> nodeBC(nN, 0);
+    auto nodesOf = [&](int f, auto &&fn) {
+        for (int j = m.faceNodePtr[f]; j < m.faceNodePtr[f + 1]; ++j) fn(m.faceNodes[j]);
+    };
+    for (int f = 0; f < nB; ++f)
+        if (m.bcType[f] == PHL_BC_SOLID_SURFACE) nodesOf(f, [&](int n) { nodeBC[n] = PHL_BC_SOLID_SURFACE; });
+    for (int f = 0; f < nB; ++f)
+        if (m.bcType[f] == PHL_BC_FARFIELD)
+            nodesOf(f, [&](int n) { if (nodeBC[n] != PHL_BC_SOLID_SURFACE) nodeBC[n] = PHL_BC_FARFIELD; });
+    for (int code = 1; code <= 6; ++code) {
+        if (code == PHL_BC_SOLID_SURFACE || code == PHL_BC_FARFIELD || code == PHL_BC_SYMMETRY) continue;
+        for (int f = 0; f < nB; ++f)
+            if (m.bcType[f] == code)
+                nodesOf(f, [&](int n) {
+                    if (nodeBC[n] != PHL_BC_SOLID_SURFACE && nodeBC[n] != PHL_BC_FARFIELD) nodeBC[n] = code;
+                });
+    }
+    std::vector<std::vector<std::pair<int2, double>>> src(nN);
+    auto add = [&](int n, double px, double py, double pz, int2 s2) {
+        const double dx = m.x[n] - px, dy = m.y[n] - py, dz = m.z[n] - pz;
+        const double dist = std::sqrt(dx * dx + dy * dy + dz * dz);
+        src[n].push_back({s2, 1.0 / dist});
+    };
+    for (int step = 0; step < 3; ++step)
+        for (int code = 1; code <= 6; ++code) {
+            if (code == PHL_BC_SYMMETRY) continue;
+            if (step == 0 && code != PHL_BC_SOLID_SURFACE) continue;
+            if (step == 1 && code != PHL_BC_FARFIELD) continue;
+            if (step == 2 && (code == PHL_BC_SOLID_SURFACE || code == PHL_BC_FARFIELD)) continue;
+            for (int f = 0; f < nB; ++f) {
+                if (m.bcType[f] != code) continue;
+                const int2 s2 = make_int2(slotOf[m.left[f]], nC + f);
+                nodesOf(f, [&](int n) {
+                    if (step == 1 && nodeBC[n] == PHL_BC_SOLID_SURFACE) return;
+                    if (step == 2 && (nodeBC[n] == PHL_BC_SOLID_SURFACE || nodeBC[n] == PHL_BC_FARFIELD)) return;
+                    add(n, m.xfc[f], m.yfc[f], m.zfc[f], s2);
+                });
+            }
+        }
+    for (int i = 0; i < nC; ++i)
+        for (int j = m.cellNodePtr[i]; j < m.cellNodePtr[i + 1]; ++j) {
+            const int n = m.cellNodes[j];
+            if (nodeBC[n] != 0) continue;
+            add(n, m.xcc[i], m.ycc[i], m.zcc[i], make_int2(slotOf[i], -1));
+        }
+    std::vector<int> ptr(nN + 1, 0);
+    std::vector<int2> list;
+    std::vector<double> w, wsum(nN, 0.0);
+    for (int n = 0; n < nN; ++n) {
+        for (const auto &e : src[n]) {
+            list.push_back(e.first);
+            w.push_back(e.second);
+            wsum[n] += e.second;
+        }
+        ptr[n + 1] = (int)list.size();
+    }
+    if (list.empty()) { list.push_back(make_int2(0, -1)); w.push_back(0.0); }
+    std::vector<int> fslotInv(nF), fptr(nF + 1, 0), flist;
+    for (int f = 0; f < nF; ++f) fslotInv[fslot[f]] = f;
+    for (int sf = 0; sf < nF; ++sf) {
+        const int f = fslotInv[sf];
+        for (int j = m.faceNodePtr[f]; j < m.faceNodePtr[f + 1]; ++j) flist.push_back(m.faceNodes[j]);
+        fptr[sf + 1] = (int)flist.size();
+    }
+    dm.nNode = nN;
+    TRY(upload(c, &dm.nodeSrcPtr, ptr));
+    TRY(upload(c, &dm.nodeSrc, list));
+    TRY(upload(c, &dm.nodeW, w));
+    TRY(upload(c, &dm.nodeWsum, wsum));
+    TRY(upload(c, &dm.fnPtr, fptr));
+    TRY(upload(c, &dm.fnList, flist));
+    TRY(dalloc(c, &c->f.nodeQ, (size_t)6 * nN));
+    TRY(dalloc(c, &c->f.faceQ, (size_t)6 * nF));
+    HIPTRY(hipMemset(c->f.nodeQ, 0, sizeof(double) * 6 * nN));
+    return PHL_OK;
+}
+
 static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
 {
     const phl_params &p = c->p;
@@ -948,6 +1050,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     dm.lsqInv = nullptr;
     if (!lsqInvS.empty()) TRY(upload(c, &dm.lsqInv, lsqInvS));
     TRY(upload(c, &dm.cflSkew, cflSkew));
+    TRY(upload_node_lists(c, m, fslot));
     TRY(upload(c, &dm.limZero, limZero));
     dm.limGhost = nullptr;
     TRY(upload(c, &dm.visFlag, visFlag));
@@ -1269,8 +1372,41 @@ phl_status phl_set_state(phl_ctx *c, const double *q)
     if (!c || !q) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
     ++c->stateVersion;
+    c->tFresh = true;
     TRY(upload_cells(c, c->f.q, 5, c->m.nTotal, q));
     return clear_gave_up(c);
+}
+
+// ggnode: before a node pass, the temperature of a state set since the last
+// one (the state's ghosts as they are).
+static void fresh_temperature(phl_ctx *c, hipStream_t s)
+{
+    if (c->k.gradient == 2 && c->tFresh) {
+        phl::launch_gamma_t(c->m, c->k, c->f, s);
+        c->tFresh = false;
+    }
+}
+
+// ComputeNodeValue + ModifyNodeValue (NSSolverUnstruct.cpp:4734-5117;
+// CallGPUComputeNodeValue :4738, CallGPUModifyNodeValue :5087).
+phl_status phl_compute_node_value(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    if (c->k.gradient != 2) return fail(PHL_ERR_ARG, "phl_compute_node_value: gradient is not ggnode");
+    HIPTRY(hipSetDevice(c->device));
+    fresh_temperature(c, c->stream);
+    phl::launch_node_values(c->m, c->f, c->stream);
+    return check_launch(c);
+}
+
+phl_status phl_get_node_values(phl_ctx *c, double *out)
+{
+    if (!c || !out) return fail(PHL_ERR_ARG, "null argument");
+    if (c->k.gradient != 2) return fail(PHL_ERR_ARG, "phl_get_node_values: gradient is not ggnode");
+    HIPTRY(hipSetDevice(c->device));
+    HIPTRY(hipMemcpyAsync(out, c->f.nodeQ, sizeof(double) * 6 * c->m.nNode, hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    return PHL_OK;
 }
 
 phl_status phl_get_state(phl_ctx *c, double *q)
@@ -1369,6 +1505,7 @@ phl_status phl_init_freestream(phl_ctx *c)
 {
     if (!c) return fail(PHL_ERR_ARG, "null argument");
     HIPTRY(hipSetDevice(c->device));
+    c->tFresh = true;
     const int nT = c->m.nTotal, nC = c->m.nCell;
     std::vector<double> h((size_t)5 * nT);
     if (c->k.nCases <= 1) {
@@ -1459,10 +1596,21 @@ phl_status phl_update_residuals(phl_ctx *c) { return face_pass(c, 3); }
 // The fused residual evaluation: LoadResiduals, Boundary, the gradient and
 // limiter pass, and both face loops in one face kernel + one gather, with the
 // residual sums of the result.
+// ggnode, after the boundary conditions: the node values (the state and the
+// previous iteration's temperature), then this iteration's temperature.
+static void node_pass(phl_ctx *c, hipStream_t s)
+{
+    if (c->k.gradient != 2) return;
+    fresh_temperature(c, s);
+    phl::launch_node_values(c->m, c->f, s);
+    phl::launch_gamma_t(c->m, c->k, c->f, s);
+}
+
 static void enqueue_residual(phl_ctx *c)
 {
     reset_step_counter(c);
     phl::launch_boundary(c->m, c->k, c->f, c->stream);
+    node_pass(c, c->stream);
     phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
     phl::launch_flux(c->m, c->k, c->f, c->stream, false);
     phl::launch_norm_final(c->m, c->k, c->f, false, c->stream);
@@ -1632,6 +1780,10 @@ phl_status phl_forces_case(phl_ctx *c, int cs, double *coef)
     hipStream_t s = c->stream;
     phl::launch_boundary(c->m, c->k, c->f, s);
     phl::launch_gamma_t(c->m, c->k, c->f, s);
+    if (c->k.gradient == 2) {  // node values of the current state and its temperature
+        c->tFresh = false;
+        phl::launch_node_values(c->m, c->f, s);
+    }
     phl::launch_grad_lim(c->m, c->k, c->f, s, true);
     // the case's wall faces are a contiguous run of the sorted wall list
     phl::DevMesh dm = c->m;
@@ -1689,6 +1841,7 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     };
     mark();
     phl::launch_boundary(c->m, c->k, c->f, s);
+    node_pass(c, s);
     // the turbulence model reads the laminar viscosity of the step's state
     if (c->k.viscous == 3) phl::launch_gamma_t(c->m, c->k, c->f, s);
     mark();
@@ -1850,6 +2003,10 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
     if (!c || n_steps < 0) return fail(PHL_ERR_ARG, "bad argument");
     HIPTRY(hipSetDevice(c->device));
     ++c->stateVersion;
+    if (c->k.gradient == 2 && c->tFresh) {  // a new state's own temperature, outside the captured steps
+        phl::launch_boundary(c->m, c->k, c->f, c->stream);
+        fresh_temperature(c, c->stream);
+    }
     const int K = PHL_N_KERNEL_CLASSES;
     if (c->timing && c->events.empty()) {
         c->events.resize((size_t)kTimedChunk * (K + 1));

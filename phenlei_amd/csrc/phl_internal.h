@@ -38,6 +38,18 @@ struct DevMesh {
     const double *venkatVf;     // [nCell] (vol^(1/3))^3, Limiter.cpp:1126-1128
     const double *lsqInv;       // [6][nCell] least-squares inverse normal matrix xx, yy, zz, xy, xz, yz (gradient 1)
     const double *cflSkew;      // [nCell] skewness CFL factor, NSSolverUnstruct.cpp:6106-6116
+    // Node-based Green-Gauss gradient (gradient 2): the contributors of each
+    // node's inverse-distance average in the reference's accumulation order
+    // (ComputeNodeValue, NSSolverUnstruct.cpp:4734-5081), as storage indices
+    // (cell, -1) or, for a boundary face, (cell, its ghost), with their
+    // weights and the weight sums added in the same order; the node lists of
+    // the stored faces for the face means.
+    int nNode;
+    const int *nodeSrcPtr;      // [nNode+1]
+    const int2 *nodeSrc;
+    const double *nodeW;
+    const double *nodeWsum;     // [nNode]
+    const int *fnPtr, *fnList;  // [nFace+1], node lists per stored face
     const unsigned char *limZero;  // [nCell] 1 if next to a farfield-type face
     const double *limGhost;     // [nBFace] limiter value of the ghost cell
     const unsigned char *visFlag;  // [nFace] bit0: skewness correction, bit1: normal derivative
@@ -102,7 +114,7 @@ struct Consts {
     int fluxTile, gradTile; // cells per workgroup of the flux and gradient kernels (64, 128, 256)
     int coefF32;            // mixed form: LU-SGS face coefficients stored in FP32 (sweep arithmetic stays FP64)
     int coefInSweep;        // dataflow sweeps evaluate the face coefficients themselves (none stored)
-    int gradient;           // 0 Green-Gauss cell, 1 least squares
+    int gradient;           // 0 Green-Gauss cell, 1 least squares, 2 Green-Gauss node
     int sweepForm;          // 0, 2: dataflow sweep (launch order, ticket order); 1: one launch per level
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     const CaseConsts *cases;
@@ -153,6 +165,8 @@ struct DevFlow {
     double *forceOut;   // [15] wall force and moment sums
     unsigned char *frozen;  // [nCases] batched contexts: 1 once a case has converged
     double *vist;     // [nTotal] eddy viscosity (0 unless SA)
+    double *nodeQ;    // gradient 2: [6][nNode] node values of the five primitives and T
+    double *faceQ;    // gradient 2: [6][nFace] face means of the node values
     GatherRec *rec;   // [nTotal] face-loop gather records
     double *faceInv;  // [nFace][6] inviscid flux (area-scaled) and spectral-radius term of each face
     double *faceVis;  // [nFace][6] viscous flux and spectral-radius term (viscous flow only)
@@ -214,6 +228,9 @@ void launch_gamma_t(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s
 // read them there).  fp64Grad: also store FP64 SoA gradients, even in the
 // mixed form (the force integration).
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad = false);
+// Node values of the node-based gradient (ComputeNodeValue +
+// ModifyNodeValue): from the current q and the temperature in f.t.
+void launch_node_values(const DevMesh &m, DevFlow &f, hipStream_t s);
 // Copy the gradients of the gather records to the SoA arrays of the
 // context's precision (grad, or gradF in the mixed form).
 void launch_rec_grad(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);

@@ -28,6 +28,7 @@ namespace phl {
 namespace {
 
 constexpr double kSMALL = 1.0e-40;
+constexpr int kBcSymmetry = 3;  // SYMMETRY, Geo_SimpleBC.h:32-39
 constexpr double kTINY = 1.0e-40;
 constexpr int kBC_EXTRAP = 1, kBC_WALL = 2, kBC_SYMM = 3, kBC_FAR = 4, kBC_OUTFLOW = 6;
 constexpr int kBlock = 256;
@@ -374,6 +375,56 @@ __device__ __forceinline__ void sweep_apply(const Consts &K, const double *co, c
     }
 }
 
+// ------------------------------------------------ node values (ggnode)
+// ComputeNodeValue + ModifyNodeValue (NSSolverUnstruct.cpp:4734-5117), one
+// thread per node: the inverse-distance sum of the node's contributors in
+// the reference's order (a boundary face contributes the average of its cell
+// and ghost, an interior node the real cells around it), divided by the
+// weight sum.  t is the temperature array as the last gamma/T pass left it.
+__global__ void __launch_bounds__(kBlock) k_node_values(DevMesh M, const double *__restrict__ q,
+                                                        const double *__restrict__ t, double *__restrict__ nodeQ)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nN = M.nNode, nT = M.nTotal;
+    if (i >= nN) return;
+    double acc[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = M.nodeSrcPtr[i]; k < M.nodeSrcPtr[i + 1]; ++k) {
+        const int2 src = M.nodeSrc[k];
+        const double w = M.nodeW[k];
+        if (src.y >= 0) {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) acc[m] += 0.5 * (q[(size_t)m * nT + src.x] + q[(size_t)m * nT + src.y]) * w;
+            acc[5] += 0.5 * (t[src.x] + t[src.y]) * w;
+        } else {
+#pragma unroll
+            for (int m = 0; m < 5; ++m) acc[m] += q[(size_t)m * nT + src.x] * w;
+            acc[5] += t[src.x] * w;
+        }
+    }
+    const double ws = M.nodeWsum[i];
+#pragma unroll
+    for (int m = 0; m < 6; ++m) nodeQ[(size_t)m * nN + i] = ws > kSMALL ? acc[m] / ws : 0.0;
+}
+
+// Face means of the node values (CompGradientGGNode_NEW's face value,
+// Geo_UnstructGrid.cpp:4238-4250,4268-4273): the node values added in the
+// face's node order, divided by the node count.
+__global__ void __launch_bounds__(kBlock) k_face_node_mean(DevMesh M, const double *__restrict__ nodeQ,
+                                                           double *__restrict__ faceQ)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nF = M.nFace, nN = M.nNode;
+    if (f >= nF) return;
+    const int b = M.fnPtr[f], e = M.fnPtr[f + 1];
+    const double cnt = (double)(e - b);
+#pragma unroll
+    for (int m = 0; m < 6; ++m) {
+        double v = 0.0;
+        for (int j = b; j < e; ++j) v += nodeQ[(size_t)m * nN + M.fnList[j]];
+        faceQ[(size_t)m * nF + f] = v / cnt;
+    }
+}
+
 // ------------------------------------------------ gradients + limiter
 // GT is the storage type of the gradients: double, or float for the mixed
 // form (computed in FP64, stored rounded to FP32).
@@ -382,7 +433,7 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
                                                      GT *__restrict__ grad, GT *__restrict__ gradT,
                                                      double *__restrict__ lim, double *__restrict__ coef,
                                                      const double *__restrict__ vist, GatherRec *__restrict__ rec,
-                                                     bool gradSoA)
+                                                     bool gradSoA, const double *__restrict__ faceQ)
 {
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const int nC = M.nCell, nT = M.nTotal;
@@ -422,16 +473,22 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
                 tgx += dq * dx; tgy += dq * dy; tgz += dq * dz;
             }
         } else {
+            // Green-Gauss: the face value is the average of the two cells
+            // (ggcell), or for ggnode the mean of the face's node values on
+            // interior and symmetry faces (other boundary faces keep the
+            // average of the cell and its ghost)
             const double nx = fr.nx * fr.area, ny = fr.ny * fr.area, nz = fr.nz * fr.area;
+            const bool nodeFace = K.gradient == 2 && (f >= M.nBFace || M.bcType[f] == kBcSymmetry);
+            const int nF = M.nFace;
 #pragma unroll
             for (int m = 0; m < 5; ++m) {
-                const double qfc = 0.5 * (qc[m] + qn[m]);
+                const double qfc = nodeFace ? faceQ[(size_t)m * nF + f] : 0.5 * (qc[m] + qn[m]);
                 if (isL) { gx[m] += qfc * nx; gy[m] += qfc * ny; gz[m] += qfc * nz; }
                 else     { gx[m] -= qfc * nx; gy[m] -= qfc * ny; gz[m] -= qfc * nz; }
             }
             if (K.viscous) {
                 const double tn = qn[4] / (K.R * qn[0]);
-                const double tfc = 0.5 * (tc + tn);
+                const double tfc = nodeFace ? faceQ[(size_t)5 * nF + f] : 0.5 * (tc + tn);
                 if (isL) { tgx += tfc * nx; tgy += tfc * ny; tgz += tfc * nz; }
                 else     { tgx -= tfc * nx; tgy -= tfc * ny; tgz -= tfc * nz; }
             }
@@ -474,6 +531,11 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
             gy[m] = lxy * tx + lyy * ty + lyz * tz;
             gz[m] = lxz * tx + lyz * ty + lzz * tz;
         }
+    } else if (K.gradient == 2) {  // CompGradientGGNode_NEW divides by the volume
+#pragma unroll
+        for (int m = 0; m < 5; ++m) {
+            gx[m] /= M.vol[c]; gy[m] /= M.vol[c]; gz[m] /= M.vol[c];
+        }
     } else {
 #pragma unroll
         for (int m = 0; m < 5; ++m) {
@@ -494,6 +556,8 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
             tgx = lxx * tx + lxy * ty + lxz * tz;
             tgy = lxy * tx + lyy * ty + lyz * tz;
             tgz = lxz * tx + lyz * ty + lzz * tz;
+        } else if (K.gradient == 2) {
+            tgx /= M.vol[c]; tgy /= M.vol[c]; tgz /= M.vol[c];
         } else {
             tgx *= ovol; tgy *= ovol; tgz *= ovol;
         }
@@ -540,7 +604,8 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
 #pragma unroll
         for (int m = 0; m < 5; ++m) { g[m] = GT(gx[m]); g[5 + m] = GT(gy[m]); g[10 + m] = GT(gz[m]); }
         r.lim = l;
-        if (K.gradient == 1) {  // a ghost takes its owner's least-squares gradient (Geo_UnstructGrid.cpp:4077-4084)
+        if (K.gradient != 0) {  // a ghost takes its owner's gradient (lsq: Geo_UnstructGrid.cpp:4077-4084;
+                                // ggnode: :4301-4314); ggcell leaves it zero
             for (int s = 0; s < M.ellW; ++s) {
                 const int enc = M.c2fEnc[(size_t)s * nC + c];
                 if (enc == 0) break;
@@ -906,7 +971,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                     T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
                     if (li < nC) { gTL[0] = gradT[li]; gTL[1] = gradT[(size_t)nC + li]; gTL[2] = gradT[(size_t)2 * nC + li]; }
                     if (ri < nC) { gTR[0] = gradT[ri]; gTR[1] = gradT[(size_t)nC + ri]; gTR[2] = gradT[(size_t)2 * nC + ri]; }
-                    else if (K.gradient == 1) { gTR[0] = gTL[0]; gTR[1] = gTL[1]; gTR[2] = gTL[2]; }  // ghost: owner's
+                    else if (K.gradient != 0) { gTR[0] = gTL[0]; gTR[1] = gTL[1]; gTR[2] = gTL[2]; }  // ghost: owner's
                     const bool wall = bnd && M.bcType[f] == kBC_WALL;
                     double fl[5];
                     viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[li], vist[ri], fl);
@@ -1077,7 +1142,7 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
         T gTL[3] = {T(0.0), T(0.0), T(0.0)}, gTR[3] = {T(0.0), T(0.0), T(0.0)};
         if (le < nC) { gTL[0] = gradT[le]; gTL[1] = gradT[(size_t)nC + le]; gTL[2] = gradT[(size_t)2 * nC + le]; }
         if (re < nC) { gTR[0] = gradT[re]; gTR[1] = gradT[(size_t)nC + re]; gTR[2] = gradT[(size_t)2 * nC + re]; }
-        else if (K.gradient == 1) { gTR[0] = gTL[0]; gTR[1] = gTL[1]; gTR[2] = gTL[2]; }  // ghost: owner's lsq gradient
+        else if (K.gradient != 0) { gTR[0] = gTL[0]; gTR[1] = gTL[1]; gTR[2] = gTL[2]; }  // ghost: owner's lsq gradient
         const bool wall = bnd && bc == kBC_WALL;
         double vis[6];
         viscous_face(M, K, f, fr, L, R, gTL, gTR, bnd, wall, vist[le], vist[re], vis);
@@ -1164,7 +1229,7 @@ __global__ void __launch_bounds__(kBlock) k_load_res(double *__restrict__ res, i
 // Block partials of res^2 over 256-cell blocks of the storage order, grouped
 // and summed exactly as k_cell_gather<256> groups them, so a norm taken after
 // the hook sequence equals the one of the fused residual evaluation.
-__global__ void __launch_bounds__(256) k_res_partial(int nC, const double *__restrict__ res,
+__global__ void __launch_bounds__(256) k_res_partial(int nC, size_t ld, const double *__restrict__ res,
                                                      double *__restrict__ normPart)
 {
     const int lb = xcd_block(blockIdx.x, gridDim.x);
@@ -1172,7 +1237,7 @@ __global__ void __launch_bounds__(256) k_res_partial(int nC, const double *__res
     __shared__ double red[5][256];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        const double v = c < nC ? res[(size_t)m * nC + c] : 0.0;
+        const double v = c < nC ? res[(size_t)m * ld + c] : 0.0;
         red[m][threadIdx.x] = c < nC ? v * v : 0.0;
     }
     __syncthreads();
@@ -1382,11 +1447,12 @@ __global__ void __launch_bounds__(BS) k_flux_tile(DevMesh M, Consts K0, const Ga
 // Residual sums of the step (row *stepCtr of normOut) and, optionally, the
 // global minimum of the local time step (ComputeMinTimeStep; a minimum is
 // independent of the reduction order).
-__global__ void k_norm_final(const double *__restrict__ part, const double *__restrict__ dtPart, int nPart,
-                             double *__restrict__ normOut, const int *__restrict__ stepCtr,
-                             double *__restrict__ dtMin, int withDt)
+// The partials are summed in a fixed order: thread t takes parts t, t+kBlock,
+// ..., then a pairwise tree over the threads; red[m][0] holds sum m (and
+// red[5][0] the minimum time step).
+__device__ __forceinline__ void sum_parts(const double *__restrict__ part, const double *__restrict__ dtPart,
+                                          int nPart, int withDt, double (*red)[kBlock])
 {
-    __shared__ double red[6][kBlock];
     double a[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     double mn = 1.0e300;
     for (int i = threadIdx.x; i < nPart; i += blockDim.x) {
@@ -1406,6 +1472,16 @@ __global__ void k_norm_final(const double *__restrict__ part, const double *__re
         }
         __syncthreads();
     }
+}
+
+__global__ void __launch_bounds__(kBlock) k_norm_final(const double *__restrict__ part,
+                                                       const double *__restrict__ dtPart, int nPart,
+                                                       double *__restrict__ normOut,
+                                                       const int *__restrict__ stepCtr,
+                                                       double *__restrict__ dtMin, int withDt)
+{
+    __shared__ double red[6][kBlock];
+    sum_parts(part, dtPart, nPart, withDt, red);
     if (threadIdx.x < 5) normOut[(size_t)(*stepCtr) * 5 + threadIdx.x] = red[threadIdx.x][0];
     if (threadIdx.x == 5 && withDt) *dtMin = red[5][0];
 }
@@ -2289,12 +2365,19 @@ static void launch_grad_lim_t(const DevMesh &m, const Consts &k, DevFlow &f, hip
 {
     // the gather records hold the gradient type of the context's face loop
     GatherRec *rec = (fp32 == (k.mixed != 0)) ? f.rec : nullptr;
+    if (k.gradient == 2)  // ggnode: face means of the node values in place
+        hipLaunchKernelGGL(k_face_node_mean, dim3(nblk(m.nFace, kBlock)), dim3(kBlock), 0, s, m, f.nodeQ, f.faceQ);
     if (fp32)
         hipLaunchKernelGGL((k_grad_lim<float, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.gradF,
-                           f.gradTF, f.lim, f.coef, f.vist, rec, soa);
+                           f.gradTF, f.lim, f.coef, f.vist, rec, soa, f.faceQ);
     else
         hipLaunchKernelGGL((k_grad_lim<double, BS>), dim3(nblk(m.nCell, BS)), dim3(BS), 0, s, m, k, f.q, f.grad,
-                           f.gradT, f.lim, f.coef, f.vist, rec, soa);
+                           f.gradT, f.lim, f.coef, f.vist, rec, soa, f.faceQ);
+}
+
+void launch_node_values(const DevMesh &m, DevFlow &f, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_node_values, dim3(nblk(m.nNode, kBlock)), dim3(kBlock), 0, s, m, f.q, f.t, f.nodeQ);
 }
 
 void launch_grad_lim(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, bool fp64Grad)
@@ -2418,7 +2501,7 @@ void launch_residual_norm(const DevMesh &m, const Consts &k, DevFlow &f, hipStre
 {
     if (k.nCases > 1) return launch_norm_final(m, k, f, false, s);  // per-case sums read res directly
     const int nb = nblk(m.nCell, 256);
-    hipLaunchKernelGGL(k_res_partial, dim3(nb), dim3(256), 0, s, m.nCell, f.res, f.normPart);
+    hipLaunchKernelGGL(k_res_partial, dim3(nb), dim3(256), 0, s, m.nCell, (size_t)m.nCell, f.res, f.normPart);
     hipLaunchKernelGGL(k_norm_final, dim3(1), dim3(kBlock), 0, s, f.normPart, f.dtMinPart, nb, f.normOut, f.stepCtr,
                        f.dtMin, 0);
 }

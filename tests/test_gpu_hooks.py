@@ -24,9 +24,11 @@ def _pair(mesh, **pover):
     return gpu, cpu
 
 
-def _gpu_hooks(s, step, split=False, turb=False):
+def _gpu_hooks(s, step, split=False, turb=False, ggnode=False):
     s.load_residuals()
     s.boundary()
+    if ggnode:  # ComputeNodeValue of the previous PostSolve (Controller.cpp:5151-5198)
+        s.compute_node_value()
     s.gamma_t()
     s.gradient()
     s.limiter()
@@ -43,9 +45,11 @@ def _gpu_hooks(s, step, split=False, turb=False):
     return n
 
 
-def _cpu_hooks(c, step, viscous):
+def _cpu_hooks(c, step, viscous, ggnode=False):
     c.zero_residual()
     c.boundary()
+    if ggnode:
+        c.compute_node_value()
     c.gamma_t()
     if viscous:
         c.viscosity()
@@ -64,6 +68,10 @@ CASES = {
     "hex_inviscid": (lambda: pa.Mesh.bump_hex(16, 8, 4, 0.05), dict(mach=0.5, cfl_start=10.0, cfl_end=10.0)),
     "hybrid_laminar": (lambda: pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7),
                        dict(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0)),
+    "hex_inviscid_ggnode": (lambda: pa.Mesh.bump_hex(16, 8, 4, 0.05),
+                            dict(mach=0.5, cfl_start=10.0, cfl_end=10.0, gradient=2)),
+    "hybrid_laminar_ggnode": (lambda: pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7),
+                              dict(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0, gradient=2)),
     "wing_sa": (lambda: pa.Mesh.swept_wing(10, 5, 4, 1e-3, 3),
                 dict(mach=0.6, alpha_deg=2.0, viscous=3, reynolds=3e6, cfl_start=1.0, cfl_end=5.0,
                      cfl_vary_step=50)),
@@ -86,9 +94,10 @@ def test_hook_sequence_equals_iterate_and_oracle(kind, split):
     if viscous == 3:
         wd = gpu.get_field("wdist")[:m.n_cell]
         cpu.set_wall_distance(wd)
+    gg = pover.get("gradient", 0) == 2
     for step in (1, 2, 3):
-        ng = _gpu_hooks(gpu, step, split=split, turb=viscous == 3)
-        nc = _cpu_hooks(cpu, step, viscous)
+        ng = _gpu_hooks(gpu, step, split=split, turb=viscous == 3, ggnode=gg)
+        nc = _cpu_hooks(cpu, step, viscous, ggnode=gg)
         ref.iterate(step, 1)
         assert ng == nc == 0
         qg, qr = gpu.get_state(), ref.get_state()
