@@ -17,6 +17,7 @@ Besides the contract fields the JSON line carries:
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -30,11 +31,6 @@ import phenlei_amd as pa  # noqa: E402
 METRIC = "cell-updates/s and time-to-1e-8 residual drop: mixed+warm-start vs plain FP64"
 HBM_PEAK_GBS = 8000.0
 
-# HBM bytes of one face loop (k_face_flux + k_cell_gather) measured with
-# rocprofv3 FETCH_SIZE (x2.0, the factor a 1 GiB copy kernel calibrates) +
-# WRITE_SIZE (x1.0), per workload: profiles/k_flux_bump_hybrid_1m.md.  None
-# where not measured.
-MEASURED_FLUX_TRAFFIC = {"c2": 1380.6e6}
 
 # Workloads.  'c1' is the reference's CPU-runnable case (64x32x8 hex bump
 # channel, inviscid, M0.5, CFL 10).
@@ -63,8 +59,12 @@ CONFIGS["c3"] = dict(workload="swept wing (root chord 1, taper 0.5, AR 4, quarte
                      params=dict(mach=0.6, alpha_deg=2.0, viscous=3, reynolds=3e6, cfl_start=0.01, cfl_end=5.0,
                                  cfl_vary_step=500, force_ref_area=1.125, force_ref_length=0.7778,
                                  force_ref_span=1.5),
-                     cpu_steps=1)
-DEFAULT_CONFIG = "c2"
+                     cpu_steps=1,
+                     # the headline leg: time to a residual drop, plain FP64 from freestream against the mixed
+                     # form started from the learned warm start (judged by the solver's residual)
+                     converge=dict(drop=0.25, max_steps=5000))
+DEFAULT_CONFIG = "c3"
+WING_WEIGHTS = os.path.join(ROOT, "phenlei_amd", "data", "wing_warmstart.npz")
 
 # Batched sweep: 64 inviscid bump-channel cases in one context, sizes and
 # freestream drawn from a fixed seed.
@@ -142,7 +142,21 @@ def sweep_algorithmic_bytes(n_cell, n_face, mixed=False, viscous=False, levels_f
     return cell * n_cell + 2 * n_face * side
 
 
-def cpu_baseline(mesh, params, steps, wall_distance=None):
+def host_cpu():
+    """(logical CPUs, CPU model) of this host."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for ln in f:
+                if ln.startswith("model name"):
+                    model = ln.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return os.cpu_count(), model
+
+
+def cpu_baseline(mesh, params, steps, wall_distance=None, warmup=1):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as orc  # the checker, timed here as the CPU baseline only
     conn, met = mesh.connectivity(), mesh.metrics()
@@ -150,13 +164,17 @@ def cpu_baseline(mesh, params, steps, wall_distance=None):
     if wall_distance is not None:
         cpu.set_wall_distance(wall_distance)  # preprocessing, outside the timed iterations
     cpu.init_freestream()
+    if warmup:
+        cpu.iterate(1, warmup)
     t0 = time.perf_counter()
-    cpu.iterate(1, steps)
+    cpu.iterate(1 + warmup, steps)
     dt = time.perf_counter() - t0
     cpu.close()
+    nproc, model = host_cpu()
     return dict(value=mesh.n_cell * steps / dt, unit="cell-updates/s", cores=1, kind="port",
-                sample=f"{steps} FP64 iterations of the same mesh from freestream "
-                       f"({mesh.n_cell} cells, {dt:.1f} s, sequential C oracle)")
+                nproc=nproc, cpu_model=model,
+                sample=f"{steps} FP64 iteration(s) after {warmup} warm-up iteration(s) of the same mesh from "
+                       f"freestream ({mesh.n_cell} cells, {dt:.1f} s timed, sequential C oracle on one core)")
 
 
 def time_to_drop(mesh, pover, drop, max_steps, chunk=256, stall=2048):
@@ -201,6 +219,160 @@ def time_to_drop(mesh, pover, drop, max_steps, chunk=256, stall=2048):
     out["drop"] = drop
     out["target_abs"] = target
     out["warm_start"] = None  # the warm-start network is trained per family of cases (tests, warmstart.py)
+    return out
+
+
+COPY_BYTES = 2 * 1024**3  # the calibration copy: 1 GiB read and 1 GiB written per launch
+
+
+def traffic_child(cfg):
+    """The workload of a counter pass: two warm-up and two eager steps,
+    then the calibration copy."""
+    mesh = build_mesh(cfg["mesh"])
+    s = pa.Solver(mesh, pa.default_params(**cfg["params"]))
+    s.set_launch_mode("eager")
+    s.init_freestream()
+    s.iterate(1, 4, history=False)
+    s.synchronize()
+    s.close()
+    pa.hbm_copy_gbps(0, COPY_BYTES, 2)
+
+
+def _counter_means(out_dir):
+    """Mean counter value per dispatch, by (kernel, counter), from a
+    rocprofv3 csv counter collection."""
+    import csv
+    import glob
+    sums = {}
+    for path in glob.glob(os.path.join(out_dir, "**", "*counter_collection.csv"), recursive=True):
+        with open(path) as f:
+            for row in csv.DictReader(f):
+                m = re.search(r"\b(k_\w+)(<[^()]*>)?", row["Kernel_Name"])
+                if not m:
+                    continue
+                key = (m.group(0), row["Counter_Name"])
+                v = sums.setdefault(key, [0.0, 0])
+                v[0] += float(row["Counter_Value"])
+                v[1] += 1
+    return {k: v[0] / v[1] for k, v in sums.items() if v[1]}
+
+
+def measure_traffic(config, prefixes, counters=(("FETCH_SIZE",), ("WRITE_SIZE",)), timeout=240):
+    """HBM bytes per dispatch of the kernels whose names start with one of
+    `prefixes`, measured in this run: one rocprofv3 counter pass per counter
+    set over a short child run of the same workload, each counter scaled by
+    what it reads for the copy kernel of known size in the same pass (1 GiB
+    read, 1 GiB written).  Returns {kernel: bytes} and the scale factors, or
+    None if the profiler is not available."""
+    import shutil
+    import subprocess
+    import tempfile
+    exe = shutil.which("rocprofv3")
+    if not exe:
+        return None
+    per_kernel, factors = {}, {}
+    work = tempfile.mkdtemp(prefix="phl_pmc_", dir="/tmp")
+    env = dict(os.environ, TMPDIR="/tmp")
+    try:
+        for group in counters:
+            out = os.path.join(work, group[0].lower())
+            cmd = [exe, "--pmc", *group, "--output-format", "csv", "-d", out, "-o", "pass", "--",
+                   sys.executable, os.path.join(ROOT, "bench.py"), "--traffic-child", "--config", config]
+            print(f"[bench] counter pass {' '.join(group)}", file=sys.stderr, flush=True)
+            r = subprocess.run(cmd, cwd="/tmp", env=env, capture_output=True, text=True, timeout=timeout)
+            if r.returncode != 0:
+                print(f"[bench] counter pass failed ({r.returncode}): {r.stderr[-400:]}", file=sys.stderr, flush=True)
+                return None
+            means = _counter_means(out)
+            for cname in group:
+                copy = means.get(("k_copy", cname))
+                if not copy:
+                    return None
+                # a copy launch reads and writes COPY_BYTES / 2 each
+                factor = (COPY_BYTES / 2) / (copy * 1024.0)
+                factors[cname] = factor
+                for (kname, c2), v in means.items():
+                    if c2 == cname and any(kname.startswith(p) for p in prefixes):
+                        per_kernel.setdefault(kname, {})[cname] = v * 1024.0 * factor
+    except (OSError, subprocess.TimeoutExpired) as e:
+        print(f"[bench] counter pass not run: {e}", file=sys.stderr, flush=True)
+        return None
+    finally:
+        shutil.rmtree(work, ignore_errors=True)
+    return dict(kernels=per_kernel, factors=factors)
+
+
+def wing_time_to_drop(mesh, pover, drop, max_steps, weights, chunk=256, stall=1024):
+    """The headline comparison on the wing: wall time until the density-
+    residual RMS falls to drop * R_1, R_1 being the first residual of the
+    plain FP64 run from freestream (one absolute target for both runs), for
+    (a) plain FP64 from freestream and (b) the mixed form started from the
+    learned warm start: the network (trained on coarse wing cases, loaded
+    from `weights`) predicts the field, the solver's first residual judges
+    the prediction against the freestream start (the fallback), and the run
+    switches to FP64 if it stalls for `stall` steps.  (b)'s clock covers the
+    network load, the feature assembly, the forward pass and the judging.
+    A run that misses the target counts at its full wall time."""
+    from phenlei_amd import warmstart as ws
+    case = dict(mach=pover["mach"], alpha=pover["alpha_deg"])
+    out = dict(drop=drop, max_steps=max_steps, weights=os.path.relpath(weights, ROOT) if weights else None)
+    fields = {}
+    r1 = None
+    for form in ("fp64", "mixed_warm_start"):
+        s = pa.Solver(mesh, pa.default_params(**dict(pover, precision=0 if form == "fp64" else 1)))
+        s.init_freestream()
+        s.synchronize()
+        rec = {}
+        t0 = time.perf_counter()
+        if form != "fp64":
+            if weights and os.path.exists(weights):
+                net, meta = ws.load_network(weights)
+                cands = ws.wing_candidates(net, mesh, case, s.get_field("wdist"))
+                label, r_start, seen = ws.judged_start(s, cands, r1)
+                net.close()
+                rec.update(start=label, start_residual=r_start, candidates=[
+                    dict(fade=w, first_residual=r, ratio_to_freestream=r / r1) for w, r in seen],
+                    network=dict(train_cases=int(meta["n_cases"]), train_cells=int(np.prod(meta["train_dims"])),
+                                 solve_steps=int(meta["solve_steps"]), adam_steps=int(meta["adam_steps"])))
+            else:
+                rec.update(start="freestream (no trained network)")
+        n, hit, best, mark, since, switched = 0, -1, np.inf, np.inf, 0, None
+        while n < max_steps and hit < 0:
+            h = s.iterate(1 + n, min(chunk, max_steps - n))
+            if r1 is None:
+                r1 = float(h[0])
+            below = np.nonzero(h <= drop * r1)[0]
+            if below.size:
+                hit = n + int(below[0]) + 1
+            for i, v in enumerate(h):
+                best = min(best, float(v))
+                if best < 0.5 * mark:
+                    mark, since = best, n + i
+            n += h.size
+            if form != "fp64" and switched is None and s.precision() == "mixed" and n - since >= stall:
+                s.set_precision("fp64")
+                switched = n
+            print(f"[bench]   {form}: step {n}, residual {h[-1]:.3e} ({h[-1] / r1:.3e} of R_1)", file=sys.stderr,
+                  flush=True)
+        s.synchronize()
+        wall = time.perf_counter() - t0
+        f = s.forces()
+        rec.update(iterations=hit if hit > 0 else -n, seconds=round(wall, 3), reached=hit > 0,
+                   switched_to_fp64_at=switched, cl=f["CL"], cd=f["CD"])
+        fields[form] = s.get_state()[:, :mesh.n_cell]
+        out[form] = rec
+        print(f"[bench] time to {drop:g} x R_1, {form}: {rec}", file=sys.stderr, flush=True)
+        s.close()
+    out["r1"] = r1
+    a, b = out["fp64"], out["mixed_warm_start"]
+    out["speedup"] = round(a["seconds"] / b["seconds"], 3)
+    out["speedup_note"] = ("wall time of plain FP64 over mixed + warm start to the same absolute residual; a run "
+                           "that missed it counts at its full wall time")
+    out["dcl"], out["dcd"] = abs(a["cl"] - b["cl"]), abs(a["cd"] - b["cd"])
+    qa, qb = fields["fp64"], fields["mixed_warm_start"]
+    dev = qa - np.mean(qa, axis=1, keepdims=True)
+    out["field_l2_rel"] = [float(np.linalg.norm(qa[m] - qb[m]) / max(np.linalg.norm(dev[m]), 1e-300))
+                           for m in range(5)]
     return out
 
 
@@ -370,7 +542,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--config", default=DEFAULT_CONFIG, choices=sorted(CONFIGS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--copy-probe", action="store_true", help="also run the HBM copy probe")
+    ap.add_argument("--copy-probe", action="store_true", help="(the HBM copy probe always runs)")
     ap.add_argument("--autotune", action="store_true", help="pick the flux/gradient tiles by timing first")
     ap.add_argument("--launch", default="graph", choices=["graph", "eager", "eager_overlap"],
                     help="replay captured hipGraphs (default), launch every kernel eagerly on one stream, or "
@@ -400,9 +572,16 @@ def main():
     ap.add_argument("--converge", type=float, default=None, metavar="DROP",
                     help="also time the FP64 and mixed-then-FP64 forms to this residual drop from freestream")
     ap.add_argument("--max-steps", type=int, default=20000, help="iteration cap of --converge")
+    ap.add_argument("--no-converge", action="store_true",
+                    help="skip the configuration's default time-to-drop leg (the wing's headline comparison)")
     ap.add_argument("--forms", default="fp64,mixed_then_fp64",
                     help="forms timed by --converge on a batch (comma separated)")
+    ap.add_argument("--traffic-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--no-traffic", action="store_true", help="skip the in-run HBM counter passes")
     args = ap.parse_args()
+    if args.traffic_child:
+        traffic_child(CONFIGS[args.config])
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
@@ -440,6 +619,8 @@ def main():
         print(f"[bench] {mesh.n_cell} cells: creating the context", file=sys.stderr, flush=True)
         solver = pa.Solver(mesh, params, device=0)
     solver.set_launch_mode(args.launch)
+    # the device's exact wall distance, for the CPU baseline's SA model (preprocessing)
+    wdist = solver.get_field("wdist")[:mesh.n_cell].copy() if params.viscous == 3 else None
     solver.init_freestream()
     tuned = None
     if args.autotune or cfg.get("batch"):
@@ -466,25 +647,42 @@ def main():
     mixed = params.precision == 1
     fb = flux_algorithmic_bytes(mesh.n_cell, mesh.n_face, params.viscous, mixed)
     achieved = fb / (flux_ms * 1e-3) / 1e9
-    traffic = MEASURED_FLUX_TRAFFIC.get(args.config) if not mixed else None
-    roof = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
-                frac=round(achieved / HBM_PEAK_GBS, 4), traffic=traffic,
-                kernel="k_face_flux + k_cell_gather", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb)
+    roof_face = dict(bound="hbm", achieved=round(achieved, 1), peak=HBM_PEAK_GBS, unit="GB/s",
+                     frac=round(achieved / HBM_PEAK_GBS, 4), traffic=None,
+                     kernel="k_face_flux + k_cell_gather", kernel_ms=round(flux_ms, 5), algorithmic_bytes=fb,
+                     bytes_note="240N + 64F (+32N viscous; FP32 gradients: 180N + 64F)")
     per_class = {k: round(v / max(n_timed, 1), 5) for k, v in times.items()}
     sweep_ms = (times["sweep_fwd"] + times["sweep_bwd"]) / max(n_timed, 1)
-    sb = sweep_algorithmic_bytes(mesh.n_cell, mesh.n_face, mixed, params.viscous != 0, params.sweep_form == 1)
-    sweep_gbs = sb / (sweep_ms * 1e-3) / 1e9
     levels_form = params.sweep_form == 1
+    # the LU-SGS pair's compulsory bytes as the hot-path survey prices them:
+    # per sweep read res, q, dq, the off-diagonal sums (160N), the diagonal
+    # (8N), normal and area (32F), connectivity (16F); write dq and the sums
+    # (80N): 496N + 96F for both sweeps
+    sb = 496 * mesh.n_cell + 96 * mesh.n_face
+    sweep_gbs = sb / (sweep_ms * 1e-3) / 1e9
     roof_sweep = dict(bound="hbm", achieved=round(sweep_gbs, 1), peak=HBM_PEAK_GBS, unit="GB/s",
                       frac=round(sweep_gbs / HBM_PEAK_GBS, 4), traffic=None,
-                      kernel="k_sweep_level_so + k_sweep_block_so" if levels_form else "k_sweep_flow",
-                      kernel_ms=round(sweep_ms, 5), algorithmic_bytes=sb,
+                      kernel="k_sweep_level_so + k_sweep_block_so (forward + backward)" if levels_form
+                      else "k_sweep_flow (forward + backward launch)",
+                      kernel_ms=round(sweep_ms, 5), algorithmic_bytes=sb, bytes_note="496N + 96F",
                       note="latency bound: each dependency level is a launch with two dependent memory rounds"
                       if levels_form else "latency bound: one launch per direction; each dependency level costs "
                       "a flag round trip, the neighbours' dq loads and the store acknowledgement")
     print(f"[bench] mean ms per step by kernel class: {per_class}", file=sys.stderr, flush=True)
-    if args.copy_probe:
-        roof["copy_probe_gbs"] = round(pa.hbm_copy_gbps(0, 2 * 1024**3, 10), 1)
+    # the dominant kernel of the step carries the contract's roofline object
+    roof, other = (roof_sweep, roof_face) if sweep_ms >= flux_ms else (roof_face, roof_sweep)
+    copy_gbs = round(pa.hbm_copy_gbps(0, COPY_BYTES, 10), 1)
+    roof["copy_probe_gbs"] = copy_gbs
+    if not args.no_traffic and not cfg.get("batch"):
+        tr = measure_traffic(args.config, ("k_sweep_flow", "k_face_flux", "k_cell_gather"))
+        if tr:
+            def total(prefixes):
+                ks = [k for k in tr["kernels"] if any(k.startswith(p) for p in prefixes)]
+                return sum(sum(tr["kernels"][k].values()) for k in ks) if ks else None
+            roof_sweep["traffic"] = total(("k_sweep_flow",))
+            roof_face["traffic"] = total(("k_face_flux", "k_cell_gather"))
+            roof["traffic_per_kernel"] = {k: {c: round(v) for c, v in d.items()} for k, d in tr["kernels"].items()}
+            roof["traffic_calibration"] = {c: round(f, 4) for c, f in tr["factors"].items()}
 
     line = {
         "metric": METRIC, "value": value, "unit": "cell-updates/s", "n_gpus": 1,
@@ -498,7 +696,7 @@ def main():
                    "sweep_form": ["dataflow", "levels", "ticket"][params.sweep_form],
                    "tiles": None if tuned is None else {"flux": tuned["flux_tile"], "grad": tuned["grad_tile"]}},
         "roofline": roof,
-        "roofline_sweep": roof_sweep,
+        "roofline_other": other,
         "kernel_ms_per_step": per_class,
     }
     if args.warm_start:
@@ -516,6 +714,14 @@ def main():
     if args.precision_compare is not None and not cfg.get("batch"):
         solver.close()
         line["precision_compare"] = precision_compare(mesh, pover, args.precision_compare)
+    conv = cfg.get("converge")
+    if conv and args.converge is None and not args.no_converge:
+        solver.close()
+        line["time_to_drop"] = wing_time_to_drop(mesh, pover, conv["drop"], conv["max_steps"], WING_WEIGHTS)
+        line["time_to_drop"]["target_note"] = (
+            f"drop {conv['drop']} x R_1 within {conv['max_steps']} steps: on this mesh the plain FP64 run needs "
+            "about 4,400 steps for a 0.25 drop (the residual RMS is dominated by the large far-field cells, whose "
+            "relaxation sets the rate), so a 1e-8 drop does not fit the run")
     if args.converge is not None:
         solver.close()
         if cfg.get("batch"):
@@ -525,12 +731,7 @@ def main():
             line["time_to_drop"] = time_to_drop(mesh, pover, args.converge, args.max_steps)
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        wd = None
-        if params.viscous == 3:
-            if solver._h is None:
-                solver = pa.Solver(mesh, params, device=0)
-            wd = solver.get_field("wdist")[:mesh.n_cell]
-        line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_steps"], wd)
+        line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_steps"], wdist, cfg.get("cpu_warmup", 1))
     solver.close()
     print(json.dumps(line), flush=True)
 

@@ -289,8 +289,10 @@ phl_status phl_viscous_flux(phl_ctx *ctx);
  * inviscid and viscous face loops (one face kernel for both), added to res. */
 phl_status phl_update_residuals(phl_ctx *ctx);
 /* Controller::LoadResiduals + Controller::UpdateResiduals fused
- * (Controller.cpp:1176,1002): res = -0.0, Boundary, the gradient and limiter
- * pass, both face loops, and the residual sums of the result. */
+ * (Controller.cpp:1176,1002): res = -0.0, Boundary, the independent fields
+ * (the laminar viscosity the turbulence hooks read; ggnode node values), the
+ * gradient and limiter pass, both face loops, and the residual sums of the
+ * result. */
 phl_status phl_residual(phl_ctx *ctx);
 /* CallGPUSpectrumRadiusInviscid / Viscous (:5751,5854). */
 phl_status phl_spectrum_radius(phl_ctx *ctx);

@@ -1611,6 +1611,9 @@ static void enqueue_residual(phl_ctx *c)
     reset_step_counter(c);
     phl::launch_boundary(c->m, c->k, c->f, c->stream);
     node_pass(c, c->stream);
+    // UpdateIndependentField's laminar viscosity, which the turbulence hooks
+    // read (the NS kernels evaluate it themselves)
+    if (c->k.viscous == 3) phl::launch_gamma_t(c->m, c->k, c->f, c->stream);
     phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
     phl::launch_flux(c->m, c->k, c->f, c->stream, false);
     phl::launch_norm_final(c->m, c->k, c->f, false, c->stream);

@@ -213,3 +213,166 @@ def compare(n_train=32, n_held=4, dims=(24, 12, 4), drop=1e-8, max_steps=12000,
     out["speedup_note"] = ("total wall time over all held-out cases; a case that missed the target counts at its "
                            "max_steps wall time")
     return out
+
+
+# ---------------------------------------------------------------- swept wing
+# The RANS configuration's wing: root chord 1 on the symmetry plane z = 0,
+# semi-span 1.5, farfield 20 root chords away.
+WING_SEMI_SPAN, WING_FARFIELD = 1.5, 20.0
+
+
+def wing_features(mesh, mach, alpha_deg, wall_distance):
+    """[n_cell, 8] features of a swept-wing cell: position scaled by the
+    semi-span (x, y normal to the wing plane, z spanwise), the wall distance
+    on a log scale (0 at 1e-6, 1 at 10 chords), the distance from the root
+    quarter chord relative to the farfield, Mach, sin/cos of the angle of
+    attack."""
+    met = mesh.metrics()
+    n = mesh.n_cell
+    x, y, z = met["xcc"][:n], met["ycc"][:n], met["zcc"][:n]
+    d = np.asarray(wall_distance[:n], dtype=np.float64)
+    a = np.radians(alpha_deg)
+    X = np.empty((n, 8), np.float32)
+    X[:, 0] = x / WING_SEMI_SPAN
+    X[:, 1] = y / WING_SEMI_SPAN
+    X[:, 2] = z / WING_SEMI_SPAN
+    X[:, 3] = (np.log10(d + 1e-6) + 6.0) / 7.0
+    X[:, 4] = np.sqrt((x - 0.25) ** 2 + y ** 2 + z ** 2) / WING_FARFIELD
+    X[:, 5] = mach
+    X[:, 6] = np.sin(a)
+    X[:, 7] = np.cos(a)
+    return X
+
+
+def sample_wing_cases(n, seed, mach=(0.5, 0.7), alpha=(0.0, 4.0)):
+    rng = np.random.default_rng(seed)
+    return [dict(mach=float(rng.uniform(*mach)), alpha=float(rng.uniform(*alpha))) for _ in range(n)]
+
+
+def wing_params(case, **over):
+    """The RANS wing configuration's numerics at a case's Mach and angle."""
+    p = dict(mach=case["mach"], alpha_deg=case["alpha"], viscous=3, reynolds=3e6, cfl_start=0.01, cfl_end=5.0,
+             cfl_vary_step=500, force_ref_area=1.125, force_ref_length=0.7778, force_ref_span=1.5)
+    p.update(over)
+    return default_params(**p)
+
+
+def wing_training_set(cases, dims, steps, log=None):
+    """Late-time FP64 fields of seeded wing cases on a coarse member of the
+    wing mesh family (one context per case): (mesh, [(X, T)], per-case
+    final residual drops)."""
+    say = log or (lambda *a: None)
+    mesh = Mesh.swept_wing(*dims)
+    data, drops = [], []
+    for k, case in enumerate(cases):
+        s = Solver(mesh, wing_params(case))
+        s.init_freestream()
+        h = s.iterate(1, steps)
+        wd = s.get_field("wdist")
+        q = s.get_state()
+        data.append((wing_features(mesh, case["mach"], case["alpha"], wd),
+                     targets(q, freestream(case["mach"], case["alpha"]), mesh.n_cell)))
+        drops.append(float(h[-1] / h[0]))
+        say(f"wing case {k}: M{case['mach']:.3f} a{case['alpha']:.2f}, {steps} steps, drop {drops[-1]:.3e}")
+        s.close()
+    return mesh, data, drops
+
+
+def train_wing_network(mesh, data, steps, lr, layers=6, seed=0, device=0):
+    """The device network trained on the union graph of the training cases
+    (the same coarse mesh once per case)."""
+    union = Mesh.concat([mesh] * len(data))
+    net = Mpnn(layers=layers, seed=seed, device=device)
+    net.set_graph(mesh=union)
+    net.set_data(np.concatenate([x for x, _ in data]), np.concatenate([t for _, t in data]))
+    hist = np.concatenate([net.train(int(n), float(r)) for n, r in zip(steps, lr)])
+    return net, hist
+
+
+def wing_candidates(net, mesh, case, wall_distance, windows=(None, 0.5)):
+    """Initial states of a wing case from the network: the prediction
+    itself, and the prediction faded to the freestream beyond a wall
+    distance (exp(-d / width)), which keeps the large far-field cells at the
+    freestream they start from."""
+    q_inf = freestream(case["mach"], case["alpha"])
+    net.set_graph(mesh=mesh)
+    net.set_data(wing_features(mesh, case["mach"], case["alpha"], wall_distance))
+    y = net.forward()
+    out = []
+    d = np.asarray(wall_distance[:mesh.n_cell], dtype=np.float64)
+    for w in windows:
+        yy = y if w is None else (y * np.exp(-d / w)[:, None]).astype(np.float32)
+        out.append((w, prediction_to_state(yy, q_inf, mesh.n_total)))
+    return out
+
+
+def judged_start(solver, candidates, r1_freestream):
+    """The solver's residual judges the predicted starts: the candidate with
+    the smallest first density-residual RMS is used if it is below the
+    freestream start's, else the freestream start (the fallback).  Returns
+    (label, its first residual, all candidates' residuals)."""
+    seen = []
+    for w, q in candidates:
+        solver.set_state(q)
+        solver.residual()
+        seen.append((w, float(np.sqrt(solver.residual_norm()[0] / solver.mesh.n_cell))))
+    w, r = min(seen, key=lambda t: t[1])
+    if r < r1_freestream:
+        solver.set_state(dict(candidates)[w])
+        return ("prediction" if w is None else f"prediction faded beyond d={w}"), r, seen
+    solver.init_freestream()
+    return "freestream (fallback)", r1_freestream, seen
+
+
+WING_TRAIN_DIMS = (24, 12, 10, 1e-4, 8)
+
+
+def save_network(net, path, **meta):
+    np.savez(path, params=net.get_params(), layers=np.int32(net.layers), **meta)
+
+
+def load_network(path, device=0):
+    """A network saved by save_network (plain arrays; nothing is unpickled)."""
+    with np.load(path, allow_pickle=False) as z:
+        net = Mpnn(layers=int(z["layers"]), seed=0, device=device)
+        net.set_params(z["params"])
+        meta = {k: z[k] for k in z.files if k not in ("params", "layers")}
+    return net, meta
+
+
+def main(argv=None):
+    """python -m phenlei_amd.warmstart train-wing OUT.npz [n_cases solve_steps adam_steps]:
+    generate the wing training set on the GPU, train the network on it and
+    save its parameters."""
+    import sys
+    import time
+    args = list(sys.argv[1:] if argv is None else argv)
+    if not args or args[0] != "train-wing" or len(args) < 2:
+        raise SystemExit(main.__doc__)
+    out = args[1]
+    n_cases = int(args[2]) if len(args) > 2 else 16
+    solve_steps = int(args[3]) if len(args) > 3 else 3000
+    adam = int(args[4]) if len(args) > 4 else 2400
+
+    def say(m):
+        print("[warmstart] " + m, flush=True)
+
+    t0 = time.perf_counter()
+    cases = sample_wing_cases(n_cases, seed=2026)
+    mesh, data, drops = wing_training_set(cases, WING_TRAIN_DIMS, solve_steps, log=say)
+    t_data = time.perf_counter() - t0
+    say(f"training set: {n_cases} cases of {mesh.n_cell} cells in {t_data:.1f} s")
+    t0 = time.perf_counter()
+    stages = [adam // 2, adam // 4, adam - adam // 2 - adam // 4]
+    net, hist = train_wing_network(mesh, data, stages, [1e-3, 3e-4, 1e-4])
+    t_train = time.perf_counter() - t0
+    say(f"training: loss {hist[0]:.3e} -> {hist[-1]:.3e} in {len(hist)} Adam steps, {t_train:.1f} s")
+    save_network(net, out, train_dims=np.array(WING_TRAIN_DIMS[:3], np.int32), train_h1=WING_TRAIN_DIMS[3],
+                 n_cases=np.int32(n_cases), solve_steps=np.int32(solve_steps), adam_steps=np.int32(len(hist)),
+                 loss=np.array([hist[0], hist[-1]]), data_seconds=t_data, train_seconds=t_train,
+                 cases=np.array([[c["mach"], c["alpha"]] for c in cases]))
+    say(f"saved {out}")
+
+
+if __name__ == "__main__":
+    main()

@@ -23,6 +23,7 @@ from conftest import perturbed_state
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
+CL, CD = pa.FORCE_NAMES.index("CL"), pa.FORCE_NAMES.index("CD")
 
 
 def rel_err(a, b, floor):
@@ -67,14 +68,15 @@ def test_batched_sweep_matches_oracle_per_case(precision):
         qerr = np.max(np.abs(qb[:, cs] - qc) / dev)
         fb = batch.forces_case(k)
         fc = cpu.forces()
+        dcl, dcd = abs(fb["CL"] - fc[CL]), abs(fb["CD"] - fc[CD])
         print(f"case {k} ({m.n_cell} cells, M{p['mach']:.3f}, a{p['alpha_deg']:.2f}) precision {precision}: "
-              f"history {herr:.2e}, state {qerr:.2e}, dCL {abs(fb['CL'] - fc['CL']):.2e}")
+              f"history {herr:.2e}, state {qerr:.2e}, dCL {dcl:.2e}, dCD {dcd:.2e}")
         if precision == 0:
             assert herr <= 1e-10 and qerr <= 1e-10, k
-            assert abs(fb["CL"] - fc["CL"]) <= 1e-12 and abs(fb["CD"] - fc["CD"]) <= 1e-12, k
+            assert dcl <= 1e-12 and dcd <= 1e-12, k
         else:
             assert 0.0 < herr <= 1e-3 and qerr <= 1e-3, k
-            assert abs(fb["CL"] - fc["CL"]) <= 1e-4 and abs(fb["CD"] - fc["CD"]) <= 1e-4, k
+            assert dcl <= 1e-4 and dcd <= 1e-4, k
         cpu.close()
 
 
@@ -120,12 +122,13 @@ def _one_step(name, seed):
     print(f"{name}: dq rel err {err:.2e}, bitwise equal: {np.array_equal(dg, dc)}", flush=True)
     assert err <= TOL
     qg, qc = gpu.get_state()[:, :nc], cpu.get("q")[:, :nc]
-    assert rel_err(qg, qc, 1e-300) <= TOL
+    assert rel_err(qg, qc, 1e-3 * np.max(np.abs(qc), axis=1, keepdims=True)) <= TOL
     if sa:
+        # one turbulence relaxation on the updated state (the oracle's
+        # TurbSolver relaxation: boundary, laminar and eddy viscosity, nu~
+        # gradient, residual, spectrum, scalar LU-SGS)
         gpu.turb_relax()
-        for hook in ("sa_boundary", "sa_viscosity", "sa_gradient", "sa_update_residuals", "sa_spectrum",
-                     "sa_lusgs"):
-            getattr(cpu, hook)()
+        cpu.sa_relax()
         tg, tc = gpu.get_field("nut")[:nc], cpu.get("nut")[:nc]
         err = rel_err(tg, tc, 1e-300)
         print(f"{name}: nu~ after the turbulence relaxation rel err {err:.2e}", flush=True)
