@@ -306,6 +306,21 @@ phl_status phl_lusgs(phl_ctx *ctx, int *n_fixed);
  * residual currently in res: sum5[m] = sum over real cells of res_m^2 (case 0
  * of a batched context).  Deterministic (fixed blocks, fixed order). */
 phl_status phl_residual_norm(phl_ctx *ctx, double *sum5);
+/* No reference counterpart.  For callers that cannot allocate device memory
+ * (code under hipGraph capture, tight serving loops, harnesses that forbid
+ * hipMalloc): the sums of phl_residual_norm over a caller-owned device array,
+ * in the same fixed order (256-cell blocks, then the block sums), with no
+ * context and no allocation.  The block partials live in a static device
+ * array of 2.5 MiB that each GPU holds until the process ends.
+ *   res    device pointer, rows [5][ld]; the first n_cell entries of each
+ *          row are summed (the residual's real cells)
+ *   sum5   device pointer to 5 doubles: sum over the cells of res_m^2
+ *   stream a hipStream_t (NULL: the null stream); the work is only enqueued
+ * n_cell above PHL_NOALLOC_MAX_CELLS (16,777,216) returns PHL_ERR_ARG.  All
+ * calls on one GPU share the array: calls that overlap on one GPU (other
+ * streams, other threads) race. */
+#define PHL_NOALLOC_MAX_CELLS 16777216
+phl_status phl_residual_norm_noalloc(const double *res, int n_cell, int ld, double *sum5, void *stream);
 
 /* Wall force and moment coefficients: NSSolverUnstruct::AirForceCoefParts
  * over all solid-surface faces (:16026-16250) and Force::CompForce

@@ -21,6 +21,7 @@ PHL_ERR_HIP = 2
 PHL_ERR_NUMERIC = 3
 PHL_ERR_UNSUPPORTED = 4
 PHL_ERR_ALLOC = 5
+PHL_NOALLOC_MAX_CELLS = 16777216  # phl.h: cells of phl_residual_norm_noalloc's static scratch array
 
 # Field identifiers (include/phl.h), shared with the oracle.
 FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
@@ -151,6 +152,8 @@ def _declare(lib):
         "phl_time_step": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_lusgs": (ctypes.c_int, [_P, _I]),
         "phl_residual_norm": (ctypes.c_int, [_P, _D]),
+        "phl_residual_norm_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
+                                                     ctypes.c_void_p]),
         "phl_forces": (ctypes.c_int, [_P, _D]),
         "phl_forces_case": (ctypes.c_int, [_P, ctypes.c_int, _D]),
         "phl_mesh_concat": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
@@ -370,6 +373,14 @@ def read_restart(path, n_eq, n_total):
     step = ctypes.c_int()
     _check(load_library().phl_read_restart(os.fsencode(path), n_eq, n_total, ctypes.byref(step), _dptr(q)))
     return step.value, q
+
+
+def residual_norm_noalloc(res_ptr, n_cell, ld, sum5_ptr, stream=None):
+    """phl_residual_norm_noalloc on caller-owned device memory: res_ptr and
+    sum5_ptr are device addresses (ints), stream a hipStream_t or None.  The
+    work is only enqueued on the stream."""
+    _check(load_library().phl_residual_norm_noalloc(ctypes.c_void_p(res_ptr), n_cell, ld, ctypes.c_void_p(sum5_ptr),
+                                                    ctypes.c_void_p(stream)))
 
 
 def device_info(device=0):

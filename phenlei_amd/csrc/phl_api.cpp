@@ -1744,6 +1744,21 @@ phl_status phl_residual_norm(phl_ctx *c, double *sum5)
     return PHL_OK;
 }
 
+phl_status phl_residual_norm_noalloc(const double *res, int n_cell, int ld, double *sum5, void *stream)
+{
+    static_assert(phl::kNoAllocMaxCells == PHL_NOALLOC_MAX_CELLS, "scratch size and header disagree");
+    if (!res || !sum5) return fail(PHL_ERR_ARG, "phl_residual_norm_noalloc: null argument");
+    if (n_cell < 1 || ld < n_cell) return fail(PHL_ERR_ARG, "phl_residual_norm_noalloc: need 1 <= n_cell <= ld");
+    if (n_cell > PHL_NOALLOC_MAX_CELLS)
+        return fail(PHL_ERR_ARG, "phl_residual_norm_noalloc: " + std::to_string(n_cell) +
+                                     " cells exceed the static scratch array's 16,777,216");
+    double *part = nullptr;
+    HIPTRY(phl::noalloc_scratch(&part));
+    phl::launch_residual_norm_noalloc(res, n_cell, (size_t)ld, sum5, part, static_cast<hipStream_t>(stream));
+    HIPTRY(hipGetLastError());
+    return PHL_OK;
+}
+
 // Force::CompForce (API/Physics/src/Force.cpp:439-533) for the all-wall
 // part: scale the sums by the reference area and lengths, then lift, drag,
 // side force, pressure and friction drag, induced-drag estimate and centre

@@ -73,6 +73,11 @@ constexpr int kTileCells = 128;
 // Flagged cells of one update that SolutionFix sorts in LDS (more fall back
 // to a walk over the mesh order).
 constexpr int kFixList = 4096;
+// Residual sums without allocation: 256-cell block partials of at most
+// kNoAllocParts blocks fit the static scratch array (2.5 MiB per GPU), so
+// arrays of up to 16,777,216 cells.
+constexpr int kNoAllocParts = 65536;
+constexpr int kNoAllocMaxCells = kNoAllocParts * 256;
 
 // Logical block of hardware block b of nblk: workgroups are dispatched to
 // the 8 XCDs round-robin (b % 8), so this gives XCD x the contiguous logical
@@ -243,6 +248,9 @@ void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 void launch_load_residuals(const DevMesh &m, DevFlow &f, hipStream_t s);
 void launch_face_pass(const DevMesh &m, const Consts &k, DevFlow &f, int parts, hipStream_t s);
 void launch_residual_norm(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
+// Device address of the no-allocation scratch array on the current device.
+hipError_t noalloc_scratch(double **out);
+void launch_residual_norm_noalloc(const double *res, int nC, size_t ld, double *sum5, double *part, hipStream_t s);
 // Workgroups of the fused flux launch (one partial-sum row each).
 int flux_blocks(const DevMesh &m, const Consts &k);
 void launch_norm_final(const DevMesh &m, const Consts &k, DevFlow &f, bool withDt, hipStream_t s);

@@ -22,6 +22,7 @@
 #include "phl_internal.h"
 
 #include <algorithm>
+#include <mutex>
 
 namespace phl {
 
@@ -1486,6 +1487,15 @@ __global__ void __launch_bounds__(kBlock) k_norm_final(const double *__restrict_
     if (threadIdx.x == 5 && withDt) *dtMin = red[5][0];
 }
 
+// The five sums alone, to a caller's array (the no-allocation entry point).
+__global__ void __launch_bounds__(kBlock) k_norm_sum5(const double *__restrict__ part, int nPart,
+                                                      double *__restrict__ sum5)
+{
+    __shared__ double red[6][kBlock];
+    sum_parts(part, part, nPart, 0, red);
+    if (threadIdx.x < 5) sum5[threadIdx.x] = red[threadIdx.x][0];
+}
+
 // Batched contexts: per-case residual sums of the step and minimum local time
 // step in a fixed order.  Workgroup (case cs, slice j) reduces the j-th of
 // kCaseSlices contiguous parts of the case's cells into casePart; then one
@@ -2635,6 +2645,51 @@ void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *
 {
     hipLaunchKernelGGL(k_forces, dim3(1), dim3(kBlock), 0, s, m, k, f.q, f.grad, f.visl, torqueRef[0], torqueRef[1],
                        torqueRef[2], f.forceOut);
+}
+
+}  // namespace phl
+
+// ---------------------------------------------------------------------------
+// Residual sums without allocation: the block partials of the reduction live
+// in this fixed device array instead of a context's buffer.  Every GPU the
+// library runs on holds its own copy until the process ends; calls on one GPU
+// share it, so calls that overlap on one GPU race.  Each call writes every
+// partial it reads, so nothing has to be cleared between calls.
+static __device__ __align__(256) double scratch[(size_t)phl::kNoAllocParts * 5];
+
+namespace phl {
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+double *g_scratchPtr[kMaxDevices] = {};
+}  // namespace
+
+// The device address of `scratch` on the current device, looked up once per
+// device with that device current.  The array's name is only a host-side
+// stand-in: kernels get this address.
+hipError_t noalloc_scratch(double **out)
+{
+    int dev = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e != hipSuccess) return e;
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratchMu);
+    if (!g_scratchPtr[dev]) {
+        void *ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        g_scratchPtr[dev] = static_cast<double *>(ptr);
+    }
+    *out = g_scratchPtr[dev];
+    return hipSuccess;
+}
+
+void launch_residual_norm_noalloc(const double *res, int nC, size_t ld, double *sum5, double *part, hipStream_t s)
+{
+    const int nb = nblk(nC, 256);
+    hipLaunchKernelGGL(k_res_partial, dim3(nb), dim3(256), 0, s, nC, ld, res, part);
+    hipLaunchKernelGGL(k_norm_sum5, dim3(1), dim3(kBlock), 0, s, part, nb, sum5);
 }
 
 }  // namespace phl
