@@ -38,16 +38,17 @@ CONFIGS = {
     "c1": dict(workload="bump channel, 64x32x8 hex (16,384 cells), inviscid Euler, M0.5, AoA 0, "
                         "Roe + GG-cell + Venkat(K=5), LU-SGS CFL 10, FP64",
                mesh=("bump_hex", 64, 32, 8, 0.05),
-               params=dict(mach=0.5, cfl_start=10.0, cfl_end=10.0), cpu_steps=500),
+               params=dict(mach=0.5, cfl_start=10.0, cfl_end=10.0), cpu_sample=(5, 100, 5, 100)),
     "c2": dict(workload="bump channel, hybrid 1,007,616 cells (15 prism layers, first height 1e-4, over a "
                         "128x16 wall; tet fill of a seeded jittered lattice), laminar NS, M0.3, Re 5e5/unit "
                         "length, Sutherland, Roe + GG-cell + Venkat(K=5), LU-SGS CFL 10, FP64",
                mesh=("bump_hybrid", 128, 16, 15, 1e-4, 77, 0.1, 1, 0.05),
-               params=dict(mach=0.3, viscous=1, reynolds=5e5, cfl_start=10.0, cfl_end=10.0), cpu_steps=6),
+               params=dict(mach=0.3, viscous=1, reynolds=5e5, cfl_start=10.0, cfl_end=10.0),
+               cpu_sample=(5, 20, 1, 5)),
     "hex1m": dict(workload="bump channel, 256x64x64 hex (1,048,576 cells), inviscid Euler, M0.5, "
                            "Roe + GG-cell + Venkat(K=5), LU-SGS CFL 10, FP64",
                   mesh=("bump_hex", 256, 64, 64, 0.05),
-                  params=dict(mach=0.5, cfl_start=10.0, cfl_end=10.0), cpu_steps=8),
+                  params=dict(mach=0.5, cfl_start=10.0, cfl_end=10.0), cpu_sample=(2, 10, 1, 4)),
 }
 CONFIGS["c3"] = dict(workload="swept wing (root chord 1, taper 0.5, AR 4, quarter-chord sweep 25 deg, symmetric "
                               "cubic 12% section), hybrid 3,983,616 cells (30 prism layers each side of the wing plane, "
@@ -59,7 +60,7 @@ CONFIGS["c3"] = dict(workload="swept wing (root chord 1, taper 0.5, AR 4, quarte
                      params=dict(mach=0.6, alpha_deg=2.0, viscous=3, reynolds=3e6, cfl_start=0.01, cfl_end=5.0,
                                  cfl_vary_step=500, force_ref_area=1.125, force_ref_length=0.7778,
                                  force_ref_span=1.5),
-                     cpu_steps=1,
+                     cpu_sample=(2, 8, 1, 2),
                      # the headline leg: time to a residual drop, plain FP64 from freestream against the mixed
                      # form started from the learned warm start (judged by the solver's residual)
                      converge=dict(drop=0.25, max_steps=5000))
@@ -90,7 +91,7 @@ def c4_cases(n_cases=C4_CASES, seed=C4_SEED):
 CONFIGS["c4"] = dict(workload="batched sweep: 64 inviscid bump-channel hex cases in one context, cells ~ U[50k,400k] "
                               "(seeded), Mach ~ U[0.3,0.8], AoA ~ U[-2,6] deg, bump ~ U[0.02,0.08]; Roe + GG-cell + "
                               "Venkat(K=5), LU-SGS CFL 10; FP32 face fluxes + FP64 accumulation; autotuned tiles; hipGraph",
-                     batch=True, params=dict(cfl_start=10.0, cfl_end=10.0, precision=1), cpu_steps=20)
+                     batch=True, params=dict(cfl_start=10.0, cfl_end=10.0, precision=1), cpu_sample=(2, 20, 1, 10))
 
 
 def build_batch(cfg):
@@ -156,11 +157,8 @@ def host_cpu():
     return os.cpu_count(), model
 
 
-def cpu_baseline(mesh, params, steps, wall_distance=None, warmup=1):
-    sys.path.insert(0, os.path.join(ROOT, "oracle"))
-    import oracle as orc  # the checker, timed here as the CPU baseline only
-    conn, met = mesh.connectivity(), mesh.metrics()
-    cpu = orc.OracleSolver(conn, met, params)
+def _oracle_rate(orc, conn, met, params, n_cell, steps, warmup, wall_distance, omp):
+    cpu = orc.OracleSolver(conn, met, params, omp=omp)
     if wall_distance is not None:
         cpu.set_wall_distance(wall_distance)  # preprocessing, outside the timed iterations
     cpu.init_freestream()
@@ -170,11 +168,32 @@ def cpu_baseline(mesh, params, steps, wall_distance=None, warmup=1):
     cpu.iterate(1 + warmup, steps)
     dt = time.perf_counter() - t0
     cpu.close()
+    return n_cell * steps / dt, dt
+
+
+def cpu_baseline(mesh, params, sample, wall_distance=None):
+    """The CPU oracle on this host: the OpenMP build over the host's cores
+    (face loops split by cell owner, LU-SGS by dependency level; the same
+    bits as the sequential build) and the sequential build on one core, each
+    on a bounded sample of iterations of the same mesh from freestream.
+    sample = (omp warm-up, omp timed, one-core warm-up, one-core timed)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as orc  # the checker, timed here as the CPU baseline only
+    conn, met = mesh.connectivity(), mesh.metrics()
     nproc, model = host_cpu()
-    return dict(value=mesh.n_cell * steps / dt, unit="cell-updates/s", cores=1, kind="port",
-                nproc=nproc, cpu_model=model,
-                sample=f"{steps} FP64 iteration(s) after {warmup} warm-up iteration(s) of the same mesh from "
-                       f"freestream ({mesh.n_cell} cells, {dt:.1f} s timed, sequential C oracle on one core)")
+    threads = int(os.environ.get("OMP_NUM_THREADS", nproc or 1))
+    w_omp, n_omp, w_seq, n_seq = sample
+    print(f"[bench] CPU oracle, OpenMP on {threads} threads: {w_omp} + {n_omp} iterations", file=sys.stderr, flush=True)
+    v_omp, t_omp = _oracle_rate(orc, conn, met, params, mesh.n_cell, n_omp, w_omp, wall_distance, True)
+    print(f"[bench] CPU oracle, one core: {w_seq} + {n_seq} iterations", file=sys.stderr, flush=True)
+    v_seq, t_seq = _oracle_rate(orc, conn, met, params, mesh.n_cell, n_seq, w_seq, wall_distance, False)
+    return dict(value=v_omp, unit="cell-updates/s", cores=threads, kind="port", nproc=nproc, cpu_model=model,
+                sample=f"{n_omp} FP64 iterations after {w_omp} warm-up of the same mesh from freestream "
+                       f"({mesh.n_cell} cells, {t_omp:.1f} s timed), the C oracle built with OpenMP on {threads} "
+                       f"threads (same results as the sequential oracle)",
+                single_core=dict(value=v_seq, cores=1,
+                                 sample=f"{n_seq} iteration(s) after {w_seq} warm-up, {t_seq:.1f} s timed, "
+                                        f"sequential C oracle"))
 
 
 def time_to_drop(mesh, pover, drop, max_steps, chunk=256, stall=2048):
@@ -731,7 +750,7 @@ def main():
             line["time_to_drop"] = time_to_drop(mesh, pover, args.converge, args.max_steps)
     if not args.no_cpu:
         print("[bench] CPU baseline (oracle)", file=sys.stderr, flush=True)
-        line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_steps"], wdist, cfg.get("cpu_warmup", 1))
+        line["cpu_baseline"] = cpu_baseline(parts[0], params, cfg["cpu_sample"], wdist)
     solver.close()
     print(json.dumps(line), flush=True)
 

@@ -10,6 +10,10 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "liborc.so")
+# the same restatement built with OpenMP: face loops split by cell owner, LU-SGS
+# by dependency level; the same bits as liborc.so (the CPU baseline over all
+# host cores)
+LIB_PATH_OMP = os.path.join(HERE, "liborc_omp.so")
 
 FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
              diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16,
@@ -21,7 +25,7 @@ FIELD_ROWS = dict(q=5, res=5, dqdx=5, dqdy=5, dqdz=5, limit=1, dq=5, spec_inv=1,
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
 _P = ctypes.c_void_p
-_lib = None
+_libs = {}
 
 
 def _dp(a):
@@ -32,12 +36,12 @@ def _ip(a):
     return a.ctypes.data_as(_I)
 
 
-def load():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(LIB_PATH):
-            raise ImportError(f"{LIB_PATH} not built (run phenlei_amd/build.py)")
-        lib = ctypes.CDLL(LIB_PATH)
+def load(omp=False):
+    path = LIB_PATH_OMP if omp else LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise ImportError(f"{path} not built (run phenlei_amd/build.py)")
+        lib = ctypes.CDLL(path)
         lib.orc_face_metrics.argtypes = [ctypes.c_int, _I, _I, _D, _D, _D] + [_D] * 7
         lib.orc_cell_metrics.argtypes = [ctypes.c_int] * 3 + [_I, _I, _I, _I, _I, _I, _D, _D, _D] + [_D] * 7 + [_D] * 4
         lib.orc_skewness.argtypes = [ctypes.c_int] * 3 + [_I, _I] + [_D] * 10 + [_D]
@@ -82,8 +86,8 @@ def load():
         lib.orc_get_node_values.argtypes = [_P, _D]
         lib.orc_ptotal.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.orc_ptotal.restype = ctypes.c_double
-        _lib = lib
-    return _lib
+        _libs[path] = lib
+    return _libs[path]
 
 
 def face_metrics(conn):
@@ -143,8 +147,9 @@ def lusgs_rad(ql, qr, n, area, gamma=1.4):
 class OracleSolver:
     """Sequential FP64 restatement of NSSolverUnstruct on one zone."""
 
-    def __init__(self, conn, metrics, params):
-        lib = load()
+    def __init__(self, conn, metrics, params, omp=False):
+        """omp: the OpenMP build (same results, all host cores)."""
+        self._L = lib = load(omp)
         self.n_cell = len(conn["cell_node_ptr"]) - 1
         self.n_bface = conn["n_bface"]
         self.n_total = self.n_cell + self.n_bface
@@ -168,73 +173,75 @@ class OracleSolver:
 
     def compute_node_value(self):
         """ComputeNodeValue + ModifyNodeValue on the current state."""
-        load().orc_node_values(self._h)
+        self._L.orc_node_values(self._h)
 
     def node_values(self):
         """Node values [6][nNode] (five primitives, then T) of the last node pass."""
         out = np.empty((6, self.n_node))
-        load().orc_get_node_values(self._h, _dp(out))
+        self._L.orc_get_node_values(self._h, _dp(out))
         return out
 
     def get(self, name):
         rows = FIELD_ROWS[name]
         out = np.empty((rows, self.n_total))
-        load().orc_get(self._h, FIELD[name], _dp(out))
+        self._L.orc_get(self._h, FIELD[name], _dp(out))
         return out if rows > 1 else out[0]
 
     def set(self, name, value):
         v = np.ascontiguousarray(value, dtype=np.float64).reshape(-1)
-        load().orc_set(self._h, FIELD[name], _dp(v))
+        self._L.orc_set(self._h, FIELD[name], _dp(v))
 
     def freestream(self):
         out = np.empty(5)
-        load().orc_freestream(self._h, _dp(out))
+        self._L.orc_freestream(self._h, _dp(out))
         return out
 
     def __getattr__(self, name):
-        fn = getattr(load(), "orc_" + name, None)
+        if name.startswith("_"):
+            raise AttributeError(name)
+        fn = getattr(self._L, "orc_" + name, None)
         if fn is None:
             raise AttributeError(name)
         return lambda *a: fn(self._h, *a)
 
     def set_wall_distance(self, wd):
         wd = np.ascontiguousarray(wd, dtype=np.float64)
-        load().orc_set_wall_distance(self._h, _dp(wd))
+        self._L.orc_set_wall_distance(self._h, _dp(wd))
 
     def sa_face_matrices(self, n_face):
         a, b = np.empty(n_face), np.empty(n_face)
-        load().orc_sa_face_matrices(self._h, _dp(a), _dp(b))
+        self._L.orc_sa_face_matrices(self._h, _dp(a), _dp(b))
         return a, b
 
     def face_states(self, f):
         """(qL, qR) of face f: MUSCL with the positivity fallback and BoundaryQlQrFix."""
         a, b = np.empty(5), np.empty(5)
-        load().orc_face_states(self._h, int(f), _dp(a), _dp(b))
+        self._L.orc_face_states(self._h, int(f), _dp(a), _dp(b))
         return a, b
 
     def lusgs(self):
         sweeps = np.empty(8)
-        return load().orc_lusgs(self._h, _dp(sweeps))
+        return self._L.orc_lusgs(self._h, _dp(sweeps))
 
     def residual_norm(self):
         out = np.empty(5)
-        load().orc_residual_norm(self._h, _dp(out))
+        self._L.orc_residual_norm(self._h, _dp(out))
         return out
 
     def forces(self):
         """AirForceCoefParts + CompForce at the current state (22 values)."""
         out = np.empty(22)
-        load().orc_forces(self._h, _dp(out))
+        self._L.orc_forces(self._h, _dp(out))
         return out
 
     def iterate(self, first_step, n_steps):
         hist = np.empty(max(n_steps, 1))
-        nfix = load().orc_iterate(self._h, first_step, n_steps, _dp(hist))
+        nfix = self._L.orc_iterate(self._h, first_step, n_steps, _dp(hist))
         return hist[:n_steps], nfix
 
     def close(self):
         if getattr(self, "_h", None):
-            load().orc_destroy(self._h)
+            self._L.orc_destroy(self._h)
             self._h = None
 
     def __del__(self):

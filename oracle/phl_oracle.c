@@ -7,7 +7,9 @@
  * CPU-baseline leg of bench.py, never by the product library.
  *
  * Build with -O2 -ffp-contract=off so that every multiply and add rounds as
- * written, like the reference's own x86-64 build.
+ * written, like the reference's own x86-64 build.  Built with -fopenmp as
+ * well (liborc_omp.so), the same loops run on all host cores with results
+ * identical to the sequential build (see the OpenMP notes below).
  *
  * Index conventions follow the reference: faces 0..nBFace-1 are boundary
  * faces, the ghost cell of boundary face f is nCell+f
@@ -31,6 +33,49 @@ static const double SMALL = 1.0e-40; /* API/Common/include/Precision.h:25 */
 static const double TINY  = 1.0e-40; /* API/Common/include/Precision.h:26 */
 static const double LARGE = 1.0e40;  /* API/Common/include/Precision.h:24 */
 static const double PI_   = 3.141592653589793; /* Constants.h:110 */
+
+/* ------------------------------------------------------------------ */
+/* OpenMP build (liborc_omp.so, the CPU baseline over the host's cores): */
+/* thread t owns a contiguous range of cells (a ghost belongs to its     */
+/* face's owner).  A face loop runs, on every thread, the faces that     */
+/* touch one of its cells in increasing order and applies a face's       */
+/* contributions only to the cells the thread owns, so each cell still   */
+/* receives its faces in increasing face order, exactly as the           */
+/* sequential loop adds them: the results are bit for bit those of the   */
+/* sequential build.  A face between two threads' cells is evaluated by  */
+/* both.  The LU-SGS sweeps run level by level (the cells of a level are */
+/* independent); SolutionFix stays sequential.  Without OpenMP the       */
+/* macros below reduce to the plain sequential loops.                    */
+/* ------------------------------------------------------------------ */
+struct orc_part;
+#ifdef _OPENMP
+#include <omp.h>
+struct orc_part {
+    int nThreads;
+    int *owner;   /* [nTotal] thread of each cell and ghost */
+    int *fptr;    /* [nThreads+1] */
+    int *flist;   /* faces of each thread, increasing */
+    int *lvPtrF, *lvCellsF, nLvF;  /* forward sweep levels (cells in increasing order within a level) */
+    int *lvPtrB, *lvCellsB, nLvB;  /* backward sweep levels (cells in decreasing order) */
+};
+static struct orc_part *orc_partition(orc_solver *s);
+#define FACE_LOOP(s, f0, f1, f)                                                                  \
+    {                                                                                            \
+        const struct orc_part *P_ = orc_partition(s);                                           \
+        _Pragma("omp parallel num_threads(P_->nThreads)")                                        \
+        for (int T_ = omp_get_thread_num(); T_ < P_->nThreads; T_ += omp_get_num_threads())      \
+            for (int i_ = P_->fptr[T_]; i_ < P_->fptr[T_ + 1]; ++i_) {                             \
+                const int f = P_->flist[i_];                                                      \
+                if (f < (f0) || f >= (f1)) continue;
+#define FACE_LOOP_END }}
+#define OWN(c) (P_->owner[c] == T_)
+#define CELL_LOOP _Pragma("omp parallel for schedule(static)")
+#else
+#define FACE_LOOP(s, f0, f1, f) { for (int f = (f0); f < (f1); ++f) {
+#define FACE_LOOP_END }}
+#define OWN(c) 1
+#define CELL_LOOP
+#endif
 
 static inline double MIN(double a, double b) { return (a < b) ? a : b; } /* Math_BasisFunction.h:31 */
 static inline double MAX(double a, double b) { return (a > b) ? a : b; } /* Math_BasisFunction.h:44 */
@@ -63,6 +108,8 @@ struct orc_solver {
     /* Spalart-Allmaras (viscous == 3): nu~ and its residual, spectrum,
      * face matrices, increments, gradient and the wall distance */
     double *nut, *resT, *specT, *matL, *matR, *dqT, *dfluxT, *gnx, *gny, *gnz, *wdist;
+    /* OpenMP build: thread partition and sweep levels (NULL otherwise) */
+    struct orc_part *part;
     /* derived constants */
     double primInf[5];
     double R;           /* coefficientOfStateEquation, PerfectGas.cpp:1401 */
@@ -437,9 +484,84 @@ orc_solver *orc_create(const orc_params *p, int nCell, int nBFace, int nFace,
     return s;
 }
 
+#ifdef _OPENMP
+/* Levels of a sweep: a cell's level is one more than the highest level of
+ * its real neighbours swept before it (lower indices forward, higher
+ * backward).  Cells of a level depend only on earlier levels. */
+static void sweep_levels(const orc_solver *s, int forward, int **ptrOut, int **cellsOut, int *nOut)
+{
+    const int nC = s->nCell;
+    int *lv = (int *)calloc((size_t)nC, sizeof(int));
+    int nl = 0;
+    for (int i = 0; i < nC; ++i) {
+        const int c = forward ? i : nC - 1 - i;
+        int l = 0;
+        for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
+            const int f = s->c2f[k];
+            if (f < s->nBFace) continue;
+            const int nb = s->left[f] == c ? s->right[f] : s->left[f];
+            if ((forward && nb < c) || (!forward && nb > c)) l = lv[nb] + 1 > l ? lv[nb] + 1 : l;
+        }
+        lv[c] = l;
+        if (l + 1 > nl) nl = l + 1;
+    }
+    int *ptr = (int *)calloc((size_t)nl + 1, sizeof(int)), *cells = (int *)malloc(sizeof(int) * (size_t)nC);
+    for (int c = 0; c < nC; ++c) ptr[lv[c] + 1]++;
+    for (int l = 0; l < nl; ++l) ptr[l + 1] += ptr[l];
+    int *fill = (int *)calloc((size_t)nl, sizeof(int));
+    for (int i = 0; i < nC; ++i) {
+        const int c = forward ? i : nC - 1 - i;
+        cells[ptr[lv[c]] + fill[lv[c]]++] = c;
+    }
+    free(fill);
+    free(lv);
+    *ptrOut = ptr; *cellsOut = cells; *nOut = nl;
+}
+
+static struct orc_part *orc_partition(orc_solver *s)
+{
+    if (s->part) return s->part;
+    struct orc_part *P = (struct orc_part *)calloc(1, sizeof(struct orc_part));
+    omp_set_dynamic(0);
+    int T = omp_get_max_threads();
+    if (T < 1) T = 1;
+    const int nC = s->nCell, nT = s->nTotal;
+    P->nThreads = T;
+    P->owner = (int *)malloc(sizeof(int) * (size_t)nT);
+    for (int c = 0; c < nC; ++c) P->owner[c] = (int)((long long)c * T / nC);
+    for (int f = 0; f < s->nBFace; ++f) P->owner[nC + f] = P->owner[s->left[f]];
+    P->fptr = (int *)calloc((size_t)T + 1, sizeof(int));
+    for (int f = 0; f < s->nFace; ++f) {
+        const int a = P->owner[s->left[f]], b = P->owner[s->right[f]];
+        P->fptr[a + 1]++;
+        if (b != a) P->fptr[b + 1]++;
+    }
+    for (int t = 0; t < T; ++t) P->fptr[t + 1] += P->fptr[t];
+    P->flist = (int *)malloc(sizeof(int) * (size_t)(P->fptr[T] > 0 ? P->fptr[T] : 1));
+    int *fill = (int *)calloc((size_t)T, sizeof(int));
+    for (int f = 0; f < s->nFace; ++f) {
+        const int a = P->owner[s->left[f]], b = P->owner[s->right[f]];
+        P->flist[P->fptr[a] + fill[a]++] = f;
+        if (b != a) P->flist[P->fptr[b] + fill[b]++] = f;
+    }
+    free(fill);
+    sweep_levels(s, 1, &P->lvPtrF, &P->lvCellsF, &P->nLvF);
+    sweep_levels(s, 0, &P->lvPtrB, &P->lvCellsB, &P->nLvB);
+    s->part = P;
+    return P;
+}
+#endif
+
 void orc_destroy(orc_solver *s)
 {
     if (!s) return;
+#ifdef _OPENMP
+    if (s->part) {
+        free(s->part->owner); free(s->part->fptr); free(s->part->flist);
+        free(s->part->lvPtrF); free(s->part->lvCellsF); free(s->part->lvPtrB); free(s->part->lvCellsB);
+        free(s->part);
+    }
+#endif
     free(s->left); free(s->right); free(s->bcType);
     free(s->xfc); free(s->yfc); free(s->zfc); free(s->xfn); free(s->yfn); free(s->zfn);
     free(s->area); free(s->xcc); free(s->ycc); free(s->zcc); free(s->vol); free(s->skew);
@@ -620,6 +742,7 @@ static void bc_outflow(orc_solver *s, int f)
 
 void orc_boundary(orc_solver *s)
 {
+    CELL_LOOP
     for (int f = 0; f < s->nBFace; ++f) {
         int bc = s->bcType[f];
         if (bc == ORC_BC_EXTRAPOLATION || bc == ORC_BC_OUTFLOW) bc_outflow(s, f);
@@ -634,6 +757,7 @@ void orc_boundary(orc_solver *s)
 /* ComputeGamaAndTemperature, NSSolverUnstruct.cpp:4666-4732 (perfect gas). */
 void orc_gamma_t(orc_solver *s)
 {
+    CELL_LOOP
     for (int c = 0; c < s->nTotal; ++c) {
         s->gama[c] = s->p.gamma;
         s->t[c] = s->q[IP][c] / (s->R * s->q[IR][c]);
@@ -644,6 +768,7 @@ void orc_gamma_t(orc_solver *s)
 void orc_viscosity(orc_solver *s)
 {
     double ts = s->tsuth;
+    CELL_LOOP
     for (int c = 0; c < s->nTotal; ++c) {
         double tt = s->t[c];
         s->visl[c] = tt * sqrt(tt) * (1.0 + ts) / (tt + ts);
@@ -654,23 +779,25 @@ void orc_viscosity(orc_solver *s)
 /* Gradients: UnstructGrid::CompGradientGGCell                         */
 /* API/Geometry/src/Geo_UnstructGrid.cpp:3220-3284 (ghost gradient 0)  */
 /* ------------------------------------------------------------------ */
-static void grad_ggcell(const orc_solver *s, const double *q, double *gx, double *gy, double *gz)
+static void grad_ggcell(orc_solver *s, const double *q, double *gx, double *gy, double *gz)
 {
     int nT = s->nTotal;
+    CELL_LOOP
     for (int c = 0; c < nT; ++c) { gx[c] = 0.0; gy[c] = 0.0; gz[c] = 0.0; }
-    for (int f = 0; f < s->nBFace; ++f) {
+    FACE_LOOP(s, 0, s->nBFace, f)
         int le = s->left[f], re = f + s->nCell;
         double qfc = 0.5 * (q[le] + q[re]);
         double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
-        gx[le] += qfc * nx; gy[le] += qfc * ny; gz[le] += qfc * nz;
-    }
-    for (int f = s->nBFace; f < s->nFace; ++f) {
+        if (OWN(le)) { gx[le] += qfc * nx; gy[le] += qfc * ny; gz[le] += qfc * nz; }
+    FACE_LOOP_END
+    FACE_LOOP(s, s->nBFace, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         double qfc = 0.5 * (q[le] + q[re]);
         double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
-        gx[le] += qfc * nx; gy[le] += qfc * ny; gz[le] += qfc * nz;
-        gx[re] -= qfc * nx; gy[re] -= qfc * ny; gz[re] -= qfc * nz;
-    }
+        if (OWN(le)) { gx[le] += qfc * nx; gy[le] += qfc * ny; gz[le] += qfc * nz; }
+        if (OWN(re)) { gx[re] -= qfc * nx; gy[re] -= qfc * ny; gz[re] -= qfc * nz; }
+    FACE_LOOP_END
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) {
         double ovol = 1.0 / s->vol[c];
         gx[c] *= ovol; gy[c] *= ovol; gz[c] *= ovol;
@@ -680,19 +807,21 @@ static void grad_ggcell(const orc_solver *s, const double *q, double *gx, double
 /* UnstructGrid::CompGradientLSQ, Geo_UnstructGrid.cpp:3995-4085: the
  * weighted differences right - left summed into both cells in face order,
  * times the cell's inverse normal matrix; a ghost takes its owner's gradient. */
-static void grad_lsq(const orc_solver *s, const double *q, double *gx, double *gy, double *gz)
+static void grad_lsq(orc_solver *s, const double *q, double *gx, double *gy, double *gz)
 {
     int nT = s->nTotal, nC = s->nCell;
+    CELL_LOOP
     for (int c = 0; c < nT; ++c) { gx[c] = 0.0; gy[c] = 0.0; gz[c] = 0.0; }
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = f < s->nBFace ? f + nC : s->right[f];
         double dx = s->xcc[re] - s->xcc[le], dy = s->ycc[re] - s->ycc[le], dz = s->zcc[re] - s->zcc[le];
         double dq = s->iwt[f] * (q[re] - q[le]) * 1.0;
         double tmpx = dq * dx, tmpy = dq * dy, tmpz = dq * dz;
-        gx[le] += tmpx; gy[le] += tmpy; gz[le] += tmpz;
+        if (OWN(le)) { gx[le] += tmpx; gy[le] += tmpy; gz[le] += tmpz; }
         if (f < s->nBFace) continue;
-        gx[re] += tmpx; gy[re] += tmpy; gz[re] += tmpz;
-    }
+        if (OWN(re)) { gx[re] += tmpx; gy[re] += tmpy; gz[re] += tmpz; }
+    FACE_LOOP_END
+    CELL_LOOP
     for (int c = 0; c < nC; ++c) {
         double tx = gx[c], ty = gy[c], tz = gz[c];
         gx[c] = s->ixx[c] * tx + s->ixy[c] * ty + s->ixz[c] * tz;
@@ -841,24 +970,25 @@ static double face_node_mean(const orc_solver *s, int f, const double *qnode)
  * face values are node means (interior and symmetry faces) or the average
  * of the cell and its ghost (other boundary faces); the sums are divided by
  * the volume, and every ghost takes its owner's gradient. */
-static void grad_ggnode(const orc_solver *s, const double *q, const double *qnode, double *gx, double *gy,
-                        double *gz)
+static void grad_ggnode(orc_solver *s, const double *q, const double *qnode, double *gx, double *gy, double *gz)
 {
     int nT = s->nTotal;
+    CELL_LOOP
     for (int c = 0; c < nT; ++c) { gx[c] = 0.0; gy[c] = 0.0; gz[c] = 0.0; }
-    for (int f = 0; f < s->nBFace; ++f) {
+    FACE_LOOP(s, 0, s->nBFace, f)
         int le = s->left[f], re = f + s->nCell;
         double qf = s->bcType[f] == ORC_BC_SYMMETRY ? face_node_mean(s, f, qnode) : 0.5 * (q[le] + q[re]);
         double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
-        gx[le] += qf * nx; gy[le] += qf * ny; gz[le] += qf * nz;
-    }
-    for (int f = s->nBFace; f < s->nFace; ++f) {
+        if (OWN(le)) { gx[le] += qf * nx; gy[le] += qf * ny; gz[le] += qf * nz; }
+    FACE_LOOP_END
+    FACE_LOOP(s, s->nBFace, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         double qf = face_node_mean(s, f, qnode);
         double nx = s->xfn[f] * s->area[f], ny = s->yfn[f] * s->area[f], nz = s->zfn[f] * s->area[f];
-        gx[le] += qf * nx; gy[le] += qf * ny; gz[le] += qf * nz;
-        gx[re] -= qf * nx; gy[re] -= qf * ny; gz[re] -= qf * nz;
-    }
+        if (OWN(le)) { gx[le] += qf * nx; gy[le] += qf * ny; gz[le] += qf * nz; }
+        if (OWN(re)) { gx[re] -= qf * nx; gy[re] -= qf * ny; gz[re] -= qf * nz; }
+    FACE_LOOP_END
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) { gx[c] /= s->vol[c]; gy[c] /= s->vol[c]; gz[c] /= s->vol[c]; }
     for (int f = 0; f < s->nBFace; ++f) {
         int le = s->left[f], re = f + s->nCell;
@@ -900,14 +1030,16 @@ double orc_venfun(double x, double y, double eps)
 }
 
 /* Limiter::MinMaxDiff, Limiter.cpp:1202-1254 (no interface faces here). */
-static void min_max_diff(const orc_solver *s, double *dmin, double *dmax, const double *q)
+static void min_max_diff(orc_solver *s, double *dmin, double *dmax, const double *q)
 {
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) { dmin[c] = q[c]; dmax[c] = q[c]; }
-    for (int f = s->nBFace; f < s->nFace; ++f) {
+    FACE_LOOP(s, s->nBFace, s->nFace, f)
         int le = s->left[f], re = s->right[f];
-        dmin[le] = MIN(dmin[le], q[re]); dmax[le] = MAX(dmax[le], q[re]);
-        dmin[re] = MIN(dmin[re], q[le]); dmax[re] = MAX(dmax[re], q[le]);
-    }
+        if (OWN(le)) { dmin[le] = MIN(dmin[le], q[re]); dmax[le] = MAX(dmax[le], q[re]); }
+        if (OWN(re)) { dmin[re] = MIN(dmin[re], q[le]); dmax[re] = MAX(dmax[re], q[le]); }
+    FACE_LOOP_END
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) { dmin[c] -= q[c]; dmax[c] -= q[c]; }
 }
 
@@ -923,22 +1055,25 @@ static void vencat7(orc_solver *s, double *limit, const double *q, const double 
     double *eps = (double *)malloc((size_t)nC * sizeof(double));
     double eps_tmp = s->p.venkat_k * s->p.venkat_k * s->p.venkat_k;
     eps_tmp *= 1.0 / pow(pow(s->avgVol, 1.0 / 3), n);
+    CELL_LOOP
     for (int c = 0; c < nC; ++c) {
         eps[c] = eps_tmp * q[c] * q[c];
         eps[c] *= pow(pow(s->vol[c], 1.0 / 3), n);
     }
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
-        double dx = s->xfc[f] - s->xcc[le], dy = s->yfc[f] - s->ycc[le], dz = s->zfc[f] - s->zcc[le];
-        double dqf = gx[le] * dx + gy[le] * dy + gz[le] * dz;
-        if (dqf > SMALL) limit[le] = MIN(limit[le], orc_venfun(dmax[le], dqf, eps[le]));
-        else if (dqf < -SMALL) limit[le] = MIN(limit[le], orc_venfun(dmin[le], dqf, eps[le]));
-        if (f < s->nBFace) continue;
-        dx = s->xfc[f] - s->xcc[re]; dy = s->yfc[f] - s->ycc[re]; dz = s->zfc[f] - s->zcc[re];
-        dqf = gx[re] * dx + gy[re] * dy + gz[re] * dz;
+        if (OWN(le)) {
+            double dx = s->xfc[f] - s->xcc[le], dy = s->yfc[f] - s->ycc[le], dz = s->zfc[f] - s->zcc[le];
+            double dqf = gx[le] * dx + gy[le] * dy + gz[le] * dz;
+            if (dqf > SMALL) limit[le] = MIN(limit[le], orc_venfun(dmax[le], dqf, eps[le]));
+            else if (dqf < -SMALL) limit[le] = MIN(limit[le], orc_venfun(dmin[le], dqf, eps[le]));
+        }
+        if (f < s->nBFace || !OWN(re)) continue;
+        double dx = s->xfc[f] - s->xcc[re], dy = s->yfc[f] - s->ycc[re], dz = s->zfc[f] - s->zcc[re];
+        double dqf = gx[re] * dx + gy[re] * dy + gz[re] * dz;
         if (dqf > SMALL) limit[re] = MIN(limit[re], orc_venfun(dmax[re], dqf, eps[re]));
         else if (dqf < -SMALL) limit[re] = MIN(limit[re], orc_venfun(dmin[re], dqf, eps[re]));
-    }
+    FACE_LOOP_END
     free(dmin); free(dmax); free(eps);
 }
 
@@ -1138,18 +1273,19 @@ void orc_zero_residual(orc_solver *s)
 
 void orc_inviscid_flux(orc_solver *s)
 {
-    double qL[5], qR[5], fl[5];
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
+        double qL[5], qR[5], fl[5];
         face_values(s, f, qL, qR);
         orc_roe_flux(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma,
                      s->p.entropy_fix, s->entropyCoef, s->entropyCoef, fl);
         double a = s->area[f];
         for (int m = 0; m < 5; ++m) fl[m] *= a;
         int le = s->left[f], re = s->right[f];
-        for (int m = 0; m < 5; ++m) s->res[m][le] -= fl[m];
-        if (f >= s->nBFace)
+        if (OWN(le))
+            for (int m = 0; m < 5; ++m) s->res[m][le] -= fl[m];
+        if (f >= s->nBFace && OWN(re))
             for (int m = 0; m < 5; ++m) s->res[m][re] += fl[m];
-    }
+    FACE_LOOP_END
 }
 
 /* ------------------------------------------------------------------ */
@@ -1166,7 +1302,7 @@ void orc_viscous_flux(orc_solver *s)
     double oprl = 1.0 / s->p.prandtl, oprt = 1.0 / s->p.prandtl_turb;
     double M2 = s->p.mach * s->p.mach;
     double cp = 1.0 / ((s->p.gamma - 1.0) * M2);
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         int bnd = f < s->nBFace;
         int wall = bnd && is_wall(s->bcType[f]);
@@ -1283,10 +1419,11 @@ void orc_viscous_flux(orc_solver *s)
         fv[4] = um * fv[1] + vm * fv[2] + wm * fv[3] + qNorm;
         double fl[5];
         for (int m = 0; m < 5; ++m) fl[m] = -oRe * s->area[f] * fv[m];
-        for (int m = 0; m < 5; ++m) s->res[m][le] -= fl[m];
-        if (!bnd)
+        if (OWN(le))
+            for (int m = 0; m < 5; ++m) s->res[m][le] -= fl[m];
+        if (!bnd && OWN(re))
             for (int m = 0; m < 5; ++m) s->res[m][re] += fl[m];
-    }
+    FACE_LOOP_END
 }
 
 /* Controller::UpdateResiduals (Controller.cpp:1002-1061) + LoadResiduals:
@@ -1326,8 +1463,9 @@ void orc_residual(orc_solver *s)
 void orc_spectrum(orc_solver *s)
 {
     int nC = s->nCell;
+    CELL_LOOP
     for (int c = 0; c < nC; ++c) s->specInv[c] = 0.0;
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         double nx = s->xfn[f], ny = s->yfn[f], nz = s->zfn[f];
         double rl = s->q[IR][le], ul = s->q[IU][le], vl = s->q[IV][le], wl = s->q[IW][le], pl = s->q[IP][le];
@@ -1340,12 +1478,13 @@ void orc_spectrum(orc_solver *s)
         double vn = 0.5 * (vnl + vnr);
         double cm = sqrt(g * pm / rm);
         double sr = 0.5 * s->area[f] * (ABS(vn) + cm);
-        s->specInv[le] += sr;
-        if (re < nC) s->specInv[re] += sr;
-    }
+        if (OWN(le)) s->specInv[le] += sr;
+        if (re < nC && OWN(re)) s->specInv[re] += sr;
+    FACE_LOOP_END
+    CELL_LOOP
     for (int c = 0; c < nC; ++c) s->specVis[c] = 0.0;
     if (!s->p.viscous) return;
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         double dx = s->xcc[re] - s->xcc[le], dy = s->ycc[re] - s->ycc[le], dz = s->zcc[re] - s->zcc[le];
         double ds = ABS(s->xfn[f] * dx + s->yfn[f] * dy + s->zfn[f] * dz);
@@ -1355,9 +1494,9 @@ void orc_spectrum(orc_solver *s)
         double vis = vl + vt;
         double sr = 2.0 * vis / (rho * ds * s->p.reynolds + SMALL);
         sr *= 0.5 * s->area[f];
-        s->specVis[le] += sr;
-        if (re < nC) s->specVis[re] += sr;
-    }
+        if (OWN(le)) s->specVis[le] += sr;
+        if (re < nC && OWN(re)) s->specVis[re] += sr;
+    FACE_LOOP_END
 }
 
 /* NSSolver::ComputeCFL (CFD/NSSolver/src/NSSolver.cpp:595-668), CFLMethod 0. */
@@ -1375,6 +1514,7 @@ void orc_time_step(orc_solver *s, int step)
     double cfl = compute_cfl(s, step);
     double cflStart = s->p.cfl_start;
     double pLower = 0.001 * s->pTotal, pUpper = 0.02 * s->pTotal;
+    CELL_LOOP
     for (int c = 0; c < nC; ++c) {
         double spec = s->specInv[c];
         if (s->p.viscous) spec += s->specVis[c];
@@ -1389,8 +1529,11 @@ void orc_time_step(orc_solver *s, int step)
     }
     double mn = LARGE;
     for (int c = 0; c < nC; ++c) mn = MIN(mn, s->dt[c]);
-    if (s->p.ktmax > 0)
+    if (s->p.ktmax > 0) {
+        CELL_LOOP
         for (int c = 0; c < nC; ++c) s->dt[c] = MIN(s->dt[c], s->p.ktmax * mn);
+    }
+    CELL_LOOP
     for (int c = 0; c < nC; ++c) s->dt[c] /= s->vol[c];
 }
 
@@ -1486,10 +1629,10 @@ static void lusgs_face(const orc_solver *s, int f, int cell, double *df)
 }
 
 /* SolveLUSGSForward, NSSolverUnstruct.cpp:15118-15444. */
-static double lusgs_forward(orc_solver *s)
+/* One cell of the forward sweep; d2[m] receives (dq - dq_old)^2. */
+static void lusgs_forward_cell(orc_solver *s, int c, double *d2)
 {
-    double sweep = 0.0;
-    for (int c = 0; c < s->nCell; ++c) {
+    {
         double rhs0[5], dqOld[5], Ux[5], df[5];
         for (int m = 0; m < 5; ++m) {
             rhs0[m] = 0.0;
@@ -1513,17 +1656,16 @@ static double lusgs_forward(orc_solver *s)
         for (int m = 0; m < 5; ++m) {
             s->dq[m][c] = dqTry[m];
             double d = s->dq[m][c] - dqOld[m];
-            sweep += d * d;
+            d2[m] = d * d;
         }
     }
-    return sweep;
 }
 
-/* SolveLUSGSBackward, NSSolverUnstruct.cpp:15446-15764. */
-static double lusgs_backward(orc_solver *s)
+/* One cell of the backward sweep (SolveLUSGSBackward,
+ * NSSolverUnstruct.cpp:15446-15764). */
+static void lusgs_backward_cell(orc_solver *s, int c, double *d2)
 {
-    double sweep = 0.0;
-    for (int c = s->nCell - 1; c >= 0; --c) {
+    {
         double rhs0[5], dqOld[5], Lx[5], df[5];
         for (int m = 0; m < 5; ++m) {
             rhs0[m] = 0.0;
@@ -1547,9 +1689,46 @@ static double lusgs_backward(orc_solver *s)
         for (int m = 0; m < 5; ++m) {
             s->dq[m][c] = dqTry[m];
             double d = s->dq[m][c] - dqOld[m];
-            sweep += d * d;
+            d2[m] = d * d;
         }
     }
+}
+
+/* A sweep in cell order (forward: increasing, backward: decreasing), the
+ * squared changes summed in that order.  The OpenMP build runs the cells of
+ * each dependency level in parallel and sums the same terms in the same
+ * order afterwards. */
+static double lusgs_sweep(orc_solver *s, int forward)
+{
+    const int nC = s->nCell;
+    double sweep = 0.0;
+#ifdef _OPENMP
+    struct orc_part *P = orc_partition(s);
+    double *d2 = (double *)malloc(sizeof(double) * 5 * (size_t)nC);
+    const int nl = forward ? P->nLvF : P->nLvB;
+    const int *ptr = forward ? P->lvPtrF : P->lvPtrB, *cells = forward ? P->lvCellsF : P->lvCellsB;
+    for (int l = 0; l < nl; ++l) {
+#pragma omp parallel for schedule(static)
+        for (int i = ptr[l]; i < ptr[l + 1]; ++i) {
+            const int c = cells[i];
+            if (forward) lusgs_forward_cell(s, c, d2 + 5 * (size_t)c);
+            else lusgs_backward_cell(s, c, d2 + 5 * (size_t)c);
+        }
+    }
+    for (int i = 0; i < nC; ++i) {
+        const int c = forward ? i : nC - 1 - i;
+        for (int m = 0; m < 5; ++m) sweep += d2[5 * (size_t)c + m];
+    }
+    free(d2);
+#else
+    for (int i = 0; i < nC; ++i) {
+        const int c = forward ? i : nC - 1 - i;
+        double d2[5];
+        if (forward) lusgs_forward_cell(s, c, d2);
+        else lusgs_backward_cell(s, c, d2);
+        for (int m = 0; m < 5; ++m) sweep += d2[m];
+    }
+#endif
     return sweep;
 }
 
@@ -1570,6 +1749,7 @@ static void prim2cons(const double *prim, double gama, double *Q)
  * SetWallBCGhostDQLUSGS (:14921-15000), SetExtraBCGhostDQLUSGS (:15002-15090). */
 static void set_ghost_dq(orc_solver *s)
 {
+    CELL_LOOP
     for (int f = 0; f < s->nBFace; ++f) {
         int bc = s->bcType[f];
         int le = s->left[f], re = s->right[f];
@@ -1611,6 +1791,7 @@ static void set_ghost_dq(orc_solver *s)
 /* Diagonal (:15867-15960), steady. */
 static void diagonal(orc_solver *s)
 {
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) {
         s->diag[c] = 1.0 / s->dt[c];
         s->diag[c] += s->specInv[c];
@@ -1620,7 +1801,9 @@ static void diagonal(orc_solver *s)
 
 /* SolutionFix (:14509-14585): average of admissible neighbours, read from
  * the partially updated field exactly as the in-place reference loop does. */
-static void solution_fix(const orc_solver *s, double *prim, int c)
+/* lower: the field the neighbours below c are read from (the in-place
+ * update has already replaced them); neighbours above c come from s->q. */
+static void solution_fix(const orc_solver *s, double *const *lower, double *prim, int c)
 {
     double roo = s->primInf[IR], poo = s->primInf[IP];
     double dl = 1.0e-6 * roo, pl = 1.0e-6 * poo;
@@ -1631,11 +1814,12 @@ static void solution_fix(const orc_solver *s, double *prim, int c)
         int le = s->left[f], re = s->right[f];
         int nb = le;
         if (c == le) nb = re;
-        if (s->q[IR][nb] < dl || s->q[IP][nb] < pl) continue;
+        double *const *src = nb < c ? lower : s->q;
+        if (src[IR][nb] < dl || src[IP][nb] < pl) continue;
         double vsur = 1.0;
         vs += vsur;
         for (int m = 0; m < 5; ++m) {
-            double v = s->q[m][nb];
+            double v = src[m][nb];
             if (m == IR || m == IP) v = ABS(v);
             prim[m] += v * vsur;
         }
@@ -1649,12 +1833,11 @@ static void solution_fix(const orc_solver *s, double *prim, int c)
 }
 
 /* UpdateFlowField (:14587-14765), perfect gas, in place. */
-static int update_flow(orc_solver *s)
+/* Q += dq for one cell, converted back to primitives (out). */
+static void update_cell(const orc_solver *s, int c, double *prim)
 {
-    double dl = 1.0e-6 * s->primInf[IR], pl = 1.0e-6 * s->primInf[IP];
-    int nNeg = 0;
-    for (int c = 0; c < s->nCell; ++c) {
-        double prim[5], Q[5];
+    {
+        double Q[5];
         for (int m = 0; m < 5; ++m) prim[m] = s->q[m][c];
         double g = s->gama[c];
         prim2cons(prim, g, Q);
@@ -1669,12 +1852,53 @@ static int update_flow(orc_solver *s)
             prim[IR] = Q[0]; prim[IU] = um; prim[IV] = vm; prim[IW] = wm;
             prim[IP] = (g - 1.0) * ie;
         }
+    }
+}
+
+/* UpdateFlowField: in cell order, each cell updated in place and repaired by
+ * SolutionFix if its density or pressure falls below 1e-6 of freestream.
+ * The OpenMP build updates every cell into a new field first, then repairs
+ * the flagged cells in increasing order reading lower neighbours from the
+ * new field and higher ones from the old (what the in-place loop sees). */
+static int update_flow(orc_solver *s)
+{
+    double dl = 1.0e-6 * s->primInf[IR], pl = 1.0e-6 * s->primInf[IP];
+    int nNeg = 0;
+#ifdef _OPENMP
+    const int nC = s->nCell;
+    double *nw[5];
+    for (int m = 0; m < 5; ++m) nw[m] = (double *)malloc(sizeof(double) * (size_t)nC);
+    unsigned char *bad = (unsigned char *)malloc((size_t)nC);
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nC; ++c) {
+        double prim[5];
+        update_cell(s, c, prim);
+        for (int m = 0; m < 5; ++m) nw[m][c] = prim[m];
+        bad[c] = prim[IR] < dl || prim[IP] < pl;
+    }
+    for (int c = 0; c < nC; ++c) {
+        if (!bad[c]) continue;
+        double prim[5];
+        nNeg++;
+        solution_fix(s, nw, prim, c);
+        for (int m = 0; m < 5; ++m) nw[m][c] = prim[m];
+    }
+#pragma omp parallel for schedule(static)
+    for (int c = 0; c < nC; ++c)
+        for (int m = 0; m < 5; ++m) s->q[m][c] = nw[m][c];
+    for (int m = 0; m < 5; ++m) free(nw[m]);
+    free(bad);
+#else
+    for (int c = 0; c < s->nCell; ++c) {
+        double prim[5];
+        update_cell(s, c, prim);
         if (prim[IR] < dl || prim[IP] < pl) {
             nNeg++;
-            solution_fix(s, prim, c);
+            solution_fix(s, s->q, prim, c);
         }
         for (int m = 0; m < 5; ++m) s->q[m][c] = prim[m];
     }
+#endif
     return nNeg;
 }
 
@@ -1691,9 +1915,9 @@ int orc_lusgs(orc_solver *s, double *sweep_normals)
         }
     double init = 0.0, eps = 1.1;
     for (int it = 0; it < nS; ++it) {
-        double sn = lusgs_forward(s);
+        double sn = lusgs_sweep(s, 1);
         set_ghost_dq(s);
-        sn += lusgs_backward(s);
+        sn += lusgs_sweep(s, 0);
         if (sweep_normals) sweep_normals[it] = sn;
         if (it == 0) init = sn;
         else eps = sn / init;
@@ -1958,6 +2182,7 @@ void orc_sa_boundary(orc_solver *s)
     const double nuInf = s->p.sa_nu_inf, vistInf = s->p.sa_vist_inf;
     const double roo = s->primInf[IR], uoo = s->primInf[IU], voo = s->primInf[IV], woo = s->primInf[IW];
     const double poo = s->primInf[IP], g = s->p.gamma;
+    CELL_LOOP
     for (int f = 0; f < s->nBFace; ++f) {
         int le = s->left[f], re = s->right[f], bc = s->bcType[f];
         if (is_wall(bc)) {
@@ -1998,6 +2223,7 @@ void orc_sa_boundary(orc_solver *s)
 void orc_sa_viscosity(orc_solver *s)
 {
     const double cv13 = 7.1 * 7.1 * 7.1;
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) {
         double nue = s->nut[c];
         if (nue > 0.0) {
@@ -2021,7 +2247,9 @@ void orc_sa_gradient(orc_solver *s)
 /* InitSpectrum (:1025-1071, steady): spec = 1/(turbCFLScale dt + SMALL). */
 static void sa_init_spectrum(orc_solver *s)
 {
+    CELL_LOOP
     for (int c = 0; c < s->nTotal; ++c) s->specT[c] = 0.0;
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) s->specT[c] = 1.0 / (s->p.turb_cfl_scale * s->dt[c] + SMALL);
 }
 
@@ -2104,6 +2332,7 @@ void orc_sa_source_point(double nut, double omega, double wdist, double rho, dou
 static void sa_source(orc_solver *s)
 {
     const double oRe = 1.0 / s->p.reynolds;
+    CELL_LOOP
     for (int c = 0; c < s->nCell; ++c) {
         double dudy = s->dqdy[IU][c], dudz = s->dqdz[IU][c];
         double dvdx = s->dqdx[IV][c], dvdz = s->dqdz[IV][c];
@@ -2123,7 +2352,7 @@ static void sa_source(orc_solver *s)
  * and NNDFlux (:5721) in the non-conservative form; LoadFlux (:2205). */
 static void sa_inviscid(orc_solver *s)
 {
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         double qLv[5], qRv[5];
         for (int m = 0; m < 5; ++m) { qLv[m] = s->q[m][le]; qRv[m] = s->q[m][re]; }
@@ -2145,9 +2374,9 @@ static void sa_inviscid(orc_solver *s)
         double vnLL = 0.5 * (vnL + ABS(vnL));
         double vnRR = 0.5 * (vnR - ABS(vnR));
         double fl = (vnLL * (1.0 * tl) + vnRR * (1.0 * tr)) * s->area[f];
-        s->resT[le] -= fl;
-        if (f >= s->nBFace) s->resT[re] += fl;
-    }
+        if (OWN(le)) s->resT[le] -= fl;
+        if (f >= s->nBFace && OWN(re)) s->resT[re] += fl;
+    FACE_LOOP_END
 }
 
 /* ViscousFlux: GetVisFaceValue mlt (:3893-4000, ONE_EQU) and ComputeVisflux
@@ -2157,7 +2386,7 @@ static void sa_viscous(orc_solver *s)
     const double cb2 = 0.622, sigma = 2.0 / 3.0, osigma = 1.0 / sigma;
     const double oRe = 1.0 / s->p.reynolds;
     const double turbSkew = s->p.turb_skew_angle;
-    for (int f = 0; f < s->nFace; ++f) {
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         double nueLeft = s->nut[le], visLeft = s->visl[le], rhoLeft = s->q[IR][le];
         double nueRight = s->nut[re], visRight = s->visl[re], rhoRight = s->q[IR][re];
@@ -2201,13 +2430,15 @@ static void sa_viscous(orc_solver *s)
         }
         double flux1 = -oRe * (1.0 + cb2) * mlt * dfdn * s->area[f];
         double flux2 = oRe * cb2 * osigma * dfdn * s->area[f];
-        s->resT[le] -= flux1;
-        s->resT[le] -= flux2 * (s->visl[le] / s->q[IR][le] + nueLeft);
-        if (f >= s->nBFace) {
+        if (OWN(le)) {
+            s->resT[le] -= flux1;
+            s->resT[le] -= flux2 * (s->visl[le] / s->q[IR][le] + nueLeft);
+        }
+        if (f >= s->nBFace && OWN(re)) {
             s->resT[re] += flux1;
             s->resT[re] += flux2 * (s->visl[re] / s->q[IR][re] + nueRight);
         }
-    }
+    FACE_LOOP_END
 }
 
 /* SpectrumRadiusOfOneEquation (:4690-4855), NSMB formation: convective
@@ -2217,7 +2448,8 @@ void orc_sa_spectrum(orc_solver *s)
 {
     const double oRe = 1.0 / s->p.reynolds;
     const double sigma = 2.0 / 3.0, osigma = 1.0 / sigma, cb2 = 0.622;
-    for (int f = 0; f < s->nFace; ++f) {
+    /* the face matrices are written by the thread that owns the left cell */
+    FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
         double rhoL = s->q[IR][le], uL = s->q[IU][le], vL = s->q[IV][le], wL = s->q[IW][le];
         double vnL = s->xfn[f] * uL + s->yfn[f] * vL + s->zfn[f] * wL - 0.0;
@@ -2227,10 +2459,12 @@ void orc_sa_spectrum(orc_solver *s)
         double vnR = s->xfn[f] * uR + s->yfn[f] * vR + s->zfn[f] * wR - 0.0;
         double rcR = 0 * 1.0 + (1 - 0) * rhoR;
         double invJacobianR = vnR < 0.0 ? rcR * vnR * s->area[f] : 0.0;
-        s->specT[le] += invJacobianL;
-        s->specT[re] -= invJacobianR;
-        s->matR[f] = invJacobianR;
-        s->matL[f] = -invJacobianL;
+        if (OWN(le)) s->specT[le] += invJacobianL;
+        if (OWN(re)) s->specT[re] -= invJacobianR;
+        if (OWN(le)) {
+            s->matR[f] = invJacobianR;
+            s->matL[f] = -invJacobianL;
+        }
         double orL = 1.0 / (rhoL + SMALL), orR = 1.0 / (rhoR + SMALL);
         double nueL = positive_sa(s->nut[le], s->visl[le], rhoL);
         double nueR = positive_sa(s->nut[re], s->visl[re], rhoR);
@@ -2242,13 +2476,15 @@ void orc_sa_spectrum(orc_solver *s)
         double oVolume = 1.0 / volume;
         double ns2 = s->area[f] * s->area[f];
         double term = oRe * oVolume * (1.0 + cb2) * (osigma * nueff * ns2);
-        s->specT[le] += term;
-        s->specT[re] += term;
-        s->matL[f] += -term;
-        s->matR[f] += -term;
-        s->matL[f] += oRe * oVolume * ns2 * (cb2 * osigma * nueffL);
-        s->matR[f] += oRe * oVolume * ns2 * (cb2 * osigma * nueffR);
-    }
+        if (OWN(le)) s->specT[le] += term;
+        if (OWN(re)) s->specT[re] += term;
+        if (OWN(le)) {
+            s->matL[f] += -term;
+            s->matR[f] += -term;
+            s->matL[f] += oRe * oVolume * ns2 * (cb2 * osigma * nueffL);
+            s->matR[f] += oRe * oVolume * ns2 * (cb2 * osigma * nueffR);
+        }
+    FACE_LOOP_END
 }
 
 /* TurbSolver::UpdateResiduals (TurbSolver.cpp:316-347) for the steady
@@ -2269,11 +2505,10 @@ void orc_sa_update_residuals(orc_solver *s)
  * SolveLUSGSBackward (:1521-1637), UpdateFlowField (:1180-1255, RANS with
  * qTurbLimit = 0: no clipping).  One sweep pair (dq initialised to the
  * residual, LUplusDQ to zero). */
-void orc_sa_lusgs(orc_solver *s)
+/* One cell of the scalar forward (forward = 1) or backward sweep. */
+static void sa_sweep_cell(orc_solver *s, int c, int forward)
 {
-    const int nC = s->nCell;
-    for (int c = 0; c < s->nTotal; ++c) { s->dqT[c] = s->resT[c]; s->dfluxT[c] = 0.0; }
-    for (int c = 0; c < nC; ++c) {
+    if (forward) {
         double rhs0 = 0.0;
         double ux = s->dfluxT[c];
         s->dfluxT[c] = 0.0;
@@ -2288,15 +2523,9 @@ void orc_sa_lusgs(orc_solver *s)
         }
         s->dqT[c] = (dqc - ux - rhs0) / s->specT[c];
         s->dfluxT[c] += rhs0;
+        return;
     }
-    for (int f = 0; f < s->nBFace; ++f) {
-        int bc = s->bcType[f], le = s->left[f], re = s->right[f];
-        if (is_wall(bc)) s->dqT[re] = 0.0;
-        else if (bc == ORC_BC_FARFIELD || bc == ORC_BC_SYMMETRY || bc == ORC_BC_INFLOW || bc == ORC_BC_OUTFLOW)
-            s->dqT[re] = s->dqT[le];
-        else s->dqT[re] = 0.0;
-    }
-    for (int c = nC - 1; c >= 0; --c) {
+    {
         double rhs0 = 0.0;
         double lx = s->dfluxT[c];
         s->dfluxT[c] = 0.0;
@@ -2312,6 +2541,39 @@ void orc_sa_lusgs(orc_solver *s)
         s->dqT[c] = (dqc - lx - rhs0) / s->specT[c];
         s->dfluxT[c] += rhs0;
     }
+}
+
+static void sa_sweep(orc_solver *s, int forward)
+{
+#ifdef _OPENMP
+    struct orc_part *P = orc_partition(s);
+    const int nl = forward ? P->nLvF : P->nLvB;
+    const int *ptr = forward ? P->lvPtrF : P->lvPtrB, *cells = forward ? P->lvCellsF : P->lvCellsB;
+    for (int l = 0; l < nl; ++l) {
+#pragma omp parallel for schedule(static)
+        for (int i = ptr[l]; i < ptr[l + 1]; ++i) sa_sweep_cell(s, cells[i], forward);
+    }
+#else
+    const int nC = s->nCell;
+    for (int i = 0; i < nC; ++i) sa_sweep_cell(s, forward ? i : nC - 1 - i, forward);
+#endif
+}
+
+void orc_sa_lusgs(orc_solver *s)
+{
+    const int nC = s->nCell;
+    CELL_LOOP
+    for (int c = 0; c < s->nTotal; ++c) { s->dqT[c] = s->resT[c]; s->dfluxT[c] = 0.0; }
+    sa_sweep(s, 1);
+    for (int f = 0; f < s->nBFace; ++f) {
+        int bc = s->bcType[f], le = s->left[f], re = s->right[f];
+        if (is_wall(bc)) s->dqT[re] = 0.0;
+        else if (bc == ORC_BC_FARFIELD || bc == ORC_BC_SYMMETRY || bc == ORC_BC_INFLOW || bc == ORC_BC_OUTFLOW)
+            s->dqT[re] = s->dqT[le];
+        else s->dqT[re] = 0.0;
+    }
+    sa_sweep(s, 0);
+    CELL_LOOP
     for (int c = 0; c < nC; ++c) s->nut[c] += s->dqT[c];
 }
 

@@ -13,6 +13,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libphl.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 ORACLE_LIB = os.path.join(ORACLE_DIR, "liborc.so")
+ORACLE_LIB_OMP = os.path.join(ORACLE_DIR, "liborc_omp.so")
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 HIP_FLAGS = [
@@ -44,12 +45,15 @@ def build_library(verbose=False):
 
 
 def build_oracle(verbose=False):
-    cmd = ["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wextra",
-           "-o", ORACLE_LIB + ".tmp", os.path.join(ORACLE_DIR, "phl_oracle.c"), "-lm"]
-    if verbose:
-        print(" ".join(cmd))
-    _run(cmd)
-    os.replace(ORACLE_LIB + ".tmp", ORACLE_LIB)
+    """The sequential oracle, and the same source built with OpenMP (the CPU
+    baseline over all host cores; identical results)."""
+    for lib, extra in ((ORACLE_LIB, []), (ORACLE_LIB_OMP, ["-fopenmp"])):
+        cmd = ["gcc", "-O2", "-ffp-contract=off", "-fPIC", "-shared", "-Wall", "-Wextra", *extra,
+               "-o", lib + ".tmp", os.path.join(ORACLE_DIR, "phl_oracle.c"), "-lm"]
+        if verbose:
+            print(" ".join(cmd))
+        _run(cmd)
+        os.replace(lib + ".tmp", lib)
     return ORACLE_LIB
 
 

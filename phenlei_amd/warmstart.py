@@ -328,14 +328,18 @@ WING_TRAIN_DIMS = (24, 12, 10, 1e-4, 8)
 
 
 def save_network(net, path, **meta):
-    np.savez(path, params=net.get_params(), layers=np.int32(net.layers), **meta)
+    """The parameters as one flat FP32 array in the network's layout order,
+    with plain numeric metadata (nothing that needs unpickling)."""
+    p = net.get_params()
+    flat = np.concatenate([np.asarray(p[k], np.float32).reshape(-1) for k, _ in net.layout()])
+    np.savez(path, params=flat, layers=np.int32(net.layers), **meta)
 
 
 def load_network(path, device=0):
-    """A network saved by save_network (plain arrays; nothing is unpickled)."""
+    """A network saved by save_network."""
     with np.load(path, allow_pickle=False) as z:
         net = Mpnn(layers=int(z["layers"]), seed=0, device=device)
-        net.set_params(z["params"])
+        net.set_params(net.unflatten(np.asarray(z["params"], np.float32)))
         meta = {k: z[k] for k in z.files if k not in ("params", "layers")}
     return net, meta
 
