@@ -584,7 +584,7 @@ def main():
                     help="also run STEPS iterations in FP64 and in the mixed form and compare CL/CD and fields")
     ap.add_argument("--face-loop", default=None, choices=["tile", "split"],
                     help="face-loop form (default: the library's, split)")
-    ap.add_argument("--sweep-form", default=None, choices=["dataflow", "levels", "ticket"],
+    ap.add_argument("--sweep-form", default=None, choices=["dataflow", "levels", "ticket", "xcd"],
                     help="LU-SGS sweep form (default: the library's, dataflow in launch order)")
     ap.add_argument("--storage-order", default=None, choices=["locality", "mesh"],
                     help="device storage order of the cells (default: the library's, locality)")
@@ -609,7 +609,7 @@ def main():
     print(f"[bench] config {args.config}: building mesh", file=sys.stderr, flush=True)
     so = {"locality": 0, "mesh": 1}.get(args.storage_order)
     fl = {"split": 0, "tile": 1}.get(args.face_loop)
-    sf = {"dataflow": 0, "levels": 1, "ticket": 2}.get(args.sweep_form)
+    sf = {"dataflow": 0, "levels": 1, "ticket": 2, "xcd": 3}.get(args.sweep_form)
     if cfg.get("batch"):
         parts, params_list = build_batch(cfg)
         for p in params_list:
@@ -712,7 +712,7 @@ def main():
         "config": {"workload": cfg["workload"], "cells": mesh.n_cell, "faces": mesh.n_face,
                    "cases": mesh.n_cases, "precision": "mixed" if mixed else "fp64",
                    "parallelism": "single GPU", "launch": args.launch,
-                   "sweep_form": ["dataflow", "levels", "ticket"][params.sweep_form],
+                   "sweep_form": ["dataflow", "levels", "ticket", "xcd"][params.sweep_form],
                    "tiles": None if tuned is None else {"flux": tuned["flux_tile"], "grad": tuned["grad_tile"]}},
         "roofline": roof,
         "roofline_other": other,

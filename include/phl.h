@@ -83,7 +83,9 @@ typedef struct phl_params {
                              * hold its cells' swept neighbours, chunks taken in launch order;
                              * 1 one launch per dependency level;
                              * 2 dataflow with chunks taken from a ticket counter (independent of the
-                             * workgroup dispatch order; slower on large meshes).
+                             * workgroup dispatch order; slower on large meshes);
+                             * 3 dataflow in launch order with each level's chunks split into eighths
+                             * (storage order), the k-th eighth on the blocks of the k-th XCD.
                              * The dataflow waits are bounded (0.2 s, phl_set_sweep_wait): a wait that
                              * runs out makes phl_iterate/phl_lusgs fail with PHL_ERR_NUMERIC. */
 } phl_params;
@@ -164,6 +166,22 @@ phl_status phl_mesh_metrics(const phl_mesh *m, double *xfc, double *yfc, double 
                             double *xfn, double *yfn, double *zfn, double *area,
                             double *xcc, double *ycc, double *zcc, double *vol, double *skew);
 void phl_mesh_destroy(phl_mesh *m);
+
+/* The reference's partitioned grid file (.fts, HDF5; Region::ReadGrid,
+ * API/Physics/src/Region.cpp:52, through IO_HDF5Read::ReadUnstructuredGrid,
+ * API/PreProcess/src/Pre_HDF5File.cpp:1750-1900), read without the HDF5
+ * library (superblock 0, symbol-table groups, contiguous datasets only).  The
+ * zones are joined into one: cells zone by zone, nodes shared across zones
+ * merged, the non-interface boundary faces first, then each zone's interior
+ * faces, then one face per interface pair.  A grid of two-node faces is a 2-D
+ * grid (the reference's 2-D metrics).  *n_zones receives the zone count. */
+phl_status phl_read_fts(const char *path, phl_mesh **out, int *n_zones);
+/* Its wall-distance file (.wdt: one dataset "Grid-<i>" of cell distances per
+ * zone; IO_HDF5Read::ReadWallDist, Pre_HDF5File.cpp:909-975) in the joined
+ * cell order; n_cell must match. */
+phl_status phl_read_wdt(const char *path, int n_cell, double *wdist);
+/* 2 for a 2-D grid (every face an edge), else 3. */
+int phl_mesh_dim(const phl_mesh *m);
 
 /* ---------------- output and restart files (host) ----------------------
  * Tecplot ASCII field file of a state, laid out like the reference's block

@@ -45,6 +45,9 @@ def load(omp=False):
         lib.orc_face_metrics.argtypes = [ctypes.c_int, _I, _I, _D, _D, _D] + [_D] * 7
         lib.orc_cell_metrics.argtypes = [ctypes.c_int] * 3 + [_I, _I, _I, _I, _I, _I, _D, _D, _D] + [_D] * 7 + [_D] * 4
         lib.orc_skewness.argtypes = [ctypes.c_int] * 3 + [_I, _I] + [_D] * 10 + [_D]
+        lib.orc_face_metrics_2d.argtypes = [ctypes.c_int, _I, _I, _D, _D, _D] + [_D] * 7
+        lib.orc_cell_metrics_2d.argtypes = [ctypes.c_int] * 3 + [_I, _I] + [_D] * 7 + [_D] * 4
+        lib.orc_set_dim.argtypes = [_P, ctypes.c_int]
         lib.orc_create.restype = _P
         lib.orc_create.argtypes = [_P, ctypes.c_int, ctypes.c_int, ctypes.c_int, _I, _I, _I] + [_D] * 12
         lib.orc_destroy.argtypes = [_P]
@@ -59,6 +62,8 @@ def load(omp=False):
             getattr(lib, name).argtypes = [_P]
         lib.orc_time_step.argtypes = [_P, ctypes.c_int]
         lib.orc_lusgs.argtypes = [_P, _D]
+        lib.orc_sweep_normals.argtypes = [_P, ctypes.c_int, _D]
+        lib.orc_sweep_normals.restype = ctypes.c_int
         lib.orc_lusgs.restype = ctypes.c_int
         lib.orc_residual_norm.argtypes = [_P, _D]
         lib.orc_iterate.argtypes = [_P, ctypes.c_int, ctypes.c_int, _D]
@@ -67,6 +72,7 @@ def load(omp=False):
         lib.orc_set_wall_distance.argtypes = [_P, _D]
         lib.orc_sa_face_matrices.argtypes = [_P, _D, _D]
         lib.orc_wall_distance.argtypes = [ctypes.c_int, _D, _D, _D, ctypes.c_int, _I, _I] + [_D] * 6 + [_D]
+        lib.orc_wall_distance_2d.argtypes = [ctypes.c_int, _D, _D, _D, ctypes.c_int, _I, _I] + [_D] * 6 + [_D]
         lib.orc_force_coefficients.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
         lib.orc_roe_flux.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]
@@ -90,13 +96,18 @@ def load(omp=False):
     return _libs[path]
 
 
+def is_2d(conn):
+    """A grid whose faces are all edges (two nodes) is 2-D."""
+    return bool(np.all(np.diff(np.asarray(conn["face_node_ptr"])) == 2))
+
+
 def face_metrics(conn):
     lib = load()
     nf = len(conn["left"])
     out = {k: np.empty(nf) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")}
-    lib.orc_face_metrics(nf, _ip(conn["face_node_ptr"]), _ip(conn["face_nodes"]), _dp(conn["x"]),
-                         _dp(conn["y"]), _dp(conn["z"]), *[_dp(out[k]) for k in
-                                                          ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")])
+    fn = lib.orc_face_metrics_2d if is_2d(conn) else lib.orc_face_metrics
+    fn(nf, _ip(conn["face_node_ptr"]), _ip(conn["face_nodes"]), _dp(conn["x"]), _dp(conn["y"]), _dp(conn["z"]),
+       *[_dp(out[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")])
     return out
 
 
@@ -107,11 +118,16 @@ def cell_metrics(conn, fm):
     nc = len(conn["cell_node_ptr"]) - 1
     nt = nc + nb
     out = {k: np.empty(nt) for k in ("xcc", "ycc", "zcc", "vol")}
-    lib.orc_cell_metrics(nc, nb, nf, _ip(conn["left"]), _ip(conn["right"]), _ip(conn["face_node_ptr"]),
-                         _ip(conn["face_nodes"]), _ip(conn["cell_node_ptr"]), _ip(conn["cell_nodes"]),
-                         _dp(conn["x"]), _dp(conn["y"]), _dp(conn["z"]),
-                         *[_dp(fm[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")],
-                         *[_dp(out[k]) for k in ("xcc", "ycc", "zcc", "vol")])
+    if is_2d(conn):
+        lib.orc_cell_metrics_2d(nc, nb, nf, _ip(conn["left"]), _ip(conn["right"]),
+                                *[_dp(fm[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")],
+                                *[_dp(out[k]) for k in ("xcc", "ycc", "zcc", "vol")])
+    else:
+        lib.orc_cell_metrics(nc, nb, nf, _ip(conn["left"]), _ip(conn["right"]), _ip(conn["face_node_ptr"]),
+                             _ip(conn["face_nodes"]), _ip(conn["cell_node_ptr"]), _ip(conn["cell_nodes"]),
+                             _dp(conn["x"]), _dp(conn["y"]), _dp(conn["z"]),
+                             *[_dp(fm[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")],
+                             *[_dp(out[k]) for k in ("xcc", "ycc", "zcc", "vol")])
     skew = np.empty(nc)
     lib.orc_skewness(nc, nb, nf, _ip(conn["left"]), _ip(conn["right"]),
                      *[_dp(fm[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")],
@@ -170,6 +186,8 @@ class OracleSolver:
         lists = [np.ascontiguousarray(conn[k], dtype=np.int32) for k in
                  ("face_node_ptr", "face_nodes", "cell_node_ptr", "cell_nodes")]
         lib.orc_set_nodes(self._h, self.n_node, *[_dp(a) for a in nodes], *[_ip(a) for a in lists])
+        self.dim = 2 if is_2d(conn) else 3
+        lib.orc_set_dim(self._h, self.dim)
 
     def compute_node_value(self):
         """ComputeNodeValue + ModifyNodeValue on the current state."""
@@ -222,6 +240,13 @@ class OracleSolver:
     def lusgs(self):
         sweeps = np.empty(8)
         return self._L.orc_lusgs(self._h, _dp(sweeps))
+
+    def sweep_normals(self, turb=False):
+        """Squared dq changes of each sweep pair of the last LU-SGS call (NS,
+        or SA with turb), at most the first 8."""
+        out = np.empty(8)
+        n = self._L.orc_sweep_normals(self._h, int(bool(turb)), _dp(out))
+        return out[:min(n, 8)]
 
     def residual_norm(self):
         out = np.empty(5)
@@ -276,6 +301,7 @@ def wall_distance(mesh_conn, metrics, bc_wall=2):
     out = np.empty(n_cell)
     c = [np.ascontiguousarray(metrics[k][:n_cell]) for k in ("xcc", "ycc", "zcc")]
     xyz = [np.ascontiguousarray(mesh_conn[k], dtype=np.float64) for k in ("x", "y", "z")]
-    load().orc_wall_distance(n_cell, _dp(c[0]), _dp(c[1]), _dp(c[2]), len(walls), _ip(ptr), _ip(nodes),
-                             _dp(wx), _dp(wy), _dp(wz), _dp(xyz[0]), _dp(xyz[1]), _dp(xyz[2]), _dp(out))
+    fn = load().orc_wall_distance_2d if is_2d(mesh_conn) else load().orc_wall_distance
+    fn(n_cell, _dp(c[0]), _dp(c[1]), _dp(c[2]), len(walls), _ip(ptr), _ip(nodes),
+       _dp(wx), _dp(wy), _dp(wz), _dp(xyz[0]), _dp(xyz[1]), _dp(xyz[2]), _dp(out))
     return out

@@ -88,6 +88,9 @@ struct orc_solver {
     double *xfc, *yfc, *zfc, *xfn, *yfn, *zfn, *area;
     double *xcc, *ycc, *zcc, *vol, *skew;
     double avgVol;
+    int dim;  /* 3, or 2 for a 2-D grid (orc_set_dim) */
+    double nsSweepNormals[8], saSweepNormals[8];  /* squared changes of each sweep pair (first 8) */
+    int nsSweepsDone, saSweepsDone;
     int *c2fPtr, *c2f;
     /* flow fields */
     double *q[5], *res[5], *dqdx[5], *dqdy[5], *dqdz[5], *dq[5], *dflux[5];
@@ -159,6 +162,81 @@ double orc_ptotal(double mach, double gama)
 /* ------------------------------------------------------------------ */
 /* Geometry                                                            */
 /* ------------------------------------------------------------------ */
+
+/* 2-D grids (UnstructGrid::ComputeMetrics, Geo_UnstructGrid.cpp:5695-5709):
+ * ComputeFaceNormal2D (:6158-6187) and ComputeFaceCenter2D (:6370-6390).  A
+ * face is the edge p1 -> p2; normal (y2 - y1, x1 - x2)/|.|, area its length,
+ * centre its midpoint. */
+void orc_face_metrics_2d(int nFace, const int *fp, const int *fn,
+                         const double *x, const double *y, const double *z,
+                         double *xfc, double *yfc, double *zfc,
+                         double *xfn, double *yfn, double *zfn, double *area)
+{
+    for (int f = 0; f < nFace; ++f) {
+        int p1 = fn[fp[f]], p2 = fn[fp[f] + 1];
+        xfn[f] = y[p2] - y[p1];
+        yfn[f] = x[p1] - x[p2];
+        zfn[f] = 0.0;
+        area[f] = sqrt(xfn[f] * xfn[f] + yfn[f] * yfn[f] + zfn[f] * zfn[f]);
+        double oarea = 1.0 / (area[f] + SMALL);
+        xfn[f] *= oarea;
+        yfn[f] *= oarea;
+        zfn[f] *= oarea;
+    }
+    for (int f = 0; f < nFace; ++f) {
+        int p1 = fn[fp[f]], p2 = fn[fp[f] + 1];
+        xfc[f] = 0.5 * (x[p1] + x[p2]);
+        yfc[f] = 0.5 * (y[p1] + y[p2]);
+        zfc[f] = 0.5 * (z[p1] + z[p2]);
+    }
+}
+
+/* ComputeCellCenterVol2D (Geo_UnstructGrid.cpp:7058-7190): the divergence
+ * theorem over the edges, boundary faces then interior faces (x . n) A
+ * accumulated, centroid / (1.5 sum), area sum / 2; ghosts mirrored across
+ * their face (degenerate faces: point reflection), ghost area = owner's. */
+void orc_cell_metrics_2d(int nCell, int nBFace, int nFace, const int *left, const int *right,
+                         const double *xfc, const double *yfc, const double *zfc,
+                         const double *xfn, const double *yfn, const double *zfn, const double *area,
+                         double *xcc, double *ycc, double *zcc, double *vol)
+{
+    int nT = nCell + nBFace;
+    for (int c = 0; c < nT; ++c) { xcc[c] = 0.0; ycc[c] = 0.0; zcc[c] = 0.0; }
+    for (int c = 0; c < nCell; ++c) vol[c] = 0.0;
+    for (int f = 0; f < nBFace; ++f) {
+        int le = left[f];
+        double dot = (xfc[f] * xfn[f] + yfc[f] * yfn[f] + zfc[f] * zfn[f]) * area[f];
+        xcc[le] += xfc[f] * dot; ycc[le] += yfc[f] * dot; zcc[le] += zfc[f] * dot;
+        vol[le] += dot;
+    }
+    for (int f = nBFace; f < nFace; ++f) {
+        int le = left[f], re = right[f];
+        double dot = (xfc[f] * xfn[f] + yfc[f] * yfn[f] + zfc[f] * zfn[f]) * area[f];
+        xcc[le] += xfc[f] * dot; ycc[le] += yfc[f] * dot; zcc[le] += zfc[f] * dot;
+        vol[le] += dot;
+        xcc[re] -= xfc[f] * dot; ycc[re] -= yfc[f] * dot; zcc[re] -= zfc[f] * dot;
+        vol[re] -= dot;
+    }
+    for (int c = 0; c < nCell; ++c) {
+        double tmp = 1.0 / (1.5 * vol[c] + SMALL);
+        xcc[c] *= tmp; ycc[c] *= tmp; zcc[c] *= tmp;
+        vol[c] *= 0.5;
+    }
+    for (int f = 0; f < nBFace; ++f) {
+        int le = left[f], re = f + nCell;
+        if (area[f] > SMALL) {
+            double tmp = 2.0 * ((xcc[le] - xfc[f]) * xfn[f] + (ycc[le] - yfc[f]) * yfn[f] + (zcc[le] - zfc[f]) * zfn[f]);
+            xcc[re] = xcc[le] - xfn[f] * tmp;
+            ycc[re] = ycc[le] - yfn[f] * tmp;
+            zcc[re] = zcc[le] - zfn[f] * tmp;
+        } else {
+            xcc[re] = -xcc[le] + 2.0 * xfc[f];
+            ycc[re] = -ycc[le] + 2.0 * yfc[f];
+            zcc[re] = -zcc[le] + 2.0 * zfc[f];
+        }
+        vol[re] = vol[le];
+    }
+}
 
 /* UnstructGrid::ComputeFaceCenterAreaNormal3DNew2,
  * API/Geometry/src/Geo_UnstructGrid.cpp:5911-6157 (static grid). */
@@ -447,6 +525,7 @@ orc_solver *orc_create(const orc_params *p, int nCell, int nBFace, int nFace,
     double tv = 0.0;
     for (int c = 0; c < nCell; ++c) tv += vol[c];
     s->avgVol = tv / nCell;
+    s->dim = 3;
     /* cell -> face in increasing face order (as built from the face loop). */
     s->c2fPtr = (int *)calloc((size_t)nCell + 1, sizeof(int));
     for (int f = 0; f < nFace; ++f) {
@@ -1054,11 +1133,11 @@ static void vencat7(orc_solver *s, double *limit, const double *q, const double 
     double n = 3.0;
     double *eps = (double *)malloc((size_t)nC * sizeof(double));
     double eps_tmp = s->p.venkat_k * s->p.venkat_k * s->p.venkat_k;
-    eps_tmp *= 1.0 / pow(pow(s->avgVol, 1.0 / 3), n);
+    eps_tmp *= 1.0 / pow(pow(s->avgVol, 1.0 / s->dim), n);
     CELL_LOOP
     for (int c = 0; c < nC; ++c) {
         eps[c] = eps_tmp * q[c] * q[c];
-        eps[c] *= pow(pow(s->vol[c], 1.0 / 3), n);
+        eps[c] *= pow(pow(s->vol[c], 1.0 / s->dim), n);
     }
     FACE_LOOP(s, 0, s->nFace, f)
         int le = s->left[f], re = s->right[f];
@@ -1918,12 +1997,22 @@ int orc_lusgs(orc_solver *s, double *sweep_normals)
         double sn = lusgs_sweep(s, 1);
         set_ghost_dq(s);
         sn += lusgs_sweep(s, 0);
-        if (sweep_normals) sweep_normals[it] = sn;
+        if (sweep_normals && it < 8) sweep_normals[it] = sn;
+        if (it < 8) s->nsSweepNormals[it] = sn;
+        s->nsSweepsDone = it + 1;
         if (it == 0) init = sn;
         else eps = sn / init;
         if (eps < s->p.lusgs_tol) break;
     }
     return update_flow(s);
+}
+
+/* The sweep pairs the last LU-SGS call made (NS: turb = 0; SA: turb = 1)
+ * and the squared changes of the first 8 pairs. */
+int orc_sweep_normals(const orc_solver *s, int turb, double *out8)
+{
+    for (int i = 0; i < 8; ++i) out8[i] = turb ? s->saSweepNormals[i] : s->nsSweepNormals[i];
+    return turb ? s->saSweepsDone : s->nsSweepsDone;
 }
 
 /* ComputeResidualonGrid (API/Physics/src/Residual.cpp:534-617), unit grid
@@ -2095,6 +2184,10 @@ static double positive_sa(double nue, double visl, double rho)
     return nue;
 }
 
+/* A 2-D grid: the Venkatakrishnan scale takes square roots of areas
+ * (Limiter.cpp:1122-1131 with GetDim() = 2). */
+void orc_set_dim(orc_solver *s, int dim) { s->dim = (dim == 2) ? 2 : 3; }
+
 void orc_set_wall_distance(orc_solver *s, const double *wd)
 {
     for (int c = 0; c < s->nCell; ++c) s->wdist[c] = wd[c];
@@ -2148,6 +2241,74 @@ done:
     {
         double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
         return dx * dx + dy * dy + dz * dz;
+    }
+}
+
+/* Pre_WalldistCompute::ComputeDistanceOfANodeToSegment
+ * (API/PreProcess/src/Pre_WalldistCompute.cpp:4260-4352): squared distance to
+ * the edge start-end whose centre is mid, by 30 bisections. */
+static double seg_dist2(const double *p, const double *start, const double *mid, const double *end)
+{
+    double xmin = start[0], ymin = start[1], zmin = start[2];
+    double xmax = end[0], ymax = end[1], zmax = end[2];
+    double xh = mid[0], yh = mid[1], zh = mid[2];
+    double dx = p[0] - xh, dy = p[1] - yh, dz = p[2] - zh;
+    double distMid = dx * dx + dy * dy + dz * dz;
+    dx = p[0] - xmin; dy = p[1] - ymin; dz = p[2] - zmin;
+    double distStart = dx * dx + dy * dy + dz * dz;
+    dx = p[0] - xmax; dy = p[1] - ymax; dz = p[2] - zmax;
+    double distEnd = dx * dx + dy * dy + dz * dz;
+    if (distStart < distMid) return distStart;
+    if (distEnd < distMid) return distEnd;
+    double distance = distMid;
+    for (int iter = 0; iter < 30; ++iter) {
+        double x1 = (xmin + xh) * 0.5, y1 = (ymin + yh) * 0.5, z1 = (zmin + zh) * 0.5;
+        dx = p[0] - x1; dy = p[1] - y1; dz = p[2] - z1;
+        double dist1 = dx * dx + dy * dy + dz * dz;
+        double x2 = (xmax + xh) * 0.5, y2 = (ymax + yh) * 0.5, z2 = (zmax + zh) * 0.5;
+        dx = p[0] - x2; dy = p[1] - y2; dz = p[2] - z2;
+        double dist2 = dx * dx + dy * dy + dz * dz;
+        if (distance < dist1 && distance < dist2) {
+            xmin = x1; ymin = y1; zmin = z1;
+            xmax = x2; ymax = y2; zmax = z2;
+        } else if (dist1 < dist2) {
+            xmax = xh; ymax = yh; zmax = zh;
+            xh = x1; yh = y1; zh = z1;
+            distance = dist1;
+        } else {
+            xmin = xh; ymin = yh; zmin = zh;
+            xh = x2; yh = y2; zh = z2;
+            distance = dist2;
+        }
+    }
+    return distance;
+}
+
+/* ComputeWallDist2DByBinarySearching (Pre_WalldistCompute.cpp:359-507) and
+ * PostWallDistance (:3913-3923): every wall edge taken from each of its nodes
+ * in turn with the face centre as the first midpoint, edges of squared length
+ * <= 1e-16 skipped; the square root of the minimum. */
+void orc_wall_distance_2d(int nCell, const double *xcc, const double *ycc, const double *zcc, int nWall,
+                          const int *wallFaceNodePtr, const int *wallFaceNodes, const double *wxfc,
+                          const double *wyfc, const double *wzfc, const double *x, const double *y,
+                          const double *z, double *out)
+{
+    for (int c = 0; c < nCell; ++c) {
+        double p[3] = {xcc[c], ycc[c], zcc[c]};
+        double best = LARGE;
+        for (int f = 0; f < nWall; ++f) {
+            double mid[3] = {wxfc[f], wyfc[f], wzfc[f]};
+            int b0 = wallFaceNodePtr[f], n = wallFaceNodePtr[f + 1] - b0;
+            for (int i = 0; i < n; ++i) {
+                int n1 = wallFaceNodes[b0 + i], n2 = wallFaceNodes[b0 + (i + 1) % n];
+                double a[3] = {x[n1], y[n1], z[n1]}, e[3] = {x[n2], y[n2], z[n2]};
+                double dx = e[0] - a[0], dy = e[1] - a[1], dz = e[2] - a[2];
+                if (dx * dx + dy * dy + dz * dz <= 1.0e-16) continue;
+                double d2 = seg_dist2(p, a, mid, e);
+                if (best > d2) best = d2;
+            }
+        }
+        out[c] = sqrt(best);
     }
 }
 
@@ -2503,76 +2664,77 @@ void orc_sa_update_residuals(orc_solver *s)
 /* Scalar LU-SGS: SolveLUSGSForward (:1400-1519), SetGhostDQLUSGS
  * (:1639-1731: wall 0; farfield, symmetry, inflow, outflow copy; others 0),
  * SolveLUSGSBackward (:1521-1637), UpdateFlowField (:1180-1255, RANS with
- * qTurbLimit = 0: no clipping).  One sweep pair (dq initialised to the
- * residual, LUplusDQ to zero). */
-/* One cell of the scalar forward (forward = 1) or backward sweep. */
-static void sa_sweep_cell(orc_solver *s, int c, int forward)
+ * qTurbLimit = 0: no clipping), in Controller::LUSGS's loop of nLUSGSSweeps
+ * sweep pairs (Controller.cpp:1878-1961): dq starts from the residual for one
+ * pair and from zero for several, LUplusDQ from zero; the pairs stop once
+ * the squared changes of a pair fall below LUSGSTolerance times those of the
+ * first. */
+/* One cell of the scalar forward (forward = 1) or backward sweep; returns
+ * (dq - dq_old)^2. */
+static double sa_sweep_cell(orc_solver *s, int c, int forward)
 {
-    if (forward) {
-        double rhs0 = 0.0;
-        double ux = s->dfluxT[c];
-        s->dfluxT[c] = 0.0;
-        double dqc = s->resT[c];
-        for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
-            int f = s->c2f[k], le = s->left[f], re = s->right[f];
-            if (le > c || re > c) continue;
-            double df;
-            if (re == c) { int t = le; le = re; re = t; df = s->matL[f] * s->dqT[re]; }
-            else df = s->matR[f] * s->dqT[re];
-            rhs0 += df;
-        }
-        s->dqT[c] = (dqc - ux - rhs0) / s->specT[c];
-        s->dfluxT[c] += rhs0;
-        return;
+    double rhs0 = 0.0;
+    const double dqOld = s->dqT[c];
+    const double ux = s->dfluxT[c];  /* Ux (forward) or Lx (backward) */
+    s->dfluxT[c] = 0.0;
+    const double dqc = s->resT[c];
+    for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
+        int f = s->c2f[k], le = s->left[f], re = s->right[f];
+        if (forward ? (le > c || re > c) : (le < c || re < c)) continue;
+        double df;
+        if (re == c) { int t = le; le = re; re = t; df = s->matL[f] * s->dqT[re]; }
+        else df = s->matR[f] * s->dqT[re];
+        rhs0 += df;
     }
-    {
-        double rhs0 = 0.0;
-        double lx = s->dfluxT[c];
-        s->dfluxT[c] = 0.0;
-        double dqc = s->resT[c];
-        for (int k = s->c2fPtr[c]; k < s->c2fPtr[c + 1]; ++k) {
-            int f = s->c2f[k], le = s->left[f], re = s->right[f];
-            if (le < c || re < c) continue;
-            double df;
-            if (re == c) { int t = le; le = re; re = t; df = s->matL[f] * s->dqT[re]; }
-            else df = s->matR[f] * s->dqT[re];
-            rhs0 += df;
-        }
-        s->dqT[c] = (dqc - lx - rhs0) / s->specT[c];
-        s->dfluxT[c] += rhs0;
-    }
+    s->dqT[c] = (dqc - ux - rhs0) / s->specT[c];
+    s->dfluxT[c] += rhs0;
+    const double d = s->dqT[c] - dqOld;
+    return d * d;
 }
 
-static void sa_sweep(orc_solver *s, int forward)
+static double sa_sweep(orc_solver *s, int forward)
 {
+    const int nC = s->nCell;
+    double sweep = 0.0;
 #ifdef _OPENMP
     struct orc_part *P = orc_partition(s);
+    double *d2 = (double *)malloc(sizeof(double) * (size_t)nC);
     const int nl = forward ? P->nLvF : P->nLvB;
     const int *ptr = forward ? P->lvPtrF : P->lvPtrB, *cells = forward ? P->lvCellsF : P->lvCellsB;
     for (int l = 0; l < nl; ++l) {
 #pragma omp parallel for schedule(static)
-        for (int i = ptr[l]; i < ptr[l + 1]; ++i) sa_sweep_cell(s, cells[i], forward);
+        for (int i = ptr[l]; i < ptr[l + 1]; ++i) d2[cells[i]] = sa_sweep_cell(s, cells[i], forward);
     }
+    for (int i = 0; i < nC; ++i) sweep += d2[forward ? i : nC - 1 - i];
+    free(d2);
 #else
-    const int nC = s->nCell;
-    for (int i = 0; i < nC; ++i) sa_sweep_cell(s, forward ? i : nC - 1 - i, forward);
+    for (int i = 0; i < nC; ++i) sweep += sa_sweep_cell(s, forward ? i : nC - 1 - i, forward);
 #endif
+    return sweep;
 }
 
 void orc_sa_lusgs(orc_solver *s)
 {
-    const int nC = s->nCell;
+    const int nC = s->nCell, nS = s->p.n_lusgs_sweeps;
     CELL_LOOP
-    for (int c = 0; c < s->nTotal; ++c) { s->dqT[c] = s->resT[c]; s->dfluxT[c] = 0.0; }
-    sa_sweep(s, 1);
-    for (int f = 0; f < s->nBFace; ++f) {
-        int bc = s->bcType[f], le = s->left[f], re = s->right[f];
-        if (is_wall(bc)) s->dqT[re] = 0.0;
-        else if (bc == ORC_BC_FARFIELD || bc == ORC_BC_SYMMETRY || bc == ORC_BC_INFLOW || bc == ORC_BC_OUTFLOW)
-            s->dqT[re] = s->dqT[le];
-        else s->dqT[re] = 0.0;
+    for (int c = 0; c < s->nTotal; ++c) { s->dqT[c] = (nS == 1) ? s->resT[c] : 0.0; s->dfluxT[c] = 0.0; }
+    double init = 0.0, eps = 1.1;
+    for (int it = 0; it < nS; ++it) {
+        double sn = sa_sweep(s, 1);
+        for (int f = 0; f < s->nBFace; ++f) {
+            int bc = s->bcType[f], le = s->left[f], re = s->right[f];
+            if (is_wall(bc)) s->dqT[re] = 0.0;
+            else if (bc == ORC_BC_FARFIELD || bc == ORC_BC_SYMMETRY || bc == ORC_BC_INFLOW || bc == ORC_BC_OUTFLOW)
+                s->dqT[re] = s->dqT[le];
+            else s->dqT[re] = 0.0;
+        }
+        sn += sa_sweep(s, 0);
+        if (it < 8) s->saSweepNormals[it] = sn;
+        s->saSweepsDone = it + 1;
+        if (it == 0) init = sn;
+        else eps = sn / init;
+        if (eps < s->p.lusgs_tol) break;
     }
-    sa_sweep(s, 0);
     CELL_LOOP
     for (int c = 0; c < nC; ++c) s->nut[c] += s->dqT[c];
 }

@@ -70,6 +70,12 @@ enum { ORC_Q = 0, ORC_RES = 1, ORC_DQDX = 2, ORC_DQDY = 3, ORC_DQDZ = 4,
 typedef struct orc_solver orc_solver;
 
 /* ---- geometry (restates API/Geometry/src/Geo_UnstructGrid.cpp) ---- */
+void orc_face_metrics_2d(int nFace, const int *fp, const int *fn, const double *x, const double *y, const double *z,
+                         double *xfc, double *yfc, double *zfc, double *xfn, double *yfn, double *zfn, double *area);
+void orc_cell_metrics_2d(int nCell, int nBFace, int nFace, const int *left, const int *right,
+                         const double *xfc, const double *yfc, const double *zfc,
+                         const double *xfn, const double *yfn, const double *zfn, const double *area,
+                         double *xcc, double *ycc, double *zcc, double *vol);
 void orc_face_metrics(int nFace, const int *faceNodePtr, const int *faceNodes,
                       const double *x, const double *y, const double *z,
                       double *xfc, double *yfc, double *zfc,
@@ -123,14 +129,22 @@ void orc_residual(orc_solver *s);                  /* LoadResiduals + Controller
 void orc_spectrum(orc_solver *s);
 void orc_time_step(orc_solver *s, int step);
 int  orc_lusgs(orc_solver *s, double *sweep_normals); /* returns #negative cells fixed */
+/* sweep pairs of the last NS (turb 0) or SA (turb 1) LU-SGS call; the
+ * squared changes of its first 8 pairs in out8 */
+int orc_sweep_normals(const orc_solver *s, int turb, double *out8);
 void orc_residual_norm(const orc_solver *s, double *sum5);
 int  orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist);
 /* ---- Spalart-Allmaras (TurbSolverUnstr) ---- */
 void orc_set_wall_distance(orc_solver *s, const double *wd);
+void orc_set_dim(orc_solver *s, int dim);
 void orc_wall_distance(int nCell, const double *xcc, const double *ycc, const double *zcc, int nWall,
                        const int *wallFaceNodePtr, const int *wallFaceNodes, const double *wxfc,
                        const double *wyfc, const double *wzfc, const double *x, const double *y,
                        const double *z, double *out);
+void orc_wall_distance_2d(int nCell, const double *xcc, const double *ycc, const double *zcc, int nWall,
+                          const int *wallFaceNodePtr, const int *wallFaceNodes, const double *wxfc,
+                          const double *wyfc, const double *wzfc, const double *x, const double *y,
+                          const double *z, double *out);
 void orc_sa_boundary(orc_solver *s);
 void orc_sa_viscosity(orc_solver *s);
 void orc_sa_gradient(orc_solver *s);

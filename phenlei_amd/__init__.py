@@ -157,6 +157,9 @@ def _declare(lib):
         "phl_forces": (ctypes.c_int, [_P, _D]),
         "phl_forces_case": (ctypes.c_int, [_P, ctypes.c_int, _D]),
         "phl_mesh_concat": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int, ctypes.POINTER(_P)]),
+        "phl_read_fts": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_P), _I]),
+        "phl_read_wdt": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, _D]),
+        "phl_mesh_dim": (ctypes.c_int, [_P]),
         "phl_mesh_cases": (ctypes.c_int, [_P, _I, _I, _I]),
         "phl_create_batch": (ctypes.c_int, [_P, ctypes.POINTER(Params), ctypes.c_int, ctypes.POINTER(_P)]),
         "phl_batch_size": (ctypes.c_int, [_P, _I]),
@@ -243,6 +246,25 @@ class Mesh:
         self.case_cell_ptr = np.empty(self.n_cases + 1, np.int32)
         self.case_bface_ptr = np.empty(self.n_cases + 1, np.int32)
         _check(lib.phl_mesh_cases(self._h, ctypes.byref(nc), _iptr(self.case_cell_ptr), _iptr(self.case_bface_ptr)))
+        self.dim = lib.phl_mesh_dim(self._h)
+        self.n_zones = 1
+
+    @classmethod
+    def from_fts(cls, path):
+        """The reference's partitioned grid file (phl_read_fts): its zones
+        joined into one; n_zones is the count in the file."""
+        h = _P()
+        nz = ctypes.c_int()
+        _check(load_library().phl_read_fts(os.fsencode(path), ctypes.byref(h), ctypes.byref(nz)))
+        m = cls(h)
+        m.n_zones = nz.value
+        return m
+
+    def read_wdt(self, path):
+        """Cell wall distances of the reference's .wdt file, joined cell order."""
+        wd = np.empty(self.n_cell)
+        _check(load_library().phl_read_wdt(os.fsencode(path), self.n_cell, _dptr(wd)))
+        return wd
 
     @classmethod
     def bump_hex(cls, nx, ny, nz, bump=0.05):

@@ -22,8 +22,8 @@ HIP_FLAGS = [
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-Wall", "-Wno-unused-function",
 ]
-SOURCES = ["phl_kernels.hip", "phl_sa.hip", "phl_api.cpp", "phl_mesh.cpp", "phl_output.cpp", "phl_mpnn.hip",
-           "phl_mpnn_api.cpp"]
+SOURCES = ["phl_kernels.hip", "phl_sa.hip", "phl_api.cpp", "phl_mesh.cpp", "phl_output.cpp", "phl_fts.cpp",
+           "phl_mpnn.hip", "phl_mpnn_api.cpp"]
 
 
 def _run(cmd):

@@ -237,6 +237,20 @@ phl_status build_flow_order(phl_ctx *c, int nC, const std::vector<int> &cells, c
         }
     }
     if (dep.empty()) dep.push_back(0);  // keeps the device pointer valid
+    // XCD groups (sweep_form 3): block ids rise with the level, as in launch
+    // order, so the in-order dispatch argument for progress is unchanged; the
+    // padding blocks of a level return at once
+    std::vector<int> blockChunk;
+    {
+        int ch0 = 0;
+        for (size_t L = 0; L + 1 < lptr.size(); ++L) {
+            const int nL = (lptr[L + 1] - lptr[L] + 63) / 64, g = (nL + 7) / 8;
+            for (int j = 0; j < g; ++j)
+                for (int x = 0; x < 8; ++x) blockChunk.push_back(x * g + j < nL ? ch0 + x * g + j : -1);
+            ch0 += nL;
+        }
+        if (blockChunk.empty()) blockChunk.push_back(-1);
+    }
     std::vector<int> width(std::max(nCh, 1), 0);
     for (int k = 0; k < nCh; ++k)
         for (int j = 0; j < 64; ++j)
@@ -249,6 +263,8 @@ phl_status build_flow_order(phl_ctx *c, int nC, const std::vector<int> &cells, c
     TRY(upload(c, &t, fnb)); fo.nb = t;
     TRY(upload(c, &t, depPtr)); fo.depPtr = t;
     TRY(upload(c, &t, dep)); fo.dep = t;
+    TRY(upload(c, &t, blockChunk)); fo.blockChunk = t;
+    fo.nBlocks = (int)blockChunk.size();
     fo.nChunks = nCh;
     return PHL_OK;
 }
@@ -308,11 +324,21 @@ phl_status check_params(const phl_params &p)
     if (p.n_lusgs_sweeps != 1) return fail(PHL_ERR_UNSUPPORTED, "nLUSGSSweeps must be 1");
     if (p.precision != 0 && p.precision != 1)
         return fail(PHL_ERR_UNSUPPORTED, "precision must be 0 (FP64) or 1 (mixed)");
-    if (p.sweep_form < 0 || p.sweep_form > 2)
-        return fail(PHL_ERR_ARG, "sweep_form must be 0 (dataflow, launch order), 1 (one launch per level) or 2 "
-                                 "(dataflow, ticket order)");
+    if (p.sweep_form < 0 || p.sweep_form > 3)
+        return fail(PHL_ERR_ARG, "sweep_form must be 0 (dataflow, launch order), 1 (one launch per level), 2 "
+                                 "(dataflow, ticket order) or 3 (dataflow, levels split over the XCDs)");
     if (p.t_wall > 0.0) return fail(PHL_ERR_UNSUPPORTED, "isothermal walls are not implemented (t_wall <= 0)");
     if (p.cfl_vary_step <= 0) return fail(PHL_ERR_ARG, "cfl_vary_step must be positive");
+    return PHL_OK;
+}
+
+// Options that depend on the grid: the least-squares weights are restated
+// for 3-D grids only (the reference's 2-D branch, Geo_UnstructGrid.cpp:3879,
+// inverts 2x2 matrices).
+phl_status check_mesh_params(const phl::Mesh &m, const phl_params &p)
+{
+    if (m.dim == 2 && p.gradient == 1)
+        return fail(PHL_ERR_UNSUPPORTED, "the lsq gradient is implemented for 3-D grids only");
     return PHL_OK;
 }
 
@@ -472,6 +498,10 @@ phl_status phl_mesh_create(int nNode, const double *x, const double *y, const do
             return fail(PHL_ERR_ARG, "phl_mesh_create: face " + std::to_string(f) + " has bad left/right cells");
         }
     }
+    // a planar grid: every face an edge (the reference's 2-D metric path)
+    bool edges = true;
+    for (int f = 0; f < nFace && edges; ++f) edges = faceNodePtr[f + 1] - faceNodePtr[f] == 2;
+    m.dim = edges ? 2 : 3;
     m.computeMetrics();
     *out = pm;
     return PHL_OK;
@@ -627,6 +657,8 @@ phl_status phl_mesh_metrics(const phl_mesh *pm, double *xfc, double *yfc, double
 
 void phl_mesh_destroy(phl_mesh *m) { delete m; }
 
+int phl_mesh_dim(const phl_mesh *m) { return m ? m->m.dim : 0; }
+
 // ------------------------------------------------------------- lifecycle
 phl_status phl_device_info(int device, char *name, int name_len, int *n_cu, double *hbm_bytes)
 {
@@ -667,7 +699,7 @@ void phl_destroy(phl_ctx *c)
 
 // The freestream-dependent scalars of one case (Param_NSSolver and
 // PerfectGas nondimensionalisation: rho = 1, |V| = 1, p = 1/(gamma M^2)).
-static phl::CaseConsts case_consts(const phl_params &p, double avgVol)
+static phl::CaseConsts case_consts(const phl_params &p, double avgVol, int dim)
 {
     phl::CaseConsts cc{};
     cc.R = 1.0 / (p.gamma * p.mach * p.mach);
@@ -681,7 +713,7 @@ static phl::CaseConsts case_consts(const phl_params &p, double avgVol)
     cc.primInf[4] = cc.R * 1.0 * 1.0 / 1.0;
     cc.entropyCoef = entropy_coef(p.mach, p.entropy_scale);
     double e = p.venkat_k * p.venkat_k * p.venkat_k;
-    e *= 1.0 / std::pow(std::pow(avgVol, 1.0 / 3), 3.0);
+    e *= 1.0 / std::pow(std::pow(avgVol, 1.0 / dim), 3.0);
     cc.epsTmp = e;
     const double pT = ptotal(p.mach, p.gamma);
     cc.pLower = 0.001 * pT;
@@ -947,7 +979,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     // Venkatakrishnan scale and CFL skewness factor (geometry only).
     std::vector<double> vf(nC), cflSkew(nC);
     for (int i = 0; i < nC; ++i) {
-        vf[slotOf[i]] = std::pow(std::pow(m.vol[i], 1.0 / 3), 3.0);
+        vf[slotOf[i]] = std::pow(std::pow(m.vol[i], 1.0 / m.dim), 3.0);
         const double sk = m.skew[i];
         cflSkew[slotOf[i]] = (sk < 0.1) ? 0.1 : ((sk < 10.0) ? (1.0 - std::exp(0.0 - sk)) : 1.0);
     }
@@ -1112,7 +1144,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     if ((int)c->caseParams.size() != nCases) return fail(PHL_ERR_ARG, "one parameter block per case is required");
     c->hostCases.resize(nCases);
     for (int i = 0; i < nCases; ++i)
-        c->hostCases[i] = case_consts(c->caseParams[i], nCases > 1 ? m.caseAvgVol[i] : m.avgVol);
+        c->hostCases[i] = case_consts(c->caseParams[i], nCases > 1 ? m.caseAvgVol[i] : m.avgVol, m.dim);
     const phl::CaseConsts &c0 = c->hostCases[0];
     k.gamma = p.gamma;
     k.R = c0.R;
@@ -1251,14 +1283,25 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         HIPTRY(hipMemset(fl.dqT, 0, sizeof(double) * nT));
         HIPTRY(hipMemset(fl.dfluxT, 0, sizeof(double) * nC));
         HIPTRY(hipMemset(fl.gradNut, 0, sizeof(double) * 3 * nC));
-        // wall distance: exact point-to-surface distance over the wall faces,
-        // each split into (centre, node i, node i+1) triangles
+        // wall distance: in 3-D the exact point-to-surface distance over the
+        // wall faces, each split into (centre, node i, node i+1) triangles; in
+        // 2-D the reference's bisection over each wall edge taken both ways
+        // (start, centre, end), edges of squared length <= 1e-16 skipped
+        // (Pre_WalldistCompute.cpp:437-474)
         std::vector<double> tri;
         for (int i = 0; i < dm.nWall; ++i) {
             const int f = wall[i];
             const int b = m.faceNodePtr[f], n = m.faceNodePtr[f + 1] - b;
             for (int j = 0; j < n; ++j) {
                 const int n1 = m.faceNodes[b + j], n2 = m.faceNodes[b + (j + 1) % n];
+                if (m.dim == 2) {
+                    const double dx = m.x[n2] - m.x[n1], dy = m.y[n2] - m.y[n1], dz = m.z[n2] - m.z[n1];
+                    if (dx * dx + dy * dy + dz * dz <= 1.0e-16) continue;
+                    const double e9[9] = {m.x[n1], m.y[n1], m.z[n1], m.xfc[f], m.yfc[f], m.zfc[f],
+                                          m.x[n2], m.y[n2], m.z[n2]};
+                    tri.insert(tri.end(), e9, e9 + 9);
+                    continue;
+                }
                 const double t9[9] = {m.xfc[f], m.yfc[f], m.zfc[f], m.x[n1], m.y[n1], m.z[n1],
                                       m.x[n2], m.y[n2], m.z[n2]};
                 tri.insert(tri.end(), t9, t9 + 9);
@@ -1270,7 +1313,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         TRY(upload(c, &dTri, tri));
         double *wd = nullptr;
         TRY(dalloc(c, &wd, (size_t)nC));
-        phl::launch_wall_distance(dm.xcc, dm.ycc, dm.zcc, nC, dTri, nTri, wd, c->stream);
+        phl::launch_wall_distance(dm.xcc, dm.ycc, dm.zcc, nC, dTri, nTri, m.dim == 2, wd, c->stream);
         HIPTRY(hipGetLastError());
         HIPTRY(hipStreamSynchronize(c->stream));
         dm.wallDist = wd;
@@ -1304,6 +1347,7 @@ phl_status phl_create(const phl_mesh *pm, const phl_params *p, int device, phl_c
     if (!pm || !p || !out) return fail(PHL_ERR_ARG, "phl_create: null argument");
     if (pm->m.nCases() > 1) return fail(PHL_ERR_ARG, "phl_create: batched mesh, use phl_create_batch");
     TRY(check_params(*p));
+    TRY(check_mesh_params(pm->m, *p));
     int nDev = 0;
     HIPTRY(hipGetDeviceCount(&nDev));
     if (device < 0 || device >= nDev) return fail(PHL_ERR_ARG, "phl_create: no such device");
@@ -1329,6 +1373,7 @@ phl_status phl_create_batch(const phl_mesh *pm, const phl_params *params, int de
     // the cases share every option; only the freestream (Mach, angles) varies
     for (int i = 0; i < n; ++i) {
         TRY(check_params(params[i]));
+        TRY(check_mesh_params(pm->m, params[i]));
         if (params[i].viscous == 3)
             return fail(PHL_ERR_UNSUPPORTED, "phl_create_batch: the Spalart-Allmaras model runs on single-case contexts");
         phl_params a = params[0], b = params[i];

@@ -120,7 +120,7 @@ struct Consts {
     int coefF32;            // mixed form: LU-SGS face coefficients stored in FP32 (sweep arithmetic stays FP64)
     int coefInSweep;        // dataflow sweeps evaluate the face coefficients themselves (none stored)
     int gradient;           // 0 Green-Gauss cell, 1 least squares, 2 Green-Gauss node
-    int sweepForm;          // 0, 2: dataflow sweep (launch order, ticket order); 1: one launch per level
+    int sweepForm;          // 0, 2, 3: dataflow sweep (launch order, ticket order, XCD groups); 1: per level
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     const CaseConsts *cases;
 };
@@ -208,6 +208,12 @@ struct FlowOrder {
     const int *depPtr;  // [nChunks+1]
     const int *dep;     // the distinct earlier chunks holding a chunk's real neighbours
     const int *width;   // [nChunks] most active slots of any cell of the chunk (slots are packed first)
+    // sweep_form 3: block i sweeps chunk blockChunk[i] (-1: nothing).  Each
+    // level's chunks are split into 8 contiguous groups, group x going to the
+    // blocks i = x (mod 8) of the level's block range, so that the k-th XCD
+    // sweeps the k-th eighth (in storage order) of every level.
+    const int *blockChunk;
+    int nBlocks;
 };
 
 // The scalar Spalart-Allmaras sweep carried along by a dataflow NS sweep.
@@ -269,7 +275,7 @@ void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *
 // Spalart-Allmaras (phl_sa.hip).  tri: [nTri][9] wall triangles (face centre
 // and two consecutive face nodes).
 void launch_wall_distance(const double *xcc, const double *ycc, const double *zcc, int nCell, const double *tri,
-                          int nTri, double *wd, hipStream_t s);
+                          int nTri, bool edges, double *wd, hipStream_t s);
 void launch_sa_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sa_viscosity(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sa_gradient(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);

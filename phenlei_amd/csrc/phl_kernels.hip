@@ -2033,7 +2033,9 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     return true;
 }
 
-// TICKET false (sweep_form 0): block b sweeps chunk b.  Workgroups of a
+// TICKET false (sweep_form 0): block b sweeps chunk b; (sweep_form 3) block b
+// sweeps chunk blockChunk[b], the levels' chunks split into eighths over the
+// XCDs (block ids still rise with the level).  Workgroups of a
 // launch are dispatched to each XCD in increasing order, so when block b is
 // resident every smaller block of its XCD has been dispatched; the smallest
 // unfinished chunk then always has its dependencies done and proceeds.
@@ -2059,6 +2061,9 @@ __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Con
         __syncthreads();
         tk = __builtin_amdgcn_readfirstlane(ticket * 4 + ((int)threadIdx.x >> 6));
         if (tk >= fo.nChunks) return;
+    } else if (K.sweepForm == 3) {
+        tk = fo.blockChunk[blockIdx.x];
+        if (tk < 0) return;
     } else {
         tk = blockIdx.x;
     }
@@ -2566,7 +2571,8 @@ static void launch_sweep_so(const DevMesh &m, const Consts &k, DevFlow &f, const
                                k, lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa,
                                f.rec, f.vist);
         else
-            hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, false>), dim3(lv.flow.nChunks), dim3(64), 0, s, m, k,
+            hipLaunchKernelGGL((k_sweep_flow<NW, VIS, CT, false>),
+                               dim3(k.sweepForm == 3 ? lv.flow.nBlocks : lv.flow.nChunks), dim3(64), 0, s, m, k,
                                lv.flow, f.sweepCtl, f.sweepFlag, useU, f.coef, f.res, f.diag, f.dq, f.dflux, sa,
                                f.rec, f.vist);
         return;

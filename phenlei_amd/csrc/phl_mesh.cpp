@@ -24,7 +24,73 @@ const double kPi = 3.141592653589793;
 inline double dmin(double a, double b) { return (a < b) ? a : b; }
 }  // namespace
 
+// Planar metrics (Geo_UnstructGrid.cpp: ComputeFaceNormal2D :6158-6187,
+// ComputeFaceCenter2D :6370-6390, ComputeCellCenterVol2D :7058-7190): the
+// edge p1->p2 has the normal (y2 - y1, x1 - x2)/length, its midpoint as
+// centre and its length as area; cell centroid and area from the divergence
+// theorem over the edges (boundary faces first), ghosts mirrored as in 3-D.
+void Mesh::computeMetrics2D()
+{
+    const int nT = nTotal();
+    xfc.assign(nFace, 0.0); yfc.assign(nFace, 0.0); zfc.assign(nFace, 0.0);
+    xfn.assign(nFace, 0.0); yfn.assign(nFace, 0.0); zfn.assign(nFace, 0.0);
+    area.assign(nFace, 0.0);
+    xcc.assign(nT, 0.0); ycc.assign(nT, 0.0); zcc.assign(nT, 0.0); vol.assign(nT, 0.0);
+    skew.assign(nCell, 0.0);
+    for (int f = 0; f < nFace; ++f) {
+        const int p1 = faceNodes[faceNodePtr[f]], p2 = faceNodes[faceNodePtr[f] + 1];
+        xfn[f] = y[p2] - y[p1];
+        yfn[f] = x[p1] - x[p2];
+        zfn[f] = 0.0;
+        area[f] = std::sqrt(xfn[f] * xfn[f] + yfn[f] * yfn[f] + zfn[f] * zfn[f]);
+        const double oa = 1.0 / (area[f] + kSmall);
+        xfn[f] *= oa; yfn[f] *= oa; zfn[f] *= oa;
+    }
+    for (int f = 0; f < nFace; ++f) {
+        const int p1 = faceNodes[faceNodePtr[f]], p2 = faceNodes[faceNodePtr[f] + 1];
+        xfc[f] = 0.5 * (x[p1] + x[p2]);
+        yfc[f] = 0.5 * (y[p1] + y[p2]);
+        zfc[f] = 0.5 * (z[p1] + z[p2]);
+    }
+    for (int f = 0; f < nFace; ++f) {
+        const int le = left[f];
+        const double dot = (xfc[f] * xfn[f] + yfc[f] * yfn[f] + zfc[f] * zfn[f]) * area[f];
+        xcc[le] += xfc[f] * dot; ycc[le] += yfc[f] * dot; zcc[le] += zfc[f] * dot; vol[le] += dot;
+        if (f >= nBFace) {
+            const int re = right[f];
+            xcc[re] -= xfc[f] * dot; ycc[re] -= yfc[f] * dot; zcc[re] -= zfc[f] * dot; vol[re] -= dot;
+        }
+    }
+    for (int c = 0; c < nCell; ++c) {
+        const double tmp = 1.0 / (1.5 * vol[c] + kSmall);
+        xcc[c] *= tmp; ycc[c] *= tmp; zcc[c] *= tmp;
+        vol[c] *= 0.5;
+    }
+    for (int f = 0; f < nBFace; ++f) {
+        const int le = left[f], re = nCell + f;
+        if (area[f] > kSmall) {
+            const double tmp = 2.0 * ((xcc[le] - xfc[f]) * xfn[f] + (ycc[le] - yfc[f]) * yfn[f] +
+                                      (zcc[le] - zfc[f]) * zfn[f]);
+            xcc[re] = xcc[le] - xfn[f] * tmp;
+            ycc[re] = ycc[le] - yfn[f] * tmp;
+            zcc[re] = zcc[le] - zfn[f] * tmp;
+        } else {
+            xcc[re] = -xcc[le] + 2.0 * xfc[f];
+            ycc[re] = -ycc[le] + 2.0 * yfc[f];
+            zcc[re] = -zcc[le] + 2.0 * zfc[f];
+        }
+        vol[re] = vol[le];
+    }
+}
+
 void Mesh::computeMetrics()
+{
+    if (dim == 2) computeMetrics2D();
+    else computeMetrics3D();
+    computeSkewness();
+}
+
+void Mesh::computeMetrics3D()
 {
     const int nT = nTotal();
     xfc.assign(nFace, 0.0); yfc.assign(nFace, 0.0); zfc.assign(nFace, 0.0);
@@ -158,10 +224,14 @@ void Mesh::computeMetrics()
         }
         vol[re] = vol[le];
     }
+}
 
-    // Skewness: minimum complement angle between the face normal and the
-    // face-to-centroid lines, interior faces first, then boundary faces.
-    for (int c = 0; c < nCell; ++c) skew[c] = 180.0;
+// Skewness: minimum complement angle between the face normal and the
+// face-to-centroid lines, interior faces first, then boundary faces; and the
+// mean volume.
+void Mesh::computeSkewness()
+{
+    skew.assign(nCell, 180.0);
     for (int f = nBFace; f < nFace; ++f) {
         if (area[f] <= kTiny) continue;
         const int le = left[f], re = right[f];
@@ -770,6 +840,7 @@ bool renumberCells(const Mesh &in, const std::vector<int> &perm, Mesh &out, std:
     });
     Mesh m;
     m.nNode = in.nNode; m.nCell = nC; m.nBFace = in.nBFace; m.nFace = in.nFace;
+    m.dim = in.dim;
     m.x = in.x; m.y = in.y; m.z = in.z;
     m.cellNodePtr.assign(1, 0);
     for (int i = 0; i < nC; ++i) {
@@ -832,10 +903,12 @@ bool concatMeshes(const std::vector<const Mesh *> &parts, Mesh &out, std::string
     long nNode = 0, nCell = 0, nFace = 0, nB = 0;
     for (const Mesh *p : parts) {
         if (!p->caseCellPtr.empty()) { err = "cannot concatenate batched meshes"; return false; }
+        if (p->dim != parts[0]->dim) { err = "cannot concatenate 2-D and 3-D meshes"; return false; }
         nNode += p->nNode; nCell += p->nCell; nFace += p->nFace; nB += p->nBFace;
     }
     if (nCell + nB > 2147483000L || parts.size() > 65535) { err = "batch too large"; return false; }
     m.nNode = (int)nNode; m.nCell = (int)nCell; m.nFace = (int)nFace; m.nBFace = (int)nB;
+    m.dim = parts[0]->dim;
     const int nT = m.nCell + m.nBFace;
     m.left.resize(nFace); m.right.resize(nFace); m.bcType.resize(nB);
     m.faceNodePtr.assign(nFace + 1, 0);

@@ -30,9 +30,18 @@ struct Mesh {
     std::vector<int> caseCellPtr, caseBFacePtr;
     std::vector<double> caseAvgVol;
 
+    // 2 for a planar grid (every face an edge of two nodes, cells polygons,
+    // z = 0): metrics by the reference's 2-D routines
+    // (UnstructGrid::ComputeMetrics, Geo_UnstructGrid.cpp:5695-5709), and the
+    // Venkatakrishnan scale takes square roots of areas.
+    int dim = 3;
+
     int nTotal() const { return nCell + nBFace; }
     int nCases() const { return caseCellPtr.empty() ? 1 : (int)caseCellPtr.size() - 1; }
-    void computeMetrics();
+    void computeMetrics();  // by dim, then skewness and mean volume
+    void computeMetrics2D();
+    void computeMetrics3D();
+    void computeSkewness();
 };
 
 // Disjoint union of meshes, one case each: cells and nodes concatenated in

@@ -85,9 +85,44 @@ __device__ double tri_dist2(const double *p, const double *a, const double *b, c
     return dx * dx + dy * dy + dz * dz;
 }
 
-// One thread per cell; the wall triangles (face centre and two consecutive
-// nodes, 9 doubles each) are staged through LDS in tiles.
+// Squared distance from p to a wall edge of a 2-D grid by the reference's
+// bisection (Pre_WalldistCompute::ComputeDistanceOfANodeToSegment,
+// API/PreProcess/src/Pre_WalldistCompute.cpp:4260-4352): the nearer end
+// point if it beats the edge's centre; otherwise 30 halvings of the interval
+// around the current nearest point.  e holds start, centre and end.
+__device__ double seg_dist2(const double *p, const double *e)
+{
+    double lo[3] = {e[0], e[1], e[2]}, mid[3] = {e[3], e[4], e[5]}, hi[3] = {e[6], e[7], e[8]};
+    auto d2 = [&](const double *q) {
+        const double dx = p[0] - q[0], dy = p[1] - q[1], dz = p[2] - q[2];
+        return dx * dx + dy * dy + dz * dz;
+    };
+    const double dMid = d2(mid), dStart = d2(lo), dEnd = d2(hi);
+    if (dStart < dMid) return dStart;
+    if (dEnd < dMid) return dEnd;
+    double dist = dMid;
+    for (int it = 0; it < 30; ++it) {
+        double a[3], b[3];
+        for (int i = 0; i < 3; ++i) { a[i] = (lo[i] + mid[i]) * 0.5; b[i] = (hi[i] + mid[i]) * 0.5; }
+        const double d1 = d2(a), dd2 = d2(b);
+        if (dist < d1 && dist < dd2) {
+            for (int i = 0; i < 3; ++i) { lo[i] = a[i]; hi[i] = b[i]; }
+        } else if (d1 < dd2) {
+            for (int i = 0; i < 3; ++i) { hi[i] = mid[i]; mid[i] = a[i]; }
+            dist = d1;
+        } else {
+            for (int i = 0; i < 3; ++i) { lo[i] = mid[i]; mid[i] = b[i]; }
+            dist = dd2;
+        }
+    }
+    return dist;
+}
+
+// One thread per cell; the wall elements (9 doubles each: a triangle as face
+// centre and two consecutive nodes, or a 2-D edge as start, centre, end) are
+// staged through LDS in tiles.
 constexpr int kTriTile = 256;
+template <bool SEG>
 __global__ void __launch_bounds__(kBlock) k_wall_distance(const double *__restrict__ xcc,
                                                           const double *__restrict__ ycc,
                                                           const double *__restrict__ zcc, int nCell,
@@ -108,7 +143,7 @@ __global__ void __launch_bounds__(kBlock) k_wall_distance(const double *__restri
         __syncthreads();
         const int n = (nTri - t0 < kTriTile) ? nTri - t0 : kTriTile;
         for (int i = 0; i < n; ++i) {
-            const double d2 = tri_dist2(p, &t[i][0], &t[i][3], &t[i][6]);
+            const double d2 = SEG ? seg_dist2(p, &t[i][0]) : tri_dist2(p, &t[i][0], &t[i][3], &t[i][6]);
             if (d2 < best) best = d2;
         }
     }
@@ -628,10 +663,14 @@ void sa_sweep_dispatch(const DevMesh &m, DevFlow &f, const Levels &lv, bool useU
 }  // namespace
 
 void launch_wall_distance(const double *xcc, const double *ycc, const double *zcc, int nCell, const double *tri,
-                          int nTri, double *wd, hipStream_t s)
+                          int nTri, bool edges, double *wd, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_wall_distance, dim3(nblk(nCell, kBlock)), dim3(kBlock), 0, s, xcc, ycc, zcc, nCell, tri,
-                       nTri, wd);
+    if (edges)
+        hipLaunchKernelGGL(k_wall_distance<true>, dim3(nblk(nCell, kBlock)), dim3(kBlock), 0, s, xcc, ycc, zcc, nCell,
+                           tri, nTri, wd);
+    else
+        hipLaunchKernelGGL(k_wall_distance<false>, dim3(nblk(nCell, kBlock)), dim3(kBlock), 0, s, xcc, ycc, zcc, nCell,
+                           tri, nTri, wd);
 }
 
 void launch_sa_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)

@@ -78,7 +78,7 @@ def test_unsupported_options_return_a_status(override):
     assert e.value.status == pa.PHL_ERR_UNSUPPORTED
 
 
-@pytest.mark.parametrize("override", [dict(sweep_form=-1), dict(sweep_form=3)])
+@pytest.mark.parametrize("override", [dict(sweep_form=-1), dict(sweep_form=4)])
 def test_bad_sweep_form_is_an_argument_error(override):
     m = pa.Mesh.bump_hex(4, 2, 2, 0.05)
     with pytest.raises(pa.PhlError) as e:

@@ -222,7 +222,7 @@ def test_nan_state_is_a_numeric_error_and_the_context_recovers():
     assert np.array_equal(s.get_state(), ref.get_state())
 
 
-@pytest.mark.parametrize("form", [0, 2])
+@pytest.mark.parametrize("form", [0, 2, 3])
 def test_sweep_wait_bound_gives_up_and_the_context_recovers(form):
     """With the dataflow sweep's wait bound set to 0 from the host, every
     wavefront that would wait gives up at once (every kernel still exits);

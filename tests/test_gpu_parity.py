@@ -398,10 +398,12 @@ def test_face_loop_forms_agree(viscous):
 
 
 @pytest.mark.parametrize("kind,precision,form", [("hex", 0, 0), ("hybrid", 0, 0), ("hybrid", 1, 0), ("wing", 0, 0),
-                                                 ("hybrid", 0, 2), ("wing", 1, 2)])
+                                                 ("hybrid", 0, 2), ("wing", 1, 2), ("hex", 0, 3), ("hybrid", 1, 3),
+                                                 ("wing", 0, 3)])
 def test_dataflow_sweep_equals_level_sweep(kind, precision, form):
     """The dataflow sweeps (one launch per direction, per-chunk flags; chunks
-    in launch order, form 0, or ticket order, form 2) and the level-by-level
+    in launch order, form 0, ticket order, form 2, or split over the XCDs,
+    form 3) and the level-by-level
     launches (form 1) apply the same arithmetic in the same order: states,
     residual histories and dq are bitwise equal, over several calls (the
     flags carry a new epoch each sweep) and in both launch modes."""
@@ -431,12 +433,13 @@ def test_dataflow_sweep_batch_equals_level_sweep():
     parts = [pa.Mesh.bump_hex(24 + 8 * i, 8, 4, 0.08) for i in range(3)]
     u = pa.Mesh.concat(parts)
     out = []
-    for form in (0, 1):
+    for form in (0, 1, 3):
         s = pa.Solver(u, [pa.default_params(cfl_start=5.0, cfl_end=5.0, sweep_form=form) for _ in parts])
         s.init_freestream()
         out.append((s.iterate(1, 12), s.get_state()))
-    assert np.array_equal(out[0][0], out[1][0])
-    assert np.array_equal(out[0][1], out[1][1])
+    for k in (1, 2):
+        assert np.array_equal(out[0][0], out[k][0])
+        assert np.array_equal(out[0][1], out[k][1])
 
 
 @pytest.mark.parametrize("kind,viscous", [("hex", 0), ("hybrid", 0), ("hybrid", 1)])
@@ -483,7 +486,8 @@ def test_least_squares_gradient_parity(kind, viscous):
 
 
 @pytest.mark.parametrize("dims,viscous,form", [((1, 1, 1), 0, 0), ((2, 1, 1), 1, 0), ((1, 2, 3), 0, 0),
-                                               ((1, 1, 1), 0, 1), ((1, 2, 3), 1, 2)])
+                                               ((1, 1, 1), 0, 1), ((1, 2, 3), 1, 2), ((1, 1, 1), 0, 3),
+                                               ((1, 2, 3), 1, 3)])
 def test_tiny_meshes_match_oracle(dims, viscous, form):
     """Edge sizes: one cell with only boundary faces (no interior face, one
     level, one chunk), two and six cells; every sweep form; against the

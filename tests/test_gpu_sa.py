@@ -178,15 +178,16 @@ def test_sa_mixed_precision_tracks_fp64():
     assert err_n <= 1e-3
 
 
-@pytest.mark.parametrize("kind,mode", [("hybrid", "graph"), ("wing", "eager_overlap"), ("wing", "eager")])
-def test_sa_dataflow_sweeps_equal_level_sweeps(kind, mode):
+@pytest.mark.parametrize("kind,mode,form", [("hybrid", "graph", 0), ("wing", "eager_overlap", 0), ("wing", "eager", 0),
+                                            ("wing", "graph", 3)])
+def test_sa_dataflow_sweeps_equal_level_sweeps(kind, mode, form):
     """NS sweeps in the dataflow form (the default) against one launch per
     level, with the SA model: bitwise equal states, nu~ and mu_t, also with
     the (level-launched) SA sweeps beside the next NS step's dataflow sweeps
     on the second stream (graph and eager_overlap)."""
     m = mesh_of(kind)
     base = dict(viscous=SA, reynolds=1e5, cfl_start=5.0, cfl_end=5.0)
-    a = pa.Solver(m, pa.default_params(sweep_form=0, **base))
+    a = pa.Solver(m, pa.default_params(sweep_form=form, **base))
     b = pa.Solver(m, pa.default_params(sweep_form=1, **base))
     a.set_launch_mode(mode)
     b.set_launch_mode("eager")
