@@ -88,6 +88,9 @@ typedef struct phl_params {
                              * (storage order), the k-th eighth on the blocks of the k-th XCD.
                              * The dataflow waits are bounded (0.2 s, phl_set_sweep_wait): a wait that
                              * runs out makes phl_iterate/phl_lusgs fail with PHL_ERR_NUMERIC. */
+    int flow_init_step;     /* flowInitStep: outer iterations 1..flow_init_step run first order with
+                             * the Van Leer flux (isInIniting; Simulation.cpp:1190-1236, flowInitMethod 0).
+                             * The reference's default is 100; phl_params_default sets 0. */
 } phl_params;
 
 /* Field identifiers for phl_get_field (same numbering as the oracle). */
@@ -318,8 +321,24 @@ phl_status phl_spectrum_radius(phl_ctx *ctx);
  * (:6001,5705,5719); step is the outer iteration number (CFL ramp). */
 phl_status phl_time_step(phl_ctx *ctx, int step);
 /* Controller::LUSGS (CFD/Solver/src/Controller.cpp:1878): Diagonal, sweeps,
- * CallGPUUpdateFlowField.  n_fixed receives the SolutionFix count. */
+ * CallGPUUpdateFlowField.  n_fixed receives the SolutionFix count.  With
+ * n_lusgs_sweeps > 1 the sweep pairs repeat from dq = 0 until a pair's
+ * squared change of dq falls below lusgs_tol times the first pair's
+ * (:1898-1951); the stream is synchronised after each pair, and phl_iterate
+ * then launches its steps eagerly instead of replaying graphs. */
 phl_status phl_lusgs(phl_ctx *ctx, int *n_fixed);
+/* The sweep pairs the last NS (turb 0) or turbulence (turb 1) LU-SGS call
+ * ran when n_lusgs_sweeps > 1 (the reference's sweepNormal of each pair, the
+ * first 8 in normals8); *n_done is 0 for single-pair contexts.  No reference
+ * counterpart: the reference keeps sweepNormal local to Controller::LUSGS. */
+phl_status phl_sweep_normals(const phl_ctx *ctx, int turb, int *n_done, double *normals8);
+/* The reference's isInIniting flag (GlobalDataBase "isInIniting", set for
+ * the outer iterations 1..flowInitStep by Simulation::PreSolve,
+ * Main/src/Simulation.cpp:1190-1236) for hook-by-hook callers: while on, the
+ * face passes use first-order face values and the Van Leer flux
+ * (NSSolverUnstruct.cpp:6923,14017,14229-14236) and the limiter is zero
+ * (:18203).  phl_iterate sets it per step from flow_init_step itself. */
+phl_status phl_set_init_phase(phl_ctx *ctx, int on);
 /* ComputeResidualonGrid sums (API/Physics/src/Residual.cpp:534) of the
  * residual currently in res: sum5[m] = sum over real cells of res_m^2 (case 0
  * of a batched context).  Deterministic (fixed blocks, fixed order). */

@@ -63,6 +63,8 @@ def load(omp=False):
         lib.orc_time_step.argtypes = [_P, ctypes.c_int]
         lib.orc_lusgs.argtypes = [_P, _D]
         lib.orc_sweep_normals.argtypes = [_P, ctypes.c_int, _D]
+        lib.orc_set_init_phase.argtypes = [_P, ctypes.c_int]
+        lib.orc_vanleer_flux.argtypes = [_D, _D] + [ctypes.c_double] * 4 + [_D]
         lib.orc_sweep_normals.restype = ctypes.c_int
         lib.orc_lusgs.restype = ctypes.c_int
         lib.orc_residual_norm.argtypes = [_P, _D]
@@ -133,6 +135,14 @@ def cell_metrics(conn, fm):
                      *[_dp(fm[k]) for k in ("xfc", "yfc", "zfc", "xfn", "yfn", "zfn", "area")],
                      *[_dp(out[k]) for k in ("xcc", "ycc", "zcc")], _dp(skew))
     out["skew"] = skew
+    return out
+
+
+def vanleer_flux(qL, qR, n, gamma=1.4):
+    qL = np.ascontiguousarray(qL, dtype=np.float64)
+    qR = np.ascontiguousarray(qR, dtype=np.float64)
+    out = np.empty(5)
+    load().orc_vanleer_flux(_dp(qL), _dp(qR), float(n[0]), float(n[1]), float(n[2]), gamma, _dp(out))
     return out
 
 

@@ -89,6 +89,7 @@ struct orc_solver {
     double *xcc, *ycc, *zcc, *vol, *skew;
     double avgVol;
     int dim;  /* 3, or 2 for a 2-D grid (orc_set_dim) */
+    int initPhase;  /* isInIniting: first-order faces and the Van Leer flux (orc_set_init_phase) */
     double nsSweepNormals[8], saSweepNormals[8];  /* squared changes of each sweep pair (first 8) */
     int nsSweepsDone, saSweepsDone;
     int *c2fPtr, *c2f;
@@ -1178,7 +1179,9 @@ void orc_limiter(orc_solver *s)
         return;
     }
     for (int c = 0; c < nT; ++c) s->lim[c] = 0.0;
-    if (s->p.limiter == 2) return; /* first order: the limiter array is unused */
+    /* first order (uns_limiter_name "1st", or isInIniting): the limiter
+     * array stays zero and is unused (ComputeLimiter :18195-18225) */
+    if (s->p.limiter == 2 || s->initPhase) return;
     for (int c = 0; c < nT; ++c) s->lim[c] = 1.0; /* InitLimitData */
     vencat7(s, s->lim, s->q[IR], s->dqdx[IR], s->dqdy[IR], s->dqdz[IR]);
     vencat7(s, s->lim, s->q[IP], s->dqdx[IP], s->dqdy[IP], s->dqdz[IP]);
@@ -1286,6 +1289,72 @@ void orc_roe_flux(const double *qLin, const double *qRin, double xsn, double ysn
 }
 
 /* ------------------------------------------------------------------ */
+/* Van Leer flux-vector splitting (Vanleer_Scheme, API/Physics/src/     */
+/* Flux_Inviscid.cpp:12697-12949): perfect gas, no preconditioning,     */
+/* static grid (face velocity 0), one face, flux per unit area.  It is  */
+/* the flux of the flowInitStep phase (ComputeInviscidFlux :14229-14236) */
+/* ------------------------------------------------------------------ */
+void orc_vanleer_flux(const double *qL, const double *qR, double xsn, double ysn, double zsn, double gama,
+                      double *flux)
+{
+    const double vb = 0.0, gmL = gama, gmR = gama;
+    double rL = qL[IR], uL = qL[IU], vL = qL[IV], wL = qL[IW], pL = qL[IP];
+    double rR = qR[IR], uR = qR[IU], vR = qR[IV], wR = qR[IW], pR = qR[IP];
+    double v2L = uL * uL + vL * vL + wL * wL;
+    double v2R = uR * uR + vR * vR + wR * wR;
+    double hintL = (gmL / (gmL - 1.0)) * (pL / rL);
+    double hintR = (gmR / (gmR - 1.0)) * (pR / rR);
+    double HL = hintL + 0.5 * v2L;
+    double HR = hintR + 0.5 * v2R;
+    double vnL = xsn * uL + ysn * vL + zsn * wL - vb;
+    double vnR = xsn * uR + ysn * vR + zsn * wR - vb;
+    double rhoVnL = rL * vnL, rhoVnR = rR * vnR;
+    double cL = sqrt(ABS(gmL * pL / rL)), cR = sqrt(ABS(gmR * pR / rR));
+    double machL = vnL / cL, machR = vnR / cR;
+    double fL[5], fR[5];
+    if (machL > 1.0) {
+        fL[IR] = rhoVnL;
+        fL[IU] = rhoVnL * uL + xsn * pL;
+        fL[IV] = rhoVnL * vL + ysn * pL;
+        fL[IW] = rhoVnL * wL + zsn * pL;
+        fL[IP] = rhoVnL * HL + vb * pL;
+    } else if (machL < -1.0) {
+        for (int m = 0; m < 5; ++m) fL[m] = 0.0;
+    } else {
+        double fmass = 0.25 * rL * cL * (machL + 1.0) * (machL + 1.0);
+        double tmp = (-vnL + 2.0 * cL) / gmL;
+        fL[IR] = fmass;
+        fL[IU] = fmass * (uL + xsn * tmp);
+        fL[IV] = fmass * (vL + ysn * tmp);
+        fL[IW] = fmass * (wL + zsn * tmp);
+        fL[IP] = fmass * (HL + vb * tmp);
+    }
+    if (machR < -1.0) {
+        fR[IR] = rhoVnR;
+        fR[IU] = rhoVnR * uR + xsn * pR;
+        fR[IV] = rhoVnR * vR + ysn * pR;
+        fR[IW] = rhoVnR * wR + zsn * pR;
+        fR[IP] = rhoVnR * HR + vb * pR;
+    } else if (machR > 1.0) {
+        for (int m = 0; m < 5; ++m) fR[m] = 0.0;
+    } else {
+        double fmass = -0.25 * rR * cR * (machR - 1.0) * (machR - 1.0);
+        double tmp = (-vnR - 2.0 * cR) / gmR;
+        fR[IR] = fmass;
+        fR[IU] = fmass * (uR + xsn * tmp);
+        fR[IV] = fmass * (vR + ysn * tmp);
+        fR[IW] = fmass * (wR + zsn * tmp);
+        fR[IP] = fmass * (HR + vb * tmp);
+    }
+    for (int m = 0; m < 5; ++m) flux[m] = fL[m] + fR[m];
+}
+
+/* isInIniting of the iterations 1..flowInitStep (Simulation::PreSolve,
+ * Main/src/Simulation.cpp:1190-1236): first-order faces, the Van Leer flux
+ * and a zero limiter array. */
+void orc_set_init_phase(orc_solver *s, int on) { s->initPhase = on ? 1 : 0; }
+
+/* ------------------------------------------------------------------ */
 /* Inviscid flux: NSSolverUnstruct::InviscidFlux (:6912-6971)          */
 /* GetQlQr (:5507), ReConstructFaceValue reconmeth=1 (:14124-14189),   */
 /* BoundaryQlQrFix (:13725-13940), InviscidFluxWrap area scaling        */
@@ -1301,7 +1370,8 @@ static void face_values(const orc_solver *s, int f, double *qL, double *qR)
 {
     int le = s->left[f], re = s->right[f];
     for (int m = 0; m < 5; ++m) { qL[m] = s->q[m][le]; qR[m] = s->q[m][re]; }
-    if (s->p.limiter != 2) {
+    /* ReConstructFaceValue returns at once for first-order faces (:14005-14021) */
+    if (s->p.limiter != 2 && !s->initPhase) {
         double qt[5];
         double lmin = MIN(s->lim[le], s->lim[re]);
         double dx = s->xfc[f] - s->xcc[le], dy = s->yfc[f] - s->ycc[le], dz = s->zfc[f] - s->zcc[le];
@@ -1355,8 +1425,11 @@ void orc_inviscid_flux(orc_solver *s)
     FACE_LOOP(s, 0, s->nFace, f)
         double qL[5], qR[5], fl[5];
         face_values(s, f, qL, qR);
-        orc_roe_flux(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma,
-                     s->p.entropy_fix, s->entropyCoef, s->entropyCoef, fl);
+        if (s->initPhase)
+            orc_vanleer_flux(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma, fl);
+        else
+            orc_roe_flux(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma,
+                         s->p.entropy_fix, s->entropyCoef, s->entropyCoef, fl);
         double a = s->area[f];
         for (int m = 0; m < 5; ++m) fl[m] *= a;
         int le = s->left[f], re = s->right[f];
@@ -2035,7 +2108,9 @@ void orc_residual_norm(const orc_solver *s, double *sum5)
 int orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist)
 {
     int nNeg = 0;
+    const int hookPhase = s->initPhase;
     for (int i = 0; i < n_steps; ++i) {
+        s->initPhase = (first_step + i) <= s->p.flow_init_step;
         orc_residual(s);
         orc_spectrum(s);
         orc_time_step(s, first_step + i);
@@ -2047,6 +2122,7 @@ int orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist)
         nNeg += orc_lusgs(s, NULL);
         if (s->p.viscous == 3) orc_sa_relax(s);
     }
+    s->initPhase = hookPhase;
     return nNeg;
 }
 

@@ -52,6 +52,7 @@ typedef struct orc_params {
     int storage_order;  /* device storage order of the product; the oracle ignores it */
     int face_loop;      /* face-loop form of the product; the oracle ignores it */
     int sweep_form;     /* sweep launch form of the product; the oracle ignores it */
+    int flow_init_step; /* flowInitStep: iterations 1..flow_init_step first order with Van Leer */
 } orc_params;
 
 /* Boundary-condition codes (API/Geometry/include/Geo_SimpleBC.h:32-39). */
@@ -132,6 +133,10 @@ int  orc_lusgs(orc_solver *s, double *sweep_normals); /* returns #negative cells
 /* sweep pairs of the last NS (turb 0) or SA (turb 1) LU-SGS call; the
  * squared changes of its first 8 pairs in out8 */
 int orc_sweep_normals(const orc_solver *s, int turb, double *out8);
+/* isInIniting for the hook-by-hook calls (orc_iterate sets it per step) */
+void orc_set_init_phase(orc_solver *s, int on);
+void orc_vanleer_flux(const double *qL, const double *qR, double xsn, double ysn, double zsn, double gama,
+                      double *flux);
 void orc_residual_norm(const orc_solver *s, double *sum5);
 int  orc_iterate(orc_solver *s, int first_step, int n_steps, double *rms_hist);
 /* ---- Spalart-Allmaras (TurbSolverUnstr) ---- */

@@ -51,7 +51,7 @@ class Params(ctypes.Structure):
         ("viscous", ctypes.c_int), ("gradient", ctypes.c_int), ("limiter", ctypes.c_int),
         ("entropy_fix", ctypes.c_int), ("cfl_vary_step", ctypes.c_int),
         ("n_lusgs_sweeps", ctypes.c_int), ("precision", ctypes.c_int), ("storage_order", ctypes.c_int),
-        ("face_loop", ctypes.c_int), ("sweep_form", ctypes.c_int),
+        ("face_loop", ctypes.c_int), ("sweep_form", ctypes.c_int), ("flow_init_step", ctypes.c_int),
     ]
 
     def as_dict(self):
@@ -151,6 +151,8 @@ def _declare(lib):
         "phl_spectrum_radius": (ctypes.c_int, [_P]),
         "phl_time_step": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_lusgs": (ctypes.c_int, [_P, _I]),
+        "phl_sweep_normals": (ctypes.c_int, [_P, ctypes.c_int, _I, _D]),
+        "phl_set_init_phase": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_residual_norm": (ctypes.c_int, [_P, _D]),
         "phl_residual_norm_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_int, ctypes.c_void_p,
                                                      ctypes.c_void_p]),
@@ -543,6 +545,18 @@ class Solver:
         n = ctypes.c_int()
         _check(_lib.phl_lusgs(self._h, ctypes.byref(n)))
         return n.value
+
+    def set_init_phase(self, on):
+        """isInIniting for hook calls: first-order faces, Van Leer flux, zero limiter."""
+        _check(_lib.phl_set_init_phase(self._h, int(bool(on))))
+
+    def sweep_normals(self, turb=False):
+        """Squared dq changes of the sweep pairs of the last LU-SGS call
+        (n_lusgs_sweeps > 1; NS, or the turbulence model with turb), first 8."""
+        n = ctypes.c_int()
+        out = np.empty(8)
+        _check(_lib.phl_sweep_normals(self._h, int(bool(turb)), ctypes.byref(n), _dptr(out)))
+        return out[:min(n.value, 8)]
 
     def residual_norm(self):
         out = np.empty(5)

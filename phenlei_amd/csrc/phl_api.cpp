@@ -29,6 +29,12 @@ struct phl_ctx {
     int *hostCount = nullptr;    // pinned
     double *hostCfl = nullptr;   // pinned, CFL of each step of a chunk
     int *hostZero = nullptr;     // pinned constant 0
+    // nLUSGSSweeps > 1: the pinned half-sweep sums and the record of the last
+    // NS and SA LU-SGS calls (their steps run eagerly, decided on the host)
+    phl::SweepLog nsLog, saLog;
+    // isInIniting for hook-by-hook callers (phl_set_init_phase); phl_iterate
+    // sets it per step from flow_init_step and restores it afterwards
+    int hookInit = 0;
     double primInf[5];
     // batched contexts: one parameter block and freestream per case
     std::vector<phl_params> caseParams;
@@ -321,7 +327,8 @@ phl_status check_params(const phl_params &p)
     if (p.limiter < 0 || p.limiter > 2) return fail(PHL_ERR_UNSUPPORTED, "limiter must be 0 (vencat), 1 (nolim) or 2 (1st)");
     if (p.entropy_fix != 1 && p.entropy_fix != 3 && p.entropy_fix != 5)
         return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1, 3 or 5");
-    if (p.n_lusgs_sweeps != 1) return fail(PHL_ERR_UNSUPPORTED, "nLUSGSSweeps must be 1");
+    if (p.n_lusgs_sweeps < 1) return fail(PHL_ERR_ARG, "nLUSGSSweeps must be at least 1");
+    if (p.flow_init_step < 0) return fail(PHL_ERR_ARG, "flowInitStep must not be negative");
     if (p.precision != 0 && p.precision != 1)
         return fail(PHL_ERR_UNSUPPORTED, "precision must be 0 (FP64) or 1 (mixed)");
     if (p.sweep_form < 0 || p.sweep_form > 3)
@@ -682,6 +689,7 @@ void phl_destroy(phl_ctx *c)
     if (c->hostNorm) hipHostFree(c->hostNorm);
     if (c->hostFrozen) hipHostFree(c->hostFrozen);
     if (c->hostCount) hipHostFree(c->hostCount);
+    if (c->nsLog.host) hipHostFree(c->nsLog.host);
     if (c->hostCfl) hipHostFree(c->hostCfl);
     if (c->hostZero) hipHostFree(c->hostZero);
     for (hipEvent_t e : c->events) hipEventDestroy(e);
@@ -1135,7 +1143,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         TRY(upload(c, &lv.so.nb, nbs));
     }
     c->coefF32Ok = c->fwd.so.Wpad <= 6 && c->bwd.so.Wpad <= 6;
-    c->saFused = p.viscous == 3 && c->fwd.flow.nChunks > 0 && c->bwd.flow.nChunks > 0;
+    c->saFused = p.viscous == 3 && p.n_lusgs_sweeps == 1 && c->fwd.flow.nChunks > 0 && c->bwd.flow.nChunks > 0;
 
     // constants: the scalars of every case, the first one also in k
     phl::Consts &k = c->k;
@@ -1187,6 +1195,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.limiter = p.limiter;
     k.entropyFix = p.entropy_fix;
     k.nSweeps = p.n_lusgs_sweeps;
+    k.lusgsTol = p.lusgs_tol;
     k.mixed = p.precision;
     k.coefF32 = (k.mixed && c->coefF32Ok) ? 1 : 0;
     k.gradient = p.gradient;
@@ -1240,6 +1249,12 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     {
         const int ctl[4] = {0, 0, 0, kSweepWaitTicks};
         HIPTRY(hipMemcpy(fl.sweepCtl, ctl, sizeof(ctl), hipMemcpyHostToDevice));
+    }
+    if (p.n_lusgs_sweeps > 1) {
+        TRY(dalloc(c, &fl.dqPrev, (size_t)5 * nC));
+        TRY(dalloc(c, &fl.dqTPrev, (size_t)nC));
+        TRY(dalloc(c, &fl.sweepParts, (size_t)(nC + 255) / 256));
+        TRY(dalloc(c, &fl.sweepNorm, 2));
     }
     TRY(dalloc(c, &fl.negList, (size_t)phl::kFixList));
     TRY(dalloc(c, &fl.t, (size_t)nT));
@@ -1338,6 +1353,10 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     HIPTRY(hipHostMalloc((void **)&c->hostCfl, sizeof(double) * c->normSlots, hipHostMallocDefault));
     HIPTRY(hipHostMalloc((void **)&c->hostZero, sizeof(int), hipHostMallocDefault));
     *c->hostZero = 0;
+    if (p.n_lusgs_sweeps > 1) {
+        HIPTRY(hipHostMalloc((void **)&c->nsLog.host, 4 * sizeof(double), hipHostMallocDefault));
+        c->saLog.host = c->nsLog.host + 2;
+    }
     HIPTRY(hipDeviceSynchronize());
     return PHL_OK;
 }
@@ -1376,6 +1395,8 @@ phl_status phl_create_batch(const phl_mesh *pm, const phl_params *params, int de
         TRY(check_mesh_params(pm->m, params[i]));
         if (params[i].viscous == 3)
             return fail(PHL_ERR_UNSUPPORTED, "phl_create_batch: the Spalart-Allmaras model runs on single-case contexts");
+        if (params[i].n_lusgs_sweeps != 1)
+            return fail(PHL_ERR_UNSUPPORTED, "phl_create_batch: several LU-SGS sweep pairs run on single-case contexts");
         phl_params a = params[0], b = params[i];
         a.mach = b.mach = 0.0;
         a.alpha_deg = b.alpha_deg = 0.0;
@@ -1691,13 +1712,68 @@ phl_status phl_time_step(phl_ctx *c, int step)
     return check_launch(c);
 }
 
+// Controller::LUSGS with nLUSGSSweeps > 1 (Controller.cpp:1878-1961): dq
+// and LUplusDQ start from zero; after every sweep pair the squared change of
+// dq over both half-sweeps (the reference's sweepNormal, summed here in block
+// order) is read back, and the pairs stop once it falls below
+// LUSGSTolerance times the first pair's.  The host decides, so such steps
+// run eagerly.
+static void ns_sweep_pairs(phl_ctx *c, hipStream_t s)
+{
+    phl::DevFlow &f = c->f;
+    const int nT = c->m.nTotal, nC = c->m.nCell;
+    hipMemsetAsync(f.dq, 0, sizeof(double) * 5 * nT, s);
+    hipMemsetAsync(f.dflux, 0, sizeof(double) * 5 * nC, s);
+    hipMemsetAsync(f.dqPrev, 0, sizeof(double) * 5 * nC, s);
+    double init = 0.0, eps = 1.1;
+    c->nsLog.done = 0;
+    for (int it = 0; it < c->k.nSweeps; ++it) {
+        phl::launch_sweep(c->m, c->k, f, c->fwd, true, it == 0, s);
+        phl::launch_sweep_change(f.dq, (size_t)nT, 5, nC, f.dqPrev, f.sweepParts, f.sweepNorm, s);
+        phl::launch_ghost_dq(c->m, c->k, f, s);
+        phl::launch_sweep(c->m, c->k, f, c->bwd, false, false, s);
+        phl::launch_sweep_change(f.dq, (size_t)nT, 5, nC, f.dqPrev, f.sweepParts, f.sweepNorm + 1, s);
+        hipMemcpyAsync(c->nsLog.host, f.sweepNorm, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (hipStreamSynchronize(s) != hipSuccess) return;  // reported by the caller's launch check
+        const double sn = c->nsLog.host[0] + c->nsLog.host[1];
+        if (it < 8) c->nsLog.normals[it] = sn;
+        c->nsLog.done = it + 1;
+        if (it == 0) init = sn;
+        else eps = sn / init;
+        if (eps < c->k.lusgsTol) break;
+    }
+}
+
 static void enqueue_lusgs(phl_ctx *c)
 {
-    phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, c->stream);
-    phl::launch_ghost_dq(c->m, c->k, c->f, c->stream);
-    phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, c->stream);
+    if (c->k.nSweeps > 1) {
+        ns_sweep_pairs(c, c->stream);
+    } else {
+        phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, c->stream);
+        phl::launch_ghost_dq(c->m, c->k, c->f, c->stream);
+        phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, c->stream);
+    }
     phl::launch_update(c->m, c->k, c->f, false, c->stream);
     swap_state(c);
+}
+
+phl_status phl_sweep_normals(const phl_ctx *c, int turb, int *n_done, double *normals8)
+{
+    if (!c || !n_done) return fail(PHL_ERR_ARG, "null argument");
+    const phl::SweepLog &l = turb ? c->saLog : c->nsLog;
+    *n_done = c->k.nSweeps > 1 ? l.done : 0;
+    if (normals8)
+        for (int i = 0; i < 8; ++i) normals8[i] = l.normals[i];
+    return PHL_OK;
+}
+
+phl_status phl_set_init_phase(phl_ctx *c, int on)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    c->hookInit = on ? 1 : 0;
+    c->k.initPhase = c->hookInit;
+    ++c->stateVersion;  // the limiter a gradient call leaves depends on it
+    return PHL_OK;
 }
 
 phl_status phl_lusgs(phl_ctx *c, int *n_fixed)
@@ -1765,14 +1841,14 @@ phl_status phl_turb_spectrum(phl_ctx *c)
 phl_status phl_turb_lusgs(phl_ctx *c)
 {
     TRY(sa_ctx(c));
-    phl::launch_sa_lusgs(c->m, c->k, c->f, c->fwd, c->bwd, c->stream);
+    phl::launch_sa_lusgs(c->m, c->k, c->f, c->fwd, c->bwd, c->stream, &c->saLog);
     return check_launch(c);
 }
 
 phl_status phl_turb_relax(phl_ctx *c)
 {
     TRY(sa_ctx(c));
-    phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, c->stream);
+    phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, c->stream, &c->saLog);
     return check_launch(c);
 }
 
@@ -1916,12 +1992,16 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     // they read only the turbulence arrays, which this step's NS kernels so
     // far have not touched, and write only nu~'s increment
     const bool carry = c->saFused && c->saDeferred;
-    if (carry) phl::launch_sa_sweep_init(c->m, c->f, s);
-    phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, s, carry); mark();
-    phl::launch_ghost_dq(c->m, c->k, c->f, s);
-    if (carry) phl::launch_sa_ghost_dq(c->m, c->f, s);
-    mark();
-    phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, s, carry); mark();
+    if (c->k.nSweeps > 1) {
+        ns_sweep_pairs(c, s); mark(); mark(); mark();
+    } else {
+        if (carry) phl::launch_sa_sweep_init(c->m, c->f, s);
+        phl::launch_sweep(c->m, c->k, c->f, c->fwd, true, true, s, carry); mark();
+        phl::launch_ghost_dq(c->m, c->k, c->f, s);
+        if (carry) phl::launch_sa_ghost_dq(c->m, c->f, s);
+        mark();
+        phl::launch_sweep(c->m, c->k, c->f, c->bwd, false, false, s, carry); mark();
+    }
     phl::launch_update(c->m, c->k, c->f, true, s);
     if (carry) {
         phl::launch_sa_update(c->m, c->f, s);
@@ -1934,7 +2014,7 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
         if (c->saFused) {
             phl::launch_sa_pre(c->m, c->k, c->f, s);
             c->saDeferred = true;
-        } else if (!ev && c->launchMode != 0) {
+        } else if (!ev && c->launchMode != 0 && c->k.nSweeps == 1) {
             join_turb(c);  // the previous step's sweeps wrote nu~
             phl::launch_sa_pre(c->m, c->k, c->f, s);
             hipEventRecord(c->evFork, s);
@@ -1943,7 +2023,7 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
             hipEventRecord(c->evJoin, c->side);
             c->turbPending = true;
         } else {
-            phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, s);
+            phl::launch_sa_relax(c->m, c->k, c->f, c->fwd, c->bwd, s, &c->saLog);
         }
     }
     mark();
@@ -2088,7 +2168,15 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
             HIPTRY(hipMemcpyAsync(c->f.frozen, c->hostFrozen, (size_t)c->k.nCases, hipMemcpyHostToDevice, c->stream));
         reset_step_counter(c);
         int i = 0;
-        if (!c->timing && c->launchMode == 1) {
+        // iterations 1..flow_init_step: first order with the Van Leer flux,
+        // launched eagerly (the captured graphs hold the regular steps)
+        for (; i < chunk && first_step + done + i <= c->p.flow_init_step; ++i) {
+            c->k.initPhase = 1;
+            enqueue_step(c, c->timing ? &c->events[(size_t)i * (K + 1)] : nullptr);
+            ++c->eagerSteps;
+        }
+        c->k.initPhase = 0;
+        if (!c->timing && c->launchMode == 1 && c->k.nSweeps == 1) {
             // graphs are captured with a turbulence relaxation deferred at entry
             if (c->saFused && !c->saDeferred && i + kGraphSteps < chunk) enqueue_step(c, nullptr), ++i;
             for (; i + kGraphSteps <= chunk && c->saFused == c->saDeferred; i += kGraphSteps) {
@@ -2102,6 +2190,7 @@ phl_status phl_iterate(phl_ctx *c, int first_step, int n_steps, double *rms_hist
         c->eagerSteps += chunk - i;
         for (; i < chunk; ++i)
             enqueue_step(c, c->timing ? &c->events[(size_t)i * (K + 1)] : nullptr);
+        c->k.initPhase = c->hookInit;
         TRY(check_launch(c));
         const int nCases = c->k.nCases;
         HIPTRY(hipMemcpyAsync(c->hostNorm, c->f.normOut, sizeof(double) * 5 * chunk * nCases, hipMemcpyDeviceToHost,

@@ -121,8 +121,18 @@ struct Consts {
     int coefInSweep;        // dataflow sweeps evaluate the face coefficients themselves (none stored)
     int gradient;           // 0 Green-Gauss cell, 1 least squares, 2 Green-Gauss node
     int sweepForm;          // 0, 2, 3: dataflow sweep (launch order, ticket order, XCD groups); 1: per level
+    int initPhase;          // isInIniting: first-order faces, Van Leer flux, limiter 0 (flowInitStep)
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
+    double lusgsTol;        // LUSGSTolerance (nSweeps > 1)
     const CaseConsts *cases;
+};
+
+// Host side of an LU-SGS call with several sweep pairs: a pinned buffer for
+// the two half-sweep sums, and what the last call did.
+struct SweepLog {
+    double *host = nullptr;   // pinned [2]
+    double normals[8] = {};   // squared dq change of the first 8 pairs
+    int done = 0;             // pairs run by the last call
 };
 
 // What the face loop gathers of one cell, in one 192-byte record (two cache
@@ -183,6 +193,9 @@ struct DevFlow {
     double *dqT;      // [nTotal]
     double *dfluxT;   // [nCell]
     double *gradNut;  // [3][nCell]
+    // nLUSGSSweeps > 1, allocated only then: dq after the last half-sweep
+    // (NS [5][nCell], SA [nCell]), block partials and the two half-sweep sums
+    double *dqPrev, *dqTPrev, *sweepParts, *sweepNorm;
 };
 
 // LU-SGS level schedule.
@@ -274,6 +287,8 @@ void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *
 
 // Spalart-Allmaras (phl_sa.hip).  tri: [nTri][9] wall triangles (face centre
 // and two consecutive face nodes).
+void launch_sweep_change(const double *dq, size_t ld, int nv, int nC, double *prev, double *parts, double *out,
+                         hipStream_t s);
 void launch_wall_distance(const double *xcc, const double *ycc, const double *zcc, int nCell, const double *tri,
                           int nTri, bool edges, double *wd, hipStream_t s);
 void launch_sa_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
@@ -284,12 +299,15 @@ void launch_sa_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream
 void launch_sa_sweep_init(const DevMesh &m, DevFlow &f, hipStream_t s);
 void launch_sa_ghost_dq(const DevMesh &m, DevFlow &f, hipStream_t s);
 void launch_sa_update(const DevMesh &m, DevFlow &f, hipStream_t s);
+// One sweep pair (k.nSweeps == 1), or pairs until the squared change of a
+// pair falls below k.lusgsTol times the first's (at most k.nSweeps; the
+// stream is synchronised after each pair and `log` records the sums).
 void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
-                     hipStream_t s);
+                     hipStream_t s, SweepLog *log = nullptr);
 // Everything of a relaxation before the sweeps: boundary, eddy viscosity,
 // gradient, residual, spectrum and face matrices.  launch_sa_lusgs is the rest.
 void launch_sa_pre(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_sa_relax(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
-                     hipStream_t s);
+                     hipStream_t s, SweepLog *log = nullptr);
 
 }  // namespace phl

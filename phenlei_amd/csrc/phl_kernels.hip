@@ -70,6 +70,66 @@ __device__ __forceinline__ T harten(T eig, T lim)
     return e;
 }
 
+// Van Leer flux-vector splitting (Vanleer_Scheme, API/Physics/src/
+// Flux_Inviscid.cpp:12697-12949): perfect gas, no preconditioning, static
+// grid; the flux of the first-order initialisation iterations (flowInitStep,
+// ComputeInviscidFlux NSSolverUnstruct.cpp:14229-14236).  Per unit area.
+template <typename T>
+__device__ void vanleer_flux(const T *qL, const T *qR, T xsn, T ysn, T zsn, T gama, T *flux)
+{
+    const T vb = T(0.0), one = T(1.0);
+    const T rL = qL[0], uL = qL[1], vL = qL[2], wL = qL[3], pL = qL[4];
+    const T rR = qR[0], uR = qR[1], vR = qR[2], wR = qR[3], pR = qR[4];
+    const T v2L = uL * uL + vL * vL + wL * wL;
+    const T v2R = uR * uR + vR * vR + wR * wR;
+    const T hintL = (gama / (gama - one)) * (pL / rL);
+    const T hintR = (gama / (gama - one)) * (pR / rR);
+    const T HL = hintL + T(0.5) * v2L, HR = hintR + T(0.5) * v2R;
+    const T vnL = xsn * uL + ysn * vL + zsn * wL - vb;
+    const T vnR = xsn * uR + ysn * vR + zsn * wR - vb;
+    const T rhoVnL = rL * vnL, rhoVnR = rR * vnR;
+    const T gp = gama * pL / rL, gq = gama * pR / rR;
+    const T cL = sqrt(gp < T(0.0) ? -gp : gp), cR = sqrt(gq < T(0.0) ? -gq : gq);
+    const T machL = vnL / cL, machR = vnR / cR;
+    T fL[5], fR[5];
+    if (machL > one) {
+        fL[0] = rhoVnL;
+        fL[1] = rhoVnL * uL + xsn * pL;
+        fL[2] = rhoVnL * vL + ysn * pL;
+        fL[3] = rhoVnL * wL + zsn * pL;
+        fL[4] = rhoVnL * HL + vb * pL;
+    } else if (machL < -one) {
+        for (int m = 0; m < 5; ++m) fL[m] = T(0.0);
+    } else {
+        const T fmass = T(0.25) * rL * cL * (machL + one) * (machL + one);
+        const T tmp = (-vnL + T(2.0) * cL) / gama;
+        fL[0] = fmass;
+        fL[1] = fmass * (uL + xsn * tmp);
+        fL[2] = fmass * (vL + ysn * tmp);
+        fL[3] = fmass * (wL + zsn * tmp);
+        fL[4] = fmass * (HL + vb * tmp);
+    }
+    if (machR < -one) {
+        fR[0] = rhoVnR;
+        fR[1] = rhoVnR * uR + xsn * pR;
+        fR[2] = rhoVnR * vR + ysn * pR;
+        fR[3] = rhoVnR * wR + zsn * pR;
+        fR[4] = rhoVnR * HR + vb * pR;
+    } else if (machR > one) {
+        for (int m = 0; m < 5; ++m) fR[m] = T(0.0);
+    } else {
+        const T fmass = -T(0.25) * rR * cR * (machR - one) * (machR - one);
+        const T tmp = (-vnR - T(2.0) * cR) / gama;
+        fR[0] = fmass;
+        fR[1] = fmass * (uR + xsn * tmp);
+        fR[2] = fmass * (vR + ysn * tmp);
+        fR[3] = fmass * (wR + zsn * tmp);
+        fR[4] = fmass * (HR + vb * tmp);
+    }
+#pragma unroll
+    for (int m = 0; m < 5; ++m) flux[m] = fL[m] + fR[m];
+}
+
 // Roe flux per unit area, conservative form, static mesh, perfect gas.
 // T is double for the FP64 form and float for the mixed form; the double
 // instantiation performs exactly the operations of the reference.
@@ -567,7 +627,7 @@ __global__ void __launch_bounds__(BS) k_grad_lim(DevMesh M, Consts K0, const dou
     double l;
     if (K.limiter == 1) {
         l = 1.0;
-    } else if (K.limiter == 2) {
+    } else if (K.limiter == 2 || K.initPhase) {  // first order: not computed (ComputeLimiter :18195-18225)
         l = 0.0;
     } else {
         const double dminR = rmin - qc[0], dmaxR = rmax - qc[0];
@@ -897,7 +957,7 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
             const CellSide<T> &R = isL ? oth : own;
             if (!SPEC_ONLY) {
                 T qL[5], qR[5];
-                if (K.limiter != 2) {
+                if (K.limiter != 2 && !K.initPhase) {
                     const double lmin = dMIN(L.lim, R.lim);
                     reconstruct(L, lmin, fr, qL);
                     reconstruct(R, lmin, fr, qR);
@@ -929,7 +989,8 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                     }
                 }
                 T fl[5];
-                roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
+                if (K.initPhase) vanleer_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), fl);
+                else roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
                 const T area = T(fr.area);
 #pragma unroll
                 for (int m = 0; m < 5; ++m) fl[m] *= area;
@@ -1093,7 +1154,7 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
     const int bc = bnd ? M.bcType[f] : 0;
     if constexpr ((PARTS & 1) != 0) {
         T qL[5], qR[5];
-        if (K.limiter != 2) {
+        if (K.limiter != 2 && !K.initPhase) {
             const double lmin = dMIN(L.lim, R.lim);
             reconstruct(L, lmin, fr, qL);
             reconstruct(R, lmin, fr, qR);
@@ -1122,7 +1183,8 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
             }
         }
         T fl[5];
-        roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
+        if (K.initPhase) vanleer_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), fl);
+        else roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
         const T area = T(fr.area);
         // inviscid spectral radius term at the face-averaged state
         const double vnl = fr.nx * L.q[1] + fr.ny * L.q[2] + fr.nz * L.q[3] - 0.0;
@@ -1494,6 +1556,47 @@ __global__ void __launch_bounds__(kBlock) k_norm_sum5(const double *__restrict__
     __shared__ double red[6][kBlock];
     sum_parts(part, part, nPart, 0, red);
     if (threadIdx.x < 5) sum5[threadIdx.x] = red[threadIdx.x][0];
+}
+
+// The squared change of dq since the last call (the sweepNormal of
+// Controller::LUSGS for nLUSGSSweeps > 1, Controller.cpp:1920-1950; the
+// reference adds SQR(dq - dqOld) cell by cell in its sweeps): per block of
+// kBlock real cells, over nv rows of dq [nv][ld], and prev [nv][nC] takes the
+// new dq.  k_sum_into then adds the block partials in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_sweep_change(const double *__restrict__ dq, size_t ld, int nv, int nC,
+                                                         double *__restrict__ prev, double *__restrict__ parts)
+{
+    __shared__ double red[kBlock];
+    const int c = blockIdx.x * kBlock + (int)threadIdx.x;
+    double acc = 0.0;
+    if (c < nC)
+        for (int m = 0; m < nv; ++m) {
+            const double v = dq[(size_t)m * ld + c];
+            const double d = v - prev[(size_t)m * nC + c];
+            acc += d * d;
+            prev[(size_t)m * nC + c] = v;
+        }
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) parts[blockIdx.x] = red[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_sum_into(const double *__restrict__ parts, int n, double *__restrict__ out)
+{
+    __shared__ double red[kBlock];
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < n; i += kBlock) acc += parts[i];
+    red[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = kBlock / 2; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) *out = red[0];
 }
 
 // Batched contexts: per-case residual sums of the step and minimum local time
@@ -2631,6 +2734,14 @@ void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &l
             ++L;
         }
     }
+}
+
+void launch_sweep_change(const double *dq, size_t ld, int nv, int nC, double *prev, double *parts, double *out,
+                         hipStream_t s)
+{
+    const int nb = nblk(nC, kBlock);
+    hipLaunchKernelGGL(k_sweep_change, dim3(nb), dim3(kBlock), 0, s, dq, ld, nv, nC, prev, parts);
+    hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kBlock), 0, s, parts, nb, out);
 }
 
 void launch_ghost_dq(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)

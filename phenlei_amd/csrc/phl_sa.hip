@@ -725,13 +725,41 @@ void launch_sa_update(const DevMesh &m, DevFlow &f, hipStream_t s)
     hipLaunchKernelGGL(k_sa_update, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, f.nut, f.dqT);
 }
 
-void launch_sa_lusgs(const DevMesh &m, const Consts &, DevFlow &f, const Levels &fwd, const Levels &bwd,
-                     hipStream_t s)
+void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
+                     hipStream_t s, SweepLog *log)
 {
-    launch_sa_sweep_init(m, f, s);
-    sa_sweep_dispatch(m, f, fwd, true, s);
-    launch_sa_ghost_dq(m, f, s);
-    sa_sweep_dispatch(m, f, bwd, true, s);
+    if (k.nSweeps <= 1 || !log) {
+        launch_sa_sweep_init(m, f, s);
+        sa_sweep_dispatch(m, f, fwd, true, s);
+        launch_sa_ghost_dq(m, f, s);
+        sa_sweep_dispatch(m, f, bwd, true, s);
+        launch_sa_update(m, f, s);
+        return;
+    }
+    // Controller::LUSGS with several pairs (Controller.cpp:1878-1961): dq~
+    // and LUplusDQ start from zero; the squared change of dq~ over each pair
+    // (both half-sweeps) against the first pair's decides, on the host, after
+    // every pair
+    hipMemsetAsync(f.dqT, 0, sizeof(double) * m.nTotal, s);
+    hipMemsetAsync(f.dfluxT, 0, sizeof(double) * m.nCell, s);
+    hipMemsetAsync(f.dqTPrev, 0, sizeof(double) * m.nCell, s);
+    double init = 0.0, eps = 1.1;
+    log->done = 0;
+    for (int it = 0; it < k.nSweeps; ++it) {
+        sa_sweep_dispatch(m, f, fwd, true, s);
+        launch_sweep_change(f.dqT, m.nTotal, 1, m.nCell, f.dqTPrev, f.sweepParts, f.sweepNorm, s);
+        launch_sa_ghost_dq(m, f, s);
+        sa_sweep_dispatch(m, f, bwd, true, s);
+        launch_sweep_change(f.dqT, m.nTotal, 1, m.nCell, f.dqTPrev, f.sweepParts, f.sweepNorm + 1, s);
+        hipMemcpyAsync(log->host, f.sweepNorm, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+        if (hipStreamSynchronize(s) != hipSuccess) break;  // reported by the caller's launch check
+        const double sn = log->host[0] + log->host[1];
+        if (it < 8) log->normals[it] = sn;
+        log->done = it + 1;
+        if (it == 0) init = sn;
+        else eps = sn / init;
+        if (eps < k.lusgsTol) break;
+    }
     launch_sa_update(m, f, s);
 }
 
@@ -757,10 +785,10 @@ void launch_sa_pre(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
 }
 
 void launch_sa_relax(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &fwd, const Levels &bwd,
-                     hipStream_t s)
+                     hipStream_t s, SweepLog *log)
 {
     launch_sa_pre(m, k, f, s);
-    launch_sa_lusgs(m, k, f, fwd, bwd, s);
+    launch_sa_lusgs(m, k, f, fwd, bwd, s, log);
 }
 
 }  // namespace phl

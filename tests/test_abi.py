@@ -69,7 +69,7 @@ def test_mesh_create_rejects_bad_connectivity():
 
 @pytest.mark.parametrize("override", [dict(gradient=3), dict(gradient=1, viscous=3, reynolds=1e5),
                                       dict(gradient=2, viscous=3, reynolds=1e5),
-                                      dict(n_lusgs_sweeps=2), dict(entropy_fix=2),
+                                      dict(entropy_fix=2),
                                       dict(t_wall=300.0), dict(viscous=2), dict(precision=7)])
 def test_unsupported_options_return_a_status(override):
     m = pa.Mesh.bump_hex(4, 2, 2, 0.05)
@@ -85,6 +85,29 @@ def test_bad_sweep_form_is_an_argument_error(override):
         pa.Solver(m, pa.default_params(**override))
     assert e.value.status == pa.PHL_ERR_ARG
     assert "sweep_form" in str(e.value)
+
+
+def test_sweep_count_below_one_is_an_argument_error():
+    m = pa.Mesh.bump_hex(4, 2, 2, 0.05)
+    with pytest.raises(pa.PhlError) as e:
+        pa.Solver(m, pa.default_params(n_lusgs_sweeps=0))
+    assert e.value.status == pa.PHL_ERR_ARG
+
+
+def test_lsq_on_a_2d_grid_is_rejected():
+    import os
+    m = pa.Mesh.from_fts(os.path.join(os.path.dirname(__file__), "golden", "rae2822_hybrid2d__4_0.fts"))
+    with pytest.raises(pa.PhlError) as e:
+        pa.Solver(m, pa.default_params(gradient=1))
+    assert e.value.status == pa.PHL_ERR_UNSUPPORTED
+
+
+def test_several_sweep_pairs_on_a_batch_are_rejected():
+    a, b = pa.Mesh.bump_hex(4, 2, 2, 0.05), pa.Mesh.bump_hex(4, 2, 2, 0.03)
+    u = pa.Mesh.concat([a, b])
+    with pytest.raises(pa.PhlError) as e:
+        pa.Solver(u, [pa.default_params(n_lusgs_sweeps=3), pa.default_params(n_lusgs_sweeps=3, mach=0.6)])
+    assert e.value.status == pa.PHL_ERR_UNSUPPORTED
 
 
 def test_turbulence_batch_is_rejected_before_any_device_call():

@@ -14,6 +14,7 @@ import pytest
 
 import phenlei_amd as pa
 import oracle as orc
+from conftest import perturbed_state
 
 pytestmark = pytest.mark.gpu
 
@@ -57,10 +58,17 @@ def test_rae2822_iterations_match_oracle(rae, mach, viscous, gradient, precision
     cpu = orc.OracleSolver(c, g, p)
     if viscous == 3:
         cpu.set_wall_distance(wd)
-    gpu.init_freestream()
-    cpu.init_freestream()
     nc = m.n_cell
     if precision == 0:
+        # a seeded state: at freestream the residual is rounding noise around
+        # the cancelling face fluxes, where one-ulp pow differences of the
+        # farfield ghosts show
+        gpu.init_freestream()  # nu~ and mu_t of the SA model at their freestream values
+        cpu.init_freestream()
+        q = perturbed_state(m.n_total, nc, cpu.freestream(), 17)
+        q[3] = 0.0  # 2-D: no w
+        gpu.set_state(q)
+        cpu.set("q", q)
         gpu.residual()
         cpu.residual()
         rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
@@ -68,8 +76,8 @@ def test_rae2822_iterations_match_oracle(rae, mach, viscous, gradient, precision
         assert not np.any(rg[3]) and not np.any(rc[3])
         assert rel_err(rg[[0, 1, 2, 4]], rc[[0, 1, 2, 4]],
                        1e-3 * np.max(np.abs(rc[[0, 1, 2, 4]]), axis=1, keepdims=True)) <= TOL
-        gpu.init_freestream()
-        cpu.init_freestream()
+    gpu.init_freestream()
+    cpu.init_freestream()
     hg = gpu.iterate(1, 6)
     hc, nfix = cpu.iterate(1, 6)
     err = np.max(np.abs(hg - hc) / hc)
