@@ -62,7 +62,8 @@ typedef struct phl_params {
     int gradient;           /* gradientName: 0 "ggcell" (ghost gradient 0), 1 "lsq" (weights 1/|dx|^2,
                              * ghost takes its owner's gradient), 2 "ggnode" (the reference default,
                              * cfd_para.hypara:494: face values from inverse-distance node averages,
-                             * ghost takes its owner's gradient); lsq and ggnode: NS only, viscous 0 or 1 */
+                             * ghost takes its owner's gradient); with the SA model the gradient of nu~
+                             * (hook and field only; the NSMB source reads none) is ggcell only */
     int limiter;            /* uns_limiter_name: 0 "vencat" (ivencat=7), 1 "nolim", 2 "1st" */
     int entropy_fix;        /* roeEntropyFixMethod (3) */
     int cfl_vary_step;      /* CFLVaryStep (500) */

@@ -2476,8 +2476,13 @@ void orc_sa_viscosity(orc_solver *s)
 
 /* GG-cell gradient of nu~ (TurbSolverUnstr::GetGradientField :8652 with
  * gradientName "ggcell"). */
+/* TurbSolverUnstr::GetGradientField (:8652-8700), gradientName "ggcell".
+ * The NSMB formation reads no gradient of nu~, so the other gradient forms
+ * (ggnode through ComputeQTurbNodeValue :3480, lsq) are not restated: the
+ * field stays zero for them. */
 void orc_sa_gradient(orc_solver *s)
 {
+    if (s->p.gradient != 0) return;
     grad_ggcell(s, s->nut, s->gnx, s->gny, s->gnz);
 }
 

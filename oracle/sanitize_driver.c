@@ -118,7 +118,6 @@ int main(void)
     for (int g = 0; g < 3; ++g)
         for (int v = 0; v < 3; ++v)
             for (int lim = 0; lim < 3; ++lim) {
-                if (g != 0 && viscous[v] == 3) continue;  /* lsq and ggnode: NS equations only */
                 orc_params p = {0};
                 p.mach = 0.5; p.alpha_deg = 2.0; p.gamma = 1.4; p.reynolds = 1e5; p.t_ref = 288.15;
                 p.prandtl = 0.72; p.prandtl_turb = 0.9; p.t_wall = -1.0; p.venkat_k = 5.0; p.entropy_scale = 1.0;

@@ -321,9 +321,6 @@ phl_status check_params(const phl_params &p)
     if (p.viscous && !(p.reynolds > 0.0)) return fail(PHL_ERR_ARG, "reynolds must be positive for viscous flow");
     if (p.gradient < 0 || p.gradient > 2)
         return fail(PHL_ERR_UNSUPPORTED, "gradient must be 0 (ggcell), 1 (lsq) or 2 (ggnode)");
-    if (p.gradient != 0 && p.viscous == 3)
-        return fail(PHL_ERR_UNSUPPORTED,
-                    "the lsq and ggnode gradients are implemented for the NS equations only (viscous 0, 1)");
     if (p.limiter < 0 || p.limiter > 2) return fail(PHL_ERR_UNSUPPORTED, "limiter must be 0 (vencat), 1 (nolim) or 2 (1st)");
     if (p.entropy_fix != 1 && p.entropy_fix != 3 && p.entropy_fix != 5)
         return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1, 3 or 5");
@@ -1537,6 +1534,8 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
         case PHL_FIELD_DQ_T: return copy_field(c, f.dqT, 1, nT, nT, out);
         case PHL_FIELD_WDIST: return copy_field(c, c->m.wallDist, 1, nC, nT, out);
         default:  // a relaxation does not need the gradient of nu~: compute it from the current field
+            if (c->k.gradient != 0)
+                return fail(PHL_ERR_UNSUPPORTED, "the gradient of nu~ is implemented for ggcell only");
             phl::launch_sa_gradient(c->m, c->k, f, c->stream);
             TRY(check_launch(c));
             return copy_field(c, f.gradNut + (size_t)(field - PHL_FIELD_DNUDX) * nC, 1, nC, nT, out);
@@ -1820,6 +1819,9 @@ phl_status phl_turb_viscosity(phl_ctx *c)
 phl_status phl_turb_gradient(phl_ctx *c)
 {
     TRY(sa_ctx(c));
+    // TurbSolverUnstr::GetGradientField (:8652-8700) with gradientName; the
+    // NSMB formation reads no gradient of nu~, so only ggcell is restated
+    if (c->k.gradient != 0) return fail(PHL_ERR_UNSUPPORTED, "the gradient of nu~ is implemented for ggcell only");
     phl::launch_sa_gradient(c->m, c->k, c->f, c->stream);
     return check_launch(c);
 }
