@@ -64,6 +64,18 @@ CONFIGS["c3"] = dict(workload="swept wing (root chord 1, taper 0.5, AR 4, quarte
                      # the headline leg: time to a residual drop, plain FP64 from freestream against the mixed
                      # form started from the learned warm start (judged by the solver's residual)
                      converge=dict(drop=0.25, max_steps=5000))
+# The reference's own example case on its own grid: the RAE2822 2-D hybrid
+# grid it ships (4 zones, joined) with cfd_para_subsonic.hypara and the
+# cfd_para.hypara defaults (ggnode gradient, 100 first-order Van Leer steps).
+CONFIGS["rae2822"] = dict(workload="RAE2822 airfoil, the reference's example grid rae2822_hybrid2d__4_0.fts (2-D hybrid, "
+                                   "36,036 cells, 4 zones joined), its subsonic example case: Spalart-Allmaras RANS, "
+                                   "M0.2, AoA 2.79 deg, Re 6.5e6, Roe + GG-node + Venkat(K=50), LU-SGS 5 sweep pairs "
+                                   "(tolerance 0.01), CFL 0.01 -> 100 over 500 steps, 100 first-order Van Leer steps",
+                          mesh=("fts", os.path.join("tests", "golden", "rae2822_hybrid2d__4_0.fts")),
+                          params=dict(mach=0.2, alpha_deg=2.79, reynolds=6.5e6, viscous=3, gradient=2,
+                                      venkat_k=50.0, cfl_start=0.01, cfl_end=100.0, cfl_vary_step=500,
+                                      n_lusgs_sweeps=5, flow_init_step=100),
+                          cpu_sample=(5, 30, 2, 10), data="the reference's RAE2822 example grid (tests/golden)")
 DEFAULT_CONFIG = "c3"
 WING_WEIGHTS = os.path.join(ROOT, "phenlei_amd", "data", "wing_warmstart.npz")
 
@@ -109,6 +121,8 @@ def build_mesh(spec):
         return pa.Mesh.bump_hybrid(*spec[1:])
     if kind == "swept_wing":
         return pa.Mesh.swept_wing(*spec[1:])
+    if kind == "fts":
+        return pa.Mesh.from_fts(os.path.join(ROOT, spec[1]))
     raise ValueError(kind)
 
 
@@ -708,7 +722,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall / args.steps * 1e3,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
         "dtype": "f32 face fluxes + f64" if mixed else "f64",
-        "data": "synthetic",
+        "data": cfg.get("data", "synthetic"),
         "config": {"workload": cfg["workload"], "cells": mesh.n_cell, "faces": mesh.n_face,
                    "cases": mesh.n_cases, "precision": "mixed" if mixed else "fp64",
                    "parallelism": "single GPU", "launch": args.launch,

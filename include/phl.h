@@ -43,7 +43,7 @@ typedef struct phl_params {
     double t_ref;           /* refDimensionalTemperature [K] (288.15) */
     double prandtl;         /* prl (0.72) */
     double prandtl_turb;    /* prt (0.9) */
-    double t_wall;          /* wallTemperature [K]; <= 0 adiabatic (-1) */
+    double t_wall;          /* wallTemperature [K]; <= 0 adiabatic (-1), > 0 isothermal viscous walls */
     double venkat_k;        /* venkatCoeff (5.0) */
     double entropy_scale;   /* roeEntropyScale (1.0) */
     double cfl_start;       /* CFLStart (0.01) */

@@ -766,6 +766,18 @@ static void bc_viscous_wall(orc_solver *s, int f)
     s->q[IU][re] = -s->q[IU][le] + 2.0 * 0.0;
     s->q[IV][re] = -s->q[IV][le] + 2.0 * 0.0;
     s->q[IW][re] = -s->q[IW][le] + 2.0 * 0.0;
+    if (s->p.t_wall > 0.0) {
+        /* ViscousIsotropicWallBCRegion (:20791-20915, perfect gas): wall
+         * pressure kept, temperature mirrored about wallTemperature / T_ref
+         * (floored at a tenth of it), density from the equation of state */
+        double tw = s->p.t_wall / s->p.t_ref;
+        double tL = (1.0 / s->R) * s->q[IP][le] / s->q[IR][le];
+        double pG = s->q[IP][le];
+        double tG = 2.0 * tw - tL;
+        if (tG <= tw * 0.1) tG = tw * 0.1;
+        s->q[IR][re] = pG / (s->R * tG * 1.0);
+        s->q[IP][re] = pG;
+    }
 }
 
 /* FarfieldBCRegion, NSSolverUnstruct.cpp:21195-21330 (no preconditioning). */
@@ -828,7 +840,7 @@ void orc_boundary(orc_solver *s)
         if (bc == ORC_BC_EXTRAPOLATION || bc == ORC_BC_OUTFLOW) bc_outflow(s, f);
         else if (is_wall(bc)) {
             if (!s->p.viscous) bc_symmetry(s, f);
-            else bc_viscous_wall(s, f); /* adiabatic wall; isothermal walls are not on the path */
+            else bc_viscous_wall(s, f); /* adiabatic, or isothermal with t_wall > 0 */
         } else if (bc == ORC_BC_SYMMETRY) bc_symmetry(s, f);
         else if (bc == ORC_BC_FARFIELD) bc_farfield(s, f);
     }

@@ -331,7 +331,6 @@ phl_status check_params(const phl_params &p)
     if (p.sweep_form < 0 || p.sweep_form > 3)
         return fail(PHL_ERR_ARG, "sweep_form must be 0 (dataflow, launch order), 1 (one launch per level), 2 "
                                  "(dataflow, ticket order) or 3 (dataflow, levels split over the XCDs)");
-    if (p.t_wall > 0.0) return fail(PHL_ERR_UNSUPPORTED, "isothermal walls are not implemented (t_wall <= 0)");
     if (p.cfl_vary_step <= 0) return fail(PHL_ERR_ARG, "cfl_vary_step must be positive");
     return PHL_OK;
 }
@@ -1193,6 +1192,7 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     k.entropyFix = p.entropy_fix;
     k.nSweeps = p.n_lusgs_sweeps;
     k.lusgsTol = p.lusgs_tol;
+    k.twall = p.t_wall > 0.0 ? p.t_wall / p.t_ref : 0.0;
     k.mixed = p.precision;
     k.coefF32 = (k.mixed && c->coefF32Ok) ? 1 : 0;
     k.gradient = p.gradient;

@@ -124,6 +124,7 @@ struct Consts {
     int initPhase;          // isInIniting: first-order faces, Van Leer flux, limiter 0 (flowInitStep)
     double saNuInf, saVistInf, turbCflScale;  // SA: freestream nu~ and eddy viscosity, turbCFLScale
     double lusgsTol;        // LUSGSTolerance (nSweeps > 1)
+    double twall;           // wallTemperature / refDimensionalTemperature; <= 0: adiabatic walls
     const CaseConsts *cases;
 };
 

@@ -249,6 +249,17 @@ __global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K0, doubl
         qr[1] = -ql[1] + 2.0 * 0.0;
         qr[2] = -ql[2] + 2.0 * 0.0;
         qr[3] = -ql[3] + 2.0 * 0.0;
+        if (K.twall > 0.0) {
+            // isothermal wall (ViscousIsotropicWallBCRegion, NSSolverUnstruct.
+            // cpp:20791-20915): the wall pressure, the temperature mirrored
+            // about the wall's (at least a tenth of it), the density from both
+            const double tL = (1.0 / K.R) * ql[4] / ql[0];
+            const double pG = ql[4];
+            double tG = 2.0 * K.twall - tL;
+            if (tG <= K.twall * 0.1) tG = K.twall * 0.1;
+            qr[0] = pG / (K.R * tG * 1.0);
+            qr[4] = pG;
+        }
     } else if (bc == kBC_FAR) {
         const double *pf = K.primInf;
         const double gl = K.gamma;

@@ -549,3 +549,26 @@ def test_residual_parity_with_first_order_fallback(limiter):
     rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
     scale = np.max(np.abs(rc), axis=1, keepdims=True)
     assert np.max(np.abs(rg - rc) / scale) <= TOL
+
+
+@pytest.mark.parametrize("t_wall", [350.0, 20.0])
+def test_isothermal_wall_matches_oracle(t_wall):
+    """wallTemperature > 0: ghost states, residual and iterations on the
+    hybrid mesh with a laminar isothermal wall (and one cold enough to hit
+    the ghost-temperature floor)."""
+    m, gpu, cpu = make_hybrid_pair(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0, t_wall=t_wall)
+    q = seeded(m, cpu, 33)
+    gpu.set_state(q)
+    cpu.set("q", q)
+    gpu.residual()
+    cpu.residual()
+    nc = m.n_cell
+    qg, qc = gpu.get_state(), cpu.get("q")
+    assert np.max(np.abs(qg[:, nc:] - qc[:, nc:]) / np.abs(qc[:, nc:]).max(axis=1, keepdims=True)) <= 1e-15
+    rg, rc = gpu.get_residual()[:, :nc], cpu.get("res")[:, :nc]
+    assert rel_err(rg, rc, 1e-3 * np.max(np.abs(rc), axis=1, keepdims=True)) <= TOL
+    gpu.init_freestream()
+    cpu.init_freestream()
+    hg = gpu.iterate(1, 10)
+    hc, _ = cpu.iterate(1, 10)
+    assert np.max(np.abs(hg - hc) / hc) <= 1e-12

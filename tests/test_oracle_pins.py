@@ -479,3 +479,38 @@ def test_sa_implicit_term_is_the_negative_part_of_the_linearised_source():
     for args in [(3.0, 50.0, 1e-3), (30.0, 5.0, 1e-2), (300.0, 2.0, 0.05), (-2.0, 10.0, 1e-2)]:
         s, prde, prdep, diag = orc.sa_source_point(*args, 1.0, 1.0, 3e6)
         assert np.isclose(diag, -min(prde, 0.0) - min(prdep, 0.0) * args[0], rtol=1e-14, atol=1e-300)
+
+
+@pytest.mark.parametrize("t_wall", [200.0, 400.0, 20.0])
+def test_isothermal_wall_ghost(t_wall):
+    """wallTemperature > 0 (ViscousIsotropicWallBCRegion): the ghost of a wall
+    face keeps the pressure, reverses the velocity, and carries the
+    temperature mirrored about T_w (T_ghost = 2 T_w - T, floored at T_w / 10);
+    the face mean of the temperature is then T_w unless floored; the density
+    follows from p = rho R T."""
+    m = pa.Mesh.bump_hex(8, 4, 2, 0.05)
+    conn, met = m.connectivity(), m.metrics()
+    p = pa.default_params(mach=0.5, viscous=1, reynolds=1e5, t_wall=t_wall)
+    s = orc.OracleSolver(conn, met, p)
+    rng = np.random.default_rng(3)
+    q = np.tile(s.freestream()[:, None], (1, m.n_total))
+    q[:, :m.n_cell] *= 1 + rng.uniform(-0.1, 0.1, size=(5, m.n_cell))
+    s.set("q", q)
+    s.boundary()
+    qb = s.get("q")
+    R = 1.0 / (1.4 * 0.5 ** 2)
+    tw = t_wall / 288.15
+    walls = np.flatnonzero(conn["bc_type"] == 2)
+    le, gh = conn["left"][walls], m.n_cell + walls
+    np.testing.assert_allclose(qb[4, gh], qb[4, le], rtol=0)
+    np.testing.assert_allclose(qb[1:4, gh], -qb[1:4, le], rtol=0)
+    t_in = qb[4, le] / (R * qb[0, le])
+    t_gh = qb[4, gh] / (R * qb[0, gh])
+    expect = np.maximum(2 * tw - t_in, 0.1 * tw)
+    np.testing.assert_allclose(t_gh, expect, rtol=1e-14)
+    unfloored = 2 * tw - t_in > 0.1 * tw
+    np.testing.assert_allclose(0.5 * (t_in + t_gh)[unfloored], tw, rtol=1e-14)
+    if t_wall == 20.0:
+        assert not unfloored.any()  # a wall far colder than the gas: the floor everywhere
+    else:
+        assert unfloored.all()
