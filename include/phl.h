@@ -65,7 +65,7 @@ typedef struct phl_params {
                              * ghost takes its owner's gradient); with the SA model the gradient of nu~
                              * (hook and field only; the NSMB source reads none) is ggcell only */
     int limiter;            /* uns_limiter_name: 0 "vencat" (ivencat=7), 1 "nolim", 2 "1st" */
-    int entropy_fix;        /* roeEntropyFixMethod (3) */
+    int entropy_fix;        /* roeEntropyFixMethod (3): 1 to 6, Flux_RoeEntropyCorrection.cpp:93-166 */
     int cfl_vary_step;      /* CFLVaryStep (500) */
     int n_lusgs_sweeps;     /* nLUSGSSweeps (1) */
     int precision;          /* 0 FP64; 1 mixed (FP32 gradients and face fluxes, FP64 accumulation/LU-SGS) */
@@ -105,7 +105,10 @@ enum {
      * LU-SGS diagonal and increment, wall distance, gradient of nu~ */
     PHL_FIELD_VIST = 17, PHL_FIELD_NUT = 18, PHL_FIELD_RES_T = 19, PHL_FIELD_SPEC_T = 20,
     PHL_FIELD_DQ_T = 21, PHL_FIELD_WDIST = 22, PHL_FIELD_DNUDX = 23, PHL_FIELD_DNUDY = 24,
-    PHL_FIELD_DNUDZ = 25
+    PHL_FIELD_DNUDZ = 25,
+    /* the pressure factor of roeEntropyFixMethod 2 and 6 (rtem, [nTotal];
+     * zeros for the other methods) */
+    PHL_FIELD_PRESSURE_FACTOR = 26
 };
 
 /* Boundary-condition codes (API/Geometry/include/Geo_SimpleBC.h:32-39). */
@@ -270,6 +273,12 @@ phl_status phl_boundary(phl_ctx *ctx);
 /* CallGPUComputeGamaAndTemperature (:4668) and
  * CallGPUComputeViscousCoefficientWithoutChemical (:6724). */
 phl_status phl_gamma_t(phl_ctx *ctx);
+/* Controller::PressureFactor (Controller.cpp:1202-1234) ->
+ * NSSolverUnstruct::CalPressureFactor (NSSolverUnstruct.cpp:6976-7045): the
+ * cell sensor of roeEntropyFixMethod 2 and 6, which the Roe face flux reads;
+ * nothing for the other methods.  UpdateIndependentField calls it after
+ * gamma/T and before the gradient. */
+phl_status phl_pressure_factor(phl_ctx *ctx);
 /* NSSolverUnstruct::ComputeGradient -> GetGradientField (:18171,18244).
  * The same kernel also evaluates the limiter (both walk a cell's faces and
  * neighbours once) and refreshes the face loops' gather records. */

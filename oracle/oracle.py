@@ -17,10 +17,12 @@ LIB_PATH_OMP = os.path.join(HERE, "liborc_omp.so")
 
 FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
              diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16,
-             vist=17, nut=18, res_t=19, spec_t=20, dq_t=21, wdist=22, dnudx=23, dnudy=24, dnudz=25)
+             vist=17, nut=18, res_t=19, spec_t=20, dq_t=21, wdist=22, dnudx=23, dnudy=24, dnudz=25,
+             pressure_factor=26)
 FIELD_ROWS = dict(q=5, res=5, dqdx=5, dqdy=5, dqdz=5, limit=1, dq=5, spec_inv=1, spec_vis=1, dt=1,
                   diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5,
-                  vist=1, nut=1, res_t=1, spec_t=1, dq_t=1, wdist=1, dnudx=1, dnudy=1, dnudz=1)
+                  vist=1, nut=1, res_t=1, spec_t=1, dq_t=1, wdist=1, dnudx=1, dnudy=1, dnudz=1,
+                  pressure_factor=1)
 
 _D = ctypes.POINTER(ctypes.c_double)
 _I = ctypes.POINTER(ctypes.c_int)
@@ -54,7 +56,8 @@ def load(omp=False):
         lib.orc_get.argtypes = [_P, ctypes.c_int, _D]
         lib.orc_set.argtypes = [_P, ctypes.c_int, _D]
         lib.orc_freestream.argtypes = [_P, _D]
-        for name in ("orc_init_freestream", "orc_boundary", "orc_gamma_t", "orc_viscosity", "orc_gradient",
+        for name in ("orc_init_freestream", "orc_boundary", "orc_gamma_t", "orc_pressure_factor", "orc_viscosity",
+                     "orc_gradient",
                      "orc_limiter", "orc_zero_residual", "orc_inviscid_flux", "orc_viscous_flux",
                      "orc_update_residuals", "orc_residual", "orc_spectrum", "orc_sa_boundary", "orc_sa_viscosity",
                      "orc_sa_gradient", "orc_sa_update_residuals", "orc_sa_spectrum", "orc_sa_lusgs",
@@ -78,6 +81,7 @@ def load(omp=False):
         lib.orc_force_coefficients.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
         lib.orc_roe_flux.argtypes = [_D, _D, ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
                                      ctypes.c_int, ctypes.c_double, ctypes.c_double, _D]
+        lib.orc_roe_flux_pc.argtypes = [_D, _D] + [ctypes.c_double] * 4 + [ctypes.c_int] + [ctypes.c_double] * 4 + [_D]
         lib.orc_harten.argtypes = [ctypes.c_double, ctypes.c_double]
         lib.orc_harten.restype = ctypes.c_double
         lib.orc_venfun.argtypes = [ctypes.c_double] * 3
@@ -146,12 +150,20 @@ def vanleer_flux(qL, qR, n, gamma=1.4):
     return out
 
 
-def roe_flux(qL, qR, n, gamma=1.4, fix=3, eps=1e-4):
+def roe_flux(qL, qR, n, gamma=1.4, fix=3, eps=1e-4, convect=None, pc=None):
+    """Roe flux per unit area; eps is the acoustic entropy-fix coefficient,
+    convect the convective one (eps when None); pc the pressure factors of
+    the face's two cells (methods 2 and 6)."""
     lib = load()
     qL = np.ascontiguousarray(qL, dtype=np.float64)
     qR = np.ascontiguousarray(qR, dtype=np.float64)
     out = np.empty(5)
-    lib.orc_roe_flux(_dp(qL), _dp(qR), float(n[0]), float(n[1]), float(n[2]), gamma, fix, eps, eps, _dp(out))
+    cv = eps if convect is None else convect
+    if pc is None:
+        lib.orc_roe_flux(_dp(qL), _dp(qR), float(n[0]), float(n[1]), float(n[2]), gamma, fix, eps, cv, _dp(out))
+    else:
+        lib.orc_roe_flux_pc(_dp(qL), _dp(qR), float(n[0]), float(n[1]), float(n[2]), gamma, fix, eps, cv,
+                            float(pc[0]), float(pc[1]), _dp(out))
     return out
 
 

@@ -97,6 +97,8 @@ struct orc_solver {
     double *q[5], *res[5], *dqdx[5], *dqdy[5], *dqdz[5], *dq[5], *dflux[5];
     double *lim, *specInv, *specVis, *dt, *diag, *gama, *t, *visl, *vist;
     double *dtdx, *dtdy, *dtdz;
+    /* pressure factor of the pressure-sensor entropy fixes (rtem; methods 2, 6) */
+    double *rtem;
     /* least-squares gradient (gradient == 1): face weights and the inverse
      * normal-matrix entries per cell */
     double *iwt, *ixx, *iyy, *izz, *ixy, *ixz, *iyz;
@@ -554,6 +556,7 @@ orc_solver *orc_create(const orc_params *p, int nCell, int nBFace, int nFace,
     s->matL = zeros(nFace); s->matR = zeros(nFace); s->dqT = zeros(nT); s->dfluxT = zeros(nT);
     s->gnx = zeros(nT); s->gny = zeros(nT); s->gnz = zeros(nT); s->wdist = zeros(nT);
     s->dtdx = zeros(nT); s->dtdy = zeros(nT); s->dtdz = zeros(nT);
+    s->rtem = zeros(nT);
     s->R = 1.0 / (p->gamma * p->mach * p->mach);
     s->pTotal = orc_ptotal(p->mach, p->gamma);
     s->entropyCoef = orc_entropy_coef(p->mach, p->entropy_scale);
@@ -654,7 +657,7 @@ void orc_destroy(orc_solver *s)
     free(s->gama); free(s->t); free(s->visl); free(s->vist);
     free(s->nut); free(s->resT); free(s->specT); free(s->matL); free(s->matR); free(s->dqT); free(s->dfluxT);
     free(s->gnx); free(s->gny); free(s->gnz); free(s->wdist);
-    free(s->dtdx); free(s->dtdy); free(s->dtdz);
+    free(s->dtdx); free(s->dtdy); free(s->dtdz); free(s->rtem);
     free(s->iwt); free(s->ixx); free(s->iyy); free(s->izz); free(s->ixy); free(s->ixz); free(s->iyz);
     free(s->xn); free(s->yn); free(s->zn); free(s->fnPtr); free(s->fn); free(s->cnPtr); free(s->cn);
     for (int m = 0; m < 5; ++m) free(s->qNode[m]);
@@ -697,6 +700,7 @@ static double *field1(orc_solver *s, int field)
     case ORC_DNUDX: return s->gnx;
     case ORC_DNUDY: return s->gny;
     case ORC_DNUDZ: return s->gnz;
+    case ORC_PRESSURE_FACTOR: return s->rtem;
     default: return NULL;
     }
 }
@@ -854,6 +858,56 @@ void orc_gamma_t(orc_solver *s)
         s->gama[c] = s->p.gamma;
         s->t[c] = s->q[IP][c] / (s->R * s->q[IR][c]);
     }
+}
+
+/* Controller::PressureFactor (CFD/Solver/src/Controller.cpp:1202-1234) with
+ * NSSolverUnstruct::CalPressureFactor (NSSolverUnstruct.cpp:6976-7045): the
+ * cell sensor rtem of the pressure-sensor entropy fixes, nothing for the
+ * other methods.  Method 2: over the cell's faces (in face order), the sums
+ * of the pressures on the faces' left and right sides (ghost states on
+ * boundary faces), |sumR - sumL| / |sumR + sumL| / nFaces.  Method 6: from
+ * the largest and smallest pressure of the cell and its interior-face
+ * neighbours (UnstructGrid::ComputeCell2Cell, Geo_UnstructGrid.cpp:
+ * 1104-1156), ratio = max/min, f = clip((2.6 - ratio) / (2.6 - 1.3), 0, 1),
+ * rtem = (1 - cos(f pi)) / 2 with pi = 3.141592653589.  Ghosts take their
+ * owner's value (SetGhostCellExceptInterface, Geo_UnstructGrid.cpp:
+ * 1857-1881).  Called after the boundary conditions and gamma/T. */
+void orc_pressure_factor(orc_solver *s)
+{
+    const int method = s->p.entropy_fix;
+    if (method != 2 && method != 6) return;
+    const double *p = s->q[IP];
+    CELL_LOOP
+    for (int c = 0; c < s->nCell; ++c) {
+        const int b = s->c2fPtr[c], e = s->c2fPtr[c + 1];
+        if (method == 2) {
+            double sl = 0.0, sr = 0.0;
+            for (int k = b; k < e; ++k) {
+                const int f = s->c2f[k];
+                sl += p[s->left[f]];
+                sr += p[s->right[f]];
+            }
+            double r = 0.0;
+            r += ABS(sr - sl) / ABS(sr + sl);
+            r /= (double)(e - b);
+            s->rtem[c] = r;
+        } else {
+            double lo = p[c], hi = p[c];
+            for (int k = b; k < e; ++k) {
+                const int f = s->c2f[k];
+                if (f < s->nBFace) continue;
+                const int nb = s->left[f] == c ? s->right[f] : s->left[f];
+                if (p[nb] < lo) lo = p[nb];
+                if (p[nb] > hi) hi = p[nb];
+            }
+            const double ratio = hi / lo;
+            const double fn = MIN(1.0, MAX(0.0, (2.6 - ratio) / (2.6 - 1.3)));
+            double cf = cos(fn * 3.141592653589);
+            cf = MAX(-1.0, MIN(cf, 1.0));
+            s->rtem[c] = 0.5 * (1.0 - cf);
+        }
+    }
+    for (int f = 0; f < s->nBFace; ++f) s->rtem[s->right[f]] = s->rtem[s->left[f]];
 }
 
 /* ComputeViscousCoefficientWithoutChemical, NSSolverUnstruct.cpp:6722-6746. */
@@ -1213,14 +1267,24 @@ double orc_harten(double eig, double limit)
 }
 
 /* Flux_RoeEntropyCorrection, API/Physics/src/Flux_RoeEntropyCorrection.cpp:93-166
- * (methods 1, 3, 5; pressure-sensor methods 2/6 need a sensor field). */
-static void entropy_fix(double *e1, double *e2, double *e3, double vn, double cm,
-                        int method, double acoustic, double convect)
+ * (the build without USE_GMRESSOLVER).  pcL, pcR: the pressure factors of
+ * the face's two cells (methods 2 and 6); absVel: |V| of the Roe average. */
+static void entropy_fix(double *e1, double *e2, double *e3, double vn, double cm, double absVel,
+                        double pcL, double pcR, int method, double acoustic, double convect)
 {
     if (method == 1) {
         double mx = MAX(*e2, *e3);
         double t0 = mx * acoustic, t1 = mx * acoustic;
         *e1 = MAX(t0, *e1); *e2 = MAX(t1, *e2); *e3 = MAX(t1, *e3);
+    } else if (method == 2) {
+        /* Lin, JCP 1995 */
+        const double k1 = 0.25, k2 = 5.0, k3 = 15.0;
+        const double ps = MAX(pcL, pcR);
+        const double l1 = (ABS(vn) + cm) * k3 * ps;
+        const double l2 = (ABS(vn) + cm) * (k1 + k2 * ps);
+        *e1 = orc_harten(*e1, l1);
+        *e2 = orc_harten(*e2, l2);
+        *e3 = orc_harten(*e3, l2);
     } else if (method == 3) {
         double me = ABS(vn) + cm;
         double al = acoustic * me;
@@ -1228,17 +1292,31 @@ static void entropy_fix(double *e1, double *e2, double *e3, double vn, double cm
         *e1 = orc_harten(*e1, cl);
         *e2 = orc_harten(*e2, al);
         *e3 = orc_harten(*e3, al);
+    } else if (method == 4) {
+        const double mx = MAX(*e2, *e3);
+        const double ratio = MIN(1.0, fabs(vn) / cm);
+        const double t0 = MAX(mx * acoustic * ratio, mx * acoustic / 100.0);
+        const double t1 = MAX(mx * acoustic * ratio, mx * acoustic / 100.0);
+        *e1 = MAX(t0, *e1); *e2 = MAX(t1, *e2); *e3 = MAX(t1, *e3);
     } else if (method == 5) {
         double mx = MAX(*e2, *e3);
         double d = mx * acoustic;
         *e1 += d; *e2 += d; *e3 += d;
+    } else if (method == 6) {
+        const double ps = MIN(pcL, pcR);
+        const double lim = (1.0 - ps) * (absVel + cm) + 0.0001 * cm;
+        *e1 = orc_harten(*e1, lim);
+        *e2 = orc_harten(*e2, lim);
+        *e3 = orc_harten(*e3, lim);
     }
 }
 
 /* Roe_Scheme_ConservativeForm, API/Physics/src/Flux_Inviscid.cpp:136-442
- * (perfect gas, no preconditioning, static mesh), one face, flux per unit area. */
-void orc_roe_flux(const double *qLin, const double *qRin, double xsn, double ysn, double zsn,
-                  double gama_, int fix_method, double acoustic, double convect, double *flux)
+ * (perfect gas, no preconditioning, static mesh), one face, flux per unit
+ * area; pcL, pcR are the face's pressureCoeffL/R (GetPressureFactorLR,
+ * NSSolverUnstruct.cpp:5574-5592). */
+void orc_roe_flux_pc(const double *qLin, const double *qRin, double xsn, double ysn, double zsn, double gama_,
+                     int fix_method, double acoustic, double convect, double pcL, double pcR, double *flux)
 {
     const double LITTLE = 1.0E-10;
     double gmL = gama_, gmR = gama_;
@@ -1273,7 +1351,6 @@ void orc_roe_flux(const double *qLin, const double *qRin, double xsn, double ysn
     double gamm1 = gama - 1.0;
     double v2Roe = uRoe * uRoe + vRoe * vRoe + wRoe * wRoe;
     double absVel = sqrt(v2Roe);
-    (void)absVel;
     double hRoe = gama / gamm1 * pRoe / rRoe + 0.5 * v2Roe;
     double vn = xsn * uRoe + ysn * vRoe + zsn * wRoe - vb;
     double theta = vn + vb;
@@ -1287,7 +1364,7 @@ void orc_roe_flux(const double *qLin, const double *qRin, double xsn, double ysn
     double dq2 = rR * vR - rL * vL;
     double dq3 = rR * wR - rL * wL;
     double dq4 = rR * ER - rL * EL;
-    entropy_fix(&e1, &e2, &e3, vn, cm, fix_method, acoustic, convect);
+    entropy_fix(&e1, &e2, &e3, vn, cm, absVel, pcL, pcR, fix_method, acoustic, convect);
     double xi1 = (2.0 * e1 - e2 - e3) / (2.0 * c2);
     double xi2 = (e2 - e3) / (2.0 * cm);
     double dc = theta * dq0 - xsn * dq1 - ysn * dq2 - zsn * dq3;
@@ -1298,6 +1375,12 @@ void orc_roe_flux(const double *qLin, const double *qRin, double xsn, double ysn
     flux[2] -= 0.5 * (e1 * dq2 + (ysn * c2dc - vRoe * dh) * xi1 + (ysn * dh - vRoe * dc) * xi2);
     flux[3] -= 0.5 * (e1 * dq3 + (zsn * c2dc - wRoe * dh) * xi1 + (zsn * dh - wRoe * dc) * xi2);
     flux[4] -= 0.5 * (e1 * dq4 + (theta * c2dc - hRoe * dh) * xi1 + (theta * dh - hRoe * dc) * xi2);
+}
+
+void orc_roe_flux(const double *qLin, const double *qRin, double xsn, double ysn, double zsn,
+                  double gama_, int fix_method, double acoustic, double convect, double *flux)
+{
+    orc_roe_flux_pc(qLin, qRin, xsn, ysn, zsn, gama_, fix_method, acoustic, convect, 0.0, 0.0, flux);
 }
 
 /* ------------------------------------------------------------------ */
@@ -1440,8 +1523,8 @@ void orc_inviscid_flux(orc_solver *s)
         if (s->initPhase)
             orc_vanleer_flux(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma, fl);
         else
-            orc_roe_flux(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma,
-                         s->p.entropy_fix, s->entropyCoef, s->entropyCoef, fl);
+            orc_roe_flux_pc(qL, qR, s->xfn[f], s->yfn[f], s->zfn[f], s->p.gamma, s->p.entropy_fix,
+                            s->entropyCoef, s->entropyCoef, s->rtem[s->left[f]], s->rtem[s->right[f]], fl);
         double a = s->area[f];
         for (int m = 0; m < 5; ++m) fl[m] *= a;
         int le = s->left[f], re = s->right[f];
@@ -1613,6 +1696,7 @@ void orc_residual(orc_solver *s)
      * ghosts, and the temperature of that iteration's UpdateIndependentField */
     if (s->p.gradient == 2) orc_node_values(s);
     orc_gamma_t(s);
+    orc_pressure_factor(s);
     if (s->p.viscous) orc_viscosity(s);
     orc_gradient(s);
     orc_limiter(s);

@@ -66,7 +66,9 @@ enum { ORC_Q = 0, ORC_RES = 1, ORC_DQDX = 2, ORC_DQDY = 3, ORC_DQDZ = 4,
        ORC_DTDY = 14, ORC_DTDZ = 15, ORC_DELTA_FLUX = 16,
        /* Spalart-Allmaras (viscous == 3) */
        ORC_VIST = 17, ORC_NUT = 18, ORC_RES_T = 19, ORC_SPEC_T = 20, ORC_DQ_T = 21,
-       ORC_WDIST = 22, ORC_DNUDX = 23, ORC_DNUDY = 24, ORC_DNUDZ = 25 };
+       ORC_WDIST = 22, ORC_DNUDX = 23, ORC_DNUDY = 24, ORC_DNUDZ = 25,
+       /* pressure factor of roeEntropyFixMethod 2 and 6 (rtem) */
+       ORC_PRESSURE_FACTOR = 26 };
 
 typedef struct orc_solver orc_solver;
 
@@ -119,6 +121,7 @@ void orc_get_node_values(const orc_solver *s, double *out6);  /* [6][nNode]: 5 p
 /* ---- one iteration, piece by piece (NSSolverUnstruct / Controller) ---- */
 void orc_boundary(orc_solver *s);
 void orc_gamma_t(orc_solver *s);
+void orc_pressure_factor(orc_solver *s);
 void orc_viscosity(orc_solver *s);
 void orc_gradient(orc_solver *s);
 void orc_limiter(orc_solver *s);
@@ -172,6 +175,9 @@ void orc_force_coefficients(const double *raw, double alpha_deg, double beta_deg
 /* ---- point functions, exposed for closed-form tests ---- */
 void   orc_roe_flux(const double *qL, const double *qR, double nx, double ny, double nz,
                     double gamma, int fix_method, double acoustic, double convect, double *flux);
+/* with the pressure factors of the face's cells (roeEntropyFixMethod 2, 6) */
+void   orc_roe_flux_pc(const double *qL, const double *qR, double nx, double ny, double nz, double gamma,
+                       int fix_method, double acoustic, double convect, double pcL, double pcR, double *flux);
 double orc_harten(double eig, double limit);
 double orc_venfun(double x, double y, double eps);
 void   orc_mxdq(const double *prim, double nx, double ny, double nz, double area,

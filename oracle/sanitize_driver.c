@@ -3,7 +3,8 @@
  * small hexahedral box under AddressSanitizer and UndefinedBehaviorSanitizer
  * (tests/test_oracle_sanitize.py builds and runs it).  Every gradient form
  * (ggcell, lsq, ggnode), limiter and flow model (inviscid, laminar,
- * Spalart-Allmaras) takes a few iterations, the forces and the node values;
+ * Spalart-Allmaras), with the Roe entropy fixes 1-6 spread over the runs,
+ * takes a few iterations, the forces and the node values;
  * the sanitizers report any out-of-bounds access, leak or undefined
  * operation.
  */
@@ -124,7 +125,8 @@ int main(void)
                 p.cfl_start = 5.0; p.cfl_end = 5.0; p.ktmax = 1e10; p.lusgs_tol = 0.01;
                 p.force_ref_length = 1.0; p.force_ref_area = 1.0; p.force_ref_span = 1.0;
                 p.sa_nu_inf = 3.0; p.sa_vist_inf = 1e-3; p.turb_cfl_scale = 1.0;
-                p.viscous = viscous[v]; p.gradient = g; p.limiter = lim; p.entropy_fix = 3;
+                p.viscous = viscous[v]; p.gradient = g; p.limiter = lim;
+                p.entropy_fix = 1 + (9 * g + 3 * v + lim) % 6;  /* every Roe entropy fix */
                 p.cfl_vary_step = 500; p.n_lusgs_sweeps = 1;
                 orc_solver *s = orc_create(&p, nCell, nb, nFace, left, right, bc, xfc, yfc, zfc, xfn, yfn, zfn, area,
                                            xcc, ycc, zcc, vol, skew);

@@ -26,10 +26,12 @@ PHL_NOALLOC_MAX_CELLS = 16777216  # phl.h: cells of phl_residual_norm_noalloc's 
 # Field identifiers (include/phl.h), shared with the oracle.
 FIELD = dict(q=0, res=1, dqdx=2, dqdy=3, dqdz=4, limit=5, dq=6, spec_inv=7, spec_vis=8, dt=9,
              diag=10, t=11, visl=12, dtdx=13, dtdy=14, dtdz=15, delta_flux=16,
-             vist=17, nut=18, res_t=19, spec_t=20, dq_t=21, wdist=22, dnudx=23, dnudy=24, dnudz=25)
+             vist=17, nut=18, res_t=19, spec_t=20, dq_t=21, wdist=22, dnudx=23, dnudy=24, dnudz=25,
+             pressure_factor=26)
 FIELD_ROWS = dict(q=5, res=5, dqdx=5, dqdy=5, dqdz=5, limit=1, dq=5, spec_inv=1, spec_vis=1, dt=1,
                   diag=1, t=1, visl=1, dtdx=1, dtdy=1, dtdz=1, delta_flux=5,
-                  vist=1, nut=1, res_t=1, spec_t=1, dq_t=1, wdist=1, dnudx=1, dnudy=1, dnudz=1)
+                  vist=1, nut=1, res_t=1, spec_t=1, dq_t=1, wdist=1, dnudx=1, dnudy=1, dnudz=1,
+                  pressure_factor=1)
 SETTABLE_ROWS_LEN = dict(q="total", dt="cell", vist="total", nut="total", wdist="cell")
 
 BC = dict(extrapolation=1, solid_surface=2, symmetry=3, farfield=4, inflow=5, outflow=6)
@@ -137,6 +139,7 @@ def _declare(lib):
         "phl_init_freestream": (ctypes.c_int, [_P]),
         "phl_boundary": (ctypes.c_int, [_P]),
         "phl_gamma_t": (ctypes.c_int, [_P]),
+        "phl_pressure_factor": (ctypes.c_int, [_P]),
         "phl_gradient": (ctypes.c_int, [_P]),
         "phl_compute_node_value": (ctypes.c_int, [_P]),
         "phl_get_node_values": (ctypes.c_int, [_P, _D]),
@@ -491,6 +494,9 @@ class Solver:
 
     def gamma_t(self):
         _check(_lib.phl_gamma_t(self._h))
+
+    def pressure_factor(self):
+        _check(_lib.phl_pressure_factor(self._h))
 
     def gradient(self):
         _check(_lib.phl_gradient(self._h))

@@ -322,8 +322,8 @@ phl_status check_params(const phl_params &p)
     if (p.gradient < 0 || p.gradient > 2)
         return fail(PHL_ERR_UNSUPPORTED, "gradient must be 0 (ggcell), 1 (lsq) or 2 (ggnode)");
     if (p.limiter < 0 || p.limiter > 2) return fail(PHL_ERR_UNSUPPORTED, "limiter must be 0 (vencat), 1 (nolim) or 2 (1st)");
-    if (p.entropy_fix != 1 && p.entropy_fix != 3 && p.entropy_fix != 5)
-        return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1, 3 or 5");
+    if (p.entropy_fix < 1 || p.entropy_fix > 6)
+        return fail(PHL_ERR_UNSUPPORTED, "roeEntropyFixMethod must be 1 to 6");
     if (p.n_lusgs_sweeps < 1) return fail(PHL_ERR_ARG, "nLUSGSSweeps must be at least 1");
     if (p.flow_init_step < 0) return fail(PHL_ERR_ARG, "flowInitStep must not be negative");
     if (p.precision != 0 && p.precision != 1)
@@ -1213,6 +1213,13 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     TRY(dalloc(c, &fl.gradF, (size_t)15 * nC));
     TRY(dalloc(c, &fl.gradTF, (size_t)3 * nC));
     TRY(dalloc(c, &fl.lim, (size_t)nT));
+    // the pressure factor of the pressure-sensor entropy fixes (methods 2, 6)
+    fl.rtem = nullptr;
+    if (p.entropy_fix == 2 || p.entropy_fix == 6) {
+        TRY(dalloc(c, &fl.rtem, (size_t)nT));
+        HIPTRY(hipMemset(fl.rtem, 0, sizeof(double) * nT));
+    }
+    k.rtem = fl.rtem;
     TRY(dalloc(c, &fl.dq, (size_t)5 * nT));
     TRY(dalloc(c, &fl.dflux, (size_t)5 * nC));
     TRY(dalloc(c, &fl.specInv, (size_t)nC));
@@ -1523,6 +1530,12 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
     }
     case PHL_FIELD_DELTA_FLUX: return copy_field(c, f.dflux, 5, nC, nT, out);
     case PHL_FIELD_VIST: return copy_field(c, f.vist, 1, nT, nT, out);
+    case PHL_FIELD_PRESSURE_FACTOR:
+        if (!f.rtem) {
+            std::fill(out, out + nT, 0.0);
+            return PHL_OK;
+        }
+        return copy_field(c, f.rtem, 1, nT, nT, out);
     default: break;
     }
     if (field >= PHL_FIELD_NUT && field <= PHL_FIELD_DNUDZ) {
@@ -1612,6 +1625,14 @@ phl_status phl_gamma_t(phl_ctx *c)
     return check_launch(c);
 }
 
+phl_status phl_pressure_factor(phl_ctx *c)
+{
+    if (!c) return fail(PHL_ERR_ARG, "null argument");
+    HIPTRY(hipSetDevice(c->device));
+    phl::launch_pressure_factor(c->m, c->k, c->f, c->stream);
+    return check_launch(c);
+}
+
 phl_status phl_gradient(phl_ctx *c)
 {
     if (!c) return fail(PHL_ERR_ARG, "null argument");
@@ -1679,6 +1700,7 @@ static void enqueue_residual(phl_ctx *c)
     // UpdateIndependentField's laminar viscosity, which the turbulence hooks
     // read (the NS kernels evaluate it themselves)
     if (c->k.viscous == 3) phl::launch_gamma_t(c->m, c->k, c->f, c->stream);
+    phl::launch_pressure_factor(c->m, c->k, c->f, c->stream);
     phl::launch_grad_lim(c->m, c->k, c->f, c->stream);
     phl::launch_flux(c->m, c->k, c->f, c->stream, false);
     phl::launch_norm_final(c->m, c->k, c->f, false, c->stream);
@@ -1985,6 +2007,7 @@ static void enqueue_step(phl_ctx *c, hipEvent_t *ev)
     node_pass(c, s);
     // the turbulence model reads the laminar viscosity of the step's state
     if (c->k.viscous == 3) phl::launch_gamma_t(c->m, c->k, c->f, s);
+    phl::launch_pressure_factor(c->m, c->k, c->f, s);
     mark();
     phl::launch_grad_lim(c->m, c->k, c->f, s); mark();
     phl::launch_flux(c->m, c->k, c->f, s, true); mark();

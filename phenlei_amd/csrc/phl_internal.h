@@ -126,6 +126,7 @@ struct Consts {
     double lusgsTol;        // LUSGSTolerance (nSweeps > 1)
     double twall;           // wallTemperature / refDimensionalTemperature; <= 0: adiabatic walls
     const CaseConsts *cases;
+    const double *rtem;     // [nTotal] pressure factor of roeEntropyFixMethod 2 and 6, else null
 };
 
 // Host side of an LU-SGS call with several sweep pairs: a pinned buffer for
@@ -160,6 +161,7 @@ struct DevFlow {
     float *gradF;     // [15][nCell] mixed form
     float *gradTF;    // [3][nCell] mixed form
     double *lim;      // [nTotal]
+    double *rtem;     // [nTotal] pressure factor (entropy fix 2, 6; null otherwise)
     double *dq;       // [5][nTotal]
     double *dflux;    // [5][nCell]
     double *casePart; // batched contexts: [nCases][32][6] partial residual sums and dt minima
@@ -265,6 +267,8 @@ void launch_spectrum(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t 
 // reference calls it): res = -0.0 (LoadResiduals), then face passes that add
 // their face values to res (parts bit 0: inviscid, bit 1: viscous), then the
 // residual sums of the current res into normOut row *stepCtr.
+// Controller::PressureFactor: the cell sensor of entropy fixes 2 and 6.
+void launch_pressure_factor(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);
 void launch_load_residuals(const DevMesh &m, DevFlow &f, hipStream_t s);
 void launch_face_pass(const DevMesh &m, const Consts &k, DevFlow &f, int parts, hipStream_t s);
 void launch_residual_norm(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);

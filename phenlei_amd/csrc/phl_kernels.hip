@@ -38,6 +38,7 @@ __device__ __forceinline__ double dMIN(double a, double b) { return (a < b) ? a 
 __device__ __forceinline__ double dMAX(double a, double b) { return (a > b) ? a : b; }
 __device__ __forceinline__ double dABS(double a) { return (a < 0) ? -a : a; }
 template <typename T> __device__ __forceinline__ T tMAX(T a, T b) { return (a > b) ? a : b; }
+template <typename T> __device__ __forceinline__ T tMIN(T a, T b) { return (a < b) ? a : b; }
 template <typename T> __device__ __forceinline__ T tABS(T a) { return (a < T(0)) ? -a : a; }
 
 // The context scalars as seen from one cell: in a batched context the
@@ -135,7 +136,7 @@ __device__ void vanleer_flux(const T *qL, const T *qR, T xsn, T ysn, T zsn, T ga
 // instantiation performs exactly the operations of the reference.
 template <typename T>
 __device__ void roe_flux(const T *qLin, const T *qRin, T xsn, T ysn, T zsn, T gama_, int fixMethod, T ent,
-                         T *flux)
+                         T pcL, T pcR, T *flux)
 {
     const T LITTLE = T(1.0E-10);
     const T gmL = gama_, gmR = gama_;
@@ -186,6 +187,26 @@ __device__ void roe_flux(const T *qLin, const T *qRin, T xsn, T ysn, T zsn, T ga
         const T mx = tMAX(e2, e3);
         const T t0 = mx * ent, t1 = mx * ent;
         e1 = tMAX(t0, e1); e2 = tMAX(t1, e2); e3 = tMAX(t1, e3);
+    } else if (fixMethod == 2) {
+        // the pressure-sensor fix of Lin (JCP 1995): k1 0.25, k2 5, k3 15
+        const T ps = tMAX(pcL, pcR);
+        const T l1 = (tABS(vn) + cm) * T(15.0) * ps;
+        const T l2 = (tABS(vn) + cm) * (T(0.25) + T(5.0) * ps);
+        e1 = harten(e1, l1);
+        e2 = harten(e2, l2);
+        e3 = harten(e3, l2);
+    } else if (fixMethod == 4) {
+        const T mx = tMAX(e2, e3);
+        const T ratio = tMIN(T(1.0), tABS(vn) / cm);
+        const T t0 = tMAX(mx * ent * ratio, mx * ent / T(100.0));
+        e1 = tMAX(t0, e1); e2 = tMAX(t0, e2); e3 = tMAX(t0, e3);
+    } else if (fixMethod == 6) {
+        // from the shock sensor, |V| of the Roe average and the sound speed
+        const T absVel = sqrt(v2Roe);
+        const T lim = (T(1.0) - tMIN(pcL, pcR)) * (absVel + cm) + T(0.0001) * cm;
+        e1 = harten(e1, lim);
+        e2 = harten(e2, lim);
+        e3 = harten(e3, lim);
     } else if (fixMethod == 3) {
         const T me = tABS(vn) + cm;
         const T al = ent * me;
@@ -1001,7 +1022,9 @@ __global__ void __launch_bounds__(BS) k_flux(DevMesh M, Consts K0, const double 
                 }
                 T fl[5];
                 if (K.initPhase) vanleer_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), fl);
-                else roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
+                else roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef),
+                                 K.rtem ? T(K.rtem[isL ? c : nb]) : T(0.0), K.rtem ? T(K.rtem[isL ? nb : c]) : T(0.0),
+                                 fl);
                 const T area = T(fr.area);
 #pragma unroll
                 for (int m = 0; m < 5; ++m) fl[m] *= area;
@@ -1195,7 +1218,8 @@ __device__ __forceinline__ void face_values(const DevMesh &M, const Consts &K0, 
         }
         T fl[5];
         if (K.initPhase) vanleer_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), fl);
-        else roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef), fl);
+        else roe_flux<T>(qL, qR, nx, ny, nz, T(K.gamma), K.entropyFix, T(K.entropyCoef),
+                         K.rtem ? T(K.rtem[le]) : T(0.0), K.rtem ? T(K.rtem[re]) : T(0.0), fl);
         const T area = T(fr.area);
         // inviscid spectral radius term at the face-averaged state
         const double vnl = fr.nx * L.q[1] + fr.ny * L.q[2] + fr.nz * L.q[3] - 0.0;
@@ -1290,6 +1314,64 @@ __global__ void __launch_bounds__(kBlock) k_cell_add(DevMesh M, const double *__
     }
 #pragma unroll
     for (int m = 0; m < 5; ++m) res[(size_t)m * nC + c] = acc[m];
+}
+
+// NSSolverUnstruct::CalPressureFactor (NSSolverUnstruct.cpp:6976-7045): the
+// cell sensor of the pressure-sensor entropy fixes, one thread per real cell.
+// Method 2: the pressures on the left and right sides of the cell's faces
+// (ghost states on boundary faces) summed in face order, then
+// |sumR - sumL| / |sumR + sumL| / nFaces.  Method 6: the pressure ratio
+// max/min over the cell and its interior-face neighbours mapped through
+// f = clip((2.6 - ratio) / 1.3, 0, 1) to (1 - cos(f pi)) / 2.  The device's
+// cos may differ from the host library's in the last bit.
+__global__ void __launch_bounds__(kBlock) k_pressure_factor(DevMesh M, int method, const double *__restrict__ q,
+                                                            double *__restrict__ rtem)
+{
+    const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+    const int nC = M.nCell;
+    if (c >= nC) return;
+    const double *p = q + (size_t)4 * M.nTotal;
+    const double pc = p[c];
+    double r;
+    if (method == 2) {
+        double sl = 0.0, sr = 0.0;
+        int n = 0;
+        for (int s = 0; s < M.ellW; ++s) {
+            const int enc = M.c2fEnc[(size_t)s * nC + c];
+            if (enc == 0) break;
+            const double pn = p[M.c2n[(size_t)s * nC + c]];
+            sl += enc > 0 ? pc : pn;
+            sr += enc > 0 ? pn : pc;
+            ++n;
+        }
+        r = 0.0;
+        r += dABS(sr - sl) / dABS(sr + sl);
+        r /= (double)n;
+    } else {
+        double lo = pc, hi = pc;
+        for (int s = 0; s < M.ellW; ++s) {
+            const int enc = M.c2fEnc[(size_t)s * nC + c];
+            if (enc == 0) break;
+            const int nb = M.c2n[(size_t)s * nC + c];
+            if (nb >= nC) continue;  // a boundary face's ghost
+            const double pn = p[nb];
+            if (pn < lo) lo = pn;
+            if (pn > hi) hi = pn;
+        }
+        const double ratio = hi / lo;
+        const double fn = dMIN(1.0, dMAX(0.0, (2.6 - ratio) / (2.6 - 1.3)));
+        const double cf = dMAX(-1.0, dMIN(cos(fn * 3.141592653589), 1.0));
+        r = 0.5 * (1.0 - cf);
+    }
+    rtem[c] = r;
+}
+
+// SetGhostCellExceptInterface (Geo_UnstructGrid.cpp:1857-1881): a ghost
+// takes its owner's pressure factor.
+__global__ void __launch_bounds__(kBlock) k_ghost_copy(DevMesh M, double *__restrict__ v)
+{
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f < M.nBFace) v[M.nCell + f] = v[M.fle[f]];
 }
 
 // LoadResiduals (NSSolverUnstruct.cpp:4116): res = -rhs, and rhs is zero on
@@ -2596,6 +2678,15 @@ void launch_flux(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s, b
     case 128: launch_flux_t<true, 128>(m, k, f, s); break;
     default: launch_flux_t<true, 256>(m, k, f, s); break;
     }
+}
+
+void launch_pressure_factor(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s)
+{
+    if (!f.rtem || (k.entropyFix != 2 && k.entropyFix != 6) || m.nCell == 0) return;
+    hipLaunchKernelGGL(k_pressure_factor, dim3(nblk(m.nCell, kBlock)), dim3(kBlock), 0, s, m, k.entropyFix, f.q,
+                       f.rtem);
+    if (m.nBFace > 0)
+        hipLaunchKernelGGL(k_ghost_copy, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, f.rtem);
 }
 
 void launch_load_residuals(const DevMesh &m, DevFlow &f, hipStream_t s)

@@ -740,9 +740,11 @@ void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels
     // and LUplusDQ start from zero; the squared change of dq~ over each pair
     // (both half-sweeps) against the first pair's decides, on the host, after
     // every pair
-    hipMemsetAsync(f.dqT, 0, sizeof(double) * m.nTotal, s);
-    hipMemsetAsync(f.dfluxT, 0, sizeof(double) * m.nCell, s);
-    hipMemsetAsync(f.dqTPrev, 0, sizeof(double) * m.nCell, s);
+    // (an error of these calls stays the thread's last HIP error, which the
+    // caller's launch check reports)
+    (void)hipMemsetAsync(f.dqT, 0, sizeof(double) * m.nTotal, s);
+    (void)hipMemsetAsync(f.dfluxT, 0, sizeof(double) * m.nCell, s);
+    (void)hipMemsetAsync(f.dqTPrev, 0, sizeof(double) * m.nCell, s);
     double init = 0.0, eps = 1.1;
     log->done = 0;
     for (int it = 0; it < k.nSweeps; ++it) {
@@ -751,7 +753,7 @@ void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels
         launch_sa_ghost_dq(m, f, s);
         sa_sweep_dispatch(m, f, bwd, true, s);
         launch_sweep_change(f.dqT, m.nTotal, 1, m.nCell, f.dqTPrev, f.sweepParts, f.sweepNorm + 1, s);
-        hipMemcpyAsync(log->host, f.sweepNorm, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
+        (void)hipMemcpyAsync(log->host, f.sweepNorm, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
         if (hipStreamSynchronize(s) != hipSuccess) break;  // reported by the caller's launch check
         const double sn = log->host[0] + log->host[1];
         if (it < 8) log->normals[it] = sn;

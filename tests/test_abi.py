@@ -68,7 +68,7 @@ def test_mesh_create_rejects_bad_connectivity():
 
 
 @pytest.mark.parametrize("override", [dict(gradient=3),
-                                      dict(entropy_fix=2),
+                                      dict(entropy_fix=0), dict(entropy_fix=7),
                                       dict(viscous=2), dict(precision=7)])
 def test_unsupported_options_return_a_status(override):
     m = pa.Mesh.bump_hex(4, 2, 2, 0.05)

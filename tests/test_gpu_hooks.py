@@ -30,6 +30,7 @@ def _gpu_hooks(s, step, split=False, turb=False, ggnode=False):
     if ggnode:  # ComputeNodeValue of the previous PostSolve (Controller.cpp:5151-5198)
         s.compute_node_value()
     s.gamma_t()
+    s.pressure_factor()
     s.gradient()
     s.limiter()
     if split:
@@ -51,6 +52,7 @@ def _cpu_hooks(c, step, viscous, ggnode=False):
     if ggnode:
         c.compute_node_value()
     c.gamma_t()
+    c.pressure_factor()
     if viscous:
         c.viscosity()
     c.gradient()
@@ -72,6 +74,8 @@ CASES = {
                             dict(mach=0.5, cfl_start=10.0, cfl_end=10.0, gradient=2)),
     "hybrid_laminar_ggnode": (lambda: pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7),
                               dict(mach=0.3, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0, gradient=2)),
+    "hybrid_laminar_entropy6": (lambda: pa.Mesh.bump_hybrid(16, 4, 15, 1e-4, 10, 0.1, 7),
+                                dict(mach=0.8, viscous=1, reynolds=5e5, cfl_start=5.0, cfl_end=5.0, entropy_fix=6)),
     "wing_sa": (lambda: pa.Mesh.swept_wing(10, 5, 4, 1e-3, 3),
                 dict(mach=0.6, alpha_deg=2.0, viscous=3, reynolds=3e6, cfl_start=1.0, cfl_end=5.0,
                      cfl_vary_step=50)),

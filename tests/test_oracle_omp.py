@@ -2,7 +2,8 @@
 host's cores) gives exactly the sequential build's results: face loops
 applied per owned cell in face order, LU-SGS by dependency level,
 SolutionFix in cell order.  Iterations are compared bit for bit on hex,
-hybrid and wing meshes, every gradient form, laminar and SA, with more
+hybrid and wing meshes, every gradient form, laminar and SA, the
+pressure-sensor entropy fixes, with more
 threads than the partition has cells per thread on the small meshes."""
 import os
 
@@ -16,6 +17,8 @@ from conftest import perturbed_state
 CASES = {
     "hex_ggcell": (lambda: pa.Mesh.bump_hex(16, 8, 4, 0.05), dict(mach=0.5)),
     "hex_lsq_nolim": (lambda: pa.Mesh.bump_hex(12, 6, 4, 0.05), dict(mach=0.7, gradient=1, limiter=1)),
+    "hex_entropy2": (lambda: pa.Mesh.bump_hex(12, 6, 4, 0.08), dict(mach=0.8, entropy_fix=2)),
+    "hybrid_entropy6": (lambda: pa.Mesh.bump_hybrid(12, 3, 6, 1e-3, 5, 0.1, 3), dict(mach=0.8, entropy_fix=6)),
     "hybrid_laminar_ggnode": (lambda: pa.Mesh.bump_hybrid(12, 3, 6, 1e-3, 5, 0.1, 3),
                               dict(mach=0.3, viscous=1, reynolds=5e5, gradient=2)),
     "wing_sa": (lambda: pa.Mesh.swept_wing(10, 5, 4, 1e-3, 3),
