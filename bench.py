@@ -548,7 +548,15 @@ def mpnn_throughput(n_cases, dims, steps, layers=6):
     for _ in range(steps):
         net.forward()
     t_fwd = (time.perf_counter() - t0) / steps
-    print(f"[bench] mpnn: forward {t_fwd * 1e3:.1f} ms; training x {steps}", file=sys.stderr, flush=True)
+    net.set_forward_precision(True)
+    net.forward()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        net.forward()
+    t_fwd16 = (time.perf_counter() - t0) / steps
+    net.set_forward_precision(False)
+    print(f"[bench] mpnn: forward {t_fwd * 1e3:.1f} ms (BF16 hidden layers {t_fwd16 * 1e3:.1f} ms); "
+          f"training x {steps}", file=sys.stderr, flush=True)
     net.train(1, 1e-4)
     t0 = time.perf_counter()
     net.train(steps, 1e-4)
@@ -559,12 +567,16 @@ def mpnn_throughput(n_cases, dims, steps, layers=6):
     out = dict(workload=f"{n_cases} bump-channel hex cases of {dims[0]}x{dims[1]}x{dims[2]} as one graph, "
                         f"{layers} layers, width 128, FP32 (MFMA 32x32x2 f32), synthetic features",
                cells=n, steps=steps)
-    for key, t, train in (("forward", t_fwd, False), ("train_step", t_train, True)):
+    for key, t, train, pk in (("forward", t_fwd, False, peak), ("train_step", t_train, True, peak),
+                              ("forward_bf16", t_fwd16, False, 2.5e15)):
         fl = mpnn_dense_flops(layers, train) * n
         out[key] = dict(ms=round(t * 1e3, 3), cells_per_s=n / t, dense_tflop=fl / 1e12,
-                        achieved_tflops=round(fl / t / 1e12, 2), peak_tflops=peak / 1e12,
-                        frac=round(fl / t / peak, 4))
+                        achieved_tflops=round(fl / t / 1e12, 2), peak_tflops=pk / 1e12,
+                        frac=round(fl / t / pk, 4))
     out["forward"]["note"] = "wall time per call, including the copy of y to the host"
+    out["forward_bf16"]["note"] = ("hidden layers on v_mfma_f32_32x32x16_bf16 (BF16 operands, FP32 accumulation), "
+                                   "priced against the dense BF16 peak; input/output layers FP32; wall time per call "
+                                   "including the copy of y to the host")
     return out
 
 

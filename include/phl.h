@@ -481,6 +481,12 @@ phl_status phl_mpnn_set_graph_mesh(phl_mpnn *net, const phl_mesh *mesh);
 /* Features (and targets, may be NULL) of the current graph, kept on the device. */
 phl_status phl_mpnn_set_data(phl_mpnn *net, const float *X, const float *T);
 phl_status phl_mpnn_forward(phl_mpnn *net, float *Y);
+/* Precision of phl_mpnn_forward's hidden layers: 0 FP32 (default); 1 BF16
+ * products h A_l + m B_l (h, m and the weights rounded to BF16, FP32
+ * accumulation on v_mfma_f32_32x32x16_bf16; the input and output layers, the
+ * neighbour means, biases and residual adds stay FP32).  Training and
+ * phl_mpnn_gradient always run in FP32. */
+phl_status phl_mpnn_set_forward_precision(phl_mpnn *net, int bf16);
 /* Loss (mean of squared errors) and its gradient, flat like the parameters. */
 phl_status phl_mpnn_gradient(phl_mpnn *net, double *loss, float *grad);
 /* steps of Adam (lr, beta 0.9 / 0.999, eps 1e-8); loss_hist[steps] may be NULL. */

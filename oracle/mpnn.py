@@ -60,14 +60,25 @@ def neighbour_mean_transpose(g, nbr):
     return (w[idx] * valid[..., None]).sum(axis=1)
 
 
-def forward(p, X, nbr, keep=False):
-    """Returns y (and the activations needed by backward when keep)."""
+def bf16(a):
+    """Round to bfloat16 (8 significant bits, round to nearest even) through
+    FP32, as the device's v_cvt_pk_bf16_f32 does; returned as float64."""
+    u = np.ascontiguousarray(a, dtype=np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32).astype(np.float64)
+
+
+def forward(p, X, nbr, keep=False, bf16_hidden=False):
+    """Returns y (and the activations needed by backward when keep).
+    bf16_hidden: the hidden layers' products take h, m and A_l, B_l rounded
+    to bfloat16 (the device's BF16 inference form)."""
     z0 = X @ p["W0"] + p["b0"]
     h = np.maximum(z0, 0)
     hs, ms, zs = [h], [], []
+    rnd = bf16 if bf16_hidden else (lambda a: a)
     for l in range(p["A"].shape[0]):
         m, _ = neighbour_mean(h, nbr)
-        z = h @ p["A"][l] + m @ p["B"][l] + p["c"][l]
+        z = rnd(h) @ rnd(p["A"][l]) + rnd(m) @ rnd(p["B"][l]) + p["c"][l]
         h = h + np.maximum(z, 0)
         ms.append(m)
         zs.append(z)

@@ -192,6 +192,7 @@ def _declare(lib):
         "phl_mpnn_set_graph_mesh": (ctypes.c_int, [_P, _P]),
         "phl_mpnn_set_data": (ctypes.c_int, [_P, _F, _F]),
         "phl_mpnn_forward": (ctypes.c_int, [_P, _F]),
+        "phl_mpnn_set_forward_precision": (ctypes.c_int, [_P, ctypes.c_int]),
         "phl_mpnn_gradient": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double), _F]),
         "phl_mpnn_train": (ctypes.c_int, [_P, ctypes.c_int, ctypes.c_float, _D]),
     }
@@ -670,7 +671,8 @@ def _fptr(a):
 
 class Mpnn:
     """Warm-start message-passing network on the device (phl_mpnn): hidden
-    width 128, FP32, dense layers on MFMA, Adam training."""
+    width 128, FP32 (BF16 inference optional), dense layers on MFMA, Adam
+    training."""
 
     N_IN, N_OUT, HIDDEN = 8, 5, 128
 
@@ -729,6 +731,11 @@ class Mpnn:
         y = np.empty((self.n_nodes, self.N_OUT), np.float32)
         _check(_lib.phl_mpnn_forward(self._h, _fptr(y)))
         return y
+
+    def set_forward_precision(self, bf16):
+        """forward()'s hidden-layer products: False FP32, True BF16 (FP32
+        accumulation); training stays FP32."""
+        _check(_lib.phl_mpnn_set_forward_precision(self._h, 1 if bf16 else 0))
 
     def gradient(self):
         loss = ctypes.c_double()

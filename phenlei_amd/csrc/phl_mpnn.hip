@@ -156,6 +156,135 @@ __global__ void __launch_bounds__(kThreads, 2) k_dense(const float *__restrict__
         }
 }
 
+// ---- BF16 form of the hidden layers' products (inference) ----
+// z = h A + m B + c with h, m and the weights rounded to BF16 (round to
+// nearest even, v_cvt_pk_bf16_f32) and FP32 accumulation on
+// v_mfma_f32_32x32x16_bf16; h' = h + relu(z) in FP32.  The weights come
+// transposed ([128 columns][128 k], BF16) from k_weights_bf16, so that each
+// lane's eight consecutive k of a B operand are one 16-byte LDS read, as
+// are the eight k of its A operand.  Lane l supplies A[l % 32][8 (l / 32) ..
+// + 7] and B[8 (l / 32) .. + 7][l % 32] of a K = 16 step; the accumulator
+// layout is the one of the FP32 instruction (acc_row).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef __bf16 bf16x4 __attribute__((ext_vector_type(4)));
+constexpr int KB = 32;       // K chunk staged in LDS
+constexpr int KBP = KB + 8;  // LDS row length: 80-byte rows keep every 16-byte read aligned
+static_assert(kVec * 4 * kThreads == TR * KB && 2 * 8 * kThreads == H * KB, "BF16 chunk shares");
+
+struct ChunkB {
+    float4 a[kVec];  // A [TR][KB]: row q/8, k 4(q%8)..+3
+    uint4 w[2];      // W^T [H][KB]: column q/4, k 8(q%4)..+7
+};
+
+__device__ __forceinline__ void load_chunk_bf16(ChunkB &st, const float *__restrict__ A,
+                                                const __bf16 *__restrict__ Wt, int N, int row0, int k0)
+{
+#pragma unroll
+    for (int i = 0; i < kVec; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        const int row = row0 + (q >> 3), k = k0 + 4 * (q & 7);
+        st.a[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        if (row < N && k < H) st.a[i] = *(const float4 *)(A + (size_t)row * H + k);
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        const int k = k0 + 8 * (q & 3);
+        st.w[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (k < H) st.w[i] = *(const uint4 *)(Wt + (size_t)(q >> 2) * H + k);
+    }
+}
+
+__device__ __forceinline__ void store_chunk_bf16(const ChunkB &st, __bf16 (*Al)[KBP], __bf16 (*Wl)[KBP])
+{
+#pragma unroll
+    for (int i = 0; i < kVec; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        bf16x4 v;
+        v[0] = (__bf16)st.a[i].x;
+        v[1] = (__bf16)st.a[i].y;
+        v[2] = (__bf16)st.a[i].z;
+        v[3] = (__bf16)st.a[i].w;
+        *(bf16x4 *)&Al[q >> 3][4 * (q & 7)] = v;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+        const int q = threadIdx.x + i * kThreads;
+        *(uint4 *)&Wl[q >> 2][8 * (q & 3)] = st.w[i];
+    }
+}
+
+__device__ __forceinline__ void accumulate_bf16(f32x16 (&acc)[4], __bf16 (*Al)[KBP], __bf16 (*Wl)[KBP],
+                                                const float *__restrict__ A, const __bf16 *__restrict__ Wt, int N,
+                                                int row0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    ChunkB st;
+    load_chunk_bf16(st, A, Wt, N, row0, 0);
+    for (int k0 = 0; k0 < H; k0 += KB) {
+        __syncthreads();
+        store_chunk_bf16(st, Al, Wl);
+        __syncthreads();
+        load_chunk_bf16(st, A, Wt, N, row0, k0 + KB);  // past K: zeros, no loads
+#pragma unroll
+        for (int ks = 0; ks < KB; ks += 16) {
+            const bf16x8 a = *(const bf16x8 *)&Al[wave * 32 + (lane & 31)][ks + 8 * (lane >> 5)];
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const bf16x8 w = *(const bf16x8 *)&Wl[b * 32 + (lane & 31)][ks + 8 * (lane >> 5)];
+                acc[b] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a, w, acc[b], 0, 0, 0);
+            }
+        }
+    }
+}
+
+// out = H + relu(A1 W1 + A2 W2 + bias), A1 = h and A2 = m [N, 128] FP32, W1t,
+// W2t the layer's transposed BF16 weights.
+__global__ void __launch_bounds__(kThreads, 2) k_dense_bf16(const float *__restrict__ A1,
+                                                         const __bf16 *__restrict__ W1t,
+                                                         const float *__restrict__ A2,
+                                                         const __bf16 *__restrict__ W2t,
+                                                         const float *__restrict__ bias, int N,
+                                                         const float *__restrict__ Hres, float *__restrict__ out)
+{
+    __shared__ __attribute__((aligned(16))) __bf16 Al[TR][KBP];  // 16-byte reads and stores
+    __shared__ __attribute__((aligned(16))) __bf16 Wl[H][KBP];
+    const int row0 = blockIdx.x * TR;
+    f32x16 acc[4];
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) acc[b][r] = 0.f;
+    accumulate_bf16(acc, Al, Wl, A1, W1t, N, row0);
+    accumulate_bf16(acc, Al, Wl, A2, W2t, N, row0);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+    for (int b = 0; b < 4; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+            const int row = row0 + wave * 32 + acc_row(r, lane);
+            const int col = b * 32 + (lane & 31);
+            if (row >= N) continue;
+            const size_t o = (size_t)row * H + col;
+            const float z = acc[b][r] + bias[col];
+            out[o] = Hres[o] + (z > 0.f ? z : 0.f);
+        }
+}
+
+// The hidden layers' A_l and B_l [L][128 k][128 columns] FP32 to BF16,
+// transposed: out [L][2][128 columns][128 k] (A_l, then B_l).
+__global__ void __launch_bounds__(kThreads) k_weights_bf16(const float *__restrict__ A, const float *__restrict__ B,
+                                                           int L, __bf16 *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t HH = (size_t)H * H;
+    if (i >= (size_t)L * 2 * HH) return;
+    const size_t l = i / (2 * HH), r = i % (2 * HH);
+    const int which = (int)(r / HH), col = (int)((r % HH) / H), k = (int)(r % H);
+    const float *src = which ? B : A;
+    out[i] = (__bf16)src[l * HH + (size_t)k * H + col];
+}
+
 constexpr int kPer = 32 * H / kThreads;  // elements of a 32-row slab staged per thread
 
 // Partial of G[K, 128] = A[rows, K]^T D[rows, 128] over one chunk of
@@ -443,6 +572,18 @@ void dense(const float *A1, const float *W1, int K1, const float *A2, const floa
     else
         hipLaunchKernelGGL(k_dense<false>, dim3(nb(N, TR)), dim3(kThreads), 0, s, A1, W1, K1, A2, W2, bias, N, mode,
                            Hres, Z, out);
+}
+
+void weights_bf16(const float *A, const float *B, int L, __bf16 *out, hipStream_t s)
+{
+    const size_t n = (size_t)L * 2 * H * H;
+    hipLaunchKernelGGL(k_weights_bf16, dim3((unsigned)nb(n, kThreads)), dim3(kThreads), 0, s, A, B, L, out);
+}
+
+void dense_bf16(const float *h, const __bf16 *At, const float *m, const __bf16 *Bt, const float *bias, int N,
+                float *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dense_bf16, dim3(nb(N, TR)), dim3(kThreads), 0, s, h, At, m, Bt, bias, N, h, out);
 }
 
 int grad_parts(int N) { return nb(N, kRowChunk); }

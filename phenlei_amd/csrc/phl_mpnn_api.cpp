@@ -34,6 +34,9 @@ struct phl_mpnn {
     float *dh = nullptr, *dh2 = nullptr, *dz = nullptr, *dm = nullptr, *part = nullptr;
     double *lossPart = nullptr, *lossOut = nullptr;
     std::vector<void *> act;  // activation allocations (re-made per graph)
+    // BF16 inference form of the hidden layers (phl_mpnn_set_forward_precision)
+    int fwdBF16 = 0;
+    __bf16 *wt16 = nullptr;  // [L][2][128][128] transposed BF16 A_l, B_l
 };
 
 namespace {
@@ -112,6 +115,25 @@ void forward(phl_mpnn *n)
     out_layer(n->hs + NH * n->L, n->p + n->oWo, n->p + n->obo, N, n->y, s);
 }
 
+// The inference pass with the hidden layers' products in BF16: the weights
+// are converted once per call; z is not kept (no backward pass follows).
+void forward_bf16(phl_mpnn *n)
+{
+    using namespace phl::nn;
+    hipStream_t s = n->stream;
+    const int N = n->N;
+    const size_t NH = (size_t)N * kHidden, HH = (size_t)kHidden * kHidden;
+    weights_bf16(n->p + n->oA, n->p + n->oB, n->L, n->wt16, s);
+    dense(n->X, n->p + n->oW0, kIn, nullptr, nullptr, n->p + n->ob0, N, 1, nullptr, n->z0, n->hs, false, s);
+    for (int l = 0; l < n->L; ++l) {
+        float *h = n->hs + NH * l, *hn = n->hs + NH * (l + 1);
+        neighbour_mean(n->nbr, n->W, N, n->invDeg, h, n->ms + NH * l, s);
+        dense_bf16(h, n->wt16 + 2 * HH * l, n->ms + NH * l, n->wt16 + 2 * HH * l + HH, n->p + n->oc + kHidden * l, N,
+                   hn, s);
+    }
+    out_layer(n->hs + NH * n->L, n->p + n->oWo, n->p + n->obo, N, n->y, s);
+}
+
 // gradient of the loss into n->g (forward must have run; dy holds dL/dy)
 void backward(phl_mpnn *n)
 {
@@ -166,6 +188,7 @@ phl_status phl_mpnn_create(int layers, unsigned seed, int device, phl_mpnn **out
     if (e == hipSuccess) e = hipMalloc(&n->g, bytes);
     if (e == hipSuccess) e = hipMalloc(&n->m, bytes);
     if (e == hipSuccess) e = hipMalloc(&n->v, bytes);
+    if (e == hipSuccess) e = hipMalloc(&n->wt16, sizeof(__bf16) * 2 * HH * layers);
     if (e != hipSuccess) {
         phl_mpnn_destroy(n);
         return fail(PHL_ERR_HIP, std::string("phl_mpnn_create: ") + hipGetErrorString(e));
@@ -195,6 +218,7 @@ void phl_mpnn_destroy(phl_mpnn *n)
     free_list(n->act);
     for (float *q : {n->p, n->g, n->m, n->v})
         if (q) hipFree(q);
+    if (n->wt16) hipFree(n->wt16);
     if (n->nbr) hipFree(n->nbr);
     if (n->invDeg) hipFree(n->invDeg);
     if (n->stream) hipStreamDestroy(n->stream);
@@ -310,10 +334,18 @@ phl_status phl_mpnn_forward(phl_mpnn *n, float *Y)
 {
     if (!n || !Y || n->N == 0) return fail(PHL_ERR_ARG, "phl_mpnn_forward: no graph");
     HIPTRY(hipSetDevice(n->device));
-    forward(n);
+    if (n->fwdBF16) forward_bf16(n);
+    else forward(n);
     HIPTRY(hipGetLastError());
     HIPTRY(hipMemcpyAsync(Y, n->y, sizeof(float) * (size_t)n->N * kOut, hipMemcpyDeviceToHost, n->stream));
     HIPTRY(hipStreamSynchronize(n->stream));
+    return PHL_OK;
+}
+
+phl_status phl_mpnn_set_forward_precision(phl_mpnn *n, int bf16)
+{
+    if (!n || (bf16 != 0 && bf16 != 1)) return fail(PHL_ERR_ARG, "phl_mpnn_set_forward_precision: 0 (FP32) or 1 (BF16)");
+    n->fwdBF16 = bf16;
     return PHL_OK;
 }
 

@@ -14,6 +14,11 @@ constexpr int kOut = 5;       // predicted state
 // kernel); transW takes W as [128,K] (the backward products dZ W^T).
 void dense(const float *A1, const float *W1, int K1, const float *A2, const float *W2, const float *bias, int N,
            int mode, const float *Hres, float *Z, float *out, bool transW, hipStream_t s);
+// The BF16 inference form of a hidden layer: out = h + relu(h A + m B + c)
+// with BF16 products (FP32 accumulation); At, Bt from weights_bf16.
+void weights_bf16(const float *A, const float *B, int L, __bf16 *out, hipStream_t s);
+void dense_bf16(const float *h, const __bf16 *At, const float *m, const __bf16 *Bt, const float *bias, int N,
+                float *out, hipStream_t s);
 int grad_parts(int N);
 // out[128,128] = A[N,128]^T D[N,128] (part: grad_parts(N) * 128 * 128 floats)
 void wgrad(const float *A, const float *D, int N, float *part, float *out, hipStream_t s);

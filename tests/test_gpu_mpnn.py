@@ -50,6 +50,69 @@ def test_forward_matches_oracle():
     assert err <= 1e-5
 
 
+def test_bf16_forward_matches_oracle():
+    """The BF16 inference form (hidden-layer products from BF16 operands,
+    FP32 accumulation): within 2e-3 of the largest output of the oracle with
+    the same operands rounded to bfloat16 (the rest is the FP32 activations'
+    rounding before the BF16 rounding), and within 1e-2 of the FP64 network;
+    the FP32 form is unchanged by the switch."""
+    mesh, net, p, X, T = problem(seed=2)
+    nbr = mesh_neighbours(mesh)
+    y32 = net.forward()
+    net.set_forward_precision(True)
+    y16 = net.forward()
+    net.set_forward_precision(False)
+    assert np.array_equal(net.forward(), y32)
+    yb = mpnn.forward(p, X, nbr, bf16_hidden=True)
+    yr = mpnn.forward(p, X, nbr)
+    sc = np.max(np.abs(yr))
+    eb, er = np.max(np.abs(y16 - yb)) / sc, np.max(np.abs(y16 - yr)) / sc
+    print("bf16 forward: vs bf16 oracle", eb, "vs fp64 oracle", er)
+    assert eb <= 2e-3 and er <= 1e-2
+    assert not np.array_equal(y16, y32)
+
+
+def torus(nx, ny):
+    """Every node has exactly four neighbours (periodic grid), so a
+    neighbour mean divides by 4 exactly."""
+    i = np.arange(nx * ny)
+    x, y = i % nx, i // nx
+    return np.stack([(x + 1) % nx + nx * y, (x - 1) % nx + nx * y, x + nx * ((y + 1) % ny),
+                     x + nx * ((y - 1) % ny)], axis=1)
+
+
+def test_bf16_forward_is_exact_on_dyadic_values():
+    """Pins the BF16 MFMA operand layout: with features in {0, 1} and
+    weights in {0, +-1/4, +-1/2, +-1}, every activation is a short dyadic
+    number that BF16 holds exactly and every sum is exact in FP32, so one
+    hidden layer's BF16 form, its FP32 form and the FP64 oracle agree bit for
+    bit; a misplaced row, column or k of either operand would not.  2,010
+    nodes: 15 full 128-row tiles and a partial one."""
+    nx, ny = 30, 67
+    nbr = torus(nx, ny)
+    n = nx * ny
+    rng = np.random.default_rng(7)
+    net = pa.Mpnn(layers=1, seed=0)
+    p = net.get_params()
+    p["W0"] = rng.choice([0.0, 0.5, -0.5, 1.0, -1.0], p["W0"].shape).astype(np.float32)
+    p["A"] = rng.choice([0.0, 0.25, -0.25, 0.5, -0.5], p["A"].shape).astype(np.float32)
+    p["B"] = rng.choice([0.0, 0.25, -0.25, 0.5, -0.5], p["B"].shape).astype(np.float32)
+    p["Wo"] = rng.choice([0.0, 0.5, -0.5], p["Wo"].shape).astype(np.float32)
+    for k in ("b0", "c", "bo"):
+        p[k] = np.zeros_like(p[k])
+    net.set_params(p)
+    net.set_graph(nbr.astype(np.int32))
+    X = rng.integers(0, 2, (n, 8)).astype(np.float32)
+    net.set_data(X)
+    y32 = net.forward()
+    net.set_forward_precision(True)
+    y16 = net.forward()
+    yr = mpnn.forward({k: v.astype(np.float64) for k, v in p.items()}, X.astype(np.float64), nbr)
+    assert np.abs(yr).max() > 10.0  # the sums are not trivial
+    assert np.array_equal(y32.astype(np.float64), yr)
+    assert np.array_equal(y16, y32)
+
+
 def test_gradient_matches_oracle():
     mesh, net, p, X, T = problem(seed=1)
     nbr = mesh_neighbours(mesh)

@@ -110,3 +110,29 @@ def test_gather_transpose_needs_symmetric_lists():
         nbr[i, 0], nbr[i + 1, 1] = i + 1, i
     g = np.random.default_rng(2).standard_normal((6, 3))
     assert not np.allclose(mpnn.neighbour_mean_transpose(g, nbr), _scatter_transpose(g, nbr))
+
+
+def test_bf16_rounding_is_nearest_even():
+    """mpnn.bf16 against a brute-force nearest search over the bfloat16
+    values around each input (8 significant bits; ties to the even
+    mantissa), on values spread over many binades, exact bfloat16 values and
+    exact ties."""
+    rng = np.random.default_rng(3)
+    x = np.concatenate([rng.standard_normal(2000) * 10.0 ** rng.integers(-6, 6, 2000),
+                        [1.0, -2.0, 0.375, 1.0 + 2.0 ** -8, 1.0 + 3 * 2.0 ** -8, -(3.0 + 2.0 ** -7), 0.0]])
+    x = x.astype(np.float32).astype(np.float64)
+    got = mpnn.bf16(x)
+    for v, g in zip(x, got):
+        if v == 0.0:
+            assert g == 0.0
+            continue
+        e = np.floor(np.log2(abs(v)))
+        ulp = 2.0 ** (e - 7)
+        lo = np.floor(v / ulp) * ulp
+        cands = [lo, lo + ulp]
+        d = [abs(v - c) for c in cands]
+        if d[0] == d[1]:  # a tie: the candidate with the even last mantissa bit
+            want = cands[0] if int(round(cands[0] / ulp)) % 2 == 0 else cands[1]
+        else:
+            want = cands[int(np.argmin(d))]
+        assert g == want, (v, g, want)
