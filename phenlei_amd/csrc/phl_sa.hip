@@ -243,13 +243,15 @@ __global__ void __launch_bounds__(kBlock) k_sa_gradient(DevMesh M, const double 
 // -------------------------------------------------------------- residual
 // Per real cell: spectrum initialisation, source (and its implicit part),
 // convective and diffusive faces.  GT: storage type of the NS gradients.
+// The cell's own part: the source term times the volume starts the residual
+// (acc), the time-step term and the source's implicit part start the
+// spectrum (spec).
 template <typename GT>
-__device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts &K, const double *__restrict__ q,
-                                                 const double *__restrict__ visl, const GatherRec *__restrict__ rec,
-                                                 const double *__restrict__ nut, const double *__restrict__ dt,
-                                                 int c, double &resOut, double &specOut)
+__device__ __forceinline__ void sa_source_cell(const DevMesh &M, const Consts &K, const double *__restrict__ q,
+                                               const double *__restrict__ visl, const GatherRec *__restrict__ rec,
+                                               const double *__restrict__ nut, const double *__restrict__ dt, int c,
+                                               double &acc, double &spec)
 {
-    const int nC = M.nCell, nT = M.nTotal;
     const double oRe = K.oRe;
     // --- source (SourceFluxOneEquationOriginal, NSMB: no cb2 |grad|^2 term)
     const double kap2 = kKarm * kKarm, oKap2 = 1.0 / kap2;
@@ -296,7 +298,7 @@ __device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts 
         wallDest = -destrAlpha * cw1Kap2 * nuetRs;
     }
     const double source = production - wallDest;
-    double acc = -0.0;
+    acc = -0.0;
     acc += (source) * volume;
     const double dfv2dk = (3.0 * cv13 * pow(fv1 / (xsi + kSMALL), 2) - 1.0) / pow(1.0 + xsi * fv1, 2);
     const double dsdnu = oRe * oKap2 * oDistance2 * (fv2 + xsi * dfv2dk);
@@ -318,9 +320,21 @@ __device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts 
     const double prdep = prodp - destp;
     const double part1 = -0.5 * (prde - dABS(prde));
     const double part2 = -0.5 * (prdep - dABS(prdep)) * nueSA;
-    double spec = 0.0;
+    spec = 0.0;
     spec = 1.0 / (K.turbCflScale * dt[c] + kSMALL);
     spec += (part1 + part2) * volume;
+}
+
+template <typename GT>
+__device__ __forceinline__ void sa_residual_cell(const DevMesh &M, const Consts &K, const double *__restrict__ q,
+                                                 const double *__restrict__ visl, const GatherRec *__restrict__ rec,
+                                                 const double *__restrict__ nut, const double *__restrict__ dt,
+                                                 int c, double &resOut, double &specOut)
+{
+    const int nC = M.nCell, nT = M.nTotal;
+    const double oRe = K.oRe;
+    double acc, spec;
+    sa_source_cell<GT>(M, K, q, visl, rec, nut, dt, c, acc, spec);
 
     // --- convection: first-order upwind (NNDFlux, non-conservative form)
     for (int s = 0; s < M.ellW; ++s) {
@@ -508,9 +522,171 @@ __global__ void __launch_bounds__(kBlock) k_sa_spectrum(DevMesh M, Consts K, con
     specT[c] = spec;
 }
 
-// Residual and spectrum of one relaxation in one pass over a cell's faces
-// data (the neighbours loaded for the residual are still cached for the
-// spectrum); the values are those of k_sa_residual followed by k_sa_spectrum.
+// Residual and spectrum of a cell with at most kSaW faces in one pass over
+// its faces: each face's states are loaded once for the convective,
+// diffusive and spectral terms.  The sums keep the two-pass order (the
+// source, every convective term, then every diffusive term; the spectrum in
+// face order), the diffusive terms being held until the convective ones are
+// in, so the values are those of sa_residual_cell followed by
+// sa_spectrum_cell.
+constexpr int kSaW = 6;
+template <typename GT>
+__device__ __forceinline__ void sa_residual_spectrum_cell(const DevMesh &M, const Consts &K,
+                                                          const double *__restrict__ q,
+                                                          const double *__restrict__ visl,
+                                                          const GatherRec *__restrict__ rec,
+                                                          const double *__restrict__ nut,
+                                                          const double *__restrict__ dt, int c, double &resOut,
+                                                          double &specOut, double *__restrict__ mat)
+{
+    const int nC = M.nCell, nT = M.nTotal;
+    const double oRe = K.oRe, osigma = 1.0 / kSigma;
+    double acc, spec;
+    sa_source_cell<GT>(M, K, q, visl, rec, nut, dt, c, acc, spec);
+    double d1[kSaW], d2[kSaW];
+    bool left[kSaW];
+    int n = 0;
+#pragma unroll
+    for (int s = 0; s < kSaW; ++s) {
+        if (s >= M.ellW) break;
+        const int enc = M.c2fEnc[(size_t)s * nC + c];
+        if (enc == 0) break;
+        n = s + 1;
+        const bool isL = enc > 0;
+        left[s] = isL;
+        const int f = (isL ? enc : -enc) - 1;
+        const int nb = M.c2n[(size_t)s * nC + c];
+        const int le = isL ? c : nb, re = isL ? nb : c;
+        const FaceRec fr = M.face[f];
+        const bool bnd = f < M.nBFace;
+        const int bc = bnd ? M.bcType[f] : 0;
+        const double rhoL = q[le], uL0 = q[(size_t)nT + le], vL0 = q[(size_t)2 * nT + le],
+                     wL0 = q[(size_t)3 * nT + le];
+        const double rhoR = q[re], uR0 = q[(size_t)nT + re], vR0 = q[(size_t)2 * nT + re],
+                     wR0 = q[(size_t)3 * nT + re];
+        const double nutL = nut[le], nutR = nut[re], vislL = visl[le], vislR = visl[re];
+        const double volL = M.vol[le], volR = M.vol[re];
+        const double xL = M.xcc[le], yL = M.ycc[le], zL = M.zcc[le];
+        const double xR = M.xcc[re], yR = M.ycc[re], zR = M.zcc[re];
+        const unsigned vflag = M.visFlag[f];
+
+        // convection: first-order upwind (NNDFlux, non-conservative form)
+        {
+            double uL = uL0, vL = vL0, wL = wL0, uR = uR0, vR = vR0, wR = wR0;
+            double tl = nutL, tr = nutR;
+            if (bnd) {
+                double t = 0.5 * (uR + uL); uL = t; uR = t;
+                t = 0.5 * (vR + vL); vL = t; vR = t;
+                t = 0.5 * (wR + wL); wL = t; wR = t;
+                if (bc == kBC_SYMM) {
+                    tr = tl;
+                } else {
+                    t = 0.5 * (tr + tl);
+                    tr = t; tl = t;
+                }
+            }
+            const double vnL = fr.nx * uL + fr.ny * vL + fr.nz * wL - 0.0;
+            const double vnR = fr.nx * uR + fr.ny * vR + fr.nz * wR - 0.0;
+            const double vnLL = 0.5 * (vnL + dABS(vnL));
+            const double vnRR = 0.5 * (vnR - dABS(vnR));
+            const double fl = (vnLL * (1.0 * tl) + vnRR * (1.0 * tr)) * fr.area;
+            if (isL) acc -= fl;
+            else acc += fl;
+        }
+
+        // diffusion: GetVisFaceValue mlt and ComputeVisflux, NSMB branch
+        {
+            double nueLeft = nutL, nueRight = nutR;
+            if (bnd) {
+                if (nueLeft < 0.0) {
+                    nueLeft = positive_sa(nueLeft, vislL, rhoL);
+                    if (bc == kBC_WALL) nueRight = -nueLeft;
+                }
+            } else {
+                nueLeft = positive_sa(nueLeft, vislL, rhoL);
+                nueRight = positive_sa(nueRight, vislR, rhoR);
+            }
+            const double mlt = osigma * 0.5 * (vislL / rhoL + vislR / rhoR + nueLeft + nueRight);
+            const double nxs = fr.nx, nys = fr.ny, nzs = fr.nz;
+            const double dxL = xL - fr.xc, dyL = yL - fr.yc, dzL = zL - fr.zc;
+            const double dxR = xR - fr.xc, dyR = yR - fr.yc, dzR = zR - fr.zc;
+            const double dL = nxs * dxL + nys * dyL + nzs * dzL;
+            const double dR = nxs * dxR + nys * dyR + nzs * dzR;
+            double dfdn = 0.0;
+            if ((vflag & 4u) && dABS(dL) > kTINY && dABS(dR) > kTINY) {
+                const double f1 = nutL, f2 = nutR;
+                const double fMid = 0.5 * (f1 + f2);
+                const double dfd1 = (f1 - fMid) / dL;
+                const double dfd2 = (f2 - fMid) / dR;
+                const double dtmp = dL * dL + dR * dR;
+                const double weightL = dL * dL / dtmp;
+                const double weightR = 1.0 - weightL;
+                dfdn = weightL * dfd1 + weightR * dfd2;
+                if (bnd && bc != kBC_SYMM) {
+                    const double normalDist = (xR - xL) * nxs + (yR - yL) * nys + (zR - zL) * nzs;
+                    dfdn = (f2 - f1) / normalDist;
+                }
+            }
+            d1[s] = -oRe * (1.0 + kCb2) * mlt * dfdn * fr.area;
+            const double flux2 = oRe * kCb2 * osigma * dfdn * fr.area;
+            d2[s] = isL ? flux2 * (vislL / rhoL + nueLeft) : flux2 * (vislR / rhoR + nueRight);
+        }
+
+        // spectrum and face matrices (SpectrumRadiusOfOneEquation, NSMB)
+        {
+            const double vnL = fr.nx * uL0 + fr.ny * vL0 + fr.nz * wL0 - 0.0;
+            const double vnR = fr.nx * uR0 + fr.ny * vR0 + fr.nz * wR0 - 0.0;
+            const double rcL = 0 * 1.0 + (1 - 0) * rhoL, rcR = 0 * 1.0 + (1 - 0) * rhoR;
+            const double invL = vnL > 0.0 ? rcL * vnL * fr.area : 0.0;
+            const double invR = vnR < 0.0 ? rcR * vnR * fr.area : 0.0;
+            const double orL = 1.0 / (rhoL + kSMALL), orR = 1.0 / (rhoR + kSMALL);
+            const double nueL = positive_sa(nutL, vislL, rhoL);
+            double nueR = positive_sa(nutR, vislR, rhoR);
+            if (bnd && bc == kBC_WALL) nueR = -nueL;
+            const double nueffL = vislL * orL + dABS(nueL);
+            const double nueffR = vislR * orR + dABS(nueR);
+            const double nueff = 0.5 * (nueffL + nueffR);
+            const double volume = 0.5 * (volL + volR);
+            const double oVolume = 1.0 / volume;
+            const double ns2 = fr.area * fr.area;
+            const double term = oRe * oVolume * (1.0 + kCb2) * (osigma * nueff * ns2);
+            if (isL) {
+                const double coeffL = oRe * oVolume * ns2 * (kCb2 * osigma * nueffL);
+                const double coeffR = oRe * oVolume * ns2 * (kCb2 * osigma * nueffR);
+                spec += invL;
+                spec += term;
+                double mR = invR, mL = -invL;
+                mL += -term;
+                mR += -term;
+                mL += coeffL;
+                mR += coeffR;
+                mat[(size_t)f * 2] = mR;
+                mat[(size_t)f * 2 + 1] = mL;
+            } else {
+                spec -= invR;
+                spec += term;
+            }
+        }
+    }
+#pragma unroll
+    for (int s = 0; s < kSaW; ++s) {
+        if (s >= n) break;
+        if (left[s]) {
+            acc -= d1[s];
+            acc -= d2[s];
+        } else {
+            acc += d1[s];
+            acc += d2[s];
+        }
+    }
+    resOut = acc;
+    specOut = spec;
+}
+
+// Residual and spectrum of one relaxation in one kernel: the one-pass form
+// for cells of at most kSaW faces, otherwise the residual's passes followed
+// by the spectrum's; the values are those of k_sa_residual followed by
+// k_sa_spectrum.
 template <typename GT>
 __global__ void __launch_bounds__(kBlock) k_sa_residual_spectrum(DevMesh M, Consts K, const double *__restrict__ q,
                                                                  const double *__restrict__ visl,
@@ -523,6 +699,12 @@ __global__ void __launch_bounds__(kBlock) k_sa_residual_spectrum(DevMesh M, Cons
     const int c = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (c >= M.nCell) return;
     double r, spec;
+    if (M.ellW <= kSaW) {
+        sa_residual_spectrum_cell<GT>(M, K, q, visl, grad, nut, dt, c, r, spec, mat);
+        resT[c] = r;
+        specT[c] = spec;
+        return;
+    }
     sa_residual_cell<GT>(M, K, q, visl, grad, nut, dt, c, r, spec);
     resT[c] = r;
     sa_spectrum_cell(M, K, q, visl, nut, c, spec, mat);
