@@ -7,6 +7,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdint>
 #include <utility>
 #include <cmath>
 #include <cstdio>
@@ -393,6 +394,23 @@ phl_status copy_field(phl_ctx *c, const double *dsrc, int rows, int rowLen, int 
     HIPTRY(hipStreamSynchronize(c->stream));
     std::vector<double> tmp;
     for (int r = 0; r < rows && rowLen >= c->m.nCell; ++r) row_to_mesh(c, out + (size_t)r * outRowLen, tmp);
+    return PHL_OK;
+}
+
+// The five components of a [n][kRow] device array (n rows, storage order)
+// into five host rows of outRowLen values in mesh order (zero padded).
+phl_status copy_rows(phl_ctx *c, const double *dsrc, int n, int outRowLen, double *out)
+{
+    std::vector<double> h((size_t)phl::kRow * n);
+    if (n > 0)
+        HIPTRY(hipMemcpyAsync(h.data(), dsrc, sizeof(double) * h.size(), hipMemcpyDeviceToHost, c->stream));
+    HIPTRY(hipStreamSynchronize(c->stream));
+    for (int m = 0; m < 5; ++m) {
+        double *o = out + (size_t)m * outRowLen;
+        for (int i = 0; i < outRowLen; ++i) o[i] = i < n ? h[(size_t)i * phl::kRow + m] : 0.0;
+    }
+    std::vector<double> tmp;
+    for (int m = 0; m < 5 && n >= c->m.nCell; ++m) row_to_mesh(c, out + (size_t)m * outRowLen, tmp);
     return PHL_OK;
 }
 
@@ -1130,7 +1148,10 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         std::vector<int> row, nbs;
         build_sweep_order(nC, c2fPtr, c2fS, c2n, isLeft, dir == 0 ? cellsF : cellsB, dir == 0, lv.so.W, lv.so.Wpad,
                           row, nbs);
-        if (p.sweep_form != 1 && lv.so.Wpad <= 6)
+        // the dataflow sweep addresses dq rows through a buffer descriptor
+        // with a 32-bit byte range; larger meshes take the level form
+        const bool rowsAddressable = (size_t)m.nTotal() * phl::kRow * sizeof(double) <= (size_t)INT32_MAX;
+        if (p.sweep_form != 1 && lv.so.Wpad <= 6 && rowsAddressable)
             TRY(build_flow_order(c, nC, dir == 0 ? cellsF : cellsB, dir == 0 ? c->hfwdPtr : c->hbwdPtr, row, nbs,
                                  lv.so.Wpad, slotOf, lv.flow));
         for (int &v : nbs) v = S(v);  // unused slots hold 0, a valid storage index
@@ -1220,8 +1241,8 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
         HIPTRY(hipMemset(fl.rtem, 0, sizeof(double) * nT));
     }
     k.rtem = fl.rtem;
-    TRY(dalloc(c, &fl.dq, (size_t)5 * nT));
-    TRY(dalloc(c, &fl.dflux, (size_t)5 * nC));
+    TRY(dalloc(c, &fl.dq, (size_t)phl::kRow * nT));
+    TRY(dalloc(c, &fl.dflux, (size_t)phl::kRow * nC));
     TRY(dalloc(c, &fl.specInv, (size_t)nC));
     TRY(dalloc(c, &fl.specVis, (size_t)nC));
     TRY(dalloc(c, &fl.dtLoc, (size_t)nC));
@@ -1342,8 +1363,8 @@ static phl_status create_impl(phl_ctx *c, const phl::Mesh &m)
     HIPTRY(hipMemset(fl.res, 0, sizeof(double) * 5 * nC));
     HIPTRY(hipMemset(fl.grad, 0, sizeof(double) * 15 * nC));
     HIPTRY(hipMemset(fl.gradT, 0, sizeof(double) * 3 * nC));
-    HIPTRY(hipMemset(fl.dq, 0, sizeof(double) * 5 * nT));
-    HIPTRY(hipMemset(fl.dflux, 0, sizeof(double) * 5 * nC));
+    HIPTRY(hipMemset(fl.dq, 0, sizeof(double) * phl::kRow * nT));
+    HIPTRY(hipMemset(fl.dflux, 0, sizeof(double) * phl::kRow * nC));
     HIPTRY(hipMemset(fl.specVis, 0, sizeof(double) * nC));
     HIPTRY(hipMemset(fl.negFlag, 0, (size_t)nC));
     HIPTRY(hipMemset(fl.negCount, 0, sizeof(int)));
@@ -1511,7 +1532,7 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
         return copy_field(c, f.grad + off, 5, nC, nT, out);
     }
     case PHL_FIELD_LIMIT: return copy_field(c, f.lim, 1, nT, nT, out);
-    case PHL_FIELD_DQ: return copy_field(c, f.dq, 5, nT, nT, out);
+    case PHL_FIELD_DQ: return copy_rows(c, f.dq, nT, nT, out);
     case PHL_FIELD_SPEC_INV: return copy_field(c, f.specInv, 1, nC, nT, out);
     case PHL_FIELD_SPEC_VIS: return copy_field(c, f.specVis, 1, nC, nT, out);
     case PHL_FIELD_DT: return copy_field(c, f.dt, 1, nC, nT, out);
@@ -1528,7 +1549,7 @@ phl_status phl_get_field(phl_ctx *c, int field, double *out)
         if (c->k.mixed) return copy_field_f32(c, f.gradTF + off, 1, nC, nT, out);
         return copy_field(c, f.gradT + off, 1, nC, nT, out);
     }
-    case PHL_FIELD_DELTA_FLUX: return copy_field(c, f.dflux, 5, nC, nT, out);
+    case PHL_FIELD_DELTA_FLUX: return copy_rows(c, f.dflux, nC, nT, out);
     case PHL_FIELD_VIST: return copy_field(c, f.vist, 1, nT, nT, out);
     case PHL_FIELD_PRESSURE_FACTOR:
         if (!f.rtem) {
@@ -1743,17 +1764,17 @@ static void ns_sweep_pairs(phl_ctx *c, hipStream_t s)
 {
     phl::DevFlow &f = c->f;
     const int nT = c->m.nTotal, nC = c->m.nCell;
-    hipMemsetAsync(f.dq, 0, sizeof(double) * 5 * nT, s);
-    hipMemsetAsync(f.dflux, 0, sizeof(double) * 5 * nC, s);
+    hipMemsetAsync(f.dq, 0, sizeof(double) * phl::kRow * nT, s);
+    hipMemsetAsync(f.dflux, 0, sizeof(double) * phl::kRow * nC, s);
     hipMemsetAsync(f.dqPrev, 0, sizeof(double) * 5 * nC, s);
     double init = 0.0, eps = 1.1;
     c->nsLog.done = 0;
     for (int it = 0; it < c->k.nSweeps; ++it) {
         phl::launch_sweep(c->m, c->k, f, c->fwd, true, it == 0, s);
-        phl::launch_sweep_change(f.dq, (size_t)nT, 5, nC, f.dqPrev, f.sweepParts, f.sweepNorm, s);
+        phl::launch_sweep_change(f.dq, phl::kRow, 1, 5, nC, f.dqPrev, f.sweepParts, f.sweepNorm, s);
         phl::launch_ghost_dq(c->m, c->k, f, s);
         phl::launch_sweep(c->m, c->k, f, c->bwd, false, false, s);
-        phl::launch_sweep_change(f.dq, (size_t)nT, 5, nC, f.dqPrev, f.sweepParts, f.sweepNorm + 1, s);
+        phl::launch_sweep_change(f.dq, phl::kRow, 1, 5, nC, f.dqPrev, f.sweepParts, f.sweepNorm + 1, s);
         hipMemcpyAsync(c->nsLog.host, f.sweepNorm, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
         if (hipStreamSynchronize(s) != hipSuccess) return;  // reported by the caller's launch check
         const double sn = c->nsLog.host[0] + c->nsLog.host[1];

@@ -151,6 +151,12 @@ struct alignas(64) GatherRec {
 };
 static_assert(sizeof(GatherRec) == 192, "GatherRec is three 64-byte sectors");
 
+// The LU-SGS increments dq and off-diagonal sums are per-cell rows of kRow
+// doubles (one 64-byte line): a dataflow sweep reads a neighbour's five
+// increments with two 16-byte and one 8-byte device-scope load from one
+// line, and publishes its own with three such stores.
+constexpr int kRow = 8;
+
 // Mutable flow state on the device.
 struct DevFlow {
     double *q;        // [5][nTotal] primitive variables (current)
@@ -162,8 +168,8 @@ struct DevFlow {
     float *gradTF;    // [3][nCell] mixed form
     double *lim;      // [nTotal]
     double *rtem;     // [nTotal] pressure factor (entropy fix 2, 6; null otherwise)
-    double *dq;       // [5][nTotal]
-    double *dflux;    // [5][nCell]
+    double *dq;       // [nTotal][kRow]: the five increments of a cell in one 64-byte line
+    double *dflux;    // [nCell][kRow]: the off-diagonal sums of the last sweep
     double *casePart; // batched contexts: [nCases][32][6] partial residual sums and dt minima
     int *sweepFlag;   // [chunks] dataflow sweep: epoch of the sweep that last published the chunk's dq
     int *sweepCtl;    // [4] dataflow sweep: ticket, epoch, waves that gave up waiting
@@ -292,8 +298,8 @@ void launch_forces(const DevMesh &m, const Consts &k, DevFlow &f, const double *
 
 // Spalart-Allmaras (phl_sa.hip).  tri: [nTri][9] wall triangles (face centre
 // and two consecutive face nodes).
-void launch_sweep_change(const double *dq, size_t ld, int nv, int nC, double *prev, double *parts, double *out,
-                         hipStream_t s);
+void launch_sweep_change(const double *dq, size_t cs, size_t ms, int nv, int nC, double *prev, double *parts,
+                         double *out, hipStream_t s);
 void launch_wall_distance(const double *xcc, const double *ycc, const double *zcc, int nCell, const double *tri,
                           int nTri, bool edges, double *wd, hipStream_t s);
 void launch_sa_boundary(const DevMesh &m, const Consts &k, DevFlow &f, hipStream_t s);

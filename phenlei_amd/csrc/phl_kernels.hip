@@ -245,6 +245,23 @@ __device__ __forceinline__ void load5(const double *__restrict__ a, int n, int i
     for (int m = 0; m < 5; ++m) v[m] = a[(size_t)m * n + i];
 }
 
+// The first five values of row i of a [n][kRow] array (one 64-byte line).
+__device__ __forceinline__ void load_row5(const double *a, int i, double *v)
+{
+    const double *r = a + (size_t)i * kRow;
+    const double2 x = *reinterpret_cast<const double2 *>(r);
+    const double2 y = *reinterpret_cast<const double2 *>(r + 2);
+    v[0] = x.x; v[1] = x.y; v[2] = y.x; v[3] = y.y; v[4] = r[4];
+}
+
+__device__ __forceinline__ void store_row5(double *a, int i, const double *v)
+{
+    double *r = a + (size_t)i * kRow;
+    *reinterpret_cast<double2 *>(r) = make_double2(v[0], v[1]);
+    *reinterpret_cast<double2 *>(r + 2) = make_double2(v[2], v[3]);
+    r[4] = v[4];
+}
+
 // ------------------------------------------------------------------ BCs
 __global__ void __launch_bounds__(kBlock) k_boundary(DevMesh M, Consts K0, double *__restrict__ q,
                                                      GatherRec *__restrict__ rec)
@@ -1654,17 +1671,19 @@ __global__ void __launch_bounds__(kBlock) k_norm_sum5(const double *__restrict__
 // The squared change of dq since the last call (the sweepNormal of
 // Controller::LUSGS for nLUSGSSweeps > 1, Controller.cpp:1920-1950; the
 // reference adds SQR(dq - dqOld) cell by cell in its sweeps): per block of
-// kBlock real cells, over nv rows of dq [nv][ld], and prev [nv][nC] takes the
-// new dq.  k_sum_into then adds the block partials in a fixed order.
-__global__ void __launch_bounds__(kBlock) k_sweep_change(const double *__restrict__ dq, size_t ld, int nv, int nC,
-                                                         double *__restrict__ prev, double *__restrict__ parts)
+// kBlock real cells, over nv components (component m of cell c at
+// dq[c * cs + m * ms]), and prev [nv][nC] takes the new dq.  k_sum_into then
+// adds the block partials in a fixed order.
+__global__ void __launch_bounds__(kBlock) k_sweep_change(const double *__restrict__ dq, size_t cs, size_t ms, int nv,
+                                                         int nC, double *__restrict__ prev,
+                                                         double *__restrict__ parts)
 {
     __shared__ double red[kBlock];
     const int c = blockIdx.x * kBlock + (int)threadIdx.x;
     double acc = 0.0;
     if (c < nC)
         for (int m = 0; m < nv; ++m) {
-            const double v = dq[(size_t)m * ld + c];
+            const double v = dq[(size_t)c * cs + (size_t)m * ms];
             const double d = v - prev[(size_t)m * nC + c];
             acc += d * d;
             prev[(size_t)m * nC + c] = v;
@@ -1815,20 +1834,20 @@ __device__ __forceinline__ void sweep_finish(const DevMesh &M, int c, bool useU,
                                              const double *__restrict__ res, const double *__restrict__ diag,
                                              double *dq, double *dflux)
 {
-    const int nC = M.nCell, nT = M.nTotal;
+    const int nC = M.nCell;
     // every load before the first store, so they are issued together
+    double r[5], U[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    load5(res, nC, c, r);
     const double D = diag[c];
-    double r[5], U[5];
+    if (useU) load_row5(dflux, c, U);
+    double v[5], w[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        r[m] = res[(size_t)m * nC + c];
-        U[m] = useU ? dflux[(size_t)m * nC + c] : 0.0;
+        v[m] = (r[m] - U[m] - rhs0[m]) / D;
+        w[m] = 0.0 + rhs0[m];
     }
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        dq[(size_t)m * nT + c] = (r[m] - U[m] - rhs0[m]) / D;
-        dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
-    }
+    store_row5(dq, c, v);
+    store_row5(dflux, c, w);
 }
 
 __device__ __forceinline__ void load_coef(const double *__restrict__ coef, int f, bool isL, double *co)
@@ -1851,10 +1870,10 @@ constexpr int kW = 6;
 constexpr int kSweepBlock = 512;
 template <bool FWD>
 __device__ __forceinline__ void sweep_cell_batched(const DevMesh &M, const Consts &K, int c, bool useU,
-                                                   const double *__restrict__ coef, const double *__restrict__ res,
-                                                   const double *__restrict__ diag, double *dq, double *dflux)
+                                                   const double *__restrict__ coef, const double *__restrict__ res, const double *__restrict__ diag,
+                                                   double *dq, double *dflux)
 {
-    const int nC = M.nCell, nT = M.nTotal;
+    const int nC = M.nCell;
     int enc[kW], nb[kW];
 #pragma unroll
     for (int s = 0; s < kW; ++s) {
@@ -1872,7 +1891,7 @@ __device__ __forceinline__ void sweep_cell_batched(const DevMesh &M, const Const
         const bool isL = enc[s] > 0;
         const int f = act[s] ? (isL ? enc[s] : -enc[s]) - 1 : 0;
         load_coef(coef, f, isL, co[s]);
-        load5(dq, nT, act[s] ? nb[s] : 0, dqn[s]);
+        load_row5(dq, act[s] ? nb[s] : 0, dqn[s]);
     }
     double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -1889,10 +1908,10 @@ __device__ __forceinline__ void sweep_cell_batched(const DevMesh &M, const Const
 // Any number of faces per cell (polyhedral meshes): face by face.
 template <bool FWD>
 __device__ __forceinline__ void sweep_cell_generic(const DevMesh &M, const Consts &K, int c, bool useU,
-                                                   const double *__restrict__ coef, const double *__restrict__ res,
-                                                   const double *__restrict__ diag, double *dq, double *dflux)
+                                                   const double *__restrict__ coef, const double *__restrict__ res, const double *__restrict__ diag,
+                                                   double *dq, double *dflux)
 {
-    const int nC = M.nCell, nT = M.nTotal;
+    const int nC = M.nCell;
     double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
     for (int s = 0; s < M.ellW; ++s) {
         const int enc = M.c2fEnc[(size_t)s * nC + c];
@@ -1903,7 +1922,7 @@ __device__ __forceinline__ void sweep_cell_generic(const DevMesh &M, const Const
         const int f = (isL ? enc : -enc) - 1;
         double co[15], dqn[5], df[5];
         load_coef(coef, f, isL, co);
-        load5(dq, nT, nb, dqn);
+        load_row5(dq, nb, dqn);
         sweep_apply(K, co, dqn, df);
 #pragma unroll
         for (int m = 0; m < 5; ++m) rhs0[m] += df[m];
@@ -1913,8 +1932,7 @@ __device__ __forceinline__ void sweep_cell_generic(const DevMesh &M, const Const
 
 template <bool FWD>
 __device__ __forceinline__ void sweep_cell(const DevMesh &M, const Consts &K, int c, bool useU,
-                                           const double *__restrict__ coef, const double *__restrict__ res,
-                                           const double *__restrict__ diag, double *dq, double *dflux)
+                                           const double *__restrict__ coef, const double *__restrict__ res, const double *__restrict__ diag, double *dq, double *dflux)
 {
     if (M.ellW <= kW) sweep_cell_batched<FWD>(M, K, c, useU, coef, res, diag, dq, dflux);
     else sweep_cell_generic<FWD>(M, K, c, useU, coef, res, diag, dq, dflux);
@@ -1937,8 +1955,7 @@ __global__ void __launch_bounds__(64) k_sweep_level(DevMesh M, Consts K, const i
 template <bool FWD>
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_block(DevMesh M, Consts K, const int *__restrict__ ptr,
                                                       const int *__restrict__ cells, int l0, int l1, bool useU,
-                                                      const double *__restrict__ coef, const double *__restrict__ res,
-                                                      const double *__restrict__ diag, double *dq, double *dflux)
+                                                      const double *__restrict__ coef, const double *__restrict__ res, const double *__restrict__ diag, double *dq, double *dflux)
 {
     for (int L = l0; L < l1; ++L) {
         const int b = ptr[L], e = ptr[L + 1];
@@ -1982,7 +1999,7 @@ __device__ __forceinline__ void sweep_position(const DevMesh &M, const Consts &K
                                                const double *__restrict__ res, const double *__restrict__ diag,
                                                double *dq, double *dflux)
 {
-    const int nT = M.nTotal, P = M.nCell;
+    const int P = M.nCell;
     const int c = so.cells[p];
     int row[NW], nb[NW];
 #pragma unroll
@@ -1991,20 +2008,16 @@ __device__ __forceinline__ void sweep_position(const DevMesh &M, const Consts &K
         nb[s] = so.nb[(size_t)s * P + p];
     }
     // the cell's own operands first: they depend on c only
-    const int nC = M.nCell;
+    double r[5], U[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    load5(res, P, c, r);
     const double D = diag[c];
-    double r[5], U[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        r[m] = res[(size_t)m * nC + c];
-        U[m] = useU ? dflux[(size_t)m * nC + c] : 0.0;
-    }
+    if (useU) load_row5(dflux, c, U);
     double co[NW][16], dqn[NW][5];
 #pragma unroll
     for (int s = 0; s < NW; ++s) {
         const bool act = row[s] >= 0;
         load_coef_row<CT>(coef, act ? row[s] : 0, co[s]);
-        load5(dq, nT, act ? nb[s] : 0, dqn[s]);
+        load_row5(dq, act ? nb[s] : 0, dqn[s]);
     }
     // Unused slots add +0.0, which leaves the sum unchanged (it starts at
     // +0.0, so it is never -0.0); no branch lets the loads of a slot sink.
@@ -2017,18 +2030,20 @@ __device__ __forceinline__ void sweep_position(const DevMesh &M, const Consts &K
 #pragma unroll
         for (int m = 0; m < 5; ++m) rhs0[m] += act ? df[m] : 0.0;
     }
+    double v[5], w[5];
 #pragma unroll
     for (int m = 0; m < 5; ++m) {
-        dq[(size_t)m * nT + c] = (r[m] - U[m] - rhs0[m]) / D;
-        dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
+        v[m] = (r[m] - U[m] - rhs0[m]) / D;
+        w[m] = 0.0 + rhs0[m];
     }
+    store_row5(dq, c, v);
+    store_row5(dflux, c, w);
 }
 
 template <int NW, int VIS, typename CT>
 __global__ void __launch_bounds__(64) k_sweep_level_so(DevMesh M, Consts K, SweepOrder so, int begin, int end,
                                                        bool useU, const double *__restrict__ coef,
-                                                       const double *__restrict__ res,
-                                                       const double *__restrict__ diag, double *dq, double *dflux)
+                                                       const double *__restrict__ res, const double *__restrict__ diag, double *dq, double *dflux)
 {
     const int p = begin + xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (p >= end) return;
@@ -2039,8 +2054,7 @@ template <int NW, int VIS, typename CT>
 __global__ void __launch_bounds__(kSweepBlock) k_sweep_block_so(DevMesh M, Consts K, SweepOrder so,
                                                                const int *__restrict__ ptr, int l0, int l1,
                                                                bool useU, const double *__restrict__ coef,
-                                                               const double *__restrict__ res,
-                                                               const double *__restrict__ diag, double *dq,
+                                                               const double *__restrict__ res, const double *__restrict__ diag, double *dq,
                                                                double *dflux)
 {
     for (int L = l0; L < l1; ++L) {
@@ -2066,6 +2080,13 @@ __global__ void __launch_bounds__(kSweepBlock) k_sweep_block_so(DevMesh M, Const
 // [3] the wait bound in ticks of the 100 MHz wall clock (set by the host;
 // 20,000,000 = 0.2 s by default).
 
+// 128- and 64-bit payloads of the raw buffer loads and stores, and word 3
+// of their buffer descriptor (untyped 32-bit data, as raw buffer accesses
+// on gfx9-family parts take it); cache policy 16 is sc1 (device scope).
+typedef int v4i __attribute__((ext_vector_type(4)));
+typedef int v2i __attribute__((ext_vector_type(2)));
+constexpr int kRawBufferWord3 = 0x00020000;
+constexpr int kDeviceScope = 16;
 __global__ void k_sweep_flow_prep(int *ctl)
 {
     ctl[0] = 0;
@@ -2115,13 +2136,11 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
             nb[s] = fo.nb[(size_t)s * P + p];
         }
     }
+    double r[5];
+    load5(res, nC, cc, r);
     const double D = diag[cc];
-    double r[5], U[5];
-#pragma unroll
-    for (int m = 0; m < 5; ++m) {
-        r[m] = res[(size_t)m * nC + cc];
-        U[m] = useU ? dflux[(size_t)m * nC + cc] : 0.0;
-    }
+    double U[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
+    if (useU) load_row5(dflux, cc, U);
     double co[NW][16];
     if (K.coefInSweep) {
         // the face terms of k_grad_lim, evaluated here while the chunk waits:
@@ -2184,8 +2203,18 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     for (int s = 0; s < NW; ++s) {
         if (s >= width) break;
         const int i = row[s] >= 0 ? nb[s] : 0;
-#pragma unroll
-        for (int m = 0; m < 5; ++m) dqn[s][m] = dq_load(&dq[(size_t)m * nT + i]);
+        // the neighbour's row with device-scope (sc1) vector loads: three
+        // requests to one line instead of five 8-byte loads
+        {
+            const __amdgpu_buffer_rsrc_t dqr =
+                __builtin_amdgcn_make_buffer_rsrc(dq, 0, (int)((size_t)nT * kRow * sizeof(double)), kRawBufferWord3);
+            const v4i a = __builtin_amdgcn_raw_buffer_load_b128(dqr, i * kRow * 8, 0, kDeviceScope);
+            const v4i b = __builtin_amdgcn_raw_buffer_load_b128(dqr, i * kRow * 8 + 16, 0, kDeviceScope);
+            const v2i e = __builtin_amdgcn_raw_buffer_load_b64(dqr, i * kRow * 8 + 32, 0, kDeviceScope);
+            const double2 da = __builtin_bit_cast(double2, a), db = __builtin_bit_cast(double2, b);
+            dqn[s][0] = da.x; dqn[s][1] = da.y; dqn[s][2] = db.x; dqn[s][3] = db.y;
+            dqn[s][4] = __builtin_bit_cast(double, e);
+        }
     }
     double dT[NW];
     if (sa.on) {
@@ -2209,10 +2238,17 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
             if (s < width) rhsT += (row[s] >= 0) ? aT[s] * dT[s] : 0.0;
     }
     if (live) {
+        // the cell's row with device-scope (sc1) vector stores
+        double v[5];
 #pragma unroll
-        for (int m = 0; m < 5; ++m)
-            __hip_atomic_store(&dq[(size_t)m * nT + c], (r[m] - U[m] - rhs0[m]) / D, __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+        for (int m = 0; m < 5; ++m) v[m] = (r[m] - U[m] - rhs0[m]) / D;
+        const __amdgpu_buffer_rsrc_t dqw =
+            __builtin_amdgcn_make_buffer_rsrc(dq, 0, (int)((size_t)nT * kRow * sizeof(double)), kRawBufferWord3);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, make_double2(v[0], v[1])), dqw, c * kRow * 8, 0,
+                                               kDeviceScope);
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, make_double2(v[2], v[3])), dqw,
+                                               c * kRow * 8 + 16, 0, kDeviceScope);
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v[4]), dqw, c * kRow * 8 + 32, 0, kDeviceScope);
         if (sa.on) __hip_atomic_store(&sa.dq[c], (rT - UT - rhsT) / DT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
     // the wave's dq stores are complete before the flag that publishes them
@@ -2222,8 +2258,10 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
     if (lane == 0) __hip_atomic_store(&chunkFlag[tk], epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     // the off-diagonal sums are read only by the next sweep (another launch)
     if (live) {
+        double w[5];
 #pragma unroll
-        for (int m = 0; m < 5; ++m) dflux[(size_t)m * nC + c] = 0.0 + rhs0[m];
+        for (int m = 0; m < 5; ++m) w[m] = 0.0 + rhs0[m];
+        store_row5(dflux, c, w);
         if (sa.on) sa.dflux[c] = 0.0 + rhsT;
     }
     return true;
@@ -2243,8 +2281,7 @@ template <int NW, int VIS, typename CT, bool TICKET>
 __global__ void __launch_bounds__(TICKET ? 256 : 64) k_sweep_flow(DevMesh M, Consts K, FlowOrder fo, int *ctl,
                                                                   int *chunkFlag, bool useU,
                                                                   const double *__restrict__ coef,
-                                                                  const double *__restrict__ res,
-                                                                  const double *__restrict__ diag, double *dq,
+                                                                  const double *__restrict__ res, const double *__restrict__ diag, double *dq,
                                                                   double *dflux, SaSweep sa,
                                                                   const GatherRec *__restrict__ rec,
                                                                   const double *__restrict__ vist)
@@ -2277,7 +2314,7 @@ __global__ void __launch_bounds__(kBlock) k_ghost_dq(DevMesh M, Consts K, const 
     const int le = M.fle[f], re = M.nCell + f;
     const int bc = M.bcType[f];
     double dl[5], dr[5];
-    load5(dq, nT, le, dl);
+    load_row5(dq, le, dl);
     if (bc == kBC_SYMM || (bc == kBC_WALL && !K.viscous)) {
         double prim[5], Q[5];
         load5(q, nT, le, prim);
@@ -2316,8 +2353,7 @@ __global__ void __launch_bounds__(kBlock) k_ghost_dq(DevMesh M, Consts K, const 
 #pragma unroll
         for (int m = 0; m < 5; ++m) dr[m] = 0.0;
     }
-#pragma unroll
-    for (int m = 0; m < 5; ++m) dq[(size_t)m * nT + re] = dr[m];
+    store_row5(dq, re, dr);
 }
 
 // ------------------------------------------------------------- update
@@ -2341,8 +2377,10 @@ __global__ void __launch_bounds__(kBlock) k_update(DevMesh M, Consts K0, const d
         double Q[5];
         Q[0] = prim[0]; Q[1] = prim[0] * prim[1]; Q[2] = prim[0] * prim[2]; Q[3] = prim[0] * prim[3];
         Q[4] = prim[0] * em;
+        double d5[5];
+        load_row5(dq, i, d5);
 #pragma unroll
-        for (int m = 0; m < 5; ++m) Q[m] += dq[(size_t)m * nT + i];
+        for (int m = 0; m < 5; ++m) Q[m] += d5[m];
         if (Q[0] <= 0.0) {
             prim[0] = Q[0];
         } else {
@@ -2838,11 +2876,11 @@ void launch_sweep(const DevMesh &m, const Consts &k, DevFlow &f, const Levels &l
     }
 }
 
-void launch_sweep_change(const double *dq, size_t ld, int nv, int nC, double *prev, double *parts, double *out,
-                         hipStream_t s)
+void launch_sweep_change(const double *dq, size_t cs, size_t ms, int nv, int nC, double *prev, double *parts,
+                         double *out, hipStream_t s)
 {
     const int nb = nblk(nC, kBlock);
-    hipLaunchKernelGGL(k_sweep_change, dim3(nb), dim3(kBlock), 0, s, dq, ld, nv, nC, prev, parts);
+    hipLaunchKernelGGL(k_sweep_change, dim3(nb), dim3(kBlock), 0, s, dq, cs, ms, nv, nC, prev, parts);
     hipLaunchKernelGGL(k_sum_into, dim3(1), dim3(kBlock), 0, s, parts, nb, out);
 }
 

@@ -931,10 +931,10 @@ void launch_sa_lusgs(const DevMesh &m, const Consts &k, DevFlow &f, const Levels
     log->done = 0;
     for (int it = 0; it < k.nSweeps; ++it) {
         sa_sweep_dispatch(m, f, fwd, true, s);
-        launch_sweep_change(f.dqT, m.nTotal, 1, m.nCell, f.dqTPrev, f.sweepParts, f.sweepNorm, s);
+        launch_sweep_change(f.dqT, 1, 0, 1, m.nCell, f.dqTPrev, f.sweepParts, f.sweepNorm, s);
         launch_sa_ghost_dq(m, f, s);
         sa_sweep_dispatch(m, f, bwd, true, s);
-        launch_sweep_change(f.dqT, m.nTotal, 1, m.nCell, f.dqTPrev, f.sweepParts, f.sweepNorm + 1, s);
+        launch_sweep_change(f.dqT, 1, 0, 1, m.nCell, f.dqTPrev, f.sweepParts, f.sweepNorm + 1, s);
         (void)hipMemcpyAsync(log->host, f.sweepNorm, 2 * sizeof(double), hipMemcpyDeviceToHost, s);
         if (hipStreamSynchronize(s) != hipSuccess) break;  // reported by the caller's launch check
         const double sn = log->host[0] + log->host[1];
