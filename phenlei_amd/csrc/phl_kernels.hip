@@ -2198,28 +2198,32 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
         }
     }
     __atomic_signal_fence(__ATOMIC_SEQ_CST);  // the dq loads stay after the flag loads
-    double dqn[NW][5];
+    double dqn[NW][5], dT[NW];
 #pragma unroll
     for (int s = 0; s < NW; ++s) {
+        dT[s] = 0.0;
         if (s >= width) break;
         const int i = row[s] >= 0 ? nb[s] : 0;
         // the neighbour's row with device-scope (sc1) vector loads: three
-        // requests to one line instead of five 8-byte loads
+        // requests to one line instead of five 8-byte loads; with the SA
+        // sweep carried, the row's slot 5 holds the neighbour's nu~ increment
         {
             const __amdgpu_buffer_rsrc_t dqr =
                 __builtin_amdgcn_make_buffer_rsrc(dq, 0, (int)((size_t)nT * kRow * sizeof(double)), kRawBufferWord3);
             const v4i a = __builtin_amdgcn_raw_buffer_load_b128(dqr, i * kRow * 8, 0, kDeviceScope);
             const v4i b = __builtin_amdgcn_raw_buffer_load_b128(dqr, i * kRow * 8 + 16, 0, kDeviceScope);
-            const v2i e = __builtin_amdgcn_raw_buffer_load_b64(dqr, i * kRow * 8 + 32, 0, kDeviceScope);
             const double2 da = __builtin_bit_cast(double2, a), db = __builtin_bit_cast(double2, b);
             dqn[s][0] = da.x; dqn[s][1] = da.y; dqn[s][2] = db.x; dqn[s][3] = db.y;
-            dqn[s][4] = __builtin_bit_cast(double, e);
+            if (sa.on) {
+                const double2 dc = __builtin_bit_cast(
+                    double2, __builtin_amdgcn_raw_buffer_load_b128(dqr, i * kRow * 8 + 32, 0, kDeviceScope));
+                dqn[s][4] = dc.x;
+                dT[s] = dc.y;
+            } else {
+                dqn[s][4] = __builtin_bit_cast(
+                    double, __builtin_amdgcn_raw_buffer_load_b64(dqr, i * kRow * 8 + 32, 0, kDeviceScope));
+            }
         }
-    }
-    double dT[NW];
-    if (sa.on) {
-#pragma unroll
-        for (int s = 0; s < NW; ++s) dT[s] = s < width ? dq_load(&sa.dq[row[s] >= 0 ? nb[s] : 0]) : 0.0;
     }
     double rhs0[5] = {0.0, 0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -2248,8 +2252,12 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
                                                kDeviceScope);
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, make_double2(v[2], v[3])), dqw,
                                                c * kRow * 8 + 16, 0, kDeviceScope);
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v[4]), dqw, c * kRow * 8 + 32, 0, kDeviceScope);
-        if (sa.on) __hip_atomic_store(&sa.dq[c], (rT - UT - rhsT) / DT, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (sa.on)
+            __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(v4i, make_double2(v[4], (rT - UT - rhsT) / DT)),
+                                                   dqw, c * kRow * 8 + 32, 0, kDeviceScope);
+        else
+            __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(v2i, v[4]), dqw, c * kRow * 8 + 32, 0,
+                                                  kDeviceScope);
     }
     // the wave's dq stores are complete before the flag that publishes them
     __atomic_signal_fence(__ATOMIC_SEQ_CST);
@@ -2262,7 +2270,10 @@ __device__ __forceinline__ bool flow_chunk(const DevMesh &M, const Consts &K, co
 #pragma unroll
         for (int m = 0; m < 5; ++m) w[m] = 0.0 + rhs0[m];
         store_row5(dflux, c, w);
-        if (sa.on) sa.dflux[c] = 0.0 + rhsT;
+        if (sa.on) {
+            sa.dflux[c] = 0.0 + rhsT;
+            sa.dq[c] = (rT - UT - rhsT) / DT;  // for the ghost, update and field kernels
+        }
     }
     return true;
 }

@@ -784,13 +784,18 @@ __global__ void __launch_bounds__(kSaBlock) k_sa_sweep_block(DevMesh M, SweepOrd
 }
 
 // SetGhostDQLUSGS: wall 0; farfield, symmetry, inflow, outflow copy; else 0.
-__global__ void __launch_bounds__(kBlock) k_sa_ghost_dq(DevMesh M, double *__restrict__ dq)
+// The ghost's value also goes to slot 5 of its NS increment row, where the
+// dataflow NS sweeps that carry the SA sweep read a neighbour's nu~ increment.
+__global__ void __launch_bounds__(kBlock) k_sa_ghost_dq(DevMesh M, double *__restrict__ dq,
+                                                        double *__restrict__ rows)
 {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= M.nBFace) return;
     const int bc = M.bcType[f];
     const bool copy = bc == kBC_FAR || bc == kBC_SYMM || bc == kBC_INFLOW || bc == kBC_OUTFLOW;
-    dq[M.nCell + f] = copy ? dq[M.fle[f]] : 0.0;
+    const double v = copy ? dq[M.fle[f]] : 0.0;
+    dq[M.nCell + f] = v;
+    rows[(size_t)(M.nCell + f) * kRow + 5] = v;
 }
 
 __global__ void __launch_bounds__(kBlock) k_sa_update(DevMesh M, double *__restrict__ nut,
@@ -899,7 +904,7 @@ void launch_sa_sweep_init(const DevMesh &m, DevFlow &f, hipStream_t s)
 void launch_sa_ghost_dq(const DevMesh &m, DevFlow &f, hipStream_t s)
 {
     if (m.nBFace > 0)
-        hipLaunchKernelGGL(k_sa_ghost_dq, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, f.dqT);
+        hipLaunchKernelGGL(k_sa_ghost_dq, dim3(nblk(m.nBFace, kBlock)), dim3(kBlock), 0, s, m, f.dqT, f.dq);
 }
 
 void launch_sa_update(const DevMesh &m, DevFlow &f, hipStream_t s)
